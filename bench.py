@@ -1,0 +1,169 @@
+"""Benchmark of the CG advection-diffusion hot path on one MI355X.
+
+Workload (default): the graded pressure-Poisson configuration, 110^3 hexes x
+6 Kuhn tetrahedra = 7,986,000 P1 tets, 1,367,631 nodes, z graded 100:1
+toward z = 0.5, log-normal diffusivity (contrast 1e3), f = sin sin sin,
+Dirichlet 0, synthetic inputs from a splitmix64 seed.
+
+A step = one assembly of the whole mesh (matrix + rhs) with every input
+resident in HBM.  K timed steps give `value` (Melem/s).  After the steps the
+assembled system is solved to a true relative residual of 1e-8 twice:
+plain FP64 Jacobi-PCG and the mixed-precision (FP32 inner, FP64 refinement)
+solve; both times are reported.
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def asm_bytes(cfg, nnz: int) -> int:
+    """Compulsory assembly traffic: conn 4*loc*E + coords 8*dim*Nx + nodal
+    inputs (T, kappa if nodal, f) 8*N each + A 8*nnz + rhs 8*N."""
+    m = cfg.mesh
+    nodal = 1  # T
+    if cfg.kappa is not None and not np.isscalar(cfg.kappa):
+        nodal += 1
+    if cfg.source is not None:
+        nodal += 1
+    if cfg.U is not None:
+        nodal += m.dim
+    return 4 * m.loc * m.nele + 8 * m.dim * m.nx + 8 * nodal * m.nnodes + 8 * nnz + 8 * m.nnodes
+
+
+def spmv_bytes(n: int, nnz: int, value_bytes: int = 8) -> int:
+    return (value_bytes + 4) * nnz + 4 * (n + 1) + (value_bytes * 2) * n
+
+
+def cpu_baseline(cfg, budget_s: float = 20.0) -> dict:
+    """The oracle (C restatement, OpenMP over the colour loop like the
+    reference) timed on this host: the assembly of the whole mesh and a
+    bounded sample of Jacobi-PCG iterations."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.time()
+    ref = O.reference_system(cfg, nthreads=threads)
+    asm_s = ref["asm_seconds"]
+    its = 50
+    x, k, reason, relres, secs = O.petsc_solve("cg", "jacobi", 1e-30, ref["sparsity"], ref["A"], ref["inactive"],
+                                               ref["rhs"], max_its=its, nthreads=threads)
+    per_it = secs / max(k, 1)
+    log(f"cpu baseline: assembly {asm_s:.3f}s, PCG {per_it * 1e3:.2f} ms/it ({threads} threads), "
+        f"wall {time.time() - t0:.1f}s")
+    return {"value": cfg.mesh.nele / asm_s / 1e6, "unit": "Melem/s", "cores": threads, "kind": "port",
+            "sample": f"oracle assembly of the full mesh (colour loop, {threads} OpenMP threads) + "
+                      f"{k} Jacobi-PCG iterations ({per_it * 1e3:.2f} ms/iteration)",
+            "pcg_ms_per_iteration": per_it * 1e3}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c5")
+    ap.add_argument("--scale", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--scheme", default="auto")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
+        return 2
+
+    from fluidity_amd import cg, lib as L, meshgen as M
+    import ctypes as C
+
+    t0 = time.time()
+    cfg = M.config(args.config, seed=1, scale=args.scale)
+    log(f"config {cfg.name}: {cfg.mesh.nele} elements, {cfg.mesh.nnodes} nodes ({time.time() - t0:.1f}s)")
+    pr = cg.problem_from_config(cfg)
+    nnz = pr.sparsity.nnz
+    log(f"bound: nnz {nnz}, colours {pr.ncolours} ({time.time() - t0:.1f}s)")
+    opts = cg.options_from_config(cfg, scheme=args.scheme)
+    p = opts.params(cfg.dt)
+    lib = L.load()
+    ms = C.c_double(0)
+    for _ in range(max(args.warmup, 0)):
+        L.check(lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(p)), "assemble")
+    L.check(lib.flgpu_time_assemble(pr.ctx, C.byref(p), args.steps, C.byref(ms)), "time_assemble")
+    t_asm = ms.value * 1e-3
+    melems = cfg.mesh.nele / t_asm / 1e6
+    B = asm_bytes(cfg, nnz)
+    achieved = B / t_asm / 1e9
+    log(f"assembly: {ms.value:.3f} ms/step, {melems:.1f} Melem/s, {achieved:.1f} GB/s algorithmic")
+
+    out = {
+        "metric": "Melem/s assembly; solve time to 1e-8 res: learned+mixed vs FP64 PCG",
+        "value": round(melems, 3),
+        "unit": "Melem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms.value, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (splitmix64-seeded mesh jitter and log-normal diffusivity)",
+        "config": {"workload": f"{cfg.name}: graded pressure Poisson, {cfg.mesh.nele} P1 tets, "
+                               f"{cfg.mesh.nnodes} nodes, nnz {nnz}", "elements": cfg.mesh.nele,
+                   "nodes": cfg.mesh.nnodes, "nnz": nnz, "colours": pr.ncolours,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "kernel": "assembly (all launches of one step)", "algorithmic_bytes": B},
+    }
+
+    if not args.no_solve:
+        cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
+        solves = {}
+        for label, so in (
+            ("fp64_jacobi_pcg", cg.SolverOptions("cg", "jacobi", relative_error=1e-8, norm="unpreconditioned",
+                                                 max_iterations=50000, check_every=32)),
+            ("mixed_jacobi_pcg", cg.SolverOptions("cg", "jacobi", relative_error=1e-8, norm="unpreconditioned",
+                                                  max_iterations=50000, precision="mixed", check_every=32)),
+        ):
+            x, info = cg.petsc_solve(pr, so)
+            solves[label] = {"seconds": round(info.seconds, 4), "its": info.its, "reason": info.reason,
+                             "true_relres": info.true_relres, "refinements": info.refinements}
+            log(f"solve {label}: {info.seconds:.3f}s, {info.its} its, reason {info.reason}, "
+                f"relres {info.true_relres:.2e}")
+        for prec, vb in (("fp64", 8), ("mixed", 4)):
+            L.check(lib.flgpu_solver_prepare(pr.ctx, L.FLGPU_FP64 if prec == "fp64" else L.FLGPU_MIXED), "prep")
+            L.check(lib.flgpu_time_spmv(pr.ctx, L.FLGPU_FP64 if prec == "fp64" else L.FLGPU_MIXED, 50,
+                                        C.byref(ms)), "time_spmv")
+            gbs = spmv_bytes(cfg.mesh.nnodes, nnz, vb) / (ms.value * 1e-3) / 1e9
+            solves[f"spmv_{prec}"] = {"ms": round(ms.value, 4), "GB/s": round(gbs, 1),
+                                      "frac": round(gbs / PEAK_HBM_GBS, 4)}
+            log(f"spmv {prec}: {ms.value:.4f} ms, {gbs:.1f} GB/s")
+        out["solve"] = solves
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg)
+    pr.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,269 @@
+"""Host-side mirror of Fluidity's CG scalar-field solve over the gfx950
+library (the caller side of include/flgpu.h).
+
+The call sequence is the one of solve_field_equation_cg
+(assemble/Advection_Diffusion_CG.F90:127-174):
+
+    problem = CGProblem(mesh, element)            # state, sparsity, colouring
+    problem.set_fields(T=..., velocity=..., ...)  # scalar/vector/tensor fields
+    assemble_advection_diffusion_cg(problem, opts, dt)      # :228-659
+    apply_dirichlet_conditions(problem, nodes, values, dt)  # :643
+    delta_t, info = petsc_solve(problem, solver_options)    # :1389
+    T += dt * delta_t                                       # :1396-1407
+
+Option names follow the .flml schema (schemas/solvers.rnc,
+temporal_discretisation/theta, conservative_advection, ...).  Every call
+goes to libflgpu.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import lib as L
+from .elements import Element, make_element
+from .meshgen import Mesh
+
+
+@dataclass
+class CGOptions:
+    """spatial_discretisation / temporal_discretisation options read by
+    assemble_advection_diffusion_cg (:282-556)."""
+    theta: float = 0.5
+    beta: float = 0.0                    # conservative_advection
+    exclude_mass_terms: bool = False
+    lump_mass_matrix: bool = False
+    exclude_advection_terms: bool = False
+    integrate_advection_by_parts: bool = False
+    have_diffusivity: bool = True
+    isotropic_diffusivity: bool = True
+    have_source: bool = False
+    have_absorption: bool = False
+    precision: str = "fp64"              # element arithmetic: fp64 | fp32
+    scheme: str = "auto"                 # auto | colour | blocked
+
+    def params(self, dt: float) -> L.AdvdifParams:
+        return L.AdvdifParams(
+            dt=dt, theta=self.theta, beta=self.beta,
+            have_mass=int(not self.exclude_mass_terms), lump_mass=int(self.lump_mass_matrix),
+            have_advection=int(not self.exclude_advection_terms),
+            integrate_by_parts=int(self.integrate_advection_by_parts),
+            have_diffusivity=int(self.have_diffusivity),
+            isotropic_diffusivity=int(self.isotropic_diffusivity),
+            have_source=int(self.have_source), have_absorption=int(self.have_absorption),
+            precision={"fp64": L.FLGPU_FP64, "fp32": L.FLGPU_FP32}[self.precision],
+            scheme={"auto": L.FLGPU_ASM_AUTO, "colour": L.FLGPU_ASM_COLOUR,
+                    "blocked": L.FLGPU_ASM_BLOCKED}[self.scheme])
+
+
+@dataclass
+class SolverOptions:
+    """solver options (schemas/solvers.rnc:107-612)."""
+    iterative_method: str = "cg"         # cg | gmres | fcg
+    preconditioner: str = "jacobi"       # jacobi | none | learned
+    relative_error: float = 1e-8
+    absolute_error: float = 0.0
+    max_iterations: int = 10000
+    restart: int = 30
+    precision: str = "fp64"              # fp64 | mixed
+    start_from_zero: bool = True
+    norm: str = "preconditioned"         # preconditioned | unpreconditioned
+    check_every: int = 16
+    stall_factor: float = 0.0
+    stall_window: int = 0
+
+    def params(self) -> L.SolverParams:
+        return L.SolverParams(
+            ksp={"cg": L.FLGPU_KSP_CG, "gmres": L.FLGPU_KSP_GMRES, "fcg": L.FLGPU_KSP_FCG}[self.iterative_method],
+            pc={"none": L.FLGPU_PC_NONE, "jacobi": L.FLGPU_PC_JACOBI, "learned": L.FLGPU_PC_LEARNED}[self.preconditioner],
+            rtol=self.relative_error, atol=self.absolute_error, max_its=self.max_iterations,
+            restart=self.restart, precision={"fp64": L.FLGPU_FP64, "mixed": L.FLGPU_MIXED}[self.precision],
+            start_from_zero=int(self.start_from_zero), check_every=self.check_every,
+            stall_factor=self.stall_factor, stall_window=self.stall_window,
+            norm_type={"preconditioned": L.FLGPU_NORM_PRECONDITIONED,
+                       "unpreconditioned": L.FLGPU_NORM_UNPRECONDITIONED}[self.norm])
+
+
+@dataclass
+class Sparsity:
+    """csr_sparsity (femtools/Sparse_Tools.F90:48-71), 1-based."""
+    findrm: np.ndarray
+    colm: np.ndarray
+    centrm: np.ndarray
+
+    @property
+    def nnz(self) -> int:
+        return int(self.colm.size)
+
+
+def make_sparsity(mesh: Mesh) -> Sparsity:
+    """make_sparsity(mesh, mesh) through the library's host helper."""
+    lib = L.load()
+    n = mesh.nnodes
+    findrm = np.zeros(n + 1, np.int32)
+    centrm = np.zeros(n, np.int32)
+    nnz = C.c_int64(0)
+    L.check(lib.flgpu_make_sparsity(n, mesh.nele, mesh.loc, L.ptr(mesh.ndglno), L.ptr(findrm), None,
+                                    L.ptr(centrm), C.byref(nnz)), "flgpu_make_sparsity")
+    colm = np.zeros(nnz.value, np.int32)
+    L.check(lib.flgpu_make_sparsity(n, mesh.nele, mesh.loc, L.ptr(mesh.ndglno), L.ptr(findrm), L.ptr(colm),
+                                    L.ptr(centrm), C.byref(nnz)), "flgpu_make_sparsity")
+    return Sparsity(findrm, colm, centrm)
+
+
+def get_mesh_colouring(mesh: Mesh) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """COLOURING_CG1 colour per element (1-based) and the flattened sets
+    (col_ptr 1-based offsets, elems ascending inside each colour)."""
+    lib = L.load()
+    colour = np.zeros(mesh.nele, np.int32)
+    ncol = C.c_int32(0)
+    L.check(lib.flgpu_colour_mesh(mesh.nnodes, mesh.nele, mesh.loc, L.ptr(mesh.ndglno), L.ptr(colour),
+                                  C.byref(ncol)), "flgpu_colour_mesh")
+    order = np.argsort(colour, kind="stable").astype(np.int32)  # ascending ids inside a colour
+    counts = np.bincount(colour, minlength=ncol.value + 1)[1:]
+    col_ptr = np.concatenate([[1], 1 + np.cumsum(counts)]).astype(np.int32)
+    return colour, col_ptr, (order + 1).astype(np.int32)
+
+
+class CGProblem:
+    """One mesh + element bound to a device context (the state_type of the
+    path: positions, the field mesh, its sparsity and colouring)."""
+
+    def __init__(self, mesh: Mesh, element: Element | None = None, quad_degree: int = 2,
+                 device: int = 0, sparsity: Sparsity | None = None,
+                 colouring: tuple | None = None):
+        self.lib = L.load()
+        self.mesh = mesh
+        self.element = element or make_element(mesh.dim, mesh.degree, quad_degree)
+        xel = make_element(mesh.dim, 1, self.element.quad_degree)
+        self.sparsity = sparsity or make_sparsity(mesh)
+        self.colouring = colouring or get_mesh_colouring(mesh)
+        self.ctx = C.c_void_p()
+        L.check(self.lib.flgpu_init(C.byref(self.ctx), device), "flgpu_init")
+        e = self.element
+        X = np.ascontiguousarray(mesh.X, dtype=np.float64)
+        L.check(self.lib.flgpu_mesh_bind(self.ctx, 1, 0, mesh.dim, mesh.dim + 1, mesh.nele, mesh.nx,
+                                         L.ptr(mesh.ndglno_x), L.ptr(X)), "flgpu_mesh_bind")
+        L.check(self.lib.flgpu_field_mesh_bind(self.ctx, e.loc, e.ngi, mesh.nnodes, L.ptr(mesh.ndglno),
+                                               L.ptr(e.fortran(e.n)), L.ptr(e.fortran(e.dn)),
+                                               L.ptr(np.ascontiguousarray(e.w)), L.ptr(xel.fortran(xel.dn))),
+                "flgpu_field_mesh_bind")
+        s = self.sparsity
+        L.check(self.lib.flgpu_sparsity_bind(self.ctx, mesh.nnodes, mesh.nnodes, L.ptr(s.findrm),
+                                             L.ptr(s.colm), L.ptr(s.centrm)), "flgpu_sparsity_bind")
+        _, col_ptr, elems = self.colouring
+        self.ncolours = col_ptr.size - 1
+        L.check(self.lib.flgpu_colouring_bind(self.ctx, self.ncolours, L.ptr(col_ptr), L.ptr(elems)),
+                "flgpu_colouring_bind")
+        self.T = None
+
+    # -- fields --------------------------------------------------------
+    def set_field(self, which: int, values, ncomp: int) -> None:
+        """Nodal array (nnodes,) / (nnodes, ncomp) or a constant."""
+        if np.isscalar(values) or np.asarray(values).size == ncomp:
+            v = np.ascontiguousarray(np.broadcast_to(np.asarray(values, np.float64), (ncomp,)))
+            const = 1
+        else:
+            v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+            const = 0
+        L.check(self.lib.flgpu_field_set(self.ctx, which, const, L.ptr(v)), "flgpu_field_set")
+
+    def set_fields(self, T=None, velocity=None, diffusivity=None, source=None, absorption=None,
+                   isotropic: bool = True) -> None:
+        d = self.mesh.dim
+        if T is not None:
+            self.T = np.ascontiguousarray(T, dtype=np.float64)
+            self.set_field(L.FIELD_T, self.T, 1)
+        if velocity is not None:
+            self.set_field(L.FIELD_VELOCITY, velocity, d)
+        if diffusivity is not None:
+            k = diffusivity
+            if isotropic and not np.isscalar(k) and np.asarray(k).ndim == 1 and np.asarray(k).size == self.mesh.nnodes:
+                full = np.zeros((self.mesh.nnodes, d, d))
+                for a in range(d):
+                    full[:, a, a] = k
+                k = full
+            elif np.isscalar(k):
+                k = (np.eye(d) * k).ravel()
+            self.set_field(L.FIELD_DIFFUSIVITY, k, d * d)
+        if source is not None:
+            self.set_field(L.FIELD_SOURCE, source, 1)
+        if absorption is not None:
+            self.set_field(L.FIELD_ABSORPTION, absorption, 1)
+
+    # -- system --------------------------------------------------------
+    def matrix(self):
+        """(A values, rhs, inactive mask) copied from the device."""
+        A = np.zeros(self.sparsity.nnz)
+        rhs = np.zeros(self.mesh.nnodes)
+        inact = np.zeros(self.mesh.nnodes, np.int32)
+        L.check(self.lib.flgpu_matrix_get(self.ctx, L.ptr(A), L.ptr(rhs), L.ptr(inact)), "flgpu_matrix_get")
+        return A, rhs, inact
+
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.flgpu_finalize(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def assemble_advection_diffusion_cg(problem: CGProblem, opts: CGOptions, dt: float) -> None:
+    """The element loop of assemble_advection_diffusion_cg on the device
+    (zero matrix and rhs, colour loop, insertion)."""
+    p = opts.params(dt)
+    L.check(problem.lib.flgpu_advdif_assemble_resident(problem.ctx, C.byref(p)),
+            "flgpu_advdif_assemble_resident")
+
+
+def apply_dirichlet_conditions(problem: CGProblem, nodes: np.ndarray, values, dt: float | None) -> None:
+    """apply_dirichlet_conditions_scalar (femtools/Boundary_Conditions.F90:
+    1983-2025) in rate-of-change form when dt is given."""
+    nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+    vals = np.ascontiguousarray(np.broadcast_to(np.asarray(values, np.float64), nodes.shape))
+    L.check(problem.lib.flgpu_dirichlet_apply(problem.ctx, nodes.size, L.ptr(nodes), L.ptr(vals),
+                                              -1.0 if dt is None else float(dt)), "flgpu_dirichlet_apply")
+
+
+def petsc_solve(problem: CGProblem, solver: SolverOptions, x0: np.ndarray | None = None):
+    """petsc_solve(x, matrix, rhs) on the device-resident system; returns
+    (x, SolveInfo).  info.reason > 0 means converged (KSPConvergedReason)."""
+    x = np.zeros(problem.mesh.nnodes) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).copy()
+    info = L.SolveInfo()
+    sp = solver.params()
+    L.check(problem.lib.flgpu_solve(problem.ctx, C.byref(sp), L.ptr(x), C.byref(info)), "flgpu_solve")
+    return x, info
+
+
+def solve_field_equation_cg(problem: CGProblem, opts: CGOptions, solver: SolverOptions, dt: float,
+                            bc_nodes: np.ndarray | None = None, bc_values=0.0):
+    """One solve_field_equation_cg step: assemble, strong Dirichlet, solve
+    for the rate of change and update T (T += dt*dT).  Returns (T_new,
+    iterations_taken, info)."""
+    assemble_advection_diffusion_cg(problem, opts, dt)
+    if bc_nodes is not None and bc_nodes.size:
+        apply_dirichlet_conditions(problem, bc_nodes, bc_values, dt)
+    dT, info = petsc_solve(problem, solver)
+    T_new = problem.T + dt * dT
+    return T_new, info.its, info
+
+
+def problem_from_config(cfg, device: int = 0, **kw) -> CGProblem:
+    """Bind a meshgen.Config and upload its fields."""
+    pr = CGProblem(cfg.mesh, quad_degree=cfg.quad_degree, device=device, **kw)
+    pr.set_fields(T=cfg.T, velocity=cfg.U, diffusivity=cfg.kappa, source=cfg.source,
+                  absorption=cfg.absorption)
+    return pr
+
+
+def options_from_config(cfg, precision: str = "fp64", scheme: str = "auto") -> CGOptions:
+    return CGOptions(theta=cfg.theta, beta=cfg.beta, exclude_mass_terms=not cfg.have_mass,
+                     exclude_advection_terms=not cfg.have_advection, have_diffusivity=cfg.kappa is not None,
+                     have_source=cfg.source is not None, have_absorption=cfg.absorption is not None,
+                     precision=precision, scheme=scheme)

@@ -1,0 +1,467 @@
+// Element arithmetic of the CG advection-diffusion assembly: one row of
+// the element matrix and rhs per call.  Shared by the gfx950 kernels
+// (assemble.hip) and by a host build used to unit-test the same code on
+// the CPU (tests compile it with g++); the device path never runs it on
+// the host.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define FLGPU_HD __host__ __device__
+#define FLGPU_HD_INLINE __host__ __device__ __forceinline__
+#else
+#define FLGPU_HD
+#define FLGPU_HD_INLINE inline
+#endif
+
+namespace flgpu {
+
+#ifndef FLGPU_ELEM_LIMITS
+#define FLGPU_ELEM_LIMITS
+constexpr int kElemMaxLoc = 10;   // P2 tetrahedron
+constexpr int kElemMaxGi = 11;    // degree-4 tetrahedron rule
+constexpr int kElemMaxDim = 3;
+#endif
+
+using std::fabs;
+
+FLGPU_HD_INLINE double FMA(double a, double b, double c) { return fma(a, b, c); }
+FLGPU_HD_INLINE float FMA(float a, float b, float c) { return fmaf(a, b, c); }
+
+struct AsmFlags {
+    double dt_theta, beta;
+    int have_mass, lump, adv, ibp, diff, iso, src, absorp;
+    int need_div, dtt_on;
+};
+
+// Field pointers for the kernels (constant fields point at one value).
+struct AsmFields {
+    const double* T;
+    const double* U; int u_c;
+    const double* K; int k_c;
+    const double* S; int s_c;
+    const double* B; int b_c;
+};
+
+// Tables in LDS, converted to the arithmetic type R.
+template <typename R>
+struct LTab {
+    R n[kElemMaxLoc * kElemMaxGi];
+    R dn[kElemMaxLoc * kElemMaxGi * kElemMaxDim];
+    R w[kElemMaxGi];
+    R what[kElemMaxLoc];
+    R mhat[kElemMaxLoc * kElemMaxLoc];
+    R that[64];
+    R xdn[4 * kElemMaxDim];
+    int loc, ngi, lx;
+};
+
+// Gathered element data.
+template <int DIM, int LOC, typename R>
+struct Elem {
+    int nodes[LOC];
+    R X[DIM][DIM + 1];
+    R T[LOC];
+};
+
+// Jacobian of the linear coordinate element, its cofactor inverse and
+// determinant (construct_cache, femtools/Transform_elements.F90:263-304).
+template <int DIM, typename R>
+FLGPU_HD_INLINE void geometry(const R (&X)[DIM][DIM + 1], const LTab<R>& tb,
+                                         R (&iJ)[DIM][DIM], R& adet) {
+    R JT[DIM][DIM];
+#pragma unroll
+    for (int a = 0; a < DIM; a++)
+#pragma unroll
+        for (int b = 0; b < DIM; b++) {
+            R s = (R)0;
+#pragma unroll
+            for (int nn = 0; nn < DIM + 1; nn++) s = FMA(X[a][nn], tb.xdn[nn + (DIM + 1) * b], s);
+            JT[a][b] = s;
+        }
+    R C[DIM][DIM];
+    if constexpr (DIM == 3) {
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+                C[i][k] = FMA(JT[i1][k1], JT[i2][k2], -(JT[i2][k1] * JT[i1][k2]));
+            }
+    } else {
+        C[0][0] = JT[1][1];
+        C[1][0] = -JT[0][1];
+        C[0][1] = -JT[1][0];
+        C[1][1] = JT[0][0];
+    }
+    R det = JT[0][0] * C[0][0];
+#pragma unroll
+    for (int a = 1; a < DIM; a++) det = FMA(JT[a][0], C[a][0], det);
+    const R rdet = (R)1 / det;
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+#pragma unroll
+        for (int k = 0; k < DIM; k++) iJ[i][k] = C[i][k] * rdet;
+    adet = fabs(det);
+}
+
+template <typename R>
+FLGPU_HD_INLINE R fld(const double* __restrict__ f, int c, int node, int comp, int ncomp) {
+    return (R)(c ? f[comp] : f[(int64_t)node * ncomp + comp]);
+}
+
+// Row i of the element matrix (vals[j], j = 0..LOC-1) and of the element
+// rhs, for linear T (gradients constant over the element).
+template <int DIM, int LOC, typename R>
+FLGPU_HD void row_linear(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, const AsmFlags& fl,
+                           const AsmFields& fd, int i, R (&vals)[LOC], R& rhs_i) {
+    R iJ[DIM][DIM], adet;
+    geometry<DIM, R>(e.X, tb, iJ, adet);
+    // physical gradients G[n][d] = sum_k dn(n,k) invJ(d,k)
+    R G[LOC][DIM];
+#pragma unroll
+    for (int nn = 0; nn < LOC; nn++)
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            R s = (R)0;
+#pragma unroll
+            for (int k = 0; k < DIM; k++) s = FMA(tb.dn[nn + LOC * (0 + tb.ngi * k)], iJ[d][k], s);
+            G[nn][d] = s;
+        }
+    const R dtt = (R)fl.dt_theta;
+    R M[LOC], Aa[LOC], Bb[LOC], Kk[LOC];
+#pragma unroll
+    for (int j = 0; j < LOC; j++) {
+        M[j] = (R)0;
+        Aa[j] = (R)0;
+        Bb[j] = (R)0;
+        Kk[j] = (R)0;
+    }
+    if (fl.have_mass) {
+#pragma unroll
+        for (int j = 0; j < LOC; j++) M[j] = adet * tb.mhat[i + LOC * j];
+    }
+    if (fl.adv) {
+        R u[LOC][DIM];
+#pragma unroll
+        for (int k = 0; k < LOC; k++)
+#pragma unroll
+            for (int d = 0; d < DIM; d++) u[k][d] = fld<R>(fd.U, fd.u_c, e.nodes[k], d, DIM);
+        R div = (R)0;
+        if (fl.need_div) {
+#pragma unroll
+            for (int d = 0; d < DIM; d++)
+#pragma unroll
+                for (int k = 0; k < LOC; k++) div = FMA(u[k][d], G[k][d], div);
+        }
+        if (!fl.ibp) {
+            // A(i,j) = |J| sum_k mhat(i,k) u_k . G_j  (shape_vector_dot_dshape)
+            R ut[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                R s = (R)0;
+#pragma unroll
+                for (int k = 0; k < LOC; k++) s = FMA(tb.mhat[i + LOC * k], u[k][d], s);
+                ut[d] = s;
+            }
+#pragma unroll
+            for (int j = 0; j < LOC; j++) {
+                R s = ut[0] * G[j][0];
+#pragma unroll
+                for (int d = 1; d < DIM; d++) s = FMA(ut[d], G[j][d], s);
+                R a = adet * s;
+                if (fl.need_div) a = FMA((R)fl.beta * div, adet * tb.mhat[i + LOC * j], a);
+                Aa[j] = a;
+            }
+        } else {
+            // A(i,j) = -|J| G_i . sum_k mhat(k,j) u_k  (dshape_dot_vector_shape)
+#pragma unroll
+            for (int j = 0; j < LOC; j++) {
+                R s = (R)0;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    R t = (R)0;
+#pragma unroll
+                    for (int k = 0; k < LOC; k++) t = FMA(tb.mhat[k + LOC * j], u[k][d], t);
+                    s = FMA(G[i][d], t, s);
+                }
+                R a = -(adet * s);
+                if (fl.need_div) a = FMA(-((R)1 - (R)fl.beta) * div, adet * tb.mhat[i + LOC * j], a);
+                Aa[j] = a;
+            }
+        }
+    }
+    if (fl.absorp) {
+        R sg[LOC];
+#pragma unroll
+        for (int k = 0; k < LOC; k++) sg[k] = fld<R>(fd.B, fd.b_c, e.nodes[k], 0, 1);
+#pragma unroll
+        for (int j = 0; j < LOC; j++) {
+            R s = (R)0;
+#pragma unroll
+            for (int k = 0; k < LOC; k++) s = FMA(tb.that[i + 4 * (j + 4 * k)], sg[k], s);
+            Bb[j] = adet * s;
+        }
+    }
+    if (fl.diff) {
+        if (fl.iso) {
+            R kw = (R)0;
+#pragma unroll
+            for (int k = 0; k < LOC; k++)
+                kw = FMA(tb.what[k], fld<R>(fd.K, fd.k_c, e.nodes[k], 0, DIM * DIM), kw);
+            const R scale = adet * kw;
+#pragma unroll
+            for (int j = 0; j < LOC; j++) {
+                R s = G[i][0] * G[j][0];
+#pragma unroll
+                for (int d = 1; d < DIM; d++) s = FMA(G[i][d], G[j][d], s);
+                Kk[j] = s * scale;
+            }
+        } else {
+            R kb[DIM][DIM];
+#pragma unroll
+            for (int a = 0; a < DIM; a++)
+#pragma unroll
+                for (int b = 0; b < DIM; b++) {
+                    R s = (R)0;
+#pragma unroll
+                    for (int k = 0; k < LOC; k++)
+                        s = FMA(tb.what[k], fld<R>(fd.K, fd.k_c, e.nodes[k], a + DIM * b, DIM * DIM), s);
+                    kb[a][b] = s;
+                }
+            R v[DIM];
+#pragma unroll
+            for (int b = 0; b < DIM; b++) {
+                R s = (R)0;
+#pragma unroll
+                for (int a = 0; a < DIM; a++) s = FMA(G[i][a], kb[a][b], s);
+                v[b] = s;
+            }
+#pragma unroll
+            for (int j = 0; j < LOC; j++) {
+                R s = (R)0;
+#pragma unroll
+                for (int b = 0; b < DIM; b++) s = FMA(v[b], G[j][b], s);
+                Kk[j] = adet * s;
+            }
+        }
+    }
+    R r = (R)0;
+    if (fl.adv)
+#pragma unroll
+        for (int j = 0; j < LOC; j++) r = FMA(-Aa[j], e.T[j], r);
+    if (fl.absorp)
+#pragma unroll
+        for (int j = 0; j < LOC; j++) r = FMA(-Bb[j], e.T[j], r);
+    if (fl.diff)
+#pragma unroll
+        for (int j = 0; j < LOC; j++) r = FMA(-Kk[j], e.T[j], r);
+    if (fl.src) {
+        R s = (R)0;
+#pragma unroll
+        for (int k = 0; k < LOC; k++) s = FMA(tb.mhat[i + LOC * k], fld<R>(fd.S, fd.s_c, e.nodes[k], 0, 1), s);
+        r = FMA(adet, s, r);
+    }
+    rhs_i = r;
+    R lump = (R)0;
+    if (fl.have_mass && fl.lump) {
+#pragma unroll
+        for (int j = 0; j < LOC; j++) lump += M[j];
+    }
+#pragma unroll
+    for (int j = 0; j < LOC; j++) {
+        R m = (fl.have_mass && !fl.lump) ? M[j] : (R)0;
+        if (fl.have_mass && fl.lump && j == i) m = lump;
+        if (fl.dtt_on) {
+            if (fl.adv) m = FMA(dtt, Aa[j], m);
+            if (fl.absorp) m = FMA(dtt, Bb[j], m);
+            if (fl.diff) m = FMA(dtt, Kk[j], m);
+        }
+        vals[j] = m;
+    }
+}
+
+// Row i for quadratic T: per-quadrature-point loops
+// (Transform_elements.F90:771-775; FETools.F90 contractions).
+template <int DIM, int LOC, typename R>
+FLGPU_HD void row_general(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, const AsmFlags& fl,
+                            const AsmFields& fd, int i, R (&vals)[LOC], R& rhs_i) {
+    R iJ[DIM][DIM], adet;
+    geometry<DIM, R>(e.X, tb, iJ, adet);
+    const int ngi = tb.ngi;
+    const R dtt = (R)fl.dt_theta;
+    R M[LOC], Aa[LOC], Bb[LOC], Kk[LOC];
+#pragma unroll
+    for (int j = 0; j < LOC; j++) {
+        M[j] = (R)0;
+        Aa[j] = (R)0;
+        Bb[j] = (R)0;
+        Kk[j] = (R)0;
+    }
+    R src_i = (R)0;
+    for (int g = 0; g < ngi; g++) {
+        const R dw = adet * tb.w[g];
+        const R ni = tb.n[i + LOC * g];
+        // gradient of N_i at g
+        R Gi[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            R s = (R)0;
+#pragma unroll
+            for (int k = 0; k < DIM; k++) s = FMA(tb.dn[i + LOC * (g + ngi * k)], iJ[d][k], s);
+            Gi[d] = s;
+        }
+        R uq[DIM] = {};
+        R div = (R)0;
+        if (fl.adv) {
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                R s = (R)0;
+#pragma unroll
+                for (int k = 0; k < LOC; k++) s = FMA(fld<R>(fd.U, fd.u_c, e.nodes[k], d, DIM), tb.n[k + LOC * g], s);
+                uq[d] = s;
+            }
+            if (fl.need_div) {
+#pragma unroll
+                for (int k = 0; k < LOC; k++)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) {
+                        R s = (R)0;
+#pragma unroll
+                        for (int q = 0; q < DIM; q++) s = FMA(tb.dn[k + LOC * (g + ngi * q)], iJ[d][q], s);
+                        div = FMA(fld<R>(fd.U, fd.u_c, e.nodes[k], d, DIM), s, div);
+                    }
+            }
+        }
+        R kq = (R)0, sq = (R)0;
+        R kt[DIM][DIM];
+        if (fl.diff) {
+            if (fl.iso) {
+#pragma unroll
+                for (int k = 0; k < LOC; k++) kq = FMA(fld<R>(fd.K, fd.k_c, e.nodes[k], 0, DIM * DIM), tb.n[k + LOC * g], kq);
+            } else {
+#pragma unroll
+                for (int a = 0; a < DIM; a++)
+#pragma unroll
+                    for (int b = 0; b < DIM; b++) {
+                        R s = (R)0;
+#pragma unroll
+                        for (int k = 0; k < LOC; k++)
+                            s = FMA(fld<R>(fd.K, fd.k_c, e.nodes[k], a + DIM * b, DIM * DIM), tb.n[k + LOC * g], s);
+                        kt[a][b] = s;
+                    }
+            }
+        }
+        if (fl.absorp) {
+#pragma unroll
+            for (int k = 0; k < LOC; k++) sq = FMA(fld<R>(fd.B, fd.b_c, e.nodes[k], 0, 1), tb.n[k + LOC * g], sq);
+        }
+        if (fl.src) {
+            R fq = (R)0;
+#pragma unroll
+            for (int k = 0; k < LOC; k++) fq = FMA(fld<R>(fd.S, fd.s_c, e.nodes[k], 0, 1), tb.n[k + LOC * g], fq);
+            src_i = FMA(ni, dw * fq, src_i);
+        }
+        R ug_i = (R)0;
+        if (fl.adv && fl.ibp) {
+#pragma unroll
+            for (int d = 0; d < DIM; d++) ug_i = FMA(Gi[d], uq[d], ug_i);
+        }
+        R vK[DIM];
+        if (fl.diff && !fl.iso) {
+#pragma unroll
+            for (int b = 0; b < DIM; b++) {
+                R s = (R)0;
+#pragma unroll
+                for (int a = 0; a < DIM; a++) s = FMA(Gi[a], kt[a][b], s);
+                vK[b] = s;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < LOC; j++) {
+            const R nj = tb.n[j + LOC * g];
+            const R nn = ni * nj;
+            if (fl.have_mass) M[j] = FMA(nn, dw, M[j]);
+            if (fl.adv || fl.diff) {
+                R Gj[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    R s = (R)0;
+#pragma unroll
+                    for (int k = 0; k < DIM; k++) s = FMA(tb.dn[j + LOC * (g + ngi * k)], iJ[d][k], s);
+                    Gj[d] = s;
+                }
+                if (fl.adv) {
+                    R a;
+                    if (!fl.ibp) {
+                        R t = (R)0;
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) t = FMA(Gj[d], uq[d], t);
+                        a = ni * t;
+                        if (fl.need_div) a = FMA((R)fl.beta * nn, div, a);
+                    } else {
+                        a = -(ug_i * nj);
+                        if (fl.need_div) a = FMA(-((R)1 - (R)fl.beta) * nn, div, a);
+                    }
+                    Aa[j] = FMA(a, dw, Aa[j]);
+                }
+                if (fl.diff) {
+                    R t = (R)0;
+                    if (fl.iso) {
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) t = FMA(Gi[d], Gj[d], t);
+                        Kk[j] = FMA(t, dw * kq, Kk[j]);
+                    } else {
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) t = FMA(vK[d], Gj[d], t);
+                        Kk[j] = FMA(t, dw, Kk[j]);
+                    }
+                }
+            }
+            if (fl.absorp) Bb[j] = FMA(nn, dw * sq, Bb[j]);
+        }
+    }
+    R r = (R)0;
+    if (fl.adv)
+#pragma unroll
+        for (int j = 0; j < LOC; j++) r = FMA(-Aa[j], e.T[j], r);
+    if (fl.absorp)
+#pragma unroll
+        for (int j = 0; j < LOC; j++) r = FMA(-Bb[j], e.T[j], r);
+    if (fl.diff)
+#pragma unroll
+        for (int j = 0; j < LOC; j++) r = FMA(-Kk[j], e.T[j], r);
+    if (fl.src) r = r + src_i;
+    rhs_i = r;
+    R lump = (R)0;
+    if (fl.have_mass && fl.lump) {
+#pragma unroll
+        for (int j = 0; j < LOC; j++) lump += M[j];
+    }
+#pragma unroll
+    for (int j = 0; j < LOC; j++) {
+        R m = (fl.have_mass && !fl.lump) ? M[j] : (R)0;
+        if (fl.have_mass && fl.lump && j == i) m = lump;
+        if (fl.dtt_on) {
+            if (fl.adv) m = FMA(dtt, Aa[j], m);
+            if (fl.absorp) m = FMA(dtt, Bb[j], m);
+            if (fl.diff) m = FMA(dtt, Kk[j], m);
+        }
+        vals[j] = m;
+    }
+}
+
+template <int DIM, int LOC, typename R>
+FLGPU_HD_INLINE void row_contrib(const Elem<DIM, LOC, R>& e, const LTab<R>& tb,
+                                            const AsmFlags& fl, const AsmFields& fd, int i,
+                                            R (&vals)[LOC], R& rhs_i) {
+    if constexpr (LOC == DIM + 1)
+        row_linear<DIM, LOC, R>(e, tb, fl, fd, i, vals, rhs_i);
+    else
+        row_general<DIM, LOC, R>(e, tb, fl, fd, i, vals, rhs_i);
+}
+
+}  // namespace flgpu

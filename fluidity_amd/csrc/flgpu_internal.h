@@ -1,0 +1,172 @@
+// Internal declarations of the gfx950 CG advection-diffusion library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/flgpu.h"
+
+namespace flgpu {
+
+constexpr int kMaxLoc = 10;   // P2 tetrahedron
+constexpr int kMaxGi = 11;    // degree-4 tetrahedron rule
+constexpr int kMaxDim = 3;
+
+int set_error(int code, const char* fmt, ...);
+
+#define FLGPU_HIP(call)                                                     \
+    do {                                                                    \
+        hipError_t e_ = (call);                                             \
+        if (e_ != hipSuccess)                                               \
+            return ::flgpu::set_error(FLGPU_EHIP, "%s failed: %s (%s:%d)",  \
+                                      #call, hipGetErrorString(e_),         \
+                                      __FILE__, __LINE__);                  \
+    } while (0)
+
+#define FLGPU_TRY(expr)                 \
+    do {                                \
+        int rc_ = (expr);               \
+        if (rc_ != FLGPU_OK) return rc_; \
+    } while (0)
+
+// Owning device buffer.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t count) {
+        if (count == n && p) return FLGPU_OK;
+        release();
+        if (count == 0) return FLGPU_OK;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return set_error(FLGPU_ENOMEM, "hipMalloc of %zu bytes failed: %s",
+                             count * sizeof(T), hipGetErrorString(e));
+        }
+        n = count;
+        return FLGPU_OK;
+    }
+    int upload(const T* host, size_t count, hipStream_t s) {
+        FLGPU_TRY(alloc(count));
+        if (count)
+            FLGPU_HIP(hipMemcpyAsync(p, host, count * sizeof(T),
+                                     hipMemcpyHostToDevice, s));
+        return FLGPU_OK;
+    }
+};
+
+// Element tables, device-side copy (staged into LDS by the kernels).
+struct ElemTables {
+    int32_t dim, loc, ngi, loc_x;
+    double n[kMaxLoc * kMaxGi];             // n[i + loc*g]
+    double dn[kMaxLoc * kMaxGi * kMaxDim];  // dn[i + loc*(g + ngi*d)]
+    double w[kMaxGi];
+    double xdn[4 * kMaxDim];                // coordinate dn at g=1: [i + loc_x*d]
+    // Contractions used when the physical gradients are constant over the
+    // element (linear T on linear simplices):
+    double what[kMaxLoc];                   // sum_g w_g n(i,g)
+    double mhat[kMaxLoc * kMaxLoc];         // sum_g w_g n(i,g) n(j,g)
+    double that[4 * 4 * 4];                 // sum_g w_g n_i n_j n_k (P1 only)
+};
+
+// Fields on the device.  A constant field keeps one value (ncomp numbers)
+// and `constant` is set.
+struct DevField {
+    DevBuf<double> v;
+    int32_t constant = 0;
+    int32_t set = 0;
+};
+
+// Row-block plan of the LDS-blocked P1 assembly.
+struct BlockPlan {
+    int32_t rows_per_block = 0;
+    int32_t nblocks = 0;
+    DevBuf<int32_t> seg;      // nblocks*(ncol+1) offsets into visits
+    DevBuf<int32_t> visits;   // element ids (0-based) in colour order per block
+    int32_t max_row_nnz = 0;  // longest row in any block
+    int32_t max_block_nnz = 0;
+    bool built = false;
+};
+
+struct KrylovWork;   // krylov.hip
+struct LearnedState; // learned.hip
+
+}  // namespace flgpu
+
+struct flgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t prop{};
+
+    // coordinate mesh
+    int64_t key_id = -1, key_event = -1;
+    int32_t dim = 0, nele = 0, loc_x = 0, nx = 0;
+    flgpu::DevBuf<int32_t> conn_x;   // loc_x*nele, 0-based, element-major
+    flgpu::DevBuf<double> X4;        // 4 doubles per node (x,y,z,0)
+    std::vector<int32_t> h_conn_x;
+
+    // field mesh
+    int32_t loc = 0, ngi = 0, nnodes = 0;
+    flgpu::DevBuf<int32_t> conn;     // loc*nele, 0-based, element-major
+    std::vector<int32_t> h_conn;
+    flgpu::ElemTables tab{};
+    flgpu::DevBuf<flgpu::ElemTables> d_tab;
+    bool linear = false;             // T is degree 1
+
+    // sparsity (0-based)
+    int32_t rows = 0, cols = 0;
+    int64_t nnz = 0;
+    flgpu::DevBuf<int32_t> rowptr, col, diag;
+    std::vector<int32_t> h_rowptr, h_col;
+
+    // colouring (0-based)
+    int32_t ncol = 0;
+    std::vector<int32_t> h_col_ptr, h_col_elems;
+    flgpu::DevBuf<int32_t> col_elems;
+
+    // fields
+    flgpu::DevField fT, fU, fK, fS, fA;
+
+    // assembled system
+    flgpu::DevBuf<double> A, rhs;
+    flgpu::DevBuf<int32_t> inactive;
+    bool have_inactive = false;
+    bool assembled = false;
+
+    flgpu::BlockPlan plan;
+    flgpu::KrylovWork* kw = nullptr;
+    flgpu::LearnedState* ls = nullptr;
+};
+
+namespace flgpu {
+// assemble.hip
+int assemble(flgpu_ctx* c, const flgpu_advdif_params* p);
+int build_block_plan(flgpu_ctx* c);
+int time_assemble(flgpu_ctx* c, const flgpu_advdif_params* p, int reps, double* ms);
+// krylov.hip
+int solve(flgpu_ctx* c, const flgpu_solver_params* sp, double* x_host,
+          flgpu_solve_info* info);
+void krylov_free(flgpu_ctx* c);
+int solver_prepare(flgpu_ctx* c, int precision);
+int time_spmv(flgpu_ctx* c, int precision, int reps, double* ms);
+int csr_mult(flgpu_ctx* c, const double* x, double* y);
+int solution_get(flgpu_ctx* c, double* x);
+// learned.hip
+void learned_free(flgpu_ctx* c);
+// shared helpers
+unsigned grid_for(int64_t n, int block);
+}  // namespace flgpu

@@ -1,0 +1,970 @@
+// Krylov solve of the assembled system on gfx950.
+//
+// The hand-off reproduces petsc_solve (femtools/Solvers.F90:111-187):
+//   - inactive (strong Dirichlet) rows become "ghost" rows holding only the
+//     diagonal ghost_pivot = sqrt((maxdiag+mindiag)*maxdiag/2) over all
+//     |A_ii| (femtools/Petsc_Tools.F90:959-984, :1045-1057);
+//   - inactive columns are dropped and lifted into the rhs,
+//     b = rhs - A*ghost_rhs (femtools/Solvers.F90:1072-1099), and ghost rows
+//     get b = 0;
+//   - after the solve the ghost entries of x take the rhs values
+//     (femtools/Petsc_Tools.F90:724-731).
+// Methods: KSPCG-style preconditioned CG, flexible CG (Polak-Ribiere beta)
+// for the learned preconditioner, restarted GMRES with left Jacobi.
+// Precision: FP64, or FP32 inner iterations with FP64 solution updates,
+// FP64 dot-product accumulation and FP64 true-residual replacement
+// ("reliable updates", iterative refinement without restarting CG).
+// Every reduction has a fixed order: results repeat bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "flgpu_internal.h"
+#include "krylov_common.h"
+#include "learned.h"
+
+namespace flgpu {
+
+
+struct KrylovWork {
+    int n = 0;
+    int64_t nnz = 0;
+    DevBuf<double> val64, b, x, r, z, p, w, dinv, y;
+    DevBuf<float> val32, r32, z32, p32, w32, dinv32, y32;
+    DevBuf<double> partials;
+    DevBuf<unsigned> counter;
+    DevBuf<Scal> scal;
+    Scal* h_scal = nullptr;  // pinned
+    int nblk = 1;
+    // GMRES
+    DevBuf<double> V, H, gv, cs, sn, hcol;
+    DevBuf<float> V32;
+    int gm = 0;
+    bool have_x = false;
+    bool prepared = false, prepared_mixed = false;
+    ~KrylovWork() {
+        if (h_scal) (void)hipHostFree(h_scal);
+    }
+};
+
+// ---------------------------------------------------------------------
+// Reduced system
+// ---------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kVecBlock)
+k_diag_minmax(int n, const int32_t* __restrict__ diag, const double* __restrict__ A,
+              double* __restrict__ partials, unsigned* counter, Scal* sc) {
+    double mn = 1.79769313486231570e308, mx = -1.79769313486231570e308;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int d = diag[i];
+        const double v = d >= 0 ? fabs(A[d]) : 0.0;
+        mn = fmin(mn, v);
+        mx = fmax(mx, v);
+    }
+    // min/max are order independent; reuse the sum machinery on -min.
+    __shared__ double sh[kVecBlock / 64][2];
+    __shared__ unsigned ticket;
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = fmin(mn, __shfl_xor(mn, off, 64));
+        mx = fmax(mx, __shfl_xor(mx, off, 64));
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        sh[wave][0] = mn;
+        sh[wave][1] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
+            sh[0][0] = fmin(sh[0][0], sh[q][0]);
+            sh[0][1] = fmax(sh[0][1], sh[q][1]);
+        }
+        partials[2 * blockIdx.x] = sh[0][0];
+        partials[2 * blockIdx.x + 1] = sh[0][1];
+        __threadfence();
+        ticket = atomicAdd(counter, 1u);
+    }
+    __syncthreads();
+    if (ticket != gridDim.x - 1 || threadIdx.x != 0) return;
+    __threadfence();
+    double a = 1.79769313486231570e308, b = -1.79769313486231570e308;
+    for (int q = 0; q < (int)gridDim.x; q++) {
+        a = fmin(a, load_coherent(&partials[2 * q]));
+        b = fmax(b, load_coherent(&partials[2 * q + 1]));
+    }
+    sc->dmin = a;
+    sc->dmax = b;
+    sc->pivot = sqrt((b + a) * b / 2.0);
+    *counter = 0u;
+}
+
+// Thread per row: copy to the reduced values, lift inactive columns, set
+// the ghost pivot, the Jacobi inverse diagonal and the initial x.
+__global__ void __launch_bounds__(kVecBlock)
+k_build(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+        const int32_t* __restrict__ diag, const double* __restrict__ A,
+        const double* __restrict__ rhs, const int32_t* __restrict__ inactive, int have_inactive,
+        const Scal* __restrict__ sc, double* __restrict__ v64, float* __restrict__ v32,
+        double* __restrict__ b, double* __restrict__ dinv, float* __restrict__ dinv32,
+        double* __restrict__ x, int zero_x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool ghost = have_inactive && inactive[i];
+    const double pivot = sc->pivot;
+    double lift = 0.0, dval = 0.0;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+        const int cj = col[k];
+        double v = A[k];
+        if (ghost) {
+            v = (cj == i) ? pivot : 0.0;
+        } else if (have_inactive && inactive[cj]) {
+            lift = fma(v, rhs[cj], lift);
+            v = 0.0;
+        }
+        if (cj == i) dval = v;
+        v64[k] = v;
+        if (v32) v32[k] = (float)v;
+    }
+    b[i] = ghost ? 0.0 : rhs[i] - lift;
+    const double di = dval != 0.0 ? 1.0 / dval : 1.0;
+    dinv[i] = di;
+    if (dinv32) dinv32[i] = (float)di;
+    if (zero_x || ghost) x[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------
+// CSR products: L lanes per row, fixed grid, fixed reduction order
+// ---------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ T lane_sum(T v, int L) {
+    for (int off = L >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// y = A x  (optionally y = b - A x);  partial sums: dot(x_own, y), dot(y,y)
+template <int L, typename V, typename XV, typename YV>
+__global__ void __launch_bounds__(kVecBlock)
+k_spmv(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+       const V* __restrict__ val, const XV* __restrict__ x, YV* __restrict__ y,
+       const double* __restrict__ bsub, int want_dot, double* __restrict__ partials,
+       unsigned* counter, Scal* sc, int mode, const int* __restrict__ done_flag) {
+    if (done_flag && *done_flag) return;
+    constexpr int ROWS = kVecBlock / L;
+    const int lane = threadIdx.x % L, rloc = threadIdx.x / L;
+    double d0 = 0.0, d1 = 0.0;
+    for (int grp = blockIdx.x; grp * ROWS < n; grp += gridDim.x) {
+        const int row = grp * ROWS + rloc;
+        V acc = (V)0;
+        if (row < n) {
+            const int e = rowptr[row + 1];
+            for (int k = rowptr[row] + lane; k < e; k += L) acc += val[k] * (V)x[col[k]];
+        }
+        acc = lane_sum<V>(acc, L);
+        if (row < n && lane == 0) {
+            double yv = (double)acc;
+            if (bsub) yv = bsub[row] - yv;
+            y[row] = (YV)yv;
+            if (want_dot) {
+                d0 += (double)x[row] * yv;
+                d1 += yv * yv;
+            }
+        }
+    }
+    if (!want_dot) return;
+    double v[2] = {d0, d1}, out[2];
+    if (grid_sum<2>(v, partials, counter, out) && threadIdx.x == 0) {
+        if (mode == 0) {  // CG: w = A p, pAp
+            sc->pAp = out[0];
+            if (!(out[0] > 0.0)) {
+                sc->reason = (out[0] != out[0]) ? KSP_DIVERGED_NANORINF : KSP_DIVERGED_INDEFINITE_MAT;
+                sc->done = 1;
+            } else {
+                sc->alpha = sc->rz / out[0];
+            }
+        } else if (mode == 1) {  // residual: y = b - A x
+            sc->true_rr = out[1];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// PCG / FCG vector kernels
+// ---------------------------------------------------------------------
+
+// x += alpha p; r -= alpha w; [Jacobi: z = D^-1 r];
+// partials: r.z, r.r, z.z, z.w (for the flexible beta)
+template <typename V>
+__global__ void __launch_bounds__(kVecBlock)
+k_cg_update(int n, double* __restrict__ x, const V* __restrict__ p, V* __restrict__ r,
+            const V* __restrict__ w, V* __restrict__ z, const V* __restrict__ dinv, int jacobi,
+            double* __restrict__ partials, unsigned* counter, Scal* sc) {
+    if (sc->done) return;
+    const double alpha = sc->alpha;
+    const V a = (V)alpha;
+    double s_rz = 0.0, s_rr = 0.0, s_zz = 0.0, s_zw = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const V pi = p[i], wi = w[i];
+        x[i] = fma(alpha, (double)pi, x[i]);
+        const V ri = r[i] - a * wi;
+        r[i] = ri;
+        if (jacobi) {
+            const V zi = dinv[i] * ri;
+            z[i] = zi;
+            s_rz += (double)ri * (double)zi;
+            s_zz += (double)zi * (double)zi;
+            s_zw += (double)zi * (double)wi;
+        }
+        s_rr += (double)ri * (double)ri;
+    }
+    double v[4] = {s_rz, s_rr, s_zz, s_zw}, out[4];
+    if (grid_sum<4>(v, partials, counter, out) && threadIdx.x == 0) {
+        sc->rr = out[1];
+        if (jacobi) {
+            sc->rz_old = sc->rz;
+            sc->rz = out[0];
+            sc->zz = out[2];
+            sc->zw = out[3];
+        }
+    }
+}
+
+// Convergence test of KSPConvergedDefault on the norm chosen by norm_type,
+// then beta (Fletcher-Reeves, or Polak-Ribiere when flexible).
+__device__ void cg_test_and_beta(Scal* sc) {
+    sc->its += 1;
+    const double nrm = sc->norm_type == FLGPU_NORM_UNPRECONDITIONED ? sqrt(sc->rr) : sqrt(sc->zz);
+    sc->last_norm = nrm;
+    if (nrm != nrm || isinf(nrm)) {
+        sc->reason = KSP_DIVERGED_NANORINF;
+        sc->done = 1;
+    } else if (nrm <= sc->atol) {
+        sc->reason = KSP_CONVERGED_ATOL;
+        sc->done = 1;
+    } else if (nrm <= sc->rtol * sc->rnorm0) {
+        sc->reason = KSP_CONVERGED_RTOL;
+        sc->done = 1;
+    } else if (sc->rnorm0 > 0 && nrm >= 1e5 * sc->rnorm0) {
+        sc->reason = KSP_DIVERGED_DTOL;
+        sc->done = 1;
+    } else if (sc->its >= sc->max_its) {
+        sc->reason = KSP_DIVERGED_ITS;
+        sc->done = 1;
+    }
+    if (sc->flexible)
+        sc->beta = -sc->alpha * sc->zw / sc->rz_old;  // z.(r_new - r_old) / (z_old.r_old)
+    else
+        sc->beta = sc->rz / sc->rz_old;
+    if (!sc->done && sc->rz < 0.0) {
+        sc->reason = KSP_DIVERGED_INDEFINITE_PC;
+        sc->done = 1;
+    }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kVecBlock)
+k_cg_p(int n, const V* __restrict__ z, V* __restrict__ p, Scal* sc, int first) {
+    // One block (the last of k_cg_update) could run the test; instead every
+    // block recomputes nothing and reads the finalized beta.
+    if (sc->done) return;
+    const V beta = first ? (V)0 : (V)sc->beta;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = first ? z[i] : z[i] + beta * p[i];
+}
+
+__global__ void k_cg_test(Scal* sc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !sc->done) cg_test_and_beta(sc);
+}
+
+// r32 = (V) r64 ; z = D^-1 r ; sums r.z, r.r, z.z
+template <typename V>
+__global__ void __launch_bounds__(kVecBlock)
+k_restart_vecs(int n, const double* __restrict__ r64, V* __restrict__ r, V* __restrict__ z,
+               const V* __restrict__ dinv, int jacobi, double* __restrict__ partials,
+               unsigned* counter, Scal* sc, int set_norm0) {
+    double s_rz = 0.0, s_rr = 0.0, s_zz = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const V ri = (V)r64[i];
+        r[i] = ri;
+        s_rr += (double)ri * (double)ri;
+        if (jacobi) {
+            const V zi = dinv[i] * ri;
+            z[i] = zi;
+            s_rz += (double)ri * (double)zi;
+            s_zz += (double)zi * (double)zi;
+        }
+    }
+    double v[3] = {s_rz, s_rr, s_zz}, out[3];
+    if (grid_sum<3>(v, partials, counter, out) && threadIdx.x == 0) {
+        sc->rr = out[1];
+        if (jacobi) {
+            sc->rz = out[0];
+            sc->zz = out[2];
+        }
+        if (set_norm0) {
+            const double nrm = sc->norm_type == FLGPU_NORM_UNPRECONDITIONED ? sqrt(sc->rr) : sqrt(sc->zz);
+            sc->rnorm0 = nrm;
+            sc->last_norm = nrm;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kVecBlock)
+k_dot2(int n, const double* __restrict__ a, double* __restrict__ partials, unsigned* counter,
+       Scal* sc) {
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        s += a[i] * a[i];
+    double v[1] = {s}, out[1];
+    if (grid_sum<1>(v, partials, counter, out) && threadIdx.x == 0) sc->bb = out[0];
+}
+
+__global__ void __launch_bounds__(kVecBlock)
+k_finish_x(int n, double* __restrict__ x, const double* __restrict__ rhs,
+           const int32_t* __restrict__ inactive, int have_inactive) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && have_inactive && inactive[i]) x[i] = rhs[i];
+}
+
+// ---------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------
+
+static int lanes_for(const flgpu_ctx* c) {
+    const double avg = (double)c->nnz / std::max(1, c->rows);
+    if (avg <= 6) return 4;
+    if (avg <= 12) return 8;
+    if (avg <= 24) return 16;
+    if (avg <= 48) return 32;
+    return 64;
+}
+
+template <typename V, typename XV, typename YV>
+static int spmv(flgpu_ctx* c, KrylovWork* k, const V* val, const XV* x, YV* y, const double* bsub,
+                int want_dot, int mode, const int* done_flag) {
+    const int L = lanes_for(c);
+    const int rows_per_blk = kVecBlock / L;
+    int grid = (int)std::min<int64_t>(kMaxVecGrid, (c->rows + rows_per_blk - 1) / rows_per_blk);
+    grid = std::max(grid, 1);
+#define FL_SPMV(LL)                                                                                   \
+    case LL:                                                                                          \
+        hipLaunchKernelGGL((k_spmv<LL, V, XV, YV>), dim3(grid), dim3(kVecBlock), 0, c->stream, c->rows, \
+                           c->rowptr.p, c->col.p, val, x, y, bsub, want_dot, k->partials.p,            \
+                           k->counter.p, k->scal.p, mode, done_flag);                                 \
+        break;
+    switch (L) {
+        FL_SPMV(4)
+        FL_SPMV(8)
+        FL_SPMV(16)
+        FL_SPMV(32)
+        FL_SPMV(64)
+    }
+#undef FL_SPMV
+    FLGPU_HIP(hipGetLastError());
+    return FLGPU_OK;
+}
+
+static int ensure_work(flgpu_ctx* c, bool mixed) {
+    if (!c->kw) c->kw = new KrylovWork();
+    KrylovWork* k = c->kw;
+    const int n = c->rows;
+    if (k->n != n || k->nnz != c->nnz) {
+        k->n = n;
+        k->nnz = c->nnz;
+        k->have_x = false;
+    }
+    FLGPU_TRY(k->val64.alloc(c->nnz));
+    FLGPU_TRY(k->b.alloc(n));
+    FLGPU_TRY(k->x.alloc(n));
+    FLGPU_TRY(k->r.alloc(n));
+    FLGPU_TRY(k->z.alloc(n));
+    FLGPU_TRY(k->p.alloc(n));
+    FLGPU_TRY(k->w.alloc(n));
+    FLGPU_TRY(k->dinv.alloc(n));
+    FLGPU_TRY(k->y.alloc(n));
+    if (mixed) {
+        FLGPU_TRY(k->val32.alloc(c->nnz));
+        FLGPU_TRY(k->r32.alloc(n));
+        FLGPU_TRY(k->z32.alloc(n));
+        FLGPU_TRY(k->p32.alloc(n));
+        FLGPU_TRY(k->w32.alloc(n));
+        FLGPU_TRY(k->dinv32.alloc(n));
+        FLGPU_TRY(k->y32.alloc(n));
+    }
+    FLGPU_TRY(k->partials.alloc((size_t)kMaxVecGrid * 8));
+    if (!k->counter.p) {
+        FLGPU_TRY(k->counter.alloc(1));
+        FLGPU_HIP(hipMemsetAsync(k->counter.p, 0, sizeof(unsigned), c->stream));
+    }
+    FLGPU_TRY(k->scal.alloc(1));
+    if (!k->h_scal) FLGPU_HIP(hipHostMalloc((void**)&k->h_scal, sizeof(Scal), hipHostMallocDefault));
+    k->nblk = (int)std::min<int64_t>(kMaxVecGrid, (n + kVecBlock - 1) / kVecBlock);
+    k->nblk = std::max(k->nblk, 1);
+    return FLGPU_OK;
+}
+
+static int read_scal(flgpu_ctx* c, KrylovWork* k) {
+    FLGPU_HIP(hipMemcpyAsync(k->h_scal, k->scal.p, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
+    FLGPU_HIP(hipStreamSynchronize(c->stream));
+    return FLGPU_OK;
+}
+
+static int write_scal(flgpu_ctx* c, KrylovWork* k) {
+    FLGPU_HIP(hipMemcpyAsync(k->scal.p, k->h_scal, sizeof(Scal), hipMemcpyHostToDevice, c->stream));
+    return FLGPU_OK;
+}
+
+// Build the reduced (PETSc-side) system from the assembled one.
+static int build_reduced(flgpu_ctx* c, KrylovWork* k, bool mixed, bool zero_x) {
+    const int n = c->rows;
+    hipLaunchKernelGGL(k_diag_minmax, dim3(k->nblk), dim3(kVecBlock), 0, c->stream, n, c->diag.p, c->A.p,
+                       k->partials.p, k->counter.p, k->scal.p);
+    hipLaunchKernelGGL(k_build, dim3(grid_for(n, kVecBlock)), dim3(kVecBlock), 0, c->stream, n, c->rowptr.p,
+                       c->col.p, c->diag.p, c->A.p, c->rhs.p, c->inactive.p, c->have_inactive ? 1 : 0,
+                       k->scal.p, k->val64.p, mixed ? k->val32.p : nullptr, k->b.p, k->dinv.p,
+                       mixed ? k->dinv32.p : nullptr, k->x.p, zero_x ? 1 : 0);
+    FLGPU_HIP(hipGetLastError());
+    return FLGPU_OK;
+}
+
+// True residual r64 = b - A x (FP64) and its squared norm in scal.true_rr.
+static int true_residual(flgpu_ctx* c, KrylovWork* k) {
+    return spmv<double, double, double>(c, k, k->val64.p, k->x.p, k->r.p, k->b.p, 1, 1, nullptr);
+}
+
+// ------------------------- (F)PCG ----------------------------------------
+
+template <typename V>
+struct CgVecs {
+    V *r, *z, *p, *w, *dinv, *y;
+    const V* val;
+};
+
+template <typename V>
+static CgVecs<V> cg_vecs(KrylovWork* k);
+template <>
+CgVecs<double> cg_vecs<double>(KrylovWork* k) {
+    return {k->r.p, k->z.p, k->p.p, k->w.p, k->dinv.p, k->y.p, k->val64.p};
+}
+template <>
+CgVecs<float> cg_vecs<float>(KrylovWork* k) {
+    return {k->r32.p, k->z32.p, k->p32.p, k->w32.p, k->dinv32.p, k->y32.p, k->val32.p};
+}
+
+// Apply the learned preconditioner z = G^T (G r) and finalize r.z, r.r,
+// z.z, z.w (w = A p of this iteration).
+template <typename V>
+static int learned_apply(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, bool set_norm0);
+
+template <typename V>
+static int cg_restart(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, bool jacobi, bool set_norm0) {
+    // r = b - A x (FP64), r_V = r, z = M^-1 r, p = z
+    FLGPU_TRY(true_residual(c, k));
+    hipLaunchKernelGGL((k_restart_vecs<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->r.p,
+                       v.r, v.z, v.dinv, jacobi ? 1 : 0, k->partials.p, k->counter.p, k->scal.p,
+                       set_norm0 ? 1 : 0);
+    FLGPU_HIP(hipGetLastError());
+    if (!jacobi) FLGPU_TRY(learned_apply<V>(c, k, v, set_norm0));
+    hipLaunchKernelGGL((k_cg_p<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, v.z, v.p,
+                       k->scal.p, 1);
+    FLGPU_HIP(hipGetLastError());
+    return FLGPU_OK;
+}
+
+template <typename V>
+static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info* info, bool learned) {
+    KrylovWork* k = c->kw;
+    const bool mixed = std::is_same<V, float>::value;
+    CgVecs<V> v = cg_vecs<V>(k);
+    bool jacobi = !learned && sp->pc == FLGPU_PC_JACOBI;
+    bool use_learned = learned;
+    const int check_every = sp->check_every > 0 ? sp->check_every : 16;
+    const double stall_factor = sp->stall_factor > 0 ? sp->stall_factor : 0.9;
+    const int stall_window = sp->stall_window > 0 ? sp->stall_window : 50;
+
+    // identity preconditioner = Jacobi with dinv = 1: handled by the
+    // "jacobi" kernels when pc == NONE by filling dinv with ones
+    if (!learned && sp->pc == FLGPU_PC_NONE) {
+        std::vector<double> ones(c->rows, 1.0);
+        std::vector<float> ones32(c->rows, 1.0f);
+        FLGPU_HIP(hipMemcpyAsync(k->dinv.p, ones.data(), sizeof(double) * c->rows, hipMemcpyHostToDevice, c->stream));
+        if (mixed)
+            FLGPU_HIP(hipMemcpyAsync(k->dinv32.p, ones32.data(), sizeof(float) * c->rows, hipMemcpyHostToDevice,
+                                     c->stream));
+        FLGPU_HIP(hipStreamSynchronize(c->stream));
+        jacobi = true;
+    }
+    Scal& h = *k->h_scal;
+    std::memset(&h, 0, sizeof h);
+    h.max_its = sp->max_its > 0 ? sp->max_its : 10000;
+    h.rtol = sp->rtol;
+    h.atol = sp->atol;
+    h.norm_type = sp->norm_type;
+    h.flexible = (use_learned || sp->ksp == FLGPU_KSP_FCG) ? 1 : 0;
+    // keep pivot computed by build_reduced
+    {
+        Scal tmp;
+        FLGPU_HIP(hipMemcpyAsync(&tmp, k->scal.p, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
+        FLGPU_HIP(hipStreamSynchronize(c->stream));
+        h.pivot = tmp.pivot;
+        h.dmin = tmp.dmin;
+        h.dmax = tmp.dmax;
+    }
+    FLGPU_TRY(write_scal(c, k));
+    hipLaunchKernelGGL(k_dot2, dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->b.p, k->partials.p,
+                       k->counter.p, k->scal.p);
+    FLGPU_TRY(cg_restart<V>(c, k, v, !use_learned ? true : false, true));
+    FLGPU_TRY(read_scal(c, k));
+    info->refinements = 0;
+    info->fallback = 0;
+    const double bnorm = std::sqrt(h.bb);
+    if (h.last_norm == 0.0 || h.last_norm <= sp->atol) {
+        h.reason = KSP_CONVERGED_ATOL;
+        h.done = 1;
+    }
+    double rmax = std::sqrt(h.rr);          // reliable-update reference
+    const double delta = 0.1;
+    int window_its = 0;
+    double window_norm = h.last_norm;
+    auto t0 = std::chrono::steady_clock::now();
+    while (!h.done) {
+        for (int q = 0; q < check_every; q++) {
+            FLGPU_TRY((spmv<V, V, V>(c, k, v.val, v.p, v.w, nullptr, 1, 0, &k->scal.p->done)));
+            hipLaunchKernelGGL((k_cg_update<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->x.p,
+                               v.p, v.r, v.w, v.z, v.dinv, jacobi ? 1 : 0, k->partials.p, k->counter.p,
+                               k->scal.p);
+            if (!jacobi) FLGPU_TRY(learned_apply<V>(c, k, v, false));
+            hipLaunchKernelGGL(k_cg_test, dim3(1), dim3(64), 0, c->stream, k->scal.p);
+            hipLaunchKernelGGL((k_cg_p<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, v.z, v.p,
+                               k->scal.p, 0);
+        }
+        FLGPU_HIP(hipGetLastError());
+        FLGPU_TRY(read_scal(c, k));
+        const double rnow = std::sqrt(h.rr);
+        if (mixed && (h.done ? h.reason > 0 : rnow < delta * rmax)) {
+            // reliable update: replace the FP32 residual by the FP64 one
+            const int reason = h.reason, done = h.done;
+            FLGPU_TRY(true_residual(c, k));
+            FLGPU_TRY(read_scal(c, k));
+            const double tr = std::sqrt(h.true_rr);
+            info->refinements++;
+            const double target = std::max(sp->rtol * (sp->norm_type == FLGPU_NORM_UNPRECONDITIONED ? h.rnorm0 : bnorm),
+                                           sp->atol);
+            if (done && reason > 0 && tr > target && h.its < h.max_its) {
+                h.done = 0;
+                h.reason = 0;
+                FLGPU_TRY(write_scal(c, k));
+            }
+            if (!h.done) {
+                hipLaunchKernelGGL((k_restart_vecs<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows,
+                                   k->r.p, v.r, v.z, v.dinv, jacobi ? 1 : 0, k->partials.p, k->counter.p,
+                                   k->scal.p, 0);
+                if (!jacobi) FLGPU_TRY(learned_apply<V>(c, k, v, false));
+                FLGPU_TRY(read_scal(c, k));
+                rmax = std::sqrt(h.rr);
+            }
+        }
+        if (use_learned && !h.done) {
+            window_its += check_every;
+            if (window_its >= stall_window) {
+                if (h.last_norm > stall_factor * window_norm) {
+                    // stalled: fall back to Jacobi-PCG from the current x
+                    use_learned = false;
+                    jacobi = true;
+                    info->fallback = 1;
+                    h.flexible = sp->ksp == FLGPU_KSP_FCG ? 1 : 0;
+                    FLGPU_TRY(write_scal(c, k));
+                    FLGPU_TRY(cg_restart<V>(c, k, v, true, false));
+                    FLGPU_TRY(read_scal(c, k));
+                    rmax = std::sqrt(h.rr);
+                }
+                window_its = 0;
+                window_norm = h.last_norm;
+            }
+        }
+    }
+    FLGPU_TRY(true_residual(c, k));
+    FLGPU_TRY(read_scal(c, k));
+    auto t1 = std::chrono::steady_clock::now();
+    info->seconds = std::chrono::duration<double>(t1 - t0).count();
+    info->its = h.its;
+    info->reason = h.reason;
+    info->true_relres = bnorm > 0 ? std::sqrt(h.true_rr) / bnorm : std::sqrt(h.true_rr);
+    info->precond_relres = h.rnorm0 > 0 ? h.last_norm / h.rnorm0 : h.last_norm;
+    return FLGPU_OK;
+}
+
+// ------------------------- learned apply ---------------------------------
+
+template <typename V>
+static int learned_apply(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, bool set_norm0) {
+    return learned_precond_apply<V>(c, v.r, v.z, v.y, v.w, k->partials.p, k->counter.p,
+                                    k->scal.p, set_norm0);
+}
+
+// ------------------------- GMRES ------------------------------------------
+
+// w = D^-1 (A v)  [left Jacobi]
+template <int L, typename V>
+__global__ void __launch_bounds__(kVecBlock)
+k_gm_matvec(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+            const V* __restrict__ val, const V* __restrict__ dinv, const V* __restrict__ x,
+            V* __restrict__ y) {
+    constexpr int ROWS = kVecBlock / L;
+    const int lane = threadIdx.x % L, rloc = threadIdx.x / L;
+    for (int grp = blockIdx.x; grp * ROWS < n; grp += gridDim.x) {
+        const int row = grp * ROWS + rloc;
+        V acc = (V)0;
+        if (row < n)
+            for (int k = rowptr[row] + lane; k < rowptr[row + 1]; k += L) acc += val[k] * x[col[k]];
+        acc = lane_sum<V>(acc, L);
+        if (row < n && lane == 0) y[row] = dinv[row] * acc;
+    }
+}
+
+// h[j] = w . V_j for j <= kk (one pass over w), partials per block
+template <typename V>
+__global__ void __launch_bounds__(kVecBlock)
+k_gm_dots(int n, int kk, const V* __restrict__ Vb, const V* __restrict__ w,
+          double* __restrict__ partials) {
+    __shared__ double sh[kVecBlock / 64][1];
+    for (int j = 0; j <= kk; j++) {
+        const V* vj = Vb + (size_t)n * j;
+        double s = 0.0;
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+            s += (double)w[i] * (double)vj[i];
+        double a[1] = {s};
+        block_sum<1>(a, sh);
+        if (threadIdx.x == 0) partials[(size_t)j * gridDim.x + blockIdx.x] = a[0];
+    }
+}
+
+// hcol[j] (+)= sum over blocks (fixed order); one block per j
+__global__ void k_gm_reduce(int kk, int nblk, const double* __restrict__ partials,
+                            double* __restrict__ hcol, int accumulate) {
+    __shared__ double sh[kVecBlock / 64][1];
+    const int j = blockIdx.x;
+    if (j > kk) return;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += partials[(size_t)j * nblk + b];
+    double a[1] = {s};
+    block_sum<1>(a, sh);
+    if (threadIdx.x == 0) hcol[j] = accumulate ? hcol[j] + a[0] : a[0];
+}
+
+// w -= sum_j h[j] V_j  (h from hcol, or its second-pass increment)
+template <typename V>
+__global__ void __launch_bounds__(kVecBlock)
+k_gm_orth(int n, int kk, const V* __restrict__ Vb, const double* __restrict__ h, V* __restrict__ w) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double acc = (double)w[i];
+        for (int j = 0; j <= kk; j++) acc -= h[j] * (double)Vb[(size_t)n * j + i];
+        w[i] = (V)acc;
+    }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kVecBlock)
+k_gm_scale(int n, const V* __restrict__ w, V* __restrict__ out, const double* __restrict__ nrm) {
+    const double s = nrm[0] > 0 ? 1.0 / nrm[0] : 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = (V)((double)w[i] * s);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kVecBlock)
+k_gm_norm(int n, const V* __restrict__ w, double* __restrict__ partials) {
+    __shared__ double sh[kVecBlock / 64][1];
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        s += (double)w[i] * (double)w[i];
+    double a[1] = {s};
+    block_sum<1>(a, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = a[0];
+}
+
+__global__ void k_gm_sqrt_reduce(int nblk, const double* __restrict__ partials, double* __restrict__ out) {
+    __shared__ double sh[kVecBlock / 64][1];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += partials[b];
+    double a[1] = {s};
+    block_sum<1>(a, sh);
+    if (threadIdx.x == 0) out[0] = sqrt(a[0]);
+}
+
+// x += sum_j y_j V_j
+template <typename V>
+__global__ void __launch_bounds__(kVecBlock)
+k_gm_update_x(int n, int kk, const V* __restrict__ Vb, const double* __restrict__ y, double* __restrict__ x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double acc = x[i];
+        for (int j = 0; j < kk; j++) acc += y[j] * (double)Vb[(size_t)n * j + i];
+        x[i] = acc;
+    }
+}
+
+// r_V = D^-1 r64  (left-preconditioned residual of the FP64 residual)
+template <typename V>
+__global__ void __launch_bounds__(kVecBlock)
+k_gm_precres(int n, const double* __restrict__ r64, const double* __restrict__ dinv, V* __restrict__ out) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = (V)(dinv[i] * r64[i]);
+}
+
+template <typename V>
+static int run_gmres(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info* info) {
+    KrylovWork* k = c->kw;
+    const int n = c->rows;
+    const int m = sp->restart > 0 ? sp->restart : 30;
+    const bool mixed = std::is_same<V, float>::value;
+    const V* val = mixed ? (const V*)k->val32.p : (const V*)k->val64.p;
+    const V* dinv = mixed ? (const V*)k->dinv32.p : (const V*)k->dinv.p;
+    V* Vb;
+    if (mixed) {
+        FLGPU_TRY(k->V32.alloc((size_t)n * (m + 1)));
+        Vb = (V*)k->V32.p;
+    } else {
+        FLGPU_TRY(k->V.alloc((size_t)n * (m + 1)));
+        Vb = (V*)k->V.p;
+    }
+    if (sp->pc == FLGPU_PC_NONE) {
+        std::vector<double> ones(n, 1.0);
+        std::vector<float> ones32(n, 1.0f);
+        FLGPU_HIP(hipMemcpyAsync(k->dinv.p, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        if (mixed)
+            FLGPU_HIP(hipMemcpyAsync(k->dinv32.p, ones32.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    }
+    FLGPU_TRY(k->hcol.alloc(m + 2));
+    FLGPU_TRY(k->partials.alloc((size_t)kMaxVecGrid * (m + 2)));
+    const int nb = k->nblk;
+    const int L = lanes_for(c);
+    const int rows_per_blk = kVecBlock / L;
+    int mgrid = (int)std::min<int64_t>(kMaxVecGrid, (n + rows_per_blk - 1) / rows_per_blk);
+    mgrid = std::max(mgrid, 1);
+    auto matvec = [&](const V* xin, V* yout) -> int {
+#define FL_GMV(LL)                                                                                         \
+    case LL:                                                                                               \
+        hipLaunchKernelGGL((k_gm_matvec<LL, V>), dim3(mgrid), dim3(kVecBlock), 0, c->stream, n, c->rowptr.p, \
+                           c->col.p, val, dinv, xin, yout);                                                \
+        break;
+        switch (L) {
+            FL_GMV(4)
+            FL_GMV(8)
+            FL_GMV(16)
+            FL_GMV(32)
+            FL_GMV(64)
+        }
+#undef FL_GMV
+        return FLGPU_OK;
+    };
+    std::vector<double> H((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), y(m), hc(m + 2);
+    double hn = 0.0;
+    int total = 0, reason = 0;
+    double rnorm0 = -1.0, last = 0.0;
+    Scal& h = *k->h_scal;
+    std::memset(&h, 0, sizeof h);
+    FLGPU_TRY(write_scal(c, k));
+    hipLaunchKernelGGL(k_dot2, dim3(nb), dim3(kVecBlock), 0, c->stream, n, k->b.p, k->partials.p, k->counter.p,
+                       k->scal.p);
+    FLGPU_TRY(read_scal(c, k));
+    const double bnorm = std::sqrt(h.bb);
+    auto t0 = std::chrono::steady_clock::now();
+    int refinements = 0;
+    while (!reason) {
+        // residual in FP64, preconditioned, in the working precision
+        FLGPU_TRY(true_residual(c, k));
+        hipLaunchKernelGGL((k_gm_precres<V>), dim3(nb), dim3(kVecBlock), 0, c->stream, n, k->r.p, k->dinv.p, Vb);
+        hipLaunchKernelGGL((k_gm_norm<V>), dim3(nb), dim3(kVecBlock), 0, c->stream, n, Vb, k->partials.p);
+        hipLaunchKernelGGL(k_gm_sqrt_reduce, dim3(1), dim3(kVecBlock), 0, c->stream, nb, k->partials.p, k->hcol.p);
+        FLGPU_HIP(hipMemcpyAsync(&hn, k->hcol.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        FLGPU_HIP(hipStreamSynchronize(c->stream));
+        refinements++;
+        const double beta = hn;
+        if (rnorm0 < 0) {
+            rnorm0 = beta;
+            last = beta;
+            if (beta == 0.0 || beta <= sp->atol) {
+                reason = KSP_CONVERGED_ATOL;
+                break;
+            }
+        }
+        hipLaunchKernelGGL((k_gm_scale<V>), dim3(nb), dim3(kVecBlock), 0, c->stream, n, Vb, Vb, k->hcol.p);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int kk;
+        for (kk = 0; kk < m && !reason; kk++) {
+            V* vk = Vb + (size_t)n * kk;
+            V* vn = Vb + (size_t)n * (kk + 1);
+            FLGPU_TRY(matvec(vk, vn));
+            // classical Gram-Schmidt, twice
+            for (int pass = 0; pass < 2; pass++) {
+                hipLaunchKernelGGL((k_gm_dots<V>), dim3(nb), dim3(kVecBlock), 0, c->stream, n, kk, Vb, vn,
+                                   k->partials.p);
+                hipLaunchKernelGGL(k_gm_reduce, dim3(kk + 1), dim3(kVecBlock), 0, c->stream, kk, nb, k->partials.p,
+                                   k->hcol.p + 1, 0);
+                hipLaunchKernelGGL((k_gm_orth<V>), dim3(nb), dim3(kVecBlock), 0, c->stream, n, kk, Vb,
+                                   k->hcol.p + 1, vn);
+                FLGPU_HIP(hipMemcpyAsync(hc.data(), k->hcol.p + 1, sizeof(double) * (kk + 1),
+                                         hipMemcpyDeviceToHost, c->stream));
+                FLGPU_HIP(hipStreamSynchronize(c->stream));
+                for (int j = 0; j <= kk; j++) H[j + (m + 1) * kk] = (pass == 0 ? 0.0 : H[j + (m + 1) * kk]) + hc[j];
+            }
+            hipLaunchKernelGGL((k_gm_norm<V>), dim3(nb), dim3(kVecBlock), 0, c->stream, n, vn, k->partials.p);
+            hipLaunchKernelGGL(k_gm_sqrt_reduce, dim3(1), dim3(kVecBlock), 0, c->stream, nb, k->partials.p, k->hcol.p);
+            hipLaunchKernelGGL((k_gm_scale<V>), dim3(nb), dim3(kVecBlock), 0, c->stream, n, vn, vn, k->hcol.p);
+            FLGPU_HIP(hipMemcpyAsync(&hn, k->hcol.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            FLGPU_HIP(hipStreamSynchronize(c->stream));
+            H[kk + 1 + (m + 1) * kk] = hn;
+            for (int j = 0; j < kk; j++) {
+                const double a = H[j + (m + 1) * kk], b2 = H[j + 1 + (m + 1) * kk];
+                H[j + (m + 1) * kk] = cs[j] * a + sn[j] * b2;
+                H[j + 1 + (m + 1) * kk] = -sn[j] * a + cs[j] * b2;
+            }
+            const double a = H[kk + (m + 1) * kk], b2 = H[kk + 1 + (m + 1) * kk];
+            const double den = std::sqrt(a * a + b2 * b2);
+            cs[kk] = den == 0.0 ? 1.0 : a / den;
+            sn[kk] = den == 0.0 ? 0.0 : b2 / den;
+            H[kk + (m + 1) * kk] = cs[kk] * a + sn[kk] * b2;
+            H[kk + 1 + (m + 1) * kk] = 0.0;
+            g[kk + 1] = -sn[kk] * g[kk];
+            g[kk] = cs[kk] * g[kk];
+            total++;
+            last = std::fabs(g[kk + 1]);
+            if (last != last) reason = KSP_DIVERGED_NANORINF;
+            else if (last <= sp->atol) reason = KSP_CONVERGED_ATOL;
+            else if (last <= sp->rtol * rnorm0) reason = KSP_CONVERGED_RTOL;
+            else if (total >= (sp->max_its > 0 ? sp->max_its : 10000)) reason = KSP_DIVERGED_ITS;
+        }
+        for (int i = kk - 1; i >= 0; i--) {
+            double acc = g[i];
+            for (int j = i + 1; j < kk; j++) acc -= H[i + (m + 1) * j] * y[j];
+            y[i] = acc / H[i + (m + 1) * i];
+        }
+        FLGPU_HIP(hipMemcpyAsync(k->hcol.p, y.data(), sizeof(double) * kk, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL((k_gm_update_x<V>), dim3(nb), dim3(kVecBlock), 0, c->stream, n, kk, Vb, k->hcol.p,
+                           k->x.p);
+        FLGPU_HIP(hipGetLastError());
+        if (mixed && reason > 0) {
+            // the FP32 estimate converged: test the FP64 residual
+            FLGPU_TRY(true_residual(c, k));
+            FLGPU_TRY(read_scal(c, k));
+            if (std::sqrt(h.true_rr) > std::max(sp->rtol * bnorm, sp->atol) &&
+                total < (sp->max_its > 0 ? sp->max_its : 10000))
+                reason = 0;
+        }
+    }
+    FLGPU_TRY(true_residual(c, k));
+    FLGPU_TRY(read_scal(c, k));
+    auto t1 = std::chrono::steady_clock::now();
+    info->seconds = std::chrono::duration<double>(t1 - t0).count();
+    info->its = total;
+    info->reason = reason;
+    info->true_relres = bnorm > 0 ? std::sqrt(h.true_rr) / bnorm : std::sqrt(h.true_rr);
+    info->precond_relres = rnorm0 > 0 ? last / rnorm0 : last;
+    info->refinements = mixed ? refinements : 0;
+    info->fallback = 0;
+    return FLGPU_OK;
+}
+
+int solve(flgpu_ctx* c, const flgpu_solver_params* sp, double* x_host, flgpu_solve_info* info_out) {
+    flgpu_solve_info info{};
+    const bool mixed = sp->precision == FLGPU_MIXED;
+    if (sp->precision != FLGPU_FP64 && sp->precision != FLGPU_MIXED)
+        return set_error(FLGPU_EINVAL, "flgpu_solve: precision %d", sp->precision);
+    if (sp->ksp != FLGPU_KSP_CG && sp->ksp != FLGPU_KSP_GMRES && sp->ksp != FLGPU_KSP_FCG)
+        return set_error(FLGPU_EINVAL, "flgpu_solve: ksp %d", sp->ksp);
+    if (sp->pc != FLGPU_PC_NONE && sp->pc != FLGPU_PC_JACOBI && sp->pc != FLGPU_PC_LEARNED)
+        return set_error(FLGPU_EINVAL, "flgpu_solve: pc %d", sp->pc);
+    if (sp->pc == FLGPU_PC_LEARNED && sp->ksp == FLGPU_KSP_GMRES)
+        return set_error(FLGPU_EUNSUPPORTED, "flgpu_solve: the learned preconditioner runs with cg/fcg");
+    FLGPU_TRY(ensure_work(c, mixed));
+    KrylovWork* k = c->kw;
+    const bool zero_x = sp->start_from_zero || !x_host;
+    if (x_host && !sp->start_from_zero)
+        FLGPU_HIP(hipMemcpyAsync(k->x.p, x_host, sizeof(double) * c->rows, hipMemcpyHostToDevice, c->stream));
+    FLGPU_TRY(build_reduced(c, k, mixed, zero_x));
+    if (sp->pc == FLGPU_PC_LEARNED) FLGPU_TRY(learned_prepare(c, mixed));
+    if (sp->ksp == FLGPU_KSP_GMRES) {
+        FLGPU_TRY(mixed ? run_gmres<float>(c, sp, &info) : run_gmres<double>(c, sp, &info));
+    } else {
+        const bool learned = sp->pc == FLGPU_PC_LEARNED;
+        FLGPU_TRY(mixed ? run_cg<float>(c, sp, &info, learned) : run_cg<double>(c, sp, &info, learned));
+    }
+    hipLaunchKernelGGL(k_finish_x, dim3(grid_for(c->rows, kVecBlock)), dim3(kVecBlock), 0, c->stream, c->rows,
+                       k->x.p, c->rhs.p, c->inactive.p, c->have_inactive ? 1 : 0);
+    FLGPU_HIP(hipGetLastError());
+    if (x_host)
+        FLGPU_HIP(hipMemcpyAsync(x_host, k->x.p, sizeof(double) * c->rows, hipMemcpyDeviceToHost, c->stream));
+    FLGPU_HIP(hipStreamSynchronize(c->stream));
+    k->have_x = true;
+    if (info_out) *info_out = info;
+    return FLGPU_OK;
+}
+
+void krylov_free(flgpu_ctx* c) {
+    delete c->kw;
+    c->kw = nullptr;
+}
+
+int solution_get(flgpu_ctx* c, double* x) {
+    if (!c->kw || !c->kw->have_x) return set_error(FLGPU_EINVAL, "flgpu_solution_get: no solution");
+    FLGPU_HIP(hipMemcpyAsync(x, c->kw->x.p, sizeof(double) * c->rows, hipMemcpyDeviceToHost, c->stream));
+    FLGPU_HIP(hipStreamSynchronize(c->stream));
+    return FLGPU_OK;
+}
+
+int csr_mult(flgpu_ctx* c, const double* x, double* y) {
+    FLGPU_TRY(ensure_work(c, false));
+    KrylovWork* k = c->kw;
+    DevBuf<double> dx, dy;
+    FLGPU_TRY(dx.upload(x, c->cols, c->stream));
+    FLGPU_TRY(dy.alloc(c->rows));
+    FLGPU_TRY((spmv<double, double, double>(c, k, c->A.p, dx.p, dy.p, nullptr, 0, 0, nullptr)));
+    FLGPU_HIP(hipMemcpyAsync(y, dy.p, sizeof(double) * c->rows, hipMemcpyDeviceToHost, c->stream));
+    FLGPU_HIP(hipStreamSynchronize(c->stream));
+    return FLGPU_OK;
+}
+
+int solver_prepare(flgpu_ctx* c, int precision) {
+    const bool mixed = precision == FLGPU_MIXED;
+    FLGPU_TRY(ensure_work(c, mixed));
+    FLGPU_TRY(build_reduced(c, c->kw, mixed, true));
+    FLGPU_HIP(hipMemsetAsync(c->kw->p.p, 0, sizeof(double) * c->rows, c->stream));
+    if (mixed) FLGPU_HIP(hipMemsetAsync(c->kw->p32.p, 0, sizeof(float) * c->rows, c->stream));
+    FLGPU_HIP(hipStreamSynchronize(c->stream));
+    c->kw->prepared_mixed = mixed;
+    c->kw->prepared = true;
+    return FLGPU_OK;
+}
+
+// Bare solver-matrix SpMV w = A p, timed with HIP events on the stream.
+int time_spmv(flgpu_ctx* c, int precision, int reps, double* ms) {
+    const bool mixed = precision == FLGPU_MIXED;
+    if (!c->kw || !c->kw->prepared || (mixed && !c->kw->prepared_mixed))
+        FLGPU_TRY(solver_prepare(c, precision));
+    KrylovWork* k = c->kw;
+    hipEvent_t e0, e1;
+    FLGPU_HIP(hipEventCreate(&e0));
+    FLGPU_HIP(hipEventCreate(&e1));
+    FLGPU_HIP(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; r++) {
+        if (mixed)
+            FLGPU_TRY((spmv<float, float, float>(c, k, k->val32.p, k->p32.p, k->w32.p, nullptr, 0, 0, nullptr)));
+        else
+            FLGPU_TRY((spmv<double, double, double>(c, k, k->val64.p, k->p.p, k->w.p, nullptr, 0, 0, nullptr)));
+    }
+    FLGPU_HIP(hipEventRecord(e1, c->stream));
+    FLGPU_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    FLGPU_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / reps;
+    return FLGPU_OK;
+}
+
+}  // namespace flgpu

@@ -1,0 +1,276 @@
+"""Synthetic meshes and fields of the benchmark configurations.
+
+Everything is regenerated bit-identically from (configuration, seed) with a
+portable splitmix64 counter RNG, so the GPU library, the tests and the CPU
+baseline see the same inputs.  Meshes are Fluidity-shaped: 1-based,
+element-major ndglno; coordinates dim x nodes column-major (stored here as
+a C-ordered (nodes, dim) array, which is the same memory).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    """splitmix64 finaliser of a uint64 array (wrapping arithmetic)."""
+    with np.errstate(over="ignore"):
+        z = (x.astype(np.uint64) + np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform_pm1(seed: int, ids: np.ndarray) -> np.ndarray:
+    """Uniform [-1, 1) from splitmix64(seed ^ id)."""
+    s = np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
+    z = splitmix64(np.asarray(ids, dtype=np.uint64) ^ s)
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return 2.0 * u - 1.0
+
+
+def uniform01(seed: int, ids: np.ndarray) -> np.ndarray:
+    return 0.5 * (uniform_pm1(seed, ids) + 1.0)
+
+
+def normal(seed: int, ids: np.ndarray) -> np.ndarray:
+    """Standard normals by Box-Muller on two splitmix streams."""
+    ids = np.asarray(ids, dtype=np.uint64)
+    u1 = uniform01(seed, 2 * ids)
+    u2 = uniform01(seed, 2 * ids + np.uint64(1))
+    u1 = np.maximum(u1, 1e-300)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+@dataclass
+class Mesh:
+    dim: int
+    nele: int
+    # coordinate (linear) mesh
+    nx: int
+    ndglno_x: np.ndarray          # (nele*(dim+1),) int32, 1-based
+    X: np.ndarray                 # (nx, dim) float64
+    # field mesh (P1: same as the coordinate mesh)
+    degree: int
+    nnodes: int
+    ndglno: np.ndarray            # (nele*loc,) int32, 1-based
+    node_pos: np.ndarray          # (nnodes, dim) positions of the field nodes
+    boundary: np.ndarray          # 1-based field nodes on the domain boundary
+    name: str = ""
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def loc(self) -> int:
+        return self.ndglno.size // self.nele
+
+    @property
+    def loc_x(self) -> int:
+        return self.dim + 1
+
+
+# Kuhn (Freudenthal) split of a hexahedron into 6 tetrahedra along the main
+# diagonal: for each axis permutation (a, b, c) the path
+# v000 -> +e_a -> +e_a+e_b -> v111.
+_KUHN_PERMS = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+
+
+def _kuhn_corners():
+    out = []
+    for a, b, _ in _KUHN_PERMS:
+        p0 = (0, 0, 0)
+        p1 = [0, 0, 0]
+        p1[a] = 1
+        p2 = list(p1)
+        p2[b] = 1
+        out.append([p0, tuple(p1), tuple(p2), (1, 1, 1)])
+    return np.array(out, dtype=np.int64)  # (6, 4, 3)
+
+
+def cube_tets(n: int, jitter: float = 0.0, seed: int = 0, grading: float | None = None,
+              degree: int = 1) -> Mesh:
+    """Unit cube, n^3 hexahedra x 6 Kuhn tetrahedra.  Interior vertices are
+    moved by jitter*h*U(-1,1) per coordinate; grading = size ratio of a tanh
+    map of z concentrated at z = 0.5 (None: uniform).  degree 2 adds the
+    edge nodes, numbered on the (2n+1)^3 half-spacing lattice."""
+    m = n + 1
+    i, j, k = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+    # node id = i + m*j + m*m*k (x fastest)
+    ii, jj, kk = (a.transpose(2, 1, 0).ravel() for a in (i, j, k))
+    nx = m ** 3
+    h = 1.0 / n
+    xyz = np.stack([ii * h, jj * h, kk * h], axis=1).astype(np.float64)
+    if jitter:
+        interior = (ii > 0) & (ii < n) & (jj > 0) & (jj < n) & (kk > 0) & (kk < n)
+        ids = np.arange(nx, dtype=np.uint64)
+        for d in range(3):
+            u = uniform_pm1(seed, ids * np.uint64(3) + np.uint64(d))
+            xyz[:, d] = np.where(interior, xyz[:, d] + jitter * h * u, xyz[:, d])
+    if grading is not None:
+        s = np.arccosh(np.sqrt(grading))
+        zeta = xyz[:, 2]
+        xyz[:, 2] = 0.5 + 0.5 * np.tanh(s * (2.0 * zeta - 1.0)) / np.tanh(s)
+    # elements
+    hi, hj, hk = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    hi, hj, hk = (a.transpose(2, 1, 0).ravel() for a in (hi, hj, hk))  # hex id = i + n j + n^2 k
+    corners = _kuhn_corners()  # (6,4,3)
+    nhex = n ** 3
+    vi = hi[:, None, None] + corners[None, :, :, 0]
+    vj = hj[:, None, None] + corners[None, :, :, 1]
+    vk = hk[:, None, None] + corners[None, :, :, 2]
+    conn_x = (vi + m * vj + m * m * vk).reshape(nhex * 6, 4)
+    nele = nhex * 6
+    ndglno_x = (conn_x + 1).astype(np.int32).ravel()
+    bnd_x = np.nonzero((ii == 0) | (ii == n) | (jj == 0) | (jj == n) | (kk == 0) | (kk == n))[0] + 1
+    if degree == 1:
+        return Mesh(3, nele, nx, ndglno_x, xyz, 1, nx, ndglno_x, xyz, bnd_x.astype(np.int32),
+                    name=f"cube{n}")
+    # P2 on the half-spacing lattice
+    mf = 2 * n + 1
+    fi = 2 * vi
+    fj = 2 * vj
+    fk = 2 * vk
+    fid = fi + mf * fj + mf * mf * fk  # (nhex, 6, 4) fine ids of the vertices
+    fid = fid.reshape(nele, 4)
+    F = np.stack([fi, fj, fk], axis=-1).reshape(nele, 4, 3)
+    order = []  # v1,e12,v2,e13,e23,v3,e14,e24,e34,v4
+    for b in range(4):
+        for a in range(b):
+            order.append((a, b))
+        order.append((b, b))
+    cols = []
+    for a, b in order:
+        g = (F[:, a, :] + F[:, b, :]) // 2
+        cols.append(g[:, 0] + mf * g[:, 1] + mf * mf * g[:, 2])
+    conn = np.stack(cols, axis=1)
+    nnodes = mf ** 3
+    fi3, fj3, fk3 = np.meshgrid(np.arange(mf), np.arange(mf), np.arange(mf), indexing="ij")
+    fi3, fj3, fk3 = (a.transpose(2, 1, 0).ravel() for a in (fi3, fj3, fk3))
+    # field-node positions: midpoints of the (jittered) coordinate edges
+    pos = np.zeros((nnodes, 3))
+    for q, (a, b) in enumerate(order):
+        pos[conn[:, q]] = 0.5 * (xyz[conn_x[:, a]] + xyz[conn_x[:, b]])
+    bnd = np.nonzero((fi3 == 0) | (fi3 == 2 * n) | (fj3 == 0) | (fj3 == 2 * n) | (fk3 == 0) | (fk3 == 2 * n))[0] + 1
+    return Mesh(3, nele, nx, ndglno_x, xyz, 2, nnodes, (conn + 1).astype(np.int32).ravel(), pos,
+                bnd.astype(np.int32), name=f"cube{n}p2")
+
+
+def square_tris(n: int, jitter: float = 0.0, seed: int = 0) -> Mesh:
+    """Unit square, n x n cells each split by the (i,j)-(i+1,j+1) diagonal
+    into two P1 triangles; interior vertices jittered by jitter*h."""
+    m = n + 1
+    i, j = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    ii, jj = i.T.ravel(), j.T.ravel()  # node id = i + m j
+    h = 1.0 / n
+    xy = np.stack([ii * h, jj * h], axis=1).astype(np.float64)
+    if jitter:
+        interior = (ii > 0) & (ii < n) & (jj > 0) & (jj < n)
+        ids = np.arange(m * m, dtype=np.uint64)
+        for d in range(2):
+            u = uniform_pm1(seed, ids * np.uint64(2) + np.uint64(d))
+            xy[:, d] = np.where(interior, xy[:, d] + jitter * h * u, xy[:, d])
+    ci, cj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    ci, cj = ci.T.ravel(), cj.T.ravel()  # cell id = i + n j
+    v00 = ci + m * cj
+    v10 = v00 + 1
+    v01 = v00 + m
+    v11 = v01 + 1
+    t1 = np.stack([v00, v10, v11], axis=1)
+    t2 = np.stack([v00, v11, v01], axis=1)
+    conn = np.stack([t1, t2], axis=1).reshape(-1, 3)
+    nd = (conn + 1).astype(np.int32).ravel()
+    bnd = np.nonzero((ii == 0) | (ii == n) | (jj == 0) | (jj == n))[0] + 1
+    return Mesh(2, conn.shape[0], m * m, nd, xy, 1, m * m, nd, xy, bnd.astype(np.int32), name=f"square{n}")
+
+
+def grf_lognormal(pos: np.ndarray, seed: int, corr_len: float = 0.1, contrast: float = 1e4,
+                  nfeat: int = 64) -> np.ndarray:
+    """exp of a Gaussian random field (random Fourier features, correlation
+    length corr_len), scaled so that max/min over the nodes = contrast."""
+    dim = pos.shape[1]
+    fid = np.arange(nfeat * dim, dtype=np.uint64)
+    omega = normal(seed, fid).reshape(nfeat, dim) / corr_len
+    phi = 2.0 * np.pi * uniform01(seed ^ 0x5DEECE66D, np.arange(nfeat, dtype=np.uint64))
+    g = np.zeros(pos.shape[0])
+    for f in range(nfeat):
+        g += np.cos(pos @ omega[f] + phi[f])
+    g *= np.sqrt(2.0 / nfeat)
+    g = (g - g.mean()) / max(g.std(), 1e-300)
+    span = g.max() - g.min()
+    sigma = np.log(contrast) / span if span > 0 else 0.0
+    return np.exp(sigma * (g - g.min()))
+
+
+def abc_flow(pos: np.ndarray, A: float = 1.0, B: float = 1.0, Cc: float = 1.0) -> np.ndarray:
+    """Arnold-Beltrami-Childress velocity on the unit cube (2 pi periodic
+    scaled to [0,1]); divergence free."""
+    x, y, z = (2.0 * np.pi * pos[:, d] for d in range(3))
+    return np.stack([A * np.sin(z) + Cc * np.cos(y),
+                     B * np.sin(x) + A * np.cos(z),
+                     Cc * np.sin(y) + B * np.cos(x)], axis=1)
+
+
+def sin_source(pos: np.ndarray) -> np.ndarray:
+    out = np.ones(pos.shape[0])
+    for d in range(pos.shape[1]):
+        out *= np.sin(2.0 * np.pi * pos[:, d])
+    return out
+
+
+@dataclass
+class Config:
+    """A benchmark configuration: mesh, fields and equation options."""
+    name: str
+    mesh: Mesh
+    quad_degree: int
+    T: np.ndarray
+    U: np.ndarray | None
+    kappa: np.ndarray | float | None     # nodal (N,) iso, or constant
+    source: np.ndarray | None
+    absorption: np.ndarray | None
+    dt: float
+    theta: float
+    beta: float
+    have_mass: bool
+    have_advection: bool
+    solver: str                           # "cg" or "gmres"
+    rtol: float
+
+
+def config(name: str, seed: int = 1, scale: int | None = None) -> Config:
+    """Configurations of the benchmark (scale overrides the cells per side
+    for quick tests)."""
+    if name == "c1":
+        n = scale or 64
+        m = square_tris(n, jitter=0.2, seed=seed)
+        p = m.node_pos
+        T = np.exp(-((p[:, 0] - 0.3) ** 2 + (p[:, 1] - 0.5) ** 2) / (2 * 0.05 ** 2))
+        U = np.stack([-(p[:, 1] - 0.5), p[:, 0] - 0.5], axis=1)
+        return Config("c1", m, 2, T, U, 1e-3, None, None, 0.01, 0.5, 0.0, True, True, "gmres", 1e-10)
+    if name == "c2":
+        n = scale or 64
+        m = cube_tets(n, jitter=0.15, seed=seed)
+        return Config("c2", m, 2, np.zeros(m.nnodes), None, 1.0, sin_source(m.node_pos), None,
+                      1.0, 1.0, 0.0, False, False, "cg", 1e-8)
+    if name == "c3":
+        n = scale or 40
+        m = cube_tets(n, jitter=0.0, seed=seed, degree=2)
+        p = m.node_pos
+        T = np.sin(np.pi * p[:, 0]) * np.sin(np.pi * p[:, 1]) * np.sin(np.pi * p[:, 2])
+        return Config("c3", m, 4, T, abc_flow(p), 1e-3, None, None, 0.005, 0.5, 0.0, True, True,
+                      "gmres", 1e-8)
+    if name == "c4":
+        n = scale or 40
+        m = cube_tets(n, jitter=0.15, seed=seed)
+        kap = grf_lognormal(m.node_pos, seed, 0.1, 1e4)
+        return Config("c4", m, 2, np.zeros(m.nnodes), None, kap, sin_source(m.node_pos), None,
+                      1.0, 1.0, 0.0, False, False, "cg", 1e-8)
+    if name == "c5":
+        n = scale or 110
+        m = cube_tets(n, jitter=0.15, seed=seed, grading=100.0)
+        kap = grf_lognormal(m.node_pos, seed, 0.1, 1e3)
+        return Config("c5", m, 2, np.zeros(m.nnodes), None, kap, sin_source(m.node_pos), None,
+                      1.0, 1.0, 0.0, False, False, "cg", 1e-8)
+    raise ValueError(name)

@@ -1,0 +1,302 @@
+/*
+ * flgpu.h -- C ABI of the MI355X (gfx950) continuous-Galerkin
+ * advection-diffusion library.
+ *
+ * The library replaces two cut points of Fluidity's CG scalar-field solve:
+ *   (i)  the element loop of assemble_advection_diffusion_cg
+ *        (assemble/Advection_Diffusion_CG.F90:565-598: transform cache,
+ *        colouring, colour x element loop, csr addto), and
+ *   (ii) the petsc_solve call of solve_advection_diffusion_cg
+ *        (assemble/Advection_Diffusion_CG.F90:1389 ->
+ *        assemble/Petsc_Solve_State.F90:64-117 -> femtools/Solvers.F90:111-187).
+ * Fortran reaches it through ISO_C_BINDING (fluidity_amd/fortran/flgpu.F90);
+ * every argument is a plain pointer or scalar.
+ *
+ * Conventions (the Fortran ones, so arrays can be passed straight from
+ * mesh_type / field / csr_sparsity components):
+ *   - integers are int32 (Fortran default integer), reals are double;
+ *   - node and element numbers are 1-based; ndglno is element-major
+ *     (ndglno(loc*(ele-1)+i));
+ *   - vector fields are dim x nodes column-major (vector_field%val),
+ *     tensors dim x dim x nodes (tensor_field%val);
+ *   - shape tables are element_type%n(loc,ngi), %dn(loc,ngi,dim),
+ *     quadrature%weight(ngi), all column-major.
+ * Every entry point returns 0 on success and a nonzero FLGPU_E* code on
+ * failure; flgpu_last_error() then holds a message (the Fortran shim turns
+ * it into FLAbort, as csr_addto does for an out-of-pattern insertion,
+ * femtools/Sparse_Tools.F90:2645-2647).
+ *
+ * Threading: one context per process and GPU, not re-entrant; call it from
+ * the serial region, as Fluidity does for construct_cache
+ * (femtools/Transform_elements.F90:238).
+ */
+#ifndef FLGPU_H
+#define FLGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FLGPU_OK = 0,
+    FLGPU_EINVAL = 1,     /* bad argument or call order */
+    FLGPU_EHIP = 2,       /* HIP runtime error */
+    FLGPU_EPATTERN = 3,   /* element entry outside the CSR sparsity */
+    FLGPU_ECOLOUR = 4,    /* colouring invalid for the mesh */
+    FLGPU_ENOMEM = 5,
+    FLGPU_EUNSUPPORTED = 6
+};
+
+typedef struct flgpu_ctx flgpu_ctx;
+
+/* ---- lifetime ------------------------------------------------------ */
+
+/* New context on HIP device `device` (one HIP stream).  No counterpart in
+ * Fluidity: it replaces the implicit module state (transform cache,
+ * colouring sets, cached ksp) that the Fortran modules keep. */
+int flgpu_init(flgpu_ctx **ctx, int32_t device);
+int flgpu_finalize(flgpu_ctx *ctx);
+/* Message of the last failure on this thread. */
+const char *flgpu_last_error(void);
+
+/* ---- mesh binding (setup; cached on the device) ----------------------- */
+
+/* Coordinate field: replaces the element geometry cache
+ * (femtools/Transform_elements.F90:81-100 arrays, :131-168 validity keyed
+ * by X%refcount%id and eventcount(EVENT_MESH_MOVEMENT), :227-306
+ * construct_cache).  A call with the same (key_id, key_event) is a no-op.
+ * Linear simplices only (loc_x == dim+1), dim 2 or 3.
+ * ndglno_x: loc_x*nele; X: dim x nx. */
+int flgpu_mesh_bind(flgpu_ctx *ctx, int64_t key_id, int64_t key_event,
+                    int32_t dim, int32_t loc_x, int32_t nele, int32_t nx,
+                    const int32_t *ndglno_x, const double *X);
+
+/* Field mesh and element tables of T (ele_shape(t,ele)): element_type%n,
+ * %dn, quadrature%weight (femtools/Elements.F90:38-65) and the coordinate
+ * element's dn at the same quadrature (x_dn(loc_x,ngi,dim)).  Lagrange
+ * simplices of degree 1 or 2.  Velocity, diffusivity, source and
+ * absorption live on this mesh (or are constant). */
+int flgpu_field_mesh_bind(flgpu_ctx *ctx, int32_t loc, int32_t ngi,
+                          int32_t nnodes, const int32_t *ndglno,
+                          const double *n, const double *dn, const double *w,
+                          const double *x_dn);
+
+/* First-order sparsity of the field mesh: csr_sparsity%findrm(rows+1),
+ * %colm(nnz), %centrm(rows), 1-based, sorted rows
+ * (femtools/Sparse_Tools.F90:48-71; built by make_sparsity,
+ * femtools/Sparsity_Patterns.F90:47-85). */
+int flgpu_sparsity_bind(flgpu_ctx *ctx, int32_t rows, int32_t cols,
+                        const int32_t *findrm, const int32_t *colm,
+                        const int32_t *centrm);
+
+/* COLOURING_CG1 element colour sets (femtools/Colouring.F90:85-262),
+ * flattened: col_ptr(ncol+1) 1-based offsets into elems; each set lists
+ * its 1-based element ids ascending (the Judy1 fetch order,
+ * femtools/Data_structures_C.c:42-47).  Validated on bind: no two
+ * elements of one colour may share a node (verify_colour_sparsity,
+ * femtools/Colouring.F90:202-221). */
+int flgpu_colouring_bind(flgpu_ctx *ctx, int32_t ncol, const int32_t *col_ptr,
+                         const int32_t *elems);
+
+/* ---- host-side setup helpers (the Fortran caller has its own) -------- */
+
+/* make_sparsity (femtools/Sparsity_Patterns.F90:47-85, :299-428) for a
+ * continuous mesh: row i = ascending union of the nodes of the elements
+ * that contain i.  Call with colm == NULL to get nnz in *nnz; then again
+ * with colm of that size.  findrm(nnodes+1), centrm(nnodes), 1-based. */
+int flgpu_make_sparsity(int32_t nnodes, int32_t nele, int32_t loc,
+                        const int32_t *ndglno, int32_t *findrm, int32_t *colm,
+                        int32_t *centrm, int64_t *nnz);
+
+/* get_mesh_colouring(..., COLOURING_CG1, ...) (femtools/Colouring.F90:
+ * 85-199): greedy first-fit over elements in index order, neighbours =
+ * elements sharing a node.  colour(nele) 1-based; *ncol = colours used. */
+int flgpu_colour_mesh(int32_t nnodes, int32_t nele, int32_t loc,
+                      const int32_t *ndglno, int32_t *colour, int32_t *ncol);
+
+/* ---- fields ------------------------------------------------------- */
+
+enum {
+    FLGPU_FIELD_T = 0,          /* scalar, the prognostic field (old value) */
+    FLGPU_FIELD_VELOCITY = 1,   /* vector dim x N */
+    FLGPU_FIELD_DIFFUSIVITY = 2,/* tensor dim x dim x N */
+    FLGPU_FIELD_SOURCE = 3,     /* scalar */
+    FLGPU_FIELD_ABSORPTION = 4  /* scalar */
+};
+
+/* Upload a field to its device mirror.  constant != 0: FIELD_TYPE_CONSTANT,
+ * values holds one value (ncomp numbers); it is interpolated like a nodal
+ * field, as ele_val_at_quad does (femtools/Fields_Base.F90:2256-2310). */
+int flgpu_field_set(flgpu_ctx *ctx, int32_t which, int32_t constant,
+                    const double *values);
+
+/* ---- assembly --------------------------------------------------------- */
+
+typedef struct {
+    double dt, theta, beta;          /* dt_theta = dt*theta; beta =
+                                        conservative_advection */
+    int32_t have_mass, lump_mass;
+    int32_t have_advection, integrate_by_parts;
+    int32_t have_diffusivity, isotropic_diffusivity;
+    int32_t have_source, have_absorption;
+    int32_t precision;               /* FLGPU_FP64 or FLGPU_FP32 element
+                                        arithmetic (values stored FP64) */
+    int32_t scheme;                  /* FLGPU_ASM_* */
+} flgpu_advdif_params;
+
+enum { FLGPU_FP64 = 0, FLGPU_FP32 = 1, FLGPU_MIXED = 1 };
+enum {
+    FLGPU_ASM_AUTO = 0,     /* fastest available */
+    FLGPU_ASM_COLOUR = 1,   /* one launch per colour, element per thread,
+                               read-modify-write of the CSR in HBM */
+    FLGPU_ASM_BLOCKED = 2   /* row blocks in LDS, colours in order inside
+                               each block (same bits as FLGPU_ASM_COLOUR) */
+};
+
+/* assemble_advection_diffusion_cg's element loop
+ * (assemble/Advection_Diffusion_CG.F90:558-600 -> :702-1202): zero the
+ * matrix and rhs, then add every element's contribution colour by colour.
+ * Inputs are the device mirrors set by flgpu_field_set; the result stays
+ * on the device (flgpu_matrix_get copies it out). */
+int flgpu_advdif_assemble_resident(flgpu_ctx *ctx, const flgpu_advdif_params *p);
+
+/* One-shot form for the Fortran drop-in: upload the non-NULL fields
+ * (nodal), assemble, and copy matrix%val (nnz) and rhs%val (rows) back
+ * when A_val / rhs are non-NULL. */
+int flgpu_advdif_assemble(flgpu_ctx *ctx, const flgpu_advdif_params *p,
+                          const double *T, const double *U,
+                          const double *kappa, const double *src,
+                          const double *absorp, double *A_val, double *rhs);
+
+/* Strong Dirichlet (femtools/Boundary_Conditions.F90:1983-2025): mark the
+ * rows inactive (set_inactive, femtools/Sparse_Tools.F90:374-376) and set
+ * rhs(node) = (value - T(node))/dt (dt > 0) or value (dt <= 0) on the
+ * device-resident rhs.  nodes are 1-based. */
+int flgpu_dirichlet_apply(flgpu_ctx *ctx, int32_t nbc, const int32_t *nodes,
+                          const double *values, double dt);
+
+/* Copy the device-resident matrix values / rhs / inactive mask to host
+ * arrays (any may be NULL). */
+int flgpu_matrix_get(flgpu_ctx *ctx, double *A_val, double *rhs,
+                     int32_t *inactive);
+
+/* Replace the device-resident matrix values, rhs and inactive mask (any
+ * may be NULL; inactive NULL clears the mask only when A_val is given). */
+int flgpu_matrix_set(flgpu_ctx *ctx, const double *A_val, const double *rhs,
+                     const int32_t *inactive);
+
+/* ---- solve ------------------------------------------------------------ */
+
+enum { FLGPU_KSP_CG = 0, FLGPU_KSP_GMRES = 1, FLGPU_KSP_FCG = 2 };
+enum { FLGPU_PC_NONE = 0, FLGPU_PC_JACOBI = 1, FLGPU_PC_LEARNED = 2 };
+
+typedef struct {
+    int32_t ksp, pc;
+    double rtol, atol;          /* KSPSetTolerances (femtools/Solvers.F90:1646) */
+    int32_t max_its, restart;
+    int32_t precision;          /* FLGPU_FP64, or FLGPU_MIXED: FP32 inner
+                                   iterations with FP64 residual, update and
+                                   reductions */
+    int32_t start_from_zero;    /* ignore the initial x (start_from_zero,
+                                   femtools/Solvers.F90:1648-1655) */
+    int32_t check_every;        /* host polls convergence every k its (0=16) */
+    double stall_factor;        /* learned pc: fall back to Jacobi when the
+                                   residual has not dropped by this factor
+                                   over stall_window its (0 = 0.9) */
+    int32_t stall_window;       /* (0 = 50) */
+    int32_t norm_type;          /* FLGPU_NORM_*: the norm of the convergence
+                                   test (PETSc KSPSetNormType) */
+} flgpu_solver_params;
+
+enum {
+    FLGPU_NORM_PRECONDITIONED = 0,   /* ||M^-1 r||, PETSc's default for CG */
+    FLGPU_NORM_UNPRECONDITIONED = 1  /* ||r||: the true-residual test */
+};
+
+typedef struct {
+    int32_t its;          /* iterations taken (iterations_taken) */
+    int32_t reason;       /* PETSc KSPConvergedReason numbering: >0 converged,
+                             -3 max its, -9 NaN/Inf, ... */
+    double true_relres;   /* ||b - A x||_2 / ||b||_2 of the reduced system,
+                             recomputed in FP64 after the solve */
+    double precond_relres;/* last convergence-test norm / first one */
+    int32_t fallback;     /* 1 if the learned preconditioner stalled and the
+                             solve finished with Jacobi */
+    int32_t refinements;  /* FP64 residual replacements (mixed precision) */
+    double seconds;       /* device time of the Krylov loop */
+} flgpu_solve_info;
+
+/* petsc_solve on the device-resident system (femtools/Solvers.F90:111-187):
+ * the inactive rows become ghost rows with the diagonal ghost_pivot
+ * (femtools/Petsc_Tools.F90:959-984, :1045-1057), the inactive columns are
+ * lifted into the rhs (femtools/Solvers.F90:1072-1099), the Krylov method
+ * runs on the reduced system, and the ghost entries of x are set to the
+ * rhs values (femtools/Petsc_Tools.F90:724-731).  x: in = initial guess
+ * (unless start_from_zero), out = solution; NULL keeps x on the device. */
+int flgpu_solve(flgpu_ctx *ctx, const flgpu_solver_params *sp, double *x,
+                flgpu_solve_info *info);
+
+/* Drop-in form of (ii): takes host arrays like petsc_solve(x, matrix, rhs).
+ * A_val / rhs / inactive replace the device-resident system when non-NULL. */
+int flgpu_petsc_solve(flgpu_ctx *ctx, const flgpu_solver_params *sp,
+                      const double *A_val, const int32_t *inactive,
+                      const double *rhs, double *x, int32_t *its,
+                      int32_t *reason, double *true_relres);
+
+/* Copy the last device-resident solution (rows) to the host. */
+int flgpu_solution_get(flgpu_ctx *ctx, double *x);
+
+/* y = A x with the device-resident assembled matrix (csr_mult,
+ * femtools/Sparse_Tools.F90:3822-3844); host arrays. */
+int flgpu_csr_mult(flgpu_ctx *ctx, const double *x, double *y);
+
+/* ---- learned preconditioner --------------------------------------------
+ * A message-passing network on the matrix graph (3 layers, hidden 32,
+ * BF16 MFMA with FP32 accumulation) predicts a lower-triangular factor G on
+ * the IC(0) pattern (lower triangle of A); the preconditioner is
+ * M^-1 = G^T G.  No counterpart in Fluidity (new component). */
+
+/* Number of float parameters of the network. */
+int flgpu_learned_param_count(void);
+/* Set / get the network parameters (float32, flgpu_learned_param_count). */
+int flgpu_learned_set_params(flgpu_ctx *ctx, const float *params);
+int flgpu_learned_get_params(flgpu_ctx *ctx, float *params);
+/* Deterministic initialisation from a seed. */
+int flgpu_learned_init_params(flgpu_ctx *ctx, uint64_t seed);
+/* One Adam step on the device-resident system (after assembly + Dirichlet):
+ * loss = E_z ||z - G^T G A z||^2 / ||z||^2 over nprobe Rademacher probes.
+ * *loss gets the loss before the step. */
+int flgpu_learned_train_step(flgpu_ctx *ctx, int32_t nprobe, uint64_t seed,
+                             float lr, float *loss);
+/* Loss and the parameter gradient without updating (grad may be NULL). */
+int flgpu_learned_loss_grad(flgpu_ctx *ctx, int32_t nprobe, uint64_t seed,
+                            float *loss, float *grad);
+/* Build G for the device-resident system (done by flgpu_solve when
+ * pc == FLGPU_PC_LEARNED); values copied out when G_val != NULL (lower
+ * pattern, row-major CSR of the lower triangle incl. diagonal). */
+int flgpu_learned_factor(flgpu_ctx *ctx, float *G_val, int64_t *nnz_lower);
+
+/* ---- measurement --------------------------------------------------------
+ * Device-time measurement on the library's own stream with HIP events (no
+ * counterpart in Fluidity, which times with its profiler keys
+ * advection_diffusion_loop / solve, femtools/Profiler.cpp:33-174). */
+
+/* reps calls of the resident assembly; *ms = mean device milliseconds per
+ * call (the element loop only: zeroing included, host checks excluded). */
+int flgpu_time_assemble(flgpu_ctx *ctx, const flgpu_advdif_params *p,
+                        int32_t reps, double *ms);
+/* reps solver-matrix SpMVs (FLGPU_FP64 or FLGPU_MIXED values) after a solve
+ * or flgpu_solver_prepare; *ms = mean device milliseconds per product. */
+int flgpu_time_spmv(flgpu_ctx *ctx, int32_t precision, int32_t reps,
+                    double *ms);
+/* Build the reduced system of flgpu_solve without iterating (for timing). */
+int flgpu_solver_prepare(flgpu_ctx *ctx, int32_t precision);
+/* Number of CSR rows / nonzeros / colours bound (any pointer may be NULL). */
+int flgpu_sizes(flgpu_ctx *ctx, int32_t *rows, int64_t *nnz, int32_t *ncol);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLGPU_H */

@@ -1,0 +1,955 @@
+/*
+ * fl_oracle.c -- TEST INFRASTRUCTURE ONLY (the oracle).
+ *
+ * A plain-C restatement of Fluidity's CG advection-diffusion hot path, used
+ * as the checker for the GPU library and as the timed CPU baseline.  It is
+ * compiled with -ffp-contract=off so every expression is evaluated in the
+ * order the Fortran source writes it (Fluidity's own release build adds
+ * -ffast-math, so its bits are only defined up to reassociation).
+ *
+ * What pins each part (tests/test_oracle_*.py):
+ *   orc_quadrature      monomial exactness p!/(p+d)! (femtools/tests/
+ *                       test_quadrature.F90), sum w = 1/2 (tri) and 1/6
+ *                       (tet), literal constants of Quadrature.F90.
+ *   orc_shape_tables    partition of unity, sum of gradients = 0, nodal
+ *                       interpolation of P2, exact integrals
+ *                       (femtools/tests/test_shape_functions.F90).
+ *   element arithmetic  closed forms: P1 stiffness |T| G G^T, consistent
+ *                       mass |T|/((d+1)(d+2))(1+delta), zero row sums of K
+ *                       and of the non-IBP advection matrix, P1/P2 patch
+ *                       tests, sum detwei = |T| (test_jacobian.F90).
+ *   orc_make_sparsity   brute-force dense set union on small meshes.
+ *   orc_colour_elements validity, partition, and an independent greedy.
+ *   orc_assemble        dense element-by-element sum on small meshes.
+ *   orc_petsc_solve     dense reduced-system solve (numpy) on small meshes;
+ *                       scipy spsolve on the 2-D case; true residual.
+ * Iteration counts of PETSc KSPSolve are "parity unpinned": PETSc is not
+ * vendored by the reference and only "converged" (reason > 0) is tested
+ * there.
+ */
+#include "fl_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAXLOC 10
+#define MAXGI 16
+#define MAXDIM 3
+
+static double wall_now(void)
+{
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+}
+
+/* ------------------------------------------------------------------ */
+/* Quadrature (femtools/Quadrature.F90)                                 */
+/* ------------------------------------------------------------------ */
+
+/* Expand one generator: permutation columns give, per point, the index
+ * (1-based) into coords of each barycentric coordinate
+ * (expand_quadrature_template, Quadrature.F90:572-605). */
+static int expand(int nv, int npts, const int *perm, const double *coords,
+                  double weight, double *l, double *w, int dk)
+{
+    for (int k = 0; k < npts; k++) {
+        for (int j = 0; j < nv; j++) {
+            int pj = perm[k * nv + j];
+            l[(dk + k) * nv + j] = pj ? coords[pj - 1] : 0.0;
+        }
+        w[dk + k] = weight;
+    }
+    return dk + npts;
+}
+
+int orc_quadrature(int dim, int degree, double *l, double *w)
+{
+    /* tet_permutations(7) (centroid), (8) Quadrature.F90 tet perm table,
+     * (9); tri_permutations(4), (5). */
+    static const int tet7[4] = {1, 1, 1, 1};
+    static const int tet8[16] = {1, 1, 1, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 1, 1, 1};
+    static const int tet9[24] = {1, 1, 2, 2, 1, 2, 1, 2, 1, 2, 2, 1,
+                                 2, 1, 1, 2, 2, 1, 2, 1, 2, 2, 1, 1};
+    static const int tri4[3] = {1, 1, 1};
+    static const int tri5[9] = {1, 1, 2, 1, 2, 1, 2, 1, 1};
+    double c[2];
+    int dk = 0;
+    if (dim == 3) {
+        /* template chosen as the lowest degree >= requested
+         * (Quadrature.F90:304-321) */
+        if (degree <= 1) { /* Str71 1 point, :663-672 */
+            c[0] = 0.25;
+            dk = expand(4, 1, tet7, c, 0.166666666666666666666666666666666, l, w, dk);
+        } else if (degree == 2) { /* str71 4 point, :676-688 */
+            c[0] = 0.138196601125010515179541316563436;
+            c[1] = 1.0 - 3.0 * c[0];
+            dk = expand(4, 4, tet8, c, 0.041666666666666666666666666666666, l, w, dk);
+        } else if (degree == 3) { /* str71 5 point, :692-707 */
+            c[0] = 0.25;
+            dk = expand(4, 1, tet7, c, -0.133333333333333333333333333333333, l, w, dk);
+            c[0] = 0.166666666666666666666666666666666;
+            c[1] = 1.0 - 3.0 * c[0];
+            dk = expand(4, 4, tet8, c, 0.075, l, w, dk);
+        } else if (degree == 4) { /* kea86 11 point, :711-735 */
+            c[0] = 0.25;
+            dk = expand(4, 1, tet7, c, -0.013155555555555555555555555555555, l, w, dk);
+            c[0] = 0.0714285714285714285714285714285714;
+            c[1] = 1.0 - 3.0 * c[0];
+            dk = expand(4, 4, tet8, c, 7.62222222222222222222222222222222E-3, l, w, dk);
+            c[0] = 0.399403576166799204996102147461640;
+            c[1] = 0.5 - c[0];
+            dk = expand(4, 6, tet9, c, 0.0248888888888888888888888888888888, l, w, dk);
+        } else {
+            return -1;
+        }
+        return dk;
+    }
+    if (dim == 2) {
+        if (degree <= 1) { /* :921-931 */
+            c[0] = 0.333333333333333333333333333333333;
+            dk = expand(3, 1, tri4, c, 0.5, l, w, dk);
+        } else if (degree == 2) { /* :935-949 */
+            c[0] = 0.166666666666666666666666666666666;
+            c[1] = 1.0 - 2.0 * c[0];
+            dk = expand(3, 3, tri5, c, 0.166666666666666666666666666666666, l, w, dk);
+        } else {
+            return -1;
+        }
+        return dk;
+    }
+    return -1;
+}
+
+/* ------------------------------------------------------------------ */
+/* Shape tables (Shape_Functions.F90:128-366, Elements.F90:615-717)     */
+/* ------------------------------------------------------------------ */
+
+/* number2count of the simplex Lagrange numbering: counts per barycentric
+ * coordinate of each local node (Element_Numbering; the P2 order is
+ * v1,e12,v2,e13,e23,v3,e14,e24,e34,v4 as in Elements.F90:527-537). */
+static int simplex_counts(int dim, int degree, int counts[MAXLOC][4])
+{
+    int nv = dim + 1;
+    memset(counts, 0, sizeof(int) * MAXLOC * 4);
+    if (degree == 1) {
+        for (int i = 0; i < nv; i++) counts[i][i] = 1;
+        return nv;
+    }
+    if (degree == 2) {
+        int k = 0;
+        /* ordering: for vertex b = 1..nv: edges (a,b) a<b, then vertex b */
+        for (int b = 0; b < nv; b++) {
+            for (int a = 0; a < b; a++) {
+                counts[k][a] = 1;
+                counts[k][b] = 1;
+                k++;
+            }
+            counts[k][b] = 2;
+            k++;
+        }
+        return k;
+    }
+    return -1;
+}
+
+/* lagrange_polynomial(c, c, dx) = prod_{i<c} (x - i dx) / prod_{i<c} (c dx - i dx)
+ * and its derivative (Shape_Functions.F90:331-366). */
+static double lag(int c, double dx, double x)
+{
+    double num = 1.0, den = 1.0;
+    for (int i = 0; i < c; i++) {
+        num *= (x - i * dx);
+        den *= (c * dx - i * dx);
+    }
+    return num / den;
+}
+
+static double dlag(int c, double dx, double x)
+{
+    double den = 1.0, s = 0.0;
+    for (int i = 0; i < c; i++) den *= (c * dx - i * dx);
+    for (int k = 0; k < c; k++) {
+        double t = 1.0;
+        for (int i = 0; i < c; i++)
+            if (i != k) t *= (x - i * dx);
+        s += t;
+    }
+    return s / den;
+}
+
+int orc_shape_tables(int dim, int degree, int ngi, const double *l, double *n,
+                     double *dn)
+{
+    int counts[MAXLOC][4];
+    int loc = simplex_counts(dim, degree, counts);
+    int nv = dim + 1;
+    double dx = 1.0 / degree;
+    if (loc < 0) return -1;
+    for (int i = 0; i < loc; i++) {
+        for (int g = 0; g < ngi; g++) {
+            const double *lg = l + g * nv;
+            double v = 1.0;
+            for (int j = 0; j < nv; j++) v *= lag(counts[i][j], dx, lg[j]);
+            n[i + loc * g] = v;
+            /* eval_dshape_simplex: dependent coordinate is the last one,
+             * dl4dl = -1 (diffl4, Elements.F90:693-717) */
+            for (int d = 0; d < dim; d++) {
+                double e = dlag(counts[i][d], dx, lg[d]) *
+                               lag(counts[i][dim], dx, lg[dim]) +
+                           (-1.0) * dlag(counts[i][dim], dx, lg[dim]) *
+                               lag(counts[i][d], dx, lg[d]);
+                for (int j = 0; j < dim; j++) {
+                    if (j == d) continue;
+                    e *= lag(counts[i][j], dx, lg[j]);
+                }
+                dn[i + loc * (g + ngi * d)] = e;
+            }
+        }
+    }
+    return loc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Sparsity (Sparsity_Patterns.F90:47-85, :299-428)                    */
+/* ------------------------------------------------------------------ */
+
+static int cmp_int(const void *a, const void *b)
+{
+    int x = *(const int *)a, y = *(const int *)b;
+    return (x > y) - (x < y);
+}
+
+/* node -> element adjacency (0-based), CSR */
+static void node_elements(int nnodes, int nele, int loc, const int32_t *ndglno,
+                          int **ptr_out, int **adj_out)
+{
+    int *ptr = calloc((size_t)nnodes + 1, sizeof(int));
+    int *adj = malloc(sizeof(int) * (size_t)nele * loc);
+    for (int64_t k = 0; k < (int64_t)nele * loc; k++) ptr[ndglno[k]]++;
+    for (int i = 0; i < nnodes; i++) ptr[i + 1] += ptr[i];
+    int *fill = malloc(sizeof(int) * (size_t)nnodes);
+    memcpy(fill, ptr, sizeof(int) * (size_t)nnodes);
+    for (int e = 0; e < nele; e++)
+        for (int i = 0; i < loc; i++) {
+            int nd = ndglno[(int64_t)e * loc + i] - 1;
+            adj[fill[nd]++] = e;
+        }
+    free(fill);
+    *ptr_out = ptr;
+    *adj_out = adj;
+}
+
+int64_t orc_make_sparsity(int nnodes, int nele, int loc, const int32_t *ndglno,
+                          int32_t *findrm, int32_t *colm, int32_t *centrm,
+                          int64_t cap)
+{
+    int *ptr, *adj;
+    node_elements(nnodes, nele, loc, ndglno, &ptr, &adj);
+    int *buf = malloc(sizeof(int) * (size_t)(loc * 64 + 64));
+    int bufcap = loc * 64 + 64;
+    int64_t count = 1;
+    for (int r = 0; r < nnodes; r++) {
+        int nb = 0;
+        int need = (ptr[r + 1] - ptr[r]) * loc;
+        if (need > bufcap) {
+            bufcap = need;
+            buf = realloc(buf, sizeof(int) * (size_t)bufcap);
+        }
+        for (int k = ptr[r]; k < ptr[r + 1]; k++) {
+            int e = adj[k];
+            for (int j = 0; j < loc; j++) buf[nb++] = ndglno[(int64_t)e * loc + j];
+        }
+        /* insert_ascending of every node of every element containing r */
+        qsort(buf, nb, sizeof(int), cmp_int);
+        int u = 0;
+        for (int k = 0; k < nb; k++)
+            if (u == 0 || buf[k] != buf[u - 1]) buf[u++] = buf[k];
+        findrm[r] = (int32_t)count;
+        if (count - 1 + u > cap) {
+            free(buf); free(ptr); free(adj);
+            return -1;
+        }
+        centrm[r] = 0;
+        for (int k = 0; k < u; k++) {
+            colm[count - 1 + k] = buf[k];
+            if (buf[k] == r + 1) centrm[r] = (int32_t)(count + k);
+        }
+        count += u;
+    }
+    findrm[nnodes] = (int32_t)count;
+    free(buf); free(ptr); free(adj);
+    return count - 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* Colouring (Colouring.F90:85-262)                                     */
+/* ------------------------------------------------------------------ */
+
+int orc_colour_elements(int nnodes, int nele, int loc, const int32_t *ndglno,
+                        int32_t *colour)
+{
+    int *ptr, *adj;
+    node_elements(nnodes, nele, loc, ndglno, &ptr, &adj);
+    int ncol = 0;
+    int cap = 64;
+    int *seen = calloc((size_t)cap + 2, sizeof(int)); /* stamp per colour */
+    for (int e = 0; e < nele; e++) {
+        /* neighbours = elements sharing a node (second-order sparsity of
+         * the P0 mesh against the topology); only lower ids count */
+        for (int i = 0; i < loc; i++) {
+            int nd = ndglno[(int64_t)e * loc + i] - 1;
+            for (int k = ptr[nd]; k < ptr[nd + 1]; k++) {
+                int f = adj[k];
+                if (f < e) seen[colour[f]] = e + 1;
+            }
+        }
+        int c = 1;
+        while (c <= ncol && seen[c] == e + 1) c++;
+        colour[e] = c;
+        if (c > ncol) {
+            ncol = c;
+            if (ncol + 1 >= cap) {
+                int nc = cap * 2;
+                seen = realloc(seen, sizeof(int) * ((size_t)nc + 2));
+                memset(seen + cap, 0, sizeof(int) * ((size_t)nc + 2 - cap));
+                cap = nc;
+            }
+        }
+    }
+    free(seen); free(ptr); free(adj);
+    return ncol;
+}
+
+/* ------------------------------------------------------------------ */
+/* Element assembly (Advection_Diffusion_CG.F90:702-1202)               */
+/* ------------------------------------------------------------------ */
+
+#define N_(i, g) (m->n[(i) + loc * (g)])
+#define DN_(i, g, d) (m->dn[(i) + loc * ((g) + ngi * (d))])
+#define XDN_(i, g, d) (m->x_dn[(i) + lx * ((g) + ngi * (d))])
+
+static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
+                      double *mat, double *rhs)
+{
+    const int dim = m->dim, loc = m->loc, ngi = m->ngi, lx = m->loc_x;
+    const int32_t *en = m->ndglno + (int64_t)ele * loc;
+    const int32_t *exn = m->ndglno_x + (int64_t)ele * lx;
+    const double eps = DBL_EPSILON; /* epsilon(0.0) with real*8 */
+    const double dt_theta = p->dt * p->theta;
+    double Xv[MAXDIM][MAXLOC], JT[MAXDIM][MAXDIM], iJ[MAXDIM][MAXDIM];
+    double detJ;
+    double dsh[MAXLOC][MAXGI][MAXDIM];
+    double detwei[MAXGI];
+    double Te[MAXLOC];
+
+    for (int i = 0; i < loc * loc; i++) mat[i] = 0.0;
+    for (int i = 0; i < loc; i++) rhs[i] = 0.0;
+
+    /* --- transform_to_physical, cache path (Transform_elements.F90:
+     * 227-306 construct_cache, :766-802 retrieval) --- */
+    for (int n = 0; n < lx; n++)
+        for (int a = 0; a < dim; a++)
+            Xv[a][n] = m->X[a + dim * (int64_t)(exn[n] - 1)];
+    for (int a = 0; a < dim; a++)
+        for (int b = 0; b < dim; b++) {
+            double s = 0.0;
+            for (int n = 0; n < lx; n++) s = s + Xv[a][n] * XDN_(n, 0, b);
+            JT[a][b] = s;
+        }
+    if (dim == 3) {
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++)
+                iJ[i][k] = JT[(i + 1) % 3][(k + 1) % 3] * JT[(i + 2) % 3][(k + 2) % 3] -
+                           JT[(i + 2) % 3][(k + 1) % 3] * JT[(i + 1) % 3][(k + 2) % 3];
+    } else if (dim == 2) {
+        iJ[0][0] = JT[1][1];
+        iJ[1][0] = -JT[0][1];
+        iJ[0][1] = -JT[1][0];
+        iJ[1][1] = JT[0][0];
+    } else {
+        iJ[0][0] = 1.0;
+    }
+    detJ = 0.0;
+    for (int a = 0; a < dim; a++) detJ = detJ + JT[a][0] * iJ[a][0];
+    for (int i = 0; i < dim; i++)
+        for (int k = 0; k < dim; k++) iJ[i][k] = iJ[i][k] / detJ;
+
+    /* dshape = dn . invJ^T; once for linear T (m_nonlinear false),
+     * per quadrature point otherwise (Transform_elements.F90:769-781) */
+    const int m_nonlinear = (loc != dim + 1);
+    for (int g = 0; g < ngi; g++) {
+        int gs = m_nonlinear ? g : 0;
+        for (int n = 0; n < loc; n++)
+            for (int d = 0; d < dim; d++) {
+                double s = 0.0;
+                for (int k = 0; k < dim; k++) s = s + DN_(n, gs, k) * iJ[d][k];
+                dsh[n][g][d] = s;
+            }
+    }
+    for (int g = 0; g < ngi; g++) detwei[g] = fabs(detJ) * m->w[g];
+
+    for (int i = 0; i < loc; i++) Te[i] = m->T[en[i] - 1];
+
+    /* --- mass (add_mass_element_cg, :867-941) --- */
+    if (p->have_mass) {
+        double M[MAXLOC][MAXLOC];
+        for (int i = 0; i < loc; i++)
+            for (int j = 0; j < loc; j++) {
+                double s = 0.0;
+                for (int g = 0; g < ngi; g++) s = s + (N_(i, g) * N_(j, g)) * detwei[g];
+                M[i][j] = s;
+            }
+        if (p->lump_mass) {
+            for (int i = 0; i < loc; i++) {
+                double s = 0.0;
+                for (int j = 0; j < loc; j++) s = s + M[i][j];
+                mat[i + loc * i] = mat[i + loc * i] + s;
+            }
+        } else {
+            for (int i = 0; i < loc; i++)
+                for (int j = 0; j < loc; j++)
+                    mat[i + loc * j] = mat[i + loc * j] + M[i][j];
+        }
+    }
+
+    /* --- advection (add_advection_element_cg, :943-1127) --- */
+    if (p->have_advection) {
+        double uq[MAXDIM][MAXGI], divq[MAXGI], adv[MAXLOC][MAXLOC];
+        for (int d = 0; d < dim; d++)
+            for (int g = 0; g < ngi; g++) {
+                double s = 0.0;
+                for (int i = 0; i < loc; i++) {
+                    double ui = m->u_const ? m->U[d] : m->U[d + dim * (int64_t)(en[i] - 1)];
+                    s = s + ui * N_(i, g);
+                }
+                uq[d][g] = s;
+            }
+        int need_div = p->integrate_by_parts ? (fabs(1.0 - p->beta) > eps)
+                                             : (fabs(p->beta) > eps);
+        if (need_div) { /* ele_div_at_quad, Fields_Base.F90:2557-2575 */
+            for (int g = 0; g < ngi; g++) divq[g] = 0.0;
+            for (int d = 0; d < dim; d++)
+                for (int g = 0; g < ngi; g++) {
+                    double s = 0.0;
+                    for (int i = 0; i < loc; i++) {
+                        double ui = m->u_const ? m->U[d] : m->U[d + dim * (int64_t)(en[i] - 1)];
+                        s = s + ui * dsh[i][g][d];
+                    }
+                    divq[g] = divq[g] + s;
+                }
+        }
+        for (int i = 0; i < loc; i++)
+            for (int j = 0; j < loc; j++) {
+                double s = 0.0;
+                if (!p->integrate_by_parts) {
+                    /* shape_vector_dot_dshape (FETools.F90:749-772) */
+                    for (int g = 0; g < ngi; g++) {
+                        double t = 0.0;
+                        for (int d = 0; d < dim; d++) t = t + dsh[j][g][d] * uq[d][g];
+                        s = s + (N_(i, g) * t) * detwei[g];
+                    }
+                } else {
+                    /* -dshape_dot_vector_shape (FETools.F90:700-721) */
+                    for (int g = 0; g < ngi; g++) {
+                        double t = 0.0;
+                        for (int d = 0; d < dim; d++) t = t + dsh[i][g][d] * uq[d][g];
+                        s = s + (t * N_(j, g)) * detwei[g];
+                    }
+                    s = -s;
+                }
+                adv[i][j] = s;
+            }
+        if (need_div) {
+            double coef = p->integrate_by_parts ? (1.0 - p->beta) : p->beta;
+            for (int i = 0; i < loc; i++)
+                for (int j = 0; j < loc; j++) {
+                    double s = 0.0;
+                    for (int g = 0; g < ngi; g++)
+                        s = s + (N_(i, g) * N_(j, g)) * (divq[g] * detwei[g]);
+                    if (p->integrate_by_parts)
+                        adv[i][j] = adv[i][j] - coef * s;
+                    else
+                        adv[i][j] = adv[i][j] + coef * s;
+                }
+        }
+        if (fabs(dt_theta) > eps)
+            for (int i = 0; i < loc; i++)
+                for (int j = 0; j < loc; j++)
+                    mat[i + loc * j] = mat[i + loc * j] + dt_theta * adv[i][j];
+        for (int i = 0; i < loc; i++) {
+            double s = 0.0;
+            for (int j = 0; j < loc; j++) s = s + adv[i][j] * Te[j];
+            rhs[i] = rhs[i] - s;
+        }
+    }
+
+    /* --- absorption (add_absorption_element_cg, :1143-1162) --- */
+    if (p->have_absorption) {
+        double sq[MAXGI], ab[MAXLOC][MAXLOC];
+        for (int g = 0; g < ngi; g++) {
+            double s = 0.0;
+            for (int i = 0; i < loc; i++) {
+                double v = m->absorp_const ? m->absorp[0] : m->absorp[en[i] - 1];
+                s = s + v * N_(i, g);
+            }
+            sq[g] = detwei[g] * s;
+        }
+        for (int i = 0; i < loc; i++)
+            for (int j = 0; j < loc; j++) {
+                double s = 0.0;
+                for (int g = 0; g < ngi; g++) s = s + (N_(i, g) * N_(j, g)) * sq[g];
+                ab[i][j] = s;
+            }
+        if (fabs(dt_theta) > eps)
+            for (int i = 0; i < loc; i++)
+                for (int j = 0; j < loc; j++)
+                    mat[i + loc * j] = mat[i + loc * j] + dt_theta * ab[i][j];
+        for (int i = 0; i < loc; i++) {
+            double s = 0.0;
+            for (int j = 0; j < loc; j++) s = s + ab[i][j] * Te[j];
+            rhs[i] = rhs[i] - s;
+        }
+    }
+
+    /* --- diffusivity (add_diffusivity_element_cg, :1164-1202) --- */
+    if (p->have_diffusivity) {
+        double kq[MAXDIM][MAXDIM][MAXGI], df[MAXLOC][MAXLOC];
+        for (int a = 0; a < dim; a++)
+            for (int b = 0; b < dim; b++)
+                for (int g = 0; g < ngi; g++) {
+                    double s = 0.0;
+                    for (int i = 0; i < loc; i++) {
+                        double v = m->kappa_const
+                                       ? m->kappa[a + dim * b]
+                                       : m->kappa[a + dim * b + dim * dim * (int64_t)(en[i] - 1)];
+                        s = s + v * N_(i, g);
+                    }
+                    kq[a][b][g] = s;
+                }
+        for (int i = 0; i < loc; i++)
+            for (int j = 0; j < loc; j++) df[i][j] = 0.0;
+        if (p->isotropic_diffusivity) {
+            /* dshape_dot_dshape (FETools.F90:391-453): gi outermost */
+            double wk[MAXGI];
+            for (int g = 0; g < ngi; g++) wk[g] = detwei[g] * kq[0][0][g];
+            for (int g = 0; g < ngi; g++)
+                for (int i = 0; i < loc; i++)
+                    for (int j = 0; j < loc; j++) {
+                        double t;
+                        if (dim == 3)
+                            t = ((dsh[i][g][0] * dsh[j][g][0]) + (dsh[i][g][1] * dsh[j][g][1])) +
+                                (dsh[i][g][2] * dsh[j][g][2]);
+                        else if (dim == 2)
+                            t = (dsh[i][g][0] * dsh[j][g][0]) + (dsh[i][g][1] * dsh[j][g][1]);
+                        else
+                            t = dsh[i][g][0] * dsh[j][g][0];
+                        df[i][j] = df[i][j] + t * wk[g];
+                    }
+        } else {
+            /* dshape_tensor_dshape (FETools.F90:668-698) */
+            for (int g = 0; g < ngi; g++)
+                for (int i = 0; i < loc; i++)
+                    for (int j = 0; j < loc; j++) {
+                        double v[MAXDIM], t = 0.0;
+                        for (int b = 0; b < dim; b++) {
+                            double s = 0.0;
+                            for (int a = 0; a < dim; a++) s = s + dsh[i][g][a] * kq[a][b][g];
+                            v[b] = s;
+                        }
+                        for (int b = 0; b < dim; b++) t = t + v[b] * dsh[j][g][b];
+                        df[i][j] = df[i][j] + t * detwei[g];
+                    }
+        }
+        if (fabs(dt_theta) > eps)
+            for (int i = 0; i < loc; i++)
+                for (int j = 0; j < loc; j++)
+                    mat[i + loc * j] = mat[i + loc * j] + dt_theta * df[i][j];
+        for (int i = 0; i < loc; i++) {
+            double s = 0.0;
+            for (int j = 0; j < loc; j++) s = s + df[i][j] * Te[j];
+            rhs[i] = rhs[i] - s;
+        }
+    }
+
+    /* --- source (add_source_element_cg, :1129-1141; shape_rhs
+     * FETools.F90:38-54) --- */
+    if (p->have_source) {
+        double fw[MAXGI];
+        for (int g = 0; g < ngi; g++) {
+            double s = 0.0;
+            for (int i = 0; i < loc; i++) {
+                double v = m->src_const ? m->src[0] : m->src[en[i] - 1];
+                s = s + v * N_(i, g);
+            }
+            fw[g] = detwei[g] * s;
+        }
+        for (int i = 0; i < loc; i++) {
+            double s = 0.0;
+            for (int g = 0; g < ngi; g++) s = s + N_(i, g) * fw[g];
+            rhs[i] = rhs[i] + s;
+        }
+    }
+    return 0;
+}
+
+int orc_element(const orc_params *p, const orc_mesh *m, int ele, double *mat,
+                double *rhs)
+{
+    return element_cg(p, m, ele, mat, rhs);
+}
+
+/* csr_sparsity_pos bisection (Sparse_Tools.F90:2411-2517); returns the
+ * 0-based position or -1 */
+static int64_t csr_pos(const int32_t *findrm, const int32_t *colm, int i1, int j1)
+{
+    int64_t base = findrm[i1 - 1] - 1; /* 0-based start of row */
+    int64_t len = findrm[i1] - findrm[i1 - 1];
+    if (len <= 0) return -1;
+    int64_t lo = 0, hi = len - 1;
+    int lo_j = colm[base + lo], hi_j = colm[base + hi];
+    if (hi_j < j1) return -1;
+    if (hi_j == j1) return base + hi;
+    if (lo_j > j1) return -1;
+    if (lo_j == j1) return base + lo;
+    while (hi - lo > 1) {
+        int64_t mid = (hi + lo) / 2;
+        int mj = colm[base + mid];
+        if (mj == j1) return base + mid;
+        if (mj > j1) hi = mid;
+        else lo = mid;
+    }
+    return -1;
+}
+
+int orc_assemble(const orc_params *p, const orc_mesh *m, double *A,
+                 double *rhs, int nthreads, double *seconds)
+{
+    const int loc = m->loc;
+    int64_t nnz = m->findrm[m->nrows] - 1;
+    int err = 0;
+    /* zero(matrix); zero(rhs) (:560-561) */
+    for (int64_t k = 0; k < nnz; k++) A[k] = 0.0;
+    for (int i = 0; i < m->nrows; i++) rhs[i] = 0.0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#endif
+    double t0 = wall_now();
+    /* colour loop (:574-598): colours in order, elements ascending */
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+    {
+        double mat[MAXLOC * MAXLOC], r[MAXLOC];
+        for (int c = 0; c < m->ncolours; c++) {
+#pragma omp for schedule(static)
+            for (int k = m->col_ptr[c]; k < m->col_ptr[c + 1]; k++) {
+                int ele = m->col_elems[k] - 1;
+                const int32_t *en = m->ndglno + (int64_t)ele * loc;
+                element_cg(p, m, ele, mat, r);
+                /* addto(matrix, nodes, nodes, matrix_addto) -> csr_vaddto
+                 * (:2680-2703): rows outer, columns inner, zeros skipped */
+                for (int i = 0; i < loc; i++)
+                    for (int j = 0; j < loc; j++) {
+                        double v = mat[i + loc * j];
+                        if (v == 0.0) continue;
+                        int64_t pos = csr_pos(m->findrm, m->colm, en[i], en[j]);
+                        if (pos < 0) {
+#pragma omp atomic write
+                            err = 1;
+                            continue;
+                        }
+                        A[pos] = A[pos] + v;
+                    }
+                /* addto(rhs, nodes, rhs_addto) (Fields_Manipulation.F90:
+                 * 255-273) */
+                for (int i = 0; i < loc; i++) rhs[en[i] - 1] = rhs[en[i] - 1] + r[i];
+            }
+        }
+    }
+    if (seconds) *seconds = wall_now() - t0;
+    return err ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Dirichlet, SpMV                                                      */
+/* ------------------------------------------------------------------ */
+
+void orc_dirichlet(int nbc, const int32_t *nodes, const double *values,
+                   const double *T, double dt, int32_t *inactive, double *rhs)
+{
+    for (int j = 0; j < nbc; j++) {
+        int nd = nodes[j] - 1;
+        inactive[nd] = 1;
+        if (dt > 0.0)
+            rhs[nd] = (values[j] - T[nd]) / dt;
+        else
+            rhs[nd] = values[j];
+    }
+}
+
+void orc_csr_mult(int nrows, const int32_t *findrm, const int32_t *colm,
+                  const double *val, const double *x, double *y)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nrows; i++) {
+        double s = 0.0;
+        for (int64_t k = findrm[i] - 1; k < findrm[i + 1] - 1; k++)
+            s = s + val[k] * x[colm[k] - 1];
+        y[i] = s;
+    }
+}
+
+double orc_ghost_pivot(int n, const int32_t *centrm, const double *A)
+{
+    double mindiag = DBL_MAX, maxdiag = -DBL_MAX;
+    for (int i = 0; i < n; i++) {
+        double d = centrm[i] > 0 ? fabs(A[centrm[i] - 1]) : 0.0;
+        if (d < mindiag) mindiag = d;
+        if (d > maxdiag) maxdiag = d;
+    }
+    return sqrt((maxdiag + mindiag) * maxdiag / 2.0);
+}
+
+/* ------------------------------------------------------------------ */
+/* Krylov (PETSc KSPCG / KSPGMRES semantics on the reduced system)      */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int n;
+    const int32_t *findrm, *colm;
+    double *val; /* reduced values (inactive columns zeroed, ghost pivot) */
+} rsys;
+
+static void rmult(const rsys *s, const double *x, double *y)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < s->n; i++) {
+        double t = 0.0;
+        for (int64_t k = s->findrm[i] - 1; k < s->findrm[i + 1] - 1; k++)
+            t = t + s->val[k] * x[s->colm[k] - 1];
+        y[i] = t;
+    }
+}
+
+static double dot(int n, const double *a, const double *b)
+{
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (int i = 0; i < n; i++) s += a[i] * b[i];
+    return s;
+}
+
+enum {
+    KSP_CONVERGED_RTOL = 2,
+    KSP_CONVERGED_ATOL = 3,
+    KSP_DIVERGED_ITS = -3,
+    KSP_DIVERGED_DTOL = -4,
+    KSP_DIVERGED_INDEFINITE_PC = -8,
+    KSP_DIVERGED_NANORINF = -9,
+    KSP_DIVERGED_INDEFINITE_MAT = -10
+};
+
+static int conv_test(double rnorm, double rnorm0, double rtol, double atol)
+{
+    if (isnan(rnorm) || isinf(rnorm)) return KSP_DIVERGED_NANORINF;
+    if (rnorm <= atol) return KSP_CONVERGED_ATOL;
+    if (rnorm <= rtol * rnorm0) return KSP_CONVERGED_RTOL;
+    if (rnorm >= 1e5 * rnorm0 && rnorm0 > 0) return KSP_DIVERGED_DTOL;
+    return 0;
+}
+
+static int ksp_cg(const rsys *s, const double *dinv, const double *b, double *x,
+                  double rtol, double atol, int maxits, int *its)
+{
+    int n = s->n;
+    double *r = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
+    double *p = malloc(sizeof(double) * n), *w = malloc(sizeof(double) * n);
+    int reason = 0;
+    rmult(s, x, w);
+    for (int i = 0; i < n; i++) r[i] = b[i] - w[i];
+    for (int i = 0; i < n; i++) z[i] = dinv ? dinv[i] * r[i] : r[i];
+    double dp = sqrt(dot(n, z, z)); /* KSP_NORM_PRECONDITIONED */
+    double rnorm0 = dp;
+    double beta = dot(n, z, r), betaold = 1.0;
+    *its = 0;
+    reason = conv_test(dp, rnorm0, rtol, atol);
+    if (dp == 0.0) reason = KSP_CONVERGED_ATOL;
+    for (int it = 0; it < maxits && !reason; it++) {
+        if (it == 0) {
+            for (int i = 0; i < n; i++) p[i] = z[i];
+        } else {
+            double bb = beta / betaold;
+            for (int i = 0; i < n; i++) p[i] = z[i] + bb * p[i];
+        }
+        rmult(s, p, w);
+        double dpi = dot(n, p, w);
+        if (dpi <= 0.0) {
+            reason = (dpi == 0.0) ? KSP_DIVERGED_INDEFINITE_MAT : KSP_DIVERGED_INDEFINITE_MAT;
+            break;
+        }
+        double a = beta / dpi;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; i++) {
+            x[i] = x[i] + a * p[i];
+            r[i] = r[i] - a * w[i];
+            z[i] = dinv ? dinv[i] * r[i] : r[i];
+        }
+        dp = sqrt(dot(n, z, z));
+        *its = it + 1;
+        betaold = beta;
+        beta = dot(n, z, r);
+        reason = conv_test(dp, rnorm0, rtol, atol);
+        if (!reason && beta < 0.0) reason = KSP_DIVERGED_INDEFINITE_PC;
+    }
+    if (!reason) reason = KSP_DIVERGED_ITS;
+    free(r); free(z); free(p); free(w);
+    return reason;
+}
+
+/* restarted GMRES, left preconditioning, modified Gram-Schmidt */
+static int ksp_gmres(const rsys *s, const double *dinv, const double *b,
+                     double *x, double rtol, double atol, int maxits,
+                     int restart, int *its)
+{
+    int n = s->n, m = restart > 0 ? restart : 30;
+    double *V = malloc(sizeof(double) * (size_t)n * (m + 1));
+    double *H = calloc((size_t)(m + 1) * m, sizeof(double));
+    double *cs = malloc(sizeof(double) * m), *sn = malloc(sizeof(double) * m);
+    double *g = malloc(sizeof(double) * (m + 1)), *y = malloc(sizeof(double) * m);
+    double *w = malloc(sizeof(double) * n), *t = malloc(sizeof(double) * n);
+    int reason = 0, total = 0;
+    double rnorm0 = -1.0;
+    *its = 0;
+    while (!reason) {
+        rmult(s, x, w);
+        for (int i = 0; i < n; i++) {
+            double r = b[i] - w[i];
+            V[i] = dinv ? dinv[i] * r : r;
+        }
+        double beta = sqrt(dot(n, V, V));
+        if (rnorm0 < 0) {
+            rnorm0 = beta;
+            reason = conv_test(beta, rnorm0, rtol, atol);
+            if (beta == 0.0) reason = KSP_CONVERGED_ATOL;
+            if (reason) break;
+        }
+        for (int i = 0; i < n; i++) V[i] /= beta;
+        memset(g, 0, sizeof(double) * (m + 1));
+        g[0] = beta;
+        int k;
+        for (k = 0; k < m && !reason; k++) {
+            double *vk = V + (size_t)n * k, *vn = V + (size_t)n * (k + 1);
+            rmult(s, vk, t);
+            for (int i = 0; i < n; i++) vn[i] = dinv ? dinv[i] * t[i] : t[i];
+            for (int j = 0; j <= k; j++) {
+                double *vj = V + (size_t)n * j;
+                double h = dot(n, vn, vj);
+                H[j + (m + 1) * k] = h;
+                for (int i = 0; i < n; i++) vn[i] -= h * vj[i];
+            }
+            double hn = sqrt(dot(n, vn, vn));
+            H[k + 1 + (m + 1) * k] = hn;
+            if (hn != 0.0)
+                for (int i = 0; i < n; i++) vn[i] /= hn;
+            for (int j = 0; j < k; j++) {
+                double a = H[j + (m + 1) * k], bb = H[j + 1 + (m + 1) * k];
+                H[j + (m + 1) * k] = cs[j] * a + sn[j] * bb;
+                H[j + 1 + (m + 1) * k] = -sn[j] * a + cs[j] * bb;
+            }
+            double a = H[k + (m + 1) * k], bb = H[k + 1 + (m + 1) * k];
+            double den = sqrt(a * a + bb * bb);
+            cs[k] = den == 0.0 ? 1.0 : a / den;
+            sn[k] = den == 0.0 ? 0.0 : bb / den;
+            H[k + (m + 1) * k] = cs[k] * a + sn[k] * bb;
+            H[k + 1 + (m + 1) * k] = 0.0;
+            g[k + 1] = -sn[k] * g[k];
+            g[k] = cs[k] * g[k];
+            total++;
+            *its = total;
+            reason = conv_test(fabs(g[k + 1]), rnorm0, rtol, atol);
+            if (!reason && total >= maxits) reason = KSP_DIVERGED_ITS;
+        }
+        /* solve H y = g and update x */
+        for (int i = k - 1; i >= 0; i--) {
+            double acc = g[i];
+            for (int j = i + 1; j < k; j++) acc -= H[i + (m + 1) * j] * y[j];
+            y[i] = acc / H[i + (m + 1) * i];
+        }
+        for (int j = 0; j < k; j++) {
+            double *vj = V + (size_t)n * j;
+            for (int i = 0; i < n; i++) x[i] += y[j] * vj[i];
+        }
+    }
+    free(V); free(H); free(cs); free(sn); free(g); free(y); free(w); free(t);
+    return reason;
+}
+
+int orc_petsc_solve(const orc_solver *sv, int n, const int32_t *findrm,
+                    const int32_t *colm, const int32_t *centrm,
+                    const double *A, const int32_t *inactive,
+                    const double *rhs, double *x, int32_t *its_out,
+                    int32_t *reason_out, double *true_relres, double *seconds)
+{
+#ifdef _OPENMP
+    if (sv->nthreads > 0) omp_set_num_threads(sv->nthreads);
+#endif
+    int64_t nnz = findrm[n] - 1;
+    rsys s = {n, findrm, colm, malloc(sizeof(double) * (size_t)nnz)};
+    double *b = malloc(sizeof(double) * n), *ghost = malloc(sizeof(double) * n);
+    double *tmp = malloc(sizeof(double) * n), *dinv = NULL;
+    double *y = malloc(sizeof(double) * n);
+    int any_inactive = 0;
+    if (inactive)
+        for (int i = 0; i < n; i++) any_inactive |= inactive[i];
+    /* lifting (Solvers.F90:1072-1099): tmp = A * ghost_rhs */
+    for (int i = 0; i < n; i++) ghost[i] = (any_inactive && inactive[i]) ? rhs[i] : 0.0;
+    orc_csr_mult(n, findrm, colm, A, ghost, tmp);
+    double pivot = orc_ghost_pivot(n, centrm, A);
+    for (int i = 0; i < n; i++) {
+        int ghost_row = any_inactive && inactive[i];
+        b[i] = ghost_row ? 0.0 : rhs[i] - tmp[i];
+        y[i] = ghost_row ? 0.0 : x[i];
+        for (int64_t k = findrm[i] - 1; k < findrm[i + 1] - 1; k++) {
+            int c = colm[k] - 1;
+            double v = A[k];
+            if (ghost_row) v = (c == i) ? pivot : 0.0;
+            else if (any_inactive && inactive[c]) v = 0.0;
+            s.val[k] = v;
+        }
+    }
+    if (sv->pc == 1) {
+        dinv = malloc(sizeof(double) * n);
+        for (int i = 0; i < n; i++) {
+            double d = centrm[i] > 0 ? s.val[centrm[i] - 1] : 0.0;
+            dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
+        }
+    }
+    double t0 = wall_now();
+    int its = 0, reason;
+    if (sv->ksp == 1)
+        reason = ksp_gmres(&s, dinv, b, y, sv->rtol, sv->atol, sv->max_its, sv->restart, &its);
+    else
+        reason = ksp_cg(&s, dinv, b, y, sv->rtol, sv->atol, sv->max_its, &its);
+    if (seconds) *seconds = wall_now() - t0;
+    /* true residual of the reduced system */
+    rmult(&s, y, tmp);
+    double rr = 0.0, bb = 0.0;
+    for (int i = 0; i < n; i++) {
+        double d = b[i] - tmp[i];
+        rr += d * d;
+        bb += b[i] * b[i];
+    }
+    *true_relres = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+    /* petsc2field with ghost copy-back (Petsc_Tools.F90:724-731) */
+    for (int i = 0; i < n; i++) x[i] = (any_inactive && inactive[i]) ? rhs[i] : y[i];
+    *its_out = its;
+    *reason_out = reason;
+    free(s.val); free(b); free(ghost); free(tmp); free(dinv); free(y);
+    return 0;
+}

@@ -1,0 +1,124 @@
+/*
+ * fl_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement ("oracle") of Fluidity's continuous-Galerkin
+ * advection-diffusion assembly, CSR insertion, strong-Dirichlet elimination
+ * and the Krylov hand-off.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it, and only as the checker or the
+ * timed CPU baseline.  The product library (fluidity_amd) never links it.
+ *
+ * Conventions follow Fluidity's Fortran data layout: 1-based int32 node
+ * numbers, element-major ndglno, column-major dim x nodes vector values,
+ * column-major n(loc,ngi) and dn(loc,ngi,dim) shape tables.
+ */
+#ifndef FL_ORACLE_H
+#define FL_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Quadrature.F90:164-240 (template selection), :572-605 (expansion),
+ * :649-735 (tet rules), :909-950 (tri rules).  l is ngi x (dim+1), row per
+ * point; returns ngi or -1. */
+int orc_quadrature(int dim, int degree, double *l, double *w);
+
+/* Shape_Functions.F90:128-329 + Elements.F90:615-663: Lagrange simplex
+ * tables.  n(loc,ngi), dn(loc,ngi,dim) column-major.  Returns loc or -1. */
+int orc_shape_tables(int dim, int degree, int ngi, const double *l,
+                     double *n, double *dn);
+
+/* Sparsity_Patterns.F90:47-85, :299-428.  All arrays 1-based.
+ * findrm[nnodes+1], centrm[nnodes]; colm capacity cap.  Returns nnz or -1. */
+int64_t orc_make_sparsity(int nnodes, int nele, int loc, const int32_t *ndglno,
+                          int32_t *findrm, int32_t *colm, int32_t *centrm,
+                          int64_t cap);
+
+/* Colouring.F90:85-199: greedy first-fit colouring of the element graph
+ * (elements adjacent when they share a node), elements visited in index
+ * order.  colour[e] is 1-based.  Returns the number of colours. */
+int orc_colour_elements(int nnodes, int nele, int loc, const int32_t *ndglno,
+                        int32_t *colour);
+
+typedef struct {
+    double dt, theta, beta;
+    int32_t have_mass, lump_mass, have_advection, integrate_by_parts;
+    int32_t have_diffusivity, isotropic_diffusivity, have_source,
+        have_absorption;
+} orc_params;
+
+typedef struct {
+    int32_t dim, nele;
+    /* coordinate mesh (linear simplices) */
+    int32_t loc_x, nx;
+    const int32_t *ndglno_x;   /* loc_x*nele, 1-based */
+    const double *X;           /* dim x nx */
+    const double *x_dn;        /* loc_x x ngi x dim */
+    /* field mesh of T (and of every other field) */
+    int32_t loc, ngi, nnodes;
+    const int32_t *ndglno;     /* loc*nele, 1-based */
+    const double *n, *dn, *w;  /* n(loc,ngi) dn(loc,ngi,dim) w(ngi) */
+    /* fields; *_const != 0 means FIELD_TYPE_CONSTANT (one value) */
+    const double *T;
+    const double *U;      int32_t u_const;      /* dim x N */
+    const double *kappa;  int32_t kappa_const;  /* dim x dim x N */
+    const double *src;    int32_t src_const;
+    const double *absorp; int32_t absorp_const;
+    /* colour sets: ncolours+1 offsets into elems (0-based offsets,
+     * 1-based element ids, ascending inside each colour) */
+    int32_t ncolours;
+    const int32_t *col_ptr, *col_elems;
+    /* sparsity, 1-based */
+    int32_t nrows;
+    const int32_t *findrm, *colm;
+} orc_mesh;
+
+/* Advection_Diffusion_CG.F90:558-600 (zero + colour loop), :702-1202
+ * (element), Sparse_Tools.F90:2411-2703 (bisection addto skipping zeros).
+ * Returns 0, or -1 if an entry falls outside the pattern.  *seconds gets
+ * the wall time of the colour loop (profiler key advection_diffusion_loop).
+ * nthreads <= 0 uses the OpenMP default. */
+int orc_assemble(const orc_params *p, const orc_mesh *m, double *A,
+                 double *rhs, int nthreads, double *seconds);
+
+/* Same element routine, for one element: mat(loc,loc) column-major and
+ * rhs(loc).  Used by the unit tests of the element arithmetic. */
+int orc_element(const orc_params *p, const orc_mesh *m, int ele,
+                double *mat, double *rhs);
+
+/* Boundary_Conditions.F90:1983-2025.  nodes 1-based; inactive[N] set to 1;
+ * rhs(node) = (g - T(node))/dt when dt > 0, else g. */
+void orc_dirichlet(int nbc, const int32_t *nodes, const double *values,
+                   const double *T, double dt, int32_t *inactive,
+                   double *rhs);
+
+/* Sparse_Tools.F90:3822-3844 */
+void orc_csr_mult(int nrows, const int32_t *findrm, const int32_t *colm,
+                  const double *val, const double *x, double *y);
+
+typedef struct {
+    int32_t ksp;      /* 0 = cg, 1 = gmres */
+    int32_t pc;       /* 0 = none, 1 = jacobi */
+    double rtol, atol;
+    int32_t max_its, restart;
+    int32_t nthreads;
+} orc_solver;
+
+/* Solvers.F90:111-187 / :1052-1126 (lifting), Petsc_Tools.F90:891-1074
+ * (ghost pivot), :671-749 (ghost copy-back), KSPCG/KSPGMRES semantics.
+ * x is in/out (initial guess).  Returns 0.  reason follows PETSc's
+ * KSPConvergedReason numbering. */
+int orc_petsc_solve(const orc_solver *s, int n, const int32_t *findrm,
+                    const int32_t *colm, const int32_t *centrm,
+                    const double *A, const int32_t *inactive,
+                    const double *rhs, double *x, int32_t *its,
+                    int32_t *reason, double *true_relres, double *seconds);
+
+/* Ghost pivot of Petsc_Tools.F90:959-984 over all rows' |diagonal|. */
+double orc_ghost_pivot(int n, const int32_t *centrm, const double *A);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

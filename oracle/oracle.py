@@ -1,0 +1,281 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the oracle (liboracle.so),
+the CPU restatement of Fluidity's CG advection-diffusion path.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
+as the checker or as the timed CPU baseline.  The product library never
+imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("theta", C.c_double), ("beta", C.c_double),
+                ("have_mass", C.c_int32), ("lump_mass", C.c_int32), ("have_advection", C.c_int32),
+                ("integrate_by_parts", C.c_int32), ("have_diffusivity", C.c_int32),
+                ("isotropic_diffusivity", C.c_int32), ("have_source", C.c_int32),
+                ("have_absorption", C.c_int32)]
+
+
+class OrcMesh(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("nele", C.c_int32), ("loc_x", C.c_int32), ("nx", C.c_int32),
+                ("ndglno_x", C.c_void_p), ("X", C.c_void_p), ("x_dn", C.c_void_p),
+                ("loc", C.c_int32), ("ngi", C.c_int32), ("nnodes", C.c_int32),
+                ("ndglno", C.c_void_p), ("n", C.c_void_p), ("dn", C.c_void_p), ("w", C.c_void_p),
+                ("T", C.c_void_p), ("U", C.c_void_p), ("u_const", C.c_int32),
+                ("kappa", C.c_void_p), ("kappa_const", C.c_int32),
+                ("src", C.c_void_p), ("src_const", C.c_int32),
+                ("absorp", C.c_void_p), ("absorp_const", C.c_int32),
+                ("ncolours", C.c_int32), ("col_ptr", C.c_void_p), ("col_elems", C.c_void_p),
+                ("nrows", C.c_int32), ("findrm", C.c_void_p), ("colm", C.c_void_p)]
+
+
+class OrcSolver(C.Structure):
+    _fields_ = [("ksp", C.c_int32), ("pc", C.c_int32), ("rtol", C.c_double), ("atol", C.c_double),
+                ("max_its", C.c_int32), ("restart", C.c_int32), ("nthreads", C.c_int32)]
+
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        lib.orc_quadrature.argtypes = [C.c_int, C.c_int, P, P]
+        lib.orc_shape_tables.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P]
+        lib.orc_make_sparsity.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P, P, C.c_int64]
+        lib.orc_make_sparsity.restype = C.c_int64
+        lib.orc_colour_elements.argtypes = [C.c_int, C.c_int, C.c_int, P, P]
+        lib.orc_assemble.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcMesh), P, P, C.c_int,
+                                     C.POINTER(C.c_double)]
+        lib.orc_element.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcMesh), C.c_int, P, P]
+        lib.orc_dirichlet.argtypes = [C.c_int, P, P, P, C.c_double, P, P]
+        lib.orc_dirichlet.restype = None
+        lib.orc_csr_mult.argtypes = [C.c_int, P, P, P, P, P]
+        lib.orc_csr_mult.restype = None
+        lib.orc_petsc_solve.argtypes = [C.POINTER(OrcSolver), C.c_int, P, P, P, P, P, P, P,
+                                        C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib.orc_ghost_pivot.argtypes = [C.c_int, P, P]
+        lib.orc_ghost_pivot.restype = C.c_double
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    p = C.c_void_p(a.ctypes.data)
+    p._keep = a  # the array lives as long as the argument object
+    return p
+
+
+def quadrature(dim: int, degree: int):
+    l = np.zeros((32, dim + 1))
+    w = np.zeros(32)
+    ngi = load().orc_quadrature(dim, degree, _p(l), _p(w))
+    assert ngi > 0
+    return l[:ngi].copy(), w[:ngi].copy()
+
+
+def shape_tables(dim: int, degree: int, l: np.ndarray):
+    ngi = l.shape[0]
+    lc = np.ascontiguousarray(l)
+    n = np.zeros(10 * ngi)
+    dn = np.zeros(10 * ngi * dim)
+    loc = load().orc_shape_tables(dim, degree, ngi, _p(lc), _p(n), _p(dn))
+    assert loc > 0
+    return (n[:loc * ngi].reshape((loc, ngi), order="F").copy(),
+            dn[:loc * ngi * dim].reshape((loc, ngi, dim), order="F").copy())
+
+
+def make_sparsity(mesh):
+    n = mesh.nnodes
+    findrm = np.zeros(n + 1, np.int32)
+    centrm = np.zeros(n, np.int32)
+    cap = mesh.nele * mesh.loc * mesh.loc + n
+    colm = np.zeros(cap, np.int32)
+    nnz = load().orc_make_sparsity(n, mesh.nele, mesh.loc, _p(mesh.ndglno), _p(findrm), _p(colm),
+                                   _p(centrm), cap)
+    assert nnz >= 0
+    return findrm, colm[:nnz].copy(), centrm
+
+
+def colour_elements(mesh):
+    colour = np.zeros(mesh.nele, np.int32)
+    ncol = load().orc_colour_elements(mesh.nnodes, mesh.nele, mesh.loc, _p(mesh.ndglno), _p(colour))
+    return colour, ncol
+
+
+def colour_sets(colour: np.ndarray, ncol: int):
+    order = np.argsort(colour, kind="stable").astype(np.int32)
+    counts = np.bincount(colour, minlength=ncol + 1)[1:]
+    col_ptr0 = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    return col_ptr0, (order + 1).astype(np.int32)
+
+
+class Assembly:
+    """Holds the arrays an OrcMesh points at (keeps them alive)."""
+
+    def __init__(self, mesh, element, xelement, fields: dict, sparsity, colouring):
+        self.keep = []
+
+        def k(a, dt=np.float64):
+            if a is None:
+                return None, 0
+            arr = np.ascontiguousarray(a, dtype=dt).ravel()
+            self.keep.append(arr)
+            return arr, 0
+
+        m = OrcMesh()
+        m.dim, m.nele = mesh.dim, mesh.nele
+        m.loc_x, m.nx = mesh.dim + 1, mesh.nx
+        a, _ = k(mesh.ndglno_x, np.int32); m.ndglno_x = _p(a)
+        a, _ = k(mesh.X); m.X = _p(a)
+        a, _ = k(xelement.dn.ravel(order="F")); m.x_dn = _p(a)
+        m.loc, m.ngi, m.nnodes = element.n.shape[0], element.n.shape[1], mesh.nnodes
+        a, _ = k(mesh.ndglno, np.int32); m.ndglno = _p(a)
+        a, _ = k(element.n.ravel(order="F")); m.n = _p(a)
+        a, _ = k(element.dn.ravel(order="F")); m.dn = _p(a)
+        a, _ = k(element.w); m.w = _p(a)
+        a, _ = k(fields["T"]); m.T = _p(a)
+        for key, ptrname, cname, ncomp in (("U", "U", "u_const", mesh.dim),
+                                           ("kappa", "kappa", "kappa_const", mesh.dim * mesh.dim),
+                                           ("src", "src", "src_const", 1),
+                                           ("absorp", "absorp", "absorp_const", 1)):
+            v = fields.get(key)
+            if v is None:
+                continue
+            arr = np.ascontiguousarray(v, dtype=np.float64).ravel()
+            const = 1 if arr.size == ncomp else 0
+            self.keep.append(arr)
+            setattr(m, ptrname, _p(arr))
+            setattr(m, cname, const)
+        findrm, colm, _ = sparsity
+        col_ptr0, elems = colouring
+        m.ncolours = col_ptr0.size - 1
+        a, _ = k(col_ptr0, np.int32); m.col_ptr = _p(a)
+        a, _ = k(elems, np.int32); m.col_elems = _p(a)
+        m.nrows = mesh.nnodes
+        a, _ = k(findrm, np.int32); m.findrm = _p(a)
+        a, _ = k(colm, np.int32); m.colm = _p(a)
+        self.m = m
+        self.nnz = colm.size
+        self.nrows = mesh.nnodes
+
+
+def params(dt, theta, beta, have_mass=True, lump=False, adv=True, ibp=False, diff=True, iso=True,
+           src=False, absorp=False) -> OrcParams:
+    return OrcParams(dt, theta, beta, int(have_mass), int(lump), int(adv), int(ibp), int(diff), int(iso),
+                     int(src), int(absorp))
+
+
+def assemble(p: OrcParams, asm: Assembly, nthreads: int = 0):
+    A = np.zeros(asm.nnz)
+    rhs = np.zeros(asm.nrows)
+    secs = C.c_double(0)
+    rc = load().orc_assemble(C.byref(p), C.byref(asm.m), _p(A), _p(rhs), nthreads, C.byref(secs))
+    if rc != 0:
+        raise RuntimeError("oracle: entry outside the sparsity pattern")
+    return A, rhs, secs.value
+
+
+def element(p: OrcParams, asm: Assembly, ele: int):
+    loc = asm.m.loc
+    mat = np.zeros(loc * loc)
+    r = np.zeros(loc)
+    load().orc_element(C.byref(p), C.byref(asm.m), ele, _p(mat), _p(r))
+    return mat.reshape((loc, loc), order="F"), r
+
+
+def dirichlet(nodes, values, T, dt, inactive, rhs):
+    nodes = np.ascontiguousarray(nodes, np.int32)
+    vals = np.ascontiguousarray(np.broadcast_to(np.asarray(values, np.float64), nodes.shape))
+    Tc = np.ascontiguousarray(T, np.float64)
+    load().orc_dirichlet(nodes.size, _p(nodes), _p(vals), _p(Tc), float(dt), _p(inactive), _p(rhs))
+
+
+def csr_mult(findrm, colm, val, x):
+    y = np.zeros(findrm.size - 1)
+    load().orc_csr_mult(findrm.size - 1, _p(findrm), _p(colm), _p(np.ascontiguousarray(val)),
+                        _p(np.ascontiguousarray(x)), _p(y))
+    return y
+
+
+def petsc_solve(ksp: str, pc: str, rtol: float, sparsity, A, inactive, rhs, x0=None, atol=0.0,
+                max_its=10000, restart=30, nthreads=0):
+    findrm, colm, centrm = sparsity
+    n = findrm.size - 1
+    s = OrcSolver({"cg": 0, "gmres": 1}[ksp], {"none": 0, "jacobi": 1}[pc], rtol, atol, max_its, restart,
+                  nthreads)
+    x = np.zeros(n) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+    its, reason, relres, secs = C.c_int32(0), C.c_int32(0), C.c_double(0), C.c_double(0)
+    inact = None if inactive is None else np.ascontiguousarray(inactive, np.int32)
+    load().orc_petsc_solve(C.byref(s), n, _p(findrm), _p(colm), _p(centrm), _p(np.ascontiguousarray(A)),
+                           _p(inact), _p(np.ascontiguousarray(rhs)), _p(x), C.byref(its), C.byref(reason),
+                           C.byref(relres), C.byref(secs))
+    return x, its.value, reason.value, relres.value, secs.value
+
+
+def ghost_pivot(centrm, A):
+    return load().orc_ghost_pivot(centrm.size, _p(np.ascontiguousarray(centrm, np.int32)),
+                                  _p(np.ascontiguousarray(A)))
+
+
+def reference_system_opts(cfg, variant: dict, nthreads: int = 0):
+    """reference_system with CGOptions-named overrides (lump_mass_matrix,
+    integrate_advection_by_parts, beta, have_absorption, ...)."""
+    return reference_system(cfg, nthreads, variant)
+
+
+def reference_system(cfg, nthreads: int = 0, variant: dict | None = None):
+    """Oracle assembly + Dirichlet of a meshgen.Config: returns a dict with
+    the sparsity, colouring, A, rhs, inactive and timings."""
+    from fluidity_amd.elements import make_element  # tables: the caller's element_type
+
+    mesh = cfg.mesh
+    el = make_element(mesh.dim, mesh.degree, cfg.quad_degree)
+    xel = make_element(mesh.dim, 1, cfg.quad_degree)
+    sp = make_sparsity(mesh)
+    colour, ncol = colour_elements(mesh)
+    cs = colour_sets(colour, ncol)
+    kap = cfg.kappa
+    if kap is not None:
+        d = mesh.dim
+        if np.isscalar(kap):
+            kap = (np.eye(d) * kap).ravel()
+        else:
+            full = np.zeros((mesh.nnodes, d, d))
+            for a in range(d):
+                full[:, a, a] = kap
+            kap = full
+    fields = {"T": cfg.T, "U": cfg.U, "kappa": kap, "src": cfg.source, "absorp": cfg.absorption}
+    asm = Assembly(mesh, el, xel, fields, sp, cs)
+    v = variant or {}
+    p = params(cfg.dt, cfg.theta, v.get("beta", cfg.beta), have_mass=cfg.have_mass,
+               lump=v.get("lump_mass_matrix", False), adv=cfg.have_advection,
+               ibp=v.get("integrate_advection_by_parts", False),
+               diff=cfg.kappa is not None, src=cfg.source is not None, absorp=cfg.absorption is not None)
+    A, rhs, secs = assemble(p, asm, nthreads)
+    inactive = np.zeros(mesh.nnodes, np.int32)
+    dirichlet(mesh.boundary, 0.0, cfg.T, cfg.dt, inactive, rhs)
+    return dict(sparsity=sp, colour=colour, ncol=ncol, colour_sets=cs, A=A, rhs=rhs, inactive=inactive,
+                asm_seconds=secs, asm=asm, params=p)

@@ -1,0 +1,192 @@
+"""GPU parity of the gfx950 library against the oracle (CPU restatement of
+Fluidity's CG advection-diffusion path), through the C ABI.
+
+Tolerances: values are compared with the norm-wise relative error
+max|A_gpu - A_ref| / max|A_ref| (and the same for the rhs); 1e-12 for
+FP64 element arithmetic (the kernels use fused multiply-adds and
+contracted quadrature tables, the oracle follows the Fortran source order),
+1e-5 for FP32 element arithmetic.  The sparsity pattern is compared bit for
+bit.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from fluidity_amd import cg, meshgen as M
+
+pytestmark = pytest.mark.gpu
+
+FP64_RTOL = 1e-12
+FP32_RTOL = 1e-5
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def run_case(cfg, opts_kw=None, precision="fp64", scheme="auto", isotropic=True, kappa_full=None):
+    print(f"\n[case] {cfg.name} n={cfg.mesh.nele} precision={precision} scheme={scheme} {opts_kw}", flush=True)
+    ref = O.reference_system(cfg) if kappa_full is None else None
+    pr = cg.problem_from_config(cfg)
+    opts = cg.options_from_config(cfg, precision=precision, scheme=scheme)
+    for k, v in (opts_kw or {}).items():
+        setattr(opts, k, v)
+    cg.assemble_advection_diffusion_cg(pr, opts, cfg.dt)
+    cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
+    A, rhs, inact = pr.matrix()
+    return pr, ref, A, rhs, inact
+
+
+@pytest.mark.parametrize("name,scale", [("c2", 6), ("c1", 16), ("c3", 3), ("c4", 6), ("c5", 6)])
+def test_assembly_matches_oracle(require_gpu, name, scale):
+    cfg = M.config(name, seed=3, scale=scale)
+    pr, ref, A, rhs, inact = run_case(cfg)
+    f, c, _ = ref["sparsity"]
+    assert np.array_equal(pr.sparsity.findrm, f) and np.array_equal(pr.sparsity.colm, c)
+    assert np.array_equal(inact, ref["inactive"])
+    assert rel(A, ref["A"]) < FP64_RTOL
+    assert rel(rhs, ref["rhs"]) < FP64_RTOL
+    pr.close()
+
+
+@pytest.mark.parametrize("variant", [
+    dict(lump_mass_matrix=True),
+    dict(integrate_advection_by_parts=True),
+    dict(beta=1.0),
+    dict(integrate_advection_by_parts=True, beta=0.3),
+    dict(have_absorption=True),
+])
+def test_assembly_option_variants(require_gpu, variant):
+    cfg = M.config("c1", seed=5, scale=12)
+    # give every term something to do
+    cfg.source = np.cos(3 * cfg.mesh.node_pos[:, 0])
+    cfg.absorption = 0.5 + cfg.mesh.node_pos[:, 1] if variant.get("have_absorption") else None
+    ref = O.reference_system_opts(cfg, variant)
+    pr = cg.problem_from_config(cfg)
+    opts = cg.options_from_config(cfg)
+    for k, v in variant.items():
+        setattr(opts, k, v)
+    print(f"\n[case] c1 variant {variant}", flush=True)
+    cg.assemble_advection_diffusion_cg(pr, opts, cfg.dt)
+    cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
+    A, rhs, _ = pr.matrix()
+    assert rel(A, ref["A"]) < FP64_RTOL
+    assert rel(rhs, ref["rhs"]) < FP64_RTOL
+    pr.close()
+
+
+def test_assembly_p2_fp32(require_gpu):
+    cfg = M.config("c3", seed=2, scale=3)
+    pr, ref, A, rhs, _ = run_case(cfg, precision="fp32")
+    assert rel(A, ref["A"]) < FP32_RTOL
+    assert rel(rhs, ref["rhs"]) < FP32_RTOL
+    pr.close()
+
+
+def test_assembly_deterministic(require_gpu):
+    cfg = M.config("c2", seed=7, scale=8)
+    pr = cg.problem_from_config(cfg)
+    opts = cg.options_from_config(cfg)
+    outs = []
+    for _ in range(3):
+        cg.assemble_advection_diffusion_cg(pr, opts, cfg.dt)
+        outs.append(pr.matrix())
+    for a, b in zip(outs[1:], outs[:-1]):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    pr.close()
+
+
+def test_pcg_fp64_matches_oracle(require_gpu):
+    cfg = M.config("c2", seed=3, scale=8)
+    pr, ref, A, rhs, inact = run_case(cfg)
+    so = cg.SolverOptions("cg", "jacobi", relative_error=1e-10, norm="unpreconditioned")
+    x, info = cg.petsc_solve(pr, so)
+    xr, its, reason, relres, _ = O.petsc_solve("cg", "jacobi", 1e-12, ref["sparsity"], ref["A"], ref["inactive"],
+                                               ref["rhs"])
+    print("gpu its", info.its, "reason", info.reason, "relres", info.true_relres, "oracle its", its, relres)
+    assert info.reason > 0 and reason > 0
+    assert info.true_relres <= 1e-10
+    assert rel(x, xr) < 1e-8
+    # ghost rows take the rhs values (Petsc_Tools.F90:724-731)
+    b = cfg.mesh.boundary - 1
+    assert np.array_equal(x[b], rhs[b])
+    pr.close()
+
+
+def test_pcg_mixed_precision(require_gpu):
+    cfg = M.config("c4", seed=4, scale=10)
+    pr, ref, A, rhs, inact = run_case(cfg)
+    so64 = cg.SolverOptions("cg", "jacobi", relative_error=1e-8, norm="unpreconditioned")
+    x64, i64 = cg.petsc_solve(pr, so64)
+    so32 = cg.SolverOptions("cg", "jacobi", relative_error=1e-8, norm="unpreconditioned", precision="mixed")
+    x32, i32 = cg.petsc_solve(pr, so32)
+    print("fp64", i64.its, i64.true_relres, "mixed", i32.its, i32.true_relres, "refinements", i32.refinements)
+    assert i32.reason > 0 and i32.true_relres <= 1e-8
+    assert rel(x32, x64) < 1e-6
+    pr.close()
+
+
+def test_gmres_c1_matches_scipy(require_gpu):
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spl
+
+    cfg = M.config("c1", seed=1, scale=16)
+    pr, ref, A, rhs, inact = run_case(cfg)
+    so = cg.SolverOptions("gmres", "jacobi", relative_error=1e-10, restart=30)
+    x, info = cg.petsc_solve(pr, so)
+    f, c, cen = ref["sparsity"]
+    n = f.size - 1
+    S = sps.csr_matrix((ref["A"], c - 1, f - 1), shape=(n, n)).tolil()
+    b = ref["rhs"].copy()
+    ghost = ref["inactive"].astype(bool)
+    # reduced system: lifting + ghost pivot
+    g = np.where(ghost, b, 0.0)
+    bb = b - sps.csr_matrix(S) @ g
+    S = S.tocsr()
+    S[:, ghost] = 0.0
+    S[ghost, :] = 0.0
+    piv = O.ghost_pivot(cen, ref["A"])
+    S = S.tolil()
+    for i in np.nonzero(ghost)[0]:
+        S[i, i] = piv
+    bb[ghost] = 0.0
+    xs = spl.spsolve(S.tocsr(), bb)
+    xs[ghost] = b[ghost]
+    print("gmres its", info.its, "reason", info.reason, "relres", info.true_relres)
+    assert info.reason > 0 and info.true_relres < 1e-9
+    assert rel(x, xs) < 1e-8
+    pr.close()
+
+
+def test_solve_field_equation_cg_step(require_gpu):
+    cfg = M.config("c1", seed=2, scale=16)
+    pr = cg.problem_from_config(cfg)
+    opts = cg.options_from_config(cfg)
+    so = cg.SolverOptions("gmres", "jacobi", relative_error=1e-10)
+    T1, its, info = cg.solve_field_equation_cg(pr, opts, so, cfg.dt, cfg.mesh.boundary, 0.0)
+    assert info.reason > 0 and its > 0
+    b = cfg.mesh.boundary - 1
+    np.testing.assert_allclose(T1[b], 0.0, atol=1e-14)  # Dirichlet 0 reached in one step
+    # the bump is advected and diffused but stays bounded
+    assert np.max(T1) <= np.max(cfg.T) * 1.05
+    pr.close()
+
+
+def test_out_of_pattern_insertion_is_an_error(require_gpu):
+    cfg = M.config("c2", seed=1, scale=3)
+    sp = cg.make_sparsity(cfg.mesh)
+    # drop one off-diagonal entry of row 1 from the pattern
+    f = sp.findrm.copy()
+    c = sp.colm.copy()
+    drop = 1 if c[0] == 1 else 0  # an entry that is not the diagonal
+    keep = np.ones(c.size, bool)
+    keep[drop] = False
+    f[1:] -= 1
+    bad = cg.Sparsity(f, c[keep].copy(), np.where(sp.centrm > drop + 1, sp.centrm - 1, sp.centrm).astype(np.int32))
+    pr = cg.CGProblem(cfg.mesh, quad_degree=2, sparsity=bad)
+    pr.set_fields(T=cfg.T, diffusivity=1.0, source=cfg.source)
+    opts = cg.options_from_config(cfg)
+    from fluidity_amd.lib import FlgpuError
+    with pytest.raises(FlgpuError, match="outside sparsity"):
+        cg.assemble_advection_diffusion_cg(pr, opts, cfg.dt)
+    pr.close()
