@@ -131,20 +131,29 @@ class CGProblem:
     """One mesh + element bound to a device context (the state_type of the
     path: positions, the field mesh, its sparsity and colouring)."""
 
+    _mesh_key = 0  # X%refcount%id stand-in: every bound mesh gets a new key
+
     def __init__(self, mesh: Mesh, element: Element | None = None, quad_degree: int = 2,
                  device: int = 0, sparsity: Sparsity | None = None,
                  colouring: tuple | None = None):
         self.lib = L.load()
+        self.ctx = C.c_void_p()
+        L.check(self.lib.flgpu_init(C.byref(self.ctx), device), "flgpu_init")
+        self.bind(mesh, element, quad_degree, sparsity, colouring)
+
+    def bind(self, mesh: Mesh, element: Element | None = None, quad_degree: int = 2,
+             sparsity: Sparsity | None = None, colouring: tuple | None = None) -> None:
+        """(Re)bind a mesh to this context; the learned network state of the
+        context is kept (used to train over many meshes)."""
         self.mesh = mesh
         self.element = element or make_element(mesh.dim, mesh.degree, quad_degree)
         xel = make_element(mesh.dim, 1, self.element.quad_degree)
         self.sparsity = sparsity or make_sparsity(mesh)
         self.colouring = colouring or get_mesh_colouring(mesh)
-        self.ctx = C.c_void_p()
-        L.check(self.lib.flgpu_init(C.byref(self.ctx), device), "flgpu_init")
         e = self.element
         X = np.ascontiguousarray(mesh.X, dtype=np.float64)
-        L.check(self.lib.flgpu_mesh_bind(self.ctx, 1, 0, mesh.dim, mesh.dim + 1, mesh.nele, mesh.nx,
+        CGProblem._mesh_key += 1
+        L.check(self.lib.flgpu_mesh_bind(self.ctx, CGProblem._mesh_key, 0, mesh.dim, mesh.dim + 1, mesh.nele, mesh.nx,
                                          L.ptr(mesh.ndglno_x), L.ptr(X)), "flgpu_mesh_bind")
         L.check(self.lib.flgpu_field_mesh_bind(self.ctx, e.loc, e.ngi, mesh.nnodes, L.ptr(mesh.ndglno),
                                                L.ptr(e.fortran(e.n)), L.ptr(e.fortran(e.dn)),

@@ -16,7 +16,9 @@
 // (host-side, exact algebra); quadratic T keeps the per-point loops.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "flgpu_internal.h"
@@ -122,6 +124,272 @@ static int launch_colour(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, 
     return FLGPU_OK;
 }
 
+// ---------------------------------------------------------------------
+// Blocked scheme (linear T, FP64): one workgroup per block of rows.  The
+// block's CSR rows live in LDS; the elements touching the block are
+// visited colour by colour (ascending id inside a colour) with a barrier
+// between colours, so every entry receives its contributions in the same
+// order as in the colour scheme and ends with the same bits.  Each row is
+// written to HBM once; the matrix needs no zeroing pass.
+// ---------------------------------------------------------------------
+
+// Position of column cj in lcol[lo, hi) (sorted); -1 when absent.
+__device__ __forceinline__ int lds_pos(const int32_t* lcol, int lo, int hi, int cj) {
+    hi -= 1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = lcol[mid];
+        if (v == cj) return mid;
+        if (v < cj) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    return -1;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256)
+k_asm_blocked(const ElemTables* __restrict__ tab, int ncol, int rows_cap, int nnz_cap,
+              const int32_t* __restrict__ brow, const int32_t* __restrict__ brows,
+              const int32_t* __restrict__ bseg, const int32_t* __restrict__ visits,
+              const int32_t* __restrict__ node_block, const int32_t* __restrict__ node_lrow,
+              const int32_t* __restrict__ conn, const int32_t* __restrict__ conn_x,
+              const double* __restrict__ X4, AsmFields fd, AsmFlags fl,
+              const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+              double* __restrict__ A, double* __restrict__ rhs, int* __restrict__ err) {
+    constexpr int LOC = DIM + 1;
+    __shared__ LTab<double> tb;
+    __shared__ int wsum[256 / 64 + 1];
+    extern __shared__ double dyn[];
+    double* acc = dyn;                                      // nnz_cap
+    double* lrhs = acc + nnz_cap;                           // rows_cap
+    int32_t* loff = reinterpret_cast<int32_t*>(lrhs + rows_cap);  // rows_cap + 1
+    int32_t* lcol = loff + rows_cap + 1;                    // nnz_cap
+    stage_tables<double>(tab, tb);
+
+    const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int r0 = brow[b], R = brow[b + 1] - r0;
+    // row lengths -> exclusive prefix sum (each thread owns a chunk of rows)
+    const int per = (R + nt - 1) / nt;
+    const int l0 = min(tid * per, R), l1 = min(l0 + per, R);
+    int mysum = 0;
+    for (int l = l0; l < l1; l++) {
+        const int row = brows[r0 + l];
+        mysum += rowptr[row + 1] - rowptr[row];
+    }
+    // block exclusive scan of mysum
+    const int lane = tid & 63, wave = tid >> 6;
+    int incl = mysum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int w = 0; w < nt / 64; w++) {
+            const int t = wsum[w];
+            wsum[w] = run;
+            run += t;
+        }
+        wsum[nt / 64] = run;
+    }
+    __syncthreads();
+    int pos = wsum[wave] + incl - mysum;
+    for (int l = l0; l < l1; l++) {
+        const int row = brows[r0 + l];
+        const int s = rowptr[row], len = rowptr[row + 1] - s;
+        loff[l] = pos;
+        for (int k = 0; k < len; k++) {
+            lcol[pos + k] = col[s + k];
+            acc[pos + k] = 0.0;
+        }
+        lrhs[l] = 0.0;
+        pos += len;
+    }
+    if (tid == 0) loff[R] = wsum[nt / 64];
+    __syncthreads();
+
+    const int32_t* seg = bseg + (int64_t)b * (ncol + 1);
+    for (int cc = 0; cc < ncol; cc++) {
+        const int s1 = seg[cc + 1];
+        for (int v = seg[cc] + tid; v < s1; v += nt) {
+            const int ele = visits[v];
+            Elem<DIM, LOC, double> e;
+            gather_elem<DIM, LOC, double>(ele, conn, conn_x, X4, fd.T, e);
+            GeomLin<DIM, LOC, double> gm;
+            geom_linear<DIM, LOC, double>(e, tb, gm);
+#pragma unroll
+            for (int i = 0; i < LOC; i++) {
+                const int node = e.nodes[i];
+                if (node_block[node] != b) continue;
+                const int l = node_lrow[node];
+                double vals[LOC], r;
+                row_linear_g<DIM, LOC, double>(e, gm, tb, fl, fd, i, vals, r);
+                const int lo = loff[l], hi = loff[l + 1];
+#pragma unroll
+                for (int j = 0; j < LOC; j++) {
+                    if (vals[j] == 0.0) continue;  // csr_addto skips zeros
+                    const int q = lds_pos(lcol, lo, hi, e.nodes[j]);
+                    if (q < 0) {
+                        atomicOr(err, 1);
+                        continue;
+                    }
+                    acc[q] = acc[q] + vals[j];
+                }
+                lrhs[l] = lrhs[l] + r;
+            }
+        }
+        __syncthreads();
+    }
+    // write the block's rows and rhs entries
+    for (int l = tid; l < R; l += nt) {
+        const int row = brows[r0 + l];
+        const int s = rowptr[row];
+        const int lo = loff[l], hi = loff[l + 1];
+        for (int k = lo; k < hi; k++) A[s + (k - lo)] = acc[k];
+        rhs[row] = lrhs[l];
+    }
+}
+
+static uint32_t spread_bits(uint32_t v, int dim) {
+    uint32_t r = 0;
+    for (int k = 0; k < 10; k++) r |= ((v >> k) & 1u) << (k * dim);
+    return r;
+}
+
+int build_block_plan(flgpu_ctx* c) {
+    if (c->plan.built) return FLGPU_OK;
+    if (!c->linear) return set_error(FLGPU_EUNSUPPORTED, "block plan: linear field mesh only");
+    const int N = c->rows, dim = c->dim, loc = c->loc, ncol = c->ncol;
+    int rows_cap = 256;
+    if (const char* s = getenv("FLGPU_BLOCK_ROWS")) rows_cap = std::max(32, std::min(1024, atoi(s)));
+    // node positions of the field mesh (vertex i of T <-> vertex i of X)
+    std::vector<float> pos((size_t)N * dim, 0.f);
+    {
+        std::vector<double> X4h((size_t)c->nx * 4);
+        FLGPU_HIP(hipMemcpy(X4h.data(), c->X4.p, sizeof(double) * X4h.size(), hipMemcpyDeviceToHost));
+        for (int64_t e = 0; e < c->nele; e++)
+            for (int i = 0; i < loc; i++) {
+                const int nd = c->h_conn[e * loc + i], xn = c->h_conn_x[e * c->loc_x + i];
+                for (int d = 0; d < dim; d++) pos[(size_t)nd * dim + d] = (float)X4h[(size_t)xn * 4 + d];
+            }
+    }
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    for (int i = 0; i < N; i++)
+        for (int d = 0; d < dim; d++) {
+            lo[d] = std::min(lo[d], pos[(size_t)i * dim + d]);
+            hi[d] = std::max(hi[d], pos[(size_t)i * dim + d]);
+        }
+    std::vector<uint64_t> key(N);
+    for (int i = 0; i < N; i++) {
+        uint32_t code = 0;
+        for (int d = 0; d < dim; d++) {
+            const float span = hi[d] > lo[d] ? hi[d] - lo[d] : 1.f;
+            uint32_t q = (uint32_t)std::min(1023.f, std::max(0.f, (pos[(size_t)i * dim + d] - lo[d]) / span * 1023.f));
+            code |= spread_bits(q, dim) << d;
+        }
+        key[i] = ((uint64_t)code << 32) | (uint32_t)i;
+    }
+    std::sort(key.begin(), key.end());
+    // cut into blocks
+    std::vector<int32_t> brow(1, 0), brows(N), node_block(N), node_lrow(N);
+    int cur = 0, cur_nnz = 0, max_nnz = 0, max_rows = 0;
+    const int nnz_cap = rows_cap * 20;
+    for (int k = 0; k < N; k++) {
+        const int r = (int)(key[k] & 0xffffffffu);
+        const int len = c->h_rowptr[r + 1] - c->h_rowptr[r];
+        if (cur > 0 && (cur == rows_cap || cur_nnz + len > nnz_cap)) {
+            brow.push_back(k);
+            max_nnz = std::max(max_nnz, cur_nnz);
+            max_rows = std::max(max_rows, cur);
+            cur = 0;
+            cur_nnz = 0;
+        }
+        brows[k] = r;
+        node_block[r] = (int)brow.size() - 1;
+        node_lrow[r] = cur;
+        cur++;
+        cur_nnz += len;
+    }
+    brow.push_back(N);
+    max_nnz = std::max(max_nnz, cur_nnz);
+    max_rows = std::max(max_rows, cur);
+    const int nb = (int)brow.size() - 1;
+    // colour rank of every element
+    std::vector<int32_t> rank(c->nele), colour_of(c->nele);
+    for (int k = 0; k < ncol; k++)
+        for (int q = c->h_col_ptr[k]; q < c->h_col_ptr[k + 1]; q++) {
+            rank[c->h_col_elems[q]] = q;
+            colour_of[c->h_col_elems[q]] = k;
+        }
+    // node -> element incidence
+    std::vector<int64_t> iptr((size_t)N + 1, 0);
+    for (int64_t k = 0; k < (int64_t)c->nele * loc; k++) iptr[c->h_conn[k] + 1]++;
+    for (int i = 0; i < N; i++) iptr[i + 1] += iptr[i];
+    std::vector<int32_t> inc(iptr[N]);
+    {
+        std::vector<int64_t> fill(iptr.begin(), iptr.end() - 1);
+        for (int e = 0; e < c->nele; e++)
+            for (int i = 0; i < loc; i++) inc[fill[c->h_conn[(int64_t)e * loc + i]]++] = e;
+    }
+    std::vector<int32_t> stamp(c->nele, -1), seg((size_t)nb * (ncol + 1)), visits, tmp;
+    visits.reserve((size_t)c->nele * 2);
+    for (int b = 0; b < nb; b++) {
+        tmp.clear();
+        for (int k = brow[b]; k < brow[b + 1]; k++) {
+            const int r = brows[k];
+            for (int64_t q = iptr[r]; q < iptr[r + 1]; q++) {
+                const int e = inc[q];
+                if (stamp[e] != b) {
+                    stamp[e] = b;
+                    tmp.push_back(rank[e]);
+                }
+            }
+        }
+        std::sort(tmp.begin(), tmp.end());
+        int32_t* sg = &seg[(size_t)b * (ncol + 1)];
+        int cc = 0;
+        sg[0] = (int32_t)visits.size();
+        for (int32_t rk : tmp) {
+            const int e = c->h_col_elems[rk];
+            while (cc < colour_of[e]) sg[++cc] = (int32_t)visits.size();
+            visits.push_back(e);
+        }
+        while (cc < ncol) sg[++cc] = (int32_t)visits.size();
+    }
+    BlockPlan& P = c->plan;
+    FLGPU_TRY(P.brow.upload(brow.data(), brow.size(), c->stream));
+    FLGPU_TRY(P.brows.upload(brows.data(), brows.size(), c->stream));
+    FLGPU_TRY(P.node_block.upload(node_block.data(), node_block.size(), c->stream));
+    FLGPU_TRY(P.node_lrow.upload(node_lrow.data(), node_lrow.size(), c->stream));
+    FLGPU_TRY(P.seg.upload(seg.data(), seg.size(), c->stream));
+    FLGPU_TRY(P.visits.upload(visits.data(), visits.size(), c->stream));
+    FLGPU_HIP(hipStreamSynchronize(c->stream));
+    P.nblocks = nb;
+    P.rows_cap = max_rows;
+    P.nnz_cap = max_nnz;
+    P.nvisits = (int64_t)visits.size();
+    P.built = true;
+    return FLGPU_OK;
+}
+
+template <int DIM>
+static int launch_blocked(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
+    const BlockPlan& P = c->plan;
+    const size_t lds = sizeof(double) * (P.nnz_cap + P.rows_cap) + sizeof(int32_t) * (P.rows_cap + 1 + P.nnz_cap);
+    if (lds + sizeof(LTab<double>) + 64 > 160 * 1024)
+        return set_error(FLGPU_EUNSUPPORTED, "blocked assembly: block needs %zu bytes of LDS", lds);
+    FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_blocked<DIM>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_asm_blocked<DIM>), dim3(P.nblocks), dim3(256), lds, c->stream, c->d_tab.p, c->ncol,
+                       P.rows_cap, P.nnz_cap, P.brow.p, P.brows.p, P.seg.p, P.visits.p, P.node_block.p,
+                       P.node_lrow.p, c->conn.p, c->conn_x.p, c->X4.p, fd, fl, c->rowptr.p, c->col.p, c->A.p,
+                       c->rhs.p, d_err);
+    FLGPU_HIP(hipGetLastError());
+    return FLGPU_OK;
+}
+
 static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_err) {
     if (c->nele <= 0 || c->loc <= 0 || c->nnz <= 0 || c->ncol <= 0)
         return set_error(FLGPU_EINVAL, "flgpu assemble: bind mesh, field mesh, sparsity and colouring first");
@@ -156,17 +424,25 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     fd.S = c->fS.v.p; fd.s_c = c->fS.constant;
     fd.B = c->fA.v.p; fd.b_c = c->fA.constant;
 
-    FLGPU_HIP(hipMemsetAsync(c->A.p, 0, sizeof(double) * c->nnz, c->stream));
-    FLGPU_HIP(hipMemsetAsync(c->rhs.p, 0, sizeof(double) * c->rows, c->stream));
-    FLGPU_HIP(hipMemsetAsync(c->inactive.p, 0, sizeof(int32_t) * c->rows, c->stream));
-    c->have_inactive = false;
-
     const int dim = c->dim, loc = c->loc;
     const bool fp32 = p->precision == FLGPU_FP32;
     int rc = FLGPU_OK;
     int scheme = p->scheme;
-    if (scheme == FLGPU_ASM_BLOCKED && !(c->linear && !fp32))
+    const bool blocked_ok = c->linear && !fp32;
+    if (scheme == FLGPU_ASM_BLOCKED && !blocked_ok)
         return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: the blocked scheme covers linear T in FP64");
+    if (scheme == FLGPU_ASM_AUTO) scheme = blocked_ok ? FLGPU_ASM_BLOCKED : FLGPU_ASM_COLOUR;
+    if (scheme != FLGPU_ASM_COLOUR && scheme != FLGPU_ASM_BLOCKED)
+        return set_error(FLGPU_EINVAL, "flgpu assemble: scheme %d", p->scheme);
+
+    FLGPU_HIP(hipMemsetAsync(c->inactive.p, 0, sizeof(int32_t) * c->rows, c->stream));
+    c->have_inactive = false;
+    if (scheme == FLGPU_ASM_BLOCKED) {
+        FLGPU_TRY(build_block_plan(c));
+        return dim == 3 ? launch_blocked<3>(c, fd, fl, d_err) : launch_blocked<2>(c, fd, fl, d_err);
+    }
+    FLGPU_HIP(hipMemsetAsync(c->A.p, 0, sizeof(double) * c->nnz, c->stream));
+    FLGPU_HIP(hipMemsetAsync(c->rhs.p, 0, sizeof(double) * c->rows, c->stream));
 #define FL_DISPATCH(D, L)                                                                    \
     if (dim == D && loc == L) {                                                              \
         rc = fp32 ? launch_colour<D, L, float>(c, fd, fl, d_err)                           \
@@ -202,6 +478,7 @@ int assemble(flgpu_ctx* c, const flgpu_advdif_params* p) {
 
 int time_assemble(flgpu_ctx* c, const flgpu_advdif_params* p, int reps, double* ms) {
     if (reps < 1) return set_error(FLGPU_EINVAL, "flgpu_time_assemble: reps < 1");
+    krylov_invalidate(c);
     DevBuf<int> d_err;
     FLGPU_TRY(d_err.alloc(1));
     FLGPU_HIP(hipMemsetAsync(d_err.p, 0, sizeof(int), c->stream));

@@ -113,24 +113,33 @@ FLGPU_HD_INLINE R fld(const double* __restrict__ f, int c, int node, int comp, i
     return (R)(c ? f[comp] : f[(int64_t)node * ncomp + comp]);
 }
 
+// Geometry of a linear element: physical gradients G[n][d] = sum_k
+// dn(n,k) invJ(d,k) (femtools/Transform_elements.F90:777-780) and |detJ|.
+template <int DIM, int LOC, typename R>
+struct GeomLin {
+    R G[LOC][DIM];
+    R adet;
+};
+
+template <int DIM, int LOC, typename R>
+FLGPU_HD_INLINE void geom_linear(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, GeomLin<DIM, LOC, R>& gm) {
+    R iJ[DIM][DIM];
+    geometry<DIM, R>(e.X, tb, iJ, gm.adet);
+    for (int nn = 0; nn < LOC; nn++)
+        for (int d = 0; d < DIM; d++) {
+            R s = (R)0;
+            for (int k = 0; k < DIM; k++) s = FMA(tb.dn[nn + LOC * (0 + tb.ngi * k)], iJ[d][k], s);
+            gm.G[nn][d] = s;
+        }
+}
+
 // Row i of the element matrix (vals[j], j = 0..LOC-1) and of the element
 // rhs, for linear T (gradients constant over the element).
 template <int DIM, int LOC, typename R>
-FLGPU_HD void row_linear(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, const AsmFlags& fl,
-                           const AsmFields& fd, int i, R (&vals)[LOC], R& rhs_i) {
-    R iJ[DIM][DIM], adet;
-    geometry<DIM, R>(e.X, tb, iJ, adet);
-    // physical gradients G[n][d] = sum_k dn(n,k) invJ(d,k)
-    R G[LOC][DIM];
-#pragma unroll
-    for (int nn = 0; nn < LOC; nn++)
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            R s = (R)0;
-#pragma unroll
-            for (int k = 0; k < DIM; k++) s = FMA(tb.dn[nn + LOC * (0 + tb.ngi * k)], iJ[d][k], s);
-            G[nn][d] = s;
-        }
+FLGPU_HD void row_linear_g(const Elem<DIM, LOC, R>& e, const GeomLin<DIM, LOC, R>& gm, const LTab<R>& tb,
+                           const AsmFlags& fl, const AsmFields& fd, int i, R (&vals)[LOC], R& rhs_i) {
+    const R(&G)[LOC][DIM] = gm.G;
+    const R adet = gm.adet;
     const R dtt = (R)fl.dt_theta;
     R M[LOC], Aa[LOC], Bb[LOC], Kk[LOC];
 #pragma unroll
@@ -282,6 +291,14 @@ FLGPU_HD void row_linear(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, const As
         }
         vals[j] = m;
     }
+}
+
+template <int DIM, int LOC, typename R>
+FLGPU_HD_INLINE void row_linear(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, const AsmFlags& fl,
+                                const AsmFields& fd, int i, R (&vals)[LOC], R& rhs_i) {
+    GeomLin<DIM, LOC, R> gm;
+    geom_linear<DIM, LOC, R>(e, tb, gm);
+    row_linear_g<DIM, LOC, R>(e, gm, tb, fl, fd, i, vals, rhs_i);
 }
 
 // Row i for quadratic T: per-quadrature-point loops

@@ -208,6 +208,8 @@ int flgpu_sparsity_bind(flgpu_ctx* c, int32_t rows, int32_t cols, const int32_t*
     FLGPU_TRY(c->inactive.alloc(rows));
     FLGPU_HIP(hipMemsetAsync(c->inactive.p, 0, sizeof(int32_t) * rows, c->stream));
     FLGPU_HIP(hipStreamSynchronize(c->stream));
+    c->all_diag = true;
+    for (int i = 0; i < rows; i++) c->all_diag = c->all_diag && dg[i] >= 0;
     c->h_rowptr.swap(rp);
     c->h_col.swap(cl);
     c->rows = rows;
@@ -216,6 +218,8 @@ int flgpu_sparsity_bind(flgpu_ctx* c, int32_t rows, int32_t cols, const int32_t*
     c->have_inactive = false;
     c->assembled = false;
     c->plan.built = false;
+    c->sparsity_gen++;
+    krylov_invalidate(c);
     return FLGPU_OK;
 }
 
@@ -376,6 +380,7 @@ int flgpu_field_set(flgpu_ctx* c, int32_t which, int32_t constant, const double*
 int flgpu_advdif_assemble_resident(flgpu_ctx* c, const flgpu_advdif_params* p) {
     if (!c || !p) return set_error(FLGPU_EINVAL, "flgpu_advdif_assemble_resident: null argument");
     FLGPU_HIP(hipSetDevice(c->device));
+    krylov_invalidate(c);
     return assemble(c, p);
 }
 
@@ -415,6 +420,7 @@ int flgpu_dirichlet_apply(flgpu_ctx* c, int32_t nbc, const int32_t* nodes, const
     FLGPU_HIP(hipGetLastError());
     FLGPU_HIP(hipStreamSynchronize(c->stream));
     c->have_inactive = true;
+    krylov_invalidate(c);
     return FLGPU_OK;
 }
 
@@ -437,6 +443,7 @@ int flgpu_matrix_set(flgpu_ctx* c, const double* A_val, const double* rhs, const
     if (!c) return set_error(FLGPU_EINVAL, "flgpu_matrix_set: null ctx");
     if (c->nnz <= 0) return set_error(FLGPU_EINVAL, "flgpu_matrix_set: bind a sparsity first");
     FLGPU_HIP(hipSetDevice(c->device));
+    krylov_invalidate(c);
     if (A_val) {
         FLGPU_HIP(hipMemcpyAsync(c->A.p, A_val, sizeof(double) * c->nnz, hipMemcpyHostToDevice, c->stream));
         c->assembled = true;

@@ -91,14 +91,18 @@ struct DevField {
     int32_t set = 0;
 };
 
-// Row-block plan of the LDS-blocked P1 assembly.
+// Row-block plan of the LDS-blocked P1 assembly: rows grouped in Morton
+// order of their positions; per block the elements touching it, sorted by
+// (colour, element id).
 struct BlockPlan {
-    int32_t rows_per_block = 0;
-    int32_t nblocks = 0;
-    DevBuf<int32_t> seg;      // nblocks*(ncol+1) offsets into visits
-    DevBuf<int32_t> visits;   // element ids (0-based) in colour order per block
-    int32_t max_row_nnz = 0;  // longest row in any block
-    int32_t max_block_nnz = 0;
+    DevBuf<int32_t> brow;        // nblocks+1 offsets into brows
+    DevBuf<int32_t> brows;       // rows of each block
+    DevBuf<int32_t> node_block;  // block of every row
+    DevBuf<int32_t> node_lrow;   // position of every row inside its block
+    DevBuf<int32_t> seg;         // nblocks*(ncol+1) offsets into visits
+    DevBuf<int32_t> visits;      // element ids per block, colour-major
+    int32_t nblocks = 0, rows_cap = 0, nnz_cap = 0;
+    int64_t nvisits = 0;
     bool built = false;
 };
 
@@ -130,6 +134,8 @@ struct flgpu_ctx {
     // sparsity (0-based)
     int32_t rows = 0, cols = 0;
     int64_t nnz = 0;
+    int64_t sparsity_gen = 0;        // bumped by every flgpu_sparsity_bind
+    bool all_diag = false;           // every row holds its diagonal entry
     flgpu::DevBuf<int32_t> rowptr, col, diag;
     std::vector<int32_t> h_rowptr, h_col;
 
@@ -161,6 +167,7 @@ int time_assemble(flgpu_ctx* c, const flgpu_advdif_params* p, int reps, double* 
 int solve(flgpu_ctx* c, const flgpu_solver_params* sp, double* x_host,
           flgpu_solve_info* info);
 void krylov_free(flgpu_ctx* c);
+void krylov_invalidate(flgpu_ctx* c);  // the assembled system changed
 int solver_prepare(flgpu_ctx* c, int precision);
 int time_spmv(flgpu_ctx* c, int precision, int reps, double* ms);
 int csr_mult(flgpu_ctx* c, const double* x, double* y);

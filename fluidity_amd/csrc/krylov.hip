@@ -89,17 +89,35 @@ k_diag_minmax(int n, const int32_t* __restrict__ diag, const double* __restrict_
         ticket = atomicAdd(counter, 1u);
     }
     __syncthreads();
-    if (ticket != gridDim.x - 1 || threadIdx.x != 0) return;
+    if (ticket != gridDim.x - 1) return;
     __threadfence();
     double a = 1.79769313486231570e308, b = -1.79769313486231570e308;
-    for (int q = 0; q < (int)gridDim.x; q++) {
-        a = fmin(a, load_coherent(&partials[2 * q]));
-        b = fmax(b, load_coherent(&partials[2 * q + 1]));
+    for (int q = threadIdx.x; q < (int)gridDim.x; q += blockDim.x) {
+        a = fmin(a, partials[2 * q]);
+        b = fmax(b, partials[2 * q + 1]);
     }
-    sc->dmin = a;
-    sc->dmax = b;
-    sc->pivot = sqrt((b + a) * b / 2.0);
-    *counter = 0u;
+    for (int off = 32; off > 0; off >>= 1) {
+        a = fmin(a, __shfl_xor(a, off, 64));
+        b = fmax(b, __shfl_xor(b, off, 64));
+    }
+    __syncthreads();
+    if (lane == 0) {
+        sh[wave][0] = a;
+        sh[wave][1] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
+            sh[0][0] = fmin(sh[0][0], sh[q][0]);
+            sh[0][1] = fmax(sh[0][1], sh[q][1]);
+        }
+        a = sh[0][0];
+        b = sh[0][1];
+        sc->dmin = a;
+        sc->dmax = b;
+        sc->pivot = sqrt((b + a) * b / 2.0);
+        *counter = 0u;
+    }
 }
 
 // Thread per row: copy to the reduced values, lift inactive columns, set
@@ -196,6 +214,38 @@ k_spmv(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ co
 // PCG / FCG vector kernels
 // ---------------------------------------------------------------------
 
+// Convergence test of KSPConvergedDefault on the norm chosen by norm_type,
+// then beta (Fletcher-Reeves, or Polak-Ribiere when flexible).
+__device__ void cg_test_and_beta(Scal* sc) {
+    sc->its += 1;
+    const double nrm = sc->norm_type == FLGPU_NORM_UNPRECONDITIONED ? sqrt(sc->rr) : sqrt(sc->zz);
+    sc->last_norm = nrm;
+    if (nrm != nrm || isinf(nrm)) {
+        sc->reason = KSP_DIVERGED_NANORINF;
+        sc->done = 1;
+    } else if (nrm <= sc->atol) {
+        sc->reason = KSP_CONVERGED_ATOL;
+        sc->done = 1;
+    } else if (nrm <= sc->rtol * sc->rnorm0) {
+        sc->reason = KSP_CONVERGED_RTOL;
+        sc->done = 1;
+    } else if (sc->rnorm0 > 0 && nrm >= 1e5 * sc->rnorm0) {
+        sc->reason = KSP_DIVERGED_DTOL;
+        sc->done = 1;
+    } else if (sc->its >= sc->max_its) {
+        sc->reason = KSP_DIVERGED_ITS;
+        sc->done = 1;
+    }
+    if (sc->flexible)
+        sc->beta = -sc->alpha * sc->zw / sc->rz_old;  // z.(r_new - r_old) / (z_old.r_old)
+    else
+        sc->beta = sc->rz / sc->rz_old;
+    if (!sc->done && sc->rz < 0.0) {
+        sc->reason = KSP_DIVERGED_INDEFINITE_PC;
+        sc->done = 1;
+    }
+}
+
 // x += alpha p; r -= alpha w; [Jacobi: z = D^-1 r];
 // partials: r.z, r.r, z.z, z.w (for the flexible beta)
 template <typename V>
@@ -229,39 +279,8 @@ k_cg_update(int n, double* __restrict__ x, const V* __restrict__ p, V* __restric
             sc->rz = out[0];
             sc->zz = out[2];
             sc->zw = out[3];
+            cg_test_and_beta(sc);
         }
-    }
-}
-
-// Convergence test of KSPConvergedDefault on the norm chosen by norm_type,
-// then beta (Fletcher-Reeves, or Polak-Ribiere when flexible).
-__device__ void cg_test_and_beta(Scal* sc) {
-    sc->its += 1;
-    const double nrm = sc->norm_type == FLGPU_NORM_UNPRECONDITIONED ? sqrt(sc->rr) : sqrt(sc->zz);
-    sc->last_norm = nrm;
-    if (nrm != nrm || isinf(nrm)) {
-        sc->reason = KSP_DIVERGED_NANORINF;
-        sc->done = 1;
-    } else if (nrm <= sc->atol) {
-        sc->reason = KSP_CONVERGED_ATOL;
-        sc->done = 1;
-    } else if (nrm <= sc->rtol * sc->rnorm0) {
-        sc->reason = KSP_CONVERGED_RTOL;
-        sc->done = 1;
-    } else if (sc->rnorm0 > 0 && nrm >= 1e5 * sc->rnorm0) {
-        sc->reason = KSP_DIVERGED_DTOL;
-        sc->done = 1;
-    } else if (sc->its >= sc->max_its) {
-        sc->reason = KSP_DIVERGED_ITS;
-        sc->done = 1;
-    }
-    if (sc->flexible)
-        sc->beta = -sc->alpha * sc->zw / sc->rz_old;  // z.(r_new - r_old) / (z_old.r_old)
-    else
-        sc->beta = sc->rz / sc->rz_old;
-    if (!sc->done && sc->rz < 0.0) {
-        sc->reason = KSP_DIVERGED_INDEFINITE_PC;
-        sc->done = 1;
     }
 }
 
@@ -537,8 +556,10 @@ static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info*
             hipLaunchKernelGGL((k_cg_update<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->x.p,
                                v.p, v.r, v.w, v.z, v.dinv, jacobi ? 1 : 0, k->partials.p, k->counter.p,
                                k->scal.p);
-            if (!jacobi) FLGPU_TRY(learned_apply<V>(c, k, v, false));
-            hipLaunchKernelGGL(k_cg_test, dim3(1), dim3(64), 0, c->stream, k->scal.p);
+            if (!jacobi) {
+                FLGPU_TRY(learned_apply<V>(c, k, v, false));
+                hipLaunchKernelGGL(k_cg_test, dim3(1), dim3(64), 0, c->stream, k->scal.p);
+            }
             hipLaunchKernelGGL((k_cg_p<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, v.z, v.p,
                                k->scal.p, 0);
         }
@@ -887,6 +908,8 @@ int solve(flgpu_ctx* c, const flgpu_solver_params* sp, double* x_host, flgpu_sol
     if (x_host && !sp->start_from_zero)
         FLGPU_HIP(hipMemcpyAsync(k->x.p, x_host, sizeof(double) * c->rows, hipMemcpyHostToDevice, c->stream));
     FLGPU_TRY(build_reduced(c, k, mixed, zero_x));
+    k->prepared = true;
+    k->prepared_mixed = mixed;
     if (sp->pc == FLGPU_PC_LEARNED) FLGPU_TRY(learned_prepare(c, mixed));
     if (sp->ksp == FLGPU_KSP_GMRES) {
         FLGPU_TRY(mixed ? run_gmres<float>(c, sp, &info) : run_gmres<double>(c, sp, &info));
@@ -903,6 +926,10 @@ int solve(flgpu_ctx* c, const flgpu_solver_params* sp, double* x_host, flgpu_sol
     k->have_x = true;
     if (info_out) *info_out = info;
     return FLGPU_OK;
+}
+
+void krylov_invalidate(flgpu_ctx* c) {
+    if (c->kw) c->kw->prepared = c->kw->prepared_mixed = false;
 }
 
 void krylov_free(flgpu_ctx* c) {
@@ -927,6 +954,13 @@ int csr_mult(flgpu_ctx* c, const double* x, double* y) {
     FLGPU_HIP(hipMemcpyAsync(y, dy.p, sizeof(double) * c->rows, hipMemcpyDeviceToHost, c->stream));
     FLGPU_HIP(hipStreamSynchronize(c->stream));
     return FLGPU_OK;
+}
+
+const double* reduced_values(flgpu_ctx* c) { return c->kw ? c->kw->val64.p : nullptr; }
+
+int reduced_prepare(flgpu_ctx* c) {
+    if (c->kw && c->kw->prepared && c->kw->prepared_mixed) return FLGPU_OK;
+    return solver_prepare(c, FLGPU_MIXED);
 }
 
 int solver_prepare(flgpu_ctx* c, int precision) {

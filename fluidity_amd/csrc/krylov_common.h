@@ -78,13 +78,16 @@ __device__ bool grid_sum(double (&v)[NV], double* __restrict__ partials, unsigne
     }
     __syncthreads();
     if (ticket != gridDim.x - 1) return false;
+    // acquire (agent-scope fence: stale L1 lines are dropped), then the
+    // partials are read with ordinary, pipelined loads
     __threadfence();
+    const double* vp = partials;
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) acc[k] = 0.0;
     for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
 #pragma unroll
-        for (int k = 0; k < NV; k++) acc[k] += load_coherent(&partials[(size_t)b * NV + k]);
+        for (int k = 0; k < NV; k++) acc[k] += vp[(size_t)b * NV + k];
     block_sum<NV>(acc, sh);
     if (threadIdx.x == 0) {
 #pragma unroll

@@ -279,3 +279,21 @@ def reference_system(cfg, nthreads: int = 0, variant: dict | None = None):
     dirichlet(mesh.boundary, 0.0, cfg.T, cfg.dt, inactive, rhs)
     return dict(sparsity=sp, colour=colour, ncol=ncol, colour_sets=cs, A=A, rhs=rhs, inactive=inactive,
                 asm_seconds=secs, asm=asm, params=p)
+
+
+def reduced_system(sparsity, A, inactive):
+    """The system the Krylov method sees (femtools/Petsc_Tools.F90:891-1074,
+    femtools/Solvers.F90:1052-1126): inactive rows keep only the ghost pivot
+    on the diagonal, inactive columns are dropped.  0-based CSR."""
+    findrm, colm, centrm = sparsity
+    n = findrm.size - 1
+    rowptr = (findrm - 1).astype(np.int64)
+    col = (colm - 1).astype(np.int64)
+    row = np.repeat(np.arange(n), np.diff(rowptr))
+    val = np.asarray(A, dtype=np.float64).copy()
+    ghost = np.asarray(inactive).astype(bool)
+    piv = ghost_pivot(centrm, A)
+    val[ghost[col]] = 0.0
+    gr = ghost[row]
+    val[gr] = np.where(col[gr] == row[gr], piv, 0.0)
+    return rowptr, col, val

@@ -37,10 +37,12 @@ def run_case(cfg, opts_kw=None, precision="fp64", scheme="auto", isotropic=True,
     return pr, ref, A, rhs, inact
 
 
-@pytest.mark.parametrize("name,scale", [("c2", 6), ("c1", 16), ("c3", 3), ("c4", 6), ("c5", 6)])
-def test_assembly_matches_oracle(require_gpu, name, scale):
+@pytest.mark.parametrize("name,scale,scheme", [("c2", 6, "colour"), ("c2", 6, "blocked"), ("c1", 16, "colour"),
+                                               ("c1", 16, "blocked"), ("c3", 3, "colour"), ("c4", 6, "blocked"),
+                                               ("c5", 6, "blocked"), ("c5", 9, "colour")])
+def test_assembly_matches_oracle(require_gpu, name, scale, scheme):
     cfg = M.config(name, seed=3, scale=scale)
-    pr, ref, A, rhs, inact = run_case(cfg)
+    pr, ref, A, rhs, inact = run_case(cfg, scheme=scheme)
     f, c, _ = ref["sparsity"]
     assert np.array_equal(pr.sparsity.findrm, f) and np.array_equal(pr.sparsity.colm, c)
     assert np.array_equal(inact, ref["inactive"])
@@ -80,6 +82,22 @@ def test_assembly_p2_fp32(require_gpu):
     pr, ref, A, rhs, _ = run_case(cfg, precision="fp32")
     assert rel(A, ref["A"]) < FP32_RTOL
     assert rel(rhs, ref["rhs"]) < FP32_RTOL
+    pr.close()
+
+
+@pytest.mark.parametrize("name,scale", [("c1", 20), ("c2", 10), ("c5", 12)])
+def test_blocked_equals_colour_bitwise(require_gpu, name, scale):
+    """Both schemes add each entry's element contributions in colour order:
+    the results are identical bit for bit."""
+    cfg = M.config(name, seed=13, scale=scale)
+    pr = cg.problem_from_config(cfg)
+    out = {}
+    for scheme in ("colour", "blocked"):
+        print(f"\n[case] {name} scale {scale} scheme {scheme}", flush=True)
+        cg.assemble_advection_diffusion_cg(pr, cg.options_from_config(cfg, scheme=scheme), cfg.dt)
+        out[scheme] = pr.matrix()
+    assert np.array_equal(out["colour"][0], out["blocked"][0])
+    assert np.array_equal(out["colour"][1], out["blocked"][1])
     pr.close()
 
 
