@@ -148,17 +148,35 @@ def main() -> int:
     if not args.no_solve:
         cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
         solves = {}
+        from fluidity_amd import learned
+        learned.set_params(pr, learned.load())
+        xs = {}
         for label, so in (
             ("fp64_jacobi_pcg", cg.SolverOptions("cg", "jacobi", relative_error=1e-8, norm="unpreconditioned",
                                                  max_iterations=50000, check_every=32)),
             ("mixed_jacobi_pcg", cg.SolverOptions("cg", "jacobi", relative_error=1e-8, norm="unpreconditioned",
                                                   max_iterations=50000, precision="mixed", check_every=32)),
+            ("learned_fp64_fcg", cg.SolverOptions("fcg", "learned", relative_error=1e-8, norm="unpreconditioned",
+                                                  max_iterations=50000, check_every=32)),
+            ("learned_mixed_fcg", cg.SolverOptions("fcg", "learned", relative_error=1e-8, norm="unpreconditioned",
+                                                   max_iterations=50000, precision="mixed", check_every=32)),
         ):
+            cg.petsc_solve(pr, so)  # warm-up (allocations, first-touch)
+            t0s = time.perf_counter()
             x, info = cg.petsc_solve(pr, so)
-            solves[label] = {"seconds": round(info.seconds, 4), "its": info.its, "reason": info.reason,
-                             "true_relres": info.true_relres, "refinements": info.refinements}
-            log(f"solve {label}: {info.seconds:.3f}s, {info.its} its, reason {info.reason}, "
-                f"relres {info.true_relres:.2e}")
+            tot = time.perf_counter() - t0s
+            xs[label] = x
+            solves[label] = {"seconds": round(tot, 4), "krylov_seconds": round(info.seconds, 4), "its": info.its,
+                             "reason": info.reason, "true_relres": info.true_relres,
+                             "refinements": info.refinements, "fallback": info.fallback}
+            log(f"solve {label}: {tot:.4f}s total ({info.seconds:.4f}s Krylov), {info.its} its, "
+                f"reason {info.reason}, relres {info.true_relres:.2e}, fallback {info.fallback}")
+        ref = xs["fp64_jacobi_pcg"]
+        for label, x in xs.items():
+            solves[label]["x_rel_diff_vs_fp64_jacobi"] = float(np.max(np.abs(x - ref)) / np.max(np.abs(ref)))
+        solves["speedup_learned_mixed_vs_fp64_jacobi"] = round(
+            solves["fp64_jacobi_pcg"]["seconds"] / solves["learned_mixed_fcg"]["seconds"], 3)
+        log(f"speedup learned+mixed vs FP64 Jacobi-PCG: {solves['speedup_learned_mixed_vs_fp64_jacobi']}x")
         for prec, vb in (("fp64", 8), ("mixed", 4)):
             L.check(lib.flgpu_solver_prepare(pr.ctx, L.FLGPU_FP64 if prec == "fp64" else L.FLGPU_MIXED), "prep")
             L.check(lib.flgpu_time_spmv(pr.ctx, L.FLGPU_FP64 if prec == "fp64" else L.FLGPU_MIXED, 50,

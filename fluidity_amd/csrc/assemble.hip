@@ -125,12 +125,19 @@ static int launch_colour(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, 
 }
 
 // ---------------------------------------------------------------------
-// Blocked scheme (linear T, FP64): one workgroup per block of rows.  The
-// block's CSR rows live in LDS; the elements touching the block are
-// visited colour by colour (ascending id inside a colour) with a barrier
-// between colours, so every entry receives its contributions in the same
-// order as in the colour scheme and ends with the same bits.  Each row is
-// written to HBM once; the matrix needs no zeroing pass.
+// Blocked scheme (linear T, FP64).  Rows are grouped into small blocks
+// that are compact in space (Morton order of the node positions); one
+// workgroup assembles one block's CSR rows:
+//   phase A: every element touching the block computes the rows of its
+//     element matrix that fall in the block and stores each row in an LDS
+//     slot numbered (row, rank of the element among the row's elements in
+//     colour order) -- the ranks are fixed by the colouring and computed
+//     once per mesh;
+//   phase B: one thread per CSR entry adds the entry's contributions slot
+//     by slot, i.e. in colour order, ascending element id inside a colour
+//     -- the order of the reference's colour loop (:574-598) and of the
+//     colour scheme above, so the two schemes give the same bits.
+// Each row is written to HBM once and the matrix needs no zeroing pass.
 // ---------------------------------------------------------------------
 
 // Position of column cj in lcol[lo, hi) (sorted); -1 when absent.
@@ -146,109 +153,147 @@ __device__ __forceinline__ int lds_pos(const int32_t* lcol, int lo, int hi, int 
     return -1;
 }
 
+// Exclusive prefix sums of two per-row counts over the block's rows.
+// Each thread owns a contiguous chunk of rows; returns the thread's
+// starting offsets and leaves the totals in tot[0..1].
+__device__ __forceinline__ void block_scan2(int a, int b, int* wsa, int* wsb, int& oa, int& ob, int* tot) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+    int ia = a, ib = b;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int va = __shfl_up(ia, off, 64), vb = __shfl_up(ib, off, 64);
+        if (lane >= off) {
+            ia += va;
+            ib += vb;
+        }
+    }
+    if (lane == 63) {
+        wsa[wave] = ia;
+        wsb[wave] = ib;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int ra = 0, rb = 0;
+        for (int w = 0; w < nw; w++) {
+            const int ta = wsa[w], tb = wsb[w];
+            wsa[w] = ra;
+            wsb[w] = rb;
+            ra += ta;
+            rb += tb;
+        }
+        tot[0] = ra;
+        tot[1] = rb;
+    }
+    __syncthreads();
+    oa = wsa[wave] + ia - a;
+    ob = wsb[wave] + ib - b;
+}
+
 template <int DIM>
 __global__ void __launch_bounds__(256)
-k_asm_blocked(const ElemTables* __restrict__ tab, int ncol, int rows_cap, int nnz_cap,
-              const int32_t* __restrict__ brow, const int32_t* __restrict__ brows,
-              const int32_t* __restrict__ bseg, const int32_t* __restrict__ visits,
+k_asm_blocked(const ElemTables* __restrict__ tab, int nnz_cap, int slot_cap, const int32_t* __restrict__ brow,
+              const int32_t* __restrict__ brows, const int32_t* __restrict__ bvis, const int32_t* __restrict__ visits,
               const int32_t* __restrict__ node_block, const int32_t* __restrict__ node_lrow,
-              const int32_t* __restrict__ conn, const int32_t* __restrict__ conn_x,
-              const double* __restrict__ X4, AsmFields fd, AsmFlags fl,
-              const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+              const uint32_t* __restrict__ rank4, const int32_t* __restrict__ deg,
+              const int32_t* __restrict__ conn, const int32_t* __restrict__ conn_x, const double* __restrict__ X4,
+              AsmFields fd, AsmFlags fl, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
               double* __restrict__ A, double* __restrict__ rhs, int* __restrict__ err) {
     constexpr int LOC = DIM + 1;
     __shared__ LTab<double> tb;
-    __shared__ int wsum[256 / 64 + 1];
+    __shared__ int wsa[256 / 64], wsb[256 / 64], tot[2];
     extern __shared__ double dyn[];
-    double* acc = dyn;                                      // nnz_cap
-    double* lrhs = acc + nnz_cap;                           // rows_cap
-    int32_t* loff = reinterpret_cast<int32_t*>(lrhs + rows_cap);  // rows_cap + 1
-    int32_t* lcol = loff + rows_cap + 1;                    // nnz_cap
+    double* sval = dyn;                                            // slot_cap * LOC
+    double* srhs = sval + (size_t)slot_cap * LOC;                  // slot_cap
+    uint32_t* spos = reinterpret_cast<uint32_t*>(srhs + slot_cap); // slot_cap
+    int32_t* lcol = reinterpret_cast<int32_t*>(spos + slot_cap);   // nnz_cap
+    int32_t* loff = lcol + nnz_cap;                                // R + 1
     stage_tables<double>(tab, tb);
 
     const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     const int r0 = brow[b], R = brow[b + 1] - r0;
-    // row lengths -> exclusive prefix sum (each thread owns a chunk of rows)
+    int32_t* soff = loff + R + 1;                                  // R + 1
     const int per = (R + nt - 1) / nt;
     const int l0 = min(tid * per, R), l1 = min(l0 + per, R);
-    int mysum = 0;
+    int na = 0, ns = 0;
     for (int l = l0; l < l1; l++) {
         const int row = brows[r0 + l];
-        mysum += rowptr[row + 1] - rowptr[row];
+        na += rowptr[row + 1] - rowptr[row];
+        ns += deg[row];
     }
-    // block exclusive scan of mysum
-    const int lane = tid & 63, wave = tid >> 6;
-    int incl = mysum;
-    for (int off = 1; off < 64; off <<= 1) {
-        const int v = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += v;
-    }
-    if (lane == 63) wsum[wave] = incl;
-    __syncthreads();
-    if (tid == 0) {
-        int run = 0;
-        for (int w = 0; w < nt / 64; w++) {
-            const int t = wsum[w];
-            wsum[w] = run;
-            run += t;
-        }
-        wsum[nt / 64] = run;
-    }
-    __syncthreads();
-    int pos = wsum[wave] + incl - mysum;
+    int oa, os;
+    block_scan2(na, ns, wsa, wsb, oa, os, tot);
     for (int l = l0; l < l1; l++) {
         const int row = brows[r0 + l];
         const int s = rowptr[row], len = rowptr[row + 1] - s;
-        loff[l] = pos;
-        for (int k = 0; k < len; k++) {
-            lcol[pos + k] = col[s + k];
-            acc[pos + k] = 0.0;
-        }
-        lrhs[l] = 0.0;
-        pos += len;
+        loff[l] = oa;
+        soff[l] = os;
+        for (int k = 0; k < len; k++) lcol[oa + k] = col[s + k];
+        oa += len;
+        os += deg[row];
     }
-    if (tid == 0) loff[R] = wsum[nt / 64];
+    if (tid == 0) {
+        loff[R] = tot[0];
+        soff[R] = tot[1];
+    }
     __syncthreads();
 
-    const int32_t* seg = bseg + (int64_t)b * (ncol + 1);
-    for (int cc = 0; cc < ncol; cc++) {
-        const int s1 = seg[cc + 1];
-        for (int v = seg[cc] + tid; v < s1; v += nt) {
-            const int ele = visits[v];
-            Elem<DIM, LOC, double> e;
-            gather_elem<DIM, LOC, double>(ele, conn, conn_x, X4, fd.T, e);
-            GeomLin<DIM, LOC, double> gm;
-            geom_linear<DIM, LOC, double>(e, tb, gm);
+    // phase A: element rows into slots
+    for (int v = bvis[b] + tid; v < bvis[b + 1]; v += nt) {
+        const int ele = visits[v];
+        Elem<DIM, LOC, double> e;
+        gather_elem<DIM, LOC, double>(ele, conn, conn_x, X4, fd.T, e);
+        GeomLin<DIM, LOC, double> gm;
+        geom_linear<DIM, LOC, double>(e, tb, gm);
+        const uint32_t rk = rank4[ele];
 #pragma unroll
-            for (int i = 0; i < LOC; i++) {
-                const int node = e.nodes[i];
-                if (node_block[node] != b) continue;
-                const int l = node_lrow[node];
-                double vals[LOC], r;
-                row_linear_g<DIM, LOC, double>(e, gm, tb, fl, fd, i, vals, r);
-                const int lo = loff[l], hi = loff[l + 1];
+        for (int i = 0; i < LOC; i++) {
+            const int node = e.nodes[i];
+            if (node_block[node] != b) continue;
+            const int l = node_lrow[node];
+            const int slot = soff[l] + (int)((rk >> (8 * i)) & 255u);
+            double vals[LOC], r;
+            row_linear_g<DIM, LOC, double>(e, gm, tb, fl, fd, i, vals, r);
+            const int lo = loff[l], hi = loff[l + 1];
+            uint32_t packed = 0;
 #pragma unroll
-                for (int j = 0; j < LOC; j++) {
-                    if (vals[j] == 0.0) continue;  // csr_addto skips zeros
-                    const int q = lds_pos(lcol, lo, hi, e.nodes[j]);
-                    if (q < 0) {
-                        atomicOr(err, 1);
-                        continue;
-                    }
-                    acc[q] = acc[q] + vals[j];
+            for (int j = 0; j < LOC; j++) {
+                const int q = lds_pos(lcol, lo, hi, e.nodes[j]);
+                if (q < 0) atomicOr(err, 1);
+                packed |= (uint32_t)(q < 0 ? 255 : q - lo) << (8 * j);
+                sval[(size_t)slot * LOC + j] = vals[j];
+            }
+            spos[slot] = packed;
+            srhs[slot] = r;
+        }
+    }
+    __syncthreads();
+
+    // phase B: one thread per entry, contributions in slot (= colour) order
+    const int nent = loff[R];
+    for (int t = tid; t < nent; t += nt) {
+        int lo = 0, hi = R - 1;  // row of entry t: loff[l] <= t < loff[l+1]
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (loff[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const int l = lo, p = t - loff[l];
+        double acc = 0.0;
+        for (int k = soff[l]; k < soff[l + 1]; k++) {
+            const uint32_t pk = spos[k];
+#pragma unroll
+            for (int j = 0; j < LOC; j++) {
+                if ((int)((pk >> (8 * j)) & 255u) == p) {
+                    const double vv = sval[(size_t)k * LOC + j];
+                    if (vv != 0.0) acc = acc + vv;  // csr_addto skips zeros
                 }
-                lrhs[l] = lrhs[l] + r;
             }
         }
-        __syncthreads();
+        A[rowptr[brows[r0 + l]] + p] = acc;
     }
-    // write the block's rows and rhs entries
     for (int l = tid; l < R; l += nt) {
-        const int row = brows[r0 + l];
-        const int s = rowptr[row];
-        const int lo = loff[l], hi = loff[l + 1];
-        for (int k = lo; k < hi; k++) A[s + (k - lo)] = acc[k];
-        rhs[row] = lrhs[l];
+        double acc = 0.0;
+        for (int k = soff[l]; k < soff[l + 1]; k++) acc = acc + srhs[k];
+        rhs[brows[r0 + l]] = acc;
     }
 }
 
@@ -262,8 +307,8 @@ int build_block_plan(flgpu_ctx* c) {
     if (c->plan.built) return FLGPU_OK;
     if (!c->linear) return set_error(FLGPU_EUNSUPPORTED, "block plan: linear field mesh only");
     const int N = c->rows, dim = c->dim, loc = c->loc, ncol = c->ncol;
-    int rows_cap = 256;
-    if (const char* s = getenv("FLGPU_BLOCK_ROWS")) rows_cap = std::max(32, std::min(1024, atoi(s)));
+    int rows_cap = 64;
+    if (const char* s = getenv("FLGPU_BLOCK_ROWS")) rows_cap = std::max(16, std::min(1024, atoi(s)));
     // node positions of the field mesh (vertex i of T <-> vertex i of X)
     std::vector<float> pos((size_t)N * dim, 0.f);
     {
@@ -292,48 +337,64 @@ int build_block_plan(flgpu_ctx* c) {
         key[i] = ((uint64_t)code << 32) | (uint32_t)i;
     }
     std::sort(key.begin(), key.end());
-    // cut into blocks
+    // node -> element incidence in colour order (rank = position in it)
+    std::vector<int32_t> rank(c->nele);
+    for (int k = 0; k < ncol; k++)
+        for (int q = c->h_col_ptr[k]; q < c->h_col_ptr[k + 1]; q++) rank[c->h_col_elems[q]] = q;
+    std::vector<int64_t> iptr((size_t)N + 1, 0);
+    for (int64_t k = 0; k < (int64_t)c->nele * loc; k++) iptr[c->h_conn[k] + 1]++;
+    for (int i = 0; i < N; i++) iptr[i + 1] += iptr[i];
+    std::vector<int32_t> deg(N);
+    for (int i = 0; i < N; i++) {
+        const int64_t d = iptr[i + 1] - iptr[i];
+        if (d > 255) return set_error(FLGPU_EUNSUPPORTED, "block plan: node %d belongs to %lld elements (max 255)",
+                                      i + 1, (long long)d);
+        deg[i] = (int32_t)d;
+    }
+    std::vector<int32_t> inc(iptr[N]);
+    {
+        std::vector<int64_t> fill(iptr.begin(), iptr.end() - 1);
+        for (int q = 0; q < c->nele; q++) {  // colour order: elements visited by rank
+            const int e = c->h_col_elems[q];
+            for (int i = 0; i < loc; i++) inc[fill[c->h_conn[(int64_t)e * loc + i]]++] = e;
+        }
+    }
+    std::vector<uint32_t> rank4(c->nele, 0);
+    for (int n = 0; n < N; n++)
+        for (int64_t q = iptr[n]; q < iptr[n + 1]; q++) {
+            const int e = inc[q];
+            for (int i = 0; i < loc; i++)
+                if (c->h_conn[(int64_t)e * loc + i] == n) rank4[e] |= (uint32_t)(q - iptr[n]) << (8 * i);
+        }
+    // blocks of consecutive Morton rows, bounded by rows, entries and slots
     std::vector<int32_t> brow(1, 0), brows(N), node_block(N), node_lrow(N);
-    int cur = 0, cur_nnz = 0, max_nnz = 0, max_rows = 0;
-    const int nnz_cap = rows_cap * 20;
+    int cur = 0, cur_nnz = 0, cur_slots = 0, max_nnz = 0, max_slots = 0, max_rows = 0;
+    const int nnz_cap = rows_cap * 24, slot_cap = rows_cap * 32;
     for (int k = 0; k < N; k++) {
         const int r = (int)(key[k] & 0xffffffffu);
         const int len = c->h_rowptr[r + 1] - c->h_rowptr[r];
-        if (cur > 0 && (cur == rows_cap || cur_nnz + len > nnz_cap)) {
+        if (len > 255) return set_error(FLGPU_EUNSUPPORTED, "block plan: row %d has %d entries (max 255)", r + 1, len);
+        if (cur > 0 && (cur == rows_cap || cur_nnz + len > nnz_cap || cur_slots + deg[r] > slot_cap)) {
             brow.push_back(k);
             max_nnz = std::max(max_nnz, cur_nnz);
+            max_slots = std::max(max_slots, cur_slots);
             max_rows = std::max(max_rows, cur);
-            cur = 0;
-            cur_nnz = 0;
+            cur = cur_nnz = cur_slots = 0;
         }
         brows[k] = r;
         node_block[r] = (int)brow.size() - 1;
         node_lrow[r] = cur;
         cur++;
         cur_nnz += len;
+        cur_slots += deg[r];
     }
     brow.push_back(N);
     max_nnz = std::max(max_nnz, cur_nnz);
+    max_slots = std::max(max_slots, cur_slots);
     max_rows = std::max(max_rows, cur);
     const int nb = (int)brow.size() - 1;
-    // colour rank of every element
-    std::vector<int32_t> rank(c->nele), colour_of(c->nele);
-    for (int k = 0; k < ncol; k++)
-        for (int q = c->h_col_ptr[k]; q < c->h_col_ptr[k + 1]; q++) {
-            rank[c->h_col_elems[q]] = q;
-            colour_of[c->h_col_elems[q]] = k;
-        }
-    // node -> element incidence
-    std::vector<int64_t> iptr((size_t)N + 1, 0);
-    for (int64_t k = 0; k < (int64_t)c->nele * loc; k++) iptr[c->h_conn[k] + 1]++;
-    for (int i = 0; i < N; i++) iptr[i + 1] += iptr[i];
-    std::vector<int32_t> inc(iptr[N]);
-    {
-        std::vector<int64_t> fill(iptr.begin(), iptr.end() - 1);
-        for (int e = 0; e < c->nele; e++)
-            for (int i = 0; i < loc; i++) inc[fill[c->h_conn[(int64_t)e * loc + i]]++] = e;
-    }
-    std::vector<int32_t> stamp(c->nele, -1), seg((size_t)nb * (ncol + 1)), visits, tmp;
+    // visits: elements touching each block, ascending ids
+    std::vector<int32_t> stamp(c->nele, -1), bvis(1, 0), visits, tmp;
     visits.reserve((size_t)c->nele * 2);
     for (int b = 0; b < nb; b++) {
         tmp.clear();
@@ -343,32 +404,28 @@ int build_block_plan(flgpu_ctx* c) {
                 const int e = inc[q];
                 if (stamp[e] != b) {
                     stamp[e] = b;
-                    tmp.push_back(rank[e]);
+                    tmp.push_back(e);
                 }
             }
         }
         std::sort(tmp.begin(), tmp.end());
-        int32_t* sg = &seg[(size_t)b * (ncol + 1)];
-        int cc = 0;
-        sg[0] = (int32_t)visits.size();
-        for (int32_t rk : tmp) {
-            const int e = c->h_col_elems[rk];
-            while (cc < colour_of[e]) sg[++cc] = (int32_t)visits.size();
-            visits.push_back(e);
-        }
-        while (cc < ncol) sg[++cc] = (int32_t)visits.size();
+        visits.insert(visits.end(), tmp.begin(), tmp.end());
+        bvis.push_back((int32_t)visits.size());
     }
     BlockPlan& P = c->plan;
     FLGPU_TRY(P.brow.upload(brow.data(), brow.size(), c->stream));
     FLGPU_TRY(P.brows.upload(brows.data(), brows.size(), c->stream));
     FLGPU_TRY(P.node_block.upload(node_block.data(), node_block.size(), c->stream));
     FLGPU_TRY(P.node_lrow.upload(node_lrow.data(), node_lrow.size(), c->stream));
-    FLGPU_TRY(P.seg.upload(seg.data(), seg.size(), c->stream));
+    FLGPU_TRY(P.seg.upload(bvis.data(), bvis.size(), c->stream));
     FLGPU_TRY(P.visits.upload(visits.data(), visits.size(), c->stream));
+    FLGPU_TRY(P.rank4.upload(rank4.data(), rank4.size(), c->stream));
+    FLGPU_TRY(P.deg.upload(deg.data(), deg.size(), c->stream));
     FLGPU_HIP(hipStreamSynchronize(c->stream));
     P.nblocks = nb;
     P.rows_cap = max_rows;
     P.nnz_cap = max_nnz;
+    P.slot_cap = max_slots;
     P.nvisits = (int64_t)visits.size();
     P.built = true;
     return FLGPU_OK;
@@ -377,14 +434,16 @@ int build_block_plan(flgpu_ctx* c) {
 template <int DIM>
 static int launch_blocked(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
     const BlockPlan& P = c->plan;
-    const size_t lds = sizeof(double) * (P.nnz_cap + P.rows_cap) + sizeof(int32_t) * (P.rows_cap + 1 + P.nnz_cap);
+    constexpr int LOC = DIM + 1;
+    const size_t lds = sizeof(double) * (size_t)P.slot_cap * (LOC + 1) + sizeof(uint32_t) * P.slot_cap +
+                       sizeof(int32_t) * ((size_t)P.nnz_cap + 2 * (P.rows_cap + 1));
     if (lds + sizeof(LTab<double>) + 64 > 160 * 1024)
-        return set_error(FLGPU_EUNSUPPORTED, "blocked assembly: block needs %zu bytes of LDS", lds);
+        return set_error(FLGPU_EUNSUPPORTED, "blocked assembly: a block needs %zu bytes of LDS", lds);
     FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_blocked<DIM>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_asm_blocked<DIM>), dim3(P.nblocks), dim3(256), lds, c->stream, c->d_tab.p, c->ncol,
-                       P.rows_cap, P.nnz_cap, P.brow.p, P.brows.p, P.seg.p, P.visits.p, P.node_block.p,
-                       P.node_lrow.p, c->conn.p, c->conn_x.p, c->X4.p, fd, fl, c->rowptr.p, c->col.p, c->A.p,
+    hipLaunchKernelGGL((k_asm_blocked<DIM>), dim3(P.nblocks), dim3(256), lds, c->stream, c->d_tab.p, P.nnz_cap,
+                       P.slot_cap, P.brow.p, P.brows.p, P.seg.p, P.visits.p, P.node_block.p, P.node_lrow.p,
+                       P.rank4.p, P.deg.p, c->conn.p, c->conn_x.p, c->X4.p, fd, fl, c->rowptr.p, c->col.p, c->A.p,
                        c->rhs.p, d_err);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;

@@ -91,17 +91,19 @@ struct DevField {
     int32_t set = 0;
 };
 
-// Row-block plan of the LDS-blocked P1 assembly: rows grouped in Morton
-// order of their positions; per block the elements touching it, sorted by
-// (colour, element id).
+// Row-block plan of the blocked P1 assembly: rows grouped in Morton order
+// of their positions; per block the elements touching it; per element the
+// rank of the element in each of its rows' colour-ordered element lists.
 struct BlockPlan {
     DevBuf<int32_t> brow;        // nblocks+1 offsets into brows
     DevBuf<int32_t> brows;       // rows of each block
     DevBuf<int32_t> node_block;  // block of every row
     DevBuf<int32_t> node_lrow;   // position of every row inside its block
-    DevBuf<int32_t> seg;         // nblocks*(ncol+1) offsets into visits
-    DevBuf<int32_t> visits;      // element ids per block, colour-major
-    int32_t nblocks = 0, rows_cap = 0, nnz_cap = 0;
+    DevBuf<int32_t> seg;         // nblocks+1 offsets into visits
+    DevBuf<int32_t> visits;      // element ids per block, ascending
+    DevBuf<uint32_t> rank4;      // per element: 4 x 8-bit ranks (vertex i in byte i)
+    DevBuf<int32_t> deg;         // elements per node
+    int32_t nblocks = 0, rows_cap = 0, nnz_cap = 0, slot_cap = 0;
     int64_t nvisits = 0;
     bool built = false;
 };

@@ -1,0 +1,189 @@
+!! ISO_C_BINDING interface of the gfx950 CG advection-diffusion library
+!! (include/flgpu.h).  Fortran callers pass the arrays of Fluidity's own
+!! types straight through: mesh%ndglno, positions%val, element_type%n/%dn,
+!! quadrature%weight, csr_sparsity%findrm/%colm/%centrm, field%val.
+!!
+!! Cut points it replaces:
+!!   assemble/Advection_Diffusion_CG.F90:565-598  (element loop)
+!!     -> flgpu_advdif_assemble
+!!   assemble/Advection_Diffusion_CG.F90:1389      (petsc_solve)
+!!     -> flgpu_petsc_solve
+!! Errors: a nonzero return is turned into FLAbort by flgpu_check, as
+!! csr_addto does for an insertion outside the pattern
+!! (femtools/Sparse_Tools.F90:2645-2647).
+module flgpu
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int32_t), parameter, public :: FLGPU_OK = 0
+  integer(c_int32_t), parameter, public :: FLGPU_FP64 = 0, FLGPU_MIXED = 1
+  integer(c_int32_t), parameter, public :: FLGPU_ASM_AUTO = 0, FLGPU_ASM_COLOUR = 1, FLGPU_ASM_BLOCKED = 2
+  integer(c_int32_t), parameter, public :: FLGPU_KSP_CG = 0, FLGPU_KSP_GMRES = 1, FLGPU_KSP_FCG = 2
+  integer(c_int32_t), parameter, public :: FLGPU_PC_NONE = 0, FLGPU_PC_JACOBI = 1, FLGPU_PC_LEARNED = 2
+  integer(c_int32_t), parameter, public :: FLGPU_NORM_PRECONDITIONED = 0, FLGPU_NORM_UNPRECONDITIONED = 1
+  integer(c_int32_t), parameter, public :: FLGPU_FIELD_T = 0, FLGPU_FIELD_VELOCITY = 1, &
+       FLGPU_FIELD_DIFFUSIVITY = 2, FLGPU_FIELD_SOURCE = 3, FLGPU_FIELD_ABSORPTION = 4
+
+  !! flgpu_advdif_params: the options assemble_advection_diffusion_cg reads
+  !! (Advection_Diffusion_CG.F90:282-556); logicals as 0/1 integers.
+  type, bind(c), public :: flgpu_advdif_params
+     real(c_double) :: dt, theta, beta
+     integer(c_int32_t) :: have_mass, lump_mass
+     integer(c_int32_t) :: have_advection, integrate_by_parts
+     integer(c_int32_t) :: have_diffusivity, isotropic_diffusivity
+     integer(c_int32_t) :: have_source, have_absorption
+     integer(c_int32_t) :: precision, scheme
+  end type flgpu_advdif_params
+
+  !! flgpu_solver_params: the solver options of schemas/solvers.rnc
+  type, bind(c), public :: flgpu_solver_params
+     integer(c_int32_t) :: ksp, pc
+     real(c_double) :: rtol, atol
+     integer(c_int32_t) :: max_its, restart
+     integer(c_int32_t) :: precision, start_from_zero
+     integer(c_int32_t) :: check_every
+     real(c_double) :: stall_factor
+     integer(c_int32_t) :: stall_window, norm_type
+  end type flgpu_solver_params
+
+  public :: flgpu_init, flgpu_finalize, flgpu_mesh_bind, flgpu_field_mesh_bind, &
+       flgpu_sparsity_bind, flgpu_colouring_bind, flgpu_make_sparsity, flgpu_colour_mesh, &
+       flgpu_field_set, flgpu_advdif_assemble, flgpu_dirichlet_apply, flgpu_matrix_get, &
+       flgpu_petsc_solve, flgpu_check
+
+  interface
+     integer(c_int) function flgpu_init(ctx, device) bind(c, name="flgpu_init")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), intent(out) :: ctx
+       integer(c_int32_t), value :: device
+     end function flgpu_init
+
+     integer(c_int) function flgpu_finalize(ctx) bind(c, name="flgpu_finalize")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function flgpu_finalize
+
+     function flgpu_last_error_c() bind(c, name="flgpu_last_error")
+       import :: c_ptr
+       type(c_ptr) :: flgpu_last_error_c
+     end function flgpu_last_error_c
+
+     integer(c_int) function flgpu_mesh_bind(ctx, key_id, key_event, dim, loc_x, nele, nx, ndglno_x, X) &
+          bind(c, name="flgpu_mesh_bind")
+       import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), value :: key_id, key_event
+       integer(c_int32_t), value :: dim, loc_x, nele, nx
+       integer(c_int32_t), intent(in) :: ndglno_x(*)
+       real(c_double), intent(in) :: X(*)
+     end function flgpu_mesh_bind
+
+     integer(c_int) function flgpu_field_mesh_bind(ctx, loc, ngi, nnodes, ndglno, n, dn, w, x_dn) &
+          bind(c, name="flgpu_field_mesh_bind")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: loc, ngi, nnodes
+       integer(c_int32_t), intent(in) :: ndglno(*)
+       real(c_double), intent(in) :: n(*), dn(*), w(*), x_dn(*)
+     end function flgpu_field_mesh_bind
+
+     integer(c_int) function flgpu_sparsity_bind(ctx, rows, cols, findrm, colm, centrm) &
+          bind(c, name="flgpu_sparsity_bind")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: rows, cols
+       integer(c_int32_t), intent(in) :: findrm(*), colm(*), centrm(*)
+     end function flgpu_sparsity_bind
+
+     integer(c_int) function flgpu_colouring_bind(ctx, ncol, col_ptr, elems) bind(c, name="flgpu_colouring_bind")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: ncol
+       integer(c_int32_t), intent(in) :: col_ptr(*), elems(*)
+     end function flgpu_colouring_bind
+
+     integer(c_int) function flgpu_make_sparsity(nnodes, nele, loc, ndglno, findrm, colm, centrm, nnz) &
+          bind(c, name="flgpu_make_sparsity")
+       import :: c_int, c_int32_t, c_int64_t, c_ptr
+       integer(c_int32_t), value :: nnodes, nele, loc
+       integer(c_int32_t), intent(in) :: ndglno(*)
+       integer(c_int32_t), intent(out) :: findrm(*), centrm(*)
+       type(c_ptr), value :: colm
+       integer(c_int64_t), intent(out) :: nnz
+     end function flgpu_make_sparsity
+
+     integer(c_int) function flgpu_colour_mesh(nnodes, nele, loc, ndglno, colour, ncol) &
+          bind(c, name="flgpu_colour_mesh")
+       import :: c_int, c_int32_t
+       integer(c_int32_t), value :: nnodes, nele, loc
+       integer(c_int32_t), intent(in) :: ndglno(*)
+       integer(c_int32_t), intent(out) :: colour(*)
+       integer(c_int32_t), intent(out) :: ncol
+     end function flgpu_colour_mesh
+
+     integer(c_int) function flgpu_field_set(ctx, which, constant, values) bind(c, name="flgpu_field_set")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: which, constant
+       real(c_double), intent(in) :: values(*)
+     end function flgpu_field_set
+
+     integer(c_int) function flgpu_advdif_assemble(ctx, p, T, U, kappa, src, absorp, A_val, rhs) &
+          bind(c, name="flgpu_advdif_assemble")
+       import :: c_int, c_ptr, flgpu_advdif_params
+       type(c_ptr), value :: ctx
+       type(flgpu_advdif_params), intent(in) :: p
+       type(c_ptr), value :: T, U, kappa, src, absorp, A_val, rhs
+     end function flgpu_advdif_assemble
+
+     integer(c_int) function flgpu_dirichlet_apply(ctx, nbc, nodes, values, dt) bind(c, name="flgpu_dirichlet_apply")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nbc
+       integer(c_int32_t), intent(in) :: nodes(*)
+       real(c_double), intent(in) :: values(*)
+       real(c_double), value :: dt
+     end function flgpu_dirichlet_apply
+
+     integer(c_int) function flgpu_matrix_get(ctx, A_val, rhs, inactive) bind(c, name="flgpu_matrix_get")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, A_val, rhs, inactive
+     end function flgpu_matrix_get
+
+     integer(c_int) function flgpu_petsc_solve(ctx, sp, A_val, inactive, rhs, x, its, reason, true_relres) &
+          bind(c, name="flgpu_petsc_solve")
+       import :: c_int, c_ptr, c_int32_t, c_double, flgpu_solver_params
+       type(c_ptr), value :: ctx
+       type(flgpu_solver_params), intent(in) :: sp
+       type(c_ptr), value :: A_val, inactive, rhs
+       real(c_double), intent(inout) :: x(*)
+       integer(c_int32_t), intent(out) :: its, reason
+       real(c_double), intent(out) :: true_relres
+     end function flgpu_petsc_solve
+  end interface
+
+contains
+
+  !! FLAbort-style check: print the library's message and stop.
+  subroutine flgpu_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    type(c_ptr) :: msg
+    character(kind=c_char), pointer :: chars(:)
+    integer :: n
+    interface
+       integer(c_size_t) function c_strlen(s) bind(c, name="strlen")
+         import :: c_ptr, c_size_t
+         type(c_ptr), value :: s
+       end function c_strlen
+    end interface
+    if (rc == FLGPU_OK) return
+    msg = flgpu_last_error_c()
+    n = int(c_strlen(msg))
+    call c_f_pointer(msg, chars, [n])
+    write(0, '(a,a,i0,a,*(a))') trim(what), " failed (", rc, "): ", chars(1:n)
+    error stop 1
+  end subroutine flgpu_check
+
+end module flgpu

@@ -1,0 +1,40 @@
+"""The Fortran ISO_C_BINDING caller (fluidity_amd/fortran/flgpu_driver.F90)
+reaches the same device path as the Python host mirror: one
+solve_field_equation_cg step of the rotating bump, compared number for
+number."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from fluidity_amd import cg, meshgen as M
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "fluidity_amd", "fortran", "flgpu_driver")
+
+
+def test_fortran_driver_matches_python_path(require_gpu):
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built (no flang at build time)")
+    n = 16
+    print(f"\n[case] fortran driver n={n}", flush=True)
+    out = subprocess.run([DRIVER, str(n)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    f = out.stdout.split()
+    nn, nnz, ncol, its, reason = (int(v) for v in f[:5])
+    relres, tsum, tmax = (float(v) for v in f[5:8])
+    mesh = M.square_tris(n, jitter=0.0)
+    p = mesh.node_pos
+    T0 = np.exp(-((p[:, 0] - 0.3) ** 2 + (p[:, 1] - 0.5) ** 2) / (2 * 0.05 ** 2))
+    U = np.stack([-(p[:, 1] - 0.5), p[:, 0] - 0.5], axis=1)
+    cfg = M.Config("c1-nojitter", mesh, 2, T0, U, 1e-3, None, None, 0.01, 0.5, 0.0, True, True, "gmres", 1e-10)
+    pr = cg.problem_from_config(cfg)
+    so = cg.SolverOptions("gmres", "jacobi", relative_error=1e-10, restart=30)
+    T1, pits, info = cg.solve_field_equation_cg(pr, cg.options_from_config(cfg), so, cfg.dt, mesh.boundary, 0.0)
+    assert (nn, nnz, ncol) == (mesh.nnodes, pr.sparsity.nnz, pr.ncolours)
+    assert reason > 0 and its == pits
+    assert abs(tsum - T1.sum()) <= 1e-12 * abs(T1.sum())
+    assert abs(tmax - T1.max()) <= 1e-12 * abs(T1.max())
+    pr.close()
