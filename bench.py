@@ -86,8 +86,8 @@ def main() -> int:
     ap.add_argument("--scheme", default="auto")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (sharding the path "
+                                   "over GPUs, by element partition with a node halo, is not implemented)"}))
         return 2
 
     from fluidity_amd import cg, lib as L, meshgen as M

@@ -125,19 +125,23 @@ static int launch_colour(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, 
 }
 
 // ---------------------------------------------------------------------
-// Blocked scheme (linear T, FP64).  Rows are grouped into small blocks
-// that are compact in space (Morton order of the node positions); one
-// workgroup assembles one block's CSR rows:
-//   phase A: every element touching the block computes the rows of its
-//     element matrix that fall in the block and stores each row in an LDS
-//     slot numbered (row, rank of the element among the row's elements in
-//     colour order) -- the ranks are fixed by the colouring and computed
-//     once per mesh;
-//   phase B: one thread per CSR entry adds the entry's contributions slot
-//     by slot, i.e. in colour order, ascending element id inside a colour
-//     -- the order of the reference's colour loop (:574-598) and of the
-//     colour scheme above, so the two schemes give the same bits.
-// Each row is written to HBM once and the matrix needs no zeroing pass.
+// Blocked scheme (linear T, FP64).  The CSR rows are cut into slabs of
+// consecutive rows, small enough that every element contribution to the
+// slab fits in LDS; one workgroup assembles one slab.  A slot is a pair
+// (row, element containing the row's node); the slots of a row follow the
+// node's element list in colour order, ascending element id inside a
+// colour (fixed by the colouring, built once per mesh):
+//   phase A: one thread per slot computes the element's geometry and the
+//     row of its element matrix and rhs, and stores them in the slot;
+//   phase B: one thread per CSR entry of the slab adds the entry's
+//     contributions slot by slot, i.e. in colour order -- the order of the
+//     reference's colour loop (:574-598) and of the colour scheme above,
+//     so both schemes give the same bits.  An element contributes to an
+//     entry at most once, and an addend of zero leaves the sum unchanged,
+//     exactly as csr_addto's skip of zeros (Sparse_Tools.F90:2640) does.
+// The slab's entries and slots are contiguous in the CSR and incidence
+// arrays, so they are read and written once, coalesced, and the matrix
+// needs no zeroing pass.
 // ---------------------------------------------------------------------
 
 // Position of column cj in lcol[lo, hi) (sorted); -1 when absent.
@@ -153,300 +157,186 @@ __device__ __forceinline__ int lds_pos(const int32_t* lcol, int lo, int hi, int 
     return -1;
 }
 
-// Exclusive prefix sums of two per-row counts over the block's rows.
-// Each thread owns a contiguous chunk of rows; returns the thread's
-// starting offsets and leaves the totals in tot[0..1].
-__device__ __forceinline__ void block_scan2(int a, int b, int* wsa, int* wsb, int& oa, int& ob, int* tot) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
-    int ia = a, ib = b;
-    for (int off = 1; off < 64; off <<= 1) {
-        const int va = __shfl_up(ia, off, 64), vb = __shfl_up(ib, off, 64);
-        if (lane >= off) {
-            ia += va;
-            ib += vb;
-        }
-    }
-    if (lane == 63) {
-        wsa[wave] = ia;
-        wsb[wave] = ib;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int ra = 0, rb = 0;
-        for (int w = 0; w < nw; w++) {
-            const int ta = wsa[w], tb = wsb[w];
-            wsa[w] = ra;
-            wsb[w] = rb;
-            ra += ta;
-            rb += tb;
-        }
-        tot[0] = ra;
-        tot[1] = rb;
-    }
-    __syncthreads();
-    oa = wsa[wave] + ia - a;
-    ob = wsb[wave] + ib - b;
-}
-
-template <int DIM>
+template <int DIM, int FEAT>
 __global__ void __launch_bounds__(256)
-k_asm_blocked(const ElemTables* __restrict__ tab, int nnz_cap, int slot_cap, const int32_t* __restrict__ brow,
-              const int32_t* __restrict__ brows, const int32_t* __restrict__ bvis, const int32_t* __restrict__ visits,
-              const int32_t* __restrict__ node_block, const int32_t* __restrict__ node_lrow,
-              const uint32_t* __restrict__ rank4, const int32_t* __restrict__ deg,
+k_asm_blocked(const ElemTables* __restrict__ tab, int rows_cap, int nnz_cap, int slot_cap,
+              const int32_t* __restrict__ brow, const int32_t* __restrict__ iptr, const int32_t* __restrict__ inc,
               const int32_t* __restrict__ conn, const int32_t* __restrict__ conn_x, const double* __restrict__ X4,
               AsmFields fd, AsmFlags fl, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
               double* __restrict__ A, double* __restrict__ rhs, int* __restrict__ err) {
     constexpr int LOC = DIM + 1;
     __shared__ LTab<double> tb;
-    __shared__ int wsa[256 / 64], wsb[256 / 64], tot[2];
     extern __shared__ double dyn[];
     double* sval = dyn;                                            // slot_cap * LOC
     double* srhs = sval + (size_t)slot_cap * LOC;                  // slot_cap
     uint32_t* spos = reinterpret_cast<uint32_t*>(srhs + slot_cap); // slot_cap
     int32_t* lcol = reinterpret_cast<int32_t*>(spos + slot_cap);   // nnz_cap
-    int32_t* loff = lcol + nnz_cap;                                // R + 1
+    int32_t* lrp = lcol + nnz_cap;                                 // rows_cap + 1
+    int32_t* soff = lrp + rows_cap + 1;                            // rows_cap + 1
+    uint16_t* srow = reinterpret_cast<uint16_t*>(soff + rows_cap + 1);  // slot_cap: row of each slot
+    uint16_t* erow = srow + slot_cap;                                   // nnz_cap: row of each entry
     stage_tables<double>(tab, tb);
 
     const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
-    const int r0 = brow[b], R = brow[b + 1] - r0;
-    int32_t* soff = loff + R + 1;                                  // R + 1
-    const int per = (R + nt - 1) / nt;
-    const int l0 = min(tid * per, R), l1 = min(l0 + per, R);
-    int na = 0, ns = 0;
-    for (int l = l0; l < l1; l++) {
-        const int row = brows[r0 + l];
-        na += rowptr[row + 1] - rowptr[row];
-        ns += deg[row];
+    const int r0 = brow[b], r1 = brow[b + 1], R = r1 - r0;
+    const int k0 = rowptr[r0], E = rowptr[r1] - k0;
+    const int s0 = iptr[r0], S = iptr[r1] - s0;
+    for (int l = tid; l <= R; l += nt) {
+        lrp[l] = rowptr[r0 + l] - k0;
+        soff[l] = iptr[r0 + l] - s0;
     }
-    int oa, os;
-    block_scan2(na, ns, wsa, wsb, oa, os, tot);
-    for (int l = l0; l < l1; l++) {
-        const int row = brows[r0 + l];
-        const int s = rowptr[row], len = rowptr[row + 1] - s;
-        loff[l] = oa;
-        soff[l] = os;
-        for (int k = 0; k < len; k++) lcol[oa + k] = col[s + k];
-        oa += len;
-        os += deg[row];
-    }
-    if (tid == 0) {
-        loff[R] = tot[0];
-        soff[R] = tot[1];
+    for (int k = tid; k < E; k += nt) lcol[k] = col[k0 + k];
+    __syncthreads();
+    for (int l = tid; l < R; l += nt) {
+        for (int k = soff[l]; k < soff[l + 1]; k++) srow[k] = (uint16_t)l;
+        for (int k = lrp[l]; k < lrp[l + 1]; k++) erow[k] = (uint16_t)l;
     }
     __syncthreads();
 
-    // phase A: element rows into slots
-    for (int v = bvis[b] + tid; v < bvis[b + 1]; v += nt) {
-        const int ele = visits[v];
+    // phase A: one element row per slot
+    for (int sl = tid; sl < S; sl += nt) {
+        // the element tables are re-read from LDS in every iteration rather
+        // than held in ~50 registers across the loop
+        __asm__ __volatile__("" ::: "memory");
+        const int ele = inc[s0 + sl];
+        const int l = srow[sl], row = r0 + l;
         Elem<DIM, LOC, double> e;
         gather_elem<DIM, LOC, double>(ele, conn, conn_x, X4, fd.T, e);
+        int i = 0;
+#pragma unroll
+        for (int k = 1; k < LOC; k++)
+            if (e.nodes[k] == row) i = k;
         GeomLin<DIM, LOC, double> gm;
         geom_linear<DIM, LOC, double>(e, tb, gm);
-        const uint32_t rk = rank4[ele];
+        double vals[LOC], r;
+        row_linear_g<DIM, LOC, double, FEAT>(e, gm, tb, fl, fd, i, vals, r);
+        const int lo = lrp[l], hi = lrp[l + 1];
+        uint32_t packed = 0;
 #pragma unroll
-        for (int i = 0; i < LOC; i++) {
-            const int node = e.nodes[i];
-            if (node_block[node] != b) continue;
-            const int l = node_lrow[node];
-            const int slot = soff[l] + (int)((rk >> (8 * i)) & 255u);
-            double vals[LOC], r;
-            row_linear_g<DIM, LOC, double>(e, gm, tb, fl, fd, i, vals, r);
-            const int lo = loff[l], hi = loff[l + 1];
-            uint32_t packed = 0;
-#pragma unroll
-            for (int j = 0; j < LOC; j++) {
-                const int q = lds_pos(lcol, lo, hi, e.nodes[j]);
-                if (q < 0) atomicOr(err, 1);
-                packed |= (uint32_t)(q < 0 ? 255 : q - lo) << (8 * j);
-                sval[(size_t)slot * LOC + j] = vals[j];
-            }
-            spos[slot] = packed;
-            srhs[slot] = r;
+        for (int j = 0; j < LOC; j++) {
+            const int q = lds_pos(lcol, lo, hi, e.nodes[j]);
+            if (q < 0) atomicOr(err, 1);
+            packed |= (uint32_t)(q < 0 ? 255 : q - lo) << (8 * j);
+            sval[(size_t)sl * LOC + j] = vals[j];
         }
+        spos[sl] = packed;
+        srhs[sl] = r;
     }
     __syncthreads();
 
     // phase B: one thread per entry, contributions in slot (= colour) order
-    const int nent = loff[R];
-    for (int t = tid; t < nent; t += nt) {
-        int lo = 0, hi = R - 1;  // row of entry t: loff[l] <= t < loff[l+1]
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (loff[mid] <= t) lo = mid;
-            else hi = mid - 1;
-        }
-        const int l = lo, p = t - loff[l];
+    for (int t = tid; t < E; t += nt) {
+        const int l = erow[t];
+        const uint32_t p = (uint32_t)(t - lrp[l]);
         double acc = 0.0;
         for (int k = soff[l]; k < soff[l + 1]; k++) {
             const uint32_t pk = spos[k];
+            int j = -1;
 #pragma unroll
-            for (int j = 0; j < LOC; j++) {
-                if ((int)((pk >> (8 * j)) & 255u) == p) {
-                    const double vv = sval[(size_t)k * LOC + j];
-                    if (vv != 0.0) acc = acc + vv;  // csr_addto skips zeros
-                }
-            }
+            for (int jj = 0; jj < LOC; jj++)
+                if (((pk >> (8 * jj)) & 255u) == p) j = jj;
+            const double vv = sval[(size_t)k * LOC + (j < 0 ? 0 : j)];
+            acc = acc + (j < 0 ? 0.0 : vv);
         }
-        A[rowptr[brows[r0 + l]] + p] = acc;
+        A[k0 + t] = acc;
     }
     for (int l = tid; l < R; l += nt) {
         double acc = 0.0;
         for (int k = soff[l]; k < soff[l + 1]; k++) acc = acc + srhs[k];
-        rhs[brows[r0 + l]] = acc;
+        rhs[r0 + l] = acc;
     }
 }
 
-static uint32_t spread_bits(uint32_t v, int dim) {
-    uint32_t r = 0;
-    for (int k = 0; k < 10; k++) r |= ((v >> k) & 1u) << (k * dim);
-    return r;
+static size_t blocked_lds(int loc, int rows_cap, int nnz_cap, int slot_cap) {
+    return sizeof(double) * (size_t)slot_cap * (loc + 1) + sizeof(uint32_t) * (size_t)slot_cap +
+           sizeof(int32_t) * ((size_t)nnz_cap + 2 * (size_t)(rows_cap + 1)) +
+           sizeof(uint16_t) * ((size_t)slot_cap + nnz_cap);
 }
 
 int build_block_plan(flgpu_ctx* c) {
     if (c->plan.built) return FLGPU_OK;
     if (!c->linear) return set_error(FLGPU_EUNSUPPORTED, "block plan: linear field mesh only");
-    const int N = c->rows, dim = c->dim, loc = c->loc, ncol = c->ncol;
-    int rows_cap = 64;
-    if (const char* s = getenv("FLGPU_BLOCK_ROWS")) rows_cap = std::max(16, std::min(1024, atoi(s)));
-    // node positions of the field mesh (vertex i of T <-> vertex i of X)
-    std::vector<float> pos((size_t)N * dim, 0.f);
-    {
-        std::vector<double> X4h((size_t)c->nx * 4);
-        FLGPU_HIP(hipMemcpy(X4h.data(), c->X4.p, sizeof(double) * X4h.size(), hipMemcpyDeviceToHost));
-        for (int64_t e = 0; e < c->nele; e++)
-            for (int i = 0; i < loc; i++) {
-                const int nd = c->h_conn[e * loc + i], xn = c->h_conn_x[e * c->loc_x + i];
-                for (int d = 0; d < dim; d++) pos[(size_t)nd * dim + d] = (float)X4h[(size_t)xn * 4 + d];
-            }
-    }
-    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
-    for (int i = 0; i < N; i++)
-        for (int d = 0; d < dim; d++) {
-            lo[d] = std::min(lo[d], pos[(size_t)i * dim + d]);
-            hi[d] = std::max(hi[d], pos[(size_t)i * dim + d]);
-        }
-    std::vector<uint64_t> key(N);
-    for (int i = 0; i < N; i++) {
-        uint32_t code = 0;
-        for (int d = 0; d < dim; d++) {
-            const float span = hi[d] > lo[d] ? hi[d] - lo[d] : 1.f;
-            uint32_t q = (uint32_t)std::min(1023.f, std::max(0.f, (pos[(size_t)i * dim + d] - lo[d]) / span * 1023.f));
-            code |= spread_bits(q, dim) << d;
-        }
-        key[i] = ((uint64_t)code << 32) | (uint32_t)i;
-    }
-    std::sort(key.begin(), key.end());
-    // node -> element incidence in colour order (rank = position in it)
-    std::vector<int32_t> rank(c->nele);
-    for (int k = 0; k < ncol; k++)
-        for (int q = c->h_col_ptr[k]; q < c->h_col_ptr[k + 1]; q++) rank[c->h_col_elems[q]] = q;
-    std::vector<int64_t> iptr((size_t)N + 1, 0);
+    const int N = c->rows, loc = c->loc;
+    // LDS budget per slab: four workgroups per CU
+    int slot_cap = 640;
+    if (const char* s = getenv("FLGPU_BLOCK_SLOTS")) slot_cap = std::max(64, std::min(4096, atoi(s)));
+    const int rows_cap = 256, nnz_cap = 1024;
+    // node -> element incidence in colour order
+    std::vector<int32_t> iptr((size_t)N + 1, 0);
     for (int64_t k = 0; k < (int64_t)c->nele * loc; k++) iptr[c->h_conn[k] + 1]++;
-    for (int i = 0; i < N; i++) iptr[i + 1] += iptr[i];
-    std::vector<int32_t> deg(N);
     for (int i = 0; i < N; i++) {
-        const int64_t d = iptr[i + 1] - iptr[i];
-        if (d > 255) return set_error(FLGPU_EUNSUPPORTED, "block plan: node %d belongs to %lld elements (max 255)",
-                                      i + 1, (long long)d);
-        deg[i] = (int32_t)d;
+        const int d = iptr[i + 1];
+        if (d > slot_cap)
+            return set_error(FLGPU_EUNSUPPORTED, "block plan: node %d belongs to %d elements (a slab holds %d)",
+                             i + 1, d, slot_cap);
+        const int len = c->h_rowptr[i + 1] - c->h_rowptr[i];
+        if (len > 255)
+            return set_error(FLGPU_EUNSUPPORTED, "block plan: row %d has %d entries (max 255)", i + 1, len);
+        if ((int64_t)iptr[i] + d > INT32_MAX) return set_error(FLGPU_EUNSUPPORTED, "block plan: too many slots");
+        iptr[i + 1] += iptr[i];
     }
     std::vector<int32_t> inc(iptr[N]);
     {
-        std::vector<int64_t> fill(iptr.begin(), iptr.end() - 1);
-        for (int q = 0; q < c->nele; q++) {  // colour order: elements visited by rank
+        std::vector<int32_t> fill(iptr.begin(), iptr.end() - 1);
+        for (int q = 0; q < c->nele; q++) {  // colour sets in order, ascending ids inside each
             const int e = c->h_col_elems[q];
             for (int i = 0; i < loc; i++) inc[fill[c->h_conn[(int64_t)e * loc + i]]++] = e;
         }
     }
-    std::vector<uint32_t> rank4(c->nele, 0);
-    for (int n = 0; n < N; n++)
-        for (int64_t q = iptr[n]; q < iptr[n + 1]; q++) {
-            const int e = inc[q];
-            for (int i = 0; i < loc; i++)
-                if (c->h_conn[(int64_t)e * loc + i] == n) rank4[e] |= (uint32_t)(q - iptr[n]) << (8 * i);
-        }
-    // blocks of consecutive Morton rows, bounded by rows, entries and slots
-    std::vector<int32_t> brow(1, 0), brows(N), node_block(N), node_lrow(N);
-    int cur = 0, cur_nnz = 0, cur_slots = 0, max_nnz = 0, max_slots = 0, max_rows = 0;
-    const int nnz_cap = rows_cap * 24, slot_cap = rows_cap * 32;
-    for (int k = 0; k < N; k++) {
-        const int r = (int)(key[k] & 0xffffffffu);
-        const int len = c->h_rowptr[r + 1] - c->h_rowptr[r];
-        if (len > 255) return set_error(FLGPU_EUNSUPPORTED, "block plan: row %d has %d entries (max 255)", r + 1, len);
-        if (cur > 0 && (cur == rows_cap || cur_nnz + len > nnz_cap || cur_slots + deg[r] > slot_cap)) {
-            brow.push_back(k);
-            max_nnz = std::max(max_nnz, cur_nnz);
-            max_slots = std::max(max_slots, cur_slots);
-            max_rows = std::max(max_rows, cur);
-            cur = cur_nnz = cur_slots = 0;
-        }
-        brows[k] = r;
-        node_block[r] = (int)brow.size() - 1;
-        node_lrow[r] = cur;
-        cur++;
-        cur_nnz += len;
-        cur_slots += deg[r];
-    }
-    brow.push_back(N);
-    max_nnz = std::max(max_nnz, cur_nnz);
-    max_slots = std::max(max_slots, cur_slots);
-    max_rows = std::max(max_rows, cur);
-    const int nb = (int)brow.size() - 1;
-    // visits: elements touching each block, ascending ids
-    std::vector<int32_t> stamp(c->nele, -1), bvis(1, 0), visits, tmp;
-    visits.reserve((size_t)c->nele * 2);
-    for (int b = 0; b < nb; b++) {
-        tmp.clear();
-        for (int k = brow[b]; k < brow[b + 1]; k++) {
-            const int r = brows[k];
-            for (int64_t q = iptr[r]; q < iptr[r + 1]; q++) {
-                const int e = inc[q];
-                if (stamp[e] != b) {
-                    stamp[e] = b;
-                    tmp.push_back(e);
-                }
-            }
-        }
-        std::sort(tmp.begin(), tmp.end());
-        visits.insert(visits.end(), tmp.begin(), tmp.end());
-        bvis.push_back((int32_t)visits.size());
+    // slabs of consecutive rows bounded by rows, entries and slots
+    std::vector<int32_t> brow(1, 0);
+    int max_rows = 0, max_nnz = 0, max_slots = 0;
+    for (int r = 0; r < N;) {
+        int r1 = r;
+        while (r1 < N && r1 - r < rows_cap && c->h_rowptr[r1 + 1] - c->h_rowptr[r] <= nnz_cap &&
+               iptr[r1 + 1] - iptr[r] <= slot_cap)
+            r1++;
+        max_rows = std::max(max_rows, r1 - r);
+        max_nnz = std::max(max_nnz, c->h_rowptr[r1] - c->h_rowptr[r]);
+        max_slots = std::max(max_slots, iptr[r1] - iptr[r]);
+        brow.push_back(r1);
+        r = r1;
     }
     BlockPlan& P = c->plan;
     FLGPU_TRY(P.brow.upload(brow.data(), brow.size(), c->stream));
-    FLGPU_TRY(P.brows.upload(brows.data(), brows.size(), c->stream));
-    FLGPU_TRY(P.node_block.upload(node_block.data(), node_block.size(), c->stream));
-    FLGPU_TRY(P.node_lrow.upload(node_lrow.data(), node_lrow.size(), c->stream));
-    FLGPU_TRY(P.seg.upload(bvis.data(), bvis.size(), c->stream));
-    FLGPU_TRY(P.visits.upload(visits.data(), visits.size(), c->stream));
-    FLGPU_TRY(P.rank4.upload(rank4.data(), rank4.size(), c->stream));
-    FLGPU_TRY(P.deg.upload(deg.data(), deg.size(), c->stream));
+    FLGPU_TRY(P.iptr.upload(iptr.data(), iptr.size(), c->stream));
+    FLGPU_TRY(P.inc.upload(inc.data(), inc.size(), c->stream));
     FLGPU_HIP(hipStreamSynchronize(c->stream));
-    P.nblocks = nb;
+    P.nblocks = (int)brow.size() - 1;
     P.rows_cap = max_rows;
     P.nnz_cap = max_nnz;
     P.slot_cap = max_slots;
-    P.nvisits = (int64_t)visits.size();
     P.built = true;
     return FLGPU_OK;
 }
 
-template <int DIM>
-static int launch_blocked(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
+template <int DIM, int FEAT>
+static int launch_blocked_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
     const BlockPlan& P = c->plan;
     constexpr int LOC = DIM + 1;
-    const size_t lds = sizeof(double) * (size_t)P.slot_cap * (LOC + 1) + sizeof(uint32_t) * P.slot_cap +
-                       sizeof(int32_t) * ((size_t)P.nnz_cap + 2 * (P.rows_cap + 1));
+    const size_t lds = blocked_lds(LOC, P.rows_cap, P.nnz_cap, P.slot_cap);
     if (lds + sizeof(LTab<double>) + 64 > 160 * 1024)
-        return set_error(FLGPU_EUNSUPPORTED, "blocked assembly: a block needs %zu bytes of LDS", lds);
-    FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_blocked<DIM>),
+        return set_error(FLGPU_EUNSUPPORTED, "blocked assembly: a slab needs %zu bytes of LDS", lds);
+    FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_blocked<DIM, FEAT>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_asm_blocked<DIM>), dim3(P.nblocks), dim3(256), lds, c->stream, c->d_tab.p, P.nnz_cap,
-                       P.slot_cap, P.brow.p, P.brows.p, P.seg.p, P.visits.p, P.node_block.p, P.node_lrow.p,
-                       P.rank4.p, P.deg.p, c->conn.p, c->conn_x.p, c->X4.p, fd, fl, c->rowptr.p, c->col.p, c->A.p,
-                       c->rhs.p, d_err);
+    hipLaunchKernelGGL((k_asm_blocked<DIM, FEAT>), dim3(P.nblocks), dim3(256), lds, c->stream, c->d_tab.p,
+                       P.rows_cap, P.nnz_cap, P.slot_cap, P.brow.p, P.iptr.p, P.inc.p, c->conn.p, c->conn_x.p,
+                       c->X4.p, fd, fl, c->rowptr.p, c->col.p, c->A.p, c->rhs.p, d_err);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
+}
+
+// The smallest compiled feature set that holds the terms switched on.
+template <int DIM>
+static int launch_blocked(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
+    const int m = (fl.have_mass ? F_MASS : 0) | (fl.adv ? F_ADV : 0) | (fl.absorp ? F_ABS : 0) |
+                  (fl.diff ? F_DIFF : 0) | (fl.src ? F_SRC : 0);
+    auto fits = [m](int f) { return (m & ~f) == 0; };
+    if (fits(F_DIFF | F_SRC)) return launch_blocked_feat<DIM, F_DIFF | F_SRC>(c, fd, fl, d_err);
+    if (fits(F_MASS | F_ADV | F_DIFF)) return launch_blocked_feat<DIM, F_MASS | F_ADV | F_DIFF>(c, fd, fl, d_err);
+    if (fits(F_MASS | F_ADV | F_DIFF | F_SRC))
+        return launch_blocked_feat<DIM, F_MASS | F_ADV | F_DIFF | F_SRC>(c, fd, fl, d_err);
+    return launch_blocked_feat<DIM, F_ALL>(c, fd, fl, d_err);
 }
 
 static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_err) {

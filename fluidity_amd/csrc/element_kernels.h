@@ -46,6 +46,13 @@ struct AsmFields {
     const double* B; int b_c;
 };
 
+// Compile-time feature sets: a kernel instantiated for a subset of terms
+// drops the code (and registers) of the others.  A term is computed when
+// its bit is set here and its flag is set at run time; the arithmetic of a
+// term does not depend on the set, so every instantiation gives the same
+// bits for the same flags.
+enum : int { F_MASS = 1, F_ADV = 2, F_ABS = 4, F_DIFF = 8, F_SRC = 16, F_ALL = 31 };
+
 // Tables in LDS, converted to the arithmetic type R.
 template <typename R>
 struct LTab {
@@ -135,12 +142,28 @@ FLGPU_HD_INLINE void geom_linear(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, 
 
 // Row i of the element matrix (vals[j], j = 0..LOC-1) and of the element
 // rhs, for linear T (gradients constant over the element).
-template <int DIM, int LOC, typename R>
-FLGPU_HD void row_linear_g(const Elem<DIM, LOC, R>& e, const GeomLin<DIM, LOC, R>& gm, const LTab<R>& tb,
-                           const AsmFlags& fl, const AsmFields& fd, int i, R (&vals)[LOC], R& rhs_i) {
+template <int DIM, int LOC, typename R, int FEAT = F_ALL>
+FLGPU_HD_INLINE void row_linear_g(const Elem<DIM, LOC, R>& e, const GeomLin<DIM, LOC, R>& gm, const LTab<R>& tb,
+                                  const AsmFlags& fl_in, const AsmFields& fd, int i, R (&vals)[LOC], R& rhs_i) {
+    AsmFlags fl = fl_in;
+    fl.have_mass = (FEAT & F_MASS) ? fl.have_mass : 0;
+    fl.adv = (FEAT & F_ADV) ? fl.adv : 0;
+    fl.absorp = (FEAT & F_ABS) ? fl.absorp : 0;
+    fl.diff = (FEAT & F_DIFF) ? fl.diff : 0;
+    fl.src = (FEAT & F_SRC) ? fl.src : 0;
     const R(&G)[LOC][DIM] = gm.G;
     const R adet = gm.adet;
     const R dtt = (R)fl.dt_theta;
+    // gradient of N_i picked by selects: i may be a run-time value, and an
+    // indexed register array would be moved to scratch memory
+    R Gi[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        Gi[d] = G[0][d];
+#pragma unroll
+        for (int k = 1; k < LOC; k++)
+            if (i == k) Gi[d] = G[k][d];
+    }
     R M[LOC], Aa[LOC], Bb[LOC], Kk[LOC];
 #pragma unroll
     for (int j = 0; j < LOC; j++) {
@@ -195,7 +218,7 @@ FLGPU_HD void row_linear_g(const Elem<DIM, LOC, R>& e, const GeomLin<DIM, LOC, R
                     R t = (R)0;
 #pragma unroll
                     for (int k = 0; k < LOC; k++) t = FMA(tb.mhat[k + LOC * j], u[k][d], t);
-                    s = FMA(G[i][d], t, s);
+                    s = FMA(Gi[d], t, s);
                 }
                 R a = -(adet * s);
                 if (fl.need_div) a = FMA(-((R)1 - (R)fl.beta) * div, adet * tb.mhat[i + LOC * j], a);
@@ -224,9 +247,9 @@ FLGPU_HD void row_linear_g(const Elem<DIM, LOC, R>& e, const GeomLin<DIM, LOC, R
             const R scale = adet * kw;
 #pragma unroll
             for (int j = 0; j < LOC; j++) {
-                R s = G[i][0] * G[j][0];
+                R s = Gi[0] * G[j][0];
 #pragma unroll
-                for (int d = 1; d < DIM; d++) s = FMA(G[i][d], G[j][d], s);
+                for (int d = 1; d < DIM; d++) s = FMA(Gi[d], G[j][d], s);
                 Kk[j] = s * scale;
             }
         } else {
@@ -246,7 +269,7 @@ FLGPU_HD void row_linear_g(const Elem<DIM, LOC, R>& e, const GeomLin<DIM, LOC, R
             for (int b = 0; b < DIM; b++) {
                 R s = (R)0;
 #pragma unroll
-                for (int a = 0; a < DIM; a++) s = FMA(G[i][a], kb[a][b], s);
+                for (int a = 0; a < DIM; a++) s = FMA(Gi[a], kb[a][b], s);
                 v[b] = s;
             }
 #pragma unroll

@@ -60,6 +60,11 @@ struct DevBuf {
         n = count;
         return FLGPU_OK;
     }
+    // grow only: keeps the buffer when it is already large enough
+    int reserve(size_t count) {
+        if (p && count <= n) return FLGPU_OK;
+        return alloc(count);
+    }
     int upload(const T* host, size_t count, hipStream_t s) {
         FLGPU_TRY(alloc(count));
         if (count)
@@ -95,16 +100,10 @@ struct DevField {
 // of their positions; per block the elements touching it; per element the
 // rank of the element in each of its rows' colour-ordered element lists.
 struct BlockPlan {
-    DevBuf<int32_t> brow;        // nblocks+1 offsets into brows
-    DevBuf<int32_t> brows;       // rows of each block
-    DevBuf<int32_t> node_block;  // block of every row
-    DevBuf<int32_t> node_lrow;   // position of every row inside its block
-    DevBuf<int32_t> seg;         // nblocks+1 offsets into visits
-    DevBuf<int32_t> visits;      // element ids per block, ascending
-    DevBuf<uint32_t> rank4;      // per element: 4 x 8-bit ranks (vertex i in byte i)
-    DevBuf<int32_t> deg;         // elements per node
+    DevBuf<int32_t> brow;        // nblocks+1 row boundaries: block b owns rows [brow[b], brow[b+1])
+    DevBuf<int32_t> iptr;        // nodes+1: start of every node's incidence list in inc
+    DevBuf<int32_t> inc;         // elements of every node, in colour order (ascending id in a colour)
     int32_t nblocks = 0, rows_cap = 0, nnz_cap = 0, slot_cap = 0;
-    int64_t nvisits = 0;
     bool built = false;
 };
 

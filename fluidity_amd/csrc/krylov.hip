@@ -36,10 +36,15 @@ struct KrylovWork {
     DevBuf<double> val64, b, x, r, z, p, w, dinv, y;
     DevBuf<float> val32, r32, z32, p32, w32, dinv32, y32;
     DevBuf<double> partials;
-    DevBuf<unsigned> counter;
-    DevBuf<Scal> scal;
+    DevBuf<Scal> scal;       // two slots, see carry_if_done
+    int cur = 0;             // slot holding the current state
+    DevBuf<int32_t> rb;      // row blocks of the streamed CSR product
+    int nrb = 0;
+    int64_t rb_key = -1;     // sparsity generation rb was built for
     Scal* h_scal = nullptr;  // pinned
     int nblk = 1;
+    Scal* S() { return scal.p + cur; }
+    double* part(int region) { return partials.p + (size_t)region * kPartStride; }
     // GMRES
     DevBuf<double> V, H, gv, cs, sn, hcol;
     DevBuf<float> V32;
@@ -55,9 +60,10 @@ struct KrylovWork {
 // Reduced system
 // ---------------------------------------------------------------------
 
+// Per-block min and max of |A_ii| (min/max do not depend on the order).
 __global__ void __launch_bounds__(kVecBlock)
 k_diag_minmax(int n, const int32_t* __restrict__ diag, const double* __restrict__ A,
-              double* __restrict__ partials, unsigned* counter, Scal* sc) {
+              double* __restrict__ partials) {
     double mn = 1.79769313486231570e308, mx = -1.79769313486231570e308;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int d = diag[i];
@@ -65,9 +71,7 @@ k_diag_minmax(int n, const int32_t* __restrict__ diag, const double* __restrict_
         mn = fmin(mn, v);
         mx = fmax(mx, v);
     }
-    // min/max are order independent; reuse the sum machinery on -min.
     __shared__ double sh[kVecBlock / 64][2];
-    __shared__ unsigned ticket;
     for (int off = 32; off > 0; off >>= 1) {
         mn = fmin(mn, __shfl_xor(mn, off, 64));
         mx = fmax(mx, __shfl_xor(mx, off, 64));
@@ -83,40 +87,27 @@ k_diag_minmax(int n, const int32_t* __restrict__ diag, const double* __restrict_
             sh[0][0] = fmin(sh[0][0], sh[q][0]);
             sh[0][1] = fmax(sh[0][1], sh[q][1]);
         }
-        partials[2 * blockIdx.x] = sh[0][0];
-        partials[2 * blockIdx.x + 1] = sh[0][1];
-        __threadfence();
-        ticket = atomicAdd(counter, 1u);
+        partials[blockIdx.x] = sh[0][0];
+        partials[kPartStride + blockIdx.x] = sh[0][1];
     }
-    __syncthreads();
-    if (ticket != gridDim.x - 1) return;
-    __threadfence();
+}
+
+// One wave: ghost pivot sqrt((max+min)*max/2) from the per-block extrema.
+__global__ void __launch_bounds__(64)
+k_pivot(int nparts, const double* __restrict__ partials, Scal* sc) {
     double a = 1.79769313486231570e308, b = -1.79769313486231570e308;
-    for (int q = threadIdx.x; q < (int)gridDim.x; q += blockDim.x) {
-        a = fmin(a, partials[2 * q]);
-        b = fmax(b, partials[2 * q + 1]);
+    for (int q = threadIdx.x; q < nparts; q += 64) {
+        a = fmin(a, partials[q]);
+        b = fmax(b, partials[kPartStride + q]);
     }
     for (int off = 32; off > 0; off >>= 1) {
         a = fmin(a, __shfl_xor(a, off, 64));
         b = fmax(b, __shfl_xor(b, off, 64));
     }
-    __syncthreads();
-    if (lane == 0) {
-        sh[wave][0] = a;
-        sh[wave][1] = b;
-    }
-    __syncthreads();
     if (threadIdx.x == 0) {
-        for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
-            sh[0][0] = fmin(sh[0][0], sh[q][0]);
-            sh[0][1] = fmax(sh[0][1], sh[q][1]);
-        }
-        a = sh[0][0];
-        b = sh[0][1];
         sc->dmin = a;
         sc->dmax = b;
         sc->pivot = sqrt((b + a) * b / 2.0);
-        *counter = 0u;
     }
 }
 
@@ -164,52 +155,6 @@ __device__ __forceinline__ T lane_sum(T v, int L) {
     return v;
 }
 
-// y = A x  (optionally y = b - A x);  partial sums: dot(x_own, y), dot(y,y)
-template <int L, typename V, typename XV, typename YV>
-__global__ void __launch_bounds__(kVecBlock)
-k_spmv(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-       const V* __restrict__ val, const XV* __restrict__ x, YV* __restrict__ y,
-       const double* __restrict__ bsub, int want_dot, double* __restrict__ partials,
-       unsigned* counter, Scal* sc, int mode, const int* __restrict__ done_flag) {
-    if (done_flag && *done_flag) return;
-    constexpr int ROWS = kVecBlock / L;
-    const int lane = threadIdx.x % L, rloc = threadIdx.x / L;
-    double d0 = 0.0, d1 = 0.0;
-    for (int grp = blockIdx.x; grp * ROWS < n; grp += gridDim.x) {
-        const int row = grp * ROWS + rloc;
-        V acc = (V)0;
-        if (row < n) {
-            const int e = rowptr[row + 1];
-            for (int k = rowptr[row] + lane; k < e; k += L) acc += val[k] * (V)x[col[k]];
-        }
-        acc = lane_sum<V>(acc, L);
-        if (row < n && lane == 0) {
-            double yv = (double)acc;
-            if (bsub) yv = bsub[row] - yv;
-            y[row] = (YV)yv;
-            if (want_dot) {
-                d0 += (double)x[row] * yv;
-                d1 += yv * yv;
-            }
-        }
-    }
-    if (!want_dot) return;
-    double v[2] = {d0, d1}, out[2];
-    if (grid_sum<2>(v, partials, counter, out) && threadIdx.x == 0) {
-        if (mode == 0) {  // CG: w = A p, pAp
-            sc->pAp = out[0];
-            if (!(out[0] > 0.0)) {
-                sc->reason = (out[0] != out[0]) ? KSP_DIVERGED_NANORINF : KSP_DIVERGED_INDEFINITE_MAT;
-                sc->done = 1;
-            } else {
-                sc->alpha = sc->rz / out[0];
-            }
-        } else if (mode == 1) {  // residual: y = b - A x
-            sc->true_rr = out[1];
-        }
-    }
-}
-
 // ---------------------------------------------------------------------
 // PCG / FCG vector kernels
 // ---------------------------------------------------------------------
@@ -246,15 +191,46 @@ __device__ void cg_test_and_beta(Scal* sc) {
     }
 }
 
-// x += alpha p; r -= alpha w; [Jacobi: z = D^-1 r];
-// partials: r.z, r.r, z.z, z.w (for the flexible beta)
+// The solver scalars advance through two slots: a kernel that changes
+// them reads slot "in" in every block and block 0 writes the next state
+// to slot "out", so no block can see a value written during its own
+// kernel.  A finished solve carries the state over unchanged.
+__device__ __forceinline__ bool carry_if_done(const Scal* __restrict__ in, Scal* __restrict__ out) {
+    if (!in->done) return false;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && out != in) *out = *in;
+    return true;
+}
+
+// pAp from the SpMV partials, alpha = rz / pAp; x += alpha p;
+// r -= alpha w; [Jacobi: z = D^-1 r]; partials r.z, r.r, z.z, z.w
 template <typename V>
 __global__ void __launch_bounds__(kVecBlock)
 k_cg_update(int n, double* __restrict__ x, const V* __restrict__ p, V* __restrict__ r,
             const V* __restrict__ w, V* __restrict__ z, const V* __restrict__ dinv, int jacobi,
-            double* __restrict__ partials, unsigned* counter, Scal* sc) {
-    if (sc->done) return;
-    const double alpha = sc->alpha;
+            const double* __restrict__ pspmv, int nspmv, double* __restrict__ pvec,
+            const Scal* __restrict__ in, Scal* __restrict__ out) {
+    __shared__ double s_alpha;
+    __shared__ int s_stop;
+    if (carry_if_done(in, out)) return;
+    double pap[1];
+    sum_partials<1>(pspmv, nspmv, pap);
+    if (threadIdx.x == 0) {
+        Scal s = *in;
+        s.pAp = pap[0];
+        s.rz_old = s.rz;
+        if (!(pap[0] > 0.0)) {
+            s.reason = (pap[0] != pap[0]) ? KSP_DIVERGED_NANORINF : KSP_DIVERGED_INDEFINITE_MAT;
+            s.done = 1;
+        } else {
+            s.alpha = s.rz / pap[0];
+        }
+        s_alpha = s.alpha;
+        s_stop = s.done;
+        if (blockIdx.x == 0) *out = s;
+    }
+    __syncthreads();
+    if (s_stop) return;
+    const double alpha = s_alpha;
     const V a = (V)alpha;
     double s_rz = 0.0, s_rr = 0.0, s_zz = 0.0, s_zw = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -271,40 +247,57 @@ k_cg_update(int n, double* __restrict__ x, const V* __restrict__ p, V* __restric
         }
         s_rr += (double)ri * (double)ri;
     }
-    double v[4] = {s_rz, s_rr, s_zz, s_zw}, out[4];
-    if (grid_sum<4>(v, partials, counter, out) && threadIdx.x == 0) {
-        sc->rr = out[1];
-        if (jacobi) {
-            sc->rz_old = sc->rz;
-            sc->rz = out[0];
-            sc->zz = out[2];
-            sc->zw = out[3];
-            cg_test_and_beta(sc);
+    double v[4] = {s_rz, s_rr, s_zz, s_zw};
+    block_partials<4>(v, pvec);
+}
+
+// r.z, r.r, z.z, z.w from the update partials (or r.z, z.z, z.w from the
+// learned apply), convergence test and beta in every block; p = z + beta p.
+template <typename V>
+__global__ void __launch_bounds__(kVecBlock)
+k_cg_p(int n, const V* __restrict__ z, V* __restrict__ p, const double* __restrict__ pvec, int nvec,
+       const double* __restrict__ ppc, int npc, const Scal* __restrict__ in, Scal* __restrict__ out) {
+    __shared__ double s_beta;
+    __shared__ int s_stop;
+    if (carry_if_done(in, out)) return;
+    double u[4], q[3] = {0.0, 0.0, 0.0};
+    sum_partials<4>(pvec, nvec, u);
+    if (ppc) sum_partials<3>(ppc, npc, q);
+    if (threadIdx.x == 0) {
+        Scal s = *in;
+        s.rr = u[1];
+        if (ppc) {
+            s.rz = q[0];
+            s.zz = q[1];
+            s.zw = q[2];
+        } else {
+            s.rz = u[0];
+            s.zz = u[2];
+            s.zw = u[3];
         }
+        cg_test_and_beta(&s);
+        s_beta = s.beta;
+        s_stop = s.done;
+        if (blockIdx.x == 0) *out = s;
     }
+    __syncthreads();
+    if (s_stop) return;
+    const V beta = (V)s_beta;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = z[i] + beta * p[i];
 }
 
 template <typename V>
 __global__ void __launch_bounds__(kVecBlock)
-k_cg_p(int n, const V* __restrict__ z, V* __restrict__ p, Scal* sc, int first) {
-    // One block (the last of k_cg_update) could run the test; instead every
-    // block recomputes nothing and reads the finalized beta.
-    if (sc->done) return;
-    const V beta = first ? (V)0 : (V)sc->beta;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        p[i] = first ? z[i] : z[i] + beta * p[i];
+k_copy(int n, const V* __restrict__ a, V* __restrict__ b) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = a[i];
 }
 
-__global__ void k_cg_test(Scal* sc) {
-    if (threadIdx.x == 0 && blockIdx.x == 0 && !sc->done) cg_test_and_beta(sc);
-}
-
-// r32 = (V) r64 ; z = D^-1 r ; sums r.z, r.r, z.z
+// r_V = (V) r64 ; z = D^-1 r ; partials r.z, r.r, z.z
 template <typename V>
 __global__ void __launch_bounds__(kVecBlock)
 k_restart_vecs(int n, const double* __restrict__ r64, V* __restrict__ r, V* __restrict__ z,
-               const V* __restrict__ dinv, int jacobi, double* __restrict__ partials,
-               unsigned* counter, Scal* sc, int set_norm0) {
+               const V* __restrict__ dinv, int jacobi, double* __restrict__ pvec) {
     double s_rz = 0.0, s_rr = 0.0, s_zz = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const V ri = (V)r64[i];
@@ -317,29 +310,59 @@ k_restart_vecs(int n, const double* __restrict__ r64, V* __restrict__ r, V* __re
             s_zz += (double)zi * (double)zi;
         }
     }
-    double v[3] = {s_rz, s_rr, s_zz}, out[3];
-    if (grid_sum<3>(v, partials, counter, out) && threadIdx.x == 0) {
-        sc->rr = out[1];
-        if (jacobi) {
-            sc->rz = out[0];
-            sc->zz = out[2];
-        }
-        if (set_norm0) {
-            const double nrm = sc->norm_type == FLGPU_NORM_UNPRECONDITIONED ? sqrt(sc->rr) : sqrt(sc->zz);
-            sc->rnorm0 = nrm;
-            sc->last_norm = nrm;
-        }
-    }
+    double v[3] = {s_rz, s_rr, s_zz};
+    block_partials<3>(v, pvec);
 }
 
 __global__ void __launch_bounds__(kVecBlock)
-k_dot2(int n, const double* __restrict__ a, double* __restrict__ partials, unsigned* counter,
-       Scal* sc) {
+k_dot2(int n, const double* __restrict__ a, double* __restrict__ partials) {
     double s = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         s += a[i] * a[i];
-    double v[1] = {s}, out[1];
-    if (grid_sum<1>(v, partials, counter, out) && threadIdx.x == 0) sc->bb = out[0];
+    double v[1] = {s};
+    block_partials<1>(v, partials);
+}
+
+// One-block finalisation of the rarely used reductions, in place on the
+// current scalar slot.
+enum FinOp { FIN_TRUE_RR = 0, FIN_BB = 1, FIN_RESTART = 2, FIN_LEARNED = 3 };
+
+__device__ __forceinline__ void set_norm0(Scal& s) {
+    const double nrm = s.norm_type == FLGPU_NORM_UNPRECONDITIONED ? sqrt(s.rr) : sqrt(s.zz);
+    s.rnorm0 = nrm;
+    s.last_norm = nrm;
+}
+
+__global__ void __launch_bounds__(kVecBlock)
+k_finalize(int op, const double* __restrict__ partials, int nparts, int jacobi, int norm0, Scal* sc) {
+    double v[3];
+    if (op == FIN_TRUE_RR || op == FIN_BB) {
+        double one[1];
+        sum_partials<1>(partials + (op == FIN_TRUE_RR ? kPartStride : 0), nparts, one);
+        if (threadIdx.x == 0) {
+            if (op == FIN_TRUE_RR)
+                sc->true_rr = one[0];
+            else
+                sc->bb = one[0];
+        }
+        return;
+    }
+    sum_partials<3>(partials, nparts, v);
+    if (threadIdx.x != 0) return;
+    Scal s = *sc;
+    if (op == FIN_RESTART) {  // r.z, r.r, z.z
+        s.rr = v[1];
+        if (jacobi) {
+            s.rz = v[0];
+            s.zz = v[2];
+        }
+    } else {  // learned apply: r.z, z.z, z.w
+        s.rz = v[0];
+        s.zz = v[1];
+        s.zw = v[2];
+    }
+    if (norm0) set_norm0(s);
+    *sc = s;
 }
 
 __global__ void __launch_bounds__(kVecBlock)
@@ -362,27 +385,22 @@ static int lanes_for(const flgpu_ctx* c) {
     return 64;
 }
 
+static int spmv_grid(const flgpu_ctx* c) { return stream_grid(c->kw->nrb); }
+
+// y = A x (or y = bsub - A x); with want_dot the per-block sums x.y, y.y
+// go to the PART_SPMV region, spmv_grid(c) of them.
 template <typename V, typename XV, typename YV>
 static int spmv(flgpu_ctx* c, KrylovWork* k, const V* val, const XV* x, YV* y, const double* bsub,
-                int want_dot, int mode, const int* done_flag) {
-    const int L = lanes_for(c);
-    const int rows_per_blk = kVecBlock / L;
-    int grid = (int)std::min<int64_t>(kMaxVecGrid, (c->rows + rows_per_blk - 1) / rows_per_blk);
-    grid = std::max(grid, 1);
-#define FL_SPMV(LL)                                                                                   \
-    case LL:                                                                                          \
-        hipLaunchKernelGGL((k_spmv<LL, V, XV, YV>), dim3(grid), dim3(kVecBlock), 0, c->stream, c->rows, \
-                           c->rowptr.p, c->col.p, val, x, y, bsub, want_dot, k->partials.p,            \
-                           k->counter.p, k->scal.p, mode, done_flag);                                 \
-        break;
-    switch (L) {
-        FL_SPMV(4)
-        FL_SPMV(8)
-        FL_SPMV(16)
-        FL_SPMV(32)
-        FL_SPMV(64)
-    }
-#undef FL_SPMV
+                int want_dot, const int* done_flag) {
+    const int grid = stream_grid(k->nrb);
+    if (want_dot)
+        hipLaunchKernelGGL((k_csr_stream<V, V, XV, YV, DOT_XY>), dim3(grid), dim3(kRowBlock), 0, c->stream, k->nrb,
+                           k->rb.p, c->rowptr.p, c->col.p, val, x, y, bsub, (const V*)nullptr, (const V*)nullptr,
+                           k->part(PART_SPMV), done_flag);
+    else
+        hipLaunchKernelGGL((k_csr_stream<V, V, XV, YV, DOT_NONE>), dim3(grid), dim3(kRowBlock), 0, c->stream,
+                           k->nrb, k->rb.p, c->rowptr.p, c->col.p, val, x, y, bsub, (const V*)nullptr,
+                           (const V*)nullptr, k->part(PART_SPMV), done_flag);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
@@ -414,12 +432,16 @@ static int ensure_work(flgpu_ctx* c, bool mixed) {
         FLGPU_TRY(k->dinv32.alloc(n));
         FLGPU_TRY(k->y32.alloc(n));
     }
-    FLGPU_TRY(k->partials.alloc((size_t)kMaxVecGrid * 8));
-    if (!k->counter.p) {
-        FLGPU_TRY(k->counter.alloc(1));
-        FLGPU_HIP(hipMemsetAsync(k->counter.p, 0, sizeof(unsigned), c->stream));
+    FLGPU_TRY(k->partials.reserve((size_t)kPartStride * PART_COMPONENTS));
+    if (k->rb_key != c->sparsity_gen) {
+        std::vector<int32_t> rb;
+        stream_row_blocks(c->h_rowptr.data(), n, rb);
+        FLGPU_TRY(k->rb.upload(rb.data(), rb.size(), c->stream));
+        FLGPU_HIP(hipStreamSynchronize(c->stream));  // rb is freed on return
+        k->nrb = (int)rb.size() - 1;
+        k->rb_key = c->sparsity_gen;
     }
-    FLGPU_TRY(k->scal.alloc(1));
+    FLGPU_TRY(k->scal.alloc(2));
     if (!k->h_scal) FLGPU_HIP(hipHostMalloc((void**)&k->h_scal, sizeof(Scal), hipHostMallocDefault));
     k->nblk = (int)std::min<int64_t>(kMaxVecGrid, (n + kVecBlock - 1) / kVecBlock);
     k->nblk = std::max(k->nblk, 1);
@@ -427,13 +449,20 @@ static int ensure_work(flgpu_ctx* c, bool mixed) {
 }
 
 static int read_scal(flgpu_ctx* c, KrylovWork* k) {
-    FLGPU_HIP(hipMemcpyAsync(k->h_scal, k->scal.p, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
+    FLGPU_HIP(hipMemcpyAsync(k->h_scal, k->S(), sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
     FLGPU_HIP(hipStreamSynchronize(c->stream));
     return FLGPU_OK;
 }
 
 static int write_scal(flgpu_ctx* c, KrylovWork* k) {
-    FLGPU_HIP(hipMemcpyAsync(k->scal.p, k->h_scal, sizeof(Scal), hipMemcpyHostToDevice, c->stream));
+    FLGPU_HIP(hipMemcpyAsync(k->S(), k->h_scal, sizeof(Scal), hipMemcpyHostToDevice, c->stream));
+    return FLGPU_OK;
+}
+
+static int finalize(flgpu_ctx* c, KrylovWork* k, int op, int region, int nparts, bool jacobi, bool norm0) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kVecBlock), 0, c->stream, op, k->part(region), nparts,
+                       jacobi ? 1 : 0, norm0 ? 1 : 0, k->S());
+    FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
 
@@ -441,10 +470,11 @@ static int write_scal(flgpu_ctx* c, KrylovWork* k) {
 static int build_reduced(flgpu_ctx* c, KrylovWork* k, bool mixed, bool zero_x) {
     const int n = c->rows;
     hipLaunchKernelGGL(k_diag_minmax, dim3(k->nblk), dim3(kVecBlock), 0, c->stream, n, c->diag.p, c->A.p,
-                       k->partials.p, k->counter.p, k->scal.p);
+                       k->part(PART_MISC));
+    hipLaunchKernelGGL(k_pivot, dim3(1), dim3(64), 0, c->stream, k->nblk, k->part(PART_MISC), k->S());
     hipLaunchKernelGGL(k_build, dim3(grid_for(n, kVecBlock)), dim3(kVecBlock), 0, c->stream, n, c->rowptr.p,
                        c->col.p, c->diag.p, c->A.p, c->rhs.p, c->inactive.p, c->have_inactive ? 1 : 0,
-                       k->scal.p, k->val64.p, mixed ? k->val32.p : nullptr, k->b.p, k->dinv.p,
+                       k->S(), k->val64.p, mixed ? k->val32.p : nullptr, k->b.p, k->dinv.p,
                        mixed ? k->dinv32.p : nullptr, k->x.p, zero_x ? 1 : 0);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
@@ -452,7 +482,13 @@ static int build_reduced(flgpu_ctx* c, KrylovWork* k, bool mixed, bool zero_x) {
 
 // True residual r64 = b - A x (FP64) and its squared norm in scal.true_rr.
 static int true_residual(flgpu_ctx* c, KrylovWork* k) {
-    return spmv<double, double, double>(c, k, k->val64.p, k->x.p, k->r.p, k->b.p, 1, 1, nullptr);
+    FLGPU_TRY((spmv<double, double, double>(c, k, k->val64.p, k->x.p, k->r.p, k->b.p, 1, nullptr)));
+    return finalize(c, k, FIN_TRUE_RR, PART_SPMV, spmv_grid(c), false, false);
+}
+
+static int norm_b(flgpu_ctx* c, KrylovWork* k) {
+    hipLaunchKernelGGL(k_dot2, dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->b.p, k->part(PART_MISC));
+    return finalize(c, k, FIN_BB, PART_MISC, k->nblk, false, false);
 }
 
 // ------------------------- (F)PCG ----------------------------------------
@@ -474,22 +510,56 @@ CgVecs<float> cg_vecs<float>(KrylovWork* k) {
     return {k->r32.p, k->z32.p, k->p32.p, k->w32.p, k->dinv32.p, k->y32.p, k->val32.p};
 }
 
-// Apply the learned preconditioner z = G^T (G r) and finalize r.z, r.r,
-// z.z, z.w (w = A p of this iteration).
+// Apply the learned preconditioner z = G^T (G r); the per-block sums
+// r.z, z.z, z.w (w = A p of this iteration) go to the PART_PC region.
+// Returns the number of partial sums in *nparts.
 template <typename V>
-static int learned_apply(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, bool set_norm0);
+static int learned_apply(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, const int* done_flag, int* nparts) {
+    return learned_precond_apply<V>(c, v.r, v.z, v.y, v.w, k->part(PART_PC), nparts, done_flag);
+}
+
+// r = b - A x (FP64), r_V = r, z = M^-1 r with the sums r.z, r.r, z.z
+// (and the initial norm when norm0) in the current scalar slot.
+template <typename V>
+static int restart_residual(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, bool jacobi, bool norm0,
+                            bool recompute) {
+    if (recompute) FLGPU_TRY(true_residual(c, k));
+    hipLaunchKernelGGL((k_restart_vecs<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->r.p,
+                       v.r, v.z, v.dinv, jacobi ? 1 : 0, k->part(PART_VEC));
+    FLGPU_HIP(hipGetLastError());
+    FLGPU_TRY(finalize(c, k, FIN_RESTART, PART_VEC, k->nblk, jacobi, norm0 && jacobi));
+    if (!jacobi) {
+        int np = 0;
+        FLGPU_TRY(learned_apply<V>(c, k, v, nullptr, &np));
+        FLGPU_TRY(finalize(c, k, FIN_LEARNED, PART_PC, np, false, norm0));
+    }
+    return FLGPU_OK;
+}
 
 template <typename V>
 static int cg_restart(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, bool jacobi, bool set_norm0) {
-    // r = b - A x (FP64), r_V = r, z = M^-1 r, p = z
-    FLGPU_TRY(true_residual(c, k));
-    hipLaunchKernelGGL((k_restart_vecs<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->r.p,
-                       v.r, v.z, v.dinv, jacobi ? 1 : 0, k->partials.p, k->counter.p, k->scal.p,
-                       set_norm0 ? 1 : 0);
+    FLGPU_TRY(restart_residual<V>(c, k, v, jacobi, set_norm0, true));
+    hipLaunchKernelGGL((k_copy<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, (const V*)v.z, v.p);
     FLGPU_HIP(hipGetLastError());
-    if (!jacobi) FLGPU_TRY(learned_apply<V>(c, k, v, set_norm0));
+    return FLGPU_OK;
+}
+
+// One (F)PCG iteration: w = A p; update x, r, z; [z = G^T G r]; test,
+// beta, p.  Four (Jacobi) or six (learned) kernels, no host round trip.
+template <typename V>
+static int cg_iteration(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, bool jacobi) {
+    FLGPU_TRY((spmv<V, V, V>(c, k, v.val, v.p, v.w, nullptr, 1, &k->S()->done)));
+    Scal* in = k->S();
+    k->cur ^= 1;
+    hipLaunchKernelGGL((k_cg_update<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->x.p, v.p, v.r,
+                       v.w, v.z, v.dinv, jacobi ? 1 : 0, k->part(PART_SPMV), spmv_grid(c), k->part(PART_VEC), in,
+                       k->S());
+    int npc = 0;
+    if (!jacobi) FLGPU_TRY(learned_apply<V>(c, k, v, &k->S()->done, &npc));
+    in = k->S();
+    k->cur ^= 1;
     hipLaunchKernelGGL((k_cg_p<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, v.z, v.p,
-                       k->scal.p, 1);
+                       k->part(PART_VEC), k->nblk, jacobi ? nullptr : k->part(PART_PC), npc, in, k->S());
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
@@ -518,25 +588,23 @@ static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info*
         jacobi = true;
     }
     Scal& h = *k->h_scal;
-    std::memset(&h, 0, sizeof h);
+    // keep the pivot computed by build_reduced
+    FLGPU_TRY(read_scal(c, k));
+    {
+        const double pivot = h.pivot, dmin = h.dmin, dmax = h.dmax;
+        std::memset(&h, 0, sizeof h);
+        h.pivot = pivot;
+        h.dmin = dmin;
+        h.dmax = dmax;
+    }
     h.max_its = sp->max_its > 0 ? sp->max_its : 10000;
     h.rtol = sp->rtol;
     h.atol = sp->atol;
     h.norm_type = sp->norm_type;
     h.flexible = (use_learned || sp->ksp == FLGPU_KSP_FCG) ? 1 : 0;
-    // keep pivot computed by build_reduced
-    {
-        Scal tmp;
-        FLGPU_HIP(hipMemcpyAsync(&tmp, k->scal.p, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
-        FLGPU_HIP(hipStreamSynchronize(c->stream));
-        h.pivot = tmp.pivot;
-        h.dmin = tmp.dmin;
-        h.dmax = tmp.dmax;
-    }
     FLGPU_TRY(write_scal(c, k));
-    hipLaunchKernelGGL(k_dot2, dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->b.p, k->partials.p,
-                       k->counter.p, k->scal.p);
-    FLGPU_TRY(cg_restart<V>(c, k, v, !use_learned ? true : false, true));
+    FLGPU_TRY(norm_b(c, k));
+    FLGPU_TRY(cg_restart<V>(c, k, v, !use_learned, true));
     FLGPU_TRY(read_scal(c, k));
     info->refinements = 0;
     info->fallback = 0;
@@ -551,19 +619,7 @@ static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info*
     double window_norm = h.last_norm;
     auto t0 = std::chrono::steady_clock::now();
     while (!h.done) {
-        for (int q = 0; q < check_every; q++) {
-            FLGPU_TRY((spmv<V, V, V>(c, k, v.val, v.p, v.w, nullptr, 1, 0, &k->scal.p->done)));
-            hipLaunchKernelGGL((k_cg_update<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->x.p,
-                               v.p, v.r, v.w, v.z, v.dinv, jacobi ? 1 : 0, k->partials.p, k->counter.p,
-                               k->scal.p);
-            if (!jacobi) {
-                FLGPU_TRY(learned_apply<V>(c, k, v, false));
-                hipLaunchKernelGGL(k_cg_test, dim3(1), dim3(64), 0, c->stream, k->scal.p);
-            }
-            hipLaunchKernelGGL((k_cg_p<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, v.z, v.p,
-                               k->scal.p, 0);
-        }
-        FLGPU_HIP(hipGetLastError());
+        for (int q = 0; q < check_every; q++) FLGPU_TRY(cg_iteration<V>(c, k, v, jacobi));
         FLGPU_TRY(read_scal(c, k));
         const double rnow = std::sqrt(h.rr);
         if (mixed && (h.done ? h.reason > 0 : rnow < delta * rmax)) {
@@ -581,10 +637,7 @@ static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info*
                 FLGPU_TRY(write_scal(c, k));
             }
             if (!h.done) {
-                hipLaunchKernelGGL((k_restart_vecs<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows,
-                                   k->r.p, v.r, v.z, v.dinv, jacobi ? 1 : 0, k->partials.p, k->counter.p,
-                                   k->scal.p, 0);
-                if (!jacobi) FLGPU_TRY(learned_apply<V>(c, k, v, false));
+                FLGPU_TRY(restart_residual<V>(c, k, v, jacobi, false, false));
                 FLGPU_TRY(read_scal(c, k));
                 rmax = std::sqrt(h.rr);
             }
@@ -619,23 +672,15 @@ static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info*
     return FLGPU_OK;
 }
 
-// ------------------------- learned apply ---------------------------------
-
-template <typename V>
-static int learned_apply(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, bool set_norm0) {
-    return learned_precond_apply<V>(c, v.r, v.z, v.y, v.w, k->partials.p, k->counter.p,
-                                    k->scal.p, set_norm0);
-}
-
 // ------------------------- GMRES ------------------------------------------
 
 // w = D^-1 (A v)  [left Jacobi]
 template <int L, typename V>
-__global__ void __launch_bounds__(kVecBlock)
+__global__ void __launch_bounds__(kRowBlock)
 k_gm_matvec(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
             const V* __restrict__ val, const V* __restrict__ dinv, const V* __restrict__ x,
             V* __restrict__ y) {
-    constexpr int ROWS = kVecBlock / L;
+    constexpr int ROWS = kRowBlock / L;
     const int lane = threadIdx.x % L, rloc = threadIdx.x / L;
     for (int grp = blockIdx.x; grp * ROWS < n; grp += gridDim.x) {
         const int row = grp * ROWS + rloc;
@@ -760,16 +805,16 @@ static int run_gmres(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_in
             FLGPU_HIP(hipMemcpyAsync(k->dinv32.p, ones32.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
     }
     FLGPU_TRY(k->hcol.alloc(m + 2));
-    FLGPU_TRY(k->partials.alloc((size_t)kMaxVecGrid * (m + 2)));
+    FLGPU_TRY(k->partials.reserve((size_t)kMaxVecGrid * (m + 2)));
     const int nb = k->nblk;
     const int L = lanes_for(c);
-    const int rows_per_blk = kVecBlock / L;
-    int mgrid = (int)std::min<int64_t>(kMaxVecGrid, (n + rows_per_blk - 1) / rows_per_blk);
+    const int rows_per_blk = kRowBlock / L;
+    int mgrid = (int)std::min<int64_t>(kMaxRowGrid, (n + rows_per_blk - 1) / rows_per_blk);
     mgrid = std::max(mgrid, 1);
     auto matvec = [&](const V* xin, V* yout) -> int {
 #define FL_GMV(LL)                                                                                         \
     case LL:                                                                                               \
-        hipLaunchKernelGGL((k_gm_matvec<LL, V>), dim3(mgrid), dim3(kVecBlock), 0, c->stream, n, c->rowptr.p, \
+        hipLaunchKernelGGL((k_gm_matvec<LL, V>), dim3(mgrid), dim3(kRowBlock), 0, c->stream, n, c->rowptr.p, \
                            c->col.p, val, dinv, xin, yout);                                                \
         break;
         switch (L) {
@@ -789,8 +834,7 @@ static int run_gmres(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_in
     Scal& h = *k->h_scal;
     std::memset(&h, 0, sizeof h);
     FLGPU_TRY(write_scal(c, k));
-    hipLaunchKernelGGL(k_dot2, dim3(nb), dim3(kVecBlock), 0, c->stream, n, k->b.p, k->partials.p, k->counter.p,
-                       k->scal.p);
+    FLGPU_TRY(norm_b(c, k));
     FLGPU_TRY(read_scal(c, k));
     const double bnorm = std::sqrt(h.bb);
     auto t0 = std::chrono::steady_clock::now();
@@ -950,7 +994,7 @@ int csr_mult(flgpu_ctx* c, const double* x, double* y) {
     DevBuf<double> dx, dy;
     FLGPU_TRY(dx.upload(x, c->cols, c->stream));
     FLGPU_TRY(dy.alloc(c->rows));
-    FLGPU_TRY((spmv<double, double, double>(c, k, c->A.p, dx.p, dy.p, nullptr, 0, 0, nullptr)));
+    FLGPU_TRY((spmv<double, double, double>(c, k, c->A.p, dx.p, dy.p, nullptr, 0, nullptr)));
     FLGPU_HIP(hipMemcpyAsync(y, dy.p, sizeof(double) * c->rows, hipMemcpyDeviceToHost, c->stream));
     FLGPU_HIP(hipStreamSynchronize(c->stream));
     return FLGPU_OK;
@@ -987,9 +1031,9 @@ int time_spmv(flgpu_ctx* c, int precision, int reps, double* ms) {
     FLGPU_HIP(hipEventRecord(e0, c->stream));
     for (int r = 0; r < reps; r++) {
         if (mixed)
-            FLGPU_TRY((spmv<float, float, float>(c, k, k->val32.p, k->p32.p, k->w32.p, nullptr, 0, 0, nullptr)));
+            FLGPU_TRY((spmv<float, float, float>(c, k, k->val32.p, k->p32.p, k->w32.p, nullptr, 0, nullptr)));
         else
-            FLGPU_TRY((spmv<double, double, double>(c, k, k->val64.p, k->p.p, k->w.p, nullptr, 0, 0, nullptr)));
+            FLGPU_TRY((spmv<double, double, double>(c, k, k->val64.p, k->p.p, k->w.p, nullptr, 0, nullptr)));
     }
     FLGPU_HIP(hipEventRecord(e1, c->stream));
     FLGPU_HIP(hipEventSynchronize(e1));

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace flgpu {
 
@@ -29,10 +30,23 @@ struct Scal {
     int32_t norm_type, flexible, pad0, pad1;
 };
 
-constexpr int kVecBlock = 256;    // threads per block of the vector kernels
-constexpr int kMaxVecGrid = 2048; // fixed grid: partial sums have a fixed order
+constexpr int kVecBlock = 512;     // threads per block of the vector kernels (8 waves)
+constexpr int kMaxVecGrid = 512;   // their fixed grid (2 blocks per CU)
+constexpr int kRowBlock = 256;     // threads per block of the CSR row products
+constexpr int kMaxRowGrid = 2048;  // their fixed grid
+constexpr int kPartStride = 2048;  // partials[k * kPartStride + block], >= either grid
 
-// Sum over a 256-thread block in a fixed order; result valid in thread 0.
+// Regions of the partial-sum buffer (in units of kPartStride).  A producer
+// kernel writes one per-block value per component; the consumer sums them.
+enum {
+    PART_SPMV = 0,  // 2 components: x.y, y.y of the CSR product
+    PART_VEC = 2,   // 4 components: r.z, r.r, z.z, z.w of the CG update
+    PART_PC = 6,    // 3 components: r.z, z.z, z.w of the learned apply
+    PART_MISC = 9,  // 2 components: b.b, diagonal min/max
+    PART_COMPONENTS = 11
+};
+
+// Sum over a block in a fixed order; result valid in thread 0.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double (*sh)[NV]) {
 #pragma unroll
@@ -55,46 +69,143 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*sh)[NV]) {
     __syncthreads();
 }
 
-__device__ __forceinline__ double load_coherent(const double* p) {
-    unsigned long long u = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
-                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double((long long)u);
+// Per-block sums of NV values: thread 0 of each block stores them at
+// partials[k * kPartStride + blockIdx.x].
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ partials) {
+    __shared__ double sh[kVecBlock / 64][NV];
+    block_sum<NV>(v, sh);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) partials[(size_t)k * kPartStride + blockIdx.x] = v[k];
 }
 
-// Every block writes its NV partial sums; the last block to finish adds
-// all partials in block order (so the result does not depend on which
-// block finished last) and returns true with the totals in out (thread 0).
+// Total of the nparts per-block sums a previous kernel wrote.  Every block
+// that calls it forms the same sum in the same order (thread t adds parts
+// t, t + blockDim, ..., then the fixed block order), so all blocks agree
+// bit for bit and no grid-wide atomics or last-block election are needed.
+// Result valid in thread 0.
 template <int NV>
-__device__ bool grid_sum(double (&v)[NV], double* __restrict__ partials, unsigned* counter,
-                         double (&out)[NV]) {
+__device__ __forceinline__ void sum_partials(const double* __restrict__ partials, int nparts,
+                                             double (&out)[NV]) {
     __shared__ double sh[kVecBlock / 64][NV];
-    __shared__ unsigned ticket;
-    block_sum<NV>(v, sh);
-    if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < NV; k++) partials[(size_t)blockIdx.x * NV + k] = v[k];
-        __threadfence();
-        ticket = atomicAdd(counter, 1u);
+    for (int k = 0; k < NV; k++) out[k] = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < NV; k++) out[k] += partials[(size_t)k * kPartStride + b];
+    block_sum<NV>(out, sh);
+}
+
+// ---------------------------------------------------------------------
+// Streamed CSR products
+// ---------------------------------------------------------------------
+constexpr int kStreamPerThread = 8;
+constexpr int kStreamNnz = kRowBlock * kStreamPerThread;  // entries staged per row block
+
+// Row blocks for k_csr_stream: runs of consecutive rows holding at most
+// kStreamNnz entries and at most kRowBlock rows; a longer row is a block of
+// its own.  rb has nblocks + 1 entries.
+inline void stream_row_blocks(const int32_t* rowptr, int n, std::vector<int32_t>& rb) {
+    rb.clear();
+    rb.push_back(0);
+    int r = 0;
+    while (r < n) {
+        const int k0 = rowptr[r];
+        int r1 = r;
+        while (r1 < n && r1 - r < kRowBlock && rowptr[r1 + 1] - k0 <= kStreamNnz) r1++;
+        if (r1 == r) r1 = r + 1;
+        rb.push_back(r1);
+        r = r1;
     }
-    __syncthreads();
-    if (ticket != gridDim.x - 1) return false;
-    // acquire (agent-scope fence: stale L1 lines are dropped), then the
-    // partials are read with ordinary, pipelined loads
-    __threadfence();
-    const double* vp = partials;
-    double acc[NV];
-#pragma unroll
-    for (int k = 0; k < NV; k++) acc[k] = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-#pragma unroll
-        for (int k = 0; k < NV; k++) acc[k] += vp[(size_t)b * NV + k];
-    block_sum<NV>(acc, sh);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) out[k] = acc[k];
-        *counter = 0u;
+}
+
+enum { DOT_NONE = 0, DOT_XY = 1, DOT_PC = 2 };
+
+// y = A x (or y = bsub - A x) over row blocks.  A block reads its entries
+// once, coalesced, stages the products val*x[col] in LDS and one thread per
+// row adds them in entry order: the sequential row sum of a plain CSR
+// product, with the product rounded before the addition.  A fixed grid
+// walks the row blocks; with a grid that is a multiple of 8 each XCD (the
+// hardware hands consecutive workgroups to the 8 XCDs in turn) takes a
+// contiguous share, so the x entries its rows gather stay in its own L2.
+// Per-block dot products into partials: DOT_XY: x_i y_i, y_i^2;
+// DOT_PC: r_i y_i, y_i^2, y_i w_i.
+template <typename V, typename AV, typename XV, typename YV, int DOTS>
+__global__ void __launch_bounds__(kRowBlock)
+k_csr_stream(int nrb, const int32_t* __restrict__ rb, const int32_t* __restrict__ rowptr,
+             const int32_t* __restrict__ col, const AV* __restrict__ val, const XV* __restrict__ x,
+             YV* __restrict__ y, const double* __restrict__ bsub, const V* __restrict__ dr,
+             const V* __restrict__ dw, double* __restrict__ partials, const int* __restrict__ done_flag) {
+    if (done_flag && *done_flag) return;
+    __shared__ V prod[kStreamNnz];
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+    int b = blockIdx.x, hi = nrb, step = gridDim.x;
+    if ((gridDim.x & 7) == 0) {
+        const int xcd = blockIdx.x & 7;
+        b = (int)((int64_t)nrb * xcd / 8) + (blockIdx.x >> 3);
+        hi = (int)((int64_t)nrb * (xcd + 1) / 8);
+        step = gridDim.x >> 3;
     }
-    return true;
+    for (; b < hi; b += step) {
+        const int r0 = rb[b], r1 = rb[b + 1];
+        const int k0 = rowptr[r0], k1 = rowptr[r1];
+        const int row = r0 + threadIdx.x;
+        V acc = (V)0;
+        if (k1 - k0 <= kStreamNnz) {
+            int kb = 0, ke = 0;
+            if (row < r1) {
+                kb = rowptr[row] - k0;
+                ke = rowptr[row + 1] - k0;
+            }
+#pragma unroll
+            for (int j = 0; j < kStreamPerThread; j++) {
+                const int k = k0 + threadIdx.x + j * kRowBlock;
+                if (k < k1) prod[k - k0] = (V)val[k] * (V)x[col[k]];
+            }
+            __syncthreads();
+            for (int k = kb; k < ke; k++) acc += prod[k];
+            __syncthreads();
+        } else {  // one row longer than the staging buffer: in order, chunk by chunk
+            for (int c0 = k0; c0 < k1; c0 += kStreamNnz) {
+                const int c1 = min(k1, c0 + kStreamNnz);
+                for (int k = c0 + threadIdx.x; k < c1; k += kRowBlock) prod[k - c0] = (V)val[k] * (V)x[col[k]];
+                __syncthreads();
+                if (threadIdx.x == 0)
+                    for (int k = 0; k < c1 - c0; k++) acc += prod[k];
+                __syncthreads();
+            }
+        }
+        if (row < r1) {
+            double yv = (double)acc;
+            if (bsub) yv = bsub[row] - yv;
+            const YV yo = (YV)yv;
+            y[row] = yo;
+            if (DOTS == DOT_XY) {
+                d0 += (double)x[row] * yv;
+                d1 += yv * yv;
+            } else if (DOTS == DOT_PC) {
+                const double z = (double)yo;
+                d0 += (double)dr[row] * z;
+                d1 += z * z;
+                d2 += z * (double)dw[row];
+            }
+        }
+    }
+    if (DOTS == DOT_XY) {
+        double v[2] = {d0, d1};
+        block_partials<2>(v, partials);
+    } else if (DOTS == DOT_PC) {
+        double v[3] = {d0, d1, d2};
+        block_partials<3>(v, partials);
+    }
+}
+
+// Fixed grid for k_csr_stream: a multiple of 8, at most kMaxRowGrid.
+inline int stream_grid(int nrb) {
+    if (nrb < 8) return nrb > 0 ? nrb : 1;
+    const int g = nrb < kMaxRowGrid ? nrb : kMaxRowGrid;
+    return g & ~7;
 }
 
 }  // namespace flgpu

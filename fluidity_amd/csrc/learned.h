@@ -12,12 +12,12 @@ namespace flgpu {
 // working precision).  Called once per solve.
 int learned_prepare(flgpu_ctx* c, bool mixed);
 
-// z = G^T (G r) with y as scratch; then the fixed-order sums r.z, r.r,
-// z.z, z.w are finalised into the solver scalars (scal points at the
-// solver's Scal: rz_old <- rz, rz, zz, zw; and the initial norm when
-// set_norm0).
+// z = G^T (G r) with y as scratch.  The per-block sums r.z, z.z, z.w
+// (w = A p of the iteration) are written to partials[k * kPartStride + b]
+// for the solver to reduce; *nparts receives the number of blocks.  With
+// done_flag set the kernels return at once.
 template <typename V>
-int learned_precond_apply(flgpu_ctx* c, const V* r, V* z, V* y, const V* w, double* partials,
-                          unsigned* counter, Scal* scal, bool set_norm0);
+int learned_precond_apply(flgpu_ctx* c, const V* r, V* z, V* y, const V* w, double* partials, int* nparts,
+                          const int* done_flag);
 
 }  // namespace flgpu

@@ -81,11 +81,19 @@ struct LearnedState {
     DevBuf<float> h, PQ, M, U, ab;  // ab: alpha,beta,psi per node (3N)
     DevBuf<float> gv;               // preconditioner values in A's pattern
     DevBuf<double> gv64;
+    // solver copies of the factor: the lower part (j <= i, with the
+    // diagonal) and the upper part (j >= i) as two compact CSR matrices,
+    // so that each product streams only its own half
+    int64_t lu_key = -1;
+    DevBuf<int32_t> lptr, lcol, lsrc, uptr, ucol, usrc, rbl, rbu;
+    int nrbl = 0, nrbu = 0;
+    int64_t nnzl = 0, nnzu = 0;
+    DevBuf<float> gl, gu;
+    DevBuf<double> gl64, gu64;
     // training
     DevBuf<float> z, u, v, w, dv, dw, dphi, dpsi, dab, dh, dHM, dU, dPQ, dM;
     DevBuf<float> wpart;            // weight-gradient partial tiles
     DevBuf<double> lpart;           // loss partials
-    DevBuf<unsigned> counter;
     DevBuf<double> lossbuf;
 };
 
@@ -352,66 +360,12 @@ k_build_g(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__
     }
 }
 
-// ---------------------------------------------------------------------------
-// preconditioner apply (solver): y = G r (lower part), z = G^T y (upper)
-// ---------------------------------------------------------------------------
-template <int L, typename V, typename GV>
-__global__ void __launch_bounds__(kVecBlock)
-k_g_lower(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-          const int32_t* __restrict__ diag, const GV* __restrict__ gv, const V* __restrict__ r,
-          V* __restrict__ y, const int* __restrict__ done_flag) {
-    if (done_flag && *done_flag) return;
-    constexpr int ROWS = kVecBlock / L;
-    const int lane = threadIdx.x % L, rloc = threadIdx.x / L;
-    for (int grp = blockIdx.x; grp * ROWS < n; grp += gridDim.x) {
-        const int row = grp * ROWS + rloc;
-        V acc = (V)0;
-        if (row < n) {
-            const int e = diag[row] + 1;
-            for (int k = rowptr[row] + lane; k < e; k += L) acc += (V)gv[k] * r[col[k]];
-        }
-        for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-        if (row < n && lane == 0) y[row] = acc;
-    }
-}
-
-template <int L, typename V, typename GV>
-__global__ void __launch_bounds__(kVecBlock)
-k_g_upper(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-          const int32_t* __restrict__ diag, const GV* __restrict__ gv, const V* __restrict__ y,
-          V* __restrict__ z, const V* __restrict__ r, const V* __restrict__ w, double* __restrict__ partials,
-          unsigned* counter, Scal* sc, int set_norm0, const int* __restrict__ done_flag) {
-    if (done_flag && *done_flag) return;
-    constexpr int ROWS = kVecBlock / L;
-    const int lane = threadIdx.x % L, rloc = threadIdx.x / L;
-    double s_rz = 0.0, s_zz = 0.0, s_zw = 0.0;
-    for (int grp = blockIdx.x; grp * ROWS < n; grp += gridDim.x) {
-        const int row = grp * ROWS + rloc;
-        V acc = (V)0;
-        if (row < n) {
-            const int e = rowptr[row + 1];
-            for (int k = diag[row] + lane; k < e; k += L) acc += (V)gv[k] * y[col[k]];
-        }
-        for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-        if (row < n && lane == 0) {
-            z[row] = acc;
-            s_rz += (double)r[row] * (double)acc;
-            s_zz += (double)acc * (double)acc;
-            if (w) s_zw += (double)acc * (double)w[row];
-        }
-    }
-    double v[3] = {s_rz, s_zz, s_zw}, out[3];
-    if (grid_sum<3>(v, partials, counter, out) && threadIdx.x == 0) {
-        sc->rz_old = sc->rz;
-        sc->rz = out[0];
-        sc->zz = out[1];
-        sc->zw = out[2];
-        if (set_norm0) {
-            const double nrm = sc->norm_type == FLGPU_NORM_UNPRECONDITIONED ? sqrt(sc->rr) : sqrt(sc->zz);
-            sc->rnorm0 = nrm;
-            sc->last_norm = nrm;
-        }
-    }
+// gathers the factor values of one half from A's pattern
+template <typename GV>
+__global__ void k_gather_half(int64_t m, const int32_t* __restrict__ src, const GV* __restrict__ gv,
+                              GV* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) out[t] = gv[src[t]];
 }
 
 // ---------------------------------------------------------------------------
@@ -443,15 +397,23 @@ k_batch_spmv(int n, int P, const int32_t* __restrict__ rowptr, const int32_t* __
 // res = z - w; loss partials; dw = -2 res / (N P)
 __global__ void __launch_bounds__(kVecBlock)
 k_loss(int np, float scale, const float* __restrict__ z, const float* __restrict__ w, float* __restrict__ dw,
-       double* __restrict__ partials, unsigned* counter, double* lossout) {
+       double* __restrict__ partials) {
     double s = 0.0;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < np; t += gridDim.x * blockDim.x) {
         const float r = z[t] - w[t];
         s += (double)r * (double)r;
         dw[t] = -2.f * r * scale;
     }
-    double v[1] = {s}, out[1];
-    if (grid_sum<1>(v, partials, counter, out) && threadIdx.x == 0) lossout[0] = out[0] * (double)scale;
+    double v[1] = {s};
+    block_partials<1>(v, partials);
+}
+
+// One block: loss = scale * sum of the k_loss partials.
+__global__ void __launch_bounds__(kVecBlock)
+k_loss_fin(int nparts, float scale, const double* __restrict__ partials, double* lossout) {
+    double v[1];
+    sum_partials<1>(partials, nparts, v);
+    if (threadIdx.x == 0) lossout[0] = v[0] * (double)scale;
 }
 
 // dGs per stored lower/diagonal entry -> dphi (lower), dpsi (diag)
@@ -485,8 +447,7 @@ __global__ void __launch_bounds__(kVecBlock)
 k_out_back(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
            const int32_t* __restrict__ diag, const int32_t* __restrict__ tmap, const float* __restrict__ sval,
            const float* __restrict__ dphi, const float* __restrict__ dpsi, const float* __restrict__ th,
-           float* __restrict__ dab, float* __restrict__ dh, double* __restrict__ partials, unsigned* counter,
-           float* __restrict__ grad) {
+           float* __restrict__ dab, float* __restrict__ dh, double* __restrict__ partials) {
     double g1 = 0.0, g2 = 0.0, gb = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float da = 0.f, db = 0.f;
@@ -508,11 +469,19 @@ k_out_back(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict_
         for (int c = 0; c < H; c++)
             dh[(size_t)i * H + c] = da * th[Lay::WOA + c] + db * th[Lay::WOB + c] + dp * th[Lay::WD + c];
     }
-    double v[3] = {g1, g2, gb}, out[3];
-    if (grid_sum<3>(v, partials, counter, out) && threadIdx.x == 0) {
-        grad[Lay::G1] += (float)out[0];
-        grad[Lay::G2] += (float)out[1];
-        grad[Lay::BO] += (float)out[2];
+    double v[3] = {g1, g2, gb};
+    block_partials<3>(v, partials);
+}
+
+// One block: gradients of gamma1, gamma2, b_o from the k_out_back partials.
+__global__ void __launch_bounds__(kVecBlock)
+k_out_back_fin(int nparts, const double* __restrict__ partials, float* __restrict__ grad) {
+    double v[3];
+    sum_partials<3>(partials, nparts, v);
+    if (threadIdx.x == 0) {
+        grad[Lay::G1] += (float)v[0];
+        grad[Lay::G2] += (float)v[1];
+        grad[Lay::BO] += (float)v[2];
     }
 }
 
@@ -827,6 +796,9 @@ static int build_g(flgpu_ctx* c, LearnedState* s, GV* out) {
 const double* reduced_values(flgpu_ctx* c);
 int reduced_prepare(flgpu_ctx* c);
 
+template <typename GV>
+static int split_halves(flgpu_ctx* c, LearnedState* s, const GV* gv, DevBuf<GV>& gl, DevBuf<GV>& gu);
+
 int learned_prepare(flgpu_ctx* c, bool mixed) {
     if (!c->all_diag) return set_error(FLGPU_EUNSUPPORTED, "learned preconditioner: a row has no diagonal entry");
     LearnedState* s = state(c);
@@ -836,60 +808,105 @@ int learned_prepare(flgpu_ctx* c, bool mixed) {
     if (mixed) {
         FLGPU_TRY(s->gv.alloc(c->nnz));
         FLGPU_TRY(build_g<float>(c, s, s->gv.p));
+        FLGPU_TRY(split_halves<float>(c, s, s->gv.p, s->gl, s->gu));
     } else {
         FLGPU_TRY(s->gv64.alloc(c->nnz));
         FLGPU_TRY(build_g<double>(c, s, s->gv64.p));
+        FLGPU_TRY(split_halves<double>(c, s, s->gv64.p, s->gl64, s->gu64));
     }
     FLGPU_HIP(hipStreamSynchronize(c->stream));
     return FLGPU_OK;
 }
 
-static int lanes(const flgpu_ctx* c) {
-    const double avg = (double)c->nnz / std::max(1, c->rows) / 2 + 1;
-    if (avg <= 6) return 4;
-    if (avg <= 12) return 8;
-    return 16;
+// Compact lower/upper CSR of A's pattern (positions into A for the values)
+// and their stream row blocks; rebuilt when the sparsity changes.
+static int lu_prepare(flgpu_ctx* c, LearnedState* s) {
+    if (s->lu_key == c->sparsity_gen) return FLGPU_OK;
+    const int n = c->rows;
+    const int32_t* rp = c->h_rowptr.data();
+    const int32_t* cl = c->h_col.data();
+    std::vector<int32_t> lp(n + 1), up(n + 1), lc, uc, ls, us;
+    lc.reserve((size_t)c->nnz / 2 + n);
+    uc.reserve((size_t)c->nnz / 2 + n);
+    ls.reserve((size_t)c->nnz / 2 + n);
+    us.reserve((size_t)c->nnz / 2 + n);
+    for (int i = 0; i < n; i++) {
+        lp[i] = (int32_t)lc.size();
+        up[i] = (int32_t)uc.size();
+        for (int k = rp[i]; k < rp[i + 1]; k++) {
+            if (cl[k] <= i) {
+                lc.push_back(cl[k]);
+                ls.push_back(k);
+            }
+            if (cl[k] >= i) {
+                uc.push_back(cl[k]);
+                us.push_back(k);
+            }
+        }
+    }
+    lp[n] = (int32_t)lc.size();
+    up[n] = (int32_t)uc.size();
+    std::vector<int32_t> rbl, rbu;
+    stream_row_blocks(lp.data(), n, rbl);
+    stream_row_blocks(up.data(), n, rbu);
+    FLGPU_TRY(s->lptr.upload(lp.data(), lp.size(), c->stream));
+    FLGPU_TRY(s->lcol.upload(lc.data(), lc.size(), c->stream));
+    FLGPU_TRY(s->lsrc.upload(ls.data(), ls.size(), c->stream));
+    FLGPU_TRY(s->uptr.upload(up.data(), up.size(), c->stream));
+    FLGPU_TRY(s->ucol.upload(uc.data(), uc.size(), c->stream));
+    FLGPU_TRY(s->usrc.upload(us.data(), us.size(), c->stream));
+    FLGPU_TRY(s->rbl.upload(rbl.data(), rbl.size(), c->stream));
+    FLGPU_TRY(s->rbu.upload(rbu.data(), rbu.size(), c->stream));
+    FLGPU_HIP(hipStreamSynchronize(c->stream));
+    s->nnzl = (int64_t)lc.size();
+    s->nnzu = (int64_t)uc.size();
+    s->nrbl = (int)rbl.size() - 1;
+    s->nrbu = (int)rbu.size() - 1;
+    s->lu_key = c->sparsity_gen;
+    return FLGPU_OK;
 }
 
+template <typename GV>
+static int split_halves(flgpu_ctx* c, LearnedState* s, const GV* gv, DevBuf<GV>& gl, DevBuf<GV>& gu) {
+    FLGPU_TRY(lu_prepare(c, s));
+    FLGPU_TRY(gl.alloc(s->nnzl));
+    FLGPU_TRY(gu.alloc(s->nnzu));
+    hipLaunchKernelGGL((k_gather_half<GV>), dim3(grid_for(s->nnzl, 256)), dim3(256), 0, c->stream, s->nnzl,
+                       s->lsrc.p, gv, gl.p);
+    hipLaunchKernelGGL((k_gather_half<GV>), dim3(grid_for(s->nnzu, 256)), dim3(256), 0, c->stream, s->nnzu,
+                       s->usrc.p, gv, gu.p);
+    FLGPU_HIP(hipGetLastError());
+    return FLGPU_OK;
+}
+
+// y = (lower half) r, then z = (upper half) y with the sums r.z, z.z, z.w
 template <typename V, typename GV>
-static int apply_impl(flgpu_ctx* c, const GV* gv, const V* r, V* z, V* y, const V* w, double* partials,
-                      unsigned* counter, Scal* scal, bool set_norm0) {
-    const int n = c->rows;
-    const int L = lanes(c);
-    const int rows_per_blk = kVecBlock / L;
-    int grid = (int)std::min<int64_t>(kMaxVecGrid, (n + rows_per_blk - 1) / rows_per_blk);
-    grid = std::max(grid, 1);
-    const int* done = &scal->done;
-#define FL_APPLY(LL)                                                                                              \
-    case LL:                                                                                                      \
-        hipLaunchKernelGGL((k_g_lower<LL, V, GV>), dim3(grid), dim3(kVecBlock), 0, c->stream, n, c->rowptr.p,      \
-                           c->col.p, c->diag.p, gv, r, y, done);                                                  \
-        hipLaunchKernelGGL((k_g_upper<LL, V, GV>), dim3(grid), dim3(kVecBlock), 0, c->stream, n, c->rowptr.p,      \
-                           c->col.p, c->diag.p, gv, (const V*)y, z, r, w, partials, counter, scal,                \
-                           set_norm0 ? 1 : 0, set_norm0 ? (const int*)nullptr : done);                            \
-        break;
-    switch (L) {
-        FL_APPLY(4)
-        FL_APPLY(8)
-        FL_APPLY(16)
-    }
-#undef FL_APPLY
+static int apply_impl(flgpu_ctx* c, LearnedState* s, const GV* gl, const GV* gu, const V* r, V* z, V* y,
+                      const V* w, double* partials, int* nparts, const int* done) {
+    const int gl_grid = stream_grid(s->nrbl), gu_grid = stream_grid(s->nrbu);
+    *nparts = gu_grid;
+    hipLaunchKernelGGL((k_csr_stream<V, GV, V, V, DOT_NONE>), dim3(gl_grid), dim3(kRowBlock), 0, c->stream, s->nrbl,
+                       s->rbl.p, s->lptr.p, s->lcol.p, gl, r, y, (const double*)nullptr, (const V*)nullptr,
+                       (const V*)nullptr, partials, done);
+    hipLaunchKernelGGL((k_csr_stream<V, GV, V, V, DOT_PC>), dim3(gu_grid), dim3(kRowBlock), 0, c->stream, s->nrbu,
+                       s->rbu.p, s->uptr.p, s->ucol.p, gu, (const V*)y, z, (const double*)nullptr, r, w, partials,
+                       done);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
 
 template <typename V>
-int learned_precond_apply(flgpu_ctx* c, const V* r, V* z, V* y, const V* w, double* partials, unsigned* counter,
-                          Scal* scal, bool set_norm0) {
+int learned_precond_apply(flgpu_ctx* c, const V* r, V* z, V* y, const V* w, double* partials, int* nparts,
+                          const int* done_flag) {
     LearnedState* s = state(c);
     if (std::is_same<V, float>::value)
-        return apply_impl<V, float>(c, s->gv.p, r, z, y, w, partials, counter, scal, set_norm0);
-    return apply_impl<V, double>(c, s->gv64.p, r, z, y, w, partials, counter, scal, set_norm0);
+        return apply_impl<V, float>(c, s, s->gl.p, s->gu.p, r, z, y, w, partials, nparts, done_flag);
+    return apply_impl<V, double>(c, s, s->gl64.p, s->gu64.p, r, z, y, w, partials, nparts, done_flag);
 }
 template int learned_precond_apply<double>(flgpu_ctx*, const double*, double*, double*, const double*, double*,
-                                           unsigned*, Scal*, bool);
+                                           int*, const int*);
 template int learned_precond_apply<float>(flgpu_ctx*, const float*, float*, float*, const float*, double*,
-                                          unsigned*, Scal*, bool);
+                                          int*, const int*);
 
 void learned_free(flgpu_ctx* c) {
     delete c->ls;
@@ -917,12 +934,8 @@ static int loss_grad(flgpu_ctx* c, int P, uint64_t seed, double* loss_out) {
     FLGPU_TRY(s->dU.alloc((size_t)n * H));
     FLGPU_TRY(s->dHM.alloc((size_t)n * 2 * H));
     FLGPU_TRY(s->dPQ.alloc((size_t)n * 2 * H));
-    FLGPU_TRY(s->lpart.alloc((size_t)kMaxVecGrid * 4));
+    FLGPU_TRY(s->lpart.alloc((size_t)kPartStride * 4));
     FLGPU_TRY(s->lossbuf.alloc(1));
-    if (!s->counter.p) {
-        FLGPU_TRY(s->counter.alloc(1));
-        FLGPU_HIP(hipMemsetAsync(s->counter.p, 0, sizeof(unsigned), c->stream));
-    }
     FLGPU_HIP(hipMemsetAsync(s->grad.p, 0, sizeof(float) * Lay::TOTAL, c->stream));
     const unsigned gnp = grid_for((int64_t)np, 256);
     hipLaunchKernelGGL(k_probes, dim3(gnp), dim3(256), 0, c->stream, n, P, seed, s->z.p);
@@ -934,7 +947,9 @@ static int loss_grad(flgpu_ctx* c, int P, uint64_t seed, double* loss_out) {
                        c->diag.p, (const float*)s->gv.p, (const float*)s->v.p, s->w.p, 2);
     const int lgrid = (int)std::min<int64_t>(kMaxVecGrid, grid_for((int64_t)np, kVecBlock));
     hipLaunchKernelGGL(k_loss, dim3(lgrid), dim3(kVecBlock), 0, c->stream, (int)np, 1.0f / (float)np, s->z.p, s->w.p,
-                       s->dw.p, s->lpart.p, s->counter.p, s->lossbuf.p);
+                       s->dw.p, s->lpart.p);
+    hipLaunchKernelGGL(k_loss_fin, dim3(1), dim3(kVecBlock), 0, c->stream, lgrid, 1.0f / (float)np, s->lpart.p,
+                       s->lossbuf.p);
     // dv = G dw
     hipLaunchKernelGGL((k_batch_spmv<float>), dim3(gnp), dim3(256), 0, c->stream, n, P, c->rowptr.p, c->col.p,
                        c->diag.p, (const float*)s->gv.p, (const float*)s->dw.p, s->dv.p, 1);
@@ -942,8 +957,8 @@ static int loss_grad(flgpu_ctx* c, int P, uint64_t seed, double* loss_out) {
                        s->sval.p, s->dsq.p, s->gv.p, s->u.p, s->v.p, s->dv.p, s->dw.p, s->dphi.p, s->dpsi.p);
     const int ogrid = (int)std::min<int64_t>(kMaxVecGrid, grid_for(n, kVecBlock));
     hipLaunchKernelGGL(k_out_back, dim3(ogrid), dim3(kVecBlock), 0, c->stream, n, c->rowptr.p, c->col.p, c->diag.p,
-                       s->tmap.p, s->sval.p, s->dphi.p, s->dpsi.p, s->theta.p, s->dab.p, s->dh.p, s->lpart.p,
-                       s->counter.p, s->grad.p);
+                       s->tmap.p, s->sval.p, s->dphi.p, s->dpsi.p, s->theta.p, s->dab.p, s->dh.p, s->lpart.p);
+    hipLaunchKernelGGL(k_out_back_fin, dim3(1), dim3(kVecBlock), 0, c->stream, ogrid, s->lpart.p, s->grad.p);
     // output-layer weights: grad[WOA/WOB/WD][c] = sum_n dab[n][q] h_L[n][c], b_d = sum dpsi
     const int chunk = 4096;
     const int nchunk = (n + chunk - 1) / chunk;

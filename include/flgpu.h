@@ -274,9 +274,13 @@ int flgpu_learned_train_step(flgpu_ctx *ctx, int32_t nprobe, uint64_t seed,
 int flgpu_learned_loss_grad(flgpu_ctx *ctx, int32_t nprobe, uint64_t seed,
                             float *loss, float *grad);
 /* Build G for the device-resident system (done by flgpu_solve when
- * pc == FLGPU_PC_LEARNED); values copied out when G_val != NULL (lower
- * pattern, row-major CSR of the lower triangle incl. diagonal). */
-int flgpu_learned_factor(flgpu_ctx *ctx, float *G_val, int64_t *nnz_lower);
+ * pc == FLGPU_PC_LEARNED).  *nnz = nnz of A; with G_val != NULL the factor
+ * is copied out in A's pattern, with D = diag(A)^(-1/2) and Gs the factor
+ * of the scaled matrix D A D: entry (i,j) of the lower triangle holds
+ * Gs_ij D_jj, entry (i,j) of the upper triangle holds D_ii Gs_ji and the
+ * diagonal holds Gs_ii D_ii (so G = Gs D is the lower part with the
+ * diagonal, and G^T the upper part with the diagonal). */
+int flgpu_learned_factor(flgpu_ctx *ctx, float *G_val, int64_t *nnz);
 
 /* ---- measurement --------------------------------------------------------
  * Device-time measurement on the library's own stream with HIP events (no
