@@ -157,9 +157,11 @@ __device__ __forceinline__ int lds_pos(const int32_t* lcol, int lo, int hi, int 
     return -1;
 }
 
+// Five waves per SIMD: the register cap (102) spills a few values of the
+// FP64 element arithmetic, which costs less than the latency it hides.
 template <int DIM, int FEAT>
-__global__ void __launch_bounds__(256)
-k_asm_blocked(const ElemTables* __restrict__ tab, int rows_cap, int nnz_cap, int slot_cap,
+__global__ void __launch_bounds__(256, 5)
+k_asm_blocked(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int nnz_cap, int slot_cap,
               const int32_t* __restrict__ brow, const int32_t* __restrict__ iptr, const int32_t* __restrict__ inc,
               const int32_t* __restrict__ conn, const int32_t* __restrict__ conn_x, const double* __restrict__ X4,
               AsmFields fd, AsmFlags fl, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
@@ -167,7 +169,7 @@ k_asm_blocked(const ElemTables* __restrict__ tab, int rows_cap, int nnz_cap, int
     constexpr int LOC = DIM + 1;
     __shared__ LTab<double> tb;
     extern __shared__ double dyn[];
-    double* sval = dyn;                                            // slot_cap * LOC
+    double* sval = dyn;                                            // LOC * slot_cap, [j][slot]
     double* srhs = sval + (size_t)slot_cap * LOC;                  // slot_cap
     uint32_t* spos = reinterpret_cast<uint32_t*>(srhs + slot_cap); // slot_cap
     int32_t* lcol = reinterpret_cast<int32_t*>(spos + slot_cap);   // nnz_cap
@@ -177,7 +179,13 @@ k_asm_blocked(const ElemTables* __restrict__ tab, int rows_cap, int nnz_cap, int
     uint16_t* erow = srow + slot_cap;                                   // nnz_cap: row of each entry
     stage_tables<double>(tab, tb);
 
-    const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    // Workgroups go to the 8 XCDs in turn (workgroup w to XCD w % 8); each
+    // XCD takes a contiguous run of slabs so that neighbouring slabs, which
+    // share elements and nodes, meet in the same L2.
+    const int per = (nslab + 7) >> 3;
+    const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (b >= nslab) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
     const int r0 = brow[b], r1 = brow[b + 1], R = r1 - r0;
     const int k0 = rowptr[r0], E = rowptr[r1] - k0;
     const int s0 = iptr[r0], S = iptr[r1] - s0;
@@ -217,7 +225,7 @@ k_asm_blocked(const ElemTables* __restrict__ tab, int rows_cap, int nnz_cap, int
             const int q = lds_pos(lcol, lo, hi, e.nodes[j]);
             if (q < 0) atomicOr(err, 1);
             packed |= (uint32_t)(q < 0 ? 255 : q - lo) << (8 * j);
-            sval[(size_t)sl * LOC + j] = vals[j];
+            sval[j * slot_cap + sl] = vals[j];
         }
         spos[sl] = packed;
         srhs[sl] = r;
@@ -229,13 +237,14 @@ k_asm_blocked(const ElemTables* __restrict__ tab, int rows_cap, int nnz_cap, int
         const int l = erow[t];
         const uint32_t p = (uint32_t)(t - lrp[l]);
         double acc = 0.0;
+#pragma unroll 4
         for (int k = soff[l]; k < soff[l + 1]; k++) {
             const uint32_t pk = spos[k];
             int j = -1;
 #pragma unroll
             for (int jj = 0; jj < LOC; jj++)
                 if (((pk >> (8 * jj)) & 255u) == p) j = jj;
-            const double vv = sval[(size_t)k * LOC + (j < 0 ? 0 : j)];
+            const double vv = sval[(j < 0 ? 0 : j) * slot_cap + k];
             acc = acc + (j < 0 ? 0.0 : vv);
         }
         A[k0 + t] = acc;
@@ -257,8 +266,8 @@ int build_block_plan(flgpu_ctx* c) {
     if (c->plan.built) return FLGPU_OK;
     if (!c->linear) return set_error(FLGPU_EUNSUPPORTED, "block plan: linear field mesh only");
     const int N = c->rows, loc = c->loc;
-    // LDS budget per slab: four workgroups per CU
-    int slot_cap = 640;
+    // LDS budget per slab: five workgroups per CU
+    int slot_cap = 448;
     if (const char* s = getenv("FLGPU_BLOCK_SLOTS")) slot_cap = std::max(64, std::min(4096, atoi(s)));
     const int rows_cap = 256, nnz_cap = 1024;
     // node -> element incidence in colour order
@@ -319,7 +328,8 @@ static int launch_blocked_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags
         return set_error(FLGPU_EUNSUPPORTED, "blocked assembly: a slab needs %zu bytes of LDS", lds);
     FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_blocked<DIM, FEAT>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_asm_blocked<DIM, FEAT>), dim3(P.nblocks), dim3(256), lds, c->stream, c->d_tab.p,
+    const int grid = 8 * ((P.nblocks + 7) / 8);
+    hipLaunchKernelGGL((k_asm_blocked<DIM, FEAT>), dim3(grid), dim3(256), lds, c->stream, c->d_tab.p, P.nblocks,
                        P.rows_cap, P.nnz_cap, P.slot_cap, P.brow.p, P.iptr.p, P.inc.p, c->conn.p, c->conn_x.p,
                        c->X4.p, fd, fl, c->rowptr.p, c->col.p, c->A.p, c->rhs.p, d_err);
     FLGPU_HIP(hipGetLastError());

@@ -8,6 +8,7 @@ a C-ordered (nodes, dim) array, which is the same memory).
 """
 from __future__ import annotations
 
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -185,6 +186,50 @@ def square_tris(n: int, jitter: float = 0.0, seed: int = 0) -> Mesh:
     return Mesh(2, conn.shape[0], m * m, nd, xy, 1, m * m, nd, xy, bnd.astype(np.int32), name=f"square{n}")
 
 
+def read_gmsh(path: str) -> Mesh:
+    """P1 mesh from a gmsh 2.2 ASCII file (the format of Fluidity's
+    read_mesh_files(..., format="gmsh")): triangles (type 2) in 2D or
+    tetrahedra (type 4) in 3D, node numbers compacted to 1..N in file order,
+    element vertex order kept.  Boundary nodes are those of facets that
+    belong to one element only.  A 2D mesh is detected by z = 0 everywhere."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    k = lines.index("$Nodes")
+    nn = int(lines[k + 1])
+    ids = np.empty(nn, np.int64)
+    xyz = np.empty((nn, 3))
+    for q in range(nn):
+        t = lines[k + 2 + q].split()
+        ids[q] = int(t[0])
+        xyz[q] = [float(v) for v in t[1:4]]
+    k = lines.index("$Elements")
+    ne = int(lines[k + 1])
+    tris, tets = [], []
+    for q in range(ne):
+        t = [int(v) for v in lines[k + 2 + q].split()]
+        etype, ntags = t[1], t[2]
+        nodes = t[3 + ntags:]
+        if etype == 2:
+            tris.append(nodes)
+        elif etype == 4:
+            tets.append(nodes)
+    dim = 3 if tets else 2
+    cells = np.array(tets if tets else tris, np.int64)
+    remap = {int(v): q + 1 for q, v in enumerate(ids)}
+    conn = np.vectorize(remap.__getitem__)(cells).astype(np.int32)
+    X = np.ascontiguousarray(xyz[:, :dim])
+    loc = dim + 1
+    faces = {}
+    for e in range(conn.shape[0]):
+        for skip in range(loc):
+            fkey = tuple(sorted(int(conn[e, a]) for a in range(loc) if a != skip))
+            faces[fkey] = faces.get(fkey, 0) + 1
+    bnd = sorted({v for fk, cnt in faces.items() if cnt == 1 for v in fk})
+    nd = conn.ravel()
+    name = os.path.splitext(os.path.basename(path))[0]
+    return Mesh(dim, conn.shape[0], nn, nd, X, 1, nn, nd, X, np.array(bnd, np.int32), name=name)
+
+
 def grf_lognormal(pos: np.ndarray, seed: int, corr_len: float = 0.1, contrast: float = 1e4,
                   nfeat: int = 64) -> np.ndarray:
     """exp of a Gaussian random field (random Fourier features, correlation
@@ -273,4 +318,14 @@ def config(name: str, seed: int = 1, scale: int | None = None) -> Config:
         kap = grf_lognormal(m.node_pos, seed, 0.1, 1e3)
         return Config("c5", m, 2, np.zeros(m.nnodes), None, kap, sin_source(m.node_pos), None,
                       1.0, 1.0, 0.0, False, False, "cg", 1e-8)
+    if name == "cavity":
+        # the square-cavity mesh of Fluidity's colouring test
+        # (tests/data/square-cavity-2d.msh), with the rotating-bump
+        # advection-diffusion of the 2D configuration
+        here = os.path.dirname(os.path.abspath(__file__))
+        m = read_gmsh(os.path.join(os.path.dirname(here), "tests", "golden", "square-cavity-2d.msh"))
+        p = m.node_pos
+        T = np.exp(-((p[:, 0] - 0.3) ** 2 + (p[:, 1] - 0.5) ** 2) / (2 * 0.05 ** 2))
+        U = np.stack([-(p[:, 1] - 0.5), p[:, 0] - 0.5], axis=1)
+        return Config("cavity", m, 2, T, U, 1e-3, None, None, 0.01, 0.5, 0.0, True, True, "gmres", 1e-10)
     raise ValueError(name)

@@ -39,7 +39,8 @@ def run_case(cfg, opts_kw=None, precision="fp64", scheme="auto", isotropic=True,
 
 @pytest.mark.parametrize("name,scale,scheme", [("c2", 6, "colour"), ("c2", 6, "blocked"), ("c1", 16, "colour"),
                                                ("c1", 16, "blocked"), ("c3", 3, "colour"), ("c4", 6, "blocked"),
-                                               ("c5", 6, "blocked"), ("c5", 9, "colour")])
+                                               ("c5", 6, "blocked"), ("c5", 9, "colour"),
+                                               ("cavity", None, "colour"), ("cavity", None, "blocked")])
 def test_assembly_matches_oracle(require_gpu, name, scale, scheme):
     cfg = M.config(name, seed=3, scale=scale)
     pr, ref, A, rhs, inact = run_case(cfg, scheme=scheme)
@@ -85,7 +86,7 @@ def test_assembly_p2_fp32(require_gpu):
     pr.close()
 
 
-@pytest.mark.parametrize("name,scale", [("c1", 20), ("c2", 10), ("c5", 12)])
+@pytest.mark.parametrize("name,scale", [("c1", 20), ("c2", 10), ("c5", 12), ("cavity", None)])
 def test_blocked_equals_colour_bitwise(require_gpu, name, scale):
     """Both schemes add each entry's element contributions in colour order:
     the results are identical bit for bit."""

@@ -203,7 +203,7 @@ def _brute_sparsity(mesh):
     return findrm, colm
 
 
-@pytest.mark.parametrize("name,scale", [("c1", 5), ("c2", 3), ("c3", 2)])
+@pytest.mark.parametrize("name,scale", [("c1", 5), ("c2", 3), ("c3", 2), ("cavity", None)])
 def test_sparsity_matches_brute_force(name, scale):
     mesh = M.config(name, seed=1, scale=scale).mesh
     f, c, cen = O.make_sparsity(mesh)
@@ -229,8 +229,12 @@ def _greedy(mesh):
     return colour
 
 
-@pytest.mark.parametrize("name,scale", [("c1", 6), ("c2", 3)])
+@pytest.mark.parametrize("name,scale", [("c1", 6), ("c2", 3), ("cavity", None)])
 def test_colouring_valid_and_greedy(name, scale):
+    """Validity, partition and the degree bound checked by the reference's
+    femtools/tests/test_colouring.F90, on its own mesh (cavity =
+    tests/data/square-cavity-2d.msh) among others; plus the exact colours of
+    the greedy first-fit order."""
     mesh = M.config(name, seed=1, scale=scale).mesh
     colour, ncol = O.colour_elements(mesh)
     assert np.array_equal(colour, _greedy(mesh))
@@ -313,3 +317,22 @@ def test_poisson_second_order_convergence():
         errs.append(np.max(np.abs(T1 - u)))
     rates = [errs[i] / errs[i + 1] for i in range(2)]
     assert all(3.3 < r < 4.7 for r in rates), (errs, rates)
+
+
+def test_reference_tet_fixture_closed_forms():
+    """The single tetrahedron of the reference's tests/data/tet.msh (the unit
+    simplex, vertices listed 3 4 1 2): |T| = 1/6, consistent mass
+    |T|/20 (1 + delta_ij), stiffness |T| G G^T with the barycentric
+    gradients -- exact under the degree-2 rule."""
+    import os
+    m = M.read_gmsh(os.path.join(os.path.dirname(__file__), "golden", "tet.msh"))
+    assert (m.dim, m.nele, m.nnodes) == (3, 1, 4)
+    cfg = M.Config("tet", m, 2, np.zeros(4), None, 1.0, None, None, 1.0, 1.0, 0.0, True, False, "cg", 1e-8)
+    X = m.X[m.ndglno - 1]
+    G, vol = _bary_gradients(X)
+    assert abs(vol - 1.0 / 6.0) < 1e-16
+    asm, p, _ = _setup(cfg, have_mass=True, have_advection=False, source=None)
+    mat, _ = O.element(p, asm, 0)
+    Mref = vol / 20.0 * (np.ones((4, 4)) + np.eye(4))
+    Kref = vol * G @ G.T
+    assert np.allclose(mat, Mref + Kref, rtol=1e-14, atol=1e-16)
