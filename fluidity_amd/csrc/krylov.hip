@@ -39,6 +39,8 @@ struct KrylovWork {
     DevBuf<Scal> scal;       // two slots, see carry_if_done
     int cur = 0;             // slot holding the current state
     DevBuf<int32_t> rb;      // row blocks of the streamed CSR product
+    DevBuf<int16_t> col16;   // int16 column offsets (when use16)
+    bool use16 = false;
     int nrb = 0;
     int64_t rb_key = -1;     // sparsity generation rb was built for
     Scal* h_scal = nullptr;  // pinned
@@ -111,48 +113,82 @@ k_pivot(int nparts, const double* __restrict__ partials, Scal* sc) {
     }
 }
 
-// Thread per row: copy to the reduced values, lift inactive columns, set
-// the ghost pivot, the Jacobi inverse diagonal and the initial x.
-__global__ void __launch_bounds__(kVecBlock)
-k_build(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-        const int32_t* __restrict__ diag, const double* __restrict__ A,
-        const double* __restrict__ rhs, const int32_t* __restrict__ inactive, int have_inactive,
-        const Scal* __restrict__ sc, double* __restrict__ v64, float* __restrict__ v32,
-        double* __restrict__ b, double* __restrict__ dinv, float* __restrict__ dinv32,
-        double* __restrict__ x, int zero_x) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const bool ghost = have_inactive && inactive[i];
+// The reduced (PETSc-side) system, per row block: the block's entries are
+// read once, coalesced, and one thread per row then finishes the row from
+// LDS.  Ghost rows keep only the pivot on the diagonal, inactive columns
+// are dropped and lifted into the rhs -- b_i = rhs_i - sum_k A_ik rhs_k over
+// the inactive columns k, in entry order with the product rounded first,
+// as the reference's csr_mult of the ghost values does -- and the Jacobi
+// inverse diagonal and the initial x are set.
+__global__ void __launch_bounds__(kRowBlock)
+k_build(int nrb, const int32_t* __restrict__ rb, const int32_t* __restrict__ rowptr,
+        const int32_t* __restrict__ col, const double* __restrict__ A, const double* __restrict__ rhs,
+        const int32_t* __restrict__ inactive, int have_inactive, const Scal* __restrict__ sc,
+        double* __restrict__ v64, float* __restrict__ v32, double* __restrict__ b, double* __restrict__ dinv,
+        float* __restrict__ dinv32, double* __restrict__ x, int zero_x) {
+    __shared__ double slift[kStreamNnz];
+    __shared__ double sdiag[kRowBlock];
+    __shared__ uint8_t serow[kStreamNnz];
     const double pivot = sc->pivot;
-    double lift = 0.0, dval = 0.0;
-    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
-        const int cj = col[k];
-        double v = A[k];
-        if (ghost) {
-            v = (cj == i) ? pivot : 0.0;
-        } else if (have_inactive && inactive[cj]) {
-            lift = fma(v, rhs[cj], lift);
-            v = 0.0;
+    const RowBlocks it(nrb);
+    for (int blk = it.b; blk < it.hi; blk += it.step) {
+        const int r0 = rb[blk], r1 = rb[blk + 1];
+        const int k0 = rowptr[r0], k1 = rowptr[r1];
+        const int row = r0 + threadIdx.x;
+        const bool staged = k1 - k0 <= kStreamNnz;
+        if (staged) {
+            if (row < r1) {
+                sdiag[threadIdx.x] = 0.0;
+                for (int k = rowptr[row]; k < rowptr[row + 1]; k++) serow[k - k0] = (uint8_t)threadIdx.x;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kStreamPerThread; j++) {
+                const int k = k0 + threadIdx.x + j * kRowBlock;
+                if (k < k1) {
+                    const int l = serow[k - k0], i = r0 + l, cj = col[k];
+                    const double a = A[k];
+                    const bool ghost = have_inactive && inactive[i];
+                    const bool off = have_inactive && inactive[cj];
+                    double v = a;
+                    if (ghost) v = (cj == i) ? pivot : 0.0;
+                    else if (off) v = 0.0;
+                    v64[k] = v;
+                    if (v32) v32[k] = (float)v;
+                    if (cj == i) sdiag[l] = v;
+                    slift[k - k0] = (!ghost && off) ? a * rhs[cj] : 0.0;
+                }
+            }
+            __syncthreads();
         }
-        if (cj == i) dval = v;
-        v64[k] = v;
-        if (v32) v32[k] = (float)v;
+        if (row < r1) {
+            const bool ghost = have_inactive && inactive[row];
+            double lift = 0.0, dval = 0.0;
+            if (staged) {
+                for (int k = rowptr[row]; k < rowptr[row + 1]; k++) lift = lift + slift[k - k0];
+                dval = sdiag[threadIdx.x];
+            } else {  // a row longer than the staging buffer (alone in its block)
+                for (int k = rowptr[row]; k < rowptr[row + 1]; k++) {
+                    const int cj = col[k];
+                    const double a = A[k];
+                    const bool off = have_inactive && inactive[cj];
+                    double v = a;
+                    if (ghost) v = (cj == row) ? pivot : 0.0;
+                    else if (off) v = 0.0;
+                    v64[k] = v;
+                    if (v32) v32[k] = (float)v;
+                    if (cj == row) dval = v;
+                    if (!ghost && off) lift = lift + a * rhs[cj];
+                }
+            }
+            b[row] = ghost ? 0.0 : rhs[row] - lift;
+            const double di = dval != 0.0 ? 1.0 / dval : 1.0;
+            dinv[row] = di;
+            if (dinv32) dinv32[row] = (float)di;
+            if (zero_x || ghost) x[row] = 0.0;
+        }
+        __syncthreads();
     }
-    b[i] = ghost ? 0.0 : rhs[i] - lift;
-    const double di = dval != 0.0 ? 1.0 / dval : 1.0;
-    dinv[i] = di;
-    if (dinv32) dinv32[i] = (float)di;
-    if (zero_x || ghost) x[i] = 0.0;
-}
-
-// ---------------------------------------------------------------------
-// CSR products: L lanes per row, fixed grid, fixed reduction order
-// ---------------------------------------------------------------------
-
-template <typename T>
-__device__ __forceinline__ T lane_sum(T v, int L) {
-    for (int off = L >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
 }
 
 // ---------------------------------------------------------------------
@@ -387,20 +423,34 @@ static int lanes_for(const flgpu_ctx* c) {
 
 static int spmv_grid(const flgpu_ctx* c) { return stream_grid(c->kw->nrb); }
 
+template <typename AV, typename CT>
+static CsrOp<AV, CT> matrix_op(flgpu_ctx* c, KrylovWork* k, const AV* val, const CT* col) {
+    return CsrOp<AV, CT>{k->nrb, k->rb.p, c->rowptr.p, col, val, nullptr, nullptr};
+}
+
+template <typename V, typename CT, typename XV, typename YV>
+static void spmv_launch(flgpu_ctx* c, KrylovWork* k, const V* val, const CT* col, const XV* x, YV* y,
+                        const double* bsub, int want_dot, const int* done_flag) {
+    const int grid = stream_grid(k->nrb);
+    const CsrOp<V, CT> m = matrix_op<V, CT>(c, k, val, col);
+    if (want_dot)
+        hipLaunchKernelGGL((k_csr_stream<V, V, CT, XV, YV, DOT_XY>), dim3(grid), dim3(kRowBlock), 0, c->stream, m,
+                           x, y, bsub, (const V*)nullptr, (const V*)nullptr, k->part(PART_SPMV), done_flag);
+    else
+        hipLaunchKernelGGL((k_csr_stream<V, V, CT, XV, YV, DOT_NONE>), dim3(grid), dim3(kRowBlock), 0, c->stream,
+                           m, x, y, bsub, (const V*)nullptr, (const V*)nullptr, k->part(PART_SPMV), done_flag);
+}
+
 // y = A x (or y = bsub - A x); with want_dot the per-block sums x.y, y.y
-// go to the PART_SPMV region, spmv_grid(c) of them.
+// go to the PART_SPMV region, spmv_grid(c) of them.  The column indices
+// are the int16 offsets when they fit.
 template <typename V, typename XV, typename YV>
 static int spmv(flgpu_ctx* c, KrylovWork* k, const V* val, const XV* x, YV* y, const double* bsub,
                 int want_dot, const int* done_flag) {
-    const int grid = stream_grid(k->nrb);
-    if (want_dot)
-        hipLaunchKernelGGL((k_csr_stream<V, V, XV, YV, DOT_XY>), dim3(grid), dim3(kRowBlock), 0, c->stream, k->nrb,
-                           k->rb.p, c->rowptr.p, c->col.p, val, x, y, bsub, (const V*)nullptr, (const V*)nullptr,
-                           k->part(PART_SPMV), done_flag);
+    if (k->use16)
+        spmv_launch<V, int16_t, XV, YV>(c, k, val, k->col16.p, x, y, bsub, want_dot, done_flag);
     else
-        hipLaunchKernelGGL((k_csr_stream<V, V, XV, YV, DOT_NONE>), dim3(grid), dim3(kRowBlock), 0, c->stream,
-                           k->nrb, k->rb.p, c->rowptr.p, c->col.p, val, x, y, bsub, (const V*)nullptr,
-                           (const V*)nullptr, k->part(PART_SPMV), done_flag);
+        spmv_launch<V, int32_t, XV, YV>(c, k, val, c->col.p, x, y, bsub, want_dot, done_flag);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
@@ -437,7 +487,11 @@ static int ensure_work(flgpu_ctx* c, bool mixed) {
         std::vector<int32_t> rb;
         stream_row_blocks(c->h_rowptr.data(), n, rb);
         FLGPU_TRY(k->rb.upload(rb.data(), rb.size(), c->stream));
-        FLGPU_HIP(hipStreamSynchronize(c->stream));  // rb is freed on return
+        std::vector<int16_t> c16;
+        k->use16 = compress_columns(c->h_rowptr.data(), c->h_col.data(), rb, c16);
+        if (k->use16) FLGPU_TRY(k->col16.upload(c16.data(), c16.size(), c->stream));
+        else k->col16.release();
+        FLGPU_HIP(hipStreamSynchronize(c->stream));  // rb and c16 are freed on return
         k->nrb = (int)rb.size() - 1;
         k->rb_key = c->sparsity_gen;
     }
@@ -472,10 +526,10 @@ static int build_reduced(flgpu_ctx* c, KrylovWork* k, bool mixed, bool zero_x) {
     hipLaunchKernelGGL(k_diag_minmax, dim3(k->nblk), dim3(kVecBlock), 0, c->stream, n, c->diag.p, c->A.p,
                        k->part(PART_MISC));
     hipLaunchKernelGGL(k_pivot, dim3(1), dim3(64), 0, c->stream, k->nblk, k->part(PART_MISC), k->S());
-    hipLaunchKernelGGL(k_build, dim3(grid_for(n, kVecBlock)), dim3(kVecBlock), 0, c->stream, n, c->rowptr.p,
-                       c->col.p, c->diag.p, c->A.p, c->rhs.p, c->inactive.p, c->have_inactive ? 1 : 0,
-                       k->S(), k->val64.p, mixed ? k->val32.p : nullptr, k->b.p, k->dinv.p,
-                       mixed ? k->dinv32.p : nullptr, k->x.p, zero_x ? 1 : 0);
+    hipLaunchKernelGGL(k_build, dim3(stream_grid(k->nrb)), dim3(kRowBlock), 0, c->stream, k->nrb, k->rb.p,
+                       c->rowptr.p, c->col.p, c->A.p, c->rhs.p, c->inactive.p, c->have_inactive ? 1 : 0, k->S(),
+                       k->val64.p, mixed ? k->val32.p : nullptr, k->b.p, k->dinv.p, mixed ? k->dinv32.p : nullptr,
+                       k->x.p, zero_x ? 1 : 0);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
@@ -673,6 +727,12 @@ static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info*
 }
 
 // ------------------------- GMRES ------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ T lane_sum(T v, int L) {
+    for (int off = L >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
 
 // w = D^-1 (A v)  [left Jacobi]
 template <int L, typename V>

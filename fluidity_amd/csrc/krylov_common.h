@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <vector>
 
 namespace flgpu {
@@ -122,33 +123,71 @@ inline void stream_row_blocks(const int32_t* rowptr, int n, std::vector<int32_t>
 
 enum { DOT_NONE = 0, DOT_XY = 1, DOT_PC = 2 };
 
+// Row blocks handled by this workgroup: with a grid that is a multiple of 8
+// each XCD (the hardware hands consecutive workgroups to the 8 XCDs in
+// turn) takes a contiguous share of the row blocks, so the vector entries
+// and nodes that neighbouring rows gather stay in one L2.
+struct RowBlocks {
+    int b, hi, step;
+    __device__ __forceinline__ explicit RowBlocks(int nrb) {
+        b = blockIdx.x;
+        hi = nrb;
+        step = gridDim.x;
+        if ((gridDim.x & 7) == 0) {
+            const int xcd = blockIdx.x & 7;
+            b = (int)((int64_t)nrb * xcd / 8) + (int)(blockIdx.x >> 3);
+            hi = (int)((int64_t)nrb * (xcd + 1) / 8);
+            step = gridDim.x >> 3;
+        }
+    }
+};
+
+// A CSR operand of k_csr_stream.  Column indices are either int32 or, when
+// every entry of a row block lies within int16 range of the block's first
+// row, int16 offsets from that row (half the index bytes).  Optional
+// scales: y_i = rs_i * sum_k val_k (cs_j x_j) -- a diagonally scaled matrix
+// stored unscaled (low-precision values of order one).
+template <typename AV, typename CT>
+struct CsrOp {
+    int nrb;
+    const int32_t* rb;
+    const int32_t* rowptr;
+    const CT* col;
+    const AV* val;
+    const float* cs;
+    const float* rs;
+};
+
+template <typename CT>
+__device__ __forceinline__ int csr_col(const CT* __restrict__ col, int k, int r0) {
+    if constexpr (sizeof(CT) == 2)
+        return r0 + (int)col[k];
+    else
+        return (int)col[k];
+}
+
 // y = A x (or y = bsub - A x) over row blocks.  A block reads its entries
 // once, coalesced, stages the products val*x[col] in LDS and one thread per
 // row adds them in entry order: the sequential row sum of a plain CSR
 // product, with the product rounded before the addition.  A fixed grid
-// walks the row blocks; with a grid that is a multiple of 8 each XCD (the
-// hardware hands consecutive workgroups to the 8 XCDs in turn) takes a
-// contiguous share, so the x entries its rows gather stay in its own L2.
+// walks the row blocks (RowBlocks).
 // Per-block dot products into partials: DOT_XY: x_i y_i, y_i^2;
 // DOT_PC: r_i y_i, y_i^2, y_i w_i.
-template <typename V, typename AV, typename XV, typename YV, int DOTS>
+template <typename V, typename AV, typename CT, typename XV, typename YV, int DOTS>
 __global__ void __launch_bounds__(kRowBlock)
-k_csr_stream(int nrb, const int32_t* __restrict__ rb, const int32_t* __restrict__ rowptr,
-             const int32_t* __restrict__ col, const AV* __restrict__ val, const XV* __restrict__ x,
-             YV* __restrict__ y, const double* __restrict__ bsub, const V* __restrict__ dr,
-             const V* __restrict__ dw, double* __restrict__ partials, const int* __restrict__ done_flag) {
+k_csr_stream(CsrOp<AV, CT> m, const XV* __restrict__ x, YV* __restrict__ y, const double* __restrict__ bsub,
+             const V* __restrict__ dr, const V* __restrict__ dw, double* __restrict__ partials,
+             const int* __restrict__ done_flag) {
     if (done_flag && *done_flag) return;
     __shared__ V prod[kStreamNnz];
     double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-    int b = blockIdx.x, hi = nrb, step = gridDim.x;
-    if ((gridDim.x & 7) == 0) {
-        const int xcd = blockIdx.x & 7;
-        b = (int)((int64_t)nrb * xcd / 8) + (blockIdx.x >> 3);
-        hi = (int)((int64_t)nrb * (xcd + 1) / 8);
-        step = gridDim.x >> 3;
-    }
-    for (; b < hi; b += step) {
-        const int r0 = rb[b], r1 = rb[b + 1];
+    const int32_t* __restrict__ rowptr = m.rowptr;
+    const CT* __restrict__ col = m.col;
+    const AV* __restrict__ val = m.val;
+    const float* __restrict__ cs = m.cs;
+    const RowBlocks it(m.nrb);
+    for (int b = it.b; b < it.hi; b += it.step) {
+        const int r0 = m.rb[b], r1 = m.rb[b + 1];
         const int k0 = rowptr[r0], k1 = rowptr[r1];
         const int row = r0 + threadIdx.x;
         V acc = (V)0;
@@ -161,7 +200,12 @@ k_csr_stream(int nrb, const int32_t* __restrict__ rb, const int32_t* __restrict_
 #pragma unroll
             for (int j = 0; j < kStreamPerThread; j++) {
                 const int k = k0 + threadIdx.x + j * kRowBlock;
-                if (k < k1) prod[k - k0] = (V)val[k] * (V)x[col[k]];
+                if (k < k1) {
+                    const int c = csr_col(col, k, r0);
+                    V xv = (V)x[c];
+                    if (cs) xv = xv * (V)cs[c];
+                    prod[k - k0] = (V)val[k] * xv;
+                }
             }
             __syncthreads();
             for (int k = kb; k < ke; k++) acc += prod[k];
@@ -169,7 +213,12 @@ k_csr_stream(int nrb, const int32_t* __restrict__ rb, const int32_t* __restrict_
         } else {  // one row longer than the staging buffer: in order, chunk by chunk
             for (int c0 = k0; c0 < k1; c0 += kStreamNnz) {
                 const int c1 = min(k1, c0 + kStreamNnz);
-                for (int k = c0 + threadIdx.x; k < c1; k += kRowBlock) prod[k - c0] = (V)val[k] * (V)x[col[k]];
+                for (int k = c0 + threadIdx.x; k < c1; k += kRowBlock) {
+                    const int c = csr_col(col, k, r0);
+                    V xv = (V)x[c];
+                    if (cs) xv = xv * (V)cs[c];
+                    prod[k - c0] = (V)val[k] * xv;
+                }
                 __syncthreads();
                 if (threadIdx.x == 0)
                     for (int k = 0; k < c1 - c0; k++) acc += prod[k];
@@ -177,6 +226,7 @@ k_csr_stream(int nrb, const int32_t* __restrict__ rb, const int32_t* __restrict_
             }
         }
         if (row < r1) {
+            if (m.rs) acc = acc * (V)m.rs[row];
             double yv = (double)acc;
             if (bsub) yv = bsub[row] - yv;
             const YV yo = (YV)yv;
@@ -201,10 +251,33 @@ k_csr_stream(int nrb, const int32_t* __restrict__ rb, const int32_t* __restrict_
     }
 }
 
+// int16 column offsets from each row block's first row; false (and out
+// untouched) when an offset does not fit.
+inline bool compress_columns(const int32_t* rowptr, const int32_t* col, const std::vector<int32_t>& rb,
+                             std::vector<int16_t>& out) {
+    const int nrb = (int)rb.size() - 1;
+    std::vector<int16_t> c16((size_t)rowptr[rb[nrb]]);
+    for (int b = 0; b < nrb; b++) {
+        const int r0 = rb[b];
+        for (int k = rowptr[r0]; k < rowptr[rb[b + 1]]; k++) {
+            const int d = col[k] - r0;
+            if (d < -32768 || d > 32767) return false;
+            c16[k] = (int16_t)d;
+        }
+    }
+    out.swap(c16);
+    return true;
+}
+
 // Fixed grid for k_csr_stream: a multiple of 8, at most kMaxRowGrid.
 inline int stream_grid(int nrb) {
+    static const int cap = [] {
+        const char* e = getenv("FLGPU_STREAM_GRID");
+        const int v = e ? atoi(e) : kMaxRowGrid;
+        return v >= 8 && v <= kMaxRowGrid ? v : kMaxRowGrid;
+    }();
     if (nrb < 8) return nrb > 0 ? nrb : 1;
-    const int g = nrb < kMaxRowGrid ? nrb : kMaxRowGrid;
+    const int g = nrb < cap ? nrb : cap;
     return g & ~7;
 }
 

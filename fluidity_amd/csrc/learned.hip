@@ -86,9 +86,13 @@ struct LearnedState {
     // so that each product streams only its own half
     int64_t lu_key = -1;
     DevBuf<int32_t> lptr, lcol, lsrc, uptr, ucol, usrc, rbl, rbu;
+    DevBuf<int16_t> lcol16, ucol16;  // int16 column offsets (when they fit)
+    bool l16 = false, u16 = false;
     int nrbl = 0, nrbu = 0;
     int64_t nnzl = 0, nnzu = 0;
-    DevBuf<float> gl, gu;
+    // factor of the scaled matrix, Gs: FP16 for the mixed solve, FP64 for
+    // the FP64 solve; the scaling D is applied by the products
+    DevBuf<_Float16> gl16, gu16;
     DevBuf<double> gl64, gu64;
     // training
     DevBuf<float> z, u, v, w, dv, dw, dphi, dpsi, dab, dh, dHM, dU, dPQ, dM;
@@ -115,8 +119,7 @@ __global__ void k_to_bf16(int n, const float* __restrict__ src, __bf16* __restri
     if (i < n) dst[i] = (__bf16)src[i];
 }
 
-// Graph preparation from the reduced system: d_i, s_ij (scaled entries),
-// node features.
+// Graph preparation from the reduced system: d_i = a_ii^(-1/2).
 template <typename V>
 __global__ void __launch_bounds__(256)
 k_graph_prep(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
@@ -124,29 +127,52 @@ k_graph_prep(int n, const int32_t* __restrict__ rowptr, const int32_t* __restric
              float* __restrict__ sval, float* __restrict__ feat, int pass) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (pass == 0) {
-        const int d = diag[i];
-        const double a = d >= 0 ? (double)val[d] : 1.0;
-        dsq[i] = (float)(a > 0 ? 1.0 / sqrt(a) : 1.0);
-        return;
-    }
+    const int d = diag[i];
+    const double a = d >= 0 ? (double)val[d] : 1.0;
+    dsq[i] = (float)(a > 0 ? 1.0 / sqrt(a) : 1.0);
+}
+
+// The scaled values s_ij = d_i a_ij d_j and the node features (row sum of
+// s, off-diagonal count / 16, largest off-diagonal |s|, no-neighbour flag),
+// half a wave per row: entries read and written coalesced, the row sums
+// formed in entry order.
+template <typename V>
+__global__ void __launch_bounds__(256)
+k_graph_prep_rows(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                  const V* __restrict__ val, const float* __restrict__ dsq, float* __restrict__ sval,
+                  float* __restrict__ feat) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t >> 5, c = t & 31;
+    if (i >= n) return;
     const float di = dsq[i];
     float rowsum = 0.f, maxoff = 0.f;
     int nb = 0;
-    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
-        const int j = col[k];
-        const float s = (float)((double)val[k] * (double)di * (double)dsq[j]);
-        sval[k] = s;
-        rowsum += s;
-        if (j != i && s != 0.f) {
-            nb++;
-            maxoff = fmaxf(maxoff, fabsf(s));
+    const int kb = rowptr[i], ke = rowptr[i + 1];
+    for (int k0 = kb; k0 < ke; k0 += 32) {
+        const int len = min(32, ke - k0);
+        int j = i;
+        float s = 0.f;
+        if (c < len) {
+            j = col[k0 + c];
+            s = (float)((double)val[k0 + c] * (double)di * (double)dsq[j]);
+            sval[k0 + c] = s;
+        }
+        for (int q = 0; q < len; q++) {
+            const int jq = __shfl(j, q, 32);
+            const float sq = __shfl(s, q, 32);
+            rowsum += sq;
+            if (jq != i && sq != 0.f) {
+                nb++;
+                maxoff = fmaxf(maxoff, fabsf(sq));
+            }
         }
     }
-    feat[(size_t)i * NFEAT + 0] = rowsum;
-    feat[(size_t)i * NFEAT + 1] = nb / 16.f;
-    feat[(size_t)i * NFEAT + 2] = maxoff;
-    feat[(size_t)i * NFEAT + 3] = nb == 0 ? 1.f : 0.f;
+    if (c == 0) {
+        feat[(size_t)i * NFEAT + 0] = rowsum;
+        feat[(size_t)i * NFEAT + 1] = nb / 16.f;
+        feat[(size_t)i * NFEAT + 2] = maxoff;
+        feat[(size_t)i * NFEAT + 3] = nb == 0 ? 1.f : 0.f;
+    }
 }
 
 // tmap[k]: position of the transposed entry (col[k], row) in row col[k]
@@ -205,15 +231,25 @@ k_node_gemm(int n, int d1, int d2, const float* __restrict__ X1, const float* __
     f32x16 acc = {};
     const int nr = n0 + row;
     for (int k0 = 0; k0 < K; k0 += 16) {
-        bf16x8 a, b;
-#pragma unroll
-        for (int t = 0; t < 8; t++) {
-            const int k = k0 + 8 * kg + t;
-            float xv = 0.f;
-            if (nr < n) xv = k < d1 ? X1[(size_t)nr * d1 + k] : X2[(size_t)nr * d2 + (k - d1)];
-            a[t] = (__bf16)xv;
-            b[t] = W[(size_t)(o0 + row) * K + k];
+        // 8 consecutive k of one node row (d1, d2 are multiples of 16, so a
+        // run never straddles X1 and X2): two 16-byte loads; 8 weights: one
+        const int k = k0 + 8 * kg;
+        float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0;
+        if (nr < n) {
+            const float* xr = k < d1 ? X1 + (size_t)nr * d1 + k : X2 + (size_t)nr * d2 + (k - d1);
+            p0 = *reinterpret_cast<const float4*>(xr);
+            p1 = *reinterpret_cast<const float4*>(xr + 4);
         }
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(W + (size_t)(o0 + row) * K + k);
+        bf16x8 a;
+        a[0] = (__bf16)p0.x;
+        a[1] = (__bf16)p0.y;
+        a[2] = (__bf16)p0.z;
+        a[3] = (__bf16)p0.w;
+        a[4] = (__bf16)p1.x;
+        a[5] = (__bf16)p1.y;
+        a[6] = (__bf16)p1.z;
+        a[7] = (__bf16)p1.w;
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
 #pragma unroll
@@ -293,22 +329,36 @@ __global__ void k_colsum(int n, int dout, int chunk, const float* __restrict__ d
 // message passing: M_i = sum_{j != i} relu(P_i + Q_j + c1 s + c2 |s| + bm)
 // 32 lanes per row (one per channel)
 // ---------------------------------------------------------------------------
+// Half a wave per row (lane = channel).  The row's columns and values are
+// read once, coalesced, and handed to the lanes with shuffles, so the
+// gathers of Q_j for the row's edges are independent and overlap; the sum
+// keeps the row's entry order.  Rows are dealt to the XCDs in contiguous
+// runs so that neighbouring rows' Q_j meet in one L2 (the grid must be a
+// multiple of 8 for the mapping to cover every row).
 __global__ void __launch_bounds__(256)
 k_edge_agg(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
            const float* __restrict__ sval, const float* __restrict__ PQ, const float* __restrict__ lp,
            float* __restrict__ M) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = t >> 5, c = t & 31;
+    const int per = (int)((gridDim.x + 7) >> 3);
+    const int blk = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    const int i = (blk * 256 + (int)threadIdx.x) >> 5, c = threadIdx.x & 31;
     if (i >= n) return;
     const float pi = PQ[(size_t)i * 2 * H + c];
     const float c1 = lp[Lay::C1 + c], c2 = lp[Lay::C2 + c], bm = lp[Lay::BM + c];
+    const int kb = rowptr[i], ke = rowptr[i + 1];
     float m = 0.f;
-    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
-        const int j = col[k];
-        if (j == i) continue;
-        const float s = sval[k];
-        const float q = PQ[(size_t)j * 2 * H + H + c];
-        m += fmaxf(pi + q + c1 * s + c2 * fabsf(s) + bm, 0.f);
+    for (int k0 = kb; k0 < ke; k0 += 32) {
+        const int len = min(32, ke - k0);
+        const int myj = c < len ? col[k0 + c] : i;
+        const float mys = c < len ? sval[k0 + c] : 0.f;
+#pragma unroll 4
+        for (int t = 0; t < len; t++) {
+            const int j = __shfl(myj, t, 32);
+            const float sv = __shfl(mys, t, 32);
+            if (j == i) continue;
+            const float q = PQ[(size_t)j * 2 * H + H + c];
+            m += fmaxf(pi + q + c1 * sv + c2 * fabsf(sv) + bm, 0.f);
+        }
     }
     M[(size_t)i * H + c] = m;
 }
@@ -335,20 +385,22 @@ __device__ __forceinline__ float gs_lower(float s, float ai, float bj, const flo
 }
 
 // Preconditioner values in A's pattern: lower slots hold Gs_ij d_j, upper
-// slots hold d_i Gs_ji, the diagonal Gs_ii d_i.
+// slots hold d_i Gs_ji, the diagonal Gs_ii d_i; with scaled != 0 the
+// factor of the scaled matrix alone (Gs_ij, Gs_ji, Gs_ii).
 template <typename GV>
 __global__ void __launch_bounds__(256)
 k_build_g(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
           const int32_t* __restrict__ tmap, const float* __restrict__ sval, const float* __restrict__ dsq,
-          const float* __restrict__ ab, const float* __restrict__ th, GV* __restrict__ gv) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+          const float* __restrict__ ab, const float* __restrict__ th, GV* __restrict__ gv, int scaled) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t >> 5, lane = t & 31;  // half a wave per row: coalesced entries
     if (i >= n) return;
-    const float di = dsq[i];
-    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+    const float di = scaled ? 1.f : dsq[i];
+    for (int k = rowptr[i] + lane; k < rowptr[i + 1]; k += 32) {
         const int j = col[k];
         float g;
         if (j < i) {
-            g = gs_lower(sval[k], ab[3 * (size_t)i], ab[3 * (size_t)j + 1], th) * dsq[j];
+            g = gs_lower(sval[k], ab[3 * (size_t)i], ab[3 * (size_t)j + 1], th) * (scaled ? 1.f : dsq[j]);
         } else if (j == i) {
             g = expf(ab[3 * (size_t)i + 2]) * di;
         } else {
@@ -361,11 +413,11 @@ k_build_g(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__
 }
 
 // gathers the factor values of one half from A's pattern
-template <typename GV>
+template <typename GV, typename HV>
 __global__ void k_gather_half(int64_t m, const int32_t* __restrict__ src, const GV* __restrict__ gv,
-                              GV* __restrict__ out) {
+                              HV* __restrict__ out) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < m) out[t] = gv[src[t]];
+    if (t < m) out[t] = (HV)gv[src[t]];
 }
 
 // ---------------------------------------------------------------------------
@@ -736,8 +788,8 @@ static int graph_prepare(flgpu_ctx* c, LearnedState* s, const double* val64) {
     FLGPU_TRY(s->feat.alloc((size_t)n * NFEAT));
     hipLaunchKernelGGL((k_graph_prep<double>), dim3(g), dim3(256), 0, c->stream, n, c->rowptr.p, c->col.p, c->diag.p,
                        val64, s->dsq.p, s->sval.p, s->feat.p, 0);
-    hipLaunchKernelGGL((k_graph_prep<double>), dim3(g), dim3(256), 0, c->stream, n, c->rowptr.p, c->col.p, c->diag.p,
-                       val64, s->dsq.p, s->sval.p, s->feat.p, 1);
+    hipLaunchKernelGGL((k_graph_prep_rows<double>), dim3(grid_for((int64_t)n * 32, 256)), dim3(256), 0, c->stream,
+                       n, c->rowptr.p, c->col.p, val64, s->dsq.p, s->sval.p, s->feat.p);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
@@ -772,7 +824,8 @@ static int forward(flgpu_ctx* c, LearnedState* s, bool keep_all) {
         hipLaunchKernelGGL((k_node_gemm<0>), dim3(gemm_grid(n, 2 * H)), dim3(256), 0, c->stream, n, H, 0, hin,
                            (const float*)nullptr, 2 * H, wl, (const float*)nullptr, (const float*)nullptr, PQ,
                            (float*)nullptr);
-        hipLaunchKernelGGL(k_edge_agg, dim3(grid_for((int64_t)n * 32, 256)), dim3(256), 0, c->stream, n, c->rowptr.p,
+        hipLaunchKernelGGL(k_edge_agg, dim3(8 * ((grid_for((int64_t)n * 32, 256) + 7) / 8)), dim3(256), 0, c->stream, n,
+                           c->rowptr.p,
                            c->col.p, s->sval.p, PQ, lp, M);
         hipLaunchKernelGGL((k_node_gemm<1>), dim3(gemm_grid(n, H)), dim3(256), 0, c->stream, n, H, H, hin,
                            (const float*)M, H, wl + 2 * H * H, lp + Lay::BU, (const float*)hin, hout, U);
@@ -784,10 +837,10 @@ static int forward(flgpu_ctx* c, LearnedState* s, bool keep_all) {
 }
 
 template <typename GV>
-static int build_g(flgpu_ctx* c, LearnedState* s, GV* out) {
+static int build_g(flgpu_ctx* c, LearnedState* s, GV* out, int scaled = 0) {
     const int n = c->rows;
-    hipLaunchKernelGGL((k_build_g<GV>), dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, c->rowptr.p, c->col.p,
-                       s->tmap.p, s->sval.p, s->dsq.p, s->ab.p, s->theta.p, out);
+    hipLaunchKernelGGL((k_build_g<GV>), dim3(grid_for((int64_t)n * 32, 256)), dim3(256), 0, c->stream, n, c->rowptr.p, c->col.p,
+                       s->tmap.p, s->sval.p, s->dsq.p, s->ab.p, s->theta.p, out, scaled);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
@@ -796,8 +849,8 @@ static int build_g(flgpu_ctx* c, LearnedState* s, GV* out) {
 const double* reduced_values(flgpu_ctx* c);
 int reduced_prepare(flgpu_ctx* c);
 
-template <typename GV>
-static int split_halves(flgpu_ctx* c, LearnedState* s, const GV* gv, DevBuf<GV>& gl, DevBuf<GV>& gu);
+template <typename GV, typename HV>
+static int split_halves(flgpu_ctx* c, LearnedState* s, const GV* gv, DevBuf<HV>& gl, DevBuf<HV>& gu);
 
 int learned_prepare(flgpu_ctx* c, bool mixed) {
     if (!c->all_diag) return set_error(FLGPU_EUNSUPPORTED, "learned preconditioner: a row has no diagonal entry");
@@ -807,12 +860,12 @@ int learned_prepare(flgpu_ctx* c, bool mixed) {
     FLGPU_TRY(forward(c, s, false));
     if (mixed) {
         FLGPU_TRY(s->gv.alloc(c->nnz));
-        FLGPU_TRY(build_g<float>(c, s, s->gv.p));
-        FLGPU_TRY(split_halves<float>(c, s, s->gv.p, s->gl, s->gu));
+        FLGPU_TRY(build_g<float>(c, s, s->gv.p, 1));
+        FLGPU_TRY((split_halves<float, _Float16>(c, s, s->gv.p, s->gl16, s->gu16)));
     } else {
         FLGPU_TRY(s->gv64.alloc(c->nnz));
-        FLGPU_TRY(build_g<double>(c, s, s->gv64.p));
-        FLGPU_TRY(split_halves<double>(c, s, s->gv64.p, s->gl64, s->gu64));
+        FLGPU_TRY(build_g<double>(c, s, s->gv64.p, 1));
+        FLGPU_TRY((split_halves<double, double>(c, s, s->gv64.p, s->gl64, s->gu64)));
     }
     FLGPU_HIP(hipStreamSynchronize(c->stream));
     return FLGPU_OK;
@@ -857,6 +910,11 @@ static int lu_prepare(flgpu_ctx* c, LearnedState* s) {
     FLGPU_TRY(s->usrc.upload(us.data(), us.size(), c->stream));
     FLGPU_TRY(s->rbl.upload(rbl.data(), rbl.size(), c->stream));
     FLGPU_TRY(s->rbu.upload(rbu.data(), rbu.size(), c->stream));
+    std::vector<int16_t> l16, u16;
+    s->l16 = compress_columns(lp.data(), lc.data(), rbl, l16);
+    s->u16 = compress_columns(up.data(), uc.data(), rbu, u16);
+    if (s->l16) FLGPU_TRY(s->lcol16.upload(l16.data(), l16.size(), c->stream));
+    if (s->u16) FLGPU_TRY(s->ucol16.upload(u16.data(), u16.size(), c->stream));
     FLGPU_HIP(hipStreamSynchronize(c->stream));
     s->nnzl = (int64_t)lc.size();
     s->nnzu = (int64_t)uc.size();
@@ -866,31 +924,47 @@ static int lu_prepare(flgpu_ctx* c, LearnedState* s) {
     return FLGPU_OK;
 }
 
-template <typename GV>
-static int split_halves(flgpu_ctx* c, LearnedState* s, const GV* gv, DevBuf<GV>& gl, DevBuf<GV>& gu) {
+template <typename GV, typename HV>
+static int split_halves(flgpu_ctx* c, LearnedState* s, const GV* gv, DevBuf<HV>& gl, DevBuf<HV>& gu) {
     FLGPU_TRY(lu_prepare(c, s));
     FLGPU_TRY(gl.alloc(s->nnzl));
     FLGPU_TRY(gu.alloc(s->nnzu));
-    hipLaunchKernelGGL((k_gather_half<GV>), dim3(grid_for(s->nnzl, 256)), dim3(256), 0, c->stream, s->nnzl,
+    hipLaunchKernelGGL((k_gather_half<GV, HV>), dim3(grid_for(s->nnzl, 256)), dim3(256), 0, c->stream, s->nnzl,
                        s->lsrc.p, gv, gl.p);
-    hipLaunchKernelGGL((k_gather_half<GV>), dim3(grid_for(s->nnzu, 256)), dim3(256), 0, c->stream, s->nnzu,
+    hipLaunchKernelGGL((k_gather_half<GV, HV>), dim3(grid_for(s->nnzu, 256)), dim3(256), 0, c->stream, s->nnzu,
                        s->usrc.p, gv, gu.p);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
 
-// y = (lower half) r, then z = (upper half) y with the sums r.z, z.z, z.w
-template <typename V, typename GV>
-static int apply_impl(flgpu_ctx* c, LearnedState* s, const GV* gl, const GV* gu, const V* r, V* z, V* y,
-                      const V* w, double* partials, int* nparts, const int* done) {
+template <typename HV, typename CT>
+static CsrOp<HV, CT> half_op(int nrb, const int32_t* rb, const int32_t* ptr, const CT* col, const HV* val,
+                             const float* cs, const float* rs) {
+    return CsrOp<HV, CT>{nrb, rb, ptr, col, val, cs, rs};
+}
+
+// y = Gs (D r) over the lower half, then z = D (Gs^T y) over the upper half
+// with the sums r.z, z.z, z.w
+template <typename V, typename HV, typename LC, typename UC>
+static void apply_launch(flgpu_ctx* c, LearnedState* s, const HV* gl, const HV* gu, const LC* lcol,
+                         const UC* ucol, const V* r, V* z, V* y, const V* w, double* partials, const int* done) {
     const int gl_grid = stream_grid(s->nrbl), gu_grid = stream_grid(s->nrbu);
-    *nparts = gu_grid;
-    hipLaunchKernelGGL((k_csr_stream<V, GV, V, V, DOT_NONE>), dim3(gl_grid), dim3(kRowBlock), 0, c->stream, s->nrbl,
-                       s->rbl.p, s->lptr.p, s->lcol.p, gl, r, y, (const double*)nullptr, (const V*)nullptr,
-                       (const V*)nullptr, partials, done);
-    hipLaunchKernelGGL((k_csr_stream<V, GV, V, V, DOT_PC>), dim3(gu_grid), dim3(kRowBlock), 0, c->stream, s->nrbu,
-                       s->rbu.p, s->uptr.p, s->ucol.p, gu, (const V*)y, z, (const double*)nullptr, r, w, partials,
-                       done);
+    const auto lo = half_op<HV, LC>(s->nrbl, s->rbl.p, s->lptr.p, lcol, gl, s->dsq.p, nullptr);
+    const auto up = half_op<HV, UC>(s->nrbu, s->rbu.p, s->uptr.p, ucol, gu, nullptr, s->dsq.p);
+    hipLaunchKernelGGL((k_csr_stream<V, HV, LC, V, V, DOT_NONE>), dim3(gl_grid), dim3(kRowBlock), 0, c->stream, lo,
+                       r, y, (const double*)nullptr, (const V*)nullptr, (const V*)nullptr, partials, done);
+    hipLaunchKernelGGL((k_csr_stream<V, HV, UC, V, V, DOT_PC>), dim3(gu_grid), dim3(kRowBlock), 0, c->stream, up,
+                       (const V*)y, z, (const double*)nullptr, r, w, partials, done);
+}
+
+template <typename V, typename HV>
+static int apply_impl(flgpu_ctx* c, LearnedState* s, const HV* gl, const HV* gu, const V* r, V* z, V* y,
+                      const V* w, double* partials, int* nparts, const int* done) {
+    *nparts = stream_grid(s->nrbu);
+    if (s->l16 && s->u16)
+        apply_launch<V, HV, int16_t, int16_t>(c, s, gl, gu, s->lcol16.p, s->ucol16.p, r, z, y, w, partials, done);
+    else
+        apply_launch<V, HV, int32_t, int32_t>(c, s, gl, gu, s->lcol.p, s->ucol.p, r, z, y, w, partials, done);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
@@ -900,7 +974,7 @@ int learned_precond_apply(flgpu_ctx* c, const V* r, V* z, V* y, const V* w, doub
                           const int* done_flag) {
     LearnedState* s = state(c);
     if (std::is_same<V, float>::value)
-        return apply_impl<V, float>(c, s, s->gl.p, s->gu.p, r, z, y, w, partials, nparts, done_flag);
+        return apply_impl<V, _Float16>(c, s, s->gl16.p, s->gu16.p, r, z, y, w, partials, nparts, done_flag);
     return apply_impl<V, double>(c, s, s->gl64.p, s->gu64.p, r, z, y, w, partials, nparts, done_flag);
 }
 template int learned_precond_apply<double>(flgpu_ctx*, const double*, double*, double*, const double*, double*,
@@ -1105,6 +1179,7 @@ int flgpu_learned_factor(flgpu_ctx* c, float* G_val, int64_t* nnz) {
     FLGPU_HIP(hipSetDevice(c->device));
     FLGPU_TRY(reduced_prepare(c));
     FLGPU_TRY(learned_prepare(c, true));
+    FLGPU_TRY(build_g<float>(c, c->ls, c->ls->gv.p, 0));
     if (nnz) *nnz = c->nnz;
     if (G_val) {
         FLGPU_HIP(hipMemcpyAsync(G_val, c->ls->gv.p, sizeof(float) * c->nnz, hipMemcpyDeviceToHost, c->stream));

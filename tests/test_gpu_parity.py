@@ -211,16 +211,18 @@ def test_out_of_pattern_insertion_is_an_error(require_gpu):
     pr.close()
 
 
-def test_csr_mult_bitwise_matches_oracle(require_gpu):
+@pytest.mark.parametrize("n", [20000, 70000])
+def test_csr_mult_bitwise_matches_oracle(require_gpu, n):
     """The streamed CSR product sums each row in entry order with the product
     rounded first, as csr_mult does (femtools/Sparse_Tools.F90:3822-3844):
     the result equals the oracle bit for bit.  The pattern mixes short rows,
-    empty rows and one row longer than a staged block (the chunked path)."""
+    empty rows and one row longer than a staged block (the chunked path);
+    with n = 20000 every column lies within int16 range of its row block
+    (16-bit column offsets), with n = 70000 it does not (int32 columns)."""
     import ctypes as C
     from fluidity_amd import lib as L
 
     rng = np.random.default_rng(5)
-    n = 20000
     lens = rng.integers(1, 30, n)
     lens[17] = 0
     lens[4321] = 5000             # longer than one staged block of entries
