@@ -38,7 +38,7 @@ struct KrylovWork {
     DevBuf<double> partials;
     DevBuf<Scal> scal;       // two slots, see carry_if_done
     int cur = 0;             // slot holding the current state
-    DevBuf<int32_t> rb;      // row blocks of the streamed CSR product
+    DevBuf<int32_t> rb, rbk; // row blocks of the streamed CSR product
     DevBuf<int16_t> col16;   // int16 column offsets (when use16)
     bool use16 = false;
     int nrb = 0;
@@ -425,7 +425,7 @@ static int spmv_grid(const flgpu_ctx* c) { return stream_grid(c->kw->nrb); }
 
 template <typename AV, typename CT>
 static CsrOp<AV, CT> matrix_op(flgpu_ctx* c, KrylovWork* k, const AV* val, const CT* col) {
-    return CsrOp<AV, CT>{k->nrb, k->rb.p, c->rowptr.p, col, val, nullptr, nullptr};
+    return CsrOp<AV, CT>{k->nrb, k->rb.p, k->rbk.p, c->rowptr.p, col, val, nullptr, nullptr};
 }
 
 template <typename V, typename CT, typename XV, typename YV>
@@ -484,9 +484,10 @@ static int ensure_work(flgpu_ctx* c, bool mixed) {
     }
     FLGPU_TRY(k->partials.reserve((size_t)kPartStride * PART_COMPONENTS));
     if (k->rb_key != c->sparsity_gen) {
-        std::vector<int32_t> rb;
-        stream_row_blocks(c->h_rowptr.data(), n, rb);
+        std::vector<int32_t> rb, rbk;
+        stream_row_blocks(c->h_rowptr.data(), n, rb, rbk);
         FLGPU_TRY(k->rb.upload(rb.data(), rb.size(), c->stream));
+        FLGPU_TRY(k->rbk.upload(rbk.data(), rbk.size(), c->stream));
         std::vector<int16_t> c16;
         k->use16 = compress_columns(c->h_rowptr.data(), c->h_col.data(), rb, c16);
         if (k->use16) FLGPU_TRY(k->col16.upload(c16.data(), c16.size(), c->stream));

@@ -101,13 +101,14 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ partials
 // ---------------------------------------------------------------------
 // Streamed CSR products
 // ---------------------------------------------------------------------
-constexpr int kStreamPerThread = 8;
+constexpr int kStreamPerThread = 4;
 constexpr int kStreamNnz = kRowBlock * kStreamPerThread;  // entries staged per row block
 
 // Row blocks for k_csr_stream: runs of consecutive rows holding at most
 // kStreamNnz entries and at most kRowBlock rows; a longer row is a block of
-// its own.  rb has nblocks + 1 entries.
-inline void stream_row_blocks(const int32_t* rowptr, int n, std::vector<int32_t>& rb) {
+// its own.  rb has nblocks + 1 entries, rbk the matching first entries.
+inline void stream_row_blocks(const int32_t* rowptr, int n, std::vector<int32_t>& rb,
+                              std::vector<int32_t>& rbk) {
     rb.clear();
     rb.push_back(0);
     int r = 0;
@@ -119,6 +120,8 @@ inline void stream_row_blocks(const int32_t* rowptr, int n, std::vector<int32_t>
         rb.push_back(r1);
         r = r1;
     }
+    rbk.resize(rb.size());
+    for (size_t q = 0; q < rb.size(); q++) rbk[q] = rowptr[rb[q]];
 }
 
 enum { DOT_NONE = 0, DOT_XY = 1, DOT_PC = 2 };
@@ -150,7 +153,8 @@ struct RowBlocks {
 template <typename AV, typename CT>
 struct CsrOp {
     int nrb;
-    const int32_t* rb;
+    const int32_t* rb;     // nrb+1 first rows of the row blocks
+    const int32_t* rbk;    // nrb+1 first entries of the row blocks (rowptr[rb[b]])
     const int32_t* rowptr;
     const CT* col;
     const AV* val;
@@ -186,9 +190,25 @@ k_csr_stream(CsrOp<AV, CT> m, const XV* __restrict__ x, YV* __restrict__ y, cons
     const AV* __restrict__ val = m.val;
     const float* __restrict__ cs = m.cs;
     const RowBlocks it(m.nrb);
+    // The bounds and column indices of the next row block are loaded while
+    // the current block's values and x entries are gathered, so a row block
+    // costs one round trip to memory instead of three.
+    int r0 = 0, r1 = 0, k0 = 0, k1 = 0, c[kStreamPerThread];
+    auto fetch = [&](int bb, int& a0, int& a1, int& e0, int& e1, int (&cc)[kStreamPerThread]) {
+        a0 = m.rb[bb];
+        a1 = m.rb[bb + 1];
+        e0 = m.rbk[bb];
+        e1 = m.rbk[bb + 1];
+#pragma unroll
+        for (int j = 0; j < kStreamPerThread; j++) {
+            const int k = e0 + (int)threadIdx.x + j * kRowBlock;
+            cc[j] = (k < e1 && e1 - e0 <= kStreamNnz) ? csr_col(col, k, a0) : 0;
+        }
+    };
+    if (it.b < it.hi) fetch(it.b, r0, r1, k0, k1, c);
     for (int b = it.b; b < it.hi; b += it.step) {
-        const int r0 = m.rb[b], r1 = m.rb[b + 1];
-        const int k0 = rowptr[r0], k1 = rowptr[r1];
+        int n0 = 0, n1 = 0, e0 = 0, e1 = 0, nc[kStreamPerThread];
+        if (b + it.step < it.hi) fetch(b + it.step, n0, n1, e0, e1, nc);
         const int row = r0 + threadIdx.x;
         V acc = (V)0;
         if (k1 - k0 <= kStreamNnz) {
@@ -197,15 +217,21 @@ k_csr_stream(CsrOp<AV, CT> m, const XV* __restrict__ x, YV* __restrict__ y, cons
                 kb = rowptr[row] - k0;
                 ke = rowptr[row + 1] - k0;
             }
+            V pv[kStreamPerThread];
 #pragma unroll
             for (int j = 0; j < kStreamPerThread; j++) {
-                const int k = k0 + threadIdx.x + j * kRowBlock;
+                const int k = k0 + (int)threadIdx.x + j * kRowBlock;
+                pv[j] = (V)0;
                 if (k < k1) {
-                    const int c = csr_col(col, k, r0);
-                    V xv = (V)x[c];
-                    if (cs) xv = xv * (V)cs[c];
-                    prod[k - k0] = (V)val[k] * xv;
+                    V xv = (V)x[c[j]];
+                    if (cs) xv = xv * (V)cs[c[j]];
+                    pv[j] = (V)val[k] * xv;
                 }
+            }
+#pragma unroll
+            for (int j = 0; j < kStreamPerThread; j++) {
+                const int k = k0 + (int)threadIdx.x + j * kRowBlock;
+                if (k < k1) prod[k - k0] = pv[j];
             }
             __syncthreads();
             for (int k = kb; k < ke; k++) acc += prod[k];
@@ -214,9 +240,9 @@ k_csr_stream(CsrOp<AV, CT> m, const XV* __restrict__ x, YV* __restrict__ y, cons
             for (int c0 = k0; c0 < k1; c0 += kStreamNnz) {
                 const int c1 = min(k1, c0 + kStreamNnz);
                 for (int k = c0 + threadIdx.x; k < c1; k += kRowBlock) {
-                    const int c = csr_col(col, k, r0);
-                    V xv = (V)x[c];
-                    if (cs) xv = xv * (V)cs[c];
+                    const int cj = csr_col(col, k, r0);
+                    V xv = (V)x[cj];
+                    if (cs) xv = xv * (V)cs[cj];
                     prod[k - c0] = (V)val[k] * xv;
                 }
                 __syncthreads();
@@ -241,6 +267,12 @@ k_csr_stream(CsrOp<AV, CT> m, const XV* __restrict__ x, YV* __restrict__ y, cons
                 d2 += z * (double)dw[row];
             }
         }
+        r0 = n0;
+        r1 = n1;
+        k0 = e0;
+        k1 = e1;
+#pragma unroll
+        for (int j = 0; j < kStreamPerThread; j++) c[j] = nc[j];
     }
     if (DOTS == DOT_XY) {
         double v[2] = {d0, d1};

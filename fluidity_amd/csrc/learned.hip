@@ -85,7 +85,7 @@ struct LearnedState {
     // diagonal) and the upper part (j >= i) as two compact CSR matrices,
     // so that each product streams only its own half
     int64_t lu_key = -1;
-    DevBuf<int32_t> lptr, lcol, lsrc, uptr, ucol, usrc, rbl, rbu;
+    DevBuf<int32_t> lptr, lcol, lsrc, uptr, ucol, usrc, rbl, rbu, rbkl, rbku;
     DevBuf<int16_t> lcol16, ucol16;  // int16 column offsets (when they fit)
     bool l16 = false, u16 = false;
     int nrbl = 0, nrbu = 0;
@@ -899,9 +899,11 @@ static int lu_prepare(flgpu_ctx* c, LearnedState* s) {
     }
     lp[n] = (int32_t)lc.size();
     up[n] = (int32_t)uc.size();
-    std::vector<int32_t> rbl, rbu;
-    stream_row_blocks(lp.data(), n, rbl);
-    stream_row_blocks(up.data(), n, rbu);
+    std::vector<int32_t> rbl, rbu, rbkl, rbku;
+    stream_row_blocks(lp.data(), n, rbl, rbkl);
+    stream_row_blocks(up.data(), n, rbu, rbku);
+    FLGPU_TRY(s->rbkl.upload(rbkl.data(), rbkl.size(), c->stream));
+    FLGPU_TRY(s->rbku.upload(rbku.data(), rbku.size(), c->stream));
     FLGPU_TRY(s->lptr.upload(lp.data(), lp.size(), c->stream));
     FLGPU_TRY(s->lcol.upload(lc.data(), lc.size(), c->stream));
     FLGPU_TRY(s->lsrc.upload(ls.data(), ls.size(), c->stream));
@@ -938,9 +940,9 @@ static int split_halves(flgpu_ctx* c, LearnedState* s, const GV* gv, DevBuf<HV>&
 }
 
 template <typename HV, typename CT>
-static CsrOp<HV, CT> half_op(int nrb, const int32_t* rb, const int32_t* ptr, const CT* col, const HV* val,
-                             const float* cs, const float* rs) {
-    return CsrOp<HV, CT>{nrb, rb, ptr, col, val, cs, rs};
+static CsrOp<HV, CT> half_op(int nrb, const int32_t* rb, const int32_t* rbk, const int32_t* ptr, const CT* col,
+                             const HV* val, const float* cs, const float* rs) {
+    return CsrOp<HV, CT>{nrb, rb, rbk, ptr, col, val, cs, rs};
 }
 
 // y = Gs (D r) over the lower half, then z = D (Gs^T y) over the upper half
@@ -949,8 +951,8 @@ template <typename V, typename HV, typename LC, typename UC>
 static void apply_launch(flgpu_ctx* c, LearnedState* s, const HV* gl, const HV* gu, const LC* lcol,
                          const UC* ucol, const V* r, V* z, V* y, const V* w, double* partials, const int* done) {
     const int gl_grid = stream_grid(s->nrbl), gu_grid = stream_grid(s->nrbu);
-    const auto lo = half_op<HV, LC>(s->nrbl, s->rbl.p, s->lptr.p, lcol, gl, s->dsq.p, nullptr);
-    const auto up = half_op<HV, UC>(s->nrbu, s->rbu.p, s->uptr.p, ucol, gu, nullptr, s->dsq.p);
+    const auto lo = half_op<HV, LC>(s->nrbl, s->rbl.p, s->rbkl.p, s->lptr.p, lcol, gl, s->dsq.p, nullptr);
+    const auto up = half_op<HV, UC>(s->nrbu, s->rbu.p, s->rbku.p, s->uptr.p, ucol, gu, nullptr, s->dsq.p);
     hipLaunchKernelGGL((k_csr_stream<V, HV, LC, V, V, DOT_NONE>), dim3(gl_grid), dim3(kRowBlock), 0, c->stream, lo,
                        r, y, (const double*)nullptr, (const V*)nullptr, (const V*)nullptr, partials, done);
     hipLaunchKernelGGL((k_csr_stream<V, HV, UC, V, V, DOT_PC>), dim3(gu_grid), dim3(kRowBlock), 0, c->stream, up,
