@@ -217,7 +217,7 @@ k_asm_blocked(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int n
         GeomLin<DIM, LOC, double> gm;
         geom_linear<DIM, LOC, double>(e, tb, gm);
         double vals[LOC], r;
-        row_linear_g<DIM, LOC, double, FEAT>(e, gm, tb, fl, fd, i, vals, r);
+        row_linear_g<DIM, LOC, double, FEAT>(e, gm, tb, fl, GlobalNodal<DIM, double>{fd, e.nodes}, i, vals, r);
         const int lo = lrp[l], hi = lrp[l + 1];
         uint32_t packed = 0;
 #pragma unroll
@@ -256,6 +256,155 @@ k_asm_blocked(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int n
     }
 }
 
+// Nodal field values from the slab's LDS copy (constant fields from their
+// one global value); diffusivity must be isotropic (component 0 staged).
+template <int DIM, typename R>
+struct StagedNodal {
+    const AsmFields& fd;
+    const int* lid;
+    const double *sU, *sK, *sS, *sB;
+    int ncap;
+    __device__ __forceinline__ R U(int k, int d) const {
+        return fd.u_c ? (R)fd.U[d] : (R)sU[d * ncap + lid[k]];
+    }
+    __device__ __forceinline__ R K(int k, int comp) const { return fd.k_c ? (R)fd.K[comp] : (R)sK[lid[k]]; }
+    __device__ __forceinline__ R S(int k) const { return fd.s_c ? (R)fd.S[0] : (R)sS[lid[k]]; }
+    __device__ __forceinline__ R B(int k) const { return fd.b_c ? (R)fd.B[0] : (R)sB[lid[k]]; }
+};
+
+// The blocked scheme with the slab's nodes staged in LDS: the slab's node
+// list (its rows and every node of their elements) is read once, the
+// coordinates and nodal fields of those nodes are gathered once into LDS,
+// and each slot names its element's vertices by slab-local number, so the
+// element rows read no global memory but their slot word.  Same slots,
+// same order, same bits as k_asm_blocked.
+template <int DIM, int FEAT>
+__global__ void __launch_bounds__(256, 5)
+k_asm_staged(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int nnz_cap, int slot_cap, int node_cap,
+             const int32_t* __restrict__ brow, const int32_t* __restrict__ iptr, const int32_t* __restrict__ nptr,
+             const int32_t* __restrict__ nodes, const int32_t* __restrict__ p0s, const uint32_t* __restrict__ slotw,
+             const double* __restrict__ X4, AsmFields fd, AsmFlags fl, int stage_u, int stage_k, int stage_s,
+             int stage_b, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, double* __restrict__ A,
+             double* __restrict__ rhs, int* __restrict__ err) {
+    constexpr int LOC = DIM + 1;
+    __shared__ LTab<double> tb;
+    extern __shared__ double dyn[];
+    double* sval = dyn;                                            // LOC * slot_cap, [j][slot]
+    double* srhs = sval + (size_t)slot_cap * LOC;                  // slot_cap
+    double* sX = srhs + slot_cap;                                  // DIM * node_cap
+    double* sT = sX + (size_t)DIM * node_cap;                      // node_cap
+    double* sU = sT + node_cap;                                    // DIM * node_cap (stage_u)
+    double* sK = sU + (stage_u ? (size_t)DIM * node_cap : 0);      // node_cap (stage_k)
+    double* sS = sK + (stage_k ? node_cap : 0);                    // node_cap (stage_s)
+    double* sB = sS + (stage_s ? node_cap : 0);                    // node_cap (stage_b)
+    uint32_t* spos = reinterpret_cast<uint32_t*>(sB + (stage_b ? node_cap : 0));  // slot_cap
+    int32_t* lcol = reinterpret_cast<int32_t*>(spos + slot_cap);   // nnz_cap
+    int32_t* lrp = lcol + nnz_cap;                                 // rows_cap + 1
+    int32_t* soff = lrp + rows_cap + 1;                            // rows_cap + 1
+    int32_t* sgid = soff + rows_cap + 1;                           // node_cap
+    uint16_t* srow = reinterpret_cast<uint16_t*>(sgid + node_cap); // slot_cap
+    uint16_t* erow = srow + slot_cap;                              // nnz_cap
+    stage_tables<double>(tab, tb);
+
+    const int per = (nslab + 7) >> 3;
+    const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (b >= nslab) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int r0 = brow[b], r1 = brow[b + 1], R = r1 - r0;
+    const int k0 = rowptr[r0], E = rowptr[r1] - k0;
+    const int s0 = iptr[r0], S = iptr[r1] - s0;
+    const int n0 = nptr[b], NN = nptr[b + 1] - n0, pr = p0s[b];
+    for (int l = tid; l <= R; l += nt) {
+        lrp[l] = rowptr[r0 + l] - k0;
+        soff[l] = iptr[r0 + l] - s0;
+    }
+    for (int k = tid; k < E; k += nt) lcol[k] = col[k0 + k];
+    for (int q = tid; q < NN; q += nt) {
+        const int g = nodes[n0 + q];
+        sgid[q] = g;
+#pragma unroll
+        for (int a = 0; a < DIM; a++) sX[a * node_cap + q] = X4[(int64_t)g * 4 + a];
+        sT[q] = fd.T[g];
+        if (stage_u)
+            for (int a = 0; a < DIM; a++) sU[a * node_cap + q] = fd.U[(int64_t)g * DIM + a];
+        if (stage_k) sK[q] = fd.K[(int64_t)g * DIM * DIM];
+        if (stage_s) sS[q] = fd.S[g];
+        if (stage_b) sB[q] = fd.B[g];
+    }
+    __syncthreads();
+    for (int l = tid; l < R; l += nt) {
+        for (int k = soff[l]; k < soff[l + 1]; k++) srow[k] = (uint16_t)l;
+        for (int k = lrp[l]; k < lrp[l + 1]; k++) erow[k] = (uint16_t)l;
+    }
+    __syncthreads();
+
+    // phase A: one element row per slot, from LDS
+    for (int sl = tid; sl < S; sl += nt) {
+        __asm__ __volatile__("" ::: "memory");
+        const uint32_t w = slotw[s0 + sl];
+        const int l = srow[sl];
+        int lid[LOC];
+        Elem<DIM, LOC, double> e;
+        int i = 0;
+#pragma unroll
+        for (int k = 0; k < LOC; k++) {
+            lid[k] = (int)((w >> (8 * k)) & 255u);
+            e.nodes[k] = sgid[lid[k]];
+            if (lid[k] == pr + l) i = k;
+#pragma unroll
+            for (int a = 0; a < DIM; a++) e.X[a][k] = sX[a * node_cap + lid[k]];
+            e.T[k] = sT[lid[k]];
+        }
+        GeomLin<DIM, LOC, double> gm;
+        geom_linear<DIM, LOC, double>(e, tb, gm);
+        double vals[LOC], r;
+        row_linear_g<DIM, LOC, double, FEAT>(e, gm, tb, fl,
+                                              StagedNodal<DIM, double>{fd, lid, sU, sK, sS, sB, node_cap}, i, vals,
+                                              r);
+        const int lo = lrp[l], hi = lrp[l + 1];
+        uint32_t packed = 0;
+#pragma unroll
+        for (int j = 0; j < LOC; j++) {
+            const int q = lds_pos(lcol, lo, hi, e.nodes[j]);
+            if (q < 0) atomicOr(err, 1);
+            packed |= (uint32_t)(q < 0 ? 255 : q - lo) << (8 * j);
+            sval[j * slot_cap + sl] = vals[j];
+        }
+        spos[sl] = packed;
+        srhs[sl] = r;
+    }
+    __syncthreads();
+
+    // phase B: one thread per entry, contributions in slot (= colour) order
+    for (int t = tid; t < E; t += nt) {
+        const int l = erow[t];
+        const uint32_t p = (uint32_t)(t - lrp[l]);
+        double acc = 0.0;
+#pragma unroll 4
+        for (int k = soff[l]; k < soff[l + 1]; k++) {
+            const uint32_t pk = spos[k];
+            int j = -1;
+#pragma unroll
+            for (int jj = 0; jj < LOC; jj++)
+                if (((pk >> (8 * jj)) & 255u) == p) j = jj;
+            const double vv = sval[(j < 0 ? 0 : j) * slot_cap + k];
+            acc = acc + (j < 0 ? 0.0 : vv);
+        }
+        A[k0 + t] = acc;
+    }
+    for (int l = tid; l < R; l += nt) {
+        double acc = 0.0;
+        for (int k = soff[l]; k < soff[l + 1]; k++) acc = acc + srhs[k];
+        rhs[r0 + l] = acc;
+    }
+}
+
+static size_t staged_lds(int dim, int loc, int rows_cap, int nnz_cap, int slot_cap, int node_cap, int nfield) {
+    return sizeof(double) * ((size_t)slot_cap * (loc + 1) + (size_t)node_cap * (dim + 1 + nfield)) +
+           sizeof(uint32_t) * (size_t)slot_cap + sizeof(int32_t) * ((size_t)nnz_cap + 2 * (size_t)(rows_cap + 1)) +
+           sizeof(int32_t) * (size_t)node_cap + sizeof(uint16_t) * ((size_t)slot_cap + nnz_cap);
+}
+
 static size_t blocked_lds(int loc, int rows_cap, int nnz_cap, int slot_cap) {
     return sizeof(double) * (size_t)slot_cap * (loc + 1) + sizeof(uint32_t) * (size_t)slot_cap +
            sizeof(int32_t) * ((size_t)nnz_cap + 2 * (size_t)(rows_cap + 1)) +
@@ -292,21 +441,88 @@ int build_block_plan(flgpu_ctx* c) {
             for (int i = 0; i < loc; i++) inc[fill[c->h_conn[(int64_t)e * loc + i]]++] = e;
         }
     }
-    // slabs of consecutive rows bounded by rows, entries and slots
-    std::vector<int32_t> brow(1, 0);
-    int max_rows = 0, max_nnz = 0, max_slots = 0;
+    // Node staging needs the field nodes to be the coordinate nodes and each
+    // slab's node list to hold at most 256 nodes (8-bit local numbers).
+    const bool stage = c->h_conn == c->h_conn_x && getenv("FLGPU_NO_STAGING") == nullptr;
+    const int node_cap_max = 256;
+    // slabs of consecutive rows bounded by rows, entries, slots (and nodes)
+    std::vector<int32_t> brow(1, 0), nptr(1, 0), nodes, p0v;
+    std::vector<uint32_t> slotw;
+    std::vector<int32_t> mark(stage ? N : 0, -1), fresh;
+    int max_rows = 0, max_nnz = 0, max_slots = 0, max_nodes = 0;
     for (int r = 0; r < N;) {
-        int r1 = r;
+        const int slab = (int)brow.size() - 1;
+        int r1 = r, nn = 0;
         while (r1 < N && r1 - r < rows_cap && c->h_rowptr[r1 + 1] - c->h_rowptr[r] <= nnz_cap &&
-               iptr[r1 + 1] - iptr[r] <= slot_cap)
+               iptr[r1 + 1] - iptr[r] <= slot_cap) {
+            if (stage) {  // nodes this row would add to the slab (the row's own node included)
+                fresh.clear();
+                if (mark[r1] != slab && mark[r1] != -2 - slab) {
+                    mark[r1] = -2 - slab;
+                    fresh.push_back(r1);
+                }
+                for (int q = iptr[r1]; q < iptr[r1 + 1]; q++)
+                    for (int v = 0; v < loc; v++) {
+                        const int g = c->h_conn[(int64_t)inc[q] * loc + v];
+                        if (mark[g] != slab && mark[g] != -2 - slab) {
+                            mark[g] = -2 - slab;
+                            fresh.push_back(g);
+                        }
+                    }
+                if (nn + (int)fresh.size() > node_cap_max) {
+                    for (int g : fresh) mark[g] = -1;
+                    if (r1 == r) return set_error(FLGPU_EUNSUPPORTED, "block plan: row %d touches > 256 nodes", r + 1);
+                    break;
+                }
+                for (int g : fresh) mark[g] = slab;
+                nn += (int)fresh.size();
+            }
             r1++;
+        }
         max_rows = std::max(max_rows, r1 - r);
         max_nnz = std::max(max_nnz, c->h_rowptr[r1] - c->h_rowptr[r]);
         max_slots = std::max(max_slots, iptr[r1] - iptr[r]);
+        if (stage) {
+            // the slab's node list, ascending (its rows form one run in it),
+            // and the slots' local vertex numbers
+            const size_t first = nodes.size();
+            for (int rr = r; rr < r1; rr++)
+                if (mark[rr] == slab) {
+                    mark[rr] = -3 - slab;
+                    nodes.push_back(rr);
+                }
+            for (int q = iptr[r]; q < iptr[r1]; q++)
+                for (int v = 0; v < loc; v++) {
+                    const int g = c->h_conn[(int64_t)inc[q] * loc + v];
+                    if (mark[g] == slab) {
+                        mark[g] = -3 - slab;
+                        nodes.push_back(g);
+                    }
+                }
+            std::sort(nodes.begin() + first, nodes.end());
+            for (size_t q = first; q < nodes.size(); q++) mark[nodes[q]] = (int)(q - first);  // local number
+            p0v.push_back(mark[r]);
+            for (int q = iptr[r]; q < iptr[r1]; q++) {
+                uint32_t w = 0;
+                for (int v = 0; v < loc; v++) w |= (uint32_t)mark[c->h_conn[(int64_t)inc[q] * loc + v]] << (8 * v);
+                slotw.push_back(w);
+            }
+            for (size_t q = first; q < nodes.size(); q++) mark[nodes[q]] = -1;
+            max_nodes = std::max(max_nodes, (int)(nodes.size() - first));
+            nptr.push_back((int32_t)nodes.size());
+        }
         brow.push_back(r1);
         r = r1;
     }
     BlockPlan& P = c->plan;
+    P.staged = stage;
+    if (stage) {
+        FLGPU_TRY(P.nptr.upload(nptr.data(), nptr.size(), c->stream));
+        FLGPU_TRY(P.nodes.upload(nodes.data(), nodes.size(), c->stream));
+        FLGPU_TRY(P.p0.upload(p0v.data(), p0v.size(), c->stream));
+        FLGPU_TRY(P.slotw.upload(slotw.data(), slotw.size(), c->stream));
+        P.node_cap = max_nodes;
+    }
     FLGPU_TRY(P.brow.upload(brow.data(), brow.size(), c->stream));
     FLGPU_TRY(P.iptr.upload(iptr.data(), iptr.size(), c->stream));
     FLGPU_TRY(P.inc.upload(inc.data(), inc.size(), c->stream));
@@ -336,12 +552,43 @@ static int launch_blocked_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags
     return FLGPU_OK;
 }
 
-// The smallest compiled feature set that holds the terms switched on.
+template <int DIM, int FEAT>
+static int launch_staged_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
+    const BlockPlan& P = c->plan;
+    constexpr int LOC = DIM + 1;
+    const int su = (FEAT & F_ADV) && fl.adv && !fd.u_c, sk = (FEAT & F_DIFF) && fl.diff && !fd.k_c;
+    const int ss = (FEAT & F_SRC) && fl.src && !fd.s_c, sb = (FEAT & F_ABS) && fl.absorp && !fd.b_c;
+    const int nfield = su * DIM + sk + ss + sb;
+    const size_t lds = staged_lds(DIM, LOC, P.rows_cap, P.nnz_cap, P.slot_cap, P.node_cap, nfield);
+    if (lds + sizeof(LTab<double>) + 64 > 160 * 1024)
+        return set_error(FLGPU_EUNSUPPORTED, "blocked assembly: a slab needs %zu bytes of LDS", lds);
+    FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_staged<DIM, FEAT>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int grid = 8 * ((P.nblocks + 7) / 8);
+    hipLaunchKernelGGL((k_asm_staged<DIM, FEAT>), dim3(grid), dim3(256), lds, c->stream, c->d_tab.p, P.nblocks,
+                       P.rows_cap, P.nnz_cap, P.slot_cap, P.node_cap, P.brow.p, P.iptr.p, P.nptr.p, P.nodes.p,
+                       P.p0.p, P.slotw.p, c->X4.p, fd, fl, su, sk, ss, sb, c->rowptr.p, c->col.p, c->A.p, c->rhs.p,
+                       d_err);
+    FLGPU_HIP(hipGetLastError());
+    return FLGPU_OK;
+}
+
+// The smallest compiled feature set that holds the terms switched on; the
+// node-staged kernel when the plan has node lists and the diffusivity is
+// isotropic (or constant).
 template <int DIM>
 static int launch_blocked(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
     const int m = (fl.have_mass ? F_MASS : 0) | (fl.adv ? F_ADV : 0) | (fl.absorp ? F_ABS : 0) |
                   (fl.diff ? F_DIFF : 0) | (fl.src ? F_SRC : 0);
     auto fits = [m](int f) { return (m & ~f) == 0; };
+    const bool staged = c->plan.staged && (!fl.diff || fl.iso || fd.k_c);
+    if (staged) {
+        if (fits(F_DIFF | F_SRC)) return launch_staged_feat<DIM, F_DIFF | F_SRC>(c, fd, fl, d_err);
+        if (fits(F_MASS | F_ADV | F_DIFF)) return launch_staged_feat<DIM, F_MASS | F_ADV | F_DIFF>(c, fd, fl, d_err);
+        if (fits(F_MASS | F_ADV | F_DIFF | F_SRC))
+            return launch_staged_feat<DIM, F_MASS | F_ADV | F_DIFF | F_SRC>(c, fd, fl, d_err);
+        return launch_staged_feat<DIM, F_ALL>(c, fd, fl, d_err);
+    }
     if (fits(F_DIFF | F_SRC)) return launch_blocked_feat<DIM, F_DIFF | F_SRC>(c, fd, fl, d_err);
     if (fits(F_MASS | F_ADV | F_DIFF)) return launch_blocked_feat<DIM, F_MASS | F_ADV | F_DIFF>(c, fd, fl, d_err);
     if (fits(F_MASS | F_ADV | F_DIFF | F_SRC))

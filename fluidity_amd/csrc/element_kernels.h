@@ -120,6 +120,19 @@ FLGPU_HD_INLINE R fld(const double* __restrict__ f, int c, int node, int comp, i
     return (R)(c ? f[comp] : f[(int64_t)node * ncomp + comp]);
 }
 
+// Nodal field values of an element's vertices, read from the global field
+// arrays (constant fields hold one value).  A kernel that stages its nodes'
+// data elsewhere passes its own accessor with the same four methods.
+template <int DIM, typename R>
+struct GlobalNodal {
+    const AsmFields& fd;
+    const int* nodes;
+    FLGPU_HD_INLINE R U(int k, int d) const { return fld<R>(fd.U, fd.u_c, nodes[k], d, DIM); }
+    FLGPU_HD_INLINE R K(int k, int comp) const { return fld<R>(fd.K, fd.k_c, nodes[k], comp, DIM * DIM); }
+    FLGPU_HD_INLINE R S(int k) const { return fld<R>(fd.S, fd.s_c, nodes[k], 0, 1); }
+    FLGPU_HD_INLINE R B(int k) const { return fld<R>(fd.B, fd.b_c, nodes[k], 0, 1); }
+};
+
 // Geometry of a linear element: physical gradients G[n][d] = sum_k
 // dn(n,k) invJ(d,k) (femtools/Transform_elements.F90:777-780) and |detJ|.
 template <int DIM, int LOC, typename R>
@@ -142,9 +155,9 @@ FLGPU_HD_INLINE void geom_linear(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, 
 
 // Row i of the element matrix (vals[j], j = 0..LOC-1) and of the element
 // rhs, for linear T (gradients constant over the element).
-template <int DIM, int LOC, typename R, int FEAT = F_ALL>
+template <int DIM, int LOC, typename R, int FEAT = F_ALL, class NA = GlobalNodal<DIM, R>>
 FLGPU_HD_INLINE void row_linear_g(const Elem<DIM, LOC, R>& e, const GeomLin<DIM, LOC, R>& gm, const LTab<R>& tb,
-                                  const AsmFlags& fl_in, const AsmFields& fd, int i, R (&vals)[LOC], R& rhs_i) {
+                                  const AsmFlags& fl_in, const NA& na, int i, R (&vals)[LOC], R& rhs_i) {
     AsmFlags fl = fl_in;
     fl.have_mass = (FEAT & F_MASS) ? fl.have_mass : 0;
     fl.adv = (FEAT & F_ADV) ? fl.adv : 0;
@@ -181,7 +194,7 @@ FLGPU_HD_INLINE void row_linear_g(const Elem<DIM, LOC, R>& e, const GeomLin<DIM,
 #pragma unroll
         for (int k = 0; k < LOC; k++)
 #pragma unroll
-            for (int d = 0; d < DIM; d++) u[k][d] = fld<R>(fd.U, fd.u_c, e.nodes[k], d, DIM);
+            for (int d = 0; d < DIM; d++) u[k][d] = na.U(k, d);
         R div = (R)0;
         if (fl.need_div) {
 #pragma unroll
@@ -229,7 +242,7 @@ FLGPU_HD_INLINE void row_linear_g(const Elem<DIM, LOC, R>& e, const GeomLin<DIM,
     if (fl.absorp) {
         R sg[LOC];
 #pragma unroll
-        for (int k = 0; k < LOC; k++) sg[k] = fld<R>(fd.B, fd.b_c, e.nodes[k], 0, 1);
+        for (int k = 0; k < LOC; k++) sg[k] = na.B(k);
 #pragma unroll
         for (int j = 0; j < LOC; j++) {
             R s = (R)0;
@@ -243,7 +256,7 @@ FLGPU_HD_INLINE void row_linear_g(const Elem<DIM, LOC, R>& e, const GeomLin<DIM,
             R kw = (R)0;
 #pragma unroll
             for (int k = 0; k < LOC; k++)
-                kw = FMA(tb.what[k], fld<R>(fd.K, fd.k_c, e.nodes[k], 0, DIM * DIM), kw);
+                kw = FMA(tb.what[k], na.K(k, 0), kw);
             const R scale = adet * kw;
 #pragma unroll
             for (int j = 0; j < LOC; j++) {
@@ -261,7 +274,7 @@ FLGPU_HD_INLINE void row_linear_g(const Elem<DIM, LOC, R>& e, const GeomLin<DIM,
                     R s = (R)0;
 #pragma unroll
                     for (int k = 0; k < LOC; k++)
-                        s = FMA(tb.what[k], fld<R>(fd.K, fd.k_c, e.nodes[k], a + DIM * b, DIM * DIM), s);
+                        s = FMA(tb.what[k], na.K(k, a + DIM * b), s);
                     kb[a][b] = s;
                 }
             R v[DIM];
@@ -294,7 +307,7 @@ FLGPU_HD_INLINE void row_linear_g(const Elem<DIM, LOC, R>& e, const GeomLin<DIM,
     if (fl.src) {
         R s = (R)0;
 #pragma unroll
-        for (int k = 0; k < LOC; k++) s = FMA(tb.mhat[i + LOC * k], fld<R>(fd.S, fd.s_c, e.nodes[k], 0, 1), s);
+        for (int k = 0; k < LOC; k++) s = FMA(tb.mhat[i + LOC * k], na.S(k), s);
         r = FMA(adet, s, r);
     }
     rhs_i = r;
@@ -321,7 +334,7 @@ FLGPU_HD_INLINE void row_linear(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, c
                                 const AsmFields& fd, int i, R (&vals)[LOC], R& rhs_i) {
     GeomLin<DIM, LOC, R> gm;
     geom_linear<DIM, LOC, R>(e, tb, gm);
-    row_linear_g<DIM, LOC, R>(e, gm, tb, fl, fd, i, vals, rhs_i);
+    row_linear_g<DIM, LOC, R>(e, gm, tb, fl, GlobalNodal<DIM, R>{fd, e.nodes}, i, vals, rhs_i);
 }
 
 // Row i for quadratic T: per-quadrature-point loops

@@ -105,6 +105,15 @@ struct BlockPlan {
     DevBuf<int32_t> inc;         // elements of every node, in colour order (ascending id in a colour)
     int32_t nblocks = 0, rows_cap = 0, nnz_cap = 0, slot_cap = 0;
     bool built = false;
+    // node-staged slabs (field mesh = coordinate mesh): every slab's nodes
+    // (its rows and their neighbours) are loaded into LDS once; slots name
+    // their element's vertices by slab-local node number
+    bool staged = false;
+    DevBuf<int32_t> nptr;        // nblocks+1 offsets into nodes
+    DevBuf<int32_t> nodes;       // global node ids of each slab, ascending
+    DevBuf<int32_t> p0;          // position of each slab's first row in its node list
+    DevBuf<uint32_t> slotw;      // per slot: 4 x 8-bit slab-local vertex numbers
+    int32_t node_cap = 0;
 };
 
 struct KrylovWork;   // krylov.hip

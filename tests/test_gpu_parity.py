@@ -102,6 +102,23 @@ def test_blocked_equals_colour_bitwise(require_gpu, name, scale):
     pr.close()
 
 
+@pytest.mark.parametrize("name,scale", [("c2", 8), ("c1", 12)])
+def test_blocked_without_node_staging_equals_colour(require_gpu, name, scale, monkeypatch):
+    """The blocked scheme's gather form (used when the field mesh is not the
+    coordinate mesh or the diffusivity is anisotropic) gives the same bits."""
+    monkeypatch.setenv("FLGPU_NO_STAGING", "1")
+    cfg = M.config(name, seed=14, scale=scale)
+    pr = cg.problem_from_config(cfg)
+    out = {}
+    for scheme in ("colour", "blocked"):
+        print(f"\n[case] {name} scale {scale} scheme {scheme} (no staging)", flush=True)
+        cg.assemble_advection_diffusion_cg(pr, cg.options_from_config(cfg, scheme=scheme), cfg.dt)
+        out[scheme] = pr.matrix()
+    assert np.array_equal(out["colour"][0], out["blocked"][0])
+    assert np.array_equal(out["colour"][1], out["blocked"][1])
+    pr.close()
+
+
 def test_assembly_deterministic(require_gpu):
     cfg = M.config("c2", seed=7, scale=8)
     pr = cg.problem_from_config(cfg)
