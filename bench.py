@@ -51,6 +51,62 @@ def spmv_bytes(n: int, nnz: int, value_bytes: int = 8) -> int:
     return (value_bytes + 4) * nnz + 4 * (n + 1) + (value_bytes * 2) * n
 
 
+def measured_traffic(kernel_prefixes) -> int | None:
+    """HBM bytes per dispatch of the first listed kernel whose name starts
+    with one of kernel_prefixes, from the committed rocprofv3 counter summary
+    (profiles/hbm_traffic_c5.txt: FETCH_SIZE and WRITE_SIZE per dispatch,
+    scaled by a 1 GiB copy); None when the summary does not list it."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_c5.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split(None, 4)
+        if len(f) == 5 and f[0].isdigit() and f[4].startswith(tuple(kernel_prefixes)):
+            return int(f[3])
+    return None
+
+
+def c3_mixed_report(log_each) -> dict:
+    """The P2 advection-diffusion configuration (40^3 x 6 P2 tets, 11-point
+    rule): FP32 element arithmetic against FP64 (A and b, norm-wise relative
+    error), and the mixed-precision GMRES solution against the FP64 one, both
+    to a 1e-8 true relative residual.  The FP64 GPU assembly is itself equal
+    to the oracle to 1e-12 (tests/test_gpu_parity.py)."""
+    from fluidity_amd import cg, meshgen as M
+
+    cfg = M.config("c3", seed=1)
+    pr = cg.problem_from_config(cfg)
+    out = {"elements": cfg.mesh.nele, "nodes": cfg.mesh.nnodes, "nnz": pr.sparsity.nnz}
+    mats = {}
+    for prec in ("fp64", "fp32"):
+        t0 = time.perf_counter()
+        cg.assemble_advection_diffusion_cg(pr, cg.options_from_config(cfg, precision=prec), cfg.dt)
+        mats[prec] = pr.matrix()
+        out[f"assembly_{prec}_seconds"] = round(time.perf_counter() - t0, 4)
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+    out["A_rel_err_fp32_vs_fp64"] = rel(mats["fp32"][0], mats["fp64"][0])
+    out["b_rel_err_fp32_vs_fp64"] = rel(mats["fp32"][1], mats["fp64"][1])
+    xs = {}
+    for label, prec in (("fp64", "fp64"), ("mixed", "mixed")):
+        so = cg.SolverOptions("gmres", "jacobi", relative_error=1e-8, restart=30, max_iterations=20000,
+                              precision=prec)
+        cg.assemble_advection_diffusion_cg(pr, cg.options_from_config(cfg, precision="fp64"), cfg.dt)
+        cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
+        t0 = time.perf_counter()
+        x, info = cg.petsc_solve(pr, so)
+        xs[label] = x
+        out[f"gmres_{label}"] = {"seconds": round(time.perf_counter() - t0, 4), "its": info.its,
+                                 "reason": info.reason, "true_relres": info.true_relres}
+    out["x_rel_err_mixed_vs_fp64"] = rel(xs["mixed"], xs["fp64"])
+    pr.close()
+    log_each(f"c3 mixed: A err {out['A_rel_err_fp32_vs_fp64']:.2e}, b err {out['b_rel_err_fp32_vs_fp64']:.2e}, "
+             f"x err {out['x_rel_err_mixed_vs_fp64']:.2e}")
+    return out
+
+
 def cpu_baseline(cfg, budget_s: float = 20.0) -> dict:
     """The oracle (C restatement, OpenMP over the colour loop like the
     reference) timed on this host: the assembly of the whole mesh and a
@@ -84,6 +140,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--scheme", default="auto")
+    ap.add_argument("--no-c3", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (sharding the path "
@@ -139,8 +196,8 @@ def main() -> int:
                    "nodes": cfg.mesh.nnodes, "nnz": nnz, "colours": pr.ncolours,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                     "kernel": "assembly (all launches of one step)", "algorithmic_bytes": B},
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(("void flgpu::k_asm_staged", "void flgpu::k_asm_blocked")),
+                     "kernel": "k_asm_staged / k_asm_blocked (one launch per step)", "algorithmic_bytes": B},
     }
     if schemes:
         out["assembly_other_schemes"] = schemes
@@ -187,6 +244,8 @@ def main() -> int:
             log(f"spmv {prec}: {ms.value:.4f} ms, {gbs:.1f} GB/s")
         out["solve"] = solves
 
+    if not args.no_c3:
+        out["c3_mixed_precision"] = c3_mixed_report(log)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg)
     pr.close()

@@ -415,8 +415,8 @@ int build_block_plan(flgpu_ctx* c) {
     if (c->plan.built) return FLGPU_OK;
     if (!c->linear) return set_error(FLGPU_EUNSUPPORTED, "block plan: linear field mesh only");
     const int N = c->rows, loc = c->loc;
-    // LDS budget per slab: five workgroups per CU
-    int slot_cap = 448;
+    // LDS budget per slab: four to five workgroups per CU
+    int slot_cap = 384;
     if (const char* s = getenv("FLGPU_BLOCK_SLOTS")) slot_cap = std::max(64, std::min(4096, atoi(s)));
     const int rows_cap = 256, nnz_cap = 1024;
     // node -> element incidence in colour order
