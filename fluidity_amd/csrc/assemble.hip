@@ -43,6 +43,27 @@ __device__ void stage_tables(const ElemTables* __restrict__ t, LTab<R>& s) {
     }
 }
 
+// The linear-element tables into LDS (dn at the first quadrature point,
+// re-indexed for ngi = 1).
+template <typename R>
+__device__ void stage_tables_lin(const ElemTables* __restrict__ t, LTabLin<R>& s) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int loc = t->loc, ngi = t->ngi;
+    for (int k = tid; k < loc * kMaxDim && k < 4 * kMaxDim; k += nt) {
+        const int nn = k % loc, d = k / loc;
+        s.dn[nn + loc * d] = (R)t->dn[nn + loc * (0 + ngi * d)];
+    }
+    for (int k = tid; k < 4; k += nt) s.what[k] = (R)t->what[k];
+    for (int k = tid; k < 16; k += nt) s.mhat[k] = (R)t->mhat[k];
+    for (int k = tid; k < 64; k += nt) s.that[k] = (R)t->that[k];
+    for (int k = tid; k < 4 * kMaxDim; k += nt) s.xdn[k] = (R)t->xdn[k];
+    if (tid == 0) {
+        s.loc = loc;
+        s.ngi = 1;
+        s.lx = t->loc_x;
+    }
+}
+
 template <int DIM, int LOC, typename R>
 __device__ __forceinline__ void gather_elem(int ele, const int32_t* __restrict__ conn,
                                             const int32_t* __restrict__ conn_x,
@@ -287,7 +308,7 @@ k_asm_staged(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int nn
              int stage_b, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, double* __restrict__ A,
              double* __restrict__ rhs, int* __restrict__ err) {
     constexpr int LOC = DIM + 1;
-    __shared__ LTab<double> tb;
+    __shared__ LTabLin<double> tb;
     extern __shared__ double dyn[];
     double* sval = dyn;                                            // LOC * slot_cap, [j][slot]
     double* srhs = sval + (size_t)slot_cap * LOC;                  // slot_cap
@@ -302,9 +323,10 @@ k_asm_staged(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int nn
     int32_t* lrp = lcol + nnz_cap;                                 // rows_cap + 1
     int32_t* soff = lrp + rows_cap + 1;                            // rows_cap + 1
     int32_t* sgid = soff + rows_cap + 1;                           // node_cap
-    uint16_t* srow = reinterpret_cast<uint16_t*>(sgid + node_cap); // slot_cap
+    uint32_t* sw = reinterpret_cast<uint32_t*>(sgid + node_cap);   // slot_cap: slot words
+    uint16_t* srow = reinterpret_cast<uint16_t*>(sw + slot_cap);   // slot_cap
     uint16_t* erow = srow + slot_cap;                              // nnz_cap
-    stage_tables<double>(tab, tb);
+    stage_tables_lin<double>(tab, tb);
 
     const int per = (nslab + 7) >> 3;
     const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
@@ -319,6 +341,7 @@ k_asm_staged(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int nn
         soff[l] = iptr[r0 + l] - s0;
     }
     for (int k = tid; k < E; k += nt) lcol[k] = col[k0 + k];
+    for (int k = tid; k < S; k += nt) sw[k] = slotw[s0 + k];
     for (int q = tid; q < NN; q += nt) {
         const int g = nodes[n0 + q];
         sgid[q] = g;
@@ -341,7 +364,7 @@ k_asm_staged(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int nn
     // phase A: one element row per slot, from LDS
     for (int sl = tid; sl < S; sl += nt) {
         __asm__ __volatile__("" ::: "memory");
-        const uint32_t w = slotw[s0 + sl];
+        const uint32_t w = sw[sl];
         const int l = srow[sl];
         int lid[LOC];
         Elem<DIM, LOC, double> e;
@@ -356,11 +379,10 @@ k_asm_staged(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int nn
             e.T[k] = sT[lid[k]];
         }
         GeomLin<DIM, LOC, double> gm;
-        geom_linear<DIM, LOC, double>(e, tb, gm);
+        geom_linear<DIM, LOC, double, LTabLin<double>>(e, tb, gm);
         double vals[LOC], r;
-        row_linear_g<DIM, LOC, double, FEAT>(e, gm, tb, fl,
-                                              StagedNodal<DIM, double>{fd, lid, sU, sK, sS, sB, node_cap}, i, vals,
-                                              r);
+        row_linear_g<DIM, LOC, double, FEAT, StagedNodal<DIM, double>, LTabLin<double>>(
+            e, gm, tb, fl, StagedNodal<DIM, double>{fd, lid, sU, sK, sS, sB, node_cap}, i, vals, r);
         const int lo = lrp[l], hi = lrp[l + 1];
         uint32_t packed = 0;
 #pragma unroll
@@ -402,7 +424,8 @@ k_asm_staged(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int nn
 static size_t staged_lds(int dim, int loc, int rows_cap, int nnz_cap, int slot_cap, int node_cap, int nfield) {
     return sizeof(double) * ((size_t)slot_cap * (loc + 1) + (size_t)node_cap * (dim + 1 + nfield)) +
            sizeof(uint32_t) * (size_t)slot_cap + sizeof(int32_t) * ((size_t)nnz_cap + 2 * (size_t)(rows_cap + 1)) +
-           sizeof(int32_t) * (size_t)node_cap + sizeof(uint16_t) * ((size_t)slot_cap + nnz_cap);
+           sizeof(int32_t) * (size_t)node_cap + sizeof(uint32_t) * (size_t)slot_cap +
+           sizeof(uint16_t) * ((size_t)slot_cap + nnz_cap);
 }
 
 static size_t blocked_lds(int loc, int rows_cap, int nnz_cap, int slot_cap) {
@@ -560,7 +583,7 @@ static int launch_staged_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags&
     const int ss = (FEAT & F_SRC) && fl.src && !fd.s_c, sb = (FEAT & F_ABS) && fl.absorp && !fd.b_c;
     const int nfield = su * DIM + sk + ss + sb;
     const size_t lds = staged_lds(DIM, LOC, P.rows_cap, P.nnz_cap, P.slot_cap, P.node_cap, nfield);
-    if (lds + sizeof(LTab<double>) + 64 > 160 * 1024)
+    if (lds + sizeof(LTabLin<double>) + 64 > 160 * 1024)
         return set_error(FLGPU_EUNSUPPORTED, "blocked assembly: a slab needs %zu bytes of LDS", lds);
     FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_staged<DIM, FEAT>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

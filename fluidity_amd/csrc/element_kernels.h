@@ -66,6 +66,19 @@ struct LTab {
     int loc, ngi, lx;
 };
 
+// The tables a linear element needs (contracted mass tables, coordinate
+// derivatives and the shape derivatives at the first quadrature point, with
+// ngi = 1 so that dn indexes like LTab's), a fraction of LTab's size.
+template <typename R>
+struct LTabLin {
+    R dn[4 * kElemMaxDim];
+    R what[4];
+    R mhat[16];
+    R that[64];
+    R xdn[4 * kElemMaxDim];
+    int loc, ngi, lx;
+};
+
 // Gathered element data.
 template <int DIM, int LOC, typename R>
 struct Elem {
@@ -76,9 +89,8 @@ struct Elem {
 
 // Jacobian of the linear coordinate element, its cofactor inverse and
 // determinant (construct_cache, femtools/Transform_elements.F90:263-304).
-template <int DIM, typename R>
-FLGPU_HD_INLINE void geometry(const R (&X)[DIM][DIM + 1], const LTab<R>& tb,
-                                         R (&iJ)[DIM][DIM], R& adet) {
+template <int DIM, typename R, class TB = LTab<R>>
+FLGPU_HD_INLINE void geometry(const R (&X)[DIM][DIM + 1], const TB& tb, R (&iJ)[DIM][DIM], R& adet) {
     R JT[DIM][DIM];
 #pragma unroll
     for (int a = 0; a < DIM; a++)
@@ -141,10 +153,10 @@ struct GeomLin {
     R adet;
 };
 
-template <int DIM, int LOC, typename R>
-FLGPU_HD_INLINE void geom_linear(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, GeomLin<DIM, LOC, R>& gm) {
+template <int DIM, int LOC, typename R, class TB = LTab<R>>
+FLGPU_HD_INLINE void geom_linear(const Elem<DIM, LOC, R>& e, const TB& tb, GeomLin<DIM, LOC, R>& gm) {
     R iJ[DIM][DIM];
-    geometry<DIM, R>(e.X, tb, iJ, gm.adet);
+    geometry<DIM, R, TB>(e.X, tb, iJ, gm.adet);
     for (int nn = 0; nn < LOC; nn++)
         for (int d = 0; d < DIM; d++) {
             R s = (R)0;
@@ -155,8 +167,8 @@ FLGPU_HD_INLINE void geom_linear(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, 
 
 // Row i of the element matrix (vals[j], j = 0..LOC-1) and of the element
 // rhs, for linear T (gradients constant over the element).
-template <int DIM, int LOC, typename R, int FEAT = F_ALL, class NA = GlobalNodal<DIM, R>>
-FLGPU_HD_INLINE void row_linear_g(const Elem<DIM, LOC, R>& e, const GeomLin<DIM, LOC, R>& gm, const LTab<R>& tb,
+template <int DIM, int LOC, typename R, int FEAT = F_ALL, class NA = GlobalNodal<DIM, R>, class TB = LTab<R>>
+FLGPU_HD_INLINE void row_linear_g(const Elem<DIM, LOC, R>& e, const GeomLin<DIM, LOC, R>& gm, const TB& tb,
                                   const AsmFlags& fl_in, const NA& na, int i, R (&vals)[LOC], R& rhs_i) {
     AsmFlags fl = fl_in;
     fl.have_mass = (FEAT & F_MASS) ? fl.have_mass : 0;

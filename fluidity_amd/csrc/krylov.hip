@@ -33,8 +33,8 @@ namespace flgpu {
 struct KrylovWork {
     int n = 0;
     int64_t nnz = 0;
-    DevBuf<double> val64, b, x, r, z, p, w, dinv, y;
-    DevBuf<float> val32, r32, z32, p32, w32, dinv32, y32;
+    DevBuf<double> val64, b, x, r, z, p, w, dinv, y, rt;
+    DevBuf<float> val32, r32, z32, p32, w32, dinv32, y32, rt32;
     DevBuf<double> partials;
     DevBuf<Scal> scal;       // two slots, see carry_if_done
     int cur = 0;             // slot holding the current state
@@ -238,11 +238,13 @@ __device__ __forceinline__ bool carry_if_done(const Scal* __restrict__ in, Scal*
 }
 
 // pAp from the SpMV partials, alpha = rz / pAp; x += alpha p;
-// r -= alpha w; [Jacobi: z = D^-1 r]; partials r.z, r.r, z.z, z.w
+// r -= alpha w; [Jacobi: z = D^-1 r]; [learned: rt = D_s r, the residual in
+// the preconditioner's scaled space]; partials r.z, r.r, z.z, z.w
 template <typename V>
 __global__ void __launch_bounds__(kVecBlock)
 k_cg_update(int n, double* __restrict__ x, const V* __restrict__ p, V* __restrict__ r,
             const V* __restrict__ w, V* __restrict__ z, const V* __restrict__ dinv, int jacobi,
+            const float* __restrict__ dsc, V* __restrict__ rt,
             const double* __restrict__ pspmv, int nspmv, double* __restrict__ pvec,
             const Scal* __restrict__ in, Scal* __restrict__ out) {
     __shared__ double s_alpha;
@@ -274,6 +276,7 @@ k_cg_update(int n, double* __restrict__ x, const V* __restrict__ p, V* __restric
         x[i] = fma(alpha, (double)pi, x[i]);
         const V ri = r[i] - a * wi;
         r[i] = ri;
+        if (dsc) rt[i] = ri * (V)dsc[i];
         if (jacobi) {
             const V zi = dinv[i] * ri;
             z[i] = zi;
@@ -333,11 +336,13 @@ k_copy(int n, const V* __restrict__ a, V* __restrict__ b) {
 template <typename V>
 __global__ void __launch_bounds__(kVecBlock)
 k_restart_vecs(int n, const double* __restrict__ r64, V* __restrict__ r, V* __restrict__ z,
-               const V* __restrict__ dinv, int jacobi, double* __restrict__ pvec) {
+               const V* __restrict__ dinv, int jacobi, const float* __restrict__ dsc, V* __restrict__ rt,
+               double* __restrict__ pvec) {
     double s_rz = 0.0, s_rr = 0.0, s_zz = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const V ri = (V)r64[i];
         r[i] = ri;
+        if (dsc) rt[i] = ri * (V)dsc[i];
         s_rr += (double)ri * (double)ri;
         if (jacobi) {
             const V zi = dinv[i] * ri;
@@ -473,7 +478,9 @@ static int ensure_work(flgpu_ctx* c, bool mixed) {
     FLGPU_TRY(k->w.alloc(n));
     FLGPU_TRY(k->dinv.alloc(n));
     FLGPU_TRY(k->y.alloc(n));
+    FLGPU_TRY(k->rt.alloc(n));
     if (mixed) {
+        FLGPU_TRY(k->rt32.alloc(n));
         FLGPU_TRY(k->val32.alloc(c->nnz));
         FLGPU_TRY(k->r32.alloc(n));
         FLGPU_TRY(k->z32.alloc(n));
@@ -550,7 +557,7 @@ static int norm_b(flgpu_ctx* c, KrylovWork* k) {
 
 template <typename V>
 struct CgVecs {
-    V *r, *z, *p, *w, *dinv, *y;
+    V *r, *z, *p, *w, *dinv, *y, *rt;
     const V* val;
 };
 
@@ -558,11 +565,11 @@ template <typename V>
 static CgVecs<V> cg_vecs(KrylovWork* k);
 template <>
 CgVecs<double> cg_vecs<double>(KrylovWork* k) {
-    return {k->r.p, k->z.p, k->p.p, k->w.p, k->dinv.p, k->y.p, k->val64.p};
+    return {k->r.p, k->z.p, k->p.p, k->w.p, k->dinv.p, k->y.p, k->rt.p, k->val64.p};
 }
 template <>
 CgVecs<float> cg_vecs<float>(KrylovWork* k) {
-    return {k->r32.p, k->z32.p, k->p32.p, k->w32.p, k->dinv32.p, k->y32.p, k->val32.p};
+    return {k->r32.p, k->z32.p, k->p32.p, k->w32.p, k->dinv32.p, k->y32.p, k->rt32.p, k->val32.p};
 }
 
 // Apply the learned preconditioner z = G^T (G r); the per-block sums
@@ -570,7 +577,7 @@ CgVecs<float> cg_vecs<float>(KrylovWork* k) {
 // Returns the number of partial sums in *nparts.
 template <typename V>
 static int learned_apply(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, const int* done_flag, int* nparts) {
-    return learned_precond_apply<V>(c, v.r, v.z, v.y, v.w, k->part(PART_PC), nparts, done_flag);
+    return learned_precond_apply<V>(c, v.r, v.rt, v.z, v.y, v.w, k->part(PART_PC), nparts, done_flag);
 }
 
 // r = b - A x (FP64), r_V = r, z = M^-1 r with the sums r.z, r.r, z.z
@@ -579,8 +586,9 @@ template <typename V>
 static int restart_residual(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, bool jacobi, bool norm0,
                             bool recompute) {
     if (recompute) FLGPU_TRY(true_residual(c, k));
+    const float* dsc = jacobi ? nullptr : learned_scale(c);
     hipLaunchKernelGGL((k_restart_vecs<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->r.p,
-                       v.r, v.z, v.dinv, jacobi ? 1 : 0, k->part(PART_VEC));
+                       v.r, v.z, v.dinv, jacobi ? 1 : 0, dsc, v.rt, k->part(PART_VEC));
     FLGPU_HIP(hipGetLastError());
     FLGPU_TRY(finalize(c, k, FIN_RESTART, PART_VEC, k->nblk, jacobi, norm0 && jacobi));
     if (!jacobi) {
@@ -607,7 +615,7 @@ static int cg_iteration(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, bool ja
     Scal* in = k->S();
     k->cur ^= 1;
     hipLaunchKernelGGL((k_cg_update<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->x.p, v.p, v.r,
-                       v.w, v.z, v.dinv, jacobi ? 1 : 0, k->part(PART_SPMV), spmv_grid(c), k->part(PART_VEC), in,
+                       v.w, v.z, v.dinv, jacobi ? 1 : 0, jacobi ? nullptr : learned_scale(c), v.rt, k->part(PART_SPMV), spmv_grid(c), k->part(PART_VEC), in,
                        k->S());
     int npc = 0;
     if (!jacobi) FLGPU_TRY(learned_apply<V>(c, k, v, &k->S()->done, &npc));

@@ -945,44 +945,48 @@ static CsrOp<HV, CT> half_op(int nrb, const int32_t* rb, const int32_t* rbk, con
     return CsrOp<HV, CT>{nrb, rb, rbk, ptr, col, val, cs, rs};
 }
 
-// y = Gs (D r) over the lower half, then z = D (Gs^T y) over the upper half
-// with the sums r.z, z.z, z.w
+// y = Gs rt (rt = D r) over the lower half, then z = D (Gs^T y) over the
+// upper half with the sums r.z, z.z, z.w
 template <typename V, typename HV, typename LC, typename UC>
 static void apply_launch(flgpu_ctx* c, LearnedState* s, const HV* gl, const HV* gu, const LC* lcol,
-                         const UC* ucol, const V* r, V* z, V* y, const V* w, double* partials, const int* done) {
+                         const UC* ucol, const V* r, const V* rt, V* z, V* y, const V* w, double* partials,
+                         const int* done) {
     const int gl_grid = stream_grid(s->nrbl), gu_grid = stream_grid(s->nrbu);
-    const auto lo = half_op<HV, LC>(s->nrbl, s->rbl.p, s->rbkl.p, s->lptr.p, lcol, gl, s->dsq.p, nullptr);
+    const auto lo = half_op<HV, LC>(s->nrbl, s->rbl.p, s->rbkl.p, s->lptr.p, lcol, gl, nullptr, nullptr);
     const auto up = half_op<HV, UC>(s->nrbu, s->rbu.p, s->rbku.p, s->uptr.p, ucol, gu, nullptr, s->dsq.p);
     hipLaunchKernelGGL((k_csr_stream<V, HV, LC, V, V, DOT_NONE>), dim3(gl_grid), dim3(kRowBlock), 0, c->stream, lo,
-                       r, y, (const double*)nullptr, (const V*)nullptr, (const V*)nullptr, partials, done);
+                       rt, y, (const double*)nullptr, (const V*)nullptr, (const V*)nullptr, partials, done);
     hipLaunchKernelGGL((k_csr_stream<V, HV, UC, V, V, DOT_PC>), dim3(gu_grid), dim3(kRowBlock), 0, c->stream, up,
                        (const V*)y, z, (const double*)nullptr, r, w, partials, done);
 }
 
 template <typename V, typename HV>
-static int apply_impl(flgpu_ctx* c, LearnedState* s, const HV* gl, const HV* gu, const V* r, V* z, V* y,
-                      const V* w, double* partials, int* nparts, const int* done) {
+static int apply_impl(flgpu_ctx* c, LearnedState* s, const HV* gl, const HV* gu, const V* r, const V* rt, V* z,
+                      V* y, const V* w, double* partials, int* nparts, const int* done) {
     *nparts = stream_grid(s->nrbu);
     if (s->l16 && s->u16)
-        apply_launch<V, HV, int16_t, int16_t>(c, s, gl, gu, s->lcol16.p, s->ucol16.p, r, z, y, w, partials, done);
+        apply_launch<V, HV, int16_t, int16_t>(c, s, gl, gu, s->lcol16.p, s->ucol16.p, r, rt, z, y, w, partials,
+                                              done);
     else
-        apply_launch<V, HV, int32_t, int32_t>(c, s, gl, gu, s->lcol.p, s->ucol.p, r, z, y, w, partials, done);
+        apply_launch<V, HV, int32_t, int32_t>(c, s, gl, gu, s->lcol.p, s->ucol.p, r, rt, z, y, w, partials, done);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
 
 template <typename V>
-int learned_precond_apply(flgpu_ctx* c, const V* r, V* z, V* y, const V* w, double* partials, int* nparts,
-                          const int* done_flag) {
+int learned_precond_apply(flgpu_ctx* c, const V* r, const V* rt, V* z, V* y, const V* w, double* partials,
+                          int* nparts, const int* done_flag) {
     LearnedState* s = state(c);
     if (std::is_same<V, float>::value)
-        return apply_impl<V, _Float16>(c, s, s->gl16.p, s->gu16.p, r, z, y, w, partials, nparts, done_flag);
-    return apply_impl<V, double>(c, s, s->gl64.p, s->gu64.p, r, z, y, w, partials, nparts, done_flag);
+        return apply_impl<V, _Float16>(c, s, s->gl16.p, s->gu16.p, r, rt, z, y, w, partials, nparts, done_flag);
+    return apply_impl<V, double>(c, s, s->gl64.p, s->gu64.p, r, rt, z, y, w, partials, nparts, done_flag);
 }
-template int learned_precond_apply<double>(flgpu_ctx*, const double*, double*, double*, const double*, double*,
-                                           int*, const int*);
-template int learned_precond_apply<float>(flgpu_ctx*, const float*, float*, float*, const float*, double*,
-                                          int*, const int*);
+template int learned_precond_apply<double>(flgpu_ctx*, const double*, const double*, double*, double*,
+                                           const double*, double*, int*, const int*);
+template int learned_precond_apply<float>(flgpu_ctx*, const float*, const float*, float*, float*, const float*,
+                                          double*, int*, const int*);
+
+const float* learned_scale(flgpu_ctx* c) { return c->ls ? c->ls->dsq.p : nullptr; }
 
 void learned_free(flgpu_ctx* c) {
     delete c->ls;
