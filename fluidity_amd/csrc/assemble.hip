@@ -19,12 +19,19 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "flgpu_internal.h"
 #include "element_kernels.h"
 
 namespace flgpu {
+
+// Workgroup of the node-staged slab kernel and the most row entries and
+// slots each of its threads stages (the slab plan keeps slabs within them).
+constexpr int kStagedThreads = 256;
+constexpr int kStagedColsPerThread = 4;
+constexpr int kStagedSlotsPerThread = 2;
 
 template <typename R>
 __device__ void stage_tables(const ElemTables* __restrict__ t, LTab<R>& s) {
@@ -277,36 +284,36 @@ k_asm_blocked(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int n
     }
 }
 
-// Nodal field values from the slab's LDS copy (constant fields from their
-// one global value); diffusivity must be isotropic (component 0 staged).
+// Nodal field values from the slab's LDS copy (a constant field is staged
+// as its one value at every node); diffusivity must be isotropic
+// (component 0 staged).
 template <int DIM, typename R>
 struct StagedNodal {
-    const AsmFields& fd;
     const int* lid;
     const double *sU, *sK, *sS, *sB;
     int ncap;
-    __device__ __forceinline__ R U(int k, int d) const {
-        return fd.u_c ? (R)fd.U[d] : (R)sU[d * ncap + lid[k]];
-    }
-    __device__ __forceinline__ R K(int k, int comp) const { return fd.k_c ? (R)fd.K[comp] : (R)sK[lid[k]]; }
-    __device__ __forceinline__ R S(int k) const { return fd.s_c ? (R)fd.S[0] : (R)sS[lid[k]]; }
-    __device__ __forceinline__ R B(int k) const { return fd.b_c ? (R)fd.B[0] : (R)sB[lid[k]]; }
+    __device__ __forceinline__ R U(int k, int d) const { return (R)sU[d * ncap + lid[k]]; }
+    __device__ __forceinline__ R K(int k, int) const { return (R)sK[lid[k]]; }
+    __device__ __forceinline__ R S(int k) const { return (R)sS[lid[k]]; }
+    __device__ __forceinline__ R B(int k) const { return (R)sB[lid[k]]; }
 };
 
 // The blocked scheme with the slab's nodes staged in LDS: the slab's node
 // list (its rows and every node of their elements) is read once, the
 // coordinates and nodal fields of those nodes are gathered once into LDS,
 // and each slot names its element's vertices by slab-local number, so the
-// element rows read no global memory but their slot word.  Same slots,
-// same order, same bits as k_asm_blocked.
+// element rows read no global memory but their slot word.  An entry's
+// contributions are the slots of its row whose element has the entry's
+// column among its vertices: phase B matches the column's slab-local number
+// (one byte per entry, from the plan) against the slot's four vertex bytes.
+// Same slots, same order, same bits as k_asm_blocked.
 template <int DIM, int FEAT>
-__global__ void __launch_bounds__(256, 5)
-k_asm_staged(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int nnz_cap, int slot_cap, int node_cap,
-             const int32_t* __restrict__ brow, const int32_t* __restrict__ iptr, const int32_t* __restrict__ nptr,
-             const int32_t* __restrict__ nodes, const int32_t* __restrict__ p0s, const uint32_t* __restrict__ slotw,
+__global__ void __launch_bounds__(kStagedThreads, 5)
+k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz_cap, int slot_cap, int node_cap,
+             const int32_t* __restrict__ meta, const int32_t* __restrict__ iptr, const int32_t* __restrict__ nodes,
+             const uint32_t* __restrict__ slotw, const uint8_t* __restrict__ lcl,
              const double* __restrict__ X4, AsmFields fd, AsmFlags fl, int stage_u, int stage_k, int stage_s,
-             int stage_b, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, double* __restrict__ A,
-             double* __restrict__ rhs, int* __restrict__ err) {
+             int stage_b, const int32_t* __restrict__ rowptr, double* __restrict__ A, double* __restrict__ rhs) {
     constexpr int LOC = DIM + 1;
     __shared__ LTabLin<double> tb;
     extern __shared__ double dyn[];
@@ -318,41 +325,79 @@ k_asm_staged(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int nn
     double* sK = sU + (stage_u ? (size_t)DIM * node_cap : 0);      // node_cap (stage_k)
     double* sS = sK + (stage_k ? node_cap : 0);                    // node_cap (stage_s)
     double* sB = sS + (stage_s ? node_cap : 0);                    // node_cap (stage_b)
-    uint32_t* spos = reinterpret_cast<uint32_t*>(sB + (stage_b ? node_cap : 0));  // slot_cap
-    int32_t* lcol = reinterpret_cast<int32_t*>(spos + slot_cap);   // nnz_cap
-    int32_t* lrp = lcol + nnz_cap;                                 // rows_cap + 1
+    int32_t* lrp = reinterpret_cast<int32_t*>(sB + (stage_b ? node_cap : 0));  // rows_cap + 1
     int32_t* soff = lrp + rows_cap + 1;                            // rows_cap + 1
-    int32_t* sgid = soff + rows_cap + 1;                           // node_cap
-    uint32_t* sw = reinterpret_cast<uint32_t*>(sgid + node_cap);   // slot_cap: slot words
-    uint16_t* srow = reinterpret_cast<uint16_t*>(sw + slot_cap);   // slot_cap
+    uint32_t* sw = reinterpret_cast<uint32_t*>(soff + rows_cap + 1);  // slot_cap: slot words
+    uint32_t* scl = sw + slot_cap;                                 // (nnz_cap + 6) / 4: column bytes
+    uint16_t* srow = reinterpret_cast<uint16_t*>(scl + ((nnz_cap + 6) >> 2));  // slot_cap
     uint16_t* erow = srow + slot_cap;                              // nnz_cap
-    stage_tables_lin<double>(tab, tb);
-
+    constexpr int NT = kStagedThreads;
+    constexpr int TW = (int)(sizeof(LTabLin<double>) / sizeof(uint64_t));
+    static_assert(TW <= NT, "one table word per thread");
     const int per = (nslab + 7) >> 3;
     const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if (b >= nslab) return;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int r0 = brow[b], r1 = brow[b + 1], R = r1 - r0;
-    const int k0 = rowptr[r0], E = rowptr[r1] - k0;
-    const int s0 = iptr[r0], S = iptr[r1] - s0;
-    const int n0 = nptr[b], NN = nptr[b + 1] - n0, pr = p0s[b];
-    for (int l = tid; l <= R; l += nt) {
-        lrp[l] = rowptr[r0 + l] - k0;
-        soff[l] = iptr[r0 + l] - s0;
+    const int tid = threadIdx.x, nt = NT;
+    // Three memory round trips stage the slab: (1) its record and the
+    // element tables, (2) what the record locates (row and slot offsets,
+    // column bytes, slot words, node ids), (3) the nodes' coordinates and fields.
+    // Every load of a step is issued before the first of them is used.
+    const uint64_t tword = tid < TW ? tabw[tid] : 0;
+    const int4* mp = reinterpret_cast<const int4*>(meta + 12 * (size_t)b);
+    const int4 m0 = mp[0], m1 = mp[1], m2 = mp[2];
+    const int r0 = m0.x, r1 = m0.y, R = r1 - r0;
+    const int k0 = m0.z, E = m0.w - k0;
+    const int s0 = m1.x, S = m1.y - s0;
+    const int n0 = m1.z, NN = m1.w - n0, pr = m2.x;
+
+    int vrp = 0, vip = 0, vn = 0;
+    uint32_t vs[kStagedSlotsPerThread], vc = 0;
+    if (tid <= R) {
+        vrp = rowptr[r0 + tid];
+        vip = iptr[r0 + tid];
     }
-    for (int k = tid; k < E; k += nt) lcol[k] = col[k0 + k];
-    for (int k = tid; k < S; k += nt) sw[k] = slotw[s0 + k];
-    for (int q = tid; q < NN; q += nt) {
-        const int g = nodes[n0 + q];
-        sgid[q] = g;
+    // the entries' column bytes, four to a thread (the slab's bytes start at
+    // byte k0 of lcl)
+    const int c0 = k0 & ~3, nw = (k0 + E - c0 + 3) >> 2;
+    if (tid < nw) vc = reinterpret_cast<const uint32_t*>(lcl + c0)[tid];
 #pragma unroll
-        for (int a = 0; a < DIM; a++) sX[a * node_cap + q] = X4[(int64_t)g * 4 + a];
-        sT[q] = fd.T[g];
+    for (int u = 0; u < kStagedSlotsPerThread; u++) vs[u] = tid + u * NT < S ? slotw[s0 + tid + u * NT] : 0u;
+    if (tid < NN) vn = nodes[n0 + tid];
+
+    double vx[DIM], vt = 0.0, vk = 0.0, vsrc = 0.0, vb = 0.0, vu[DIM];
+#pragma unroll
+    for (int a = 0; a < DIM; a++) vx[a] = vu[a] = 0.0;
+    if (tid < NN) {
+#pragma unroll
+        for (int a = 0; a < DIM; a++) vx[a] = X4[(int64_t)vn * 4 + a];
+        vt = fd.T[vn];
         if (stage_u)
-            for (int a = 0; a < DIM; a++) sU[a * node_cap + q] = fd.U[(int64_t)g * DIM + a];
-        if (stage_k) sK[q] = fd.K[(int64_t)g * DIM * DIM];
-        if (stage_s) sS[q] = fd.S[g];
-        if (stage_b) sB[q] = fd.B[g];
+#pragma unroll
+            for (int a = 0; a < DIM; a++) vu[a] = fd.U[fd.u_c ? a : (int64_t)vn * DIM + a];
+        if (stage_k) vk = fd.K[fd.k_c ? 0 : (int64_t)vn * DIM * DIM];
+        if (stage_s) vsrc = fd.S[fd.s_c ? 0 : vn];
+        if (stage_b) vb = fd.B[fd.b_c ? 0 : vn];
+    }
+    if (tid < TW) reinterpret_cast<uint64_t*>(&tb)[tid] = tword;
+    if (tid <= R) {
+        lrp[tid] = vrp - k0;
+        soff[tid] = vip - s0;
+    }
+    if (tid < nw) scl[tid] = vc;
+#pragma unroll
+    for (int u = 0; u < kStagedSlotsPerThread; u++)
+        if (tid + u * NT < S) sw[tid + u * NT] = vs[u];
+    if (tid < NN) {
+        const int q = tid;
+#pragma unroll
+        for (int a = 0; a < DIM; a++) sX[a * node_cap + q] = vx[a];
+        sT[q] = vt;
+        if (stage_u)
+#pragma unroll
+            for (int a = 0; a < DIM; a++) sU[a * node_cap + q] = vu[a];
+        if (stage_k) sK[q] = vk;
+        if (stage_s) sS[q] = vsrc;
+        if (stage_b) sB[q] = vb;
     }
     __syncthreads();
     for (int l = tid; l < R; l += nt) {
@@ -372,7 +417,6 @@ k_asm_staged(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int nn
 #pragma unroll
         for (int k = 0; k < LOC; k++) {
             lid[k] = (int)((w >> (8 * k)) & 255u);
-            e.nodes[k] = sgid[lid[k]];
             if (lid[k] == pr + l) i = k;
 #pragma unroll
             for (int a = 0; a < DIM; a++) e.X[a][k] = sX[a * node_cap + lid[k]];
@@ -382,35 +426,29 @@ k_asm_staged(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int nn
         geom_linear<DIM, LOC, double, LTabLin<double>>(e, tb, gm);
         double vals[LOC], r;
         row_linear_g<DIM, LOC, double, FEAT, StagedNodal<DIM, double>, LTabLin<double>>(
-            e, gm, tb, fl, StagedNodal<DIM, double>{fd, lid, sU, sK, sS, sB, node_cap}, i, vals, r);
-        const int lo = lrp[l], hi = lrp[l + 1];
-        uint32_t packed = 0;
+            e, gm, tb, fl, StagedNodal<DIM, double>{lid, sU, sK, sS, sB, node_cap}, i, vals, r);
 #pragma unroll
-        for (int j = 0; j < LOC; j++) {
-            const int q = lds_pos(lcol, lo, hi, e.nodes[j]);
-            if (q < 0) atomicOr(err, 1);
-            packed |= (uint32_t)(q < 0 ? 255 : q - lo) << (8 * j);
-            sval[j * slot_cap + sl] = vals[j];
-        }
-        spos[sl] = packed;
+        for (int j = 0; j < LOC; j++) sval[j * slot_cap + sl] = vals[j];
         srhs[sl] = r;
     }
     __syncthreads();
 
     // phase B: one thread per entry, contributions in slot (= colour) order
+    const uint8_t* cb = reinterpret_cast<const uint8_t*>(scl) + (k0 - c0);
     for (int t = tid; t < E; t += nt) {
         const int l = erow[t];
-        const uint32_t p = (uint32_t)(t - lrp[l]);
+        const uint32_t rep = (uint32_t)cb[t] * 0x01010101u;
         double acc = 0.0;
 #pragma unroll 4
         for (int k = soff[l]; k < soff[l + 1]; k++) {
-            const uint32_t pk = spos[k];
-            int j = -1;
-#pragma unroll
-            for (int jj = 0; jj < LOC; jj++)
-                if (((pk >> (8 * jj)) & 255u) == p) j = jj;
-            const double vv = sval[(j < 0 ? 0 : j) * slot_cap + k];
-            acc = acc + (j < 0 ? 0.0 : vv);
+            // the byte of the slot word equal to the column byte, if any (the
+            // LOC vertex bytes differ; 255 is no slab node): the lowest set
+            // bit of z marks the lowest zero byte of x
+            const uint32_t x = sw[k] ^ rep;
+            const uint32_t z = (x - 0x01010101u) & ~x & (LOC == 4 ? 0x80808080u : 0x00808080u);
+            const int j = (int)(__builtin_ctz(z | 0x80000000u) >> 3);
+            const double vv = sval[j * slot_cap + k];
+            acc = acc + (z ? vv : 0.0);
         }
         A[k0 + t] = acc;
     }
@@ -423,8 +461,8 @@ k_asm_staged(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int nn
 
 static size_t staged_lds(int dim, int loc, int rows_cap, int nnz_cap, int slot_cap, int node_cap, int nfield) {
     return sizeof(double) * ((size_t)slot_cap * (loc + 1) + (size_t)node_cap * (dim + 1 + nfield)) +
-           sizeof(uint32_t) * (size_t)slot_cap + sizeof(int32_t) * ((size_t)nnz_cap + 2 * (size_t)(rows_cap + 1)) +
-           sizeof(int32_t) * (size_t)node_cap + sizeof(uint32_t) * (size_t)slot_cap +
+           sizeof(int32_t) * (2 * (size_t)(rows_cap + 1) + (size_t)slot_cap +
+                              (size_t)((nnz_cap + 6) >> 2)) +
            sizeof(uint16_t) * ((size_t)slot_cap + nnz_cap);
 }
 
@@ -439,9 +477,16 @@ int build_block_plan(flgpu_ctx* c) {
     if (!c->linear) return set_error(FLGPU_EUNSUPPORTED, "block plan: linear field mesh only");
     const int N = c->rows, loc = c->loc;
     // LDS budget per slab: four to five workgroups per CU
+    // Node staging needs the field nodes to be the coordinate nodes and each
+    // slab's node list to hold at most 256 nodes (8-bit local numbers).
+    const bool stage = c->h_conn == c->h_conn_x && getenv("FLGPU_NO_STAGING") == nullptr;
     int slot_cap = 384;
     if (const char* s = getenv("FLGPU_BLOCK_SLOTS")) slot_cap = std::max(64, std::min(4096, atoi(s)));
-    const int rows_cap = 256, nnz_cap = 1024;
+    // the staged kernel loads a slab's rows, entries and slots in one pass
+    // of its workgroup: at most one row, four entries and two slots a thread
+    if (stage) slot_cap = std::min(slot_cap, kStagedSlotsPerThread * kStagedThreads);
+    const int rows_cap = stage ? kStagedThreads - 1 : 256;
+    const int nnz_cap = stage ? kStagedColsPerThread * kStagedThreads - 4 : 1024;
     // node -> element incidence in colour order
     std::vector<int32_t> iptr((size_t)N + 1, 0);
     for (int64_t k = 0; k < (int64_t)c->nele * loc; k++) iptr[c->h_conn[k] + 1]++;
@@ -464,14 +509,13 @@ int build_block_plan(flgpu_ctx* c) {
             for (int i = 0; i < loc; i++) inc[fill[c->h_conn[(int64_t)e * loc + i]]++] = e;
         }
     }
-    // Node staging needs the field nodes to be the coordinate nodes and each
-    // slab's node list to hold at most 256 nodes (8-bit local numbers).
-    const bool stage = c->h_conn == c->h_conn_x && getenv("FLGPU_NO_STAGING") == nullptr;
-    const int node_cap_max = 256;
+    const int node_cap_max = 255;  // local number 255: not a node of the slab
     // slabs of consecutive rows bounded by rows, entries, slots (and nodes)
     std::vector<int32_t> brow(1, 0), nptr(1, 0), nodes, p0v;
     std::vector<uint32_t> slotw;
-    std::vector<int32_t> mark(stage ? N : 0, -1), fresh;
+    std::vector<int32_t> mark(stage ? N : 0, -1), fresh, stamp(stage ? N : 0, -1);
+    // column of every entry as a slab-local node number (4-byte padded)
+    std::vector<uint8_t> lcl(stage ? ((size_t)c->nnz + 4) & ~(size_t)3 : 0, 255);
     int max_rows = 0, max_nnz = 0, max_slots = 0, max_nodes = 0;
     for (int r = 0; r < N;) {
         const int slab = (int)brow.size() - 1;
@@ -525,6 +569,20 @@ int build_block_plan(flgpu_ctx* c) {
             std::sort(nodes.begin() + first, nodes.end());
             for (size_t q = first; q < nodes.size(); q++) mark[nodes[q]] = (int)(q - first);  // local number
             p0v.push_back(mark[r]);
+            for (int rr = r; rr < r1; rr++) {
+                for (int k = c->h_rowptr[rr]; k < c->h_rowptr[rr + 1]; k++) {
+                    const int cj = c->h_col[k];
+                    stamp[cj] = rr;
+                    if (mark[cj] >= 0) lcl[k] = (uint8_t)mark[cj];
+                }
+                // every vertex of the row's elements must be a column of the
+                // row (csr_addto aborts otherwise, Sparse_Tools.F90:2645-2647)
+                for (int q = iptr[rr]; q < iptr[rr + 1]; q++)
+                    for (int v = 0; v < loc; v++)
+                        if (stamp[c->h_conn[(int64_t)inc[q] * loc + v]] != rr)
+                            return set_error(FLGPU_EPATTERN,
+                                             "Attempting to set value in matrix outside sparsity pattern.");
+            }
             for (int q = iptr[r]; q < iptr[r1]; q++) {
                 uint32_t w = 0;
                 for (int v = 0; v < loc; v++) w |= (uint32_t)mark[c->h_conn[(int64_t)inc[q] * loc + v]] << (8 * v);
@@ -540,10 +598,44 @@ int build_block_plan(flgpu_ctx* c) {
     BlockPlan& P = c->plan;
     P.staged = stage;
     if (stage) {
+        const int nb = (int)brow.size() - 1;
+        std::vector<int32_t> meta((size_t)nb * 12, 0);
+        for (int b = 0; b < nb; b++) {
+            int32_t* m = &meta[(size_t)b * 12];
+            m[0] = brow[b];
+            m[1] = brow[b + 1];
+            m[2] = c->h_rowptr[brow[b]];
+            m[3] = c->h_rowptr[brow[b + 1]];
+            m[4] = iptr[brow[b]];
+            m[5] = iptr[brow[b + 1]];
+            m[6] = nptr[b];
+            m[7] = nptr[b + 1];
+            m[8] = p0v[b];
+        }
+        FLGPU_TRY(P.meta.upload(meta.data(), meta.size(), c->stream));
+        // the P1 element tables in the layout the kernel keeps in LDS
+        LTabLin<double> lt;
+        std::memset(&lt, 0, sizeof lt);
+        const ElemTables& t = c->tab;
+        for (int k = 0; k < t.loc * kMaxDim && k < 4 * kMaxDim; k++) {
+            const int nn = k % t.loc, d = k / t.loc;
+            lt.dn[nn + t.loc * d] = t.dn[nn + t.loc * (0 + t.ngi * d)];
+        }
+        for (int k = 0; k < 4; k++) lt.what[k] = t.what[k];
+        for (int k = 0; k < 16; k++) lt.mhat[k] = t.mhat[k];
+        for (int k = 0; k < 64; k++) lt.that[k] = t.that[k];
+        for (int k = 0; k < 4 * kMaxDim; k++) lt.xdn[k] = t.xdn[k];
+        lt.loc = t.loc;
+        lt.ngi = 1;
+        lt.lx = t.loc_x;
+        std::vector<uint64_t> tw(sizeof lt / sizeof(uint64_t));
+        std::memcpy(tw.data(), &lt, sizeof lt);
+        FLGPU_TRY(P.tabw.upload(tw.data(), tw.size(), c->stream));
         FLGPU_TRY(P.nptr.upload(nptr.data(), nptr.size(), c->stream));
         FLGPU_TRY(P.nodes.upload(nodes.data(), nodes.size(), c->stream));
         FLGPU_TRY(P.p0.upload(p0v.data(), p0v.size(), c->stream));
         FLGPU_TRY(P.slotw.upload(slotw.data(), slotw.size(), c->stream));
+        FLGPU_TRY(P.lcl.upload(lcl.data(), lcl.size(), c->stream));
         P.node_cap = max_nodes;
     }
     FLGPU_TRY(P.brow.upload(brow.data(), brow.size(), c->stream));
@@ -579,8 +671,9 @@ template <int DIM, int FEAT>
 static int launch_staged_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
     const BlockPlan& P = c->plan;
     constexpr int LOC = DIM + 1;
-    const int su = (FEAT & F_ADV) && fl.adv && !fd.u_c, sk = (FEAT & F_DIFF) && fl.diff && !fd.k_c;
-    const int ss = (FEAT & F_SRC) && fl.src && !fd.s_c, sb = (FEAT & F_ABS) && fl.absorp && !fd.b_c;
+    // the fields the compiled terms read, staged per node (constant ones too)
+    const int su = (FEAT & F_ADV) && fl.adv, sk = (FEAT & F_DIFF) && fl.diff;
+    const int ss = (FEAT & F_SRC) && fl.src, sb = (FEAT & F_ABS) && fl.absorp;
     const int nfield = su * DIM + sk + ss + sb;
     const size_t lds = staged_lds(DIM, LOC, P.rows_cap, P.nnz_cap, P.slot_cap, P.node_cap, nfield);
     if (lds + sizeof(LTabLin<double>) + 64 > 160 * 1024)
@@ -588,10 +681,9 @@ static int launch_staged_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags&
     FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_staged<DIM, FEAT>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int grid = 8 * ((P.nblocks + 7) / 8);
-    hipLaunchKernelGGL((k_asm_staged<DIM, FEAT>), dim3(grid), dim3(256), lds, c->stream, c->d_tab.p, P.nblocks,
-                       P.rows_cap, P.nnz_cap, P.slot_cap, P.node_cap, P.brow.p, P.iptr.p, P.nptr.p, P.nodes.p,
-                       P.p0.p, P.slotw.p, c->X4.p, fd, fl, su, sk, ss, sb, c->rowptr.p, c->col.p, c->A.p, c->rhs.p,
-                       d_err);
+    hipLaunchKernelGGL((k_asm_staged<DIM, FEAT>), dim3(grid), dim3(kStagedThreads), lds, c->stream, P.tabw.p,
+                       P.nblocks, P.rows_cap, P.nnz_cap, P.slot_cap, P.node_cap, P.meta.p, P.iptr.p, P.nodes.p,
+                       P.slotw.p, P.lcl.p, c->X4.p, fd, fl, su, sk, ss, sb, c->rowptr.p, c->A.p, c->rhs.p);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
@@ -604,7 +696,7 @@ static int launch_blocked(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl,
     const int m = (fl.have_mass ? F_MASS : 0) | (fl.adv ? F_ADV : 0) | (fl.absorp ? F_ABS : 0) |
                   (fl.diff ? F_DIFF : 0) | (fl.src ? F_SRC : 0);
     auto fits = [m](int f) { return (m & ~f) == 0; };
-    const bool staged = c->plan.staged && (!fl.diff || fl.iso || fd.k_c);
+    const bool staged = c->plan.staged && (!fl.diff || fl.iso);
     if (staged) {
         if (fits(F_DIFF | F_SRC)) return launch_staged_feat<DIM, F_DIFF | F_SRC>(c, fd, fl, d_err);
         if (fits(F_MASS | F_ADV | F_DIFF)) return launch_staged_feat<DIM, F_MASS | F_ADV | F_DIFF>(c, fd, fl, d_err);

@@ -112,6 +112,9 @@ struct BlockPlan {
     DevBuf<int32_t> nptr;        // nblocks+1 offsets into nodes
     DevBuf<int32_t> nodes;       // global node ids of each slab, ascending
     DevBuf<int32_t> p0;          // position of each slab's first row in its node list
+    DevBuf<int32_t> meta;        // per slab: r0 r1 k0 k1 s0 s1 n0 n1 p0 (and 3 pad), one 48-byte record
+    DevBuf<uint8_t> lcl;         // per CSR entry: the column's slab-local node number (255: none)
+    DevBuf<uint64_t> tabw;       // the P1 element tables (LTabLin<double>) as 64-bit words
     DevBuf<uint32_t> slotw;      // per slot: 4 x 8-bit slab-local vertex numbers
     int32_t node_cap = 0;
 };
