@@ -31,7 +31,7 @@ namespace flgpu {
 // slots each of its threads stages (the slab plan keeps slabs within them).
 constexpr int kStagedThreads = 256;
 constexpr int kStagedColsPerThread = 4;
-constexpr int kStagedSlotsPerThread = 2;
+constexpr int kStagedSlotsPerThread = 3;
 
 template <typename R>
 __device__ void stage_tables(const ElemTables* __restrict__ t, LTab<R>& s) {
@@ -302,66 +302,79 @@ struct StagedNodal {
 // list (its rows and every node of their elements) is read once, the
 // coordinates and nodal fields of those nodes are gathered once into LDS,
 // and each slot names its element's vertices by slab-local number, so the
-// element rows read no global memory but their slot word.  An entry's
-// contributions are the slots of its row whose element has the entry's
-// column among its vertices: phase B matches the column's slab-local number
-// (one byte per entry, from the plan) against the slot's four vertex bytes.
-// Same slots, same order, same bits as k_asm_blocked.
+// element rows read no global memory but their slot words.
+//
+// Contributions: a row with n slots receives 4n values (3n for triangles),
+// which the plan orders by entry and, within an entry, by slot (colour
+// order).  Each slot's second word says where its values go in that order
+// (the row-relative place of each off-diagonal value, 9 bits each, and the
+// number of the row's vertex in the element, 2 bits); the diagonal values
+// of a row are consecutive from the row's diagonal offset.  Phase B then
+// sums every entry's run of values in order: the same values in the same
+// order as the colour scheme, so the same bits.
 template <int DIM, int FEAT>
 __global__ void __launch_bounds__(kStagedThreads, 5)
 k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz_cap, int slot_cap, int node_cap,
-             const int32_t* __restrict__ meta, const int32_t* __restrict__ iptr, const int32_t* __restrict__ nodes,
-             const uint32_t* __restrict__ slotw, const uint8_t* __restrict__ lcl,
+             const int32_t* __restrict__ meta, const int32_t* __restrict__ iptr, const uint16_t* __restrict__ rdg,
+             const int32_t* __restrict__ nodes, const uint32_t* __restrict__ slotw,
+             const uint32_t* __restrict__ slotd, const uint8_t* __restrict__ srb, const uint16_t* __restrict__ ecp,
              const double* __restrict__ X4, AsmFields fd, AsmFlags fl, int stage_u, int stage_k, int stage_s,
-             int stage_b, const int32_t* __restrict__ rowptr, double* __restrict__ A, double* __restrict__ rhs) {
+             int stage_b, double* __restrict__ A, double* __restrict__ rhs) {
     constexpr int LOC = DIM + 1;
+    constexpr int NT = kStagedThreads;
+    constexpr int TW = (int)(sizeof(LTabLin<double>) / sizeof(uint64_t));
+    static_assert(TW <= NT, "one table word per thread");
     __shared__ LTabLin<double> tb;
     extern __shared__ double dyn[];
-    double* sval = dyn;                                            // LOC * slot_cap, [j][slot]
-    double* srhs = sval + (size_t)slot_cap * LOC;                  // slot_cap
+    double* sc = dyn;                                              // LOC * slot_cap: the contributions
+    double* srhs = sc + (size_t)slot_cap * LOC;                    // slot_cap
     double* sX = srhs + slot_cap;                                  // DIM * node_cap
     double* sT = sX + (size_t)DIM * node_cap;                      // node_cap
     double* sU = sT + node_cap;                                    // DIM * node_cap (stage_u)
     double* sK = sU + (stage_u ? (size_t)DIM * node_cap : 0);      // node_cap (stage_k)
     double* sS = sK + (stage_k ? node_cap : 0);                    // node_cap (stage_s)
     double* sB = sS + (stage_s ? node_cap : 0);                    // node_cap (stage_b)
-    int32_t* lrp = reinterpret_cast<int32_t*>(sB + (stage_b ? node_cap : 0));  // rows_cap + 1
-    int32_t* soff = lrp + rows_cap + 1;                            // rows_cap + 1
-    uint32_t* sw = reinterpret_cast<uint32_t*>(soff + rows_cap + 1);  // slot_cap: slot words
-    uint32_t* scl = sw + slot_cap;                                 // (nnz_cap + 6) / 4: column bytes
-    uint16_t* srow = reinterpret_cast<uint16_t*>(scl + ((nnz_cap + 6) >> 2));  // slot_cap
-    uint16_t* erow = srow + slot_cap;                              // nnz_cap
-    constexpr int NT = kStagedThreads;
-    constexpr int TW = (int)(sizeof(LTabLin<double>) / sizeof(uint64_t));
-    static_assert(TW <= NT, "one table word per thread");
+    int32_t* soff = reinterpret_cast<int32_t*>(sB + (stage_b ? node_cap : 0));  // rows_cap + 1
+    int32_t* sdg = soff + rows_cap + 1;                            // rows_cap: diagonal offsets
+    uint32_t* sw = reinterpret_cast<uint32_t*>(sdg + rows_cap);    // slot_cap: vertex words
+    uint32_t* sd = sw + slot_cap;                                  // slot_cap: destination words
+    uint32_t* ssr = sd + slot_cap;                                 // (slot_cap + 6) / 4: slot row bytes
+    uint32_t* scp = ssr + ((slot_cap + 6) >> 2);                   // (nnz_cap + 3) / 2: entry offsets
+
     const int per = (nslab + 7) >> 3;
     const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if (b >= nslab) return;
     const int tid = threadIdx.x, nt = NT;
     // Three memory round trips stage the slab: (1) its record and the
-    // element tables, (2) what the record locates (row and slot offsets,
-    // column bytes, slot words, node ids), (3) the nodes' coordinates and fields.
-    // Every load of a step is issued before the first of them is used.
+    // element tables, (2) what the record locates (slot offsets and
+    // diagonal offsets of the rows, slot words, row bytes, entry offsets,
+    // node ids), (3) the nodes' coordinates and fields.  Every load of a
+    // step is issued before the first of them is used.
     const uint64_t tword = tid < TW ? tabw[tid] : 0;
-    const int4* mp = reinterpret_cast<const int4*>(meta + 12 * (size_t)b);
-    const int4 m0 = mp[0], m1 = mp[1], m2 = mp[2];
+    const int4* mp = reinterpret_cast<const int4*>(meta + 8 * (size_t)b);
+    const int4 m0 = mp[0], m1 = mp[1];
     const int r0 = m0.x, r1 = m0.y, R = r1 - r0;
     const int k0 = m0.z, E = m0.w - k0;
     const int s0 = m1.x, S = m1.y - s0;
-    const int n0 = m1.z, NN = m1.w - n0, pr = m2.x;
+    const int n0 = m1.z, NN = m1.w - n0;
 
-    int vrp = 0, vip = 0, vn = 0;
-    uint32_t vs[kStagedSlotsPerThread], vc = 0;
-    if (tid <= R) {
-        vrp = rowptr[r0 + tid];
-        vip = iptr[r0 + tid];
-    }
-    // the entries' column bytes, four to a thread (the slab's bytes start at
-    // byte k0 of lcl)
-    const int c0 = k0 & ~3, nw = (k0 + E - c0 + 3) >> 2;
-    if (tid < nw) vc = reinterpret_cast<const uint32_t*>(lcl + c0)[tid];
+    int vip = 0, vdg = 0, vn = 0;
+    uint32_t vs[kStagedSlotsPerThread], vd[kStagedSlotsPerThread], vsr = 0, vcp[kStagedColsPerThread / 2];
+    if (tid <= R) vip = iptr[r0 + tid];
+    if (tid < R) vdg = rdg[r0 + tid];
 #pragma unroll
-    for (int u = 0; u < kStagedSlotsPerThread; u++) vs[u] = tid + u * NT < S ? slotw[s0 + tid + u * NT] : 0u;
+    for (int u = 0; u < kStagedSlotsPerThread; u++) {
+        const bool in = tid + u * NT < S;
+        vs[u] = in ? slotw[s0 + tid + u * NT] : 0u;
+        vd[u] = in ? slotd[s0 + tid + u * NT] : 0u;
+    }
+    // bytes and half-words as whole words: the slab's start rounded down
+    const int cs0 = s0 & ~3, nws = (s0 + S - cs0 + 3) >> 2;
+    const int ce0 = k0 & ~1, nwe = (k0 + E - ce0 + 1) >> 1;
+    if (tid < nws) vsr = reinterpret_cast<const uint32_t*>(srb + cs0)[tid];
+#pragma unroll
+    for (int u = 0; u < kStagedColsPerThread / 2; u++)
+        vcp[u] = tid + u * NT < nwe ? reinterpret_cast<const uint32_t*>(ecp + ce0)[tid + u * NT] : 0u;
     if (tid < NN) vn = nodes[n0 + tid];
 
     double vx[DIM], vt = 0.0, vk = 0.0, vsrc = 0.0, vb = 0.0, vu[DIM];
@@ -379,14 +392,18 @@ k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz
         if (stage_b) vb = fd.B[fd.b_c ? 0 : vn];
     }
     if (tid < TW) reinterpret_cast<uint64_t*>(&tb)[tid] = tword;
-    if (tid <= R) {
-        lrp[tid] = vrp - k0;
-        soff[tid] = vip - s0;
-    }
-    if (tid < nw) scl[tid] = vc;
+    if (tid <= R) soff[tid] = vip - s0;
+    if (tid < R) sdg[tid] = vdg;
 #pragma unroll
     for (int u = 0; u < kStagedSlotsPerThread; u++)
-        if (tid + u * NT < S) sw[tid + u * NT] = vs[u];
+        if (tid + u * NT < S) {
+            sw[tid + u * NT] = vs[u];
+            sd[tid + u * NT] = vd[u];
+        }
+    if (tid < nws) ssr[tid] = vsr;
+#pragma unroll
+    for (int u = 0; u < kStagedColsPerThread / 2; u++)
+        if (tid + u * NT < nwe) scp[tid + u * NT] = vcp[u];
     if (tid < NN) {
         const int q = tid;
 #pragma unroll
@@ -400,24 +417,20 @@ k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz
         if (stage_b) sB[q] = vb;
     }
     __syncthreads();
-    for (int l = tid; l < R; l += nt) {
-        for (int k = soff[l]; k < soff[l + 1]; k++) srow[k] = (uint16_t)l;
-        for (int k = lrp[l]; k < lrp[l + 1]; k++) erow[k] = (uint16_t)l;
-    }
-    __syncthreads();
+    const uint8_t* srow = reinterpret_cast<const uint8_t*>(ssr) + (s0 - cs0);
+    const uint16_t* ecpl = reinterpret_cast<const uint16_t*>(scp) + (k0 - ce0);
 
-    // phase A: one element row per slot, from LDS
+    // phase A: one element row per slot, from LDS; its values to their places
     for (int sl = tid; sl < S; sl += nt) {
         __asm__ __volatile__("" ::: "memory");
-        const uint32_t w = sw[sl];
+        const uint32_t w = sw[sl], dw = sd[sl];
         const int l = srow[sl];
+        const int i = (int)(dw & 3u);
         int lid[LOC];
         Elem<DIM, LOC, double> e;
-        int i = 0;
 #pragma unroll
         for (int k = 0; k < LOC; k++) {
             lid[k] = (int)((w >> (8 * k)) & 255u);
-            if (lid[k] == pr + l) i = k;
 #pragma unroll
             for (int a = 0; a < DIM; a++) e.X[a][k] = sX[a * node_cap + lid[k]];
             e.T[k] = sT[lid[k]];
@@ -427,29 +440,24 @@ k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz
         double vals[LOC], r;
         row_linear_g<DIM, LOC, double, FEAT, StagedNodal<DIM, double>, LTabLin<double>>(
             e, gm, tb, fl, StagedNodal<DIM, double>{lid, sU, sK, sS, sB, node_cap}, i, vals, r);
+        const int ks = soff[l];
+        double* cr = sc + LOC * ks;
+        const int dg = sdg[l] + (sl - ks);
 #pragma unroll
-        for (int j = 0; j < LOC; j++) sval[j * slot_cap + sl] = vals[j];
+        for (int j = 0; j < LOC; j++) {
+            const int q = j - (j > i ? 1 : 0);  // number among the off-diagonal vertices
+            const int d = j == i ? dg : (int)((dw >> (2 + 9 * q)) & 511u);
+            cr[d] = vals[j];
+        }
         srhs[sl] = r;
     }
     __syncthreads();
 
-    // phase B: one thread per entry, contributions in slot (= colour) order
-    const uint8_t* cb = reinterpret_cast<const uint8_t*>(scl) + (k0 - c0);
+    // phase B: every entry sums its run of values (slot = colour order)
     for (int t = tid; t < E; t += nt) {
-        const int l = erow[t];
-        const uint32_t rep = (uint32_t)cb[t] * 0x01010101u;
+        const int c0 = ecpl[t], c1 = t + 1 < E ? (int)ecpl[t + 1] : LOC * S;
         double acc = 0.0;
-#pragma unroll 4
-        for (int k = soff[l]; k < soff[l + 1]; k++) {
-            // the byte of the slot word equal to the column byte, if any (the
-            // LOC vertex bytes differ; 255 is no slab node): the lowest set
-            // bit of z marks the lowest zero byte of x
-            const uint32_t x = sw[k] ^ rep;
-            const uint32_t z = (x - 0x01010101u) & ~x & (LOC == 4 ? 0x80808080u : 0x00808080u);
-            const int j = (int)(__builtin_ctz(z | 0x80000000u) >> 3);
-            const double vv = sval[j * slot_cap + k];
-            acc = acc + (z ? vv : 0.0);
-        }
+        for (int k = c0; k < c1; k++) acc = acc + sc[k];
         A[k0 + t] = acc;
     }
     for (int l = tid; l < R; l += nt) {
@@ -461,9 +469,8 @@ k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz
 
 static size_t staged_lds(int dim, int loc, int rows_cap, int nnz_cap, int slot_cap, int node_cap, int nfield) {
     return sizeof(double) * ((size_t)slot_cap * (loc + 1) + (size_t)node_cap * (dim + 1 + nfield)) +
-           sizeof(int32_t) * (2 * (size_t)(rows_cap + 1) + (size_t)slot_cap +
-                              (size_t)((nnz_cap + 6) >> 2)) +
-           sizeof(uint16_t) * ((size_t)slot_cap + nnz_cap);
+           sizeof(int32_t) * ((size_t)(rows_cap + 1) + (size_t)rows_cap + 2 * (size_t)slot_cap +
+                              (size_t)((slot_cap + 6) >> 2) + (size_t)((nnz_cap + 3) >> 1));
 }
 
 static size_t blocked_lds(int loc, int rows_cap, int nnz_cap, int slot_cap) {
@@ -479,8 +486,8 @@ int build_block_plan(flgpu_ctx* c) {
     // LDS budget per slab: four to five workgroups per CU
     // Node staging needs the field nodes to be the coordinate nodes and each
     // slab's node list to hold at most 256 nodes (8-bit local numbers).
-    const bool stage = c->h_conn == c->h_conn_x && getenv("FLGPU_NO_STAGING") == nullptr;
-    int slot_cap = 384;
+    bool stage = c->h_conn == c->h_conn_x && getenv("FLGPU_NO_STAGING") == nullptr;
+    int slot_cap = 512;
     if (const char* s = getenv("FLGPU_BLOCK_SLOTS")) slot_cap = std::max(64, std::min(4096, atoi(s)));
     // the staged kernel loads a slab's rows, entries and slots in one pass
     // of its workgroup: at most one row, four entries and two slots a thread
@@ -499,6 +506,8 @@ int build_block_plan(flgpu_ctx* c) {
         if (len > 255)
             return set_error(FLGPU_EUNSUPPORTED, "block plan: row %d has %d entries (max 255)", i + 1, len);
         if ((int64_t)iptr[i] + d > INT32_MAX) return set_error(FLGPU_EUNSUPPORTED, "block plan: too many slots");
+        // the staged kernel places a row's values with 9-bit offsets
+        if (loc * d > 512) stage = false;
         iptr[i + 1] += iptr[i];
     }
     std::vector<int32_t> inc(iptr[N]);
@@ -509,13 +518,18 @@ int build_block_plan(flgpu_ctx* c) {
             for (int i = 0; i < loc; i++) inc[fill[c->h_conn[(int64_t)e * loc + i]]++] = e;
         }
     }
-    const int node_cap_max = 255;  // local number 255: not a node of the slab
+    const int node_cap_max = 256;
     // slabs of consecutive rows bounded by rows, entries, slots (and nodes)
-    std::vector<int32_t> brow(1, 0), nptr(1, 0), nodes, p0v;
+    std::vector<int32_t> brow(1, 0), nptr(1, 0), nodes;
     std::vector<uint32_t> slotw;
-    std::vector<int32_t> mark(stage ? N : 0, -1), fresh, stamp(stage ? N : 0, -1);
-    // column of every entry as a slab-local node number (4-byte padded)
-    std::vector<uint8_t> lcl(stage ? ((size_t)c->nnz + 4) & ~(size_t)3 : 0, 255);
+    std::vector<int32_t> mark(stage ? N : 0, -1), fresh, stamp(stage ? N : 0, -1), posm(stage ? N : 0, 0);
+    // per slot: destination word and slab-local row (bytes 4-byte padded);
+    // per entry: slab-relative offset of its run of values (padded to whole
+    // words past the end); per row: row-relative offset of the diagonal run
+    std::vector<uint32_t> slotd;
+    std::vector<uint8_t> srb(stage ? ((size_t)iptr[N] + 4) & ~(size_t)3 : 0, 0);
+    std::vector<uint16_t> ecp(stage ? (((size_t)c->nnz + 2) & ~(size_t)1) + 2 : 0, 0);
+    std::vector<uint16_t> rdg(stage ? (size_t)N : 0, 0), cur;
     int max_rows = 0, max_nnz = 0, max_slots = 0, max_nodes = 0;
     for (int r = 0; r < N;) {
         const int slab = (int)brow.size() - 1;
@@ -568,20 +582,41 @@ int build_block_plan(flgpu_ctx* c) {
                 }
             std::sort(nodes.begin() + first, nodes.end());
             for (size_t q = first; q < nodes.size(); q++) mark[nodes[q]] = (int)(q - first);  // local number
-            p0v.push_back(mark[r]);
             for (int rr = r; rr < r1; rr++) {
-                for (int k = c->h_rowptr[rr]; k < c->h_rowptr[rr + 1]; k++) {
-                    const int cj = c->h_col[k];
-                    stamp[cj] = rr;
-                    if (mark[cj] >= 0) lcl[k] = (uint8_t)mark[cj];
+                const int e0 = c->h_rowptr[rr], m = c->h_rowptr[rr + 1] - e0;
+                for (int k = e0; k < e0 + m; k++) {
+                    stamp[c->h_col[k]] = rr;
+                    posm[c->h_col[k]] = k - e0;
                 }
                 // every vertex of the row's elements must be a column of the
                 // row (csr_addto aborts otherwise, Sparse_Tools.F90:2645-2647)
-                for (int q = iptr[rr]; q < iptr[rr + 1]; q++)
-                    for (int v = 0; v < loc; v++)
-                        if (stamp[c->h_conn[(int64_t)inc[q] * loc + v]] != rr)
+                cur.assign((size_t)m + 1, 0);
+                for (int q = iptr[rr]; q < iptr[rr + 1]; q++) {
+                    srb[q] = (uint8_t)(rr - r);
+                    for (int v = 0; v < loc; v++) {
+                        const int g = c->h_conn[(int64_t)inc[q] * loc + v];
+                        if (stamp[g] != rr)
                             return set_error(FLGPU_EPATTERN,
                                              "Attempting to set value in matrix outside sparsity pattern.");
+                        cur[posm[g] + 1]++;
+                    }
+                }
+                // runs of values in entry order; each run in slot order
+                for (int p = 0; p < m; p++) cur[p + 1] += cur[p];
+                const int rel = loc * (iptr[rr] - iptr[r]);
+                for (int p = 0; p < m; p++) ecp[e0 + p] = (uint16_t)(rel + cur[p]);
+                if (iptr[rr + 1] > iptr[rr]) rdg[rr] = cur[posm[rr]];
+                for (int q = iptr[rr]; q < iptr[rr + 1]; q++) {
+                    uint32_t dw = 0;
+                    int nq = 0;
+                    for (int v = 0; v < loc; v++) {
+                        const int g = c->h_conn[(int64_t)inc[q] * loc + v];
+                        const uint32_t d = cur[posm[g]]++;
+                        if (g == rr) dw |= (uint32_t)v;
+                        else dw |= d << (2 + 9 * nq++);
+                    }
+                    slotd.push_back(dw);
+                }
             }
             for (int q = iptr[r]; q < iptr[r1]; q++) {
                 uint32_t w = 0;
@@ -599,9 +634,9 @@ int build_block_plan(flgpu_ctx* c) {
     P.staged = stage;
     if (stage) {
         const int nb = (int)brow.size() - 1;
-        std::vector<int32_t> meta((size_t)nb * 12, 0);
+        std::vector<int32_t> meta((size_t)nb * 8, 0);
         for (int b = 0; b < nb; b++) {
-            int32_t* m = &meta[(size_t)b * 12];
+            int32_t* m = &meta[(size_t)b * 8];
             m[0] = brow[b];
             m[1] = brow[b + 1];
             m[2] = c->h_rowptr[brow[b]];
@@ -610,7 +645,6 @@ int build_block_plan(flgpu_ctx* c) {
             m[5] = iptr[brow[b + 1]];
             m[6] = nptr[b];
             m[7] = nptr[b + 1];
-            m[8] = p0v[b];
         }
         FLGPU_TRY(P.meta.upload(meta.data(), meta.size(), c->stream));
         // the P1 element tables in the layout the kernel keeps in LDS
@@ -633,9 +667,11 @@ int build_block_plan(flgpu_ctx* c) {
         FLGPU_TRY(P.tabw.upload(tw.data(), tw.size(), c->stream));
         FLGPU_TRY(P.nptr.upload(nptr.data(), nptr.size(), c->stream));
         FLGPU_TRY(P.nodes.upload(nodes.data(), nodes.size(), c->stream));
-        FLGPU_TRY(P.p0.upload(p0v.data(), p0v.size(), c->stream));
         FLGPU_TRY(P.slotw.upload(slotw.data(), slotw.size(), c->stream));
-        FLGPU_TRY(P.lcl.upload(lcl.data(), lcl.size(), c->stream));
+        FLGPU_TRY(P.slotd.upload(slotd.data(), slotd.size(), c->stream));
+        FLGPU_TRY(P.srb.upload(srb.data(), srb.size(), c->stream));
+        FLGPU_TRY(P.ecp.upload(ecp.data(), ecp.size(), c->stream));
+        FLGPU_TRY(P.rdg.upload(rdg.data(), rdg.size(), c->stream));
         P.node_cap = max_nodes;
     }
     FLGPU_TRY(P.brow.upload(brow.data(), brow.size(), c->stream));
@@ -682,8 +718,9 @@ static int launch_staged_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags&
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int grid = 8 * ((P.nblocks + 7) / 8);
     hipLaunchKernelGGL((k_asm_staged<DIM, FEAT>), dim3(grid), dim3(kStagedThreads), lds, c->stream, P.tabw.p,
-                       P.nblocks, P.rows_cap, P.nnz_cap, P.slot_cap, P.node_cap, P.meta.p, P.iptr.p, P.nodes.p,
-                       P.slotw.p, P.lcl.p, c->X4.p, fd, fl, su, sk, ss, sb, c->rowptr.p, c->A.p, c->rhs.p);
+                       P.nblocks, P.rows_cap, P.nnz_cap, P.slot_cap, P.node_cap, P.meta.p, P.iptr.p, P.rdg.p,
+                       P.nodes.p, P.slotw.p, P.slotd.p, P.srb.p, P.ecp.p, c->X4.p, fd, fl, su, sk, ss, sb, c->A.p,
+                       c->rhs.p);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }

@@ -111,11 +111,14 @@ struct BlockPlan {
     bool staged = false;
     DevBuf<int32_t> nptr;        // nblocks+1 offsets into nodes
     DevBuf<int32_t> nodes;       // global node ids of each slab, ascending
-    DevBuf<int32_t> p0;          // position of each slab's first row in its node list
-    DevBuf<int32_t> meta;        // per slab: r0 r1 k0 k1 s0 s1 n0 n1 p0 (and 3 pad), one 48-byte record
-    DevBuf<uint8_t> lcl;         // per CSR entry: the column's slab-local node number (255: none)
+    DevBuf<int32_t> meta;        // per slab: r0 r1 k0 k1 s0 s1 n0 n1, one 32-byte record
     DevBuf<uint64_t> tabw;       // the P1 element tables (LTabLin<double>) as 64-bit words
     DevBuf<uint32_t> slotw;      // per slot: 4 x 8-bit slab-local vertex numbers
+    DevBuf<uint32_t> slotd;      // per slot: row-relative places of its off-diagonal values (9 bits
+                                 // each) and the number of the row's vertex in the element (2 bits)
+    DevBuf<uint8_t> srb;         // per slot: slab-local row
+    DevBuf<uint16_t> ecp;        // per CSR entry: slab-relative offset of its run of values
+    DevBuf<uint16_t> rdg;        // per row: row-relative offset of its diagonal run
     int32_t node_cap = 0;
 };
 
