@@ -32,6 +32,7 @@ namespace flgpu {
 constexpr int kStagedThreads = 256;
 constexpr int kStagedColsPerThread = 4;
 constexpr int kStagedSlotsPerThread = 3;
+static_assert(kStagedSlotsPerThread == 3, "phase A picks a thread's slot words by index 0..2");
 
 template <typename R>
 __device__ void stage_tables(const ElemTables* __restrict__ t, LTab<R>& s) {
@@ -313,7 +314,7 @@ struct StagedNodal {
 // sums every entry's run of values in order: the same values in the same
 // order as the colour scheme, so the same bits.
 template <int DIM, int FEAT>
-__global__ void __launch_bounds__(kStagedThreads, 5)
+__global__ void __launch_bounds__(kStagedThreads, (FEAT & (F_ADV | F_ABS)) ? 4 : 5)
 k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz_cap, int slot_cap, int node_cap,
              const int32_t* __restrict__ meta, const int32_t* __restrict__ iptr, const uint16_t* __restrict__ rdg,
              const int32_t* __restrict__ nodes, const uint32_t* __restrict__ slotw,
@@ -336,10 +337,7 @@ k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz
     double* sB = sS + (stage_s ? node_cap : 0);                    // node_cap (stage_b)
     int32_t* soff = reinterpret_cast<int32_t*>(sB + (stage_b ? node_cap : 0));  // rows_cap + 1
     int32_t* sdg = soff + rows_cap + 1;                            // rows_cap: diagonal offsets
-    uint32_t* sw = reinterpret_cast<uint32_t*>(sdg + rows_cap);    // slot_cap: vertex words
-    uint32_t* sd = sw + slot_cap;                                  // slot_cap: destination words
-    uint32_t* ssr = sd + slot_cap;                                 // (slot_cap + 6) / 4: slot row bytes
-    uint32_t* scp = ssr + ((slot_cap + 6) >> 2);                   // (nnz_cap + 3) / 2: entry offsets
+    uint32_t* scp = reinterpret_cast<uint32_t*>(sdg + rows_cap);   // (nnz_cap + 3) / 2: entry offsets
 
     const int per = (nslab + 7) >> 3;
     const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
@@ -347,9 +345,10 @@ k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz
     const int tid = threadIdx.x, nt = NT;
     // Three memory round trips stage the slab: (1) its record and the
     // element tables, (2) what the record locates (slot offsets and
-    // diagonal offsets of the rows, slot words, row bytes, entry offsets,
-    // node ids), (3) the nodes' coordinates and fields.  Every load of a
-    // step is issued before the first of them is used.
+    // diagonal offsets of the rows, entry offsets, node ids, and the words
+    // and rows of the slots this thread computes, kept in registers), (3)
+    // the nodes' coordinates and fields.  Every load of a step is issued
+    // before the first of them is used.
     const uint64_t tword = tid < TW ? tabw[tid] : 0;
     const int4* mp = reinterpret_cast<const int4*>(meta + 8 * (size_t)b);
     const int4 m0 = mp[0], m1 = mp[1];
@@ -359,7 +358,8 @@ k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz
     const int n0 = m1.z, NN = m1.w - n0;
 
     int vip = 0, vdg = 0, vn = 0;
-    uint32_t vs[kStagedSlotsPerThread], vd[kStagedSlotsPerThread], vsr = 0, vcp[kStagedColsPerThread / 2];
+    uint32_t vs[kStagedSlotsPerThread], vd[kStagedSlotsPerThread], vr[kStagedSlotsPerThread];
+    uint32_t vcp[kStagedColsPerThread / 2];
     if (tid <= R) vip = iptr[r0 + tid];
     if (tid < R) vdg = rdg[r0 + tid];
 #pragma unroll
@@ -367,11 +367,10 @@ k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz
         const bool in = tid + u * NT < S;
         vs[u] = in ? slotw[s0 + tid + u * NT] : 0u;
         vd[u] = in ? slotd[s0 + tid + u * NT] : 0u;
+        vr[u] = in ? srb[s0 + tid + u * NT] : 0u;
     }
-    // bytes and half-words as whole words: the slab's start rounded down
-    const int cs0 = s0 & ~3, nws = (s0 + S - cs0 + 3) >> 2;
+    // half-words as whole words: the slab's start rounded down
     const int ce0 = k0 & ~1, nwe = (k0 + E - ce0 + 1) >> 1;
-    if (tid < nws) vsr = reinterpret_cast<const uint32_t*>(srb + cs0)[tid];
 #pragma unroll
     for (int u = 0; u < kStagedColsPerThread / 2; u++)
         vcp[u] = tid + u * NT < nwe ? reinterpret_cast<const uint32_t*>(ecp + ce0)[tid + u * NT] : 0u;
@@ -395,13 +394,6 @@ k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz
     if (tid <= R) soff[tid] = vip - s0;
     if (tid < R) sdg[tid] = vdg;
 #pragma unroll
-    for (int u = 0; u < kStagedSlotsPerThread; u++)
-        if (tid + u * NT < S) {
-            sw[tid + u * NT] = vs[u];
-            sd[tid + u * NT] = vd[u];
-        }
-    if (tid < nws) ssr[tid] = vsr;
-#pragma unroll
     for (int u = 0; u < kStagedColsPerThread / 2; u++)
         if (tid + u * NT < nwe) scp[tid + u * NT] = vcp[u];
     if (tid < NN) {
@@ -417,14 +409,17 @@ k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz
         if (stage_b) sB[q] = vb;
     }
     __syncthreads();
-    const uint8_t* srow = reinterpret_cast<const uint8_t*>(ssr) + (s0 - cs0);
     const uint16_t* ecpl = reinterpret_cast<const uint16_t*>(scp) + (k0 - ce0);
 
     // phase A: one element row per slot, from LDS; its values to their places
-    for (int sl = tid; sl < S; sl += nt) {
+#pragma unroll 1
+    for (int u = 0; u < kStagedSlotsPerThread; u++) {
+        const int sl = tid + u * NT;
+        if (sl >= S) break;
         __asm__ __volatile__("" ::: "memory");
-        const uint32_t w = sw[sl], dw = sd[sl];
-        const int l = srow[sl];
+        const uint32_t w = u == 0 ? vs[0] : u == 1 ? vs[1] : vs[2];
+        const uint32_t dw = u == 0 ? vd[0] : u == 1 ? vd[1] : vd[2];
+        const int l = (int)(u == 0 ? vr[0] : u == 1 ? vr[1] : vr[2]);
         const int i = (int)(dw & 3u);
         int lid[LOC];
         Elem<DIM, LOC, double> e;
@@ -436,7 +431,7 @@ k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz
             e.T[k] = sT[lid[k]];
         }
         GeomLin<DIM, LOC, double> gm;
-        geom_linear<DIM, LOC, double, LTabLin<double>>(e, tb, gm);
+        geom_linear_p1<DIM, LOC, double>(e, gm);
         double vals[LOC], r;
         row_linear_g<DIM, LOC, double, FEAT, StagedNodal<DIM, double>, LTabLin<double>>(
             e, gm, tb, fl, StagedNodal<DIM, double>{lid, sU, sK, sS, sB, node_cap}, i, vals, r);
@@ -469,8 +464,7 @@ k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz
 
 static size_t staged_lds(int dim, int loc, int rows_cap, int nnz_cap, int slot_cap, int node_cap, int nfield) {
     return sizeof(double) * ((size_t)slot_cap * (loc + 1) + (size_t)node_cap * (dim + 1 + nfield)) +
-           sizeof(int32_t) * ((size_t)(rows_cap + 1) + (size_t)rows_cap + 2 * (size_t)slot_cap +
-                              (size_t)((slot_cap + 6) >> 2) + (size_t)((nnz_cap + 3) >> 1));
+           sizeof(int32_t) * ((size_t)(rows_cap + 1) + (size_t)rows_cap + (size_t)((nnz_cap + 3) >> 1));
 }
 
 static size_t blocked_lds(int loc, int rows_cap, int nnz_cap, int slot_cap) {
@@ -487,7 +481,26 @@ int build_block_plan(flgpu_ctx* c) {
     // Node staging needs the field nodes to be the coordinate nodes and each
     // slab's node list to hold at most 256 nodes (8-bit local numbers).
     bool stage = c->h_conn == c->h_conn_x && getenv("FLGPU_NO_STAGING") == nullptr;
-    int slot_cap = 512;
+    // the P1 element tables in the layout the kernel keeps in LDS
+    LTabLin<double> lt;
+    std::memset(&lt, 0, sizeof lt);
+    {
+        const ElemTables& t = c->tab;
+        for (int k = 0; k < t.loc * kMaxDim && k < 4 * kMaxDim; k++) {
+            const int nn = k % t.loc, d = k / t.loc;
+            lt.dn[nn + t.loc * d] = t.dn[nn + t.loc * (0 + t.ngi * d)];
+        }
+        for (int k = 0; k < 4; k++) lt.what[k] = t.what[k];
+        for (int k = 0; k < 16; k++) lt.mhat[k] = t.mhat[k];
+        for (int k = 0; k < 64; k++) lt.that[k] = t.that[k];
+        for (int k = 0; k < 4 * kMaxDim; k++) lt.xdn[k] = t.xdn[k];
+        lt.loc = t.loc;
+        lt.ngi = 1;
+        lt.lx = t.loc_x;
+    }
+    // the staged kernel's geometry assumes the standard P1 tables
+    if (!p1_standard_tables(lt, c->dim)) stage = false;
+    int slot_cap = 384;
     if (const char* s = getenv("FLGPU_BLOCK_SLOTS")) slot_cap = std::max(64, std::min(4096, atoi(s)));
     // the staged kernel loads a slab's rows, entries and slots in one pass
     // of its workgroup: at most one row, four entries and two slots a thread
@@ -518,7 +531,8 @@ int build_block_plan(flgpu_ctx* c) {
             for (int i = 0; i < loc; i++) inc[fill[c->h_conn[(int64_t)e * loc + i]]++] = e;
         }
     }
-    const int node_cap_max = 256;
+    int node_cap_max = 192;
+    if (const char* s = getenv("FLGPU_BLOCK_NODES")) node_cap_max = std::max(64, std::min(256, atoi(s)));
     // slabs of consecutive rows bounded by rows, entries, slots (and nodes)
     std::vector<int32_t> brow(1, 0), nptr(1, 0), nodes;
     std::vector<uint32_t> slotw;
@@ -552,7 +566,8 @@ int build_block_plan(flgpu_ctx* c) {
                     }
                 if (nn + (int)fresh.size() > node_cap_max) {
                     for (int g : fresh) mark[g] = -1;
-                    if (r1 == r) return set_error(FLGPU_EUNSUPPORTED, "block plan: row %d touches > 256 nodes", r + 1);
+                    if (r1 == r) return set_error(FLGPU_EUNSUPPORTED, "block plan: row %d touches > %d nodes", r + 1,
+                                                    node_cap_max);
                     break;
                 }
                 for (int g : fresh) mark[g] = slab;
@@ -647,21 +662,6 @@ int build_block_plan(flgpu_ctx* c) {
             m[7] = nptr[b + 1];
         }
         FLGPU_TRY(P.meta.upload(meta.data(), meta.size(), c->stream));
-        // the P1 element tables in the layout the kernel keeps in LDS
-        LTabLin<double> lt;
-        std::memset(&lt, 0, sizeof lt);
-        const ElemTables& t = c->tab;
-        for (int k = 0; k < t.loc * kMaxDim && k < 4 * kMaxDim; k++) {
-            const int nn = k % t.loc, d = k / t.loc;
-            lt.dn[nn + t.loc * d] = t.dn[nn + t.loc * (0 + t.ngi * d)];
-        }
-        for (int k = 0; k < 4; k++) lt.what[k] = t.what[k];
-        for (int k = 0; k < 16; k++) lt.mhat[k] = t.mhat[k];
-        for (int k = 0; k < 64; k++) lt.that[k] = t.that[k];
-        for (int k = 0; k < 4 * kMaxDim; k++) lt.xdn[k] = t.xdn[k];
-        lt.loc = t.loc;
-        lt.ngi = 1;
-        lt.lx = t.loc_x;
         std::vector<uint64_t> tw(sizeof lt / sizeof(uint64_t));
         std::memcpy(tw.data(), &lt, sizeof lt);
         FLGPU_TRY(P.tabw.upload(tw.data(), tw.size(), c->stream));

@@ -87,20 +87,11 @@ struct Elem {
     R T[LOC];
 };
 
-// Jacobian of the linear coordinate element, its cofactor inverse and
-// determinant (construct_cache, femtools/Transform_elements.F90:263-304).
-template <int DIM, typename R, class TB = LTab<R>>
-FLGPU_HD_INLINE void geometry(const R (&X)[DIM][DIM + 1], const TB& tb, R (&iJ)[DIM][DIM], R& adet) {
-    R JT[DIM][DIM];
-#pragma unroll
-    for (int a = 0; a < DIM; a++)
-#pragma unroll
-        for (int b = 0; b < DIM; b++) {
-            R s = (R)0;
-#pragma unroll
-            for (int nn = 0; nn < DIM + 1; nn++) s = FMA(X[a][nn], tb.xdn[nn + (DIM + 1) * b], s);
-            JT[a][b] = s;
-        }
+// Cofactor inverse and determinant of a Jacobian (transposed, JT[a][b] =
+// dx_a/dxi_b), as construct_cache forms them
+// (femtools/Transform_elements.F90:263-304).
+template <int DIM, typename R>
+FLGPU_HD_INLINE void invert_jacobian(const R (&JT)[DIM][DIM], R (&iJ)[DIM][DIM], R& adet) {
     R C[DIM][DIM];
     if constexpr (DIM == 3) {
 #pragma unroll
@@ -125,6 +116,23 @@ FLGPU_HD_INLINE void geometry(const R (&X)[DIM][DIM + 1], const TB& tb, R (&iJ)[
 #pragma unroll
         for (int k = 0; k < DIM; k++) iJ[i][k] = C[i][k] * rdet;
     adet = fabs(det);
+}
+
+// Jacobian of the linear coordinate element from its derivative table,
+// then its inverse and determinant.
+template <int DIM, typename R, class TB = LTab<R>>
+FLGPU_HD_INLINE void geometry(const R (&X)[DIM][DIM + 1], const TB& tb, R (&iJ)[DIM][DIM], R& adet) {
+    R JT[DIM][DIM];
+#pragma unroll
+    for (int a = 0; a < DIM; a++)
+#pragma unroll
+        for (int b = 0; b < DIM; b++) {
+            R s = (R)0;
+#pragma unroll
+            for (int nn = 0; nn < DIM + 1; nn++) s = FMA(X[a][nn], tb.xdn[nn + (DIM + 1) * b], s);
+            JT[a][b] = s;
+        }
+    invert_jacobian<DIM, R>(JT, iJ, adet);
 }
 
 template <typename R>
@@ -163,6 +171,45 @@ FLGPU_HD_INLINE void geom_linear(const Elem<DIM, LOC, R>& e, const TB& tb, GeomL
             for (int k = 0; k < DIM; k++) s = FMA(tb.dn[nn + LOC * (0 + tb.ngi * k)], iJ[d][k], s);
             gm.G[nn][d] = s;
         }
+}
+
+// geom_linear for the standard P1 simplex tables (dn(n,k) = [n == k] for
+// n < DIM, dn(DIM,k) = -1, the same for the coordinate element), with the
+// multiplications by 0 and +-1 left out.  Each remaining sum runs in the
+// table order, so the results equal geom_linear's bit for bit (zeros
+// aside, whose sign may differ).
+template <int DIM, int LOC, typename R>
+FLGPU_HD_INLINE void geom_linear_p1(const Elem<DIM, LOC, R>& e, GeomLin<DIM, LOC, R>& gm) {
+    static_assert(LOC == DIM + 1, "linear simplex");
+    R JT[DIM][DIM], iJ[DIM][DIM];
+#pragma unroll
+    for (int a = 0; a < DIM; a++)
+#pragma unroll
+        for (int b = 0; b < DIM; b++) JT[a][b] = e.X[a][b] - e.X[a][DIM];
+    invert_jacobian<DIM, R>(JT, iJ, gm.adet);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        R s = -iJ[d][0];
+#pragma unroll
+        for (int k = 0; k < DIM; k++) {
+            gm.G[k][d] = iJ[d][k];
+            if (k > 0) s = s - iJ[d][k];
+        }
+        gm.G[DIM][d] = s;
+    }
+}
+
+// True when the tables are the standard P1 simplex ones geom_linear_p1
+// assumes: dn laid out dn[n + loc*k] (ngi = 1), xdn[n + (dim+1)*k].
+template <class TB>
+FLGPU_HD_INLINE bool p1_standard_tables(const TB& tb, int dim) {
+    const int loc = dim + 1;
+    for (int n = 0; n < loc; n++)
+        for (int k = 0; k < dim; k++) {
+            const double want = n == dim ? -1.0 : (n == k ? 1.0 : 0.0);
+            if ((double)tb.dn[n + loc * k] != want || (double)tb.xdn[n + loc * k] != want) return false;
+        }
+    return true;
 }
 
 // Row i of the element matrix (vals[j], j = 0..LOC-1) and of the element
