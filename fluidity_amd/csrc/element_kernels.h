@@ -353,14 +353,17 @@ FLGPU_HD_INLINE void row_linear_g(const Elem<DIM, LOC, R>& e, const GeomLin<DIM,
             }
         }
     }
+    // A term compiled in but switched off holds zeros, and adding its
+    // products leaves a sum unchanged (up to the sign of a zero): no
+    // run-time test in the sums below.
     R r = (R)0;
-    if (fl.adv)
+    if constexpr ((FEAT & F_ADV) != 0)
 #pragma unroll
         for (int j = 0; j < LOC; j++) r = FMA(-Aa[j], e.T[j], r);
-    if (fl.absorp)
+    if constexpr ((FEAT & F_ABS) != 0)
 #pragma unroll
         for (int j = 0; j < LOC; j++) r = FMA(-Bb[j], e.T[j], r);
-    if (fl.diff)
+    if constexpr ((FEAT & F_DIFF) != 0)
 #pragma unroll
         for (int j = 0; j < LOC; j++) r = FMA(-Kk[j], e.T[j], r);
     if (fl.src) {
@@ -375,15 +378,14 @@ FLGPU_HD_INLINE void row_linear_g(const Elem<DIM, LOC, R>& e, const GeomLin<DIM,
 #pragma unroll
         for (int j = 0; j < LOC; j++) lump += M[j];
     }
+    const R dtte = fl.dtt_on ? dtt : (R)0;
 #pragma unroll
     for (int j = 0; j < LOC; j++) {
         R m = (fl.have_mass && !fl.lump) ? M[j] : (R)0;
         if (fl.have_mass && fl.lump && j == i) m = lump;
-        if (fl.dtt_on) {
-            if (fl.adv) m = FMA(dtt, Aa[j], m);
-            if (fl.absorp) m = FMA(dtt, Bb[j], m);
-            if (fl.diff) m = FMA(dtt, Kk[j], m);
-        }
+        if constexpr ((FEAT & F_ADV) != 0) m = FMA(dtte, Aa[j], m);
+        if constexpr ((FEAT & F_ABS) != 0) m = FMA(dtte, Bb[j], m);
+        if constexpr ((FEAT & F_DIFF) != 0) m = FMA(dtte, Kk[j], m);
         vals[j] = m;
     }
 }
