@@ -793,8 +793,12 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     if (scheme != FLGPU_ASM_COLOUR && scheme != FLGPU_ASM_BLOCKED)
         return set_error(FLGPU_EINVAL, "flgpu assemble: scheme %d", p->scheme);
 
-    FLGPU_HIP(hipMemsetAsync(c->inactive.p, 0, sizeof(int32_t) * c->rows, c->stream));
-    c->have_inactive = false;
+    // a fresh assembly has no Dirichlet rows (the mask is all zero unless a
+    // Dirichlet application or an upload set it)
+    if (c->have_inactive) {
+        FLGPU_HIP(hipMemsetAsync(c->inactive.p, 0, sizeof(int32_t) * c->rows, c->stream));
+        c->have_inactive = false;
+    }
     if (scheme == FLGPU_ASM_BLOCKED) {
         FLGPU_TRY(build_block_plan(c));
         return dim == 3 ? launch_blocked<3>(c, fd, fl, d_err) : launch_blocked<2>(c, fd, fl, d_err);
