@@ -313,131 +313,194 @@ struct StagedNodal {
 // of a row are consecutive from the row's diagonal offset.  Phase B then
 // sums every entry's run of values in order: the same values in the same
 // order as the colour scheme, so the same bits.
+
+// The plan arrays and capacities the node-staged kernels read.
+struct StagedPlan {
+    const uint64_t* tabw;
+    const int32_t *meta, *iptr, *nodes;
+    const uint16_t *rdg, *ecp;
+    const uint32_t *slotw, *slotd;
+    const uint8_t* srb;
+    int nslab, rows_cap, nnz_cap, slot_cap, node_cap;
+    int stage_u, stage_k, stage_s, stage_b;
+};
+
+// A slab's ranges (its 32-byte record) and its entry offsets' whole words.
+struct SlabMeta {
+    int r0, R, k0, E, s0, S, n0, NN, ce0, nwe;
+};
+
+__device__ __forceinline__ SlabMeta slab_meta(const int32_t* m) {
+    SlabMeta s;
+    s.r0 = m[0];
+    s.R = m[1] - m[0];
+    s.k0 = m[2];
+    s.E = m[3] - m[2];
+    s.s0 = m[4];
+    s.S = m[5] - m[4];
+    s.n0 = m[6];
+    s.NN = m[7] - m[6];
+    s.ce0 = s.k0 & ~1;                            // half-words as whole words
+    s.nwe = (s.k0 + s.E - s.ce0 + 1) >> 1;
+    return s;
+}
+
+// What a slab's record locates, per thread: its row's slot and diagonal
+// offsets, its node id, its entry-offset words, and the vertex words,
+// destination words and row bytes of the slots it computes.
+// (Scalar members, not arrays: phase A picks a slot's words by its loop
+// counter, which on an array would be a scratch-memory index.)
+struct SlabWords {
+    int vip, vdg, vn;
+    uint32_t vs0, vs1, vs2, vd0, vd1, vd2, vr0, vr1, vr2, vcp0, vcp1;
+};
+static_assert(kStagedSlotsPerThread == 3 && kStagedColsPerThread == 4, "SlabWords holds 3 slots, 2 offset words");
+
+__device__ __forceinline__ void load_slab_words(const StagedPlan& sp, const SlabMeta& m, int tid, SlabWords& w) {
+    constexpr int NT = kStagedThreads;
+    w.vip = tid <= m.R ? sp.iptr[m.r0 + tid] : 0;
+    w.vdg = tid < m.R ? sp.rdg[m.r0 + tid] : 0;
+    const uint32_t* sw = sp.slotw + m.s0 + tid;
+    const uint32_t* sd = sp.slotd + m.s0 + tid;
+    const uint8_t* sr = sp.srb + m.s0 + tid;
+    const bool in0 = tid < m.S, in1 = tid + NT < m.S, in2 = tid + 2 * NT < m.S;
+    w.vs0 = in0 ? sw[0] : 0u;
+    w.vd0 = in0 ? sd[0] : 0u;
+    w.vr0 = in0 ? sr[0] : 0u;
+    w.vs1 = in1 ? sw[NT] : 0u;
+    w.vd1 = in1 ? sd[NT] : 0u;
+    w.vr1 = in1 ? sr[NT] : 0u;
+    w.vs2 = in2 ? sw[2 * NT] : 0u;
+    w.vd2 = in2 ? sd[2 * NT] : 0u;
+    w.vr2 = in2 ? sr[2 * NT] : 0u;
+    const uint32_t* cp = reinterpret_cast<const uint32_t*>(sp.ecp + m.ce0) + tid;
+    w.vcp0 = tid < m.nwe ? cp[0] : 0u;
+    w.vcp1 = tid + NT < m.nwe ? cp[NT] : 0u;
+    w.vn = tid < m.NN ? sp.nodes[m.n0 + tid] : 0;
+}
+
+// One staged node's coordinates and fields (thread q holds node q).
+template <int DIM>
+struct NodeVals {
+    double x[DIM], t, k, s, b, u[DIM];
+};
+
+template <int DIM>
+__device__ __forceinline__ void load_node_vals(const StagedPlan& sp, const double* __restrict__ X4,
+                                               const AsmFields& fd, int vn, bool have, NodeVals<DIM>& v) {
+#pragma unroll
+    for (int a = 0; a < DIM; a++) v.x[a] = v.u[a] = 0.0;
+    v.t = v.k = v.s = v.b = 0.0;
+    if (have) {
+#pragma unroll
+        for (int a = 0; a < DIM; a++) v.x[a] = X4[(int64_t)vn * 4 + a];
+        v.t = fd.T[vn];
+        if (sp.stage_u)
+#pragma unroll
+            for (int a = 0; a < DIM; a++) v.u[a] = fd.U[fd.u_c ? a : (int64_t)vn * DIM + a];
+        if (sp.stage_k) v.k = fd.K[fd.k_c ? 0 : (int64_t)vn * DIM * DIM];
+        if (sp.stage_s) v.s = fd.S[fd.s_c ? 0 : vn];
+        if (sp.stage_b) v.b = fd.B[fd.b_c ? 0 : vn];
+    }
+}
+
+// A slab's LDS copy: node data (struct of arrays), row slot offsets, row
+// diagonal offsets, entry offsets.
+struct SlabLds {
+    double *sX, *sT, *sU, *sK, *sS, *sB;
+    int32_t *soff, *sdg;
+    uint32_t* scp;
+};
+
+template <int DIM>
+__host__ __device__ __forceinline__ size_t slab_lds_bytes(const StagedPlan& sp) {
+    const size_t nf = (size_t)DIM + 1 + (sp.stage_u ? DIM : 0) + sp.stage_k + sp.stage_s + sp.stage_b;
+    const size_t b = sizeof(double) * nf * sp.node_cap +
+                     sizeof(int32_t) * ((size_t)(sp.rows_cap + 1) + sp.rows_cap + ((sp.nnz_cap + 3) >> 1));
+    return (b + 15) & ~(size_t)15;
+}
+
+template <int DIM>
+__device__ __forceinline__ SlabLds slab_lds(char* base, const StagedPlan& sp) {
+    SlabLds L;
+    const int nc = sp.node_cap;
+    L.sX = reinterpret_cast<double*>(base);
+    L.sT = L.sX + (size_t)DIM * nc;
+    L.sU = L.sT + nc;
+    L.sK = L.sU + (sp.stage_u ? (size_t)DIM * nc : 0);
+    L.sS = L.sK + (sp.stage_k ? nc : 0);
+    L.sB = L.sS + (sp.stage_s ? nc : 0);
+    L.soff = reinterpret_cast<int32_t*>(L.sB + (sp.stage_b ? nc : 0));
+    L.sdg = L.soff + sp.rows_cap + 1;
+    L.scp = reinterpret_cast<uint32_t*>(L.sdg + sp.rows_cap);
+    return L;
+}
+
+__device__ __forceinline__ void store_slab_words(const SlabWords& w, const SlabMeta& m, const SlabLds& L,
+                                                 int tid) {
+    constexpr int NT = kStagedThreads;
+    if (tid <= m.R) L.soff[tid] = w.vip - m.s0;
+    if (tid < m.R) L.sdg[tid] = w.vdg;
+    if (tid < m.nwe) L.scp[tid] = w.vcp0;
+    if (tid + NT < m.nwe) L.scp[tid + NT] = w.vcp1;
+}
+
+template <int DIM>
+__device__ __forceinline__ void store_node_vals(const StagedPlan& sp, const NodeVals<DIM>& v, int NN,
+                                                const SlabLds& L, int q) {
+    if (q >= NN) return;
+    const int nc = sp.node_cap;
+#pragma unroll
+    for (int a = 0; a < DIM; a++) L.sX[a * nc + q] = v.x[a];
+    L.sT[q] = v.t;
+    if (sp.stage_u)
+#pragma unroll
+        for (int a = 0; a < DIM; a++) L.sU[a * nc + q] = v.u[a];
+    if (sp.stage_k) L.sK[q] = v.k;
+    if (sp.stage_s) L.sS[q] = v.s;
+    if (sp.stage_b) L.sB[q] = v.b;
+}
+
+// Phase A: one element row per slot, from LDS; its values to their places.
 template <int DIM, int FEAT>
-__global__ void __launch_bounds__(kStagedThreads, (FEAT & (F_ADV | F_ABS)) ? 4 : 5)
-k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz_cap, int slot_cap, int node_cap,
-             const int32_t* __restrict__ meta, const int32_t* __restrict__ iptr, const uint16_t* __restrict__ rdg,
-             const int32_t* __restrict__ nodes, const uint32_t* __restrict__ slotw,
-             const uint32_t* __restrict__ slotd, const uint8_t* __restrict__ srb, const uint16_t* __restrict__ ecp,
-             const double* __restrict__ X4, AsmFields fd, AsmFlags fl, int stage_u, int stage_k, int stage_s,
-             int stage_b, double* __restrict__ A, double* __restrict__ rhs) {
+__device__ __forceinline__ void staged_phase_a(const LTabLin<double>& tb, const AsmFlags& fl, const StagedPlan& sp,
+                                               const SlabMeta& m, const SlabLds& L, const SlabWords& w,
+                                               double* sc, double* srhs, int tid) {
     constexpr int LOC = DIM + 1;
     constexpr int NT = kStagedThreads;
-    constexpr int TW = (int)(sizeof(LTabLin<double>) / sizeof(uint64_t));
-    static_assert(TW <= NT, "one table word per thread");
-    __shared__ LTabLin<double> tb;
-    extern __shared__ double dyn[];
-    double* sc = dyn;                                              // LOC * slot_cap: the contributions
-    double* srhs = sc + (size_t)slot_cap * LOC;                    // slot_cap
-    double* sX = srhs + slot_cap;                                  // DIM * node_cap
-    double* sT = sX + (size_t)DIM * node_cap;                      // node_cap
-    double* sU = sT + node_cap;                                    // DIM * node_cap (stage_u)
-    double* sK = sU + (stage_u ? (size_t)DIM * node_cap : 0);      // node_cap (stage_k)
-    double* sS = sK + (stage_k ? node_cap : 0);                    // node_cap (stage_s)
-    double* sB = sS + (stage_s ? node_cap : 0);                    // node_cap (stage_b)
-    int32_t* soff = reinterpret_cast<int32_t*>(sB + (stage_b ? node_cap : 0));  // rows_cap + 1
-    int32_t* sdg = soff + rows_cap + 1;                            // rows_cap: diagonal offsets
-    uint32_t* scp = reinterpret_cast<uint32_t*>(sdg + rows_cap);   // (nnz_cap + 3) / 2: entry offsets
-
-    const int per = (nslab + 7) >> 3;
-    const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    if (b >= nslab) return;
-    const int tid = threadIdx.x, nt = NT;
-    // Three memory round trips stage the slab: (1) its record and the
-    // element tables, (2) what the record locates (slot offsets and
-    // diagonal offsets of the rows, entry offsets, node ids, and the words
-    // and rows of the slots this thread computes, kept in registers), (3)
-    // the nodes' coordinates and fields.  Every load of a step is issued
-    // before the first of them is used.
-    const uint64_t tword = tid < TW ? tabw[tid] : 0;
-    const int4* mp = reinterpret_cast<const int4*>(meta + 8 * (size_t)b);
-    const int4 m0 = mp[0], m1 = mp[1];
-    const int r0 = m0.x, r1 = m0.y, R = r1 - r0;
-    const int k0 = m0.z, E = m0.w - k0;
-    const int s0 = m1.x, S = m1.y - s0;
-    const int n0 = m1.z, NN = m1.w - n0;
-
-    int vip = 0, vdg = 0, vn = 0;
-    uint32_t vs[kStagedSlotsPerThread], vd[kStagedSlotsPerThread], vr[kStagedSlotsPerThread];
-    uint32_t vcp[kStagedColsPerThread / 2];
-    if (tid <= R) vip = iptr[r0 + tid];
-    if (tid < R) vdg = rdg[r0 + tid];
-#pragma unroll
-    for (int u = 0; u < kStagedSlotsPerThread; u++) {
-        const bool in = tid + u * NT < S;
-        vs[u] = in ? slotw[s0 + tid + u * NT] : 0u;
-        vd[u] = in ? slotd[s0 + tid + u * NT] : 0u;
-        vr[u] = in ? srb[s0 + tid + u * NT] : 0u;
-    }
-    // half-words as whole words: the slab's start rounded down
-    const int ce0 = k0 & ~1, nwe = (k0 + E - ce0 + 1) >> 1;
-#pragma unroll
-    for (int u = 0; u < kStagedColsPerThread / 2; u++)
-        vcp[u] = tid + u * NT < nwe ? reinterpret_cast<const uint32_t*>(ecp + ce0)[tid + u * NT] : 0u;
-    if (tid < NN) vn = nodes[n0 + tid];
-
-    double vx[DIM], vt = 0.0, vk = 0.0, vsrc = 0.0, vb = 0.0, vu[DIM];
-#pragma unroll
-    for (int a = 0; a < DIM; a++) vx[a] = vu[a] = 0.0;
-    if (tid < NN) {
-#pragma unroll
-        for (int a = 0; a < DIM; a++) vx[a] = X4[(int64_t)vn * 4 + a];
-        vt = fd.T[vn];
-        if (stage_u)
-#pragma unroll
-            for (int a = 0; a < DIM; a++) vu[a] = fd.U[fd.u_c ? a : (int64_t)vn * DIM + a];
-        if (stage_k) vk = fd.K[fd.k_c ? 0 : (int64_t)vn * DIM * DIM];
-        if (stage_s) vsrc = fd.S[fd.s_c ? 0 : vn];
-        if (stage_b) vb = fd.B[fd.b_c ? 0 : vn];
-    }
-    if (tid < TW) reinterpret_cast<uint64_t*>(&tb)[tid] = tword;
-    if (tid <= R) soff[tid] = vip - s0;
-    if (tid < R) sdg[tid] = vdg;
-#pragma unroll
-    for (int u = 0; u < kStagedColsPerThread / 2; u++)
-        if (tid + u * NT < nwe) scp[tid + u * NT] = vcp[u];
-    if (tid < NN) {
-        const int q = tid;
-#pragma unroll
-        for (int a = 0; a < DIM; a++) sX[a * node_cap + q] = vx[a];
-        sT[q] = vt;
-        if (stage_u)
-#pragma unroll
-            for (int a = 0; a < DIM; a++) sU[a * node_cap + q] = vu[a];
-        if (stage_k) sK[q] = vk;
-        if (stage_s) sS[q] = vsrc;
-        if (stage_b) sB[q] = vb;
-    }
-    __syncthreads();
-    const uint16_t* ecpl = reinterpret_cast<const uint16_t*>(scp) + (k0 - ce0);
-
-    // phase A: one element row per slot, from LDS; its values to their places
+    const int nc = sp.node_cap;
+    // the slot words as plain register values (the empty asm keeps the
+    // per-slot choice below a choice of values, not of addresses into a
+    // scratch copy of the struct)
+    uint32_t s0 = w.vs0, s1 = w.vs1, s2 = w.vs2, d0 = w.vd0, d1 = w.vd1, d2 = w.vd2;
+    uint32_t q0 = w.vr0, q1 = w.vr1, q2 = w.vr2;
+    __asm__("" : "+v"(s0), "+v"(s1), "+v"(s2), "+v"(d0), "+v"(d1), "+v"(d2), "+v"(q0), "+v"(q1), "+v"(q2));
 #pragma unroll 1
     for (int u = 0; u < kStagedSlotsPerThread; u++) {
         const int sl = tid + u * NT;
-        if (sl >= S) break;
+        if (sl >= m.S) break;
         __asm__ __volatile__("" ::: "memory");
-        const uint32_t w = u == 0 ? vs[0] : u == 1 ? vs[1] : vs[2];
-        const uint32_t dw = u == 0 ? vd[0] : u == 1 ? vd[1] : vd[2];
-        const int l = (int)(u == 0 ? vr[0] : u == 1 ? vr[1] : vr[2]);
+        const uint32_t vw = u == 0 ? s0 : u == 1 ? s1 : s2;
+        const uint32_t dw = u == 0 ? d0 : u == 1 ? d1 : d2;
+        const int l = (int)(u == 0 ? q0 : u == 1 ? q1 : q2);
         const int i = (int)(dw & 3u);
         int lid[LOC];
         Elem<DIM, LOC, double> e;
 #pragma unroll
         for (int k = 0; k < LOC; k++) {
-            lid[k] = (int)((w >> (8 * k)) & 255u);
+            lid[k] = (int)((vw >> (8 * k)) & 255u);
 #pragma unroll
-            for (int a = 0; a < DIM; a++) e.X[a][k] = sX[a * node_cap + lid[k]];
-            e.T[k] = sT[lid[k]];
+            for (int a = 0; a < DIM; a++) e.X[a][k] = L.sX[a * nc + lid[k]];
+            e.T[k] = L.sT[lid[k]];
         }
         GeomLin<DIM, LOC, double> gm;
         geom_linear_p1<DIM, LOC, double>(e, gm);
         double vals[LOC], r;
         row_linear_g<DIM, LOC, double, FEAT, StagedNodal<DIM, double>, LTabLin<double>>(
-            e, gm, tb, fl, StagedNodal<DIM, double>{lid, sU, sK, sS, sB, node_cap}, i, vals, r);
-        const int ks = soff[l];
+            e, gm, tb, fl, StagedNodal<DIM, double>{lid, L.sU, L.sK, L.sS, L.sB, nc}, i, vals, r);
+        const int ks = L.soff[l];
         double* cr = sc + LOC * ks;
-        const int dg = sdg[l] + (sl - ks);
+        const int dg = L.sdg[l] + (sl - ks);
 #pragma unroll
         for (int j = 0; j < LOC; j++) {
             const int q = j - (j > i ? 1 : 0);  // number among the off-diagonal vertices
@@ -446,25 +509,63 @@ k_asm_staged(const uint64_t* __restrict__ tabw, int nslab, int rows_cap, int nnz
         }
         srhs[sl] = r;
     }
-    __syncthreads();
+}
 
-    // phase B: every entry sums its run of values (slot = colour order)
-    for (int t = tid; t < E; t += nt) {
-        const int c0 = ecpl[t], c1 = t + 1 < E ? (int)ecpl[t + 1] : LOC * S;
+// Phase B: every entry sums its run of values (slot = colour order); every
+// row its slots' rhs values.
+template <int LOC>
+__device__ __forceinline__ void staged_phase_b(const SlabMeta& m, const SlabLds& L, const double* sc,
+                                               const double* srhs, double* __restrict__ A,
+                                               double* __restrict__ rhs, int tid) {
+    constexpr int NT = kStagedThreads;
+    const uint16_t* ecpl = reinterpret_cast<const uint16_t*>(L.scp) + (m.k0 - m.ce0);
+    for (int t = tid; t < m.E; t += NT) {
+        const int c0 = ecpl[t], c1 = t + 1 < m.E ? (int)ecpl[t + 1] : LOC * m.S;
         double acc = 0.0;
         for (int k = c0; k < c1; k++) acc = acc + sc[k];
-        A[k0 + t] = acc;
+        A[m.k0 + t] = acc;
     }
-    for (int l = tid; l < R; l += nt) {
+    for (int l = tid; l < m.R; l += NT) {
         double acc = 0.0;
-        for (int k = soff[l]; k < soff[l + 1]; k++) acc = acc + srhs[k];
-        rhs[r0 + l] = acc;
+        for (int k = L.soff[l]; k < L.soff[l + 1]; k++) acc = acc + srhs[k];
+        rhs[m.r0 + l] = acc;
     }
 }
 
-static size_t staged_lds(int dim, int loc, int rows_cap, int nnz_cap, int slot_cap, int node_cap, int nfield) {
-    return sizeof(double) * ((size_t)slot_cap * (loc + 1) + (size_t)node_cap * (dim + 1 + nfield)) +
-           sizeof(int32_t) * ((size_t)(rows_cap + 1) + (size_t)rows_cap + (size_t)((nnz_cap + 3) >> 1));
+// One workgroup per slab.  Three memory round trips stage the slab: (1)
+// its record and the element tables, (2) what the record locates, (3) the
+// nodes' coordinates and fields; every load of a step is issued before the
+// first of them is used.
+template <int DIM, int FEAT>
+__global__ void __launch_bounds__(kStagedThreads, (FEAT & (F_ADV | F_ABS)) ? 4 : 5)
+k_asm_staged(StagedPlan sp, const double* __restrict__ X4, AsmFields fd, AsmFlags fl, double* __restrict__ A,
+             double* __restrict__ rhs) {
+    constexpr int LOC = DIM + 1;
+    constexpr int TW = (int)(sizeof(LTabLin<double>) / sizeof(uint64_t));
+    static_assert(TW <= kStagedThreads, "one table word per thread");
+    __shared__ LTabLin<double> tb;
+    extern __shared__ double dyn[];
+    double* sc = dyn;                                              // LOC * slot_cap: the contributions
+    double* srhs = sc + (size_t)sp.slot_cap * LOC;                 // slot_cap
+    const SlabLds L = slab_lds<DIM>(reinterpret_cast<char*>(srhs + sp.slot_cap), sp);
+
+    const int per = (sp.nslab + 7) >> 3;
+    const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (b >= sp.nslab) return;
+    const int tid = threadIdx.x;
+    const uint64_t tword = tid < TW ? sp.tabw[tid] : 0;
+    const SlabMeta m = slab_meta(sp.meta + 8 * (size_t)b);
+    SlabWords w;
+    load_slab_words(sp, m, tid, w);
+    NodeVals<DIM> nv;
+    load_node_vals<DIM>(sp, X4, fd, w.vn, tid < m.NN, nv);
+    if (tid < TW) reinterpret_cast<uint64_t*>(&tb)[tid] = tword;
+    store_slab_words(w, m, L, tid);
+    store_node_vals<DIM>(sp, nv, m.NN, L, tid);
+    __syncthreads();
+    staged_phase_a<DIM, FEAT>(tb, fl, sp, m, L, w, sc, srhs, tid);
+    __syncthreads();
+    staged_phase_b<LOC>(m, L, sc, srhs, A, rhs, tid);
 }
 
 static size_t blocked_lds(int loc, int rows_cap, int nnz_cap, int slot_cap) {
@@ -707,20 +808,35 @@ template <int DIM, int FEAT>
 static int launch_staged_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
     const BlockPlan& P = c->plan;
     constexpr int LOC = DIM + 1;
+    StagedPlan sp;
+    sp.tabw = P.tabw.p;
+    sp.meta = P.meta.p;
+    sp.iptr = P.iptr.p;
+    sp.nodes = P.nodes.p;
+    sp.rdg = P.rdg.p;
+    sp.ecp = P.ecp.p;
+    sp.slotw = P.slotw.p;
+    sp.slotd = P.slotd.p;
+    sp.srb = P.srb.p;
+    sp.nslab = P.nblocks;
+    sp.rows_cap = P.rows_cap;
+    sp.nnz_cap = P.nnz_cap;
+    sp.slot_cap = P.slot_cap;
+    sp.node_cap = P.node_cap;
     // the fields the compiled terms read, staged per node (constant ones too)
-    const int su = (FEAT & F_ADV) && fl.adv, sk = (FEAT & F_DIFF) && fl.diff;
-    const int ss = (FEAT & F_SRC) && fl.src, sb = (FEAT & F_ABS) && fl.absorp;
-    const int nfield = su * DIM + sk + ss + sb;
-    const size_t lds = staged_lds(DIM, LOC, P.rows_cap, P.nnz_cap, P.slot_cap, P.node_cap, nfield);
+    sp.stage_u = (FEAT & F_ADV) && fl.adv;
+    sp.stage_k = (FEAT & F_DIFF) && fl.diff;
+    sp.stage_s = (FEAT & F_SRC) && fl.src;
+    sp.stage_b = (FEAT & F_ABS) && fl.absorp;
+    const size_t head = sizeof(double) * (size_t)P.slot_cap * (LOC + 1), bb = slab_lds_bytes<DIM>(sp);
+    const int per = (P.nblocks + 7) / 8;  // slabs per XCD
+    const size_t lds = head + bb;
     if (lds + sizeof(LTabLin<double>) + 64 > 160 * 1024)
         return set_error(FLGPU_EUNSUPPORTED, "blocked assembly: a slab needs %zu bytes of LDS", lds);
     FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_staged<DIM, FEAT>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const int grid = 8 * ((P.nblocks + 7) / 8);
-    hipLaunchKernelGGL((k_asm_staged<DIM, FEAT>), dim3(grid), dim3(kStagedThreads), lds, c->stream, P.tabw.p,
-                       P.nblocks, P.rows_cap, P.nnz_cap, P.slot_cap, P.node_cap, P.meta.p, P.iptr.p, P.rdg.p,
-                       P.nodes.p, P.slotw.p, P.slotd.p, P.srb.p, P.ecp.p, c->X4.p, fd, fl, su, sk, ss, sb, c->A.p,
-                       c->rhs.p);
+    hipLaunchKernelGGL((k_asm_staged<DIM, FEAT>), dim3(8 * per), dim3(kStagedThreads), lds, c->stream, sp,
+                       c->X4.p, fd, fl, c->A.p, c->rhs.p);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
