@@ -168,8 +168,8 @@ def cpu_baseline(cfg, budget_s: float = 20.0) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="c5")
     ap.add_argument("--scale", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")

@@ -101,7 +101,7 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ partials
 // ---------------------------------------------------------------------
 // Streamed CSR products
 // ---------------------------------------------------------------------
-constexpr int kStreamPerThread = 4;
+constexpr int kStreamPerThread = 8;
 constexpr int kStreamNnz = kRowBlock * kStreamPerThread;  // entries staged per row block
 
 // Row blocks for k_csr_stream: runs of consecutive rows holding at most
