@@ -89,6 +89,17 @@ def c3_mixed_report(log_each) -> dict:
 
     out["A_rel_err_fp32_vs_fp64"] = rel(mats["fp32"][0], mats["fp64"][0])
     out["b_rel_err_fp32_vs_fp64"] = rel(mats["fp32"][1], mats["fp64"][1])
+    # device-resident assembly time (HIP events over 5 launches after a warm-up)
+    from fluidity_amd import lib as L
+    import ctypes as C
+    lib = L.load()
+    for prec in ("fp64", "fp32"):
+        p = cg.options_from_config(cfg, precision=prec).params(cfg.dt)
+        L.check(lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(p)), "assemble")
+        ms = C.c_double(0)
+        L.check(lib.flgpu_time_assemble(pr.ctx, C.byref(p), 5, C.byref(ms)), "time_assemble")
+        out[f"assembly_{prec}_ms_resident"] = round(ms.value, 4)
+        out[f"assembly_{prec}_Melem_per_s"] = round(cfg.mesh.nele / ms.value / 1e3, 2)
     xs = {}
     for label, prec in (("fp64", "fp64"), ("mixed", "mixed")):
         so = cg.SolverOptions("gmres", "jacobi", relative_error=1e-8, restart=30, max_iterations=20000,
@@ -103,7 +114,8 @@ def c3_mixed_report(log_each) -> dict:
     out["x_rel_err_mixed_vs_fp64"] = rel(xs["mixed"], xs["fp64"])
     pr.close()
     log_each(f"c3 mixed: A err {out['A_rel_err_fp32_vs_fp64']:.2e}, b err {out['b_rel_err_fp32_vs_fp64']:.2e}, "
-             f"x err {out['x_rel_err_mixed_vs_fp64']:.2e}")
+             f"x err {out['x_rel_err_mixed_vs_fp64']:.2e}; assembly fp64 {out['assembly_fp64_ms_resident']:.3f} ms, "
+             f"fp32 {out['assembly_fp32_ms_resident']:.3f} ms")
     return out
 
 

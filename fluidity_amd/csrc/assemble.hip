@@ -153,6 +153,245 @@ static int launch_colour(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, 
     return FLGPU_OK;
 }
 
+// Quadratic T: the fields' values at the quadrature points of one element,
+// computed once per element into LDS (the element's threads split the
+// points) instead of once per row from the nodal arrays.
+template <int DIM, typename R>
+struct LdsQuad {
+    const R *uq, *dv, *kq, *sq, *fq;  // per point: DIM, 1, kst, 1, 1 values
+    int kst;
+    __device__ __forceinline__ void u(int g, R (&out)[DIM]) const {
+#pragma unroll
+        for (int d = 0; d < DIM; d++) out[d] = uq[g * DIM + d];
+    }
+    __device__ __forceinline__ R div(int g) const { return dv[g]; }
+    __device__ __forceinline__ R k_iso(int g) const { return kq[g]; }
+    __device__ __forceinline__ void k_tensor(int g, R (&kt)[DIM][DIM]) const {
+#pragma unroll
+        for (int a = 0; a < DIM; a++)
+#pragma unroll
+            for (int b = 0; b < DIM; b++) kt[a][b] = kq[g * DIM * DIM + a + DIM * b];
+    }
+    __device__ __forceinline__ R absorp(int g) const { return sq[g]; }
+    __device__ __forceinline__ R src(int g) const { return fq[g]; }
+};
+
+// Nodal values of an element staged in LDS by its row threads (thread i
+// holds node i): velocity, diffusivity (kst components), source,
+// absorption.
+template <int DIM, int LOC, typename R>
+struct LdsNodal {
+    const R *u, *k, *s, *b;
+    int kst;
+    __device__ __forceinline__ R U(int n, int d) const { return u[n * DIM + d]; }
+    __device__ __forceinline__ R K(int n, int comp) const { return k[n * kst + comp]; }
+    __device__ __forceinline__ R S(int n) const { return s[n]; }
+    __device__ __forceinline__ R B(int n) const { return b[n]; }
+};
+
+// LDS values per element: per quadrature point DIM + 3 + kst, per node
+// DIM + kst + 3 (velocity, diffusivity, source, absorption, T).
+template <int DIM, int LOC>
+__host__ __device__ constexpr int quad_stage_per_element(int kst) {
+    return kElemMaxGi * (DIM + 3 + kst) + LOC * (DIM + kst + 3);
+}
+
+// Quadratic T, one thread per element-matrix entry (LOC^2 threads per
+// element): the element's nodal values and coordinates are staged in LDS
+// by its threads (i, 0), the fields' point values by threads 0..ngi-1;
+// thread (i, j) forms entry (i, j) with entry_general_q, and row i's
+// lumped mass and rhs are summed over the row's entries in column order by
+// thread (i, 0) -- the arithmetic and order of row_general.  The CSR
+// positions of the entries come from a table built once per mesh and
+// sparsity, the element tables from a flat image (one memory round trip).
+template <int DIM, int LOC>
+__host__ __device__ constexpr int entry_stage_per_element(int kst) {
+    return quad_stage_per_element<DIM, LOC>(kst) + DIM * (DIM + 1) + 2 * LOC * LOC;
+}
+
+template <int DIM, int LOC, typename R>
+__global__ void __launch_bounds__(256)
+k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict__ elems, int count,
+                   const int32_t* __restrict__ conn, const int32_t* __restrict__ conn_x,
+                   const double* __restrict__ X4, AsmFields fd, AsmFlags fl, const int32_t* __restrict__ epos,
+                   double* __restrict__ A, double* __restrict__ rhs, int* __restrict__ err) {
+    constexpr int L2 = LOC * LOC;
+    constexpr int EPB = 256 / L2;  // elements per workgroup
+    constexpr int NG = kElemMaxGi;
+    constexpr int TWORDS = (int)(sizeof(LTab<R>) / sizeof(uint32_t));
+    __shared__ LTab<R> tb;
+    extern __shared__ double dynq[];
+    const int kst = fl.iso ? 1 : DIM * DIM;
+    const int t = threadIdx.x, el = t / L2, rr = t - el * L2, i = rr / LOC, j = rr - i * LOC;
+    {  // the tables from their image, every load issued before the first store
+        uint32_t tw[(TWORDS + 255) / 256];
+#pragma unroll
+        for (int u = 0; u < (TWORDS + 255) / 256; u++) tw[u] = t + 256 * u < TWORDS ? ltab[t + 256 * u] : 0u;
+#pragma unroll
+        for (int u = 0; u < (TWORDS + 255) / 256; u++)
+            if (t + 256 * u < TWORDS) reinterpret_cast<uint32_t*>(&tb)[t + 256 * u] = tw[u];
+    }
+    const int q = (int)blockIdx.x * EPB + el;
+    const bool live = el < EPB && q < count;
+    R* me = reinterpret_cast<R*>(dynq) + (size_t)(live ? el : 0) * entry_stage_per_element<DIM, LOC>(kst);
+    R *s_uq = me, *s_dv = s_uq + NG * DIM, *s_kq = s_dv + NG, *s_sq = s_kq + NG * kst, *s_fq = s_sq + NG;
+    R *n_u = s_fq + NG, *n_k = n_u + LOC * DIM, *n_s = n_k + LOC * kst, *n_b = n_s + LOC, *n_t = n_b + LOC;
+    R *s_x = n_t + LOC, *s_m = s_x + DIM * (DIM + 1), *s_s = s_m + L2;
+    int ele = 0;
+    if (live) {
+        ele = elems[q];
+        if (j == 0) {  // thread (i, 0) stages node i (and vertex i's coordinates)
+            const int nd = conn[(int64_t)ele * LOC + i];
+            n_t[i] = (R)fd.T[nd];
+            if (fl.adv)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) n_u[i * DIM + d] = fld<R>(fd.U, fd.u_c, nd, d, DIM);
+            if (fl.diff)
+                for (int cpt = 0; cpt < kst; cpt++) n_k[i * kst + cpt] = fld<R>(fd.K, fd.k_c, nd, cpt, DIM * DIM);
+            if (fl.src) n_s[i] = fld<R>(fd.S, fd.s_c, nd, 0, 1);
+            if (fl.absorp) n_b[i] = fld<R>(fd.B, fd.b_c, nd, 0, 1);
+            if (i < DIM + 1) {
+                const int xn = conn_x[(int64_t)ele * (DIM + 1) + i];
+#pragma unroll
+                for (int a = 0; a < DIM; a++) s_x[a * (DIM + 1) + i] = (R)X4[(int64_t)xn * 4 + a];
+            }
+        }
+    }
+    __syncthreads();  // tables, nodal values, coordinates
+    R iJ[DIM][DIM], adet = (R)0;
+    if (live) {
+        R X[DIM][DIM + 1];
+#pragma unroll
+        for (int a = 0; a < DIM; a++)
+#pragma unroll
+            for (int nn = 0; nn < DIM + 1; nn++) X[a][nn] = s_x[a * (DIM + 1) + nn];
+        geometry<DIM, R>(X, tb, iJ, adet);
+        if (rr < tb.ngi) {  // thread rr forms the values at point rr
+            const int g = rr;
+            const LdsNodal<DIM, LOC, R> ln{n_u, n_k, n_s, n_b, kst};
+            const NodalQuad<DIM, LOC, R, LdsNodal<DIM, LOC, R>> nq{ln, tb, iJ};
+            if (fl.adv) {
+                R u[DIM];
+                nq.u(g, u);
+#pragma unroll
+                for (int d = 0; d < DIM; d++) s_uq[g * DIM + d] = u[d];
+                if (fl.need_div) s_dv[g] = nq.div(g);
+            }
+            if (fl.diff) {
+                if (fl.iso) {
+                    s_kq[g] = nq.k_iso(g);
+                } else {
+                    R kt[DIM][DIM];
+                    nq.k_tensor(g, kt);
+#pragma unroll
+                    for (int a = 0; a < DIM; a++)
+#pragma unroll
+                        for (int b = 0; b < DIM; b++) s_kq[g * DIM * DIM + a + DIM * b] = kt[a][b];
+                }
+            }
+            if (fl.absorp) s_sq[g] = nq.absorp(g);
+            if (fl.src) s_fq[g] = nq.src(g);
+        }
+    }
+    __syncthreads();  // point values
+    R m_ij = (R)0, s_ij = (R)0, src_i = (R)0;
+    if (live) {
+        entry_general_q<DIM, LOC, R>(tb, fl, iJ, adet, LdsQuad<DIM, R>{s_uq, s_dv, s_kq, s_sq, s_fq, kst}, i, j,
+                                     m_ij, s_ij, src_i);
+        s_m[rr] = m_ij;
+        s_s[rr] = s_ij;
+    }
+    __syncthreads();  // the element's entries
+    if (!live) return;
+    R m = (fl.have_mass && !fl.lump) ? m_ij : (R)0;
+    if (fl.have_mass && fl.lump && j == i) {
+        R lump = (R)0;
+        for (int jj = 0; jj < LOC; jj++) lump += s_m[i * LOC + jj];
+        m = lump;
+    }
+    const R dtte = fl.dtt_on ? (R)fl.dt_theta : (R)0;
+    const double v = (double)FMA(dtte, s_ij, m);
+    const int row = conn[(int64_t)ele * LOC + i];
+    if (v != 0.0) {  // csr_addto skips zeros (Sparse_Tools.F90:2640)
+        const int pos = epos[(int64_t)ele * L2 + rr];
+        if (pos < 0)
+            atomicOr(err, 1);
+        else
+            A[pos] = A[pos] + v;
+    }
+    if (j == 0) {
+        R r = (R)0;
+        for (int jj = 0; jj < LOC; jj++) r = FMA(-s_s[i * LOC + jj], n_t[jj], r);
+        if (fl.src) r = r + src_i;
+        rhs[row] = rhs[row] + (double)r;
+    }
+}
+
+// The CSR position of every element entry (i, j), by the bisection of
+// csr_sparsity_pos (Sparse_Tools.F90:2411-2517), once per mesh and
+// sparsity; -1 when the pattern lacks it.
+__global__ void __launch_bounds__(256)
+k_entry_pos(int64_t n, int loc, const int32_t* __restrict__ conn, const int32_t* __restrict__ rowptr,
+            const int32_t* __restrict__ col, int32_t* __restrict__ epos) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t ele = t / ((int64_t)loc * loc);
+    const int rr = (int)(t - ele * loc * loc), i = rr / loc, j = rr - i * loc;
+    epos[t] = (int32_t)row_pos(rowptr, col, conn[ele * loc + i], conn[ele * loc + j]);
+}
+
+template <int DIM, int LOC, typename R>
+static int launch_colour_entry(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
+    constexpr int EPB = 256 / (LOC * LOC);
+    BlockPlan& P = c->plan;
+    if (!P.epos_built) {
+        const int64_t n = (int64_t)c->nele * LOC * LOC;
+        if (n > INT32_MAX) return set_error(FLGPU_EUNSUPPORTED, "quadratic assembly: too many element entries");
+        FLGPU_TRY(P.epos.alloc((size_t)n));
+        hipLaunchKernelGGL(k_entry_pos, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, LOC, c->conn.p,
+                           c->rowptr.p, c->col.p, P.epos.p);
+        FLGPU_HIP(hipGetLastError());
+        P.epos_built = true;
+    }
+    // the tables in the kernel's LDS layout, in R
+    LTab<R> lt;
+    std::memset(&lt, 0, sizeof lt);
+    {
+        const ElemTables& tt = c->tab;
+        for (int k = 0; k < kMaxLoc * kMaxGi; k++) lt.n[k] = (R)tt.n[k];
+        for (int k = 0; k < kMaxLoc * kMaxGi * kMaxDim; k++) lt.dn[k] = (R)tt.dn[k];
+        for (int k = 0; k < kMaxGi; k++) lt.w[k] = (R)tt.w[k];
+        for (int k = 0; k < kMaxLoc; k++) lt.what[k] = (R)tt.what[k];
+        for (int k = 0; k < kMaxLoc * kMaxLoc; k++) lt.mhat[k] = (R)tt.mhat[k];
+        for (int k = 0; k < 64; k++) lt.that[k] = (R)tt.that[k];
+        for (int k = 0; k < 4 * kMaxDim; k++) lt.xdn[k] = (R)tt.xdn[k];
+        lt.loc = tt.loc;
+        lt.ngi = tt.ngi;
+        lt.lx = tt.loc_x;
+    }
+    DevBuf<uint32_t>& img = sizeof(R) == 8 ? P.ltab64 : P.ltab32;
+    std::vector<uint32_t>& held = sizeof(R) == 8 ? P.h_ltab64 : P.h_ltab32;
+    std::vector<uint32_t> words(sizeof lt / sizeof(uint32_t));
+    std::memcpy(words.data(), &lt, sizeof lt);
+    if (words != held || !img.p) {  // (the host copy outlives the asynchronous upload)
+        held.swap(words);
+        FLGPU_TRY(img.upload(held.data(), held.size(), c->stream));
+    }
+    const int kst = fl.iso ? 1 : DIM * DIM;
+    const size_t lds = sizeof(R) * (size_t)EPB * entry_stage_per_element<DIM, LOC>(kst);
+    FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_colour_entry<DIM, LOC, R>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    for (int k = 0; k < c->ncol; k++) {
+        const int off = c->h_col_ptr[k], cnt = c->h_col_ptr[k + 1] - off;
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL((k_asm_colour_entry<DIM, LOC, R>), dim3((cnt + EPB - 1) / EPB), dim3(256), lds,
+                           c->stream, img.p, c->col_elems.p + off, cnt, c->conn.p, c->conn_x.p, c->X4.p, fd, fl,
+                           P.epos.p, c->A.p, c->rhs.p, d_err);
+    }
+    FLGPU_HIP(hipGetLastError());
+    return FLGPU_OK;
+}
+
 // ---------------------------------------------------------------------
 // Blocked scheme (linear T, FP64).  The CSR rows are cut into slabs of
 // consecutive rows, small enough that every element contribution to the
@@ -921,15 +1160,17 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     }
     FLGPU_HIP(hipMemsetAsync(c->A.p, 0, sizeof(double) * c->nnz, c->stream));
     FLGPU_HIP(hipMemsetAsync(c->rhs.p, 0, sizeof(double) * c->rows, c->stream));
-#define FL_DISPATCH(D, L)                                                                    \
+    // linear T: a thread per element row; quadratic T: a thread per entry
+#define FL_DISPATCH(D, L, LAUNCH)                                                            \
     if (dim == D && loc == L) {                                                              \
-        rc = fp32 ? launch_colour<D, L, float>(c, fd, fl, d_err)                           \
-                  : launch_colour<D, L, double>(c, fd, fl, d_err);                         \
+        rc = fp32 ? LAUNCH<D, L, float>(c, fd, fl, d_err) : LAUNCH<D, L, double>(c, fd, fl, d_err); \
     } else
-    FL_DISPATCH(2, 3)
-    FL_DISPATCH(2, 6)
-    FL_DISPATCH(3, 4)
-    FL_DISPATCH(3, 10) { return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: dim %d loc %d", dim, loc); }
+    FL_DISPATCH(2, 3, launch_colour)
+    FL_DISPATCH(2, 6, launch_colour_entry)
+    FL_DISPATCH(3, 4, launch_colour)
+    FL_DISPATCH(3, 10, launch_colour_entry) {
+        return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: dim %d loc %d", dim, loc);
+    }
 #undef FL_DISPATCH
     return rc;
 }

@@ -398,24 +398,90 @@ FLGPU_HD_INLINE void row_linear(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, c
     row_linear_g<DIM, LOC, R>(e, gm, tb, fl, GlobalNodal<DIM, R>{fd, e.nodes}, i, vals, rhs_i);
 }
 
+// Nodal fields at quadrature point g, interpolated from the element's
+// nodes as ele_val_at_quad does (femtools/Fields_Base.F90:2256-2310), and
+// the divergence of the velocity at g.  The nodal values come from NA
+// (GlobalNodal, or a kernel's staged copy).
+template <int DIM, int LOC, typename R, class NA = GlobalNodal<DIM, R>>
+struct NodalQuad {
+    const NA& na;
+    const LTab<R>& tb;
+    const R (&iJ)[DIM][DIM];
+    FLGPU_HD_INLINE void u(int g, R (&uq)[DIM]) const {
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            R s = (R)0;
+#pragma unroll
+            for (int k = 0; k < LOC; k++) s = FMA(na.U(k, d), tb.n[k + LOC * g], s);
+            uq[d] = s;
+        }
+    }
+    FLGPU_HD_INLINE R div(int g) const {
+        const int ngi = tb.ngi;
+        R div = (R)0;
+#pragma unroll
+        for (int k = 0; k < LOC; k++)
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                R s = (R)0;
+#pragma unroll
+                for (int q = 0; q < DIM; q++) s = FMA(tb.dn[k + LOC * (g + ngi * q)], iJ[d][q], s);
+                div = FMA(na.U(k, d), s, div);
+            }
+        return div;
+    }
+    FLGPU_HD_INLINE R k_iso(int g) const {
+        R kq = (R)0;
+#pragma unroll
+        for (int k = 0; k < LOC; k++) kq = FMA(na.K(k, 0), tb.n[k + LOC * g], kq);
+        return kq;
+    }
+    FLGPU_HD_INLINE void k_tensor(int g, R (&kt)[DIM][DIM]) const {
+#pragma unroll
+        for (int a = 0; a < DIM; a++)
+#pragma unroll
+            for (int b = 0; b < DIM; b++) {
+                R s = (R)0;
+#pragma unroll
+                for (int k = 0; k < LOC; k++)
+                    s = FMA(na.K(k, a + DIM * b), tb.n[k + LOC * g], s);
+                kt[a][b] = s;
+            }
+    }
+    FLGPU_HD_INLINE R absorp(int g) const {
+        R sq = (R)0;
+#pragma unroll
+        for (int k = 0; k < LOC; k++) sq = FMA(na.B(k), tb.n[k + LOC * g], sq);
+        return sq;
+    }
+    FLGPU_HD_INLINE R src(int g) const {
+        R fq = (R)0;
+#pragma unroll
+        for (int k = 0; k < LOC; k++) fq = FMA(na.S(k), tb.n[k + LOC * g], fq);
+        return fq;
+    }
+};
+
 // Row i for quadratic T: per-quadrature-point loops
-// (Transform_elements.F90:771-775; FETools.F90 contractions).
-template <int DIM, int LOC, typename R>
-FLGPU_HD void row_general(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, const AsmFlags& fl,
-                            const AsmFields& fd, int i, R (&vals)[LOC], R& rhs_i) {
-    R iJ[DIM][DIM], adet;
-    geometry<DIM, R>(e.X, tb, iJ, adet);
+// (Transform_elements.F90:771-775; FETools.F90 contractions), with the
+// fields at the quadrature points from QV (NodalQuad, or a kernel's staged
+// copy of the same values).  The mass row and the sum of the advection,
+// absorption and diffusion rows are kept (the matrix row is
+// M + dt*theta*(A + B + K) and the rhs -(A + B + K) T): two accumulators per
+// column instead of four, which is what keeps a P2 row in registers.
+template <int DIM, int LOC, typename R, class QV, class TV>
+FLGPU_HD_INLINE void row_general_q(const LTab<R>& tb, const AsmFlags& fl, const R (&iJ)[DIM][DIM], R adet,
+                                   const QV& qv, const TV& T, int i, R (&vals)[LOC], R& rhs_i) {
     const int ngi = tb.ngi;
     const R dtt = (R)fl.dt_theta;
-    R M[LOC], Aa[LOC], Bb[LOC], Kk[LOC];
+    R M[LOC], Sm[LOC];
 #pragma unroll
     for (int j = 0; j < LOC; j++) {
         M[j] = (R)0;
-        Aa[j] = (R)0;
-        Bb[j] = (R)0;
-        Kk[j] = (R)0;
+        Sm[j] = (R)0;
     }
     R src_i = (R)0;
+#pragma unroll 1
     for (int g = 0; g < ngi; g++) {
         const R dw = adet * tb.w[g];
         const R ni = tb.n[i + LOC * g];
@@ -431,54 +497,19 @@ FLGPU_HD void row_general(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, const A
         R uq[DIM] = {};
         R div = (R)0;
         if (fl.adv) {
-#pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                R s = (R)0;
-#pragma unroll
-                for (int k = 0; k < LOC; k++) s = FMA(fld<R>(fd.U, fd.u_c, e.nodes[k], d, DIM), tb.n[k + LOC * g], s);
-                uq[d] = s;
-            }
-            if (fl.need_div) {
-#pragma unroll
-                for (int k = 0; k < LOC; k++)
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) {
-                        R s = (R)0;
-#pragma unroll
-                        for (int q = 0; q < DIM; q++) s = FMA(tb.dn[k + LOC * (g + ngi * q)], iJ[d][q], s);
-                        div = FMA(fld<R>(fd.U, fd.u_c, e.nodes[k], d, DIM), s, div);
-                    }
-            }
+            qv.u(g, uq);
+            if (fl.need_div) div = qv.div(g);
         }
         R kq = (R)0, sq = (R)0;
         R kt[DIM][DIM];
         if (fl.diff) {
-            if (fl.iso) {
-#pragma unroll
-                for (int k = 0; k < LOC; k++) kq = FMA(fld<R>(fd.K, fd.k_c, e.nodes[k], 0, DIM * DIM), tb.n[k + LOC * g], kq);
-            } else {
-#pragma unroll
-                for (int a = 0; a < DIM; a++)
-#pragma unroll
-                    for (int b = 0; b < DIM; b++) {
-                        R s = (R)0;
-#pragma unroll
-                        for (int k = 0; k < LOC; k++)
-                            s = FMA(fld<R>(fd.K, fd.k_c, e.nodes[k], a + DIM * b, DIM * DIM), tb.n[k + LOC * g], s);
-                        kt[a][b] = s;
-                    }
-            }
+            if (fl.iso)
+                kq = qv.k_iso(g);
+            else
+                qv.k_tensor(g, kt);
         }
-        if (fl.absorp) {
-#pragma unroll
-            for (int k = 0; k < LOC; k++) sq = FMA(fld<R>(fd.B, fd.b_c, e.nodes[k], 0, 1), tb.n[k + LOC * g], sq);
-        }
-        if (fl.src) {
-            R fq = (R)0;
-#pragma unroll
-            for (int k = 0; k < LOC; k++) fq = FMA(fld<R>(fd.S, fd.s_c, e.nodes[k], 0, 1), tb.n[k + LOC * g], fq);
-            src_i = FMA(ni, dw * fq, src_i);
-        }
+        if (fl.absorp) sq = qv.absorp(g);
+        if (fl.src) src_i = FMA(ni, dw * qv.src(g), src_i);
         R ug_i = (R)0;
         if (fl.adv && fl.ibp) {
 #pragma unroll
@@ -496,6 +527,12 @@ FLGPU_HD void row_general(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, const A
         }
 #pragma unroll
         for (int j = 0; j < LOC; j++) {
+#ifdef __HIP_DEVICE_COMPILE__
+            // keep each column's table reads in its own step (hoisting all
+            // of them ahead of the arithmetic needs more registers than a
+            // P2 row can have)
+            if constexpr (LOC > DIM + 1) __asm__ __volatile__("" ::: "memory");
+#endif
             const R nj = tb.n[j + LOC * g];
             const R nn = ni * nj;
             if (fl.have_mass) M[j] = FMA(nn, dw, M[j]);
@@ -520,34 +557,27 @@ FLGPU_HD void row_general(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, const A
                         a = -(ug_i * nj);
                         if (fl.need_div) a = FMA(-((R)1 - (R)fl.beta) * nn, div, a);
                     }
-                    Aa[j] = FMA(a, dw, Aa[j]);
+                    Sm[j] = FMA(a, dw, Sm[j]);
                 }
                 if (fl.diff) {
                     R t = (R)0;
                     if (fl.iso) {
 #pragma unroll
                         for (int d = 0; d < DIM; d++) t = FMA(Gi[d], Gj[d], t);
-                        Kk[j] = FMA(t, dw * kq, Kk[j]);
+                        Sm[j] = FMA(t, dw * kq, Sm[j]);
                     } else {
 #pragma unroll
                         for (int d = 0; d < DIM; d++) t = FMA(vK[d], Gj[d], t);
-                        Kk[j] = FMA(t, dw, Kk[j]);
+                        Sm[j] = FMA(t, dw, Sm[j]);
                     }
                 }
             }
-            if (fl.absorp) Bb[j] = FMA(nn, dw * sq, Bb[j]);
+            if (fl.absorp) Sm[j] = FMA(nn, dw * sq, Sm[j]);
         }
     }
     R r = (R)0;
-    if (fl.adv)
 #pragma unroll
-        for (int j = 0; j < LOC; j++) r = FMA(-Aa[j], e.T[j], r);
-    if (fl.absorp)
-#pragma unroll
-        for (int j = 0; j < LOC; j++) r = FMA(-Bb[j], e.T[j], r);
-    if (fl.diff)
-#pragma unroll
-        for (int j = 0; j < LOC; j++) r = FMA(-Kk[j], e.T[j], r);
+    for (int j = 0; j < LOC; j++) r = FMA(-Sm[j], T[j], r);
     if (fl.src) r = r + src_i;
     rhs_i = r;
     R lump = (R)0;
@@ -555,17 +585,120 @@ FLGPU_HD void row_general(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, const A
 #pragma unroll
         for (int j = 0; j < LOC; j++) lump += M[j];
     }
+    const R dtte = fl.dtt_on ? dtt : (R)0;
 #pragma unroll
     for (int j = 0; j < LOC; j++) {
         R m = (fl.have_mass && !fl.lump) ? M[j] : (R)0;
         if (fl.have_mass && fl.lump && j == i) m = lump;
-        if (fl.dtt_on) {
-            if (fl.adv) m = FMA(dtt, Aa[j], m);
-            if (fl.absorp) m = FMA(dtt, Bb[j], m);
-            if (fl.diff) m = FMA(dtt, Kk[j], m);
-        }
-        vals[j] = m;
+        vals[j] = FMA(dtte, Sm[j], m);
     }
+}
+
+// Entry (i, j) of the quadratic-T element sums of row_general_q, alone:
+// the mass term, the summed advection/absorption/diffusion term, and row
+// i's source integral, each accumulated over the quadrature points in the
+// same order with the same operations as there (so the same bits).
+template <int DIM, int LOC, typename R, class QV>
+FLGPU_HD_INLINE void entry_general_q(const LTab<R>& tb, const AsmFlags& fl, const R (&iJ)[DIM][DIM], R adet,
+                                     const QV& qv, int i, int j, R& m_ij, R& s_ij, R& src_i) {
+    const int ngi = tb.ngi;
+    R M = (R)0, Sm = (R)0, src = (R)0;
+#pragma unroll 1
+    for (int g = 0; g < ngi; g++) {
+        const R dw = adet * tb.w[g];
+        const R ni = tb.n[i + LOC * g];
+        R Gi[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            R s = (R)0;
+#pragma unroll
+            for (int k = 0; k < DIM; k++) s = FMA(tb.dn[i + LOC * (g + ngi * k)], iJ[d][k], s);
+            Gi[d] = s;
+        }
+        R uq[DIM] = {};
+        R div = (R)0;
+        if (fl.adv) {
+            qv.u(g, uq);
+            if (fl.need_div) div = qv.div(g);
+        }
+        R kq = (R)0, sq = (R)0;
+        R kt[DIM][DIM];
+        if (fl.diff) {
+            if (fl.iso)
+                kq = qv.k_iso(g);
+            else
+                qv.k_tensor(g, kt);
+        }
+        if (fl.absorp) sq = qv.absorp(g);
+        if (fl.src) src = FMA(ni, dw * qv.src(g), src);
+        R ug_i = (R)0;
+        if (fl.adv && fl.ibp) {
+#pragma unroll
+            for (int d = 0; d < DIM; d++) ug_i = FMA(Gi[d], uq[d], ug_i);
+        }
+        R vK[DIM];
+        if (fl.diff && !fl.iso) {
+#pragma unroll
+            for (int b = 0; b < DIM; b++) {
+                R s = (R)0;
+#pragma unroll
+                for (int a = 0; a < DIM; a++) s = FMA(Gi[a], kt[a][b], s);
+                vK[b] = s;
+            }
+        }
+        const R nj = tb.n[j + LOC * g];
+        const R nn = ni * nj;
+        if (fl.have_mass) M = FMA(nn, dw, M);
+        if (fl.adv || fl.diff) {
+            R Gj[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                R s = (R)0;
+#pragma unroll
+                for (int k = 0; k < DIM; k++) s = FMA(tb.dn[j + LOC * (g + ngi * k)], iJ[d][k], s);
+                Gj[d] = s;
+            }
+            if (fl.adv) {
+                R a;
+                if (!fl.ibp) {
+                    R t = (R)0;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) t = FMA(Gj[d], uq[d], t);
+                    a = ni * t;
+                    if (fl.need_div) a = FMA((R)fl.beta * nn, div, a);
+                } else {
+                    a = -(ug_i * nj);
+                    if (fl.need_div) a = FMA(-((R)1 - (R)fl.beta) * nn, div, a);
+                }
+                Sm = FMA(a, dw, Sm);
+            }
+            if (fl.diff) {
+                R t = (R)0;
+                if (fl.iso) {
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) t = FMA(Gi[d], Gj[d], t);
+                    Sm = FMA(t, dw * kq, Sm);
+                } else {
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) t = FMA(vK[d], Gj[d], t);
+                    Sm = FMA(t, dw, Sm);
+                }
+            }
+        }
+        if (fl.absorp) Sm = FMA(nn, dw * sq, Sm);
+    }
+    m_ij = M;
+    s_ij = Sm;
+    src_i = src;
+}
+
+template <int DIM, int LOC, typename R>
+FLGPU_HD void row_general(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, const AsmFlags& fl,
+                            const AsmFields& fd, int i, R (&vals)[LOC], R& rhs_i) {
+    R iJ[DIM][DIM], adet;
+    geometry<DIM, R>(e.X, tb, iJ, adet);
+    const GlobalNodal<DIM, R> na{fd, e.nodes};
+    row_general_q<DIM, LOC, R>(tb, fl, iJ, adet, NodalQuad<DIM, LOC, R>{na, tb, iJ}, e.T, i, vals, rhs_i);
 }
 
 template <int DIM, int LOC, typename R>

@@ -113,6 +113,7 @@ int flgpu_mesh_bind(flgpu_ctx* c, int64_t key_id, int64_t key_event, int32_t dim
     c->nele = nele;
     c->nx = nx;
     c->plan.built = false;
+    c->plan.epos_built = false;
     return FLGPU_OK;
 }
 
@@ -176,6 +177,7 @@ int flgpu_field_mesh_bind(flgpu_ctx* c, int32_t loc, int32_t ngi, int32_t nnodes
     c->nnodes = nnodes;
     c->linear = (loc == lin);
     c->plan.built = false;
+    c->plan.epos_built = false;
     return FLGPU_OK;
 }
 
@@ -218,6 +220,7 @@ int flgpu_sparsity_bind(flgpu_ctx* c, int32_t rows, int32_t cols, const int32_t*
     c->have_inactive = false;
     c->assembled = false;
     c->plan.built = false;
+    c->plan.epos_built = false;
     c->sparsity_gen++;
     krylov_invalidate(c);
     return FLGPU_OK;
@@ -261,6 +264,7 @@ int flgpu_colouring_bind(flgpu_ctx* c, int32_t ncol, const int32_t* col_ptr, con
     c->h_col_elems.swap(el);
     c->ncol = ncol;
     c->plan.built = false;
+    c->plan.epos_built = false;
     return FLGPU_OK;
 }
 

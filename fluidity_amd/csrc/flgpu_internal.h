@@ -120,6 +120,13 @@ struct BlockPlan {
     DevBuf<uint16_t> ecp;        // per CSR entry: slab-relative offset of its run of values
     DevBuf<uint16_t> rdg;        // per row: row-relative offset of its diagonal run
     int32_t node_cap = 0;
+    // per element, its LOC^2 entries' CSR positions (-1: not in the
+    // pattern), for the quadratic-T colour kernel; the element tables as
+    // the kernel keeps them in LDS (FP64 and FP32 images)
+    DevBuf<int32_t> epos;
+    bool epos_built = false;
+    DevBuf<uint32_t> ltab64, ltab32;
+    std::vector<uint32_t> h_ltab64, h_ltab32;  // what the images hold
 };
 
 struct KrylovWork;   // krylov.hip
