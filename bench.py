@@ -223,6 +223,14 @@ def main() -> int:
         L.check(lib.flgpu_time_assemble(pr.ctx, C.byref(p_col), max(2, args.steps // 4), C.byref(ms)), "time")
         schemes["colour"] = {"ms": round(ms.value, 4), "Melem/s": round(cfg.mesh.nele / ms.value / 1e3, 2)}
         log(f"assembly colour scheme: {ms.value:.3f} ms/step")
+        # FP32 element arithmetic (mixed precision), values widened into the FP64 matrix
+        p32 = cg.options_from_config(cfg, precision="fp32").params(cfg.dt)
+        for _ in range(3):
+            L.check(lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(p32)), "assemble fp32")
+        L.check(lib.flgpu_time_assemble(pr.ctx, C.byref(p32), args.steps, C.byref(ms)), "time")
+        schemes["blocked_fp32_elements"] = {"ms": round(ms.value, 4),
+                                            "Melem/s": round(cfg.mesh.nele / ms.value / 1e3, 2)}
+        log(f"assembly with FP32 element arithmetic: {ms.value:.3f} ms/step")
         L.check(lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(p)), "assemble")
 
     out = {

@@ -530,7 +530,7 @@ k_asm_blocked(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int n
 template <int DIM, typename R>
 struct StagedNodal {
     const int* lid;
-    const double *sU, *sK, *sS, *sB;
+    const R *sU, *sK, *sS, *sB;
     int ncap;
     __device__ __forceinline__ R U(int k, int d) const { return (R)sU[d * ncap + lid[k]]; }
     __device__ __forceinline__ R K(int k, int) const { return (R)sK[lid[k]]; }
@@ -555,7 +555,7 @@ struct StagedNodal {
 
 // The plan arrays and capacities the node-staged kernels read.
 struct StagedPlan {
-    const uint64_t* tabw;
+    const uint32_t* tabw;  // the element tables (LTabLin<R>) as 32-bit words
     const int32_t *meta, *iptr, *nodes;
     const uint16_t *rdg, *ecp;
     const uint32_t *slotw, *slotd;
@@ -643,27 +643,28 @@ __device__ __forceinline__ void load_node_vals(const StagedPlan& sp, const doubl
     }
 }
 
-// A slab's LDS copy: node data (struct of arrays), row slot offsets, row
-// diagonal offsets, entry offsets.
+// A slab's LDS copy: node data (struct of arrays, in the arithmetic type
+// R), row slot offsets, row diagonal offsets, entry offsets.
+template <typename R>
 struct SlabLds {
-    double *sX, *sT, *sU, *sK, *sS, *sB;
+    R *sX, *sT, *sU, *sK, *sS, *sB;
     int32_t *soff, *sdg;
     uint32_t* scp;
 };
 
-template <int DIM>
+template <int DIM, typename R>
 __host__ __device__ __forceinline__ size_t slab_lds_bytes(const StagedPlan& sp) {
     const size_t nf = (size_t)DIM + 1 + (sp.stage_u ? DIM : 0) + sp.stage_k + sp.stage_s + sp.stage_b;
-    const size_t b = sizeof(double) * nf * sp.node_cap +
+    const size_t b = sizeof(R) * nf * sp.node_cap +
                      sizeof(int32_t) * ((size_t)(sp.rows_cap + 1) + sp.rows_cap + ((sp.nnz_cap + 3) >> 1));
     return (b + 15) & ~(size_t)15;
 }
 
-template <int DIM>
-__device__ __forceinline__ SlabLds slab_lds(char* base, const StagedPlan& sp) {
-    SlabLds L;
+template <int DIM, typename R>
+__device__ __forceinline__ SlabLds<R> slab_lds(char* base, const StagedPlan& sp) {
+    SlabLds<R> L;
     const int nc = sp.node_cap;
-    L.sX = reinterpret_cast<double*>(base);
+    L.sX = reinterpret_cast<R*>(base);
     L.sT = L.sX + (size_t)DIM * nc;
     L.sU = L.sT + nc;
     L.sK = L.sU + (sp.stage_u ? (size_t)DIM * nc : 0);
@@ -675,7 +676,8 @@ __device__ __forceinline__ SlabLds slab_lds(char* base, const StagedPlan& sp) {
     return L;
 }
 
-__device__ __forceinline__ void store_slab_words(const SlabWords& w, const SlabMeta& m, const SlabLds& L,
+template <typename R>
+__device__ __forceinline__ void store_slab_words(const SlabWords& w, const SlabMeta& m, const SlabLds<R>& L,
                                                  int tid) {
     constexpr int NT = kStagedThreads;
     if (tid <= m.R) L.soff[tid] = w.vip - m.s0;
@@ -684,27 +686,27 @@ __device__ __forceinline__ void store_slab_words(const SlabWords& w, const SlabM
     if (tid + NT < m.nwe) L.scp[tid + NT] = w.vcp1;
 }
 
-template <int DIM>
+template <int DIM, typename R>
 __device__ __forceinline__ void store_node_vals(const StagedPlan& sp, const NodeVals<DIM>& v, int NN,
-                                                const SlabLds& L, int q) {
+                                                const SlabLds<R>& L, int q) {
     if (q >= NN) return;
     const int nc = sp.node_cap;
 #pragma unroll
-    for (int a = 0; a < DIM; a++) L.sX[a * nc + q] = v.x[a];
-    L.sT[q] = v.t;
+    for (int a = 0; a < DIM; a++) L.sX[a * nc + q] = (R)v.x[a];
+    L.sT[q] = (R)v.t;
     if (sp.stage_u)
 #pragma unroll
-        for (int a = 0; a < DIM; a++) L.sU[a * nc + q] = v.u[a];
-    if (sp.stage_k) L.sK[q] = v.k;
-    if (sp.stage_s) L.sS[q] = v.s;
-    if (sp.stage_b) L.sB[q] = v.b;
+        for (int a = 0; a < DIM; a++) L.sU[a * nc + q] = (R)v.u[a];
+    if (sp.stage_k) L.sK[q] = (R)v.k;
+    if (sp.stage_s) L.sS[q] = (R)v.s;
+    if (sp.stage_b) L.sB[q] = (R)v.b;
 }
 
 // Phase A: one element row per slot, from LDS; its values to their places.
-template <int DIM, int FEAT>
-__device__ __forceinline__ void staged_phase_a(const LTabLin<double>& tb, const AsmFlags& fl, const StagedPlan& sp,
-                                               const SlabMeta& m, const SlabLds& L, const SlabWords& w,
-                                               double* sc, double* srhs, int tid) {
+template <int DIM, int FEAT, typename R>
+__device__ __forceinline__ void staged_phase_a(const LTabLin<R>& tb, const AsmFlags& fl, const StagedPlan& sp,
+                                               const SlabMeta& m, const SlabLds<R>& L, const SlabWords& w,
+                                               R* sc, R* srhs, int tid) {
     constexpr int LOC = DIM + 1;
     constexpr int NT = kStagedThreads;
     const int nc = sp.node_cap;
@@ -724,7 +726,7 @@ __device__ __forceinline__ void staged_phase_a(const LTabLin<double>& tb, const 
         const int l = (int)(u == 0 ? q0 : u == 1 ? q1 : q2);
         const int i = (int)(dw & 3u);
         int lid[LOC];
-        Elem<DIM, LOC, double> e;
+        Elem<DIM, LOC, R> e;
 #pragma unroll
         for (int k = 0; k < LOC; k++) {
             lid[k] = (int)((vw >> (8 * k)) & 255u);
@@ -732,13 +734,13 @@ __device__ __forceinline__ void staged_phase_a(const LTabLin<double>& tb, const 
             for (int a = 0; a < DIM; a++) e.X[a][k] = L.sX[a * nc + lid[k]];
             e.T[k] = L.sT[lid[k]];
         }
-        GeomLin<DIM, LOC, double> gm;
-        geom_linear_p1<DIM, LOC, double>(e, gm);
-        double vals[LOC], r;
-        row_linear_g<DIM, LOC, double, FEAT, StagedNodal<DIM, double>, LTabLin<double>>(
-            e, gm, tb, fl, StagedNodal<DIM, double>{lid, L.sU, L.sK, L.sS, L.sB, nc}, i, vals, r);
+        GeomLin<DIM, LOC, R> gm;
+        geom_linear_p1<DIM, LOC, R>(e, gm);
+        R vals[LOC], r;
+        row_linear_g<DIM, LOC, R, FEAT, StagedNodal<DIM, R>, LTabLin<R>>(
+            e, gm, tb, fl, StagedNodal<DIM, R>{lid, L.sU, L.sK, L.sS, L.sB, nc}, i, vals, r);
         const int ks = L.soff[l];
-        double* cr = sc + LOC * ks;
+        R* cr = sc + LOC * ks;
         const int dg = L.sdg[l] + (sl - ks);
 #pragma unroll
         for (int j = 0; j < LOC; j++) {
@@ -751,22 +753,23 @@ __device__ __forceinline__ void staged_phase_a(const LTabLin<double>& tb, const 
 }
 
 // Phase B: every entry sums its run of values (slot = colour order); every
-// row its slots' rhs values.
-template <int LOC>
-__device__ __forceinline__ void staged_phase_b(const SlabMeta& m, const SlabLds& L, const double* sc,
-                                               const double* srhs, double* __restrict__ A,
+// row its slots' rhs values.  The sums are FP64 whatever R is, as the
+// colour scheme adds R values into the FP64 matrix.
+template <int LOC, typename R>
+__device__ __forceinline__ void staged_phase_b(const SlabMeta& m, const SlabLds<R>& L, const R* sc,
+                                               const R* srhs, double* __restrict__ A,
                                                double* __restrict__ rhs, int tid) {
     constexpr int NT = kStagedThreads;
     const uint16_t* ecpl = reinterpret_cast<const uint16_t*>(L.scp) + (m.k0 - m.ce0);
     for (int t = tid; t < m.E; t += NT) {
         const int c0 = ecpl[t], c1 = t + 1 < m.E ? (int)ecpl[t + 1] : LOC * m.S;
         double acc = 0.0;
-        for (int k = c0; k < c1; k++) acc = acc + sc[k];
+        for (int k = c0; k < c1; k++) acc = acc + (double)sc[k];
         A[m.k0 + t] = acc;
     }
     for (int l = tid; l < m.R; l += NT) {
         double acc = 0.0;
-        for (int k = L.soff[l]; k < L.soff[l + 1]; k++) acc = acc + srhs[k];
+        for (int k = L.soff[l]; k < L.soff[l + 1]; k++) acc = acc + (double)srhs[k];
         rhs[m.r0 + l] = acc;
     }
 }
@@ -775,36 +778,37 @@ __device__ __forceinline__ void staged_phase_b(const SlabMeta& m, const SlabLds&
 // its record and the element tables, (2) what the record locates, (3) the
 // nodes' coordinates and fields; every load of a step is issued before the
 // first of them is used.
-template <int DIM, int FEAT>
+template <int DIM, int FEAT, typename R>
 __global__ void __launch_bounds__(kStagedThreads, (FEAT & (F_ADV | F_ABS)) ? 4 : 5)
 k_asm_staged(StagedPlan sp, const double* __restrict__ X4, AsmFields fd, AsmFlags fl, double* __restrict__ A,
              double* __restrict__ rhs) {
     constexpr int LOC = DIM + 1;
-    constexpr int TW = (int)(sizeof(LTabLin<double>) / sizeof(uint64_t));
+    constexpr int TW = (int)(sizeof(LTabLin<R>) / sizeof(uint32_t));
     static_assert(TW <= kStagedThreads, "one table word per thread");
-    __shared__ LTabLin<double> tb;
+    static_assert(sizeof(LTabLin<R>) % sizeof(uint32_t) == 0, "whole words");
+    __shared__ LTabLin<R> tb;
     extern __shared__ double dyn[];
-    double* sc = dyn;                                              // LOC * slot_cap: the contributions
-    double* srhs = sc + (size_t)sp.slot_cap * LOC;                 // slot_cap
-    const SlabLds L = slab_lds<DIM>(reinterpret_cast<char*>(srhs + sp.slot_cap), sp);
+    R* sc = reinterpret_cast<R*>(dyn);                             // LOC * slot_cap: the contributions
+    R* srhs = sc + (size_t)sp.slot_cap * LOC;                      // slot_cap
+    const SlabLds<R> L = slab_lds<DIM, R>(reinterpret_cast<char*>(srhs + sp.slot_cap), sp);
 
     const int per = (sp.nslab + 7) >> 3;
     const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if (b >= sp.nslab) return;
     const int tid = threadIdx.x;
-    const uint64_t tword = tid < TW ? sp.tabw[tid] : 0;
+    const uint32_t tword = tid < TW ? sp.tabw[tid] : 0u;
     const SlabMeta m = slab_meta(sp.meta + 8 * (size_t)b);
     SlabWords w;
     load_slab_words(sp, m, tid, w);
     NodeVals<DIM> nv;
     load_node_vals<DIM>(sp, X4, fd, w.vn, tid < m.NN, nv);
-    if (tid < TW) reinterpret_cast<uint64_t*>(&tb)[tid] = tword;
+    if (tid < TW) reinterpret_cast<uint32_t*>(&tb)[tid] = tword;
     store_slab_words(w, m, L, tid);
-    store_node_vals<DIM>(sp, nv, m.NN, L, tid);
+    store_node_vals<DIM, R>(sp, nv, m.NN, L, tid);
     __syncthreads();
-    staged_phase_a<DIM, FEAT>(tb, fl, sp, m, L, w, sc, srhs, tid);
+    staged_phase_a<DIM, FEAT, R>(tb, fl, sp, m, L, w, sc, srhs, tid);
     __syncthreads();
-    staged_phase_b<LOC>(m, L, sc, srhs, A, rhs, tid);
+    staged_phase_b<LOC, R>(m, L, sc, srhs, A, rhs, tid);
 }
 
 static size_t blocked_lds(int loc, int rows_cap, int nnz_cap, int slot_cap) {
@@ -1002,9 +1006,23 @@ int build_block_plan(flgpu_ctx* c) {
             m[7] = nptr[b + 1];
         }
         FLGPU_TRY(P.meta.upload(meta.data(), meta.size(), c->stream));
-        std::vector<uint64_t> tw(sizeof lt / sizeof(uint64_t));
+        // the tables as the kernels keep them in LDS, FP64 and FP32
+        std::vector<uint32_t> tw(sizeof lt / sizeof(uint32_t));
         std::memcpy(tw.data(), &lt, sizeof lt);
         FLGPU_TRY(P.tabw.upload(tw.data(), tw.size(), c->stream));
+        LTabLin<float> lf;
+        std::memset(&lf, 0, sizeof lf);
+        for (int k = 0; k < 4 * kMaxDim; k++) lf.dn[k] = (float)lt.dn[k];
+        for (int k = 0; k < 4; k++) lf.what[k] = (float)lt.what[k];
+        for (int k = 0; k < 16; k++) lf.mhat[k] = (float)lt.mhat[k];
+        for (int k = 0; k < 64; k++) lf.that[k] = (float)lt.that[k];
+        for (int k = 0; k < 4 * kMaxDim; k++) lf.xdn[k] = (float)lt.xdn[k];
+        lf.loc = lt.loc;
+        lf.ngi = lt.ngi;
+        lf.lx = lt.lx;
+        std::vector<uint32_t> tw32(sizeof lf / sizeof(uint32_t));
+        std::memcpy(tw32.data(), &lf, sizeof lf);
+        FLGPU_TRY(P.tabw32.upload(tw32.data(), tw32.size(), c->stream));
         FLGPU_TRY(P.nptr.upload(nptr.data(), nptr.size(), c->stream));
         FLGPU_TRY(P.nodes.upload(nodes.data(), nodes.size(), c->stream));
         FLGPU_TRY(P.slotw.upload(slotw.data(), slotw.size(), c->stream));
@@ -1043,12 +1061,12 @@ static int launch_blocked_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags
     return FLGPU_OK;
 }
 
-template <int DIM, int FEAT>
+template <int DIM, int FEAT, typename R>
 static int launch_staged_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
     const BlockPlan& P = c->plan;
     constexpr int LOC = DIM + 1;
     StagedPlan sp;
-    sp.tabw = P.tabw.p;
+    sp.tabw = sizeof(R) == 8 ? P.tabw.p : P.tabw32.p;
     sp.meta = P.meta.p;
     sp.iptr = P.iptr.p;
     sp.nodes = P.nodes.p;
@@ -1067,14 +1085,14 @@ static int launch_staged_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags&
     sp.stage_k = (FEAT & F_DIFF) && fl.diff;
     sp.stage_s = (FEAT & F_SRC) && fl.src;
     sp.stage_b = (FEAT & F_ABS) && fl.absorp;
-    const size_t head = sizeof(double) * (size_t)P.slot_cap * (LOC + 1), bb = slab_lds_bytes<DIM>(sp);
+    const size_t head = sizeof(R) * (size_t)P.slot_cap * (LOC + 1), bb = slab_lds_bytes<DIM, R>(sp);
     const int per = (P.nblocks + 7) / 8;  // slabs per XCD
-    const size_t lds = head + bb;
-    if (lds + sizeof(LTabLin<double>) + 64 > 160 * 1024)
+    const size_t lds = ((head + 15) & ~(size_t)15) + bb;
+    if (lds + sizeof(LTabLin<R>) + 64 > 160 * 1024)
         return set_error(FLGPU_EUNSUPPORTED, "blocked assembly: a slab needs %zu bytes of LDS", lds);
-    FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_staged<DIM, FEAT>),
+    FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_staged<DIM, FEAT, R>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_asm_staged<DIM, FEAT>), dim3(8 * per), dim3(kStagedThreads), lds, c->stream, sp,
+    hipLaunchKernelGGL((k_asm_staged<DIM, FEAT, R>), dim3(8 * per), dim3(kStagedThreads), lds, c->stream, sp,
                        c->X4.p, fd, fl, c->A.p, c->rhs.p);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
@@ -1082,20 +1100,27 @@ static int launch_staged_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags&
 
 // The smallest compiled feature set that holds the terms switched on; the
 // node-staged kernel when the plan has node lists and the diffusivity is
-// isotropic (or constant).
+// isotropic (or constant).  FP32 element arithmetic has the node-staged
+// kernel only.
+template <int DIM, typename R>
+static int launch_staged(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err, int m) {
+    auto fits = [m](int f) { return (m & ~f) == 0; };
+    if (fits(F_DIFF | F_SRC)) return launch_staged_feat<DIM, F_DIFF | F_SRC, R>(c, fd, fl, d_err);
+    if (fits(F_MASS | F_ADV | F_DIFF)) return launch_staged_feat<DIM, F_MASS | F_ADV | F_DIFF, R>(c, fd, fl, d_err);
+    if (fits(F_MASS | F_ADV | F_DIFF | F_SRC))
+        return launch_staged_feat<DIM, F_MASS | F_ADV | F_DIFF | F_SRC, R>(c, fd, fl, d_err);
+    return launch_staged_feat<DIM, F_ALL, R>(c, fd, fl, d_err);
+}
+
+static bool stageable(const flgpu_ctx* c, const AsmFlags& fl) { return c->plan.staged && (!fl.diff || fl.iso); }
+
 template <int DIM>
-static int launch_blocked(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
+static int launch_blocked(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err, bool fp32) {
     const int m = (fl.have_mass ? F_MASS : 0) | (fl.adv ? F_ADV : 0) | (fl.absorp ? F_ABS : 0) |
                   (fl.diff ? F_DIFF : 0) | (fl.src ? F_SRC : 0);
     auto fits = [m](int f) { return (m & ~f) == 0; };
-    const bool staged = c->plan.staged && (!fl.diff || fl.iso);
-    if (staged) {
-        if (fits(F_DIFF | F_SRC)) return launch_staged_feat<DIM, F_DIFF | F_SRC>(c, fd, fl, d_err);
-        if (fits(F_MASS | F_ADV | F_DIFF)) return launch_staged_feat<DIM, F_MASS | F_ADV | F_DIFF>(c, fd, fl, d_err);
-        if (fits(F_MASS | F_ADV | F_DIFF | F_SRC))
-            return launch_staged_feat<DIM, F_MASS | F_ADV | F_DIFF | F_SRC>(c, fd, fl, d_err);
-        return launch_staged_feat<DIM, F_ALL>(c, fd, fl, d_err);
-    }
+    if (fp32) return launch_staged<DIM, float>(c, fd, fl, d_err, m);
+    if (stageable(c, fl)) return launch_staged<DIM, double>(c, fd, fl, d_err, m);
     if (fits(F_DIFF | F_SRC)) return launch_blocked_feat<DIM, F_DIFF | F_SRC>(c, fd, fl, d_err);
     if (fits(F_MASS | F_ADV | F_DIFF)) return launch_blocked_feat<DIM, F_MASS | F_ADV | F_DIFF>(c, fd, fl, d_err);
     if (fits(F_MASS | F_ADV | F_DIFF | F_SRC))
@@ -1141,9 +1166,15 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     const bool fp32 = p->precision == FLGPU_FP32;
     int rc = FLGPU_OK;
     int scheme = p->scheme;
-    const bool blocked_ok = c->linear && !fp32;
+    // the blocked scheme: linear T; in FP32 with the node-staged kernel only
+    bool blocked_ok = c->linear;
+    if (blocked_ok && fp32) {
+        FLGPU_TRY(build_block_plan(c));
+        blocked_ok = stageable(c, fl);
+    }
     if (scheme == FLGPU_ASM_BLOCKED && !blocked_ok)
-        return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: the blocked scheme covers linear T in FP64");
+        return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: the blocked scheme covers linear T (in FP32: with "
+                                             "standard P1 tables and isotropic diffusivity)");
     if (scheme == FLGPU_ASM_AUTO) scheme = blocked_ok ? FLGPU_ASM_BLOCKED : FLGPU_ASM_COLOUR;
     if (scheme != FLGPU_ASM_COLOUR && scheme != FLGPU_ASM_BLOCKED)
         return set_error(FLGPU_EINVAL, "flgpu assemble: scheme %d", p->scheme);
@@ -1156,7 +1187,7 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     }
     if (scheme == FLGPU_ASM_BLOCKED) {
         FLGPU_TRY(build_block_plan(c));
-        return dim == 3 ? launch_blocked<3>(c, fd, fl, d_err) : launch_blocked<2>(c, fd, fl, d_err);
+        return dim == 3 ? launch_blocked<3>(c, fd, fl, d_err, fp32) : launch_blocked<2>(c, fd, fl, d_err, fp32);
     }
     FLGPU_HIP(hipMemsetAsync(c->A.p, 0, sizeof(double) * c->nnz, c->stream));
     FLGPU_HIP(hipMemsetAsync(c->rhs.p, 0, sizeof(double) * c->rows, c->stream));

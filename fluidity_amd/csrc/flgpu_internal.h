@@ -112,7 +112,7 @@ struct BlockPlan {
     DevBuf<int32_t> nptr;        // nblocks+1 offsets into nodes
     DevBuf<int32_t> nodes;       // global node ids of each slab, ascending
     DevBuf<int32_t> meta;        // per slab: r0 r1 k0 k1 s0 s1 n0 n1, one 32-byte record
-    DevBuf<uint64_t> tabw;       // the P1 element tables (LTabLin<double>) as 64-bit words
+    DevBuf<uint32_t> tabw, tabw32;  // the P1 element tables (LTabLin<double>, LTabLin<float>) as words
     DevBuf<uint32_t> slotw;      // per slot: 4 x 8-bit slab-local vertex numbers
     DevBuf<uint32_t> slotd;      // per slot: row-relative places of its off-diagonal values (9 bits
                                  // each) and the number of the row's vertex in the element (2 bits)

@@ -102,6 +102,25 @@ def test_blocked_equals_colour_bitwise(require_gpu, name, scale):
     pr.close()
 
 
+@pytest.mark.parametrize("name,scale", [("c1", 20), ("c5", 12), ("cavity", None)])
+def test_blocked_fp32_equals_colour_fp32(require_gpu, name, scale):
+    """FP32 element arithmetic: the node-staged slab kernel and the colour
+    scheme add the same FP32 element values, widened, into the FP64 matrix
+    in the same order -- identical bits; and both stay within the FP32
+    bound of the FP64 assembly."""
+    cfg = M.config(name, seed=15, scale=scale)
+    pr = cg.problem_from_config(cfg)
+    out = {}
+    for scheme, prec in (("colour", "fp32"), ("blocked", "fp32"), ("blocked", "fp64")):
+        print(f"\n[case] {name} scale {scale} scheme {scheme} {prec}", flush=True)
+        cg.assemble_advection_diffusion_cg(pr, cg.options_from_config(cfg, precision=prec, scheme=scheme), cfg.dt)
+        out[(scheme, prec)] = pr.matrix()
+    assert np.array_equal(out[("colour", "fp32")][0], out[("blocked", "fp32")][0])
+    assert np.array_equal(out[("colour", "fp32")][1], out[("blocked", "fp32")][1])
+    assert rel(out[("blocked", "fp32")][0], out[("blocked", "fp64")][0]) < FP32_RTOL
+    pr.close()
+
+
 @pytest.mark.parametrize("name,scale", [("c2", 8), ("c1", 12)])
 def test_blocked_without_node_staging_equals_colour(require_gpu, name, scale, monkeypatch):
     """The blocked scheme's gather form (used when the field mesh is not the
