@@ -158,9 +158,11 @@ def cube_tets(n: int, jitter: float = 0.0, seed: int = 0, grading: float | None 
                 bnd.astype(np.int32), name=f"cube{n}p2")
 
 
-def square_tris(n: int, jitter: float = 0.0, seed: int = 0) -> Mesh:
+def square_tris(n: int, jitter: float = 0.0, seed: int = 0, degree: int = 1) -> Mesh:
     """Unit square, n x n cells each split by the (i,j)-(i+1,j+1) diagonal
-    into two P1 triangles; interior vertices jittered by jitter*h."""
+    into two P1 triangles; interior vertices jittered by jitter*h.  degree 2
+    adds the edge nodes, numbered on the (2n+1)^2 half-spacing lattice, in
+    the element order v1, e12, v2, e13, e23, v3."""
     m = n + 1
     i, j = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
     ii, jj = i.T.ravel(), j.T.ravel()  # node id = i + m j
@@ -183,7 +185,24 @@ def square_tris(n: int, jitter: float = 0.0, seed: int = 0) -> Mesh:
     conn = np.stack([t1, t2], axis=1).reshape(-1, 3)
     nd = (conn + 1).astype(np.int32).ravel()
     bnd = np.nonzero((ii == 0) | (ii == n) | (jj == 0) | (jj == n))[0] + 1
-    return Mesh(2, conn.shape[0], m * m, nd, xy, 1, m * m, nd, xy, bnd.astype(np.int32), name=f"square{n}")
+    if degree == 1:
+        return Mesh(2, conn.shape[0], m * m, nd, xy, 1, m * m, nd, xy, bnd.astype(np.int32), name=f"square{n}")
+    mf = 2 * n + 1
+    F = np.stack([2 * (conn % m), 2 * (conn // m)], axis=-1)  # (nele, 3, 2) half-lattice coordinates
+    order = [(0, 0), (0, 1), (1, 1), (0, 2), (1, 2), (2, 2)]
+    cols = []
+    for a, b in order:
+        g = (F[:, a, :] + F[:, b, :]) // 2
+        cols.append(g[:, 0] + mf * g[:, 1])
+    conn2 = np.stack(cols, axis=1)
+    nnodes = mf * mf
+    pos = np.zeros((nnodes, 2))
+    for q, (a, b) in enumerate(order):
+        pos[conn2[:, q]] = 0.5 * (xy[conn[:, a]] + xy[conn[:, b]])
+    fi, fj = np.arange(nnodes) % mf, np.arange(nnodes) // mf
+    bnd2 = np.nonzero((fi == 0) | (fi == 2 * n) | (fj == 0) | (fj == 2 * n))[0] + 1
+    return Mesh(2, conn.shape[0], m * m, nd, xy, 2, nnodes, (conn2 + 1).astype(np.int32).ravel(), pos,
+                bnd2.astype(np.int32), name=f"square{n}p2")
 
 
 def read_gmsh(path: str) -> Mesh:
@@ -294,6 +313,15 @@ def config(name: str, seed: int = 1, scale: int | None = None) -> Config:
         T = np.exp(-((p[:, 0] - 0.3) ** 2 + (p[:, 1] - 0.5) ** 2) / (2 * 0.05 ** 2))
         U = np.stack([-(p[:, 1] - 0.5), p[:, 0] - 0.5], axis=1)
         return Config("c1", m, 2, T, U, 1e-3, None, None, 0.01, 0.5, 0.0, True, True, "gmres", 1e-10)
+    if name == "c1p2":
+        # the 2D configuration on P2 triangles (straight-sided, 3-point rule)
+        n = scale or 32
+        m = square_tris(n, jitter=0.2, seed=seed, degree=2)
+        p = m.node_pos
+        T = np.exp(-((p[:, 0] - 0.3) ** 2 + (p[:, 1] - 0.5) ** 2) / (2 * 0.05 ** 2))
+        U = np.stack([-(p[:, 1] - 0.5), p[:, 0] - 0.5], axis=1)
+        return Config("c1p2", m, 2, T, U, 1e-3, np.cos(3 * p[:, 0]), None, 0.01, 0.5, 0.0, True, True, "gmres",
+                      1e-10)
     if name == "c2":
         n = scale or 64
         m = cube_tets(n, jitter=0.15, seed=seed)

@@ -132,7 +132,8 @@ def oracle_setup(cfg, variant):
 CASES = [("c1", 6, {}), ("c2", 3, {}), ("c3", 2, {}), ("c4", 3, {}), ("c5", 3, {}),
          ("c1", 6, {"lump_mass_matrix": True}), ("c1", 6, {"integrate_advection_by_parts": True}),
          ("c1", 6, {"beta": 1.0}), ("c1", 6, {"integrate_advection_by_parts": True, "beta": 0.3}),
-         ("c3", 2, {"beta": 0.7}), ("c3", 2, {"integrate_advection_by_parts": True})]
+         ("c3", 2, {"beta": 0.7}), ("c3", 2, {"integrate_advection_by_parts": True}), ("c1p2", 4, {}),
+         ("c1p2", 4, {"lump_mass_matrix": True})]
 
 
 @pytest.mark.parametrize("name,scale,variant", CASES)
