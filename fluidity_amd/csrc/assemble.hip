@@ -204,19 +204,23 @@ __host__ __device__ constexpr int quad_stage_per_element(int kst) {
 // thread (i, 0) -- the arithmetic and order of row_general.  The CSR
 // positions of the entries come from a table built once per mesh and
 // sparsity, the element tables from a flat image (one memory round trip).
+// workgroup of the per-entry kernel (512 and 768 measured slower on C3)
+constexpr int kEntryThreads = 256;
+
 template <int DIM, int LOC>
 __host__ __device__ constexpr int entry_stage_per_element(int kst) {
     return quad_stage_per_element<DIM, LOC>(kst) + DIM * (DIM + 1) + 2 * LOC * LOC;
 }
 
 template <int DIM, int LOC, typename R>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kEntryThreads)
 k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict__ elems, int count,
                    const int32_t* __restrict__ conn, const int32_t* __restrict__ conn_x,
                    const double* __restrict__ X4, AsmFields fd, AsmFlags fl, const int32_t* __restrict__ epos,
                    double* __restrict__ A, double* __restrict__ rhs, int* __restrict__ err) {
     constexpr int L2 = LOC * LOC;
-    constexpr int EPB = 256 / L2;  // elements per workgroup
+    constexpr int NT = kEntryThreads;
+    constexpr int EPB = NT / L2;  // elements per workgroup
     constexpr int NG = kElemMaxGi;
     constexpr int TWORDS = (int)(sizeof(LTab<R>) / sizeof(uint32_t));
     __shared__ LTab<R> tb;
@@ -224,12 +228,12 @@ k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict_
     const int kst = fl.iso ? 1 : DIM * DIM;
     const int t = threadIdx.x, el = t / L2, rr = t - el * L2, i = rr / LOC, j = rr - i * LOC;
     {  // the tables from their image, every load issued before the first store
-        uint32_t tw[(TWORDS + 255) / 256];
+        uint32_t tw[(TWORDS + NT - 1) / NT];
 #pragma unroll
-        for (int u = 0; u < (TWORDS + 255) / 256; u++) tw[u] = t + 256 * u < TWORDS ? ltab[t + 256 * u] : 0u;
+        for (int u = 0; u < (TWORDS + NT - 1) / NT; u++) tw[u] = t + NT * u < TWORDS ? ltab[t + NT * u] : 0u;
 #pragma unroll
-        for (int u = 0; u < (TWORDS + 255) / 256; u++)
-            if (t + 256 * u < TWORDS) reinterpret_cast<uint32_t*>(&tb)[t + 256 * u] = tw[u];
+        for (int u = 0; u < (TWORDS + NT - 1) / NT; u++)
+            if (t + NT * u < TWORDS) reinterpret_cast<uint32_t*>(&tb)[t + NT * u] = tw[u];
     }
     const int q = (int)blockIdx.x * EPB + el;
     const bool live = el < EPB && q < count;
@@ -342,7 +346,7 @@ k_entry_pos(int64_t n, int loc, const int32_t* __restrict__ conn, const int32_t*
 
 template <int DIM, int LOC, typename R>
 static int launch_colour_entry(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
-    constexpr int EPB = 256 / (LOC * LOC);
+    constexpr int EPB = kEntryThreads / (LOC * LOC);
     BlockPlan& P = c->plan;
     if (!P.epos_built) {
         const int64_t n = (int64_t)c->nele * LOC * LOC;
@@ -384,7 +388,7 @@ static int launch_colour_entry(flgpu_ctx* c, const AsmFields& fd, const AsmFlags
     for (int k = 0; k < c->ncol; k++) {
         const int off = c->h_col_ptr[k], cnt = c->h_col_ptr[k + 1] - off;
         if (cnt == 0) continue;
-        hipLaunchKernelGGL((k_asm_colour_entry<DIM, LOC, R>), dim3((cnt + EPB - 1) / EPB), dim3(256), lds,
+        hipLaunchKernelGGL((k_asm_colour_entry<DIM, LOC, R>), dim3((cnt + EPB - 1) / EPB), dim3(kEntryThreads), lds,
                            c->stream, img.p, c->col_elems.p + off, cnt, c->conn.p, c->conn_x.p, c->X4.p, fd, fl,
                            P.epos.p, c->A.p, c->rhs.p, d_err);
     }
