@@ -821,7 +821,23 @@ static size_t blocked_lds(int loc, int rows_cap, int nnz_cap, int slot_cap) {
            sizeof(uint16_t) * ((size_t)slot_cap + nnz_cap);
 }
 
+static int build_block_plan_impl(flgpu_ctx* c);
+
+// The plan once per mesh, sparsity and colouring; a failure is kept (and
+// repeated) until one of them is bound again.
 int build_block_plan(flgpu_ctx* c) {
+    BlockPlan& P = c->plan;
+    if (P.built) return FLGPU_OK;
+    if (P.fail_rc) return set_error(P.fail_rc, "%s", P.fail_msg.c_str());
+    const int rc = build_block_plan_impl(c);
+    if (rc != FLGPU_OK) {
+        P.fail_rc = rc;
+        P.fail_msg = flgpu_last_error();
+    }
+    return rc;
+}
+
+static int build_block_plan_impl(flgpu_ctx* c) {
     if (c->plan.built) return FLGPU_OK;
     if (!c->linear) return set_error(FLGPU_EUNSUPPORTED, "block plan: linear field mesh only");
     const int N = c->rows, loc = c->loc;
@@ -1170,11 +1186,19 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     const bool fp32 = p->precision == FLGPU_FP32;
     int rc = FLGPU_OK;
     int scheme = p->scheme;
-    // the blocked scheme: linear T; in FP32 with the node-staged kernel only
+    // the blocked scheme: linear T; in FP32 with the node-staged kernel
+    // only.  Chosen automatically, it gives way to the colour scheme when
+    // the mesh does not fit its plan (a node in more elements than a slab
+    // holds, a row longer than 255 entries).
     bool blocked_ok = c->linear;
-    if (blocked_ok && fp32) {
-        FLGPU_TRY(build_block_plan(c));
-        blocked_ok = stageable(c, fl);
+    if (blocked_ok && (fp32 || scheme == FLGPU_ASM_AUTO)) {
+        const int rc = build_block_plan(c);
+        if (rc == FLGPU_EUNSUPPORTED && scheme == FLGPU_ASM_AUTO)
+            blocked_ok = false;
+        else if (rc != FLGPU_OK)
+            return rc;
+        else if (fp32)
+            blocked_ok = stageable(c, fl);
     }
     if (scheme == FLGPU_ASM_BLOCKED && !blocked_ok)
         return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: the blocked scheme covers linear T (in FP32: with "

@@ -114,6 +114,7 @@ int flgpu_mesh_bind(flgpu_ctx* c, int64_t key_id, int64_t key_event, int32_t dim
     c->nx = nx;
     c->plan.built = false;
     c->plan.epos_built = false;
+    c->plan.fail_rc = 0;
     return FLGPU_OK;
 }
 
@@ -178,6 +179,7 @@ int flgpu_field_mesh_bind(flgpu_ctx* c, int32_t loc, int32_t ngi, int32_t nnodes
     c->linear = (loc == lin);
     c->plan.built = false;
     c->plan.epos_built = false;
+    c->plan.fail_rc = 0;
     return FLGPU_OK;
 }
 
@@ -221,6 +223,7 @@ int flgpu_sparsity_bind(flgpu_ctx* c, int32_t rows, int32_t cols, const int32_t*
     c->assembled = false;
     c->plan.built = false;
     c->plan.epos_built = false;
+    c->plan.fail_rc = 0;
     c->sparsity_gen++;
     krylov_invalidate(c);
     return FLGPU_OK;
@@ -265,6 +268,7 @@ int flgpu_colouring_bind(flgpu_ctx* c, int32_t ncol, const int32_t* col_ptr, con
     c->ncol = ncol;
     c->plan.built = false;
     c->plan.epos_built = false;
+    c->plan.fail_rc = 0;
     return FLGPU_OK;
 }
 

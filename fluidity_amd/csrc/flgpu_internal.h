@@ -105,6 +105,8 @@ struct BlockPlan {
     DevBuf<int32_t> inc;         // elements of every node, in colour order (ascending id in a colour)
     int32_t nblocks = 0, rows_cap = 0, nnz_cap = 0, slot_cap = 0;
     bool built = false;
+    int fail_rc = 0;             // a plan that cannot be built for this mesh: the status and message
+    std::string fail_msg;
     // node-staged slabs (field mesh = coordinate mesh): every slab's nodes
     // (its rows and their neighbours) are loaded into LDS once; slots name
     // their element's vertices by slab-local node number

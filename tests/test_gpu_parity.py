@@ -121,6 +121,33 @@ def test_blocked_fp32_equals_colour_fp32(require_gpu, name, scale):
     pr.close()
 
 
+def test_auto_scheme_falls_back_to_colour(require_gpu):
+    """A fan of 400 triangles around one node: its row (401 entries) and its
+    400 elements exceed a slab, so the automatic scheme is the colour scheme
+    (and asking for the blocked one is an error); the result matches the
+    oracle."""
+    n = 400
+    ang = np.linspace(0.0, 2.0 * np.pi, n, endpoint=False)
+    X = np.vstack([[0.0, 0.0], np.stack([np.cos(ang), np.sin(ang)], axis=1)])
+    conn = (np.array([[0, 1 + k, 1 + (k + 1) % n] for k in range(n)]) + 1).astype(np.int32).ravel()
+    mesh = M.Mesh(2, n, n + 1, conn, X, 1, n + 1, conn, X, np.arange(2, n + 2, dtype=np.int32), name="fan")
+    cfg = M.config("c1", seed=3, scale=4)
+    cfg.mesh = mesh
+    cfg.T = np.cos(2.0 * X[:, 0]) + X[:, 1]
+    cfg.U = np.stack([-X[:, 1], X[:, 0]], axis=1)
+    ref = O.reference_system(cfg)
+    pr = cg.problem_from_config(cfg)
+    from fluidity_amd.lib import FlgpuError
+    with pytest.raises(FlgpuError):
+        cg.assemble_advection_diffusion_cg(pr, cg.options_from_config(cfg, scheme="blocked"), cfg.dt)
+    cg.assemble_advection_diffusion_cg(pr, cg.options_from_config(cfg), cfg.dt)
+    cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
+    A, rhs, _ = pr.matrix()
+    assert rel(A, ref["A"]) < FP64_RTOL
+    assert rel(rhs, ref["rhs"]) < FP64_RTOL
+    pr.close()
+
+
 @pytest.mark.parametrize("name,scale", [("c2", 8), ("c1", 12)])
 def test_blocked_without_node_staging_equals_colour(require_gpu, name, scale, monkeypatch):
     """The blocked scheme's gather form (used when the field mesh is not the
