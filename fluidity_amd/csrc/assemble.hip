@@ -104,18 +104,87 @@ __device__ __forceinline__ int64_t row_pos(const int32_t* __restrict__ rowptr,
     return -1;
 }
 
+// The element tables in LDS from their flat image (table_image): every
+// load of a thread issued before its first store.
+template <typename R, int NT>
+__device__ __forceinline__ void stage_table_image(const uint32_t* __restrict__ ltab, LTab<R>& tb) {
+    constexpr int TWORDS = (int)(sizeof(LTab<R>) / sizeof(uint32_t));
+    const int t = threadIdx.x;
+    uint32_t tw[(TWORDS + NT - 1) / NT];
+#pragma unroll
+    for (int u = 0; u < (TWORDS + NT - 1) / NT; u++) tw[u] = t + NT * u < TWORDS ? ltab[t + NT * u] : 0u;
+#pragma unroll
+    for (int u = 0; u < (TWORDS + NT - 1) / NT; u++)
+        if (t + NT * u < TWORDS) reinterpret_cast<uint32_t*>(&tb)[t + NT * u] = tw[u];
+}
+
+// The CSR position of every element entry (i, j), by the bisection of
+// csr_sparsity_pos (Sparse_Tools.F90:2411-2517), once per mesh and
+// sparsity; -1 when the pattern lacks it.
+__global__ void __launch_bounds__(256)
+k_entry_pos(int64_t n, int loc, const int32_t* __restrict__ conn, const int32_t* __restrict__ rowptr,
+            const int32_t* __restrict__ col, int32_t* __restrict__ epos) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t ele = t / ((int64_t)loc * loc);
+    const int rr = (int)(t - ele * loc * loc), i = rr / loc, j = rr - i * loc;
+    epos[t] = (int32_t)row_pos(rowptr, col, conn[ele * loc + i], conn[ele * loc + j]);
+}
+
+static int ensure_entry_positions(flgpu_ctx* c) {
+    BlockPlan& P = c->plan;
+    if (P.epos_built) return FLGPU_OK;
+    const int64_t n = (int64_t)c->nele * c->loc * c->loc;
+    if (n > INT32_MAX) return set_error(FLGPU_EUNSUPPORTED, "colour assembly: too many element entries");
+    FLGPU_TRY(P.epos.alloc((size_t)n));
+    hipLaunchKernelGGL(k_entry_pos, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, c->loc, c->conn.p,
+                       c->rowptr.p, c->col.p, P.epos.p);
+    FLGPU_HIP(hipGetLastError());
+    P.epos_built = true;
+    return FLGPU_OK;
+}
+
+// The element tables in the kernels' LDS layout (LTab<R>) as 32-bit words,
+// uploaded when they change.
+template <typename R>
+static int table_image(flgpu_ctx* c, const uint32_t** out) {
+    BlockPlan& P = c->plan;
+    LTab<R> lt;
+    std::memset(&lt, 0, sizeof lt);
+    const ElemTables& tt = c->tab;
+    for (int k = 0; k < kMaxLoc * kMaxGi; k++) lt.n[k] = (R)tt.n[k];
+    for (int k = 0; k < kMaxLoc * kMaxGi * kMaxDim; k++) lt.dn[k] = (R)tt.dn[k];
+    for (int k = 0; k < kMaxGi; k++) lt.w[k] = (R)tt.w[k];
+    for (int k = 0; k < kMaxLoc; k++) lt.what[k] = (R)tt.what[k];
+    for (int k = 0; k < kMaxLoc * kMaxLoc; k++) lt.mhat[k] = (R)tt.mhat[k];
+    for (int k = 0; k < 64; k++) lt.that[k] = (R)tt.that[k];
+    for (int k = 0; k < 4 * kMaxDim; k++) lt.xdn[k] = (R)tt.xdn[k];
+    lt.loc = tt.loc;
+    lt.ngi = tt.ngi;
+    lt.lx = tt.loc_x;
+    DevBuf<uint32_t>& img = sizeof(R) == 8 ? P.ltab64 : P.ltab32;
+    std::vector<uint32_t>& held = sizeof(R) == 8 ? P.h_ltab64 : P.h_ltab32;
+    std::vector<uint32_t> words(sizeof lt / sizeof(uint32_t));
+    std::memcpy(words.data(), &lt, sizeof lt);
+    if (words != held || !img.p) {  // (the host copy outlives the asynchronous upload)
+        held.swap(words);
+        FLGPU_TRY(img.upload(held.data(), held.size(), c->stream));
+    }
+    *out = img.p;
+    return FLGPU_OK;
+}
+
 // One launch per colour; thread = (element of the colour, local row i).
 // Elements of one colour share no node, so the read-modify-writes of a
 // launch never meet.
 template <int DIM, int LOC, typename R>
 __global__ void __launch_bounds__(256)
-k_asm_colour(const ElemTables* __restrict__ tab, const int32_t* __restrict__ elems, int count,
+k_asm_colour(const uint32_t* __restrict__ ltab, const int32_t* __restrict__ elems, int count,
              const int32_t* __restrict__ conn, const int32_t* __restrict__ conn_x,
-             const double* __restrict__ X4, AsmFields fd, AsmFlags fl,
-             const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+             const double* __restrict__ X4, AsmFields fd, AsmFlags fl, const int32_t* __restrict__ epos,
              double* __restrict__ A, double* __restrict__ rhs, int* __restrict__ err) {
     __shared__ LTab<R> tb;
-    stage_tables<R>(tab, tb);
+    stage_table_image<R, 256>(ltab, tb);
     __syncthreads();
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)count * LOC) return;
@@ -125,12 +194,12 @@ k_asm_colour(const ElemTables* __restrict__ tab, const int32_t* __restrict__ ele
     gather_elem<DIM, LOC, R>(ele, conn, conn_x, X4, fd.T, e);
     R vals[LOC], r;
     row_contrib<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
-    const int row = e.nodes[i];
+    const int row = conn[(int64_t)ele * LOC + i];  // (not e.nodes[i]: a run-time index would put e in scratch)
 #pragma unroll
     for (int j = 0; j < LOC; j++) {
         const double v = (double)vals[j];
         if (v == 0.0) continue;  // csr_addto skips zeros (Sparse_Tools.F90:2640)
-        const int64_t pos = row_pos(rowptr, col, row, e.nodes[j]);
+        const int pos = epos[(int64_t)ele * LOC * LOC + i * LOC + j];
         if (pos < 0) {
             atomicOr(err, 1);
             continue;
@@ -142,12 +211,15 @@ k_asm_colour(const ElemTables* __restrict__ tab, const int32_t* __restrict__ ele
 
 template <int DIM, int LOC, typename R>
 static int launch_colour(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
+    FLGPU_TRY(ensure_entry_positions(c));
+    const uint32_t* img = nullptr;
+    FLGPU_TRY(table_image<R>(c, &img));
     for (int k = 0; k < c->ncol; k++) {
         const int off = c->h_col_ptr[k], cnt = c->h_col_ptr[k + 1] - off;
         if (cnt == 0) continue;
         hipLaunchKernelGGL((k_asm_colour<DIM, LOC, R>), dim3(grid_for((int64_t)cnt * LOC, 256)), dim3(256),
-                           0, c->stream, c->d_tab.p, c->col_elems.p + off, cnt, c->conn.p, c->conn_x.p,
-                           c->X4.p, fd, fl, c->rowptr.p, c->col.p, c->A.p, c->rhs.p, d_err);
+                           0, c->stream, img, c->col_elems.p + off, cnt, c->conn.p, c->conn_x.p, c->X4.p, fd, fl,
+                           c->plan.epos.p, c->A.p, c->rhs.p, d_err);
     }
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
@@ -222,19 +294,11 @@ k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict_
     constexpr int NT = kEntryThreads;
     constexpr int EPB = NT / L2;  // elements per workgroup
     constexpr int NG = kElemMaxGi;
-    constexpr int TWORDS = (int)(sizeof(LTab<R>) / sizeof(uint32_t));
     __shared__ LTab<R> tb;
     extern __shared__ double dynq[];
     const int kst = fl.iso ? 1 : DIM * DIM;
     const int t = threadIdx.x, el = t / L2, rr = t - el * L2, i = rr / LOC, j = rr - i * LOC;
-    {  // the tables from their image, every load issued before the first store
-        uint32_t tw[(TWORDS + NT - 1) / NT];
-#pragma unroll
-        for (int u = 0; u < (TWORDS + NT - 1) / NT; u++) tw[u] = t + NT * u < TWORDS ? ltab[t + NT * u] : 0u;
-#pragma unroll
-        for (int u = 0; u < (TWORDS + NT - 1) / NT; u++)
-            if (t + NT * u < TWORDS) reinterpret_cast<uint32_t*>(&tb)[t + NT * u] = tw[u];
-    }
+    stage_table_image<R, NT>(ltab, tb);
     const int q = (int)blockIdx.x * EPB + el;
     const bool live = el < EPB && q < count;
     R* me = reinterpret_cast<R*>(dynq) + (size_t)(live ? el : 0) * entry_stage_per_element<DIM, LOC>(kst);
@@ -331,56 +395,12 @@ k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict_
     }
 }
 
-// The CSR position of every element entry (i, j), by the bisection of
-// csr_sparsity_pos (Sparse_Tools.F90:2411-2517), once per mesh and
-// sparsity; -1 when the pattern lacks it.
-__global__ void __launch_bounds__(256)
-k_entry_pos(int64_t n, int loc, const int32_t* __restrict__ conn, const int32_t* __restrict__ rowptr,
-            const int32_t* __restrict__ col, int32_t* __restrict__ epos) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int64_t ele = t / ((int64_t)loc * loc);
-    const int rr = (int)(t - ele * loc * loc), i = rr / loc, j = rr - i * loc;
-    epos[t] = (int32_t)row_pos(rowptr, col, conn[ele * loc + i], conn[ele * loc + j]);
-}
-
 template <int DIM, int LOC, typename R>
 static int launch_colour_entry(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
     constexpr int EPB = kEntryThreads / (LOC * LOC);
-    BlockPlan& P = c->plan;
-    if (!P.epos_built) {
-        const int64_t n = (int64_t)c->nele * LOC * LOC;
-        if (n > INT32_MAX) return set_error(FLGPU_EUNSUPPORTED, "quadratic assembly: too many element entries");
-        FLGPU_TRY(P.epos.alloc((size_t)n));
-        hipLaunchKernelGGL(k_entry_pos, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, LOC, c->conn.p,
-                           c->rowptr.p, c->col.p, P.epos.p);
-        FLGPU_HIP(hipGetLastError());
-        P.epos_built = true;
-    }
-    // the tables in the kernel's LDS layout, in R
-    LTab<R> lt;
-    std::memset(&lt, 0, sizeof lt);
-    {
-        const ElemTables& tt = c->tab;
-        for (int k = 0; k < kMaxLoc * kMaxGi; k++) lt.n[k] = (R)tt.n[k];
-        for (int k = 0; k < kMaxLoc * kMaxGi * kMaxDim; k++) lt.dn[k] = (R)tt.dn[k];
-        for (int k = 0; k < kMaxGi; k++) lt.w[k] = (R)tt.w[k];
-        for (int k = 0; k < kMaxLoc; k++) lt.what[k] = (R)tt.what[k];
-        for (int k = 0; k < kMaxLoc * kMaxLoc; k++) lt.mhat[k] = (R)tt.mhat[k];
-        for (int k = 0; k < 64; k++) lt.that[k] = (R)tt.that[k];
-        for (int k = 0; k < 4 * kMaxDim; k++) lt.xdn[k] = (R)tt.xdn[k];
-        lt.loc = tt.loc;
-        lt.ngi = tt.ngi;
-        lt.lx = tt.loc_x;
-    }
-    DevBuf<uint32_t>& img = sizeof(R) == 8 ? P.ltab64 : P.ltab32;
-    std::vector<uint32_t>& held = sizeof(R) == 8 ? P.h_ltab64 : P.h_ltab32;
-    std::vector<uint32_t> words(sizeof lt / sizeof(uint32_t));
-    std::memcpy(words.data(), &lt, sizeof lt);
-    if (words != held || !img.p) {  // (the host copy outlives the asynchronous upload)
-        held.swap(words);
-        FLGPU_TRY(img.upload(held.data(), held.size(), c->stream));
-    }
+    FLGPU_TRY(ensure_entry_positions(c));
+    const uint32_t* img = nullptr;
+    FLGPU_TRY(table_image<R>(c, &img));
     const int kst = fl.iso ? 1 : DIM * DIM;
     const size_t lds = sizeof(R) * (size_t)EPB * entry_stage_per_element<DIM, LOC>(kst);
     FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_colour_entry<DIM, LOC, R>),
@@ -389,8 +409,8 @@ static int launch_colour_entry(flgpu_ctx* c, const AsmFields& fd, const AsmFlags
         const int off = c->h_col_ptr[k], cnt = c->h_col_ptr[k + 1] - off;
         if (cnt == 0) continue;
         hipLaunchKernelGGL((k_asm_colour_entry<DIM, LOC, R>), dim3((cnt + EPB - 1) / EPB), dim3(kEntryThreads), lds,
-                           c->stream, img.p, c->col_elems.p + off, cnt, c->conn.p, c->conn_x.p, c->X4.p, fd, fl,
-                           P.epos.p, c->A.p, c->rhs.p, d_err);
+                           c->stream, img, c->col_elems.p + off, cnt, c->conn.p, c->conn_x.p, c->X4.p, fd, fl,
+                           c->plan.epos.p, c->A.p, c->rhs.p, d_err);
     }
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
