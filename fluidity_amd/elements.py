@@ -2,7 +2,8 @@
 element_type carries them (the caller side of flgpu_field_mesh_bind).
 
 Restates femtools/Quadrature.F90 (template selection :304-321, expansion
-:572-605, tet rules :649-735, tri rules :909-950) and
+:572-605, tet rules :649-735, tri rules :909-993, interval rules
+:1140-1230) and
 femtools/Shape_Functions.F90:128-366 with femtools/Elements.F90:615-717
 (derivatives with respect to the first dim barycentric coordinates, the last
 one dependent).  Layouts are Fortran's: n(loc, ngi), dn(loc, ngi, dim).
@@ -19,6 +20,8 @@ _TET8 = [[1, 1, 1, 2], [1, 1, 2, 1], [1, 2, 1, 1], [2, 1, 1, 1]]
 _TET9 = [[1, 1, 2, 2], [1, 2, 1, 2], [1, 2, 2, 1], [2, 1, 1, 2], [2, 1, 2, 1], [2, 2, 1, 1]]
 _TRI4 = [[1, 1, 1]]
 _TRI5 = [[1, 1, 2], [1, 2, 1], [2, 1, 1]]
+_INT1 = [[1, 1]]
+_INT2 = [[1, 2], [2, 1]]
 
 
 def _expand(perm, coords, weight, pts, wts):
@@ -56,8 +59,34 @@ def quadrature(dim: int, degree: int) -> tuple[np.ndarray, np.ndarray]:
         elif degree == 2:
             a = 0.166666666666666666666666666666666
             _expand(_TRI5, [a, 1.0 - 2.0 * a], 0.166666666666666666666666666666666, pts, wts)
+        elif degree == 3:
+            _expand(_TRI4, [0.333333333333333333333333333333333], -0.28125, pts, wts)
+            _expand(_TRI5, [0.2, 1.0 - 2.0 * 0.2], 0.260416666666666666666666666666666, pts, wts)
+        elif degree == 4:
+            a = 0.0915762135097707434595714634022015
+            _expand(_TRI5, [a, 1.0 - 2.0 * a], 0.0549758718276609338191631624501052, pts, wts)
+            a = 0.445948490915964886318329253883051
+            _expand(_TRI5, [a, 1.0 - 2.0 * a], 0.111690794839005732847503504216561, pts, wts)
         else:
             raise ValueError(f"no triangle rule of degree {degree}")
+    elif dim == 1:
+        # interval rules (Quadrature.F90:1140-1230), the surface rules of 2-D meshes
+        if degree <= 1:
+            _expand(_INT1, [0.5], 1.0, pts, wts)
+        elif degree == 2:
+            a = 0.788675134594813
+            _expand(_INT2, [a, 1.0 - a], 0.5, pts, wts)
+        elif degree == 3:
+            a = 0.887298334620742
+            _expand(_INT2, [a, 1.0 - a], 0.277777777777777, pts, wts)
+            _expand(_INT1, [0.5], 0.444444444444444, pts, wts)
+        elif degree == 4:
+            a = 0.9305681557970262
+            _expand(_INT2, [a, 1.0 - a], 0.173927422568727, pts, wts)
+            a = 0.6699905217924281
+            _expand(_INT2, [a, 1.0 - a], 0.326072577431273, pts, wts)
+        else:
+            raise ValueError(f"no interval rule of degree {degree}")
     else:
         raise ValueError(f"dim {dim}")
     return np.array(pts, dtype=np.float64), np.array(wts, dtype=np.float64)
@@ -161,3 +190,10 @@ def make_element(dim: int, degree: int, quad_degree: int) -> Element:
     l, w = quadrature(dim, quad_degree)
     n, dn = shape_tables(dim, degree, l)
     return Element(dim, degree, quad_degree, l, w, n, dn)
+
+
+def make_face_element(dim: int, degree: int, quad_degree: int) -> Element:
+    """face_shape of a dim-dimensional Lagrange simplex mesh: the element of
+    dimension dim-1 and the same degree on the surface rule of the same
+    degree as the volume rule."""
+    return make_element(dim - 1, degree, quad_degree)

@@ -205,6 +205,109 @@ def square_tris(n: int, jitter: float = 0.0, seed: int = 0, degree: int = 1) -> 
                 bnd2.astype(np.int32), name=f"square{n}p2")
 
 
+def boundary_numbering(dim: int, degree: int, face: int) -> list[int]:
+    """Local nodes (0-based) of face `face` (1-based; the face opposite
+    vertex `face`) of a Lagrange simplex, in the order of
+    femtools/Element_Numbering.F90:1564-1615: the element's nodes whose
+    count of barycentric coordinate `face` is zero, in element order,
+    reversed for even faces (in 3-D, row by row) so that every face is
+    positively oriented with respect to its element."""
+    from .elements import simplex_counts
+    counts = simplex_counts(dim, degree)
+    num = [i for i in range(counts.shape[0]) if counts[i, face - 1] == 0]
+    if face % 2 == 0:
+        if dim == 2:
+            num = num[::-1]
+        elif dim == 3:
+            l, i = 1, len(num)
+            while True:
+                num[i - 1:i + l - 1] = num[i - 1:i + l - 1][::-1]
+                l += 1
+                i -= l
+                if i < 1:
+                    break
+    return num
+
+
+@dataclass
+class Surface:
+    """The surface elements of a mesh (its boundary facets), as the face
+    loop of assemble_advection_diffusion_cg sees them: per face its T nodes
+    (face_global_nodes(t, face)), its coordinate nodes
+    (face_global_nodes(positions, face)) and its element (face_ele)."""
+    nfaces: int
+    floc: int
+    face_nodes: np.ndarray     # (nfaces*floc,) int32, 1-based field nodes
+    face_nodes_x: np.ndarray   # (nfaces*dim,) int32, 1-based coordinate nodes
+    face_ele: np.ndarray       # (nfaces,) int32, 1-based
+    centroid: np.ndarray       # (nfaces, dim) face centroids (for tagging)
+
+
+def surface_of(mesh: Mesh) -> Surface:
+    """Boundary facets (facets of exactly one element), ordered by element
+    and then by local face number."""
+    d = mesh.dim
+    nv = d + 1
+    cx = mesh.ndglno_x.reshape(mesh.nele, nv).astype(np.int64) - 1
+    keys = []
+    for f in range(nv):
+        verts = np.sort(np.delete(cx, f, axis=1), axis=1)
+        k = np.zeros(mesh.nele, np.int64)
+        for c in range(d):
+            k = k * np.int64(mesh.nx) + verts[:, c]
+        keys.append(k)
+    allk = np.stack(keys, axis=1).ravel()  # element-major, local face minor
+    _, inv, cnt = np.unique(allk, return_inverse=True, return_counts=True)
+    bnd = np.nonzero(cnt[inv] == 1)[0]
+    ele, lf = bnd // nv, bnd % nv
+    cf = mesh.ndglno.reshape(mesh.nele, mesh.loc)
+    xnum = [boundary_numbering(d, 1, f + 1) for f in range(nv)]
+    fnum = [boundary_numbering(d, mesh.degree, f + 1) for f in range(nv)]
+    fx = np.stack([cx[ele, np.array(xnum)[lf, c]] for c in range(d)], axis=1) + 1
+    fl = len(fnum[0])
+    fn = np.stack([cf[ele, np.array(fnum)[lf, c]] for c in range(fl)], axis=1)
+    cen = mesh.X[fx - 1].mean(axis=1)
+    return Surface(int(bnd.size), fl, fn.astype(np.int32).ravel(), fx.astype(np.int32).ravel(),
+                   (ele + 1).astype(np.int32), cen)
+
+
+BC_NONE, BC_NEUMANN, BC_WEAKDIRICHLET, BC_INTERNAL, BC_ROBIN = 0, 1, 2, 3, 4
+
+
+def unit_box_bcs(mesh: Mesh, surf: Surface, kind: str = "neumann_robin"):
+    """Boundary conditions on the unit square / cube by side.
+    "neumann_robin": x = 0 Neumann with flux g = 1 + y/2, x = 1 Robin with
+    g = 0.5 and coefficient h = 2 + y, every other side strong Dirichlet 0.
+    "weak": x = 0 weak Dirichlet with value 1 - y, every other side none.
+    Returns (bc_type, bc_value, bc_value2, dirichlet_nodes): per face its
+    type, the bc fields at its nodes (nfaces*floc), and the nodes of the
+    strong-Dirichlet sides (1-based)."""
+    tol = 1e-12
+    cen = surf.centroid
+    x0 = np.abs(cen[:, 0]) < tol
+    x1 = np.abs(cen[:, 0] - 1.0) < tol
+    t = np.zeros(surf.nfaces, np.int32)
+    pos = mesh.node_pos[surf.face_nodes.reshape(surf.nfaces, surf.floc) - 1]  # (nf, floc, dim)
+    y = pos[:, :, 1]
+    v1 = np.zeros_like(y)
+    v2 = np.zeros_like(y)
+    if kind == "neumann_robin":
+        t[x0] = BC_NEUMANN
+        t[x1] = BC_ROBIN
+        v1[x0] = 1.0 + 0.5 * y[x0]
+        v1[x1] = 0.5
+        v2[x1] = 2.0 + y[x1]
+        strong = t == BC_NONE
+    elif kind == "weak":
+        t[x0] = BC_WEAKDIRICHLET
+        v1[x0] = 1.0 - y[x0]
+        strong = np.zeros(surf.nfaces, bool)
+    else:
+        raise ValueError(kind)
+    dn = np.unique(surf.face_nodes.reshape(surf.nfaces, surf.floc)[strong])
+    return t, np.ascontiguousarray(v1.ravel()), np.ascontiguousarray(v2.ravel()), dn.astype(np.int32)
+
+
 def read_gmsh(path: str) -> Mesh:
     """P1 mesh from a gmsh 2.2 ASCII file (the format of Fluidity's
     read_mesh_files(..., format="gmsh")): triangles (type 2) in 2D or

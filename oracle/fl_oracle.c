@@ -120,6 +120,49 @@ int orc_quadrature(int dim, int degree, double *l, double *w)
             c[0] = 0.166666666666666666666666666666666;
             c[1] = 1.0 - 2.0 * c[0];
             dk = expand(3, 3, tri5, c, 0.166666666666666666666666666666666, l, w, dk);
+        } else if (degree == 3) { /* str71 4 point, :953-970 */
+            c[0] = 0.333333333333333333333333333333333;
+            dk = expand(3, 1, tri4, c, -0.28125, l, w, dk);
+            c[0] = 0.2;
+            c[1] = 1.0 - 2.0 * c[0];
+            dk = expand(3, 3, tri5, c, 0.260416666666666666666666666666666, l, w, dk);
+        } else if (degree == 4) { /* 6 point, :974-993 */
+            c[0] = 0.0915762135097707434595714634022015;
+            c[1] = 1.0 - 2.0 * c[0];
+            dk = expand(3, 3, tri5, c, 0.0549758718276609338191631624501052, l, w, dk);
+            c[0] = 0.445948490915964886318329253883051;
+            c[1] = 1.0 - 2.0 * c[0];
+            dk = expand(3, 3, tri5, c, 0.111690794839005732847503504216561, l, w, dk);
+        } else {
+            return -1;
+        }
+        return dk;
+    }
+    if (dim == 1) {
+        /* interval rules (:1140-1230); interval_permutations(1) = (1,1),
+         * (2) = columns (1,2), (2,1) (:1893-1905) */
+        static const int int1[2] = {1, 1};
+        static const int int2[4] = {1, 2, 2, 1};
+        if (degree <= 1) {
+            c[0] = 0.5;
+            dk = expand(2, 1, int1, c, 1.0, l, w, dk);
+        } else if (degree == 2) {
+            c[0] = 0.788675134594813;
+            c[1] = 1.0 - c[0];
+            dk = expand(2, 2, int2, c, 0.5, l, w, dk);
+        } else if (degree == 3) {
+            c[0] = 0.887298334620742;
+            c[1] = 1.0 - c[0];
+            dk = expand(2, 2, int2, c, 0.277777777777777, l, w, dk);
+            c[0] = 0.5;
+            dk = expand(2, 1, int1, c, 0.444444444444444, l, w, dk);
+        } else if (degree == 4) {
+            c[0] = 0.9305681557970262;
+            c[1] = 1.0 - c[0];
+            dk = expand(2, 2, int2, c, 0.173927422568727, l, w, dk);
+            c[0] = 0.6699905217924281;
+            c[1] = 1.0 - c[0];
+            dk = expand(2, 2, int2, c, 0.326072577431273, l, w, dk);
         } else {
             return -1;
         }
@@ -674,6 +717,209 @@ int orc_assemble(const orc_params *p, const orc_mesh *m, double *A,
     }
     if (seconds) *seconds = wall_now() - t0;
     return err ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Surface integrals (Advection_Diffusion_CG.F90:606-638, :1228-1379)   */
+/* ------------------------------------------------------------------ */
+
+enum { BC_NEUMANN = 1, BC_WEAKDIRICHLET = 2, BC_INTERNAL = 3, BC_ROBIN = 4 }; /* :74-75 */
+
+#define FN_(i, g) (s->fn[(i) + fl * (g)])
+#define FXDN_(i, g, d) (s->fx_dn[(i) + flx * ((g) + fg * (d))])
+
+/* assemble_advection_diffusion_face_cg (:1228-1290) for one face:
+ * mat(floc,floc) column-major, rhs(floc).  Returns 0, or -2 for a weak
+ * Dirichlet face with diffusivity (the FLExit of :1374-1377). */
+static int face_cg(const orc_params *p, const orc_mesh *m, const orc_surface *s, int face,
+                   double *mat, double *rhs)
+{
+    const int dim = m->dim, fl = s->floc, fg = s->fngi, flx = s->floc_x, lx = m->loc_x;
+    const int bc = s->bc_type[face];
+    const int32_t *fnod = s->face_nodes + (int64_t)face * fl;
+    const int32_t *fxn = s->face_nodes_x + (int64_t)face * flx;
+    const double *bv = s->bc_value ? s->bc_value + (int64_t)face * fl : NULL;
+    const double *bv2 = s->bc_value2 ? s->bc_value2 + (int64_t)face * fl : NULL;
+    const double eps = DBL_EPSILON;
+    const double dt_theta = p->dt * p->theta;
+    const int adv_face = p->have_advection && p->integrate_by_parts;
+    double detwei[MAXGI], nrm[MAXDIM] = {0.0, 0.0, 0.0}, Tf[MAXLOC];
+
+    for (int i = 0; i < fl * fl; i++) mat[i] = 0.0;
+    for (int i = 0; i < fl; i++) rhs[i] = 0.0;
+    for (int g = 0; g < fg; g++) detwei[g] = 0.0;
+
+    /* Step 1: transform_facet_to_physical (Transform_elements.F90:1353-1525
+     * with the normal, :1557-1640 without), linear simplex coordinates:
+     * one Jacobian per face */
+    if (adv_face || (p->have_diffusivity && (bc == BC_NEUMANN || bc == BC_ROBIN))) {
+        double Xf[MAXDIM][MAXLOC], J[MAXDIM][2], detJ = 0.0;
+        for (int k = 0; k < flx; k++)
+            for (int a = 0; a < dim; a++) Xf[a][k] = m->X[a + dim * (int64_t)(fxn[k] - 1)];
+        for (int a = 0; a < dim; a++)
+            for (int d = 0; d < dim - 1; d++) {
+                double t = 0.0;
+                for (int k = 0; k < flx; k++) t = t + Xf[a][k] * FXDN_(k, 0, d);
+                J[a][d] = t;
+            }
+        if (dim == 2) {
+            detJ = sqrt(J[0][0] * J[0][0] + J[1][0] * J[1][0]);
+        } else if (dim == 3) {
+            for (int i = 1; i <= 3; i++) { /* cyc3(i) = mod(i-1,3)+1 */
+                int a2 = (i + 1) % 3, a1 = i % 3;
+                double t = J[a2][0] * J[a1][1] - J[a2][1] * J[a1][0];
+                detJ = detJ + t * t;
+            }
+            detJ = sqrt(detJ);
+        } else {
+            detJ = 1.0;
+        }
+        for (int g = 0; g < fg; g++) detwei[g] = detJ * s->fw[g];
+        if (adv_face) {
+            /* outward vector: face centroid minus element centroid (:1436-1437) */
+            const int32_t *exn = m->ndglno_x + (int64_t)(s->face_ele[face] - 1) * lx;
+            double o[MAXDIM], dp = 0.0, len = 0.0;
+            for (int a = 0; a < dim; a++) {
+                double sf = 0.0, se = 0.0;
+                for (int k = 0; k < flx; k++) sf = sf + Xf[a][k];
+                for (int k = 0; k < lx; k++) se = se + m->X[a + dim * (int64_t)(exn[k] - 1)];
+                o[a] = sf / flx - se / lx;
+            }
+            /* facet_normal (:1527-1555); cross_product (Vector_Tools.F90:76-85) */
+            if (dim == 2) {
+                nrm[0] = -J[1][0];
+                nrm[1] = J[0][0];
+            } else if (dim == 3) {
+                nrm[0] = J[1][0] * J[2][1] - J[2][0] * J[1][1];
+                nrm[1] = J[2][0] * J[0][1] - J[0][0] * J[2][1];
+                nrm[2] = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+            } else {
+                nrm[0] = 1.0;
+            }
+            for (int a = 0; a < dim; a++) dp = dp + nrm[a] * o[a];
+            for (int a = 0; a < dim; a++) nrm[a] = nrm[a] * dp;
+            for (int a = 0; a < dim; a++) len = len + nrm[a] * nrm[a];
+            len = sqrt(len);
+            for (int a = 0; a < dim; a++) nrm[a] = nrm[a] / len;
+        }
+    }
+    for (int i = 0; i < fl; i++) Tf[i] = m->T[fnod[i] - 1];
+
+    /* Step 2a: add_advection_face_cg (:1292-1347) */
+    if (adv_face) {
+        double cq[MAXGI], adv[MAXLOC][MAXLOC];
+        for (int g = 0; g < fg; g++) {
+            double un = 0.0;
+            for (int d = 0; d < dim; d++) {
+                double uq = 0.0; /* face_val_at_quad (Fields_Base.F90:2392-2400) */
+                for (int k = 0; k < fl; k++) {
+                    double uk = m->u_const ? m->U[d] : m->U[d + dim * (int64_t)(fnod[k] - 1)];
+                    uq = uq + uk * FN_(k, g);
+                }
+                un = un + uq * nrm[d];
+            }
+            cq[g] = detwei[g] * un;
+        }
+        for (int i = 0; i < fl; i++)
+            for (int j = 0; j < fl; j++) {
+                double t = 0.0;
+                for (int g = 0; g < fg; g++) t = t + (FN_(i, g) * FN_(j, g)) * cq[g];
+                adv[i][j] = t;
+            }
+        if (fabs(dt_theta) > eps) {
+            if (bc == BC_WEAKDIRICHLET) {
+                for (int i = 0; i < fl; i++) {
+                    double t = 0.0;
+                    for (int j = 0; j < fl; j++) t = t + adv[i][j] * (bv[j] - Tf[j]);
+                    rhs[i] = rhs[i] - p->theta * t;
+                }
+            } else {
+                for (int i = 0; i < fl; i++)
+                    for (int j = 0; j < fl; j++)
+                        mat[i + fl * j] = mat[i + fl * j] + dt_theta * adv[i][j];
+            }
+        }
+        for (int i = 0; i < fl; i++) {
+            double t = 0.0;
+            for (int j = 0; j < fl; j++) t = t + adv[i][j] * Tf[j];
+            rhs[i] = rhs[i] - t;
+        }
+    }
+
+    /* Step 2b: add_diffusivity_face_cg (:1349-1379) */
+    if (p->have_diffusivity) {
+        if (bc == BC_NEUMANN || bc == BC_ROBIN) {
+            double wq[MAXGI];
+            for (int g = 0; g < fg; g++) {
+                double t = 0.0;
+                for (int k = 0; k < fl; k++) t = t + bv[k] * FN_(k, g);
+                wq[g] = detwei[g] * t;
+            }
+            for (int i = 0; i < fl; i++) { /* shape_rhs (FETools.F90:38-54) */
+                double t = 0.0;
+                for (int g = 0; g < fg; g++) t = t + FN_(i, g) * wq[g];
+                rhs[i] = rhs[i] + t;
+            }
+        }
+        if (bc == BC_ROBIN) {
+            double wq[MAXGI], rob[MAXLOC][MAXLOC];
+            for (int g = 0; g < fg; g++) {
+                double t = 0.0;
+                for (int k = 0; k < fl; k++) t = t + bv2[k] * FN_(k, g);
+                wq[g] = detwei[g] * t;
+            }
+            for (int i = 0; i < fl; i++)
+                for (int j = 0; j < fl; j++) {
+                    double t = 0.0;
+                    for (int g = 0; g < fg; g++) t = t + (FN_(i, g) * FN_(j, g)) * wq[g];
+                    rob[i][j] = t;
+                }
+            if (fabs(dt_theta) > eps)
+                for (int i = 0; i < fl; i++)
+                    for (int j = 0; j < fl; j++)
+                        mat[i + fl * j] = mat[i + fl * j] + dt_theta * rob[i][j];
+            for (int i = 0; i < fl; i++) {
+                double t = 0.0;
+                for (int j = 0; j < fl; j++) t = t + rob[i][j] * Tf[j];
+                rhs[i] = rhs[i] - t;
+            }
+        } else if (bc == BC_WEAKDIRICHLET) {
+            return -2;
+        }
+    }
+    return 0;
+}
+
+int orc_face(const orc_params *p, const orc_mesh *m, const orc_surface *s, int face,
+             double *mat, double *rhs)
+{
+    return face_cg(p, m, s, face, mat, rhs);
+}
+
+int orc_surface_assemble(const orc_params *p, const orc_mesh *m, const orc_surface *s,
+                         double *A, double *rhs)
+{
+    const int fl = s->floc;
+    double mat[MAXLOC * MAXLOC], r[MAXLOC];
+    /* the face loop runs only with advection integrated by parts or with
+     * diffusivity (:608-611); internal faces are skipped (:627) */
+    if (!((p->integrate_by_parts && p->have_advection) || p->have_diffusivity)) return 0;
+    for (int f = 0; f < s->nfaces; f++) {
+        if (s->bc_type[f] == BC_INTERNAL) continue;
+        int rc = face_cg(p, m, s, f, mat, r);
+        if (rc) return rc;
+        const int32_t *fnod = s->face_nodes + (int64_t)f * fl;
+        for (int i = 0; i < fl; i++)
+            for (int j = 0; j < fl; j++) {
+                double v = mat[i + fl * j];
+                if (v == 0.0) continue; /* csr_addto skips zeros */
+                int64_t pos = csr_pos(m->findrm, m->colm, fnod[i], fnod[j]);
+                if (pos < 0) return -1;
+                A[pos] = A[pos] + v;
+            }
+        for (int i = 0; i < fl; i++) rhs[fnod[i] - 1] = rhs[fnod[i] - 1] + r[i];
+    }
+    return 0;
 }
 
 /* ------------------------------------------------------------------ */

@@ -20,8 +20,8 @@ extern "C" {
 #endif
 
 /* Quadrature.F90:164-240 (template selection), :572-605 (expansion),
- * :649-735 (tet rules), :909-950 (tri rules).  l is ngi x (dim+1), row per
- * point; returns ngi or -1. */
+ * :649-735 (tet rules), :909-993 (tri rules), :1140-1230 (interval
+ * rules).  l is ngi x (dim+1), row per point; returns ngi or -1. */
 int orc_quadrature(int dim, int degree, double *l, double *w);
 
 /* Shape_Functions.F90:128-329 + Elements.F90:615-663: Lagrange simplex
@@ -86,6 +86,36 @@ int orc_assemble(const orc_params *p, const orc_mesh *m, double *A,
  * rhs(loc).  Used by the unit tests of the element arithmetic. */
 int orc_element(const orc_params *p, const orc_mesh *m, int ele,
                 double *mat, double *rhs);
+
+/* Surface elements of T with their boundary conditions, as
+ * assemble_advection_diffusion_cg passes them to the face loop
+ * (Advection_Diffusion_CG.F90:606-638): face_global_nodes of T and of the
+ * positions, face_ele, face_shape(t) tables on the surface quadrature, the
+ * bc type from get_entire_boundary_condition (0, 1 neumann,
+ * 2 weakdirichlet, 3 internal, 4 robin; :74-75) and the bc fields'
+ * values at each face's nodes (ele_val(t_bc, face), ele_val(t_bc_2, face)). */
+typedef struct {
+    int32_t nfaces, floc, fngi, floc_x;
+    const int32_t *face_nodes;    /* floc*nfaces, 1-based T nodes */
+    const int32_t *face_nodes_x;  /* floc_x*nfaces, 1-based coordinate nodes */
+    const int32_t *face_ele;      /* nfaces, 1-based */
+    const double *fn, *fw;        /* fn(floc,fngi), fw(fngi) */
+    const double *fx_dn;          /* fx_dn(floc_x,fngi,dim-1) */
+    const int32_t *bc_type;       /* nfaces */
+    const double *bc_value;       /* floc*nfaces */
+    const double *bc_value2;      /* floc*nfaces (robin) */
+} orc_surface;
+
+/* The face loop (:606-638 -> :1228-1379): faces in index order, each
+ * face's matrix and rhs added with the zero-skipping csr addto.  Adds to A
+ * and rhs (after orc_assemble, before orc_dirichlet).  Returns 0, -1 for an
+ * entry outside the pattern, -2 for weak Dirichlet with diffusivity. */
+int orc_surface_assemble(const orc_params *p, const orc_mesh *m,
+                         const orc_surface *s, double *A, double *rhs);
+
+/* One face: mat(floc,floc) column-major, rhs(floc). */
+int orc_face(const orc_params *p, const orc_mesh *m, const orc_surface *s,
+             int face, double *mat, double *rhs);
 
 /* Boundary_Conditions.F90:1983-2025.  nodes 1-based; inactive[N] set to 1;
  * rhs(node) = (g - T(node))/dt when dt > 0, else g. */

@@ -38,6 +38,13 @@ class OrcMesh(C.Structure):
                 ("nrows", C.c_int32), ("findrm", C.c_void_p), ("colm", C.c_void_p)]
 
 
+class OrcSurface(C.Structure):
+    _fields_ = [("nfaces", C.c_int32), ("floc", C.c_int32), ("fngi", C.c_int32), ("floc_x", C.c_int32),
+                ("face_nodes", C.c_void_p), ("face_nodes_x", C.c_void_p), ("face_ele", C.c_void_p),
+                ("fn", C.c_void_p), ("fw", C.c_void_p), ("fx_dn", C.c_void_p),
+                ("bc_type", C.c_void_p), ("bc_value", C.c_void_p), ("bc_value2", C.c_void_p)]
+
+
 class OrcSolver(C.Structure):
     _fields_ = [("ksp", C.c_int32), ("pc", C.c_int32), ("rtol", C.c_double), ("atol", C.c_double),
                 ("max_its", C.c_int32), ("restart", C.c_int32), ("nthreads", C.c_int32)]
@@ -66,6 +73,10 @@ def load():
         lib.orc_assemble.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcMesh), P, P, C.c_int,
                                      C.POINTER(C.c_double)]
         lib.orc_element.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcMesh), C.c_int, P, P]
+        lib.orc_surface_assemble.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcMesh), C.POINTER(OrcSurface),
+                                             P, P]
+        lib.orc_face.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcMesh), C.POINTER(OrcSurface), C.c_int,
+                                 P, P]
         lib.orc_dirichlet.argtypes = [C.c_int, P, P, P, C.c_double, P, P]
         lib.orc_dirichlet.restype = None
         lib.orc_csr_mult.argtypes = [C.c_int, P, P, P, P, P]
@@ -182,6 +193,50 @@ class Assembly:
         self.nrows = mesh.nnodes
 
 
+class SurfaceData:
+    """Holds the arrays an OrcSurface points at."""
+
+    def __init__(self, mesh, surf, felement, fxelement, bc_type, bc_value, bc_value2):
+        self.keep = []
+
+        def k(a, dt=np.float64):
+            arr = np.ascontiguousarray(a, dtype=dt).ravel()
+            self.keep.append(arr)
+            return _p(arr)
+
+        s = OrcSurface()
+        s.nfaces, s.floc, s.fngi, s.floc_x = surf.nfaces, surf.floc, felement.ngi, mesh.dim
+        s.face_nodes = k(surf.face_nodes, np.int32)
+        s.face_nodes_x = k(surf.face_nodes_x, np.int32)
+        s.face_ele = k(surf.face_ele, np.int32)
+        s.fn = k(felement.n.ravel(order="F"))
+        s.fw = k(felement.w)
+        s.fx_dn = k(fxelement.dn.ravel(order="F"))
+        s.bc_type = k(bc_type, np.int32)
+        s.bc_value = k(bc_value)
+        s.bc_value2 = k(bc_value2)
+        self.s = s
+
+
+def surface_assemble(p: OrcParams, asm: Assembly, sd: SurfaceData, A, rhs):
+    """Adds the face loop's contributions to A and rhs in place."""
+    rc = load().orc_surface_assemble(C.byref(p), C.byref(asm.m), C.byref(sd.s), _p(A), _p(rhs))
+    if rc == -1:
+        raise RuntimeError("oracle: face entry outside the sparsity pattern")
+    if rc == -2:
+        raise RuntimeError("oracle: weak Dirichlet with diffusivity is not supported")
+
+
+def face(p: OrcParams, asm: Assembly, sd: SurfaceData, f: int):
+    fl = sd.s.floc
+    mat = np.zeros(fl * fl)
+    r = np.zeros(fl)
+    rc = load().orc_face(C.byref(p), C.byref(asm.m), C.byref(sd.s), f, _p(mat), _p(r))
+    if rc:
+        raise RuntimeError(f"oracle: face {f} status {rc}")
+    return mat.reshape((fl, fl), order="F"), r
+
+
 def params(dt, theta, beta, have_mass=True, lump=False, adv=True, ibp=False, diff=True, iso=True,
            src=False, absorp=False) -> OrcParams:
     return OrcParams(dt, theta, beta, int(have_mass), int(lump), int(adv), int(ibp), int(diff), int(iso),
@@ -276,7 +331,17 @@ def reference_system(cfg, nthreads: int = 0, variant: dict | None = None):
                diff=cfg.kappa is not None, src=cfg.source is not None, absorp=cfg.absorption is not None)
     A, rhs, secs = assemble(p, asm, nthreads)
     inactive = np.zeros(mesh.nnodes, np.int32)
-    dirichlet(mesh.boundary, 0.0, cfg.T, cfg.dt, inactive, rhs)
+    strong = mesh.boundary
+    if v.get("bcs"):
+        # the face loop with boundary conditions by side of the unit box
+        from fluidity_amd.elements import make_face_element
+        from fluidity_amd.meshgen import surface_of, unit_box_bcs
+        surf = surface_of(mesh)
+        bt, b1, b2, strong = unit_box_bcs(mesh, surf, v["bcs"])
+        sd = SurfaceData(mesh, surf, make_face_element(mesh.dim, mesh.degree, cfg.quad_degree),
+                         make_face_element(mesh.dim, 1, cfg.quad_degree), bt, b1, b2)
+        surface_assemble(p, asm, sd, A, rhs)
+    dirichlet(strong, 0.0, cfg.T, cfg.dt, inactive, rhs)
     return dict(sparsity=sp, colour=colour, ncol=ncol, colour_sets=cs, A=A, rhs=rhs, inactive=inactive,
                 asm_seconds=secs, asm=asm, params=p)
 
