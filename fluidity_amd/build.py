@@ -31,8 +31,9 @@ COMMON = [
 # fused multiply-adds and must not be contracted any further.
 EXTRA = {
     "assemble.hip": ["-ffp-contract=off"],
+    "surface.hip": ["-ffp-contract=off"],
 }
-SOURCES = ["flgpu_context.hip", "assemble.hip", "krylov.hip", "learned.hip"]
+SOURCES = ["flgpu_context.hip", "assemble.hip", "surface.hip", "krylov.hip", "learned.hip"]
 
 
 def _compile(src: str) -> str:

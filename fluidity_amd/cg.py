@@ -23,8 +23,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import lib as L
-from .elements import Element, make_element
-from .meshgen import Mesh
+from .elements import Element, make_element, make_face_element
+from .meshgen import Mesh, Surface
 
 
 @dataclass
@@ -168,6 +168,20 @@ class CGProblem:
                 "flgpu_colouring_bind")
         self.T = None
 
+    def bind_surface(self, surface: Surface) -> None:
+        """Surface elements for the face loop: face_global_nodes of T and of
+        the positions, face_ele, and the face elements on the surface rule
+        of the volume rule's degree."""
+        mesh = self.mesh
+        fe = make_face_element(mesh.dim, mesh.degree, self.element.quad_degree)
+        fxe = make_face_element(mesh.dim, 1, self.element.quad_degree)
+        L.check(self.lib.flgpu_surface_bind(self.ctx, surface.nfaces, surface.floc, fe.ngi,
+                                            L.ptr(surface.face_nodes), L.ptr(surface.face_nodes_x),
+                                            L.ptr(surface.face_ele), L.ptr(fe.fortran(fe.n)),
+                                            L.ptr(np.ascontiguousarray(fe.w)), L.ptr(fxe.fortran(fxe.dn))),
+                "flgpu_surface_bind")
+        self.surface = surface
+
     # -- fields --------------------------------------------------------
     def set_field(self, which: int, values, ncomp: int) -> None:
         """Nodal array (nnodes,) / (nnodes, ncomp) or a constant."""
@@ -223,12 +237,33 @@ class CGProblem:
             pass
 
 
-def assemble_advection_diffusion_cg(problem: CGProblem, opts: CGOptions, dt: float) -> None:
-    """The element loop of assemble_advection_diffusion_cg on the device
-    (zero matrix and rhs, colour loop, insertion)."""
+@dataclass
+class SurfaceBCs:
+    """What get_entire_boundary_condition hands the face loop
+    (assemble/Advection_Diffusion_CG.F90:613-622): per surface element its
+    type (BC_NONE, BC_NEUMANN, BC_WEAKDIRICHLET, BC_INTERNAL, BC_ROBIN) and
+    the bc fields at its nodes (floc per face; value2 is the Robin
+    coefficient)."""
+    bc_type: np.ndarray
+    value: np.ndarray | None = None
+    value2: np.ndarray | None = None
+
+
+def assemble_advection_diffusion_cg(problem: CGProblem, opts: CGOptions, dt: float,
+                                    bcs: SurfaceBCs | None = None) -> None:
+    """assemble_advection_diffusion_cg on the device up to the strong
+    Dirichlet conditions: the element loop (zero matrix and rhs, colour
+    loop, insertion) and, with bcs, the face loop (:606-638) over the bound
+    surface."""
     p = opts.params(dt)
     L.check(problem.lib.flgpu_advdif_assemble_resident(problem.ctx, C.byref(p)),
             "flgpu_advdif_assemble_resident")
+    if bcs is not None:
+        t = np.ascontiguousarray(bcs.bc_type, dtype=np.int32)
+        v1 = None if bcs.value is None else np.ascontiguousarray(bcs.value, dtype=np.float64)
+        v2 = None if bcs.value2 is None else np.ascontiguousarray(bcs.value2, dtype=np.float64)
+        L.check(problem.lib.flgpu_advdif_surface(problem.ctx, C.byref(p), L.ptr(t), L.ptr(v1), L.ptr(v2)),
+                "flgpu_advdif_surface")
 
 
 def apply_dirichlet_conditions(problem: CGProblem, nodes: np.ndarray, values, dt: float | None) -> None:
@@ -251,11 +286,11 @@ def petsc_solve(problem: CGProblem, solver: SolverOptions, x0: np.ndarray | None
 
 
 def solve_field_equation_cg(problem: CGProblem, opts: CGOptions, solver: SolverOptions, dt: float,
-                            bc_nodes: np.ndarray | None = None, bc_values=0.0):
-    """One solve_field_equation_cg step: assemble, strong Dirichlet, solve
-    for the rate of change and update T (T += dt*dT).  Returns (T_new,
-    iterations_taken, info)."""
-    assemble_advection_diffusion_cg(problem, opts, dt)
+                            bc_nodes: np.ndarray | None = None, bc_values=0.0, bcs: SurfaceBCs | None = None):
+    """One solve_field_equation_cg step: assemble (elements, then faces
+    with bcs), strong Dirichlet, solve for the rate of change and update T
+    (T += dt*dT).  Returns (T_new, iterations_taken, info)."""
+    assemble_advection_diffusion_cg(problem, opts, dt, bcs)
     if bc_nodes is not None and bc_nodes.size:
         apply_dirichlet_conditions(problem, bc_nodes, bc_values, dt)
     dT, info = petsc_solve(problem, solver)

@@ -180,6 +180,7 @@ int flgpu_field_mesh_bind(flgpu_ctx* c, int32_t loc, int32_t ngi, int32_t nnodes
     c->plan.built = false;
     c->plan.epos_built = false;
     c->plan.fail_rc = 0;
+    c->surf.bound = false;  // the surface names nodes of the previous field mesh
     return FLGPU_OK;
 }
 

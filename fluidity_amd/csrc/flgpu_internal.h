@@ -131,6 +131,34 @@ struct BlockPlan {
     std::vector<uint32_t> h_ltab64, h_ltab32;  // what the images hold
 };
 
+// Surface elements of T for the face loop (surface.hip).
+constexpr int kMaxFaceLoc = 6;   // P2 triangle
+constexpr int kMaxFaceGi = 16;
+
+struct FaceTables {
+    int32_t floc, fngi, floc_x, pad;
+    double fn[kMaxFaceLoc * kMaxFaceGi];      // fn[i + floc*g]: face_shape(t)%n
+    double fw[kMaxFaceGi];
+    double fxdn[kMaxDim * kMaxFaceGi * 2];    // fxdn[k + floc_x*(g + fngi*d)]: face_shape(positions)%dn
+};
+
+struct SurfaceState {
+    bool bound = false;
+    int32_t nfaces = 0, floc = 0, fngi = 0;
+    DevBuf<int32_t> fnodes, fnodes_x, fele;   // 0-based: T nodes, coordinate nodes, element of each face
+    std::vector<int32_t> h_fnodes;
+    FaceTables tab{};
+    DevBuf<FaceTables> d_tab;
+    // addition lists for the bound sparsity: per CSR entry / rhs row the
+    // face values it receives, in face order
+    int64_t lists_gen = -1;
+    int32_t nent = 0, nrow = 0;
+    DevBuf<int32_t> tpos, tptr, tsrc, rrow, rptr, rsrc;
+    DevBuf<double> fm, fr;                    // per face: floc^2 matrix values, floc rhs values
+    DevBuf<int32_t> bct;
+    DevBuf<double> bcv, bcv2;
+};
+
 struct KrylovWork;   // krylov.hip
 struct LearnedState; // learned.hip
 
@@ -179,6 +207,7 @@ struct flgpu_ctx {
     bool assembled = false;
 
     flgpu::BlockPlan plan;
+    flgpu::SurfaceState surf;
     flgpu::KrylovWork* kw = nullptr;
     flgpu::LearnedState* ls = nullptr;
 };

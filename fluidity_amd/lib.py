@@ -20,13 +20,15 @@ FLGPU_KSP_CG, FLGPU_KSP_GMRES, FLGPU_KSP_FCG = 0, 1, 2
 FLGPU_PC_NONE, FLGPU_PC_JACOBI, FLGPU_PC_LEARNED = 0, 1, 2
 FLGPU_NORM_PRECONDITIONED, FLGPU_NORM_UNPRECONDITIONED = 0, 1
 FIELD_T, FIELD_VELOCITY, FIELD_DIFFUSIVITY, FIELD_SOURCE, FIELD_ABSORPTION = range(5)
+BC_NONE, BC_NEUMANN, BC_WEAKDIRICHLET, BC_INTERNAL, BC_ROBIN = range(5)
 
 # Every symbol include/flgpu.h declares (checked by the CPU tests).
 EXPORTS = (
     "flgpu_init", "flgpu_finalize", "flgpu_last_error", "flgpu_mesh_bind",
     "flgpu_field_mesh_bind", "flgpu_sparsity_bind", "flgpu_colouring_bind",
     "flgpu_make_sparsity", "flgpu_colour_mesh", "flgpu_field_set",
-    "flgpu_advdif_assemble_resident", "flgpu_advdif_assemble", "flgpu_dirichlet_apply",
+    "flgpu_advdif_assemble_resident", "flgpu_advdif_assemble", "flgpu_surface_bind",
+    "flgpu_advdif_surface", "flgpu_dirichlet_apply",
     "flgpu_matrix_get", "flgpu_matrix_set", "flgpu_solve", "flgpu_petsc_solve",
     "flgpu_solution_get", "flgpu_csr_mult", "flgpu_learned_param_count",
     "flgpu_learned_set_params", "flgpu_learned_get_params", "flgpu_learned_init_params",
@@ -96,6 +98,8 @@ def load() -> C.CDLL:
         "flgpu_field_set": [P, I32, I32, P],
         "flgpu_advdif_assemble_resident": [P, C.POINTER(AdvdifParams)],
         "flgpu_advdif_assemble": [P, C.POINTER(AdvdifParams), P, P, P, P, P, P, P],
+        "flgpu_surface_bind": [P, I32, I32, I32, P, P, P, P, P, P],
+        "flgpu_advdif_surface": [P, C.POINTER(AdvdifParams), P, P, P],
         "flgpu_dirichlet_apply": [P, I32, P, P, D],
         "flgpu_matrix_get": [P, P, P, P],
         "flgpu_matrix_set": [P, P, P, P],

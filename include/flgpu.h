@@ -170,6 +170,48 @@ int flgpu_advdif_assemble(flgpu_ctx *ctx, const flgpu_advdif_params *p,
                           const double *kappa, const double *src,
                           const double *absorp, double *A_val, double *rhs);
 
+/* ---- surface integrals ------------------------------------------------- */
+/* Boundary-condition types of the face loop, the numbering of
+ * assemble/Advection_Diffusion_CG.F90:74-75 (0 = none of them, e.g. a strong
+ * Dirichlet face). */
+enum {
+    FLGPU_BC_NONE = 0,
+    FLGPU_BC_NEUMANN = 1,
+    FLGPU_BC_WEAKDIRICHLET = 2,
+    FLGPU_BC_INTERNAL = 3,
+    FLGPU_BC_ROBIN = 4
+};
+/* Surface elements of T for the face loop
+ * (assemble/Advection_Diffusion_CG.F90:606-638): per face
+ * face_global_nodes(t, face) (face_ndglno, floc*nfaces),
+ * face_global_nodes(positions, face) (face_ndglno_x, dim*nfaces) and
+ * face_ele (nfaces), all 1-based (femtools/Fields_Base.F90:1867-1932);
+ * the face element of T on the surface quadrature, face_shape(t)%n
+ * (fn(floc,fngi)) and %quadrature%weight (fw(fngi)), and the coordinate
+ * face element's face_shape(positions)%dn (fx_dn(dim,fngi,dim-1)).  Faces of
+ * the field mesh's degree (P1 or P2 segments / triangles); the coordinate
+ * mesh is linear.  Rebinding the field mesh unbinds the surface. */
+int flgpu_surface_bind(flgpu_ctx *ctx, int32_t nfaces, int32_t floc,
+                       int32_t fngi, const int32_t *face_ndglno,
+                       const int32_t *face_ndglno_x, const int32_t *face_ele,
+                       const double *fn, const double *fw,
+                       const double *fx_dn);
+/* The face loop of assemble_advection_diffusion_cg (:606-638 ->
+ * assemble_advection_diffusion_face_cg :1228-1379): for every face not of
+ * type FLGPU_BC_INTERNAL, the boundary term of advection integrated by parts
+ * (with the weak Dirichlet value on FLGPU_BC_WEAKDIRICHLET faces) and the
+ * Neumann / Robin terms of the diffusivity, added to the device-resident
+ * matrix and rhs in face order with csr_addto's skip of zeros.  Call after
+ * the element assembly and before flgpu_dirichlet_apply, with the same
+ * params.  bc_type(nfaces) as get_entire_boundary_condition returns it;
+ * bc_value / bc_value2 (floc*nfaces) are the bc fields at each face's
+ * nodes, ele_val(t_bc, face) and ele_val(t_bc_2, face) (the Robin
+ * coefficient); either may be NULL when no face needs it.  Weak Dirichlet
+ * with diffusivity is FLGPU_EUNSUPPORTED, as it is an FLExit in Fluidity
+ * (:1374-1377).  FP64 arithmetic whatever p->precision says. */
+int flgpu_advdif_surface(flgpu_ctx *ctx, const flgpu_advdif_params *p,
+                         const int32_t *bc_type, const double *bc_value,
+                         const double *bc_value2);
 /* Strong Dirichlet (femtools/Boundary_Conditions.F90:1983-2025): mark the
  * rows inactive (set_inactive, femtools/Sparse_Tools.F90:374-376) and set
  * rhs(node) = (value - T(node))/dt (dt > 0) or value (dt <= 0) on the
