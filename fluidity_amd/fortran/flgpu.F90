@@ -6,6 +6,8 @@
 !! Cut points it replaces:
 !!   assemble/Advection_Diffusion_CG.F90:565-598  (element loop)
 !!     -> flgpu_advdif_assemble
+!!   assemble/Advection_Diffusion_CG.F90:606-638  (face loop)
+!!     -> flgpu_surface_bind + flgpu_advdif_surface
 !!   assemble/Advection_Diffusion_CG.F90:1389      (petsc_solve)
 !!     -> flgpu_petsc_solve
 !! Errors: a nonzero return is turned into FLAbort by flgpu_check, as
@@ -24,6 +26,9 @@ module flgpu
   integer(c_int32_t), parameter, public :: FLGPU_NORM_PRECONDITIONED = 0, FLGPU_NORM_UNPRECONDITIONED = 1
   integer(c_int32_t), parameter, public :: FLGPU_FIELD_T = 0, FLGPU_FIELD_VELOCITY = 1, &
        FLGPU_FIELD_DIFFUSIVITY = 2, FLGPU_FIELD_SOURCE = 3, FLGPU_FIELD_ABSORPTION = 4
+  !! boundary-condition types of the face loop (Advection_Diffusion_CG.F90:74-75)
+  integer(c_int32_t), parameter, public :: FLGPU_BC_NONE = 0, FLGPU_BC_NEUMANN = 1, &
+       FLGPU_BC_WEAKDIRICHLET = 2, FLGPU_BC_INTERNAL = 3, FLGPU_BC_ROBIN = 4
 
   !! flgpu_advdif_params: the options assemble_advection_diffusion_cg reads
   !! (Advection_Diffusion_CG.F90:282-556); logicals as 0/1 integers.
@@ -49,7 +54,8 @@ module flgpu
 
   public :: flgpu_init, flgpu_finalize, flgpu_mesh_bind, flgpu_field_mesh_bind, &
        flgpu_sparsity_bind, flgpu_colouring_bind, flgpu_make_sparsity, flgpu_colour_mesh, &
-       flgpu_field_set, flgpu_advdif_assemble, flgpu_dirichlet_apply, flgpu_matrix_get, &
+       flgpu_field_set, flgpu_advdif_assemble, flgpu_surface_bind, flgpu_advdif_surface, &
+       flgpu_dirichlet_apply, flgpu_matrix_get, &
        flgpu_petsc_solve, flgpu_check
 
   interface
@@ -136,6 +142,30 @@ module flgpu
        type(flgpu_advdif_params), intent(in) :: p
        type(c_ptr), value :: T, U, kappa, src, absorp, A_val, rhs
      end function flgpu_advdif_assemble
+
+     !! surface elements of T for the face loop (:606-638): face_global_nodes
+     !! of t and of positions, face_ele, face_shape(t)%n, its weights and
+     !! face_shape(positions)%dn
+     integer(c_int) function flgpu_surface_bind(ctx, nfaces, floc, fngi, face_ndglno, face_ndglno_x, face_ele, &
+          fn, fw, fx_dn) bind(c, name="flgpu_surface_bind")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nfaces, floc, fngi
+       integer(c_int32_t), intent(in) :: face_ndglno(*), face_ndglno_x(*), face_ele(*)
+       real(c_double), intent(in) :: fn(*), fw(*), fx_dn(*)
+     end function flgpu_surface_bind
+
+     !! the face loop itself, after flgpu_advdif_assemble: t_bc_types from
+     !! get_entire_boundary_condition, ele_val(t_bc, face) and
+     !! ele_val(t_bc_2, face) flattened per face (c_null_ptr when unused)
+     integer(c_int) function flgpu_advdif_surface(ctx, p, bc_type, bc_value, bc_value2) &
+          bind(c, name="flgpu_advdif_surface")
+       import :: c_int, c_ptr, c_int32_t, flgpu_advdif_params
+       type(c_ptr), value :: ctx
+       type(flgpu_advdif_params), intent(in) :: p
+       integer(c_int32_t), intent(in) :: bc_type(*)
+       type(c_ptr), value :: bc_value, bc_value2
+     end function flgpu_advdif_surface
 
      integer(c_int) function flgpu_dirichlet_apply(ctx, nbc, nodes, values, dt) bind(c, name="flgpu_dirichlet_apply")
        import :: c_int, c_ptr, c_int32_t, c_double
