@@ -375,6 +375,109 @@ int orc_colour_elements(int nnodes, int nele, int loc, const int32_t *ndglno,
 /* Element assembly (Advection_Diffusion_CG.F90:702-1202)               */
 /* ------------------------------------------------------------------ */
 
+/* ------------------------------------------------------------------ */
+/* Upwind stabilisation (assemble/Upwind_Stabilisation.F90)             */
+/* ------------------------------------------------------------------ */
+
+enum { STAB_NONE = 0, STAB_SU = 1, STAB_SUPG = 2 };
+enum { NU_OPTIMAL = 1, NU_DOUBLY_ASYMPTOTIC = 2, NU_CRITICAL = 3, NU_UNITY = 4 };
+static const double upw_tolerance = 1.0e-10, tanh_tolerance = 11.859499013855018; /* :21-22 */
+
+/* inverse (femtools/Vector_Tools.F90:231-283): cofactors over det */
+static void inverse_small(int dim, double A[MAXDIM][MAXDIM], double B[MAXDIM][MAXDIM])
+{
+    if (dim == 3) {
+        double det = A[0][0] * (A[1][1] * A[2][2] - A[2][1] * A[1][2]) -
+                     A[1][0] * (A[0][1] * A[2][2] - A[2][1] * A[0][2]) +
+                     A[2][0] * (A[0][1] * A[1][2] - A[1][1] * A[0][2]);
+        B[0][0] = A[1][1] * A[2][2] - A[2][1] * A[1][2];
+        B[0][1] = A[2][1] * A[0][2] - A[0][1] * A[2][2];
+        B[0][2] = A[0][1] * A[1][2] - A[1][1] * A[0][2];
+        B[1][0] = A[1][2] * A[2][0] - A[2][2] * A[1][0];
+        B[1][1] = A[2][2] * A[0][0] - A[0][2] * A[2][0];
+        B[1][2] = A[0][2] * A[1][0] - A[1][2] * A[0][0];
+        B[2][0] = A[1][0] * A[2][1] - A[2][0] * A[1][1];
+        B[2][1] = A[2][0] * A[0][1] - A[0][0] * A[2][1];
+        B[2][2] = A[0][0] * A[1][1] - A[1][0] * A[0][1];
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) B[i][k] = B[i][k] / det;
+    } else if (dim == 2) {
+        double det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+        B[0][0] = A[1][1] / det;
+        B[1][1] = A[0][0] / det;
+        B[0][1] = -A[0][1] / det;
+        B[1][0] = -A[1][0] / det;
+    } else {
+        B[0][0] = 1.0 / A[0][0];
+    }
+}
+
+/* nu_bar_scaled_q (:243-319) at one quadrature point: u (dim), J with
+ * J[c][a] = dx_a/dxi_c (transform_to_physical's j, the transpose of the
+ * cached J_T, Transform_elements.F90:783-787), K the diffusivity there (used
+ * when have_k; else xi = sign(pe), NU_BAR_UNITY). */
+double orc_nu_bar_scaled(int dim, const double *u, const double *Jflat, const double *Kflat, int have_k,
+                         int scheme, double scale)
+{
+    double J[MAXDIM][MAXDIM], K[MAXDIM][MAXDIM], Ki[MAXDIM][MAXDIM], JK[MAXDIM][MAXDIM];
+    double uJ[MAXDIM], pe[MAXDIM], xi[MAXDIM];
+    for (int c = 0; c < dim; c++)
+        for (int a = 0; a < dim; a++) {
+            J[c][a] = Jflat[c * dim + a];
+            K[c][a] = have_k ? Kflat[c * dim + a] : 0.0;
+        }
+    int sch = have_k ? scheme : NU_UNITY;
+    double norm_u = 0.0, nu;
+    for (int d = 0; d < dim; d++) norm_u = norm_u + u[d] * u[d];
+    for (int a = 0; a < dim; a++) { /* matmul(u, j_mat) */
+        double t = 0.0;
+        for (int c = 0; c < dim; c++) t = t + u[c] * J[c][a];
+        uJ[a] = t;
+    }
+    if (norm_u < upw_tolerance) {
+        nu = 0.0;
+    } else if (sch == NU_UNITY) {
+        double t = 0.0;
+        for (int a = 0; a < dim; a++) t = t + fabs(uJ[a]);
+        nu = t / norm_u;
+    } else {
+        /* pe = 0.5 * matmul(u, matmul(j_mat, inverse(diff_q))) (:148, :182, :215) */
+        inverse_small(dim, K, Ki);
+        for (int c = 0; c < dim; c++)
+            for (int a = 0; a < dim; a++) {
+                double t = 0.0;
+                for (int b = 0; b < dim; b++) t = t + J[c][b] * Ki[b][a];
+                JK[c][a] = t;
+            }
+        for (int a = 0; a < dim; a++) {
+            double t = 0.0;
+            for (int c = 0; c < dim; c++) t = t + u[c] * JK[c][a];
+            pe[a] = 0.5 * t;
+        }
+        for (int a = 0; a < dim; a++) {
+            double p = pe[a];
+            if (sch == NU_OPTIMAL) { /* xi_optimal :128-160 */
+                if (fabs(p) < upw_tolerance) xi[a] = 0.0;
+                else if (p > tanh_tolerance) xi[a] = 1.0 - (1.0 / p);
+                else if (p < -tanh_tolerance) xi[a] = -1.0 - (1.0 / p);
+                else xi[a] = (1.0 / tanh(p)) - (1.0 / p);
+            } else if (sch == NU_DOUBLY_ASYMPTOTIC) { /* :162-194 */
+                if (fabs(p) <= 3.0) xi[a] = p / 3.0;
+                else if (p > 0.0) xi[a] = 1.0;
+                else xi[a] = -1.0;
+            } else { /* critical rule :196-228 */
+                if (fabs(p) <= 1.0) xi[a] = 0.0;
+                else if (p > 0.0) xi[a] = 1.0 - 1.0 / p;
+                else xi[a] = -1.0 - 1.0 / p;
+            }
+        }
+        double t = 0.0;
+        for (int a = 0; a < dim; a++) t = t + xi[a] * uJ[a];
+        nu = t / norm_u;
+    }
+    return nu * scale;
+}
+
 #define N_(i, g) (m->n[(i) + loc * (g)])
 #define DN_(i, g, d) (m->dn[(i) + loc * ((g) + ngi * (d))])
 #define XDN_(i, g, d) (m->x_dn[(i) + lx * ((g) + ngi * (d))])
@@ -441,13 +544,57 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
 
     for (int i = 0; i < loc; i++) Te[i] = m->T[en[i] - 1];
 
+    /* --- test function (:812-825): N, or the SUPG shape
+     * N + nu_bar u.grad N (supg_test_function, Upwind_Stabilisation.F90:
+     * 420-452); with it, for streamline upwind, u.grad N and nu_bar --- */
+    double tn[MAXLOC][MAXGI], udn[MAXLOC][MAXGI], nuq[MAXGI];
+    for (int i = 0; i < loc; i++)
+        for (int g = 0; g < ngi; g++) tn[i][g] = N_(i, g);
+    if (p->stabilisation != STAB_NONE) {
+        double Jf[MAXDIM * MAXDIM];
+        for (int c = 0; c < dim; c++)
+            for (int a = 0; a < dim; a++) Jf[c * dim + a] = JT[a][c]; /* j = transpose(J_T) */
+        for (int g = 0; g < ngi; g++) {
+            double u[MAXDIM], Kf[MAXDIM * MAXDIM];
+            for (int d = 0; d < dim; d++) {
+                double s = 0.0;
+                for (int i = 0; i < loc; i++) {
+                    double ui = m->u_const ? m->U[d] : m->U[d + dim * (int64_t)(en[i] - 1)];
+                    s = s + ui * N_(i, g);
+                }
+                u[d] = s;
+            }
+            if (p->have_diffusivity)
+                for (int a = 0; a < dim; a++)
+                    for (int b = 0; b < dim; b++) {
+                        double s = 0.0;
+                        for (int i = 0; i < loc; i++) {
+                            double v = m->kappa_const
+                                           ? m->kappa[a + dim * b]
+                                           : m->kappa[a + dim * b + dim * dim * (int64_t)(en[i] - 1)];
+                            s = s + v * N_(i, g);
+                        }
+                        Kf[a * dim + b] = s;
+                    }
+            nuq[g] = orc_nu_bar_scaled(dim, u, Jf, Kf, p->have_diffusivity, p->nu_bar_scheme, p->nu_bar_scale);
+            for (int j = 0; j < loc; j++) { /* u_nl_dn(j,i) = dot_product(u, dshape(j,i,:)) */
+                double t = 0.0;
+                for (int d = 0; d < dim; d++) t = t + u[d] * dsh[j][g][d];
+                udn[j][g] = t;
+            }
+            if (p->stabilisation == STAB_SUPG)
+                for (int i = 0; i < loc; i++) tn[i][g] = N_(i, g) + nuq[g] * udn[i][g];
+        }
+    }
+#define TN_(i, g) (tn[i][g])
+
     /* --- mass (add_mass_element_cg, :867-941) --- */
     if (p->have_mass) {
         double M[MAXLOC][MAXLOC];
         for (int i = 0; i < loc; i++)
             for (int j = 0; j < loc; j++) {
                 double s = 0.0;
-                for (int g = 0; g < ngi; g++) s = s + (N_(i, g) * N_(j, g)) * detwei[g];
+                for (int g = 0; g < ngi; g++) s = s + (TN_(i, g) * N_(j, g)) * detwei[g];
                 M[i][j] = s;
             }
         if (p->lump_mass) {
@@ -497,7 +644,7 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
                     for (int g = 0; g < ngi; g++) {
                         double t = 0.0;
                         for (int d = 0; d < dim; d++) t = t + dsh[j][g][d] * uq[d][g];
-                        s = s + (N_(i, g) * t) * detwei[g];
+                        s = s + (TN_(i, g) * t) * detwei[g];
                     }
                 } else {
                     /* -dshape_dot_vector_shape (FETools.F90:700-721) */
@@ -516,11 +663,20 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
                 for (int j = 0; j < loc; j++) {
                     double s = 0.0;
                     for (int g = 0; g < ngi; g++)
-                        s = s + (N_(i, g) * N_(j, g)) * (divq[g] * detwei[g]);
+                        s = s + (TN_(i, g) * N_(j, g)) * (divq[g] * detwei[g]);
                     if (p->integrate_by_parts)
                         adv[i][j] = adv[i][j] - coef * s;
                     else
                         adv[i][j] = adv[i][j] + coef * s;
+                }
+        }
+        if (p->stabilisation == STAB_SU) {
+            /* element_upwind_stabilisation (Upwind_Stabilisation.F90:52-84) */
+            for (int i = 0; i < loc; i++)
+                for (int j = 0; j < loc; j++) {
+                    double s = 0.0;
+                    for (int g = 0; g < ngi; g++) s = s + ((udn[i][g] * detwei[g]) * nuq[g]) * udn[j][g];
+                    adv[i][j] = adv[i][j] + s;
                 }
         }
         if (fabs(dt_theta) > eps)
@@ -548,7 +704,7 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
         for (int i = 0; i < loc; i++)
             for (int j = 0; j < loc; j++) {
                 double s = 0.0;
-                for (int g = 0; g < ngi; g++) s = s + (N_(i, g) * N_(j, g)) * sq[g];
+                for (int g = 0; g < ngi; g++) s = s + (TN_(i, g) * N_(j, g)) * sq[g];
                 ab[i][j] = s;
             }
         if (fabs(dt_theta) > eps)
@@ -636,10 +792,11 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
         }
         for (int i = 0; i < loc; i++) {
             double s = 0.0;
-            for (int g = 0; g < ngi; g++) s = s + N_(i, g) * fw[g];
+            for (int g = 0; g < ngi; g++) s = s + TN_(i, g) * fw[g];
             rhs[i] = rhs[i] + s;
         }
     }
+#undef TN_
     return 0;
 }
 

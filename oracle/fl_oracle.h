@@ -46,6 +46,11 @@ typedef struct {
     int32_t have_mass, lump_mass, have_advection, integrate_by_parts;
     int32_t have_diffusivity, isotropic_diffusivity, have_source,
         have_absorption;
+    /* stabilisation (Advection_Diffusion_CG.F90:71, :440-465): 0 none,
+     * 1 streamline upwind, 2 SUPG; nu_bar scheme (Upwind_Stabilisation.F90:
+     * 18-19): 1 optimal, 2 doubly asymptotic, 3 critical rule, 4 unity */
+    int32_t stabilisation, nu_bar_scheme;
+    double nu_bar_scale;
 } orc_params;
 
 typedef struct {
@@ -81,6 +86,12 @@ typedef struct {
  * nthreads <= 0 uses the OpenMP default. */
 int orc_assemble(const orc_params *p, const orc_mesh *m, double *A,
                  double *rhs, int nthreads, double *seconds);
+
+/* nu_bar_scaled_q of assemble/Upwind_Stabilisation.F90:243-319 at one
+ * quadrature point: u(dim); J(dim,dim) row-major with J[c][a] = dx_a/dxi_c
+ * (transform_to_physical's j); K(dim,dim) row-major, used when have_k. */
+double orc_nu_bar_scaled(int dim, const double *u, const double *J, const double *K, int have_k,
+                         int scheme, double scale);
 
 /* Same element routine, for one element: mat(loc,loc) column-major and
  * rhs(loc).  Used by the unit tests of the element arithmetic. */

@@ -22,7 +22,8 @@ class OrcParams(C.Structure):
                 ("have_mass", C.c_int32), ("lump_mass", C.c_int32), ("have_advection", C.c_int32),
                 ("integrate_by_parts", C.c_int32), ("have_diffusivity", C.c_int32),
                 ("isotropic_diffusivity", C.c_int32), ("have_source", C.c_int32),
-                ("have_absorption", C.c_int32)]
+                ("have_absorption", C.c_int32), ("stabilisation", C.c_int32), ("nu_bar_scheme", C.c_int32),
+                ("nu_bar_scale", C.c_double)]
 
 
 class OrcMesh(C.Structure):
@@ -77,6 +78,8 @@ def load():
                                              P, P]
         lib.orc_face.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcMesh), C.POINTER(OrcSurface), C.c_int,
                                  P, P]
+        lib.orc_nu_bar_scaled.argtypes = [C.c_int, P, P, P, C.c_int, C.c_int, C.c_double]
+        lib.orc_nu_bar_scaled.restype = C.c_double
         lib.orc_dirichlet.argtypes = [C.c_int, P, P, P, C.c_double, P, P]
         lib.orc_dirichlet.restype = None
         lib.orc_csr_mult.argtypes = [C.c_int, P, P, P, P, P]
@@ -237,10 +240,23 @@ def face(p: OrcParams, asm: Assembly, sd: SurfaceData, f: int):
     return mat.reshape((fl, fl), order="F"), r
 
 
+STABILISATION = {"none": 0, "streamline_upwind": 1, "supg": 2}
+NU_BAR = {"optimal": 1, "doubly_asymptotic": 2, "critical_rule": 3, "unity": 4}
+
+
 def params(dt, theta, beta, have_mass=True, lump=False, adv=True, ibp=False, diff=True, iso=True,
-           src=False, absorp=False) -> OrcParams:
+           src=False, absorp=False, stab="none", nu_bar="optimal", nu_scale=0.5) -> OrcParams:
     return OrcParams(dt, theta, beta, int(have_mass), int(lump), int(adv), int(ibp), int(diff), int(iso),
-                     int(src), int(absorp))
+                     int(src), int(absorp), STABILISATION[stab], NU_BAR[nu_bar], float(nu_scale))
+
+
+def nu_bar_scaled(u, J, K, scheme="optimal", scale=0.5):
+    """nu_bar_scaled_q at one point (K None: no diffusivity)."""
+    d = len(u)
+    uu = np.ascontiguousarray(u, np.float64)
+    JJ = np.ascontiguousarray(J, np.float64).reshape(d * d)
+    KK = np.ascontiguousarray(np.zeros((d, d)) if K is None else K, np.float64).reshape(d * d)
+    return load().orc_nu_bar_scaled(d, _p(uu), _p(JJ), _p(KK), int(K is not None), NU_BAR[scheme], scale)
 
 
 def assemble(p: OrcParams, asm: Assembly, nthreads: int = 0):
@@ -328,7 +344,9 @@ def reference_system(cfg, nthreads: int = 0, variant: dict | None = None):
     p = params(cfg.dt, cfg.theta, v.get("beta", cfg.beta), have_mass=cfg.have_mass,
                lump=v.get("lump_mass_matrix", False), adv=cfg.have_advection,
                ibp=v.get("integrate_advection_by_parts", False),
-               diff=cfg.kappa is not None, src=cfg.source is not None, absorp=cfg.absorption is not None)
+               diff=cfg.kappa is not None, src=cfg.source is not None, absorp=cfg.absorption is not None,
+               stab=v.get("stabilisation", "none"), nu_bar=v.get("nu_bar_scheme", "optimal"),
+               nu_scale=v.get("nu_bar_scale", 0.5))
     A, rhs, secs = assemble(p, asm, nthreads)
     inactive = np.zeros(mesh.nnodes, np.int32)
     strong = mesh.boundary
