@@ -43,6 +43,9 @@ class CGOptions:
     have_absorption: bool = False
     precision: str = "fp64"              # element arithmetic: fp64 | fp32
     scheme: str = "auto"                 # auto | colour | blocked
+    stabilisation: str = "none"          # none | streamline_upwind | supg
+    nu_bar_scheme: str = "optimal"       # optimal | doubly_asymptotic | critical_rule | unity
+    nu_scale: float = 0.5
 
     def params(self, dt: float) -> L.AdvdifParams:
         return L.AdvdifParams(
@@ -55,7 +58,9 @@ class CGOptions:
             have_source=int(self.have_source), have_absorption=int(self.have_absorption),
             precision={"fp64": L.FLGPU_FP64, "fp32": L.FLGPU_FP32}[self.precision],
             scheme={"auto": L.FLGPU_ASM_AUTO, "colour": L.FLGPU_ASM_COLOUR,
-                    "blocked": L.FLGPU_ASM_BLOCKED}[self.scheme])
+                    "blocked": L.FLGPU_ASM_BLOCKED}[self.scheme],
+            stabilisation=L.FLGPU_STAB[self.stabilisation], nu_bar_scheme=L.FLGPU_NU_BAR[self.nu_bar_scheme],
+            nu_bar_scale=self.nu_scale)
 
 
 @dataclass

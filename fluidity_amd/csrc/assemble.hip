@@ -174,6 +174,58 @@ static int table_image(flgpu_ctx* c, const uint32_t** out) {
     return FLGPU_OK;
 }
 
+// Stabilised assembly (streamline upwind / SUPG), any degree: one launch
+// per colour, thread = (element of the colour, local row i), the row by the
+// per-quadrature-point loop with the stabilised test function (row_stab),
+// CSR positions from the per-mesh table.
+template <int DIM, int LOC, typename R>
+__global__ void __launch_bounds__(256)
+k_asm_colour_stab(const uint32_t* __restrict__ ltab, const int32_t* __restrict__ elems, int count,
+                  const int32_t* __restrict__ conn, const int32_t* __restrict__ conn_x,
+                  const double* __restrict__ X4, AsmFields fd, AsmFlags fl, const int32_t* __restrict__ epos,
+                  double* __restrict__ A, double* __restrict__ rhs, int* __restrict__ err) {
+    __shared__ LTab<R> tb;
+    stage_table_image<R, 256>(ltab, tb);
+    __syncthreads();
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)count * LOC) return;
+    const int q = (int)(t / LOC), i = (int)(t % LOC);
+    const int ele = elems[q];
+    Elem<DIM, LOC, R> e;
+    gather_elem<DIM, LOC, R>(ele, conn, conn_x, X4, fd.T, e);
+    R vals[LOC], r;
+    row_stab<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
+    const int row = conn[(int64_t)ele * LOC + i];
+#pragma unroll
+    for (int j = 0; j < LOC; j++) {
+        const double v = (double)vals[j];
+        if (v == 0.0) continue;  // csr_addto skips zeros (Sparse_Tools.F90:2640)
+        const int pos = epos[(int64_t)ele * LOC * LOC + i * LOC + j];
+        if (pos < 0) {
+            atomicOr(err, 1);
+            continue;
+        }
+        A[pos] = A[pos] + v;
+    }
+    rhs[row] = rhs[row] + (double)r;
+}
+
+template <int DIM, int LOC, typename R>
+static int launch_colour_stab(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
+    FLGPU_TRY(ensure_entry_positions(c));
+    const uint32_t* img = nullptr;
+    FLGPU_TRY(table_image<R>(c, &img));
+    for (int k = 0; k < c->ncol; k++) {
+        const int off = c->h_col_ptr[k], cnt = c->h_col_ptr[k + 1] - off;
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL((k_asm_colour_stab<DIM, LOC, R>), dim3(grid_for((int64_t)cnt * LOC, 256)), dim3(256), 0,
+                           c->stream, img, c->col_elems.p + off, cnt, c->conn.p, c->conn_x.p, c->X4.p, fd, fl,
+                           c->plan.epos.p, c->A.p, c->rhs.p, d_err);
+    }
+    FLGPU_HIP(hipGetLastError());
+    return FLGPU_OK;
+}
+
 // One launch per colour; thread = (element of the colour, local row i).
 // Elements of one colour share no node, so the read-modify-writes of a
 // launch never meet.
@@ -1195,6 +1247,20 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     fl.absorp = p->have_absorption != 0;
     fl.need_div = fl.adv && (fl.ibp ? (std::fabs(1.0 - p->beta) > eps) : (std::fabs(p->beta) > eps));
     fl.dtt_on = std::fabs(fl.dt_theta) > eps;
+    fl.stab = p->stabilisation;
+    fl.nu_scheme = p->nu_bar_scheme;
+    fl.nu_scale = p->nu_bar_scale;
+    if (fl.stab < 0 || fl.stab > FLGPU_STAB_SUPG)
+        return set_error(FLGPU_EINVAL, "flgpu assemble: stabilisation %d", fl.stab);
+    if (fl.stab) {
+        if (fl.nu_scheme < FLGPU_NU_BAR_OPTIMAL || fl.nu_scheme > FLGPU_NU_BAR_UNITY)
+            return set_error(FLGPU_EINVAL, "flgpu assemble: nu_bar scheme %d", fl.nu_scheme);
+        if (!(fl.nu_scale >= 0.0))
+            return set_error(FLGPU_EINVAL, "flgpu assemble: nu_bar scale %g < 0", fl.nu_scale);
+        if (!c->fU.set) return set_error(FLGPU_EINVAL, "flgpu assemble: stabilisation needs the velocity field");
+        if (p->scheme == FLGPU_ASM_BLOCKED)
+            return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: stabilised assembly runs on the colour scheme");
+    }
     AsmFields fd;
     fd.T = c->fT.v.p;
     fd.U = c->fU.v.p; fd.u_c = c->fU.constant;
@@ -1210,7 +1276,7 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     // only.  Chosen automatically, it gives way to the colour scheme when
     // the mesh does not fit its plan (a node in more elements than a slab
     // holds, a row longer than 255 entries).
-    bool blocked_ok = c->linear;
+    bool blocked_ok = c->linear && !fl.stab;
     if (blocked_ok && (fp32 || scheme == FLGPU_ASM_AUTO)) {
         const int rc = build_block_plan(c);
         if (rc == FLGPU_EUNSUPPORTED && scheme == FLGPU_ASM_AUTO)
@@ -1244,6 +1310,15 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     if (dim == D && loc == L) {                                                              \
         rc = fp32 ? LAUNCH<D, L, float>(c, fd, fl, d_err) : LAUNCH<D, L, double>(c, fd, fl, d_err); \
     } else
+    if (fl.stab) {
+        FL_DISPATCH(2, 3, launch_colour_stab)
+        FL_DISPATCH(2, 6, launch_colour_stab)
+        FL_DISPATCH(3, 4, launch_colour_stab)
+        FL_DISPATCH(3, 10, launch_colour_stab) {
+            return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: dim %d loc %d", dim, loc);
+        }
+        return rc;
+    }
     FL_DISPATCH(2, 3, launch_colour)
     FL_DISPATCH(2, 6, launch_colour_entry)
     FL_DISPATCH(3, 4, launch_colour)

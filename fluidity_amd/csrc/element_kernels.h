@@ -35,6 +35,9 @@ struct AsmFlags {
     double dt_theta, beta;
     int have_mass, lump, adv, ibp, diff, iso, src, absorp;
     int need_div, dtt_on;
+    int stab;        // 0 none, 1 streamline upwind, 2 SUPG
+    int nu_scheme;   // nu_bar: 1 optimal, 2 doubly asymptotic, 3 critical rule, 4 unity
+    double nu_scale;
 };
 
 // Field pointers for the kernels (constant fields point at one value).
@@ -690,6 +693,232 @@ FLGPU_HD_INLINE void entry_general_q(const LTab<R>& tb, const AsmFlags& fl, cons
     m_ij = M;
     s_ij = Sm;
     src_i = src;
+}
+
+// ---------------------------------------------------------------------
+// Upwind stabilisation (assemble/Upwind_Stabilisation.F90).
+
+// nu_bar_scaled_q (:243-319) at one quadrature point.  transform_to_physical
+// hands the stabilisation j = transpose(J_T) (Transform_elements.F90:
+// 783-787), so (u j)_a = sum_c u_c JT[a][c]; K is the diffusivity tensor at
+// the point, used when have_k (else xi = sign(Pe), the unity scheme).  The
+// inverse of K by cofactors over the determinant (Vector_Tools.F90:231-283).
+template <int DIM, typename R>
+FLGPU_HD_INLINE R nu_bar_scaled(const R (&u)[DIM], const R (&JT)[DIM][DIM], const R (&K)[DIM][DIM], bool have_k,
+                                int scheme, R scale) {
+    const int sch = have_k ? scheme : 4;
+    R norm_u = (R)0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) norm_u = FMA(u[d], u[d], norm_u);
+    if (norm_u < (R)1.0e-10) return (R)0;
+    R uJ[DIM];
+#pragma unroll
+    for (int a = 0; a < DIM; a++) {
+        R t = (R)0;
+#pragma unroll
+        for (int c = 0; c < DIM; c++) t = FMA(u[c], JT[a][c], t);
+        uJ[a] = t;
+    }
+    R acc = (R)0;
+    if (sch == 4) {
+#pragma unroll
+        for (int a = 0; a < DIM; a++) acc += fabs(uJ[a]);
+    } else {
+        R C[DIM][DIM], det;
+        if constexpr (DIM == 3) {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+                    // C[i][k]: inverse(K) times det, entry (i, k)
+                    C[i][k] = K[k1][i1] * K[k2][i2] - K[k2][i1] * K[k1][i2];
+                }
+            det = K[0][0] * C[0][0] + K[1][0] * C[0][1] + K[2][0] * C[0][2];
+        } else {
+            C[0][0] = K[1][1];
+            C[1][1] = K[0][0];
+            C[0][1] = -K[0][1];
+            C[1][0] = -K[1][0];
+            det = K[0][0] * K[1][1] - K[0][1] * K[1][0];
+        }
+#pragma unroll
+        for (int a = 0; a < DIM; a++) {
+            // Pe_a = 0.5 (u j K^-1)_a = 0.5 sum_b (u j)_b Kinv[b][a]
+            R t = (R)0;
+#pragma unroll
+            for (int b = 0; b < DIM; b++) t = FMA(uJ[b], C[b][a], t);
+            const R pe = (R)0.5 * (t / det);
+            R xi;
+            if (sch == 1) {  // xi_optimal (:128-160)
+                if (fabs(pe) < (R)1.0e-10) xi = (R)0;
+                else if (pe > (R)11.859499013855018) xi = (R)1 - (R)1 / pe;
+                else if (pe < -(R)11.859499013855018) xi = -(R)1 - (R)1 / pe;
+                else xi = (R)1 / tanh(pe) - (R)1 / pe;
+            } else if (sch == 2) {  // doubly asymptotic (:162-194)
+                if (fabs(pe) <= (R)3) xi = pe / (R)3;
+                else xi = pe > (R)0 ? (R)1 : -(R)1;
+            } else {  // critical rule (:196-228)
+                if (fabs(pe) <= (R)1) xi = (R)0;
+                else xi = (pe > (R)0 ? (R)1 : -(R)1) - (R)1 / pe;
+            }
+            acc = FMA(xi, uJ[a], acc);
+        }
+    }
+    return (acc / norm_u) * scale;
+}
+
+// Row i with upwind stabilisation, any degree: the per-quadrature-point
+// loop of row_general_q with (Advection_Diffusion_CG.F90:812-825) the SUPG
+// test function N_i + nu_bar u.grad N_i in the mass, advection (its
+// non-by-parts product and the divergence term), absorption and source
+// terms, the diffusion and the by-parts u.grad N_i term keeping N_i; and
+// (:1110-1121, Upwind_Stabilisation.F90:52-84) for streamline upwind the
+// term nu_bar (u.grad N_i)(u.grad N_j) added to the advection matrix.
+template <int DIM, int LOC, typename R, class QV, class TV>
+FLGPU_HD_INLINE void row_stab_q(const LTab<R>& tb, const AsmFlags& fl, const R (&JT)[DIM][DIM],
+                                const R (&iJ)[DIM][DIM], R adet, const QV& qv, const TV& T, int i, R (&vals)[LOC],
+                                R& rhs_i) {
+    const int ngi = tb.ngi;
+    const bool supg = fl.stab == 2, su = fl.stab == 1 && fl.adv;
+    R M[LOC], Sm[LOC];
+#pragma unroll
+    for (int j = 0; j < LOC; j++) {
+        M[j] = (R)0;
+        Sm[j] = (R)0;
+    }
+    R src_i = (R)0;
+#pragma unroll 1
+    for (int g = 0; g < ngi; g++) {
+        const R dw = adet * tb.w[g];
+        const R ni = tb.n[i + LOC * g];
+        R Gi[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            R s = (R)0;
+#pragma unroll
+            for (int k = 0; k < DIM; k++) s = FMA(tb.dn[i + LOC * (g + ngi * k)], iJ[d][k], s);
+            Gi[d] = s;
+        }
+        R uq[DIM];
+        qv.u(g, uq);
+        const R div = (fl.adv && fl.need_div) ? qv.div(g) : (R)0;
+        R kt[DIM][DIM];
+#pragma unroll
+        for (int a = 0; a < DIM; a++)
+#pragma unroll
+            for (int b = 0; b < DIM; b++) kt[a][b] = (R)0;
+        R kq = (R)0;
+        if (fl.diff) {
+            qv.k_tensor(g, kt);
+            if (fl.iso) kq = qv.k_iso(g);
+        }
+        const R nu = nu_bar_scaled<DIM, R>(uq, JT, kt, fl.diff != 0, fl.nu_scheme, (R)fl.nu_scale);
+        R ug_i = (R)0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) ug_i = FMA(Gi[d], uq[d], ug_i);
+        const R tn_i = supg ? FMA(nu, ug_i, ni) : ni;
+        const R sq = fl.absorp ? qv.absorp(g) : (R)0;
+        if (fl.src) src_i = FMA(tn_i, dw * qv.src(g), src_i);
+        R vK[DIM];
+        if (fl.diff && !fl.iso) {
+#pragma unroll
+            for (int b = 0; b < DIM; b++) {
+                R s = (R)0;
+#pragma unroll
+                for (int a = 0; a < DIM; a++) s = FMA(Gi[a], kt[a][b], s);
+                vK[b] = s;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < LOC; j++) {
+#ifdef __HIP_DEVICE_COMPILE__
+            if constexpr (LOC > DIM + 1) __asm__ __volatile__("" ::: "memory");
+#endif
+            const R nj = tb.n[j + LOC * g];
+            const R tnn = tn_i * nj;
+            if (fl.have_mass) M[j] = FMA(tnn, dw, M[j]);
+            R Gj[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                R s = (R)0;
+#pragma unroll
+                for (int k = 0; k < DIM; k++) s = FMA(tb.dn[j + LOC * (g + ngi * k)], iJ[d][k], s);
+                Gj[d] = s;
+            }
+            if (fl.adv) {
+                R ugj = (R)0;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) ugj = FMA(Gj[d], uq[d], ugj);
+                R a;
+                if (!fl.ibp) {
+                    a = tn_i * ugj;
+                    if (fl.need_div) a = FMA((R)fl.beta * tnn, div, a);
+                } else {
+                    a = -(ug_i * nj);
+                    if (fl.need_div) a = FMA(-((R)1 - (R)fl.beta) * tnn, div, a);
+                }
+                if (su) a = FMA(ug_i * nu, ugj, a);
+                Sm[j] = FMA(a, dw, Sm[j]);
+            }
+            if (fl.diff) {
+                R t = (R)0;
+                if (fl.iso) {
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) t = FMA(Gi[d], Gj[d], t);
+                    Sm[j] = FMA(t, dw * kq, Sm[j]);
+                } else {
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) t = FMA(vK[d], Gj[d], t);
+                    Sm[j] = FMA(t, dw, Sm[j]);
+                }
+            }
+            if (fl.absorp) Sm[j] = FMA(tnn, dw * sq, Sm[j]);
+        }
+    }
+    R r = (R)0;
+#pragma unroll
+    for (int j = 0; j < LOC; j++) r = FMA(-Sm[j], T[j], r);
+    if (fl.src) r = r + src_i;
+    rhs_i = r;
+    R lump = (R)0;
+    if (fl.have_mass && fl.lump) {
+#pragma unroll
+        for (int j = 0; j < LOC; j++) lump += M[j];
+    }
+    const R dtte = fl.dtt_on ? (R)fl.dt_theta : (R)0;
+#pragma unroll
+    for (int j = 0; j < LOC; j++) {
+        R m = (fl.have_mass && !fl.lump) ? M[j] : (R)0;
+        if (fl.have_mass && fl.lump && j == i) m = lump;
+        vals[j] = FMA(dtte, Sm[j], m);
+    }
+}
+
+// Jacobian (JT[a][b] = dx_a/dxi_b), its inverse and |det| of the linear
+// coordinate element.
+template <int DIM, typename R, class TB = LTab<R>>
+FLGPU_HD_INLINE void geometry_jt(const R (&X)[DIM][DIM + 1], const TB& tb, R (&JT)[DIM][DIM], R (&iJ)[DIM][DIM],
+                                 R& adet) {
+#pragma unroll
+    for (int a = 0; a < DIM; a++)
+#pragma unroll
+        for (int b = 0; b < DIM; b++) {
+            R s = (R)0;
+#pragma unroll
+            for (int nn = 0; nn < DIM + 1; nn++) s = FMA(X[a][nn], tb.xdn[nn + (DIM + 1) * b], s);
+            JT[a][b] = s;
+        }
+    invert_jacobian<DIM, R>(JT, iJ, adet);
+}
+
+template <int DIM, int LOC, typename R>
+FLGPU_HD_INLINE void row_stab(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, const AsmFlags& fl,
+                              const AsmFields& fd, int i, R (&vals)[LOC], R& rhs_i) {
+    R JT[DIM][DIM], iJ[DIM][DIM], adet;
+    geometry_jt<DIM, R>(e.X, tb, JT, iJ, adet);
+    const GlobalNodal<DIM, R> na{fd, e.nodes};
+    row_stab_q<DIM, LOC, R>(tb, fl, JT, iJ, adet, NodalQuad<DIM, LOC, R>{na, tb, iJ}, e.T, i, vals, rhs_i);
 }
 
 template <int DIM, int LOC, typename R>

@@ -26,6 +26,11 @@ module flgpu
   integer(c_int32_t), parameter, public :: FLGPU_NORM_PRECONDITIONED = 0, FLGPU_NORM_UNPRECONDITIONED = 1
   integer(c_int32_t), parameter, public :: FLGPU_FIELD_T = 0, FLGPU_FIELD_VELOCITY = 1, &
        FLGPU_FIELD_DIFFUSIVITY = 2, FLGPU_FIELD_SOURCE = 3, FLGPU_FIELD_ABSORPTION = 4
+  !! stabilisation (Advection_Diffusion_CG.F90:71) and nu_bar schemes (Upwind_Stabilisation.F90:18-19)
+  integer(c_int32_t), parameter, public :: FLGPU_STAB_NONE = 0, FLGPU_STAB_STREAMLINE_UPWIND = 1, &
+       FLGPU_STAB_SUPG = 2
+  integer(c_int32_t), parameter, public :: FLGPU_NU_BAR_OPTIMAL = 1, FLGPU_NU_BAR_DOUBLY_ASYMPTOTIC = 2, &
+       FLGPU_NU_BAR_CRITICAL_RULE = 3, FLGPU_NU_BAR_UNITY = 4
   !! boundary-condition types of the face loop (Advection_Diffusion_CG.F90:74-75)
   integer(c_int32_t), parameter, public :: FLGPU_BC_NONE = 0, FLGPU_BC_NEUMANN = 1, &
        FLGPU_BC_WEAKDIRICHLET = 2, FLGPU_BC_INTERNAL = 3, FLGPU_BC_ROBIN = 4
@@ -39,6 +44,8 @@ module flgpu
      integer(c_int32_t) :: have_diffusivity, isotropic_diffusivity
      integer(c_int32_t) :: have_source, have_absorption
      integer(c_int32_t) :: precision, scheme
+     integer(c_int32_t) :: stabilisation, nu_bar_scheme
+     real(c_double) :: nu_bar_scale
   end type flgpu_advdif_params
 
   !! flgpu_solver_params: the solver options of schemas/solvers.rnc

@@ -21,6 +21,8 @@ FLGPU_PC_NONE, FLGPU_PC_JACOBI, FLGPU_PC_LEARNED = 0, 1, 2
 FLGPU_NORM_PRECONDITIONED, FLGPU_NORM_UNPRECONDITIONED = 0, 1
 FIELD_T, FIELD_VELOCITY, FIELD_DIFFUSIVITY, FIELD_SOURCE, FIELD_ABSORPTION = range(5)
 BC_NONE, BC_NEUMANN, BC_WEAKDIRICHLET, BC_INTERNAL, BC_ROBIN = range(5)
+FLGPU_STAB = {"none": 0, "streamline_upwind": 1, "supg": 2}
+FLGPU_NU_BAR = {"optimal": 1, "doubly_asymptotic": 2, "critical_rule": 3, "unity": 4}
 
 # Every symbol include/flgpu.h declares (checked by the CPU tests).
 EXPORTS = (
@@ -45,6 +47,7 @@ class AdvdifParams(C.Structure):
         ("have_diffusivity", C.c_int32), ("isotropic_diffusivity", C.c_int32),
         ("have_source", C.c_int32), ("have_absorption", C.c_int32),
         ("precision", C.c_int32), ("scheme", C.c_int32),
+        ("stabilisation", C.c_int32), ("nu_bar_scheme", C.c_int32), ("nu_bar_scale", C.c_double),
     ]
 
 

@@ -144,7 +144,21 @@ typedef struct {
     int32_t precision;               /* FLGPU_FP64 or FLGPU_FP32 element
                                         arithmetic (values stored FP64) */
     int32_t scheme;                  /* FLGPU_ASM_* */
+    /* spatial_discretisation/continuous_galerkin/stabilisation
+     * (assemble/Advection_Diffusion_CG.F90:440-465): FLGPU_STAB_*, the
+     * nu_bar scheme (FLGPU_NU_BAR_*, assemble/Upwind_Stabilisation.F90:
+     * 18-19) and nu_scale.  Stabilised assembly runs on the colour scheme;
+     * it needs the velocity field. */
+    int32_t stabilisation, nu_bar_scheme;
+    double nu_bar_scale;
 } flgpu_advdif_params;
+enum { FLGPU_STAB_NONE = 0, FLGPU_STAB_STREAMLINE_UPWIND = 1, FLGPU_STAB_SUPG = 2 };
+enum {
+    FLGPU_NU_BAR_OPTIMAL = 1,
+    FLGPU_NU_BAR_DOUBLY_ASYMPTOTIC = 2,
+    FLGPU_NU_BAR_CRITICAL_RULE = 3,
+    FLGPU_NU_BAR_UNITY = 4
+};
 
 enum { FLGPU_FP64 = 0, FLGPU_FP32 = 1, FLGPU_MIXED = 1 };
 enum {

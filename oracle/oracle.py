@@ -346,7 +346,7 @@ def reference_system(cfg, nthreads: int = 0, variant: dict | None = None):
                ibp=v.get("integrate_advection_by_parts", False),
                diff=cfg.kappa is not None, src=cfg.source is not None, absorp=cfg.absorption is not None,
                stab=v.get("stabilisation", "none"), nu_bar=v.get("nu_bar_scheme", "optimal"),
-               nu_scale=v.get("nu_bar_scale", 0.5))
+               nu_scale=v.get("nu_scale", 0.5))
     A, rhs, secs = assemble(p, asm, nthreads)
     inactive = np.zeros(mesh.nnodes, np.int32)
     strong = mesh.boundary

@@ -34,7 +34,10 @@ static void run(const double* tab_n, const double* tab_dn, const double* w, cons
         for (int a = 0; a < DIM; a++) e.X[a][nn] = (R)X[a + DIM * nn];
     for (int i = 0; i < LOC; i++) {
         R vals[LOC], r;
-        row_contrib<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
+        if (fl.stab)
+            row_stab<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
+        else
+            row_contrib<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
         for (int j = 0; j < LOC; j++) mat[i + LOC * j] = (double)vals[j];
         rhs[i] = (double)r;
     }
@@ -44,7 +47,8 @@ extern "C" int elem_host(int dim, int loc, int fp32, const double* n, const doub
                          const double* xdn, const double* what, const double* mhat, const double* that, int ngi,
                          const double* X, const int* nodes, const double* T, const double* U, int u_c,
                          const double* K, int k_c, const double* S, int s_c, const double* B, int b_c,
-                         const double* fparams /* dt_theta, beta */, const int* iparams /* 10 flags */,
+                         const double* fparams /* dt_theta, beta, nu_scale */,
+                         const int* iparams /* 10 flags, stabilisation, nu_bar scheme */,
                          double* mat, double* rhs) {
     AsmFlags fl;
     fl.dt_theta = fparams[0];
@@ -59,6 +63,9 @@ extern "C" int elem_host(int dim, int loc, int fp32, const double* n, const doub
     fl.absorp = iparams[7];
     fl.need_div = iparams[8];
     fl.dtt_on = iparams[9];
+    fl.stab = iparams[10];
+    fl.nu_scheme = iparams[11];
+    fl.nu_scale = fparams[2];
     AsmFields fd{T, U, u_c, K, k_c, S, s_c, B, b_c};
 #define CASE(D, L)                                                                                     \
     if (dim == D && loc == L) {                                                                        \

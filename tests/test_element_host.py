@@ -78,8 +78,10 @@ def host_element(lib, cfg, el, xel, ele, variant, fields, fp32=False):
     need_div = adv and ((abs(1 - beta) > eps) if ibp else (abs(beta) > eps))
     iparams = np.array([cfg.have_mass, variant.get("lump_mass_matrix", False), adv, ibp,
                         fields["kappa"] is not None, True, fields["src"] is not None,
-                        fields["absorp"] is not None, need_div, abs(cfg.dt * cfg.theta) > eps], np.int32)
-    fparams = np.array([cfg.dt * cfg.theta, beta])
+                        fields["absorp"] is not None, need_div, abs(cfg.dt * cfg.theta) > eps,
+                        O.STABILISATION[variant.get("stabilisation", "none")],
+                        O.NU_BAR[variant.get("nu_bar_scheme", "optimal")]], np.int32)
+    fparams = np.array([cfg.dt * cfg.theta, beta, variant.get("nu_scale", 0.5)])
     mat = np.zeros(el.loc * el.loc)
     rhs = np.zeros(el.loc)
 
@@ -125,7 +127,9 @@ def oracle_setup(cfg, variant):
     p = O.params(cfg.dt, cfg.theta, variant.get("beta", cfg.beta), have_mass=cfg.have_mass,
                  lump=variant.get("lump_mass_matrix", False), adv=cfg.have_advection,
                  ibp=variant.get("integrate_advection_by_parts", False), diff=cfg.kappa is not None,
-                 src=cfg.source is not None, absorp=cfg.absorption is not None)
+                 src=cfg.source is not None, absorp=cfg.absorption is not None,
+                 stab=variant.get("stabilisation", "none"), nu_bar=variant.get("nu_bar_scheme", "optimal"),
+                 nu_scale=variant.get("nu_scale", 0.5))
     return el, xel, fields, asm, p
 
 
@@ -133,7 +137,15 @@ CASES = [("c1", 6, {}), ("c2", 3, {}), ("c3", 2, {}), ("c4", 3, {}), ("c5", 3, {
          ("c1", 6, {"lump_mass_matrix": True}), ("c1", 6, {"integrate_advection_by_parts": True}),
          ("c1", 6, {"beta": 1.0}), ("c1", 6, {"integrate_advection_by_parts": True, "beta": 0.3}),
          ("c3", 2, {"beta": 0.7}), ("c3", 2, {"integrate_advection_by_parts": True}), ("c1p2", 4, {}),
-         ("c1p2", 4, {"lump_mass_matrix": True})]
+         ("c1p2", 4, {"lump_mass_matrix": True}),
+         # upwind stabilisation (row_stab: the general quadrature path)
+         ("c1", 6, {"stabilisation": "supg"}), ("c1", 6, {"stabilisation": "streamline_upwind"}),
+         ("c1", 6, {"stabilisation": "supg", "integrate_advection_by_parts": True, "beta": 0.3}),
+         ("c1", 6, {"stabilisation": "supg", "nu_bar_scheme": "critical_rule", "lump_mass_matrix": True}),
+         ("c1", 6, {"stabilisation": "streamline_upwind", "nu_bar_scheme": "unity", "nu_scale": 1.0}),
+         ("c1p2", 4, {"stabilisation": "supg", "beta": 0.4}),
+         ("c3", 2, {"stabilisation": "supg", "nu_bar_scheme": "doubly_asymptotic"}),
+         ("c3", 2, {"stabilisation": "streamline_upwind"})]
 
 
 @pytest.mark.parametrize("name,scale,variant", CASES)
