@@ -33,7 +33,7 @@ EXTRA = {
     "assemble.hip": ["-ffp-contract=off"],
     "surface.hip": ["-ffp-contract=off"],
 }
-SOURCES = ["flgpu_context.hip", "assemble.hip", "surface.hip", "krylov.hip", "learned.hip"]
+SOURCES = ["flgpu_context.hip", "assemble.hip", "surface.hip", "krylov.hip", "learned.hip", "matrixdump.hip"]
 
 
 def _compile(src: str) -> str:

@@ -32,7 +32,8 @@ EXPORTS = (
     "flgpu_advdif_assemble_resident", "flgpu_advdif_assemble", "flgpu_surface_bind",
     "flgpu_advdif_surface", "flgpu_dirichlet_apply",
     "flgpu_matrix_get", "flgpu_matrix_set", "flgpu_solve", "flgpu_petsc_solve",
-    "flgpu_solution_get", "flgpu_csr_mult", "flgpu_learned_param_count",
+    "flgpu_solution_get", "flgpu_csr_mult", "flgpu_matrixdump_read", "flgpu_matrixdump_write",
+    "flgpu_learned_param_count",
     "flgpu_learned_set_params", "flgpu_learned_get_params", "flgpu_learned_init_params",
     "flgpu_learned_train_step", "flgpu_learned_loss_grad", "flgpu_learned_factor",
     "flgpu_time_assemble", "flgpu_time_spmv", "flgpu_solver_prepare", "flgpu_sizes",
@@ -111,6 +112,9 @@ def load() -> C.CDLL:
                               C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_double)],
         "flgpu_solution_get": [P, P],
         "flgpu_csr_mult": [P, P, P],
+        "flgpu_matrixdump_read": [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64), P, P, P, P, P,
+                                  C.POINTER(C.c_int32)],
+        "flgpu_matrixdump_write": [C.c_char_p, I32, P, P, P, P, P],
         "flgpu_learned_param_count": [],
         "flgpu_learned_set_params": [P, P],
         "flgpu_learned_get_params": [P, P],

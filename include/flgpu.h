@@ -308,6 +308,25 @@ int flgpu_solution_get(flgpu_ctx *ctx, double *x);
  * femtools/Sparse_Tools.F90:3822-3844); host arrays. */
 int flgpu_csr_mult(flgpu_ctx *ctx, const double *x, double *y);
 
+/* ---- matrix dumps -------------------------------------------------------
+ * The PETSc binary file DumpMatrixEquation writes after a failed solve
+ * (femtools/Petsc_Tools.F90:1487-1504, called from ConvergenceCheck,
+ * femtools/Solvers.F90:1369-1479): MatView(A), VecView(b), VecView(x0),
+ * big-endian, 32-bit indices.  Host-side; the arrays are 1-based CSR as
+ * flgpu_sparsity_bind / flgpu_matrix_set take them, so a dump replays
+ * through flgpu_solve (the GPU form of tools/petsc_readnsolve.F90). */
+/* Read a dump.  With findrm == NULL only *n and *nnz are returned (size
+ * query); then findrm(n+1), colm(nnz), val(nnz), b(n) and, if non-NULL,
+ * x0(n) are filled; *has_x0 (if non-NULL) says whether the file held x0.
+ * Square matrices with ascending columns per row only. */
+int flgpu_matrixdump_read(const char *filename, int32_t *n, int64_t *nnz,
+                          int32_t *findrm, int32_t *colm, double *val,
+                          double *b, double *x0, int32_t *has_x0);
+/* Write a dump of a CSR system (x0 may be NULL: then only A and b). */
+int flgpu_matrixdump_write(const char *filename, int32_t n,
+                           const int32_t *findrm, const int32_t *colm,
+                           const double *val, const double *b,
+                           const double *x0);
 /* ---- learned preconditioner --------------------------------------------
  * A message-passing network on the matrix graph (3 layers, hidden 32,
  * BF16 MFMA with FP32 accumulation) predicts a lower-triangular factor G on
