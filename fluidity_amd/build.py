@@ -33,7 +33,8 @@ EXTRA = {
     "assemble.hip": ["-ffp-contract=off"],
     "surface.hip": ["-ffp-contract=off"],
 }
-SOURCES = ["flgpu_context.hip", "assemble.hip", "surface.hip", "krylov.hip", "learned.hip", "matrixdump.hip"]
+SOURCES = ["flgpu_context.hip", "assemble.hip", "surface.hip", "krylov.hip", "learned.hip", "matrixdump.hip",
+           "setup_gpu.hip"]
 
 
 def _compile(src: str) -> str:

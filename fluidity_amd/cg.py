@@ -140,21 +140,56 @@ class CGProblem:
 
     def __init__(self, mesh: Mesh, element: Element | None = None, quad_degree: int = 2,
                  device: int = 0, sparsity: Sparsity | None = None,
-                 colouring: tuple | None = None):
+                 colouring: tuple | None = None, setup: str = "host"):
         self.lib = L.load()
         self.ctx = C.c_void_p()
         L.check(self.lib.flgpu_init(C.byref(self.ctx), device), "flgpu_init")
-        self.bind(mesh, element, quad_degree, sparsity, colouring)
+        self.bind(mesh, element, quad_degree, sparsity, colouring, setup)
 
     def bind(self, mesh: Mesh, element: Element | None = None, quad_degree: int = 2,
-             sparsity: Sparsity | None = None, colouring: tuple | None = None) -> None:
+             sparsity: Sparsity | None = None, colouring: tuple | None = None, setup: str = "host") -> None:
         """(Re)bind a mesh to this context; the learned network state of the
-        context is kept (used to train over many meshes)."""
+        context is kept (used to train over many meshes).  setup="device"
+        builds the sparsity and an element colouring on the GPU
+        (flgpu_sparsity_build, flgpu_colouring_build) instead of taking the
+        host ones."""
         self.mesh = mesh
         self.element = element or make_element(mesh.dim, mesh.degree, quad_degree)
         xel = make_element(mesh.dim, 1, self.element.quad_degree)
+        if setup == "device":
+            self._bind_meshes(mesh, xel)
+            nnz = C.c_int64(0)
+            L.check(self.lib.flgpu_sparsity_build(self.ctx, C.byref(nnz)), "flgpu_sparsity_build")
+            f = np.zeros(mesh.nnodes + 1, np.int32)
+            cm = np.zeros(nnz.value, np.int32)
+            cen = np.zeros(mesh.nnodes, np.int32)
+            L.check(self.lib.flgpu_sparsity_get(self.ctx, L.ptr(f), L.ptr(cm), L.ptr(cen)), "flgpu_sparsity_get")
+            self.sparsity = Sparsity(f, cm, cen)
+            ncol = C.c_int32(0)
+            L.check(self.lib.flgpu_colouring_build(self.ctx, C.byref(ncol)), "flgpu_colouring_build")
+            cp = np.zeros(ncol.value + 1, np.int32)
+            el = np.zeros(mesh.nele, np.int32)
+            L.check(self.lib.flgpu_colouring_get(self.ctx, L.ptr(cp), L.ptr(el)), "flgpu_colouring_get")
+            colour = np.zeros(mesh.nele, np.int32)
+            for k in range(ncol.value):
+                colour[el[cp[k] - 1:cp[k + 1] - 1] - 1] = k + 1
+            self.colouring = (colour, cp, el)
+            self.ncolours = ncol.value
+            self.T = None
+            return
         self.sparsity = sparsity or make_sparsity(mesh)
         self.colouring = colouring or get_mesh_colouring(mesh)
+        self._bind_meshes(mesh, xel)
+        s = self.sparsity
+        L.check(self.lib.flgpu_sparsity_bind(self.ctx, mesh.nnodes, mesh.nnodes, L.ptr(s.findrm),
+                                             L.ptr(s.colm), L.ptr(s.centrm)), "flgpu_sparsity_bind")
+        _, col_ptr, elems = self.colouring
+        self.ncolours = col_ptr.size - 1
+        L.check(self.lib.flgpu_colouring_bind(self.ctx, self.ncolours, L.ptr(col_ptr), L.ptr(elems)),
+                "flgpu_colouring_bind")
+        self.T = None
+
+    def _bind_meshes(self, mesh: Mesh, xel: Element) -> None:
         e = self.element
         X = np.ascontiguousarray(mesh.X, dtype=np.float64)
         CGProblem._mesh_key += 1
@@ -164,14 +199,6 @@ class CGProblem:
                                                L.ptr(e.fortran(e.n)), L.ptr(e.fortran(e.dn)),
                                                L.ptr(np.ascontiguousarray(e.w)), L.ptr(xel.fortran(xel.dn))),
                 "flgpu_field_mesh_bind")
-        s = self.sparsity
-        L.check(self.lib.flgpu_sparsity_bind(self.ctx, mesh.nnodes, mesh.nnodes, L.ptr(s.findrm),
-                                             L.ptr(s.colm), L.ptr(s.centrm)), "flgpu_sparsity_bind")
-        _, col_ptr, elems = self.colouring
-        self.ncolours = col_ptr.size - 1
-        L.check(self.lib.flgpu_colouring_bind(self.ctx, self.ncolours, L.ptr(col_ptr), L.ptr(elems)),
-                "flgpu_colouring_bind")
-        self.T = None
 
     def bind_surface(self, surface: Surface) -> None:
         """Surface elements for the face loop: face_global_nodes of T and of

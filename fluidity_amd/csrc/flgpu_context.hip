@@ -261,17 +261,28 @@ int flgpu_colouring_bind(flgpu_ctx* c, int32_t ncol, const int32_t* col_ptr, con
             }
         }
     }
+    return colouring_adopt(c, cp, el);
+}
+
+}  // extern "C"
+
+namespace flgpu {
+// Take a checked colouring (0-based set offsets and element ids).
+int colouring_adopt(flgpu_ctx* c, std::vector<int32_t>& cp, std::vector<int32_t>& el) {
     FLGPU_HIP(hipSetDevice(c->device));
     FLGPU_TRY(c->col_elems.upload(el.data(), el.size(), c->stream));
     FLGPU_HIP(hipStreamSynchronize(c->stream));
+    c->ncol = (int32_t)cp.size() - 1;
     c->h_col_ptr.swap(cp);
     c->h_col_elems.swap(el);
-    c->ncol = ncol;
     c->plan.built = false;
     c->plan.epos_built = false;
     c->plan.fail_rc = 0;
     return FLGPU_OK;
 }
+}  // namespace flgpu
+
+extern "C" {
 
 int flgpu_make_sparsity(int32_t nnodes, int32_t nele, int32_t loc, const int32_t* ndglno,
                         int32_t* findrm, int32_t* colm, int32_t* centrm, int64_t* nnz_out) {

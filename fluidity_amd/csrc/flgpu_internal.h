@@ -230,4 +230,6 @@ int solution_get(flgpu_ctx* c, double* x);
 void learned_free(flgpu_ctx* c);
 // shared helpers
 unsigned grid_for(int64_t n, int block);
+// flgpu_context.hip: bind a colouring already known to be valid (0-based)
+int colouring_adopt(flgpu_ctx* c, std::vector<int32_t>& col_ptr, std::vector<int32_t>& elems);
 }  // namespace flgpu

@@ -28,7 +28,8 @@ FLGPU_NU_BAR = {"optimal": 1, "doubly_asymptotic": 2, "critical_rule": 3, "unity
 EXPORTS = (
     "flgpu_init", "flgpu_finalize", "flgpu_last_error", "flgpu_mesh_bind",
     "flgpu_field_mesh_bind", "flgpu_sparsity_bind", "flgpu_colouring_bind",
-    "flgpu_make_sparsity", "flgpu_colour_mesh", "flgpu_field_set",
+    "flgpu_make_sparsity", "flgpu_colour_mesh", "flgpu_sparsity_build", "flgpu_sparsity_get",
+    "flgpu_colouring_build", "flgpu_colouring_get", "flgpu_field_set",
     "flgpu_advdif_assemble_resident", "flgpu_advdif_assemble", "flgpu_surface_bind",
     "flgpu_advdif_surface", "flgpu_dirichlet_apply",
     "flgpu_matrix_get", "flgpu_matrix_set", "flgpu_solve", "flgpu_petsc_solve",
@@ -112,6 +113,10 @@ def load() -> C.CDLL:
                               C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_double)],
         "flgpu_solution_get": [P, P],
         "flgpu_csr_mult": [P, P, P],
+        "flgpu_sparsity_build": [P, C.POINTER(C.c_int64)],
+        "flgpu_sparsity_get": [P, P, P, P],
+        "flgpu_colouring_build": [P, C.POINTER(C.c_int32)],
+        "flgpu_colouring_get": [P, P, P],
         "flgpu_matrixdump_read": [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64), P, P, P, P, P,
                                   C.POINTER(C.c_int32)],
         "flgpu_matrixdump_write": [C.c_char_p, I32, P, P, P, P, P],

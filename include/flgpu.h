@@ -116,6 +116,26 @@ int flgpu_make_sparsity(int32_t nnodes, int32_t nele, int32_t loc,
 int flgpu_colour_mesh(int32_t nnodes, int32_t nele, int32_t loc,
                       const int32_t *ndglno, int32_t *colour, int32_t *ncol);
 
+/* ---- set-up on the device (meshes that change) ------------------------ */
+/* make_sparsity of the bound field mesh computed on the GPU (the arrays of
+ * flgpu_make_sparsity, bit for bit) and bound as by flgpu_sparsity_bind;
+ * *nnz (if non-NULL) gets the entry count.  No counterpart call in Fluidity:
+ * it replaces the CPU make_sparsity (femtools/Sparsity_Patterns.F90:47-85,
+ * :299-428) that every adapt repeats. */
+int flgpu_sparsity_build(flgpu_ctx *ctx, int64_t *nnz);
+/* Copy the bound sparsity out: findrm(rows+1), colm(nnz), centrm(rows),
+ * 1-based (any may be NULL). */
+int flgpu_sparsity_get(flgpu_ctx *ctx, int32_t *findrm, int32_t *colm,
+                       int32_t *centrm);
+/* An element colouring computed on the GPU and bound as by
+ * flgpu_colouring_bind (elements sharing a node differ; speculative
+ * first-fit rounds with hashed priorities, deterministic).  It is not the greedy
+ * index-order colouring of get_mesh_colouring (femtools/Colouring.F90:
+ * 85-199), so matrix values equal the reference's up to the order of the
+ * colour sums; bind the reference colouring where that order matters. */
+int flgpu_colouring_build(flgpu_ctx *ctx, int32_t *ncol);
+/* Copy the bound colour sets out: col_ptr(ncol+1), elems(nele), 1-based. */
+int flgpu_colouring_get(flgpu_ctx *ctx, int32_t *col_ptr, int32_t *elems);
 /* ---- fields ------------------------------------------------------- */
 
 enum {
