@@ -61,6 +61,7 @@ module flgpu
 
   public :: flgpu_init, flgpu_finalize, flgpu_mesh_bind, flgpu_field_mesh_bind, &
        flgpu_sparsity_bind, flgpu_colouring_bind, flgpu_make_sparsity, flgpu_colour_mesh, &
+       flgpu_sparsity_build, flgpu_sparsity_get, flgpu_colouring_build, flgpu_colouring_get, &
        flgpu_field_set, flgpu_advdif_assemble, flgpu_surface_bind, flgpu_advdif_surface, &
        flgpu_dirichlet_apply, flgpu_matrix_get, &
        flgpu_petsc_solve, flgpu_check
@@ -125,6 +126,31 @@ module flgpu
        type(c_ptr), value :: colm
        integer(c_int64_t), intent(out) :: nnz
      end function flgpu_make_sparsity
+
+     !! make_sparsity and an element colouring of the bound field mesh, on the GPU
+     integer(c_int) function flgpu_sparsity_build(ctx, nnz) bind(c, name="flgpu_sparsity_build")
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), intent(out) :: nnz
+     end function flgpu_sparsity_build
+
+     integer(c_int) function flgpu_sparsity_get(ctx, findrm, colm, centrm) bind(c, name="flgpu_sparsity_get")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: findrm(*), colm(*), centrm(*)
+     end function flgpu_sparsity_get
+
+     integer(c_int) function flgpu_colouring_build(ctx, ncol) bind(c, name="flgpu_colouring_build")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: ncol
+     end function flgpu_colouring_build
+
+     integer(c_int) function flgpu_colouring_get(ctx, col_ptr, elems) bind(c, name="flgpu_colouring_get")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: col_ptr(*), elems(*)
+     end function flgpu_colouring_get
 
      integer(c_int) function flgpu_colour_mesh(nnodes, nele, loc, ndglno, colour, ncol) &
           bind(c, name="flgpu_colour_mesh")

@@ -4,7 +4,12 @@
 !! arrays over (1-based ndglno, positions%val(dim,nodes), element_type
 !! tables, csr_sparsity, colour sets).
 !!
-!! usage: flgpu_driver [n]   (n x n cells, two P1 triangles per cell)
+!! usage: flgpu_driver [n [faces]]   (n x n cells, two P1 triangles per cell)
+!! With "faces": sparsity and colouring built on the GPU
+!! (flgpu_sparsity_build, flgpu_colouring_build) and the face loop of
+!! assemble_advection_diffusion_cg (:606-638) on the x sides -- Neumann flux
+!! 1 + y/2 on x = 0, Robin g = 0.5, h = 2 + y on x = 1 -- with strong
+!! Dirichlet 0 on the y sides only.
 !! prints: nodes nnz colours its reason relres sum(T) max(T)
 program flgpu_driver
   use iso_c_binding
@@ -20,11 +25,21 @@ program flgpu_driver
   type(flgpu_advdif_params) :: p
   type(flgpu_solver_params) :: sp
   character(len=32) :: arg
+  logical :: faces
+  integer(c_int32_t), allocatable :: fnodes(:), fele(:), ftype(:)
+  real(c_double), allocatable, target :: fv1(:), fv2(:)
+  real(c_double) :: fn(2,2), fw(2), fxdn(2,2), gq(2)
+  integer(c_int32_t) :: nf
 
   n = 16
   if (command_argument_count() >= 1) then
      call get_command_argument(1, arg)
      read(arg, *) n
+  end if
+  faces = .false.
+  if (command_argument_count() >= 2) then
+     call get_command_argument(2, arg)
+     faces = trim(arg) == "faces"
   end if
   m = n + 1
   nn = m * m
@@ -99,8 +114,17 @@ program flgpu_driver
   call flgpu_check(flgpu_init(ctx, 0_c_int32_t), "init")
   call flgpu_check(flgpu_mesh_bind(ctx, 1_c_int64_t, 0_c_int64_t, 2, 3, ne, nn, ndglno, X), "mesh_bind")
   call flgpu_check(flgpu_field_mesh_bind(ctx, 3, 3, nn, ndglno, shn, shdn, w, shdn), "field_mesh_bind")
-  call flgpu_check(flgpu_sparsity_bind(ctx, nn, nn, findrm, colm, centrm), "sparsity_bind")
-  call flgpu_check(flgpu_colouring_bind(ctx, ncol, col_ptr, elems), "colouring_bind")
+  if (faces) then
+     ! make_sparsity and a colouring on the GPU instead of the host ones
+     call flgpu_check(flgpu_sparsity_build(ctx, nnz), "sparsity_build")
+     deallocate(colm)
+     allocate(colm(nnz))
+     call flgpu_check(flgpu_sparsity_get(ctx, findrm, colm, centrm), "sparsity_get")
+     call flgpu_check(flgpu_colouring_build(ctx, ncol), "colouring_build")
+  else
+     call flgpu_check(flgpu_sparsity_bind(ctx, nn, nn, findrm, colm, centrm), "sparsity_bind")
+     call flgpu_check(flgpu_colouring_bind(ctx, ncol, col_ptr, elems), "colouring_bind")
+  end if
 
   p = flgpu_advdif_params(dt=0.01d0, theta=0.5d0, beta=0.0d0, have_mass=1, lump_mass=0, &
        have_advection=1, integrate_by_parts=0, have_diffusivity=1, isotropic_diffusivity=1, &
@@ -109,13 +133,47 @@ program flgpu_driver
   call flgpu_check(flgpu_advdif_assemble(ctx, p, c_loc(T), c_loc(U), c_loc(kap), c_null_ptr, c_null_ptr, &
        c_null_ptr, c_null_ptr), "advdif_assemble")
 
-  ! strong Dirichlet 0 on the boundary, rate-of-change form
-  nbc = 4 * n
+  if (faces) then
+     ! surface elements on x = 0 and x = 1: face_global_nodes in the order of
+     ! the element's boundary numbering (the edge off the element's vertex
+     ! 2 for x = 0, off vertex 3 for x = 1 ... here simply bottom to top),
+     ! the 2-point Gauss rule of degree 2 (Quadrature.F90:1166-1180)
+     nf = 2 * n
+     allocate(fnodes(2 * nf), fele(nf), ftype(nf), fv1(2 * nf), fv2(2 * nf))
+     do j = 0, n - 1
+        ! x = 0: element 2 of cell (0, j) = (k, k+m+1, k+m); its edge off vertex 2 is (k+m, k)
+        k = m * j + 1
+        fnodes(2 * j + 1:2 * j + 2) = [k + m, k]
+        fele(j + 1) = 2 * (n * j) + 2
+        ftype(j + 1) = FLGPU_BC_NEUMANN
+        fv1(2 * j + 1:2 * j + 2) = [1.0d0 + 0.5d0 * X(2, k + m), 1.0d0 + 0.5d0 * X(2, k)]
+        fv2(2 * j + 1:2 * j + 2) = 0.0d0
+        ! x = 1: element 1 of cell (n-1, j) = (k', k'+1, k'+m+1); edge off vertex 1 is (k'+1, k'+m+1)
+        k = n - 1 + m * j + 1
+        fnodes(2 * (n + j) + 1:2 * (n + j) + 2) = [k + 1, k + m + 1]
+        fele(n + j + 1) = 2 * (n - 1 + n * j) + 1
+        ftype(n + j + 1) = FLGPU_BC_ROBIN
+        fv1(2 * (n + j) + 1:2 * (n + j) + 2) = 0.5d0
+        fv2(2 * (n + j) + 1:2 * (n + j) + 2) = [2.0d0 + X(2, k + 1), 2.0d0 + X(2, k + m + 1)]
+     end do
+     gq = [0.788675134594813d0, 1.0d0 - 0.788675134594813d0]
+     fn(1, :) = gq
+     fn(2, :) = 1.0d0 - gq
+     fw = 0.5d0
+     fxdn(1, :) = 1.0d0
+     fxdn(2, :) = -1.0d0
+     call flgpu_check(flgpu_surface_bind(ctx, nf, 2, 2, fnodes, fnodes, fele, fn, fw, fxdn), "surface_bind")
+     call flgpu_check(flgpu_advdif_surface(ctx, p, ftype, c_loc(fv1), c_loc(fv2)), "advdif_surface")
+     nbc = 2 * (n + 1)
+  else
+     nbc = 4 * n
+  end if
+  ! strong Dirichlet 0 on the boundary (the y sides with faces), rate-of-change form
   allocate(bnodes(nbc), bvals(nbc))
   k = 0
   do j = 0, n
      do i = 0, n
-        if (i == 0 .or. j == 0 .or. i == n .or. j == n) then
+        if (j == 0 .or. j == n .or. (.not. faces .and. (i == 0 .or. i == n))) then
            k = k + 1
            bnodes(k) = i + m * j + 1
         end if

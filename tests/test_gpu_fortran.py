@@ -38,3 +38,35 @@ def test_fortran_driver_matches_python_path(require_gpu):
     assert abs(tsum - T1.sum()) <= 1e-12 * abs(T1.sum())
     assert abs(tmax - T1.max()) <= 1e-12 * abs(T1.max())
     pr.close()
+
+
+def test_fortran_driver_face_loop_and_device_setup(require_gpu):
+    """The same step with the sparsity and colouring built on the GPU and
+    the face loop (Neumann on x = 0, Robin on x = 1) bound from Fortran."""
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built (no flang at build time)")
+    n = 16
+    print(f"\n[case] fortran driver n={n} faces", flush=True)
+    out = subprocess.run([DRIVER, str(n), "faces"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    f = out.stdout.split()
+    nn, nnz, ncol, its, reason = (int(v) for v in f[:5])
+    tsum, tmax = (float(v) for v in f[6:8])
+    mesh = M.square_tris(n, jitter=0.0)
+    p = mesh.node_pos
+    T0 = np.exp(-((p[:, 0] - 0.3) ** 2 + (p[:, 1] - 0.5) ** 2) / (2 * 0.05 ** 2))
+    U = np.stack([-(p[:, 1] - 0.5), p[:, 0] - 0.5], axis=1)
+    cfg = M.Config("c1-nojitter", mesh, 2, T0, U, 1e-3, None, None, 0.01, 0.5, 0.0, True, True, "gmres", 1e-10)
+    pr = cg.CGProblem(mesh, quad_degree=2, setup="device")
+    pr.set_fields(T=T0, velocity=U, diffusivity=1e-3)
+    surf = M.surface_of(mesh)
+    pr.bind_surface(surf)
+    bt, b1, b2, strong = M.unit_box_bcs(mesh, surf, "neumann_robin")
+    so = cg.SolverOptions("gmres", "jacobi", relative_error=1e-10, restart=30)
+    T1, pits, info = cg.solve_field_equation_cg(pr, cg.options_from_config(cfg), so, cfg.dt, strong, 0.0,
+                                                bcs=cg.SurfaceBCs(bt, b1, b2))
+    assert (nn, nnz, ncol) == (mesh.nnodes, pr.sparsity.nnz, pr.ncolours)
+    assert reason > 0 and info.reason > 0 and abs(its - pits) <= 1
+    assert abs(tsum - T1.sum()) <= 1e-9 * abs(T1.sum())
+    assert abs(tmax - T1.max()) <= 1e-9 * abs(T1.max())
+    pr.close()
