@@ -154,6 +154,68 @@ def c4_heldout_report(weights, log_each) -> dict:
     return out
 
 
+def next_rows_report(cfg, pr, log_each) -> dict:
+    """Timings of the rows beyond the element assembly and the solve, on the
+    same mesh: the face loop (Neumann on x = 0, Robin on x = 1), the
+    SUPG-stabilised assembly (a constant velocity field is set for it), and
+    the sparsity and colouring built on the GPU against the host routines.
+    Leaves the context with the GPU-built sparsity and colouring bound."""
+    import ctypes as C
+    from fluidity_amd import cg, lib as L, meshgen as M
+    lib = L.load()
+    rep = {}
+    t0 = time.perf_counter()
+    surf = M.surface_of(cfg.mesh)
+    bt, b1, b2, _ = M.unit_box_bcs(cfg.mesh, surf, "neumann_robin")
+    pr.bind_surface(surf)
+    log_each(f"surface: {surf.nfaces} faces ({time.perf_counter() - t0:.1f}s host extraction)")
+    opts = cg.options_from_config(cfg)
+    p = opts.params(cfg.dt)
+    bcs = cg.SurfaceBCs(bt, b1, b2)
+    cg.assemble_advection_diffusion_cg(pr, opts, cfg.dt, bcs)  # builds the face lists once
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.check(lib.flgpu_advdif_surface(pr.ctx, C.byref(p), L.ptr(np.ascontiguousarray(bt, np.int32)),
+                                         L.ptr(b1), L.ptr(b2)), "surface")
+    face_ms = (time.perf_counter() - t0) / reps * 1e3
+    rep["face_loop"] = {"faces": surf.nfaces, "ms_per_call": round(face_ms, 4),
+                        "note": "wall time of flgpu_advdif_surface incl. the upload of the bc values"}
+    log_each(f"face loop: {face_ms:.3f} ms per call")
+    # SUPG: constant velocity on the same mesh, the stabilised row kernel
+    pr.set_fields(velocity=np.array([1.0, 0.5, 0.25]))
+    ms = C.c_double(0)
+    for stab in ("supg", "streamline_upwind"):
+        so = cg.options_from_config(cfg)
+        so.exclude_advection_terms = False
+        so.stabilisation = stab
+        ps = so.params(cfg.dt)
+        L.check(lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(ps)), "assemble " + stab)
+        L.check(lib.flgpu_time_assemble(pr.ctx, C.byref(ps), 5, C.byref(ms)), "time " + stab)
+        rep[f"assembly_{stab}"] = {"ms": round(ms.value, 4), "Melem/s": round(cfg.mesh.nele / ms.value / 1e3, 2)}
+        log_each(f"assembly with {stab}: {ms.value:.3f} ms ({cfg.mesh.nele / ms.value / 1e3:.1f} Melem/s)")
+    # set-up: host routines, then the GPU ones (bound afterwards)
+    t0 = time.perf_counter()
+    cg.make_sparsity(cfg.mesh)
+    t1 = time.perf_counter()
+    cg.get_mesh_colouring(cfg.mesh)
+    t2 = time.perf_counter()
+    nnz = C.c_int64(0)
+    ncol = C.c_int32(0)
+    L.check(lib.flgpu_sparsity_build(pr.ctx, C.byref(nnz)), "sparsity_build")  # warm
+    t3 = time.perf_counter()
+    L.check(lib.flgpu_sparsity_build(pr.ctx, C.byref(nnz)), "sparsity_build")
+    t4 = time.perf_counter()
+    L.check(lib.flgpu_colouring_build(pr.ctx, C.byref(ncol)), "colouring_build")
+    t5 = time.perf_counter()
+    rep["setup"] = {"sparsity_gpu_s": round(t4 - t3, 4), "sparsity_host_s": round(t1 - t0, 4),
+                    "colouring_gpu_s": round(t5 - t4, 4), "colouring_host_s": round(t2 - t1, 4),
+                    "colours_gpu": ncol.value, "nnz": nnz.value}
+    log_each(f"set-up: sparsity {t4 - t3:.3f}s GPU / {t1 - t0:.3f}s host, colouring {t5 - t4:.3f}s GPU "
+             f"({ncol.value} colours) / {t2 - t1:.3f}s host")
+    return rep
+
+
 def cpu_baseline(cfg, budget_s: float = 20.0) -> dict:
     """The oracle (C restatement, OpenMP over the colour loop like the
     reference) timed on this host: the assembly of the whole mesh and a
@@ -188,6 +250,7 @@ def main() -> int:
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--scheme", default="auto")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-next", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (sharding the path "
@@ -300,6 +363,8 @@ def main() -> int:
             log(f"spmv {prec}: {ms.value:.4f} ms, {gbs:.1f} GB/s")
         out["solve"] = solves
 
+    if not args.no_next:
+        out["next_rows"] = next_rows_report(cfg, pr, log)
     if not args.no_c3:
         out["c3_mixed_precision"] = c3_mixed_report(log)
         if not args.no_solve:
