@@ -175,26 +175,72 @@ static int table_image(flgpu_ctx* c, const uint32_t** out) {
 }
 
 // Stabilised assembly (streamline upwind / SUPG), any degree: one launch
-// per colour, thread = (element of the colour, local row i), the row by the
-// per-quadrature-point loop with the stabilised test function (row_stab),
-// CSR positions from the per-mesh table.
+// per colour, EPB elements per workgroup, thread = (element, local row i).
+// The velocity and nu_bar at the quadrature points are computed once per
+// element (the threads of an element share the points) into LDS; then each
+// thread forms its row by the per-quadrature-point loop with the stabilised
+// test function (row_stab_q) and adds it at the CSR positions of the
+// per-mesh table.  The values are those of row_stab, bit for bit.
+template <int DIM, int LOC, typename R>
+struct StabLdsQuad : NodalQuad<DIM, LOC, R> {
+    const R* su;   // u at the points, [g][d]
+    const R* snu;  // nu_bar at the points
+    FLGPU_HD_INLINE void u(int g, R (&uq)[DIM]) const {
+#pragma unroll
+        for (int d = 0; d < DIM; d++) uq[d] = su[g * DIM + d];
+    }
+    FLGPU_HD_INLINE R nu(int g, const R (&)[DIM], const R (&)[DIM][DIM], const R (&)[DIM][DIM],
+                         const AsmFlags&) const {
+        return snu[g];
+    }
+};
+
+template <int LOC>
+constexpr int stab_epb() { return 256 / LOC; }
+
 template <int DIM, int LOC, typename R>
 __global__ void __launch_bounds__(256)
 k_asm_colour_stab(const uint32_t* __restrict__ ltab, const int32_t* __restrict__ elems, int count,
                   const int32_t* __restrict__ conn, const int32_t* __restrict__ conn_x,
                   const double* __restrict__ X4, AsmFields fd, AsmFlags fl, const int32_t* __restrict__ epos,
                   double* __restrict__ A, double* __restrict__ rhs, int* __restrict__ err) {
+    constexpr int EPB = stab_epb<LOC>(), NT = EPB * LOC;
     __shared__ LTab<R> tb;
-    stage_table_image<R, 256>(ltab, tb);
+    __shared__ R s_u[EPB][kElemMaxGi * DIM];
+    __shared__ R s_nu[EPB][kElemMaxGi];
+    stage_table_image<R, NT>(ltab, tb);
     __syncthreads();
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)count * LOC) return;
-    const int q = (int)(t / LOC), i = (int)(t % LOC);
-    const int ele = elems[q];
+    const int ql = threadIdx.x / LOC, i = threadIdx.x % LOC;
+    const int64_t q = (int64_t)blockIdx.x * EPB + ql;
+    const bool valid = q < count;
     Elem<DIM, LOC, R> e;
-    gather_elem<DIM, LOC, R>(ele, conn, conn_x, X4, fd.T, e);
+    R JT[DIM][DIM], iJ[DIM][DIM], adet = (R)0;
+    int ele = 0;
+    if (valid) {
+        ele = elems[q];
+        gather_elem<DIM, LOC, R>(ele, conn, conn_x, X4, fd.T, e);
+        geometry_jt<DIM, R>(e.X, tb, JT, iJ, adet);
+        const GlobalNodal<DIM, R> na{fd, e.nodes};
+        const NodalQuad<DIM, LOC, R> nq{na, tb, iJ};
+        for (int g = i; g < tb.ngi; g += LOC) {
+            R uq[DIM], kt[DIM][DIM];
+            nq.u(g, uq);
+#pragma unroll
+            for (int a = 0; a < DIM; a++)
+#pragma unroll
+                for (int b = 0; b < DIM; b++) kt[a][b] = (R)0;
+            if (fl.diff) nq.k_tensor(g, kt);
+            s_nu[ql][g] = nq.nu(g, uq, JT, kt, fl);
+#pragma unroll
+            for (int d = 0; d < DIM; d++) s_u[ql][g * DIM + d] = uq[d];
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+    const GlobalNodal<DIM, R> na{fd, e.nodes};
+    const StabLdsQuad<DIM, LOC, R> qv{{na, tb, iJ}, &s_u[ql][0], &s_nu[ql][0]};
     R vals[LOC], r;
-    row_stab<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
+    row_stab_q<DIM, LOC, R>(tb, fl, JT, iJ, adet, qv, e.T, i, vals, r);
     const int row = conn[(int64_t)ele * LOC + i];
 #pragma unroll
     for (int j = 0; j < LOC; j++) {
@@ -218,7 +264,8 @@ static int launch_colour_stab(flgpu_ctx* c, const AsmFields& fd, const AsmFlags&
     for (int k = 0; k < c->ncol; k++) {
         const int off = c->h_col_ptr[k], cnt = c->h_col_ptr[k + 1] - off;
         if (cnt == 0) continue;
-        hipLaunchKernelGGL((k_asm_colour_stab<DIM, LOC, R>), dim3(grid_for((int64_t)cnt * LOC, 256)), dim3(256), 0,
+        constexpr int EPB = stab_epb<LOC>();
+        hipLaunchKernelGGL((k_asm_colour_stab<DIM, LOC, R>), dim3((cnt + EPB - 1) / EPB), dim3(EPB * LOC), 0,
                            c->stream, img, c->col_elems.p + off, cnt, c->conn.p, c->conn_x.p, c->X4.p, fd, fl,
                            c->plan.epos.p, c->A.p, c->rhs.p, d_err);
     }

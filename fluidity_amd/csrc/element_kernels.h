@@ -405,6 +405,10 @@ FLGPU_HD_INLINE void row_linear(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, c
 // nodes as ele_val_at_quad does (femtools/Fields_Base.F90:2256-2310), and
 // the divergence of the velocity at g.  The nodal values come from NA
 // (GlobalNodal, or a kernel's staged copy).
+template <int DIM, typename R>
+FLGPU_HD_INLINE R nu_bar_scaled(const R (&u)[DIM], const R (&JT)[DIM][DIM], const R (&K)[DIM][DIM], bool have_k,
+                                int scheme, R scale);
+
 template <int DIM, int LOC, typename R, class NA = GlobalNodal<DIM, R>>
 struct NodalQuad {
     const NA& na;
@@ -462,6 +466,11 @@ struct NodalQuad {
 #pragma unroll
         for (int k = 0; k < LOC; k++) fq = FMA(na.S(k), tb.n[k + LOC * g], fq);
         return fq;
+    }
+    // the stabilisation's nu_bar at g, from u and the diffusivity there
+    FLGPU_HD_INLINE R nu(int g, const R (&uq)[DIM], const R (&JT)[DIM][DIM], const R (&kt)[DIM][DIM],
+                         const AsmFlags& fl) const {
+        return nu_bar_scaled<DIM, R>(uq, JT, kt, fl.diff != 0, fl.nu_scheme, (R)fl.nu_scale);
     }
 };
 
@@ -813,7 +822,7 @@ FLGPU_HD_INLINE void row_stab_q(const LTab<R>& tb, const AsmFlags& fl, const R (
             qv.k_tensor(g, kt);
             if (fl.iso) kq = qv.k_iso(g);
         }
-        const R nu = nu_bar_scaled<DIM, R>(uq, JT, kt, fl.diff != 0, fl.nu_scheme, (R)fl.nu_scale);
+        const R nu = qv.nu(g, uq, JT, kt, fl);
         R ug_i = (R)0;
 #pragma unroll
         for (int d = 0; d < DIM; d++) ug_i = FMA(Gi[d], uq[d], ug_i);
