@@ -350,6 +350,16 @@ def main() -> int:
         ref = xs["fp64_jacobi_pcg"]
         for label, x in xs.items():
             solves[label]["x_rel_diff_vs_fp64_jacobi"] = float(np.max(np.abs(x - ref)) / np.max(np.abs(ref)))
+        # time-to-solution against its HBM roofline: every Jacobi-PCG iteration
+        # moves at least B_spmv + 128 N bytes (the SpMV plus the unfused
+        # vector updates, FP64), so its / (8 TB/s / B_it) is the floor
+        b_it = spmv_bytes(cfg.mesh.nnodes, nnz, 8) + 128 * cfg.mesh.nnodes
+        j = solves["fp64_jacobi_pcg"]
+        floor_s = j["its"] * b_it / (PEAK_HBM_GBS * 1e9)
+        j["hbm_floor_seconds"] = round(floor_s, 5)
+        j["roofline_frac"] = round(floor_s / j["krylov_seconds"], 4)
+        log(f"FP64 Jacobi-PCG: HBM floor {floor_s:.4f}s for {j['its']} its of {b_it / 1e6:.0f} MB, "
+            f"{j['roofline_frac']:.2f} of roofline")
         solves["speedup_learned_mixed_vs_fp64_jacobi"] = round(
             solves["fp64_jacobi_pcg"]["seconds"] / solves["learned_mixed_fcg"]["seconds"], 3)
         log(f"speedup learned+mixed vs FP64 Jacobi-PCG: {solves['speedup_learned_mixed_vs_fp64_jacobi']}x")
