@@ -317,6 +317,63 @@ def petsc_solve(problem: CGProblem, solver: SolverOptions, x0: np.ndarray | None
     return x, info
 
 
+def petsc_system(problem: CGProblem):
+    """The system petsc_solve hands to PETSc, from the device-resident one
+    (femtools/Petsc_Tools.F90:891-1074, femtools/Solvers.F90:1072-1099):
+    inactive (ghost) rows keep only the diagonal ghost_pivot
+    sqrt((maxdiag + mindiag) maxdiag / 2) over every row's |diagonal|, their
+    columns are dropped from the other rows and lifted into the rhs, and the
+    ghost rows' rhs stays 0.  Returns 1-based (findrm, colm, val, b)."""
+    A, rhs, inact = problem.matrix()
+    sp = problem.sparsity
+    n = sp.findrm.size - 1
+    row = np.repeat(np.arange(n), np.diff(sp.findrm))
+    col = sp.colm.astype(np.int64) - 1
+    ghost = inact.astype(bool)
+    dg = np.abs(np.where(sp.centrm > 0, A[np.maximum(sp.centrm, 1) - 1], 0.0))
+    pivot = np.sqrt((dg.max() + dg.min()) * dg.max() / 2.0)
+    g = np.where(ghost, rhs, 0.0)
+    lift = np.zeros(n)
+    np.add.at(lift, row, A * g[col])
+    b = np.where(ghost, 0.0, rhs - lift)
+    keep = np.where(ghost[row], col == row, ~ghost[col])
+    val = np.where(ghost[row] & (col == row), pivot, A)[keep]
+    lens = np.bincount(row[keep], minlength=n)
+    findrm = np.concatenate([[1], 1 + np.cumsum(lens)]).astype(np.int32)
+    return findrm, (col[keep] + 1).astype(np.int32), np.ascontiguousarray(val), np.ascontiguousarray(b)
+
+
+KSP_REASONS = {-1: "Undefined", -2: "KSP_DIVERGED_NULL", -3: "KSP_DIVERGED_ITS", -4: "KSP_DIVERGED_DTOL",
+               -5: "KSP_DIVERGED_BREAKDOWN", -6: "KSP_DIVERGED_BREAKDOWN_BICG", -7: "KSP_DIVERGED_NONSYMMETRIC",
+               -8: "KSP_DIVERGED_INDEFINITE_PC", -9: "KSP_DIVERGED_NAN", -10: "KSP_DIVERGED_INDEFINITE_MAT"}
+_matrixdumped = False
+
+
+def convergence_check(problem: CGProblem, info, name: str = "", x0: np.ndarray | None = None,
+                      filename: str = "matrixdump", ignore_all_solver_failures: bool = False,
+                      no_matrixdump: bool = False) -> bool:
+    """ConvergenceCheck (femtools/Solvers.F90:1369-1479) after a solve: a
+    failure (reason <= 0) is reported and, the first time in the process,
+    the system is written as a matrix dump (DumpMatrixEquation) with x0 (zero
+    when None, start_from_zero).  Returns sig_int: whether the run should
+    dump and finish (every failure unless ignore_all_solver_failures)."""
+    global _matrixdumped
+    import sys
+    if info.reason > 0:
+        return False
+    print(f"WARNING: Failed to converge.\nPETSc did not converge for matrix solve of: {name}\n"
+          f"Reason for non-convergence: {KSP_REASONS.get(info.reason, info.reason)}\n"
+          f"Number of iterations: {info.its}", file=sys.stderr, flush=True)
+    if no_matrixdump:
+        _matrixdumped = True
+    if not _matrixdumped:
+        from .readnsolve import matrixdump_write
+        findrm, colm, val, b = petsc_system(problem)
+        matrixdump_write(filename, findrm, colm, val, b, np.zeros(b.size) if x0 is None else x0)
+        _matrixdumped = True
+    return not ignore_all_solver_failures
+
+
 def solve_field_equation_cg(problem: CGProblem, opts: CGOptions, solver: SolverOptions, dt: float,
                             bc_nodes: np.ndarray | None = None, bc_values=0.0, bcs: SurfaceBCs | None = None):
     """One solve_field_equation_cg step: assemble (elements, then faces

@@ -52,3 +52,41 @@ def test_dump_replays_the_device_solve(require_gpu, tmp_path):
     pr.close()
     assert info2.reason > 0
     assert np.max(np.abs(x_dump - x_dir)) <= 1e-8 * np.max(np.abs(x_dir))
+
+
+def test_failed_solve_dumps_the_petsc_system(require_gpu, tmp_path):
+    """ConvergenceCheck: a solve stopped at max_its (reason -3) writes the
+    system PETSc saw once; it is the oracle's reduced system, and solving
+    it gives the converged device solve's solution off the Dirichlet rows."""
+    print("\n[case] convergence_check dump c2 scale 6", flush=True)
+    cfg = M.config("c2", seed=5, scale=6)
+    ref = O.reference_system(cfg)
+    pr = cg.problem_from_config(cfg)
+    cg.assemble_advection_diffusion_cg(pr, cg.options_from_config(cfg), cfg.dt)
+    cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
+    short = cg.SolverOptions(iterative_method="cg", preconditioner="jacobi", relative_error=1e-12, max_iterations=3,
+                             check_every=1)
+    x, info = cg.petsc_solve(pr, short)
+    assert info.reason == -3
+    path = str(tmp_path / "matrixdump")
+    cg._matrixdumped = False
+    assert cg.convergence_check(pr, info, "Tracer", filename=path) is True
+    f, c, v, b, x0 = R.matrixdump_read(path)
+    rowptr, col, val = O.reduced_system(ref["sparsity"], ref["A"], ref["inactive"])
+    n = rowptr.size - 1
+    dense_ref = sps.csr_matrix((val, col, rowptr), shape=(n, n)).toarray()
+    dense = sps.csr_matrix((v, c - 1, f - 1), shape=(n, n)).toarray()
+    assert np.max(np.abs(dense - dense_ref)) <= 1e-12 * np.max(np.abs(dense_ref))
+    assert x0 is not None and not x0.any()
+    good = cg.SolverOptions(iterative_method="cg", preconditioner="jacobi", relative_error=1e-11,
+                            norm="unpreconditioned")
+    xd, infod = R.petsc_readnsolve(path, good)
+    xf, infof = cg.petsc_solve(pr, good)
+    assert infod.reason > 0 and infof.reason > 0
+    free = ref["inactive"] == 0
+    assert np.max(np.abs(xd[free] - xf[free])) <= 1e-8 * np.max(np.abs(xf))
+    # one dump per process, as the reference's saved flag
+    os.remove(path)
+    cg.convergence_check(pr, info, "Tracer", filename=path)
+    assert not os.path.exists(path)
+    pr.close()
