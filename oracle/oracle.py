@@ -331,7 +331,9 @@ def reference_system(cfg, nthreads: int = 0, variant: dict | None = None):
     kap = cfg.kappa
     if kap is not None:
         d = mesh.dim
-        if np.isscalar(kap):
+        if isinstance(kap, np.ndarray) and kap.ndim == 3:  # nodal tensors (N, dim, dim), symmetric
+            kap = np.ascontiguousarray(kap)
+        elif np.isscalar(kap):
             kap = (np.eye(d) * kap).ravel()
         else:
             full = np.zeros((mesh.nnodes, d, d))
@@ -344,7 +346,8 @@ def reference_system(cfg, nthreads: int = 0, variant: dict | None = None):
     p = params(cfg.dt, cfg.theta, v.get("beta", cfg.beta), have_mass=cfg.have_mass,
                lump=v.get("lump_mass_matrix", False), adv=cfg.have_advection,
                ibp=v.get("integrate_advection_by_parts", False),
-               diff=cfg.kappa is not None, src=cfg.source is not None, absorp=cfg.absorption is not None,
+               diff=cfg.kappa is not None, iso=v.get("isotropic_diffusivity", True),
+               src=cfg.source is not None, absorp=cfg.absorption is not None,
                stab=v.get("stabilisation", "none"), nu_bar=v.get("nu_bar_scheme", "optimal"),
                nu_scale=v.get("nu_scale", 0.5))
     A, rhs, secs = assemble(p, asm, nthreads)

@@ -144,6 +144,35 @@ def test_p1_tet_stiffness_closed_form(name):
         assert np.allclose(mat.sum(axis=1), 0.0, atol=1e-13 * np.abs(K).max())
 
 
+@pytest.mark.parametrize("name", ["c1", "c2"])
+def test_p1_anisotropic_stiffness_closed_form(name):
+    """dshape_tensor_dshape (femtools/FETools.F90:668-698) with a nodal
+    tensor field linear over the element: K_e = |T| G Kbar G^T with Kbar the
+    element mean (exact under the degree-2 rules); a transposed K (the
+    tensor is made non-symmetric here to tell) fails it."""
+    cfg = M.config(name, seed=4, scale=3)
+    m = cfg.mesh
+    d = m.dim
+    rng = np.random.default_rng(7)
+    B = rng.normal(size=(m.nnodes, d, d)) * 0.2
+    Kn = np.eye(d)[None] + B  # non-symmetric on purpose
+    Kf = np.ascontiguousarray(np.transpose(Kn, (0, 2, 1)))  # Fortran val(a, b, node) order: a fastest
+    el = E.make_element(d, 1, 2)
+    fields = {"T": np.zeros(m.nnodes), "U": None, "kappa": Kf, "src": None, "absorp": None}
+    sp = O.make_sparsity(m)
+    col, ncol = O.colour_elements(m)
+    asm = O.Assembly(m, el, el, fields, sp, O.colour_sets(col, ncol))
+    p = O.params(1.0, 1.0, 0.0, have_mass=False, adv=False, diff=True, iso=False)
+    for e in [0, 5, m.nele // 2, m.nele - 1]:
+        mat, _ = O.element(p, asm, e)
+        nodes = m.ndglno[(d + 1) * e:(d + 1) * e + d + 1] - 1
+        G, vol = _bary_gradients(m.X[nodes])
+        Kbar = Kn[nodes].mean(axis=0)
+        want = vol * G @ Kbar @ G.T
+        assert np.allclose(mat, want, rtol=1e-12, atol=1e-14 * np.abs(want).max())
+        assert not np.allclose(want, vol * G @ Kbar.T @ G.T, rtol=1e-6)
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_p1_consistent_mass_and_detwei(dim):
     """M_e = |T| / ((d+1)(d+2)) (1 + delta_ij) exactly under degree-2 rules;
