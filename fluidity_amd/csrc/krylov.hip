@@ -673,8 +673,11 @@ static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info*
     info->fallback = 0;
     const double bnorm = std::sqrt(h.bb);
     if (h.last_norm == 0.0 || h.last_norm <= sp->atol) {
+        // converged before the first iteration: the device copy of the
+        // scalars is what the solve reports from, so it gets the reason too
         h.reason = KSP_CONVERGED_ATOL;
         h.done = 1;
+        FLGPU_TRY(write_scal(c, k));
     }
     double rmax = std::sqrt(h.rr);          // reliable-update reference
     const double delta = 0.1;
