@@ -1,0 +1,109 @@
+"""Edge cases of the C ABI on the GPU, each against the oracle or a direct
+solve: strong Dirichlet in value form (dt <= 0) with nonzero values, the
+drop-in flgpu_petsc_solve taking host arrays (matrix, inactive mask, rhs),
+GMRES with a short restart, the face loop with a constant velocity, and
+FP32 element arithmetic on P2 triangles."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from fluidity_amd import cg, lib as L, meshgen as M
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def test_dirichlet_value_form(require_gpu):
+    """dt <= 0: rhs(node) = value (Boundary_Conditions.F90:1983-2025)."""
+    cfg = M.config("c1", seed=41, scale=10)
+    pr = cg.problem_from_config(cfg)
+    cg.assemble_advection_diffusion_cg(pr, cg.options_from_config(cfg), cfg.dt)
+    vals = np.cos(np.arange(cfg.mesh.boundary.size))
+    cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, vals, None)
+    A, rhs, inact = pr.matrix()
+    pr.close()
+    ref = O.reference_system(cfg)
+    inactive = np.zeros(cfg.mesh.nnodes, np.int32)
+    A_ref, rhs_ref, _ = O.assemble(ref["params"], ref["asm"])
+    O.dirichlet(cfg.mesh.boundary, vals, cfg.T, -1.0, inactive, rhs_ref)
+    assert np.array_equal(inact, inactive)
+    assert rel(rhs, rhs_ref) < 1e-12
+    assert np.array_equal(rhs[cfg.mesh.boundary - 1], vals)
+
+
+def test_petsc_solve_with_host_arrays(require_gpu):
+    """flgpu_petsc_solve with matrix%val, the inactive mask and rhs%val from
+    the host (the oracle's assembly), on a context that only has the
+    sparsity bound, gives the oracle's solution."""
+    cfg = M.config("c2", seed=42, scale=7)
+    ref = O.reference_system(cfg)
+    pr = cg.problem_from_config(cfg)  # binds the same sparsity
+    sp = cg.SolverOptions("cg", "jacobi", relative_error=1e-11, norm="unpreconditioned").params()
+    x = np.zeros(cfg.mesh.nnodes)
+    its, reason, relres = C.c_int32(0), C.c_int32(0), C.c_double(0)
+    L.check(pr.lib.flgpu_petsc_solve(pr.ctx, C.byref(sp), L.ptr(np.ascontiguousarray(ref["A"])),
+                                     L.ptr(np.ascontiguousarray(ref["inactive"], np.int32)),
+                                     L.ptr(np.ascontiguousarray(ref["rhs"])), L.ptr(x), C.byref(its),
+                                     C.byref(reason), C.byref(relres)), "flgpu_petsc_solve")
+    pr.close()
+    xr, _, oreason, _, _ = O.petsc_solve("cg", "jacobi", 1e-12, ref["sparsity"], ref["A"], ref["inactive"],
+                                         ref["rhs"])
+    assert reason.value > 0 and oreason > 0 and relres.value <= 1e-11
+    assert rel(x, xr) < 1e-8
+
+
+def test_gmres_short_restart(require_gpu):
+    cfg = M.config("c1", seed=43, scale=16)
+    ref = O.reference_system(cfg)
+    pr = cg.problem_from_config(cfg)
+    cg.assemble_advection_diffusion_cg(pr, cg.options_from_config(cfg), cfg.dt)
+    cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
+    x5, i5 = cg.petsc_solve(pr, cg.SolverOptions("gmres", "jacobi", relative_error=1e-11, restart=5,
+                                                 norm="unpreconditioned"))
+    x30, i30 = cg.petsc_solve(pr, cg.SolverOptions("gmres", "jacobi", relative_error=1e-11, restart=30,
+                                                   norm="unpreconditioned"))
+    pr.close()
+    xr, _, _, _, _ = O.petsc_solve("gmres", "jacobi", 1e-12, ref["sparsity"], ref["A"], ref["inactive"],
+                                   ref["rhs"], restart=30)
+    print("restart 5:", i5.its, "restart 30:", i30.its)
+    assert i5.reason > 0 and i30.reason > 0
+    assert i5.its >= i30.its
+    assert rel(x5, xr) < 1e-8 and rel(x30, xr) < 1e-8
+
+
+def test_face_loop_constant_velocity(require_gpu):
+    """Advection by parts with a FIELD_TYPE_CONSTANT velocity on the P1 cube:
+    the face term reads the single value (and matches the oracle)."""
+    cfg = M.config("c2", seed=44, scale=5)
+    cfg.U = np.array([0.6, -0.2, 0.35])
+    cfg.have_advection = True
+    ref = O.reference_system(cfg, variant={"bcs": "neumann_robin", "integrate_advection_by_parts": True})
+    pr = cg.problem_from_config(cfg)
+    surf = M.surface_of(cfg.mesh)
+    pr.bind_surface(surf)
+    bt, b1, b2, strong = M.unit_box_bcs(cfg.mesh, surf, "neumann_robin")
+    opts = cg.options_from_config(cfg)
+    opts.integrate_advection_by_parts = True
+    cg.assemble_advection_diffusion_cg(pr, opts, cfg.dt, cg.SurfaceBCs(bt, b1, b2))
+    cg.apply_dirichlet_conditions(pr, strong, 0.0, cfg.dt)
+    A, rhs, _ = pr.matrix()
+    pr.close()
+    assert rel(A, ref["A"]) < 1e-12
+    assert rel(rhs, ref["rhs"]) < 1e-12
+
+
+def test_p2_triangles_fp32(require_gpu):
+    cfg = M.config("c1p2", seed=45, scale=6)
+    ref = O.reference_system(cfg)
+    pr = cg.problem_from_config(cfg)
+    cg.assemble_advection_diffusion_cg(pr, cg.options_from_config(cfg, precision="fp32"), cfg.dt)
+    cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
+    A, rhs, _ = pr.matrix()
+    pr.close()
+    assert rel(A, ref["A"]) < 1e-5
+    assert rel(rhs, ref["rhs"]) < 1e-5
