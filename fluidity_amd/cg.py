@@ -43,6 +43,7 @@ class CGOptions:
     have_absorption: bool = False
     precision: str = "fp64"              # element arithmetic: fp64 | fp32
     scheme: str = "auto"                 # auto | colour | blocked
+    add_source_directly_to_rhs: bool = False  # source/diagnostic/add_directly_to_rhs
     stabilisation: str = "none"          # none | streamline_upwind | supg
     nu_bar_scheme: str = "optimal"       # optimal | doubly_asymptotic | critical_rule | unity
     nu_scale: float = 0.5
@@ -55,7 +56,8 @@ class CGOptions:
             integrate_by_parts=int(self.integrate_advection_by_parts),
             have_diffusivity=int(self.have_diffusivity),
             isotropic_diffusivity=int(self.isotropic_diffusivity),
-            have_source=int(self.have_source), have_absorption=int(self.have_absorption),
+            have_source=int(self.have_source and not self.add_source_directly_to_rhs),
+            have_absorption=int(self.have_absorption),
             precision={"fp64": L.FLGPU_FP64, "fp32": L.FLGPU_FP32}[self.precision],
             scheme={"auto": L.FLGPU_ASM_AUTO, "colour": L.FLGPU_ASM_COLOUR,
                     "blocked": L.FLGPU_ASM_BLOCKED}[self.scheme],
@@ -290,6 +292,8 @@ def assemble_advection_diffusion_cg(problem: CGProblem, opts: CGOptions, dt: flo
     p = opts.params(dt)
     L.check(problem.lib.flgpu_advdif_assemble_resident(problem.ctx, C.byref(p)),
             "flgpu_advdif_assemble_resident")
+    if opts.have_source and opts.add_source_directly_to_rhs:  # :604, before the face loop
+        L.check(problem.lib.flgpu_rhs_add_field(problem.ctx, L.FIELD_SOURCE), "flgpu_rhs_add_field")
     if bcs is not None:
         t = np.ascontiguousarray(bcs.bc_type, dtype=np.int32)
         v1 = None if bcs.value is None else np.ascontiguousarray(bcs.value, dtype=np.float64)

@@ -42,6 +42,13 @@ __global__ void k_dirichlet(int nbc, const int32_t* __restrict__ nodes,
     rhs[nd] = dt > 0.0 ? (vals[j] - t) / dt : vals[j];
 }
 
+// rhs += f (nodal, or one value for a constant field)
+__global__ void k_rhs_add(int n, const double* __restrict__ f, int f_const, double* __restrict__ rhs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    rhs[i] = rhs[i] + (f_const ? f[0] : f[i]);
+}
+
 }  // namespace flgpu
 
 using namespace flgpu;
@@ -415,6 +422,28 @@ int flgpu_advdif_assemble(flgpu_ctx* c, const flgpu_advdif_params* p, const doub
     if (absorp) FLGPU_TRY(flgpu_field_set(c, FLGPU_FIELD_ABSORPTION, 0, absorp));
     FLGPU_TRY(flgpu_advdif_assemble_resident(c, p));
     return flgpu_matrix_get(c, A_val, rhs, nullptr);
+}
+
+int flgpu_rhs_add_field(flgpu_ctx* c, int32_t which) {
+    if (!c) return set_error(FLGPU_EINVAL, "flgpu_rhs_add_field: null ctx");
+    if (!c->assembled) return set_error(FLGPU_EINVAL, "flgpu_rhs_add_field: assemble first");
+    DevField* f;
+    switch (which) {
+        case FLGPU_FIELD_SOURCE: f = &c->fS; break;
+        case FLGPU_FIELD_ABSORPTION: f = &c->fA; break;
+        case FLGPU_FIELD_T: f = &c->fT; break;
+        default: return set_error(FLGPU_EINVAL, "flgpu_rhs_add_field: field %d is not scalar", which);
+    }
+    if (!f->set) return set_error(FLGPU_EINVAL, "flgpu_rhs_add_field: field %d not set", which);
+    if (c->nnodes != c->rows)
+        return set_error(FLGPU_EINVAL, "flgpu_rhs_add_field: field nodes %d != rows %d", c->nnodes, c->rows);
+    FLGPU_HIP(hipSetDevice(c->device));
+    krylov_invalidate(c);
+    hipLaunchKernelGGL(k_rhs_add, dim3(grid_for(c->rows, 256)), dim3(256), 0, c->stream, c->rows, f->v.p,
+                       f->constant, c->rhs.p);
+    FLGPU_HIP(hipGetLastError());
+    FLGPU_HIP(hipStreamSynchronize(c->stream));
+    return FLGPU_OK;
 }
 
 int flgpu_dirichlet_apply(flgpu_ctx* c, int32_t nbc, const int32_t* nodes, const double* values,

@@ -62,7 +62,7 @@ module flgpu
   public :: flgpu_init, flgpu_finalize, flgpu_mesh_bind, flgpu_field_mesh_bind, &
        flgpu_sparsity_bind, flgpu_colouring_bind, flgpu_make_sparsity, flgpu_colour_mesh, &
        flgpu_sparsity_build, flgpu_sparsity_get, flgpu_colouring_build, flgpu_colouring_get, &
-       flgpu_field_set, flgpu_advdif_assemble, flgpu_surface_bind, flgpu_advdif_surface, &
+       flgpu_field_set, flgpu_advdif_assemble, flgpu_rhs_add_field, flgpu_surface_bind, flgpu_advdif_surface, &
        flgpu_dirichlet_apply, flgpu_matrix_get, &
        flgpu_petsc_solve, flgpu_check
 
@@ -199,6 +199,13 @@ module flgpu
        integer(c_int32_t), intent(in) :: bc_type(*)
        type(c_ptr), value :: bc_value, bc_value2
      end function flgpu_advdif_surface
+
+     !! rhs += field (the source's add_directly_to_rhs, :604)
+     integer(c_int) function flgpu_rhs_add_field(ctx, which) bind(c, name="flgpu_rhs_add_field")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: which
+     end function flgpu_rhs_add_field
 
      integer(c_int) function flgpu_dirichlet_apply(ctx, nbc, nodes, values, dt) bind(c, name="flgpu_dirichlet_apply")
        import :: c_int, c_ptr, c_int32_t, c_double

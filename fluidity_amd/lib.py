@@ -30,7 +30,7 @@ EXPORTS = (
     "flgpu_field_mesh_bind", "flgpu_sparsity_bind", "flgpu_colouring_bind",
     "flgpu_make_sparsity", "flgpu_colour_mesh", "flgpu_sparsity_build", "flgpu_sparsity_get",
     "flgpu_colouring_build", "flgpu_colouring_get", "flgpu_field_set",
-    "flgpu_advdif_assemble_resident", "flgpu_advdif_assemble", "flgpu_surface_bind",
+    "flgpu_advdif_assemble_resident", "flgpu_advdif_assemble", "flgpu_rhs_add_field", "flgpu_surface_bind",
     "flgpu_advdif_surface", "flgpu_dirichlet_apply",
     "flgpu_matrix_get", "flgpu_matrix_set", "flgpu_solve", "flgpu_petsc_solve",
     "flgpu_solution_get", "flgpu_csr_mult", "flgpu_matrixdump_read", "flgpu_matrixdump_write",
@@ -103,6 +103,7 @@ def load() -> C.CDLL:
         "flgpu_field_set": [P, I32, I32, P],
         "flgpu_advdif_assemble_resident": [P, C.POINTER(AdvdifParams)],
         "flgpu_advdif_assemble": [P, C.POINTER(AdvdifParams), P, P, P, P, P, P, P],
+        "flgpu_rhs_add_field": [P, I32],
         "flgpu_surface_bind": [P, I32, I32, I32, P, P, P, P, P, P],
         "flgpu_advdif_surface": [P, C.POINTER(AdvdifParams), P, P, P],
         "flgpu_dirichlet_apply": [P, I32, P, P, D],

@@ -204,6 +204,13 @@ int flgpu_advdif_assemble(flgpu_ctx *ctx, const flgpu_advdif_params *p,
                           const double *kappa, const double *src,
                           const double *absorp, double *A_val, double *rhs);
 
+/* rhs += field on the device-resident rhs (scalar_field addto,
+ * femtools/Fields_Manipulation.F90:255-285): the source's
+ * "add_directly_to_rhs" option (assemble/Advection_Diffusion_CG.F90:321,
+ * :604) -- assemble with have_source = 0, then add FLGPU_FIELD_SOURCE here,
+ * before the face loop and Dirichlet.  A constant field adds its value to
+ * every row. */
+int flgpu_rhs_add_field(flgpu_ctx *ctx, int32_t which);
 /* ---- surface integrals ------------------------------------------------- */
 /* Boundary-condition types of the face loop, the numbering of
  * assemble/Advection_Diffusion_CG.F90:74-75 (0 = none of them, e.g. a strong

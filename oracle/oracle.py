@@ -347,10 +347,13 @@ def reference_system(cfg, nthreads: int = 0, variant: dict | None = None):
                lump=v.get("lump_mass_matrix", False), adv=cfg.have_advection,
                ibp=v.get("integrate_advection_by_parts", False),
                diff=cfg.kappa is not None, iso=v.get("isotropic_diffusivity", True),
-               src=cfg.source is not None, absorp=cfg.absorption is not None,
+               src=cfg.source is not None and not v.get("add_source_directly_to_rhs", False),
+               absorp=cfg.absorption is not None,
                stab=v.get("stabilisation", "none"), nu_bar=v.get("nu_bar_scheme", "optimal"),
                nu_scale=v.get("nu_scale", 0.5))
     A, rhs, secs = assemble(p, asm, nthreads)
+    if cfg.source is not None and v.get("add_source_directly_to_rhs", False):
+        rhs = rhs + np.broadcast_to(np.asarray(cfg.source, np.float64), rhs.shape)  # addto(rhs, source), :604
     inactive = np.zeros(mesh.nnodes, np.int32)
     strong = mesh.boundary
     if v.get("bcs"):

@@ -107,3 +107,23 @@ def test_p2_triangles_fp32(require_gpu):
     pr.close()
     assert rel(A, ref["A"]) < 1e-5
     assert rel(rhs, ref["rhs"]) < 1e-5
+
+
+@pytest.mark.parametrize("name,scale", [("c2", 6), ("c1", 10)])
+def test_source_added_directly_to_rhs(require_gpu, name, scale):
+    """The source's add_directly_to_rhs option (Advection_Diffusion_CG.F90:
+    321, :604, :849): no element source term, rhs += source before the
+    Dirichlet step."""
+    cfg = M.config(name, seed=46, scale=scale)
+    if cfg.source is None:
+        cfg.source = np.cos(2 * cfg.mesh.node_pos[:, 0])
+    ref = O.reference_system(cfg, variant={"add_source_directly_to_rhs": True})
+    pr = cg.problem_from_config(cfg)
+    opts = cg.options_from_config(cfg)
+    opts.add_source_directly_to_rhs = True
+    cg.assemble_advection_diffusion_cg(pr, opts, cfg.dt)
+    cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
+    A, rhs, _ = pr.matrix()
+    pr.close()
+    assert rel(A, ref["A"]) < 1e-12
+    assert rel(rhs, ref["rhs"]) < 1e-12
