@@ -271,11 +271,41 @@ def surface_of(mesh: Mesh) -> Surface:
                    (ele + 1).astype(np.int32), cen)
 
 
+def surface_from_facets(mesh: Mesh, facets) -> Surface:
+    """The surface elements a mesh file lists (vertex lists, 1-based), in
+    file order, each matched to the element it bounds and numbered as that
+    element's face (the face opposite the element vertex not on it; nodes in
+    the element's boundary numbering), as Fluidity's surface mesh is."""
+    d = mesh.dim
+    nv = d + 1
+    cx = mesh.ndglno_x.reshape(mesh.nele, nv)
+    owner = {}
+    for e in range(mesh.nele):
+        for f in range(nv):
+            owner.setdefault(tuple(sorted(int(v) for j, v in enumerate(cx[e]) if j != f)), []).append((e, f))
+    cf = mesh.ndglno.reshape(mesh.nele, mesh.loc)
+    fnum = [boundary_numbering(d, mesh.degree, f + 1) for f in range(nv)]
+    xnum = [boundary_numbering(d, 1, f + 1) for f in range(nv)]
+    fn, fx, fe = [], [], []
+    for fac in facets:
+        own = owner.get(tuple(sorted(int(v) for v in fac)))
+        if not own or len(own) != 1:
+            raise ValueError(f"surface element {fac} does not bound exactly one element")
+        e, f = own[0]
+        fn.append([cf[e, q] for q in fnum[f]])
+        fx.append([cx[e, q] for q in xnum[f]])
+        fe.append(e + 1)
+    fx = np.array(fx, np.int32)
+    return Surface(len(facets), len(fnum[0]), np.array(fn, np.int32).ravel(), fx.ravel(),
+                   np.array(fe, np.int32), mesh.X[fx - 1].mean(axis=1))
+
+
 BC_NONE, BC_NEUMANN, BC_WEAKDIRICHLET, BC_INTERNAL, BC_ROBIN = 0, 1, 2, 3, 4
 
 
 def unit_box_bcs(mesh: Mesh, surf: Surface, kind: str = "neumann_robin"):
-    """Boundary conditions on the unit square / cube by side.
+    """Boundary conditions on a box-shaped domain by side (x = 0 and x = 1
+    stand for the smallest and largest x of the mesh).
     "neumann_robin": x = 0 Neumann with flux g = 1 + y/2, x = 1 Robin with
     g = 0.5 and coefficient h = 2 + y, every other side strong Dirichlet 0.
     "weak": x = 0 weak Dirichlet with value 1 - y, every other side none.
@@ -284,8 +314,9 @@ def unit_box_bcs(mesh: Mesh, surf: Surface, kind: str = "neumann_robin"):
     strong-Dirichlet sides (1-based)."""
     tol = 1e-12
     cen = surf.centroid
-    x0 = np.abs(cen[:, 0]) < tol
-    x1 = np.abs(cen[:, 0] - 1.0) < tol
+    xmin, xmax = mesh.X[:, 0].min(), mesh.X[:, 0].max()
+    x0 = np.abs(cen[:, 0] - xmin) < tol * max(1.0, abs(xmin))
+    x1 = np.abs(cen[:, 0] - xmax) < tol * max(1.0, abs(xmax))
     t = np.zeros(surf.nfaces, np.int32)
     pos = mesh.node_pos[surf.face_nodes.reshape(surf.nfaces, surf.floc) - 1]  # (nf, floc, dim)
     y = pos[:, :, 1]
@@ -326,12 +357,14 @@ def read_gmsh(path: str) -> Mesh:
         xyz[q] = [float(v) for v in t[1:4]]
     k = lines.index("$Elements")
     ne = int(lines[k + 1])
-    tris, tets = [], []
+    lines_1d, tris, tets = [], [], []
     for q in range(ne):
         t = [int(v) for v in lines[k + 2 + q].split()]
         etype, ntags = t[1], t[2]
         nodes = t[3 + ntags:]
-        if etype == 2:
+        if etype == 1:
+            lines_1d.append(nodes)
+        elif etype == 2:
             tris.append(nodes)
         elif etype == 4:
             tets.append(nodes)
@@ -339,6 +372,10 @@ def read_gmsh(path: str) -> Mesh:
     cells = np.array(tets if tets else tris, np.int64)
     remap = {int(v): q + 1 for q, v in enumerate(ids)}
     conn = np.vectorize(remap.__getitem__)(cells).astype(np.int32)
+    # the file's surface elements (triangles of a tet mesh, lines of a
+    # triangle mesh), in file order, 1-based compacted node numbers
+    sf = tris if tets else lines_1d
+    facets = [[remap[int(v)] for v in f] for f in sf]
     X = np.ascontiguousarray(xyz[:, :dim])
     loc = dim + 1
     faces = {}
@@ -349,7 +386,8 @@ def read_gmsh(path: str) -> Mesh:
     bnd = sorted({v for fk, cnt in faces.items() if cnt == 1 for v in fk})
     nd = conn.ravel()
     name = os.path.splitext(os.path.basename(path))[0]
-    return Mesh(dim, conn.shape[0], nn, nd, X, 1, nn, nd, X, np.array(bnd, np.int32), name=name)
+    return Mesh(dim, conn.shape[0], nn, nd, X, 1, nn, nd, X, np.array(bnd, np.int32), name=name,
+                meta={"surface_facets": facets})
 
 
 def grf_lognormal(pos: np.ndarray, seed: int, corr_len: float = 0.1, contrast: float = 1e4,
@@ -449,6 +487,26 @@ def config(name: str, seed: int = 1, scale: int | None = None) -> Config:
         kap = grf_lognormal(m.node_pos, seed, 0.1, 1e3)
         return Config("c5", m, 2, np.zeros(m.nnodes), None, kap, sin_source(m.node_pos), None,
                       1.0, 1.0, 0.0, False, False, "cg", 1e-8)
+    if name == "cube1":
+        # Fluidity's small tetrahedral test mesh (tests/data/cube.1.msh: the
+        # cube [-3, 3]^3 in 12 tetrahedra around one interior node), with
+        # advection-diffusion on every term
+        here = os.path.dirname(os.path.abspath(__file__))
+        m = read_gmsh(os.path.join(os.path.dirname(here), "tests", "golden", "cube.1.msh"))
+        p = m.node_pos
+        T = np.cos(0.3 * p[:, 0]) + 0.2 * p[:, 1]
+        U = np.tile(np.array([0.3, -0.2, 0.1]), (m.nnodes, 1))
+        return Config("cube1", m, 2, T, U, 0.5, np.sin(0.5 * p[:, 2]), None, 0.1, 0.5, 0.0, True, True, "gmres",
+                      1e-10)
+    if name == "lapgrid":
+        # an unstructured triangle mesh of the unit square from Fluidity's
+        # tests (tests/data/laplacian_grid.1.msh), rotating bump
+        here = os.path.dirname(os.path.abspath(__file__))
+        m = read_gmsh(os.path.join(os.path.dirname(here), "tests", "golden", "laplacian_grid.1.msh"))
+        p = m.node_pos
+        T = np.exp(-((p[:, 0] - 0.3) ** 2 + (p[:, 1] - 0.5) ** 2) / (2 * 0.1 ** 2))
+        U = np.stack([-(p[:, 1] - 0.5), p[:, 0] - 0.5], axis=1)
+        return Config("lapgrid", m, 2, T, U, 1e-2, None, None, 0.01, 0.5, 0.0, True, True, "gmres", 1e-10)
     if name == "cavity":
         # the square-cavity mesh of Fluidity's colouring test
         # (tests/data/square-cavity-2d.msh), with the rotating-bump

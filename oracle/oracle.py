@@ -359,8 +359,9 @@ def reference_system(cfg, nthreads: int = 0, variant: dict | None = None):
     if v.get("bcs"):
         # the face loop with boundary conditions by side of the unit box
         from fluidity_amd.elements import make_face_element
-        from fluidity_amd.meshgen import surface_of, unit_box_bcs
-        surf = surface_of(mesh)
+        from fluidity_amd.meshgen import surface_from_facets, surface_of, unit_box_bcs
+        facets = mesh.meta.get("surface_facets")
+        surf = surface_from_facets(mesh, facets) if facets else surface_of(mesh)
         bt, b1, b2, strong = unit_box_bcs(mesh, surf, v["bcs"])
         sd = SurfaceData(mesh, surf, make_face_element(mesh.dim, mesh.degree, cfg.quad_degree),
                          make_face_element(mesh.dim, 1, cfg.quad_degree), bt, b1, b2)

@@ -40,7 +40,9 @@ def run_case(cfg, opts_kw=None, precision="fp64", scheme="auto", isotropic=True,
 @pytest.mark.parametrize("name,scale,scheme", [("c2", 6, "colour"), ("c2", 6, "blocked"), ("c1", 16, "colour"),
                                                ("c1", 16, "blocked"), ("c3", 3, "colour"), ("c4", 6, "blocked"),
                                                ("c5", 6, "blocked"), ("c5", 9, "colour"), ("c1p2", 6, "colour"),
-                                               ("cavity", None, "colour"), ("cavity", None, "blocked")])
+                                               ("cavity", None, "colour"), ("cavity", None, "blocked"),
+                                               ("cube1", None, "colour"), ("cube1", None, "blocked"),
+                                               ("lapgrid", None, "colour"), ("lapgrid", None, "blocked")])
 def test_assembly_matches_oracle(require_gpu, name, scale, scheme):
     cfg = M.config(name, seed=3, scale=scale)
     pr, ref, A, rhs, inact = run_case(cfg, scheme=scheme)

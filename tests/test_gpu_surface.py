@@ -25,7 +25,8 @@ def run(cfg, kind, variant, scheme="auto"):
     print(f"\n[case] {cfg.name} n={cfg.mesh.nele} bcs={kind} {variant} scheme={scheme}", flush=True)
     ref = O.reference_system(cfg, variant=dict(variant, bcs=kind))
     pr = cg.problem_from_config(cfg)
-    surf = M.surface_of(cfg.mesh)
+    facets = cfg.mesh.meta.get("surface_facets")  # a mesh file's own surface elements, in file order
+    surf = M.surface_from_facets(cfg.mesh, facets) if facets else M.surface_of(cfg.mesh)
     pr.bind_surface(surf)
     bt, b1, b2, strong = M.unit_box_bcs(cfg.mesh, surf, kind)
     opts = cg.options_from_config(cfg, scheme=scheme)
@@ -45,6 +46,8 @@ def run(cfg, kind, variant, scheme="auto"):
     ("c5", 5, {}, "blocked"),
     ("c1p2", 5, {"integrate_advection_by_parts": True, "beta": 0.3}, "colour"),
     ("c3", 2, {"integrate_advection_by_parts": True}, "colour"),
+    ("cube1", None, {"integrate_advection_by_parts": True}, "colour"),
+    ("lapgrid", None, {}, "blocked"),
 ])
 def test_surface_matches_oracle(require_gpu, name, scale, variant, scheme):
     cfg = M.config(name, seed=7, scale=scale)

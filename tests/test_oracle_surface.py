@@ -61,7 +61,8 @@ def _setup(cfg, kind, **pv):
     kap = None if cfg.kappa is None else (np.eye(mesh.dim) * (cfg.kappa if np.isscalar(cfg.kappa) else 1.0)).ravel()
     fields = {"T": cfg.T, "U": cfg.U, "kappa": kap, "src": cfg.source, "absorp": None}
     asm = O.Assembly(mesh, el, xel, fields, sp, O.colour_sets(colour, ncol))
-    surf = M.surface_of(mesh)
+    facets = mesh.meta.get("surface_facets")
+    surf = M.surface_from_facets(mesh, facets) if facets else M.surface_of(mesh)
     bt, b1, b2, strong = M.unit_box_bcs(mesh, surf, kind)
     sd = O.SurfaceData(mesh, surf, E.make_face_element(mesh.dim, mesh.degree, cfg.quad_degree),
                        E.make_face_element(mesh.dim, 1, cfg.quad_degree), bt, b1, b2)
@@ -125,14 +126,17 @@ def test_p1_face_closed_forms(name, scale):
 
 
 @pytest.mark.parametrize("name,scale,u,qd", [("c1", 6, None, 2), ("c2", 3, (0.3, -0.7, 0.45), 2),
-                                             ("c1p2", 4, None, 4), ("c3", 2, (0.3, -0.7, 0.45), 4)])
+                                             ("c1p2", 4, None, 4), ("c3", 2, (0.3, -0.7, 0.45), 4),
+                                             ("cube1", None, (0.3, -0.7, 0.45), 2), ("lapgrid", None, None, 2)])
 def test_integration_by_parts_identity(name, scale, u, qd):
     """int N_i u.grad N_j = -int grad N_i . u N_j - int N_i N_j div u
     + int_boundary N_i N_j u.n: the non-IBP matrix and rhs equal the IBP
     ones plus the face loop over every boundary face, when the rules are
     exact (P1: rotation u in 2-D, cubic face integrands on 2-point Gauss,
     a constant u in 3-D; P2 on the degree-4 rules).  Pins the normal's sign
-    and scale, detwei and the face node order against the face shape."""
+    and scale, detwei and the face node order against the face shape; on
+    Fluidity's cube.1.msh the surface elements are the file's, in file
+    order."""
     cfg = M.config(name, seed=4, scale=scale)
     cfg.quad_degree = qd
     if u is not None:
