@@ -33,6 +33,22 @@ int set_error(int code, const char* fmt, ...);
         if (rc_ != FLGPU_OK) return rc_; \
     } while (0)
 
+// Runs an entry point's body and turns a C++ exception thrown by its host
+// code (a std::vector that cannot be allocated, above all) into an error
+// status, so that no exception crosses the C ABI.
+template <typename F>
+int guarded(const char* name, F&& body) noexcept {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return set_error(FLGPU_ENOMEM, "%s: host memory allocation failed", name);
+    } catch (const std::exception& e) {
+        return set_error(FLGPU_EINVAL, "%s: %s", name, e.what());
+    } catch (...) {
+        return set_error(FLGPU_EINVAL, "%s: unexpected exception", name);
+    }
+}
+
 // Owning device buffer.
 template <typename T>
 struct DevBuf {

@@ -1125,73 +1125,85 @@ extern "C" {
 int flgpu_learned_param_count(void) { return params_count(); }
 
 int flgpu_learned_set_params(flgpu_ctx* c, const float* params) {
-    if (!c || !params) return set_error(FLGPU_EINVAL, "flgpu_learned_set_params: null argument");
-    FLGPU_HIP(hipSetDevice(c->device));
-    LearnedState* s = state(c);
-    FLGPU_TRY(ensure_params(c, s));
-    FLGPU_HIP(hipMemcpyAsync(s->theta.p, params, sizeof(float) * Lay::TOTAL, hipMemcpyHostToDevice, c->stream));
-    FLGPU_HIP(hipMemsetAsync(s->am.p, 0, sizeof(float) * Lay::TOTAL, c->stream));
-    FLGPU_HIP(hipMemsetAsync(s->av.p, 0, sizeof(float) * Lay::TOTAL, c->stream));
-    s->step = 0;
-    return refresh_bf16(c, s);
+    return guarded("flgpu_learned_set_params", [&]() -> int {
+        if (!c || !params) return set_error(FLGPU_EINVAL, "flgpu_learned_set_params: null argument");
+        FLGPU_HIP(hipSetDevice(c->device));
+        LearnedState* s = state(c);
+        FLGPU_TRY(ensure_params(c, s));
+        FLGPU_HIP(hipMemcpyAsync(s->theta.p, params, sizeof(float) * Lay::TOTAL, hipMemcpyHostToDevice, c->stream));
+        FLGPU_HIP(hipMemsetAsync(s->am.p, 0, sizeof(float) * Lay::TOTAL, c->stream));
+        FLGPU_HIP(hipMemsetAsync(s->av.p, 0, sizeof(float) * Lay::TOTAL, c->stream));
+        s->step = 0;
+        return refresh_bf16(c, s);
+    });
 }
 
 int flgpu_learned_get_params(flgpu_ctx* c, float* params) {
-    if (!c || !params) return set_error(FLGPU_EINVAL, "flgpu_learned_get_params: null argument");
-    FLGPU_HIP(hipSetDevice(c->device));
-    LearnedState* s = state(c);
-    FLGPU_TRY(ensure_params(c, s));
-    FLGPU_HIP(hipMemcpyAsync(params, s->theta.p, sizeof(float) * Lay::TOTAL, hipMemcpyDeviceToHost, c->stream));
-    FLGPU_HIP(hipStreamSynchronize(c->stream));
-    return FLGPU_OK;
+    return guarded("flgpu_learned_get_params", [&]() -> int {
+        if (!c || !params) return set_error(FLGPU_EINVAL, "flgpu_learned_get_params: null argument");
+        FLGPU_HIP(hipSetDevice(c->device));
+        LearnedState* s = state(c);
+        FLGPU_TRY(ensure_params(c, s));
+        FLGPU_HIP(hipMemcpyAsync(params, s->theta.p, sizeof(float) * Lay::TOTAL, hipMemcpyDeviceToHost, c->stream));
+        FLGPU_HIP(hipStreamSynchronize(c->stream));
+        return FLGPU_OK;
+    });
 }
 
 int flgpu_learned_init_params(flgpu_ctx* c, uint64_t seed) {
-    if (!c) return set_error(FLGPU_EINVAL, "flgpu_learned_init_params: null ctx");
-    std::vector<float> th;
-    init_params_host(th, seed);
-    return flgpu_learned_set_params(c, th.data());
+    return guarded("flgpu_learned_init_params", [&]() -> int {
+        if (!c) return set_error(FLGPU_EINVAL, "flgpu_learned_init_params: null ctx");
+        std::vector<float> th;
+        init_params_host(th, seed);
+        return flgpu_learned_set_params(c, th.data());
+    });
 }
 
 int flgpu_learned_loss_grad(flgpu_ctx* c, int32_t nprobe, uint64_t seed, float* loss, float* grad) {
-    if (!c || !loss) return set_error(FLGPU_EINVAL, "flgpu_learned_loss_grad: null argument");
-    if (nprobe < 1 || nprobe > MAXPROBE) return set_error(FLGPU_EINVAL, "flgpu_learned_loss_grad: nprobe %d", nprobe);
-    if (!c->assembled) return set_error(FLGPU_EINVAL, "flgpu_learned_loss_grad: no system on the device");
-    FLGPU_HIP(hipSetDevice(c->device));
-    double l = 0.0;
-    FLGPU_TRY(loss_grad(c, nprobe, seed, &l));
-    *loss = (float)l;
-    if (grad) {
-        FLGPU_HIP(hipMemcpyAsync(grad, c->ls->grad.p, sizeof(float) * Lay::TOTAL, hipMemcpyDeviceToHost, c->stream));
-        FLGPU_HIP(hipStreamSynchronize(c->stream));
-    }
-    return FLGPU_OK;
+    return guarded("flgpu_learned_loss_grad", [&]() -> int {
+        if (!c || !loss) return set_error(FLGPU_EINVAL, "flgpu_learned_loss_grad: null argument");
+        if (nprobe < 1 || nprobe > MAXPROBE) return set_error(FLGPU_EINVAL, "flgpu_learned_loss_grad: nprobe %d", nprobe);
+        if (!c->assembled) return set_error(FLGPU_EINVAL, "flgpu_learned_loss_grad: no system on the device");
+        FLGPU_HIP(hipSetDevice(c->device));
+        double l = 0.0;
+        FLGPU_TRY(loss_grad(c, nprobe, seed, &l));
+        *loss = (float)l;
+        if (grad) {
+            FLGPU_HIP(hipMemcpyAsync(grad, c->ls->grad.p, sizeof(float) * Lay::TOTAL, hipMemcpyDeviceToHost, c->stream));
+            FLGPU_HIP(hipStreamSynchronize(c->stream));
+        }
+        return FLGPU_OK;
+    });
 }
 
 int flgpu_learned_train_step(flgpu_ctx* c, int32_t nprobe, uint64_t seed, float lr, float* loss) {
-    if (!c || !loss) return set_error(FLGPU_EINVAL, "flgpu_learned_train_step: null argument");
-    if (nprobe < 1 || nprobe > MAXPROBE) return set_error(FLGPU_EINVAL, "flgpu_learned_train_step: nprobe %d", nprobe);
-    if (!c->assembled) return set_error(FLGPU_EINVAL, "flgpu_learned_train_step: no system on the device");
-    FLGPU_HIP(hipSetDevice(c->device));
-    double l = 0.0;
-    FLGPU_TRY(loss_grad(c, nprobe, seed, &l));
-    *loss = (float)l;
-    return adam_step(c, c->ls, lr);
+    return guarded("flgpu_learned_train_step", [&]() -> int {
+        if (!c || !loss) return set_error(FLGPU_EINVAL, "flgpu_learned_train_step: null argument");
+        if (nprobe < 1 || nprobe > MAXPROBE) return set_error(FLGPU_EINVAL, "flgpu_learned_train_step: nprobe %d", nprobe);
+        if (!c->assembled) return set_error(FLGPU_EINVAL, "flgpu_learned_train_step: no system on the device");
+        FLGPU_HIP(hipSetDevice(c->device));
+        double l = 0.0;
+        FLGPU_TRY(loss_grad(c, nprobe, seed, &l));
+        *loss = (float)l;
+        return adam_step(c, c->ls, lr);
+    });
 }
 
 int flgpu_learned_factor(flgpu_ctx* c, float* G_val, int64_t* nnz) {
-    if (!c) return set_error(FLGPU_EINVAL, "flgpu_learned_factor: null ctx");
-    if (!c->assembled) return set_error(FLGPU_EINVAL, "flgpu_learned_factor: no system on the device");
-    FLGPU_HIP(hipSetDevice(c->device));
-    FLGPU_TRY(reduced_prepare(c));
-    FLGPU_TRY(learned_prepare(c, true));
-    FLGPU_TRY(build_g<float>(c, c->ls, c->ls->gv.p, 0));
-    if (nnz) *nnz = c->nnz;
-    if (G_val) {
-        FLGPU_HIP(hipMemcpyAsync(G_val, c->ls->gv.p, sizeof(float) * c->nnz, hipMemcpyDeviceToHost, c->stream));
-        FLGPU_HIP(hipStreamSynchronize(c->stream));
-    }
-    return FLGPU_OK;
+    return guarded("flgpu_learned_factor", [&]() -> int {
+        if (!c) return set_error(FLGPU_EINVAL, "flgpu_learned_factor: null ctx");
+        if (!c->assembled) return set_error(FLGPU_EINVAL, "flgpu_learned_factor: no system on the device");
+        FLGPU_HIP(hipSetDevice(c->device));
+        FLGPU_TRY(reduced_prepare(c));
+        FLGPU_TRY(learned_prepare(c, true));
+        FLGPU_TRY(build_g<float>(c, c->ls, c->ls->gv.p, 0));
+        if (nnz) *nnz = c->nnz;
+        if (G_val) {
+            FLGPU_HIP(hipMemcpyAsync(G_val, c->ls->gv.p, sizeof(float) * c->nnz, hipMemcpyDeviceToHost, c->stream));
+            FLGPU_HIP(hipStreamSynchronize(c->stream));
+        }
+        return FLGPU_OK;
+    });
 }
 
 }  // extern "C"

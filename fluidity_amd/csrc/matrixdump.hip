@@ -98,81 +98,90 @@ using namespace flgpu;
 
 extern "C" int flgpu_matrixdump_read(const char* filename, int32_t* n, int64_t* nnz, int32_t* findrm, int32_t* colm,
                                      double* val, double* b, double* x0, int32_t* has_x0) {
-    if (!filename || !n || !nnz) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: null argument");
-    Reader r;
-    r.f = std::fopen(filename, "rb");
-    if (!r.f) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: cannot open %s", filename);
-    int32_t h[4];
-    if (!r.ints(h, 4) || h[0] != kMatClassId)
-        return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: %s does not start with a PETSc binary matrix",
-                         filename);
-    const int32_t rows = h[1], cols = h[2], total = h[3];
-    if (rows <= 0 || rows != cols || total < 0)
-        return set_error(FLGPU_EUNSUPPORTED, "flgpu_matrixdump_read: %d x %d matrix with %d entries (square only)",
-                         rows, cols, total);
-    *n = rows;
-    *nnz = total;
-    if (!findrm) return FLGPU_OK;  // size query
-    if (!colm || !val || !b) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: null output array");
-    std::vector<int32_t> len(rows);
-    if (!r.ints(len.data(), rows)) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: truncated row lengths");
-    findrm[0] = 1;
-    for (int i = 0; i < rows; i++) {
-        if (len[i] < 0) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: negative row length");
-        findrm[i + 1] = findrm[i] + len[i];
-    }
-    if (findrm[rows] - 1 != total)
-        return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: row lengths sum to %d, header says %d",
-                         findrm[rows] - 1, total);
-    if (!r.ints(colm, (size_t)total))
-        return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: truncated column indices");
-    for (int i = 0; i < rows; i++)
-        for (int k = findrm[i] - 1; k < findrm[i + 1] - 1; k++)
-            if (colm[k] < 0 || colm[k] >= cols || (k > findrm[i] - 1 && colm[k] <= colm[k - 1]))
-                return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: row %d: columns out of range or unsorted",
-                                 i + 1);
-    for (int32_t k = 0; k < total; k++) colm[k] += 1;  // 1-based, as csr_sparsity%colm
-    if (!r.reals(val, (size_t)total)) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: truncated values");
-    int32_t vh[2];
-    if (!r.ints(vh, 2) || vh[0] != kVecClassId || vh[1] != rows)
-        return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: right-hand side missing or of the wrong size");
-    if (!r.reals(b, rows)) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: truncated right-hand side");
-    int32_t got_x0 = 0;
-    if (r.ints(vh, 2)) {  // the initial guess is optional (older dumps stop after b)
-        if (vh[0] != kVecClassId || vh[1] != rows)
-            return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: initial guess of the wrong size");
-        std::vector<double> tmp(rows);
-        if (!r.reals(x0 ? x0 : tmp.data(), rows))
-            return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: truncated initial guess");
-        got_x0 = 1;
-    }
-    if (has_x0) *has_x0 = got_x0;
-    return FLGPU_OK;
+    return guarded("flgpu_matrixdump_read", [&]() -> int {
+        if (!filename || !n || !nnz) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: null argument");
+        Reader r;
+        r.f = std::fopen(filename, "rb");
+        if (!r.f) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: cannot open %s", filename);
+        int32_t h[4];
+        if (!r.ints(h, 4) || h[0] != kMatClassId)
+            return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: %s does not start with a PETSc binary matrix",
+                             filename);
+        const int32_t rows = h[1], cols = h[2], total = h[3];
+        if (rows <= 0 || rows != cols || total < 0)
+            return set_error(FLGPU_EUNSUPPORTED, "flgpu_matrixdump_read: %d x %d matrix with %d entries (square only)",
+                             rows, cols, total);
+        *n = rows;
+        *nnz = total;
+        if (!findrm) return FLGPU_OK;  // size query
+        if (!colm || !val || !b) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: null output array");
+        std::vector<int32_t> len(rows);
+        if (!r.ints(len.data(), rows)) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: truncated row lengths");
+        findrm[0] = 1;
+        int64_t sum = 0;  // 64-bit, so a corrupt length cannot wrap the running offset
+        for (int i = 0; i < rows; i++) {
+            if (len[i] < 0) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: negative row length");
+            sum += len[i];
+            if (sum > total)
+                return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: row lengths exceed the %d entries of the header",
+                                 total);
+            findrm[i + 1] = (int32_t)(sum + 1);
+        }
+        if (sum != total)
+            return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: row lengths sum to %lld, header says %d",
+                             (long long)sum, total);
+        if (!r.ints(colm, (size_t)total))
+            return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: truncated column indices");
+        for (int i = 0; i < rows; i++)
+            for (int k = findrm[i] - 1; k < findrm[i + 1] - 1; k++)
+                if (colm[k] < 0 || colm[k] >= cols || (k > findrm[i] - 1 && colm[k] <= colm[k - 1]))
+                    return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: row %d: columns out of range or unsorted",
+                                     i + 1);
+        for (int32_t k = 0; k < total; k++) colm[k] += 1;  // 1-based, as csr_sparsity%colm
+        if (!r.reals(val, (size_t)total)) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: truncated values");
+        int32_t vh[2];
+        if (!r.ints(vh, 2) || vh[0] != kVecClassId || vh[1] != rows)
+            return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: right-hand side missing or of the wrong size");
+        if (!r.reals(b, rows)) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: truncated right-hand side");
+        int32_t got_x0 = 0;
+        if (r.ints(vh, 2)) {  // the initial guess is optional (older dumps stop after b)
+            if (vh[0] != kVecClassId || vh[1] != rows)
+                return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: initial guess of the wrong size");
+            std::vector<double> tmp(rows);
+            if (!r.reals(x0 ? x0 : tmp.data(), rows))
+                return set_error(FLGPU_EINVAL, "flgpu_matrixdump_read: truncated initial guess");
+            got_x0 = 1;
+        }
+        if (has_x0) *has_x0 = got_x0;
+        return FLGPU_OK;
+    });
 }
 
 extern "C" int flgpu_matrixdump_write(const char* filename, int32_t n, const int32_t* findrm, const int32_t* colm,
                                       const double* val, const double* b, const double* x0) {
-    if (!filename || !findrm || !colm || !val || !b || n <= 0)
-        return set_error(FLGPU_EINVAL, "flgpu_matrixdump_write: null argument");
-    if (findrm[0] != 1) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_write: findrm(1) != 1");
-    const int32_t total = findrm[n] - 1;
-    std::vector<int32_t> len(n), cols0((size_t)total);
-    for (int i = 0; i < n; i++) {
-        len[i] = findrm[i + 1] - findrm[i];
-        if (len[i] < 0) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_write: findrm decreasing at %d", i + 1);
-    }
-    for (int32_t k = 0; k < total; k++) {
-        if (colm[k] < 1 || colm[k] > n) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_write: colm(%d)", k + 1);
-        cols0[k] = colm[k] - 1;
-    }
-    Writer w;
-    w.f = std::fopen(filename, "wb");
-    if (!w.f) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_write: cannot create %s", filename);
-    const int32_t mh[4] = {kMatClassId, n, n, total};
-    const int32_t vh[2] = {kVecClassId, n};
-    bool ok = w.ints(mh, 4) && w.ints(len.data(), n) && w.ints(cols0.data(), (size_t)total) &&
-              w.reals(val, (size_t)total) && w.ints(vh, 2) && w.reals(b, n);
-    if (ok && x0) ok = w.ints(vh, 2) && w.reals(x0, n);
-    if (!ok) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_write: write to %s failed", filename);
-    return FLGPU_OK;
+    return guarded("flgpu_matrixdump_write", [&]() -> int {
+        if (!filename || !findrm || !colm || !val || !b || n <= 0)
+            return set_error(FLGPU_EINVAL, "flgpu_matrixdump_write: null argument");
+        if (findrm[0] != 1) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_write: findrm(1) != 1");
+        const int32_t total = findrm[n] - 1;
+        std::vector<int32_t> len(n), cols0((size_t)total);
+        for (int i = 0; i < n; i++) {
+            len[i] = findrm[i + 1] - findrm[i];
+            if (len[i] < 0) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_write: findrm decreasing at %d", i + 1);
+        }
+        for (int32_t k = 0; k < total; k++) {
+            if (colm[k] < 1 || colm[k] > n) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_write: colm(%d)", k + 1);
+            cols0[k] = colm[k] - 1;
+        }
+        Writer w;
+        w.f = std::fopen(filename, "wb");
+        if (!w.f) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_write: cannot create %s", filename);
+        const int32_t mh[4] = {kMatClassId, n, n, total};
+        const int32_t vh[2] = {kVecClassId, n};
+        bool ok = w.ints(mh, 4) && w.ints(len.data(), n) && w.ints(cols0.data(), (size_t)total) &&
+                  w.reals(val, (size_t)total) && w.ints(vh, 2) && w.reals(b, n);
+        if (ok && x0) ok = w.ints(vh, 2) && w.reals(x0, n);
+        if (!ok) return set_error(FLGPU_EINVAL, "flgpu_matrixdump_write: write to %s failed", filename);
+        return FLGPU_OK;
+    });
 }

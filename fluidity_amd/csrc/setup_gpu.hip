@@ -302,130 +302,138 @@ k_plus_one_i32(int64_t n, int32_t* __restrict__ a) {
 using namespace flgpu;
 
 extern "C" int flgpu_sparsity_build(flgpu_ctx* c, int64_t* nnz_out) {
-    if (!c) return set_error(FLGPU_EINVAL, "flgpu_sparsity_build: null ctx");
-    if (c->nnodes <= 0 || c->nele <= 0) return set_error(FLGPU_EINVAL, "flgpu_sparsity_build: bind the meshes first");
-    FLGPU_HIP(hipSetDevice(c->device));
-    DevBuf<int64_t> iptr;
-    DevBuf<int32_t> inc;
-    int max_deg = 0;
-    FLGPU_TRY(build_incidence(c, iptr, inc, &max_deg));
-    int cap = 64;
-    while (cap < max_deg * c->loc) cap <<= 1;
-    if (cap > 1024)
-        return set_error(FLGPU_EUNSUPPORTED, "flgpu_sparsity_build: a node in %d elements (at most %d)", max_deg,
-                         1024 / c->loc);
-    DevBuf<int32_t> cnt;
-    FLGPU_TRY(cnt.alloc((size_t)c->nnodes));
-    FLGPU_TRY(launch_rows<0>(c, cap, iptr, inc, cnt.p, nullptr, nullptr, nullptr));
-    DevBuf<int64_t> rp;
-    FLGPU_TRY(exclusive_scan(c->stream, cnt.p, c->nnodes, rp));
-    int64_t nnz = 0;
-    FLGPU_HIP(hipMemcpyAsync(&nnz, rp.p + c->nnodes, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
-    FLGPU_HIP(hipStreamSynchronize(c->stream));
-    if (nnz > INT32_MAX - 1) return set_error(FLGPU_EUNSUPPORTED, "flgpu_sparsity_build: %lld entries", (long long)nnz);
-    DevBuf<int32_t> col, diag;
-    FLGPU_TRY(col.alloc((size_t)nnz));
-    FLGPU_TRY(diag.alloc((size_t)c->nnodes));
-    FLGPU_HIP(hipMemsetAsync(diag.p, 0xff, sizeof(int32_t) * c->nnodes, c->stream));  // -1: no diagonal
-    FLGPU_TRY(launch_rows<1>(c, cap, iptr, inc, nullptr, rp.p, col.p, diag.p));
-    // 1-based csr_sparsity arrays, then the host binding
-    DevBuf<int32_t> findrm;
-    FLGPU_TRY(findrm.alloc((size_t)c->nnodes + 1));
-    hipLaunchKernelGGL(k_plus_one, dim3(grid_for(c->nnodes + 1, 256)), dim3(256), 0, c->stream,
-                       (int64_t)c->nnodes + 1, rp.p, findrm.p);
-    hipLaunchKernelGGL(k_plus_one_i32, dim3(grid_for(nnz, 256)), dim3(256), 0, c->stream, nnz, col.p);
-    hipLaunchKernelGGL(k_plus_one_i32, dim3(grid_for(c->nnodes, 256)), dim3(256), 0, c->stream,
-                       (int64_t)c->nnodes, diag.p);
-    FLGPU_HIP(hipGetLastError());
-    std::vector<int32_t> hf((size_t)c->nnodes + 1), hc((size_t)nnz), hd((size_t)c->nnodes);
-    FLGPU_HIP(hipMemcpyAsync(hf.data(), findrm.p, sizeof(int32_t) * hf.size(), hipMemcpyDeviceToHost, c->stream));
-    FLGPU_HIP(hipMemcpyAsync(hc.data(), col.p, sizeof(int32_t) * hc.size(), hipMemcpyDeviceToHost, c->stream));
-    FLGPU_HIP(hipMemcpyAsync(hd.data(), diag.p, sizeof(int32_t) * hd.size(), hipMemcpyDeviceToHost, c->stream));
-    FLGPU_HIP(hipStreamSynchronize(c->stream));
-    FLGPU_TRY(flgpu_sparsity_bind(c, c->nnodes, c->nnodes, hf.data(), hc.data(), hd.data()));
-    if (nnz_out) *nnz_out = nnz;
-    return FLGPU_OK;
+    return guarded("flgpu_sparsity_build", [&]() -> int {
+        if (!c) return set_error(FLGPU_EINVAL, "flgpu_sparsity_build: null ctx");
+        if (c->nnodes <= 0 || c->nele <= 0) return set_error(FLGPU_EINVAL, "flgpu_sparsity_build: bind the meshes first");
+        FLGPU_HIP(hipSetDevice(c->device));
+        DevBuf<int64_t> iptr;
+        DevBuf<int32_t> inc;
+        int max_deg = 0;
+        FLGPU_TRY(build_incidence(c, iptr, inc, &max_deg));
+        int cap = 64;
+        while (cap < max_deg * c->loc) cap <<= 1;
+        if (cap > 1024)
+            return set_error(FLGPU_EUNSUPPORTED, "flgpu_sparsity_build: a node in %d elements (at most %d)", max_deg,
+                             1024 / c->loc);
+        DevBuf<int32_t> cnt;
+        FLGPU_TRY(cnt.alloc((size_t)c->nnodes));
+        FLGPU_TRY(launch_rows<0>(c, cap, iptr, inc, cnt.p, nullptr, nullptr, nullptr));
+        DevBuf<int64_t> rp;
+        FLGPU_TRY(exclusive_scan(c->stream, cnt.p, c->nnodes, rp));
+        int64_t nnz = 0;
+        FLGPU_HIP(hipMemcpyAsync(&nnz, rp.p + c->nnodes, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+        FLGPU_HIP(hipStreamSynchronize(c->stream));
+        if (nnz > INT32_MAX - 1) return set_error(FLGPU_EUNSUPPORTED, "flgpu_sparsity_build: %lld entries", (long long)nnz);
+        DevBuf<int32_t> col, diag;
+        FLGPU_TRY(col.alloc((size_t)nnz));
+        FLGPU_TRY(diag.alloc((size_t)c->nnodes));
+        FLGPU_HIP(hipMemsetAsync(diag.p, 0xff, sizeof(int32_t) * c->nnodes, c->stream));  // -1: no diagonal
+        FLGPU_TRY(launch_rows<1>(c, cap, iptr, inc, nullptr, rp.p, col.p, diag.p));
+        // 1-based csr_sparsity arrays, then the host binding
+        DevBuf<int32_t> findrm;
+        FLGPU_TRY(findrm.alloc((size_t)c->nnodes + 1));
+        hipLaunchKernelGGL(k_plus_one, dim3(grid_for(c->nnodes + 1, 256)), dim3(256), 0, c->stream,
+                           (int64_t)c->nnodes + 1, rp.p, findrm.p);
+        hipLaunchKernelGGL(k_plus_one_i32, dim3(grid_for(nnz, 256)), dim3(256), 0, c->stream, nnz, col.p);
+        hipLaunchKernelGGL(k_plus_one_i32, dim3(grid_for(c->nnodes, 256)), dim3(256), 0, c->stream,
+                           (int64_t)c->nnodes, diag.p);
+        FLGPU_HIP(hipGetLastError());
+        std::vector<int32_t> hf((size_t)c->nnodes + 1), hc((size_t)nnz), hd((size_t)c->nnodes);
+        FLGPU_HIP(hipMemcpyAsync(hf.data(), findrm.p, sizeof(int32_t) * hf.size(), hipMemcpyDeviceToHost, c->stream));
+        FLGPU_HIP(hipMemcpyAsync(hc.data(), col.p, sizeof(int32_t) * hc.size(), hipMemcpyDeviceToHost, c->stream));
+        FLGPU_HIP(hipMemcpyAsync(hd.data(), diag.p, sizeof(int32_t) * hd.size(), hipMemcpyDeviceToHost, c->stream));
+        FLGPU_HIP(hipStreamSynchronize(c->stream));
+        FLGPU_TRY(flgpu_sparsity_bind(c, c->nnodes, c->nnodes, hf.data(), hc.data(), hd.data()));
+        if (nnz_out) *nnz_out = nnz;
+        return FLGPU_OK;
+    });
 }
 
 extern "C" int flgpu_sparsity_get(flgpu_ctx* c, int32_t* findrm, int32_t* colm, int32_t* centrm) {
-    if (!c) return set_error(FLGPU_EINVAL, "flgpu_sparsity_get: null ctx");
-    if (c->nnz <= 0) return set_error(FLGPU_EINVAL, "flgpu_sparsity_get: no sparsity bound");
-    if (findrm)
-        for (int i = 0; i <= c->rows; i++) findrm[i] = c->h_rowptr[i] + 1;
-    if (colm)
-        for (int64_t k = 0; k < c->nnz; k++) colm[k] = c->h_col[k] + 1;
-    if (centrm) {
-        std::vector<int32_t> d(c->rows);
-        FLGPU_HIP(hipSetDevice(c->device));
-        FLGPU_HIP(hipMemcpy(d.data(), c->diag.p, sizeof(int32_t) * c->rows, hipMemcpyDeviceToHost));
-        for (int i = 0; i < c->rows; i++) centrm[i] = d[i] + 1;
-    }
-    return FLGPU_OK;
+    return guarded("flgpu_sparsity_get", [&]() -> int {
+        if (!c) return set_error(FLGPU_EINVAL, "flgpu_sparsity_get: null ctx");
+        if (c->nnz <= 0) return set_error(FLGPU_EINVAL, "flgpu_sparsity_get: no sparsity bound");
+        if (findrm)
+            for (int i = 0; i <= c->rows; i++) findrm[i] = c->h_rowptr[i] + 1;
+        if (colm)
+            for (int64_t k = 0; k < c->nnz; k++) colm[k] = c->h_col[k] + 1;
+        if (centrm) {
+            std::vector<int32_t> d(c->rows);
+            FLGPU_HIP(hipSetDevice(c->device));
+            FLGPU_HIP(hipMemcpy(d.data(), c->diag.p, sizeof(int32_t) * c->rows, hipMemcpyDeviceToHost));
+            for (int i = 0; i < c->rows; i++) centrm[i] = d[i] + 1;
+        }
+        return FLGPU_OK;
+    });
 }
 
 extern "C" int flgpu_colouring_build(flgpu_ctx* c, int32_t* ncol_out) {
-    if (!c) return set_error(FLGPU_EINVAL, "flgpu_colouring_build: null ctx");
-    if (c->nnodes <= 0 || c->nele <= 0) return set_error(FLGPU_EINVAL, "flgpu_colouring_build: bind the meshes first");
-    FLGPU_HIP(hipSetDevice(c->device));
-    DevBuf<int64_t> iptr;
-    DevBuf<int32_t> inc;
-    int max_deg = 0;
-    FLGPU_TRY(build_incidence(c, iptr, inc, &max_deg));
-    DevBuf<int32_t> ca, cb, pick;
-    DevBuf<int> cnt;  // [0] uncoloured after the round, [1] error
-    FLGPU_TRY(ca.alloc((size_t)c->nele));
-    FLGPU_TRY(cb.alloc((size_t)c->nele));
-    FLGPU_TRY(pick.alloc((size_t)c->nele));
-    FLGPU_TRY(cnt.alloc(2));
-    FLGPU_HIP(hipMemsetAsync(ca.p, 0xff, sizeof(int32_t) * c->nele, c->stream));
-    FLGPU_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * 2, c->stream));
-    int32_t* cin = ca.p;
-    int32_t* cout = cb.p;
-    // every round finalises at least the uncoloured element of highest
-    // priority, so nele rounds always suffice
-    int left = c->nele, rounds = 0;
-    const dim3 grid(grid_for(c->nele, 256)), block(256);
-    while (left > 0) {
-        if (rounds++ > c->nele) return set_error(FLGPU_EHIP, "flgpu_colouring_build: no progress");
-        FLGPU_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int), c->stream));
-        hipLaunchKernelGGL(k_colour_pick, grid, block, 0, c->stream, c->nele, c->loc, c->conn.p, iptr.p, inc.p, cin,
-                           pick.p, cnt.p + 1);
-        hipLaunchKernelGGL(k_colour_resolve, grid, block, 0, c->stream, c->nele, c->loc, c->conn.p, iptr.p, inc.p,
-                           cin, pick.p, cout, cnt.p);
-        FLGPU_HIP(hipGetLastError());
-        int h[2];
-        FLGPU_HIP(hipMemcpyAsync(h, cnt.p, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
+    return guarded("flgpu_colouring_build", [&]() -> int {
+        if (!c) return set_error(FLGPU_EINVAL, "flgpu_colouring_build: null ctx");
+        if (c->nnodes <= 0 || c->nele <= 0) return set_error(FLGPU_EINVAL, "flgpu_colouring_build: bind the meshes first");
+        FLGPU_HIP(hipSetDevice(c->device));
+        DevBuf<int64_t> iptr;
+        DevBuf<int32_t> inc;
+        int max_deg = 0;
+        FLGPU_TRY(build_incidence(c, iptr, inc, &max_deg));
+        DevBuf<int32_t> ca, cb, pick;
+        DevBuf<int> cnt;  // [0] uncoloured after the round, [1] error
+        FLGPU_TRY(ca.alloc((size_t)c->nele));
+        FLGPU_TRY(cb.alloc((size_t)c->nele));
+        FLGPU_TRY(pick.alloc((size_t)c->nele));
+        FLGPU_TRY(cnt.alloc(2));
+        FLGPU_HIP(hipMemsetAsync(ca.p, 0xff, sizeof(int32_t) * c->nele, c->stream));
+        FLGPU_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * 2, c->stream));
+        int32_t* cin = ca.p;
+        int32_t* cout = cb.p;
+        // every round finalises at least the uncoloured element of highest
+        // priority, so nele rounds always suffice
+        int left = c->nele, rounds = 0;
+        const dim3 grid(grid_for(c->nele, 256)), block(256);
+        while (left > 0) {
+            if (rounds++ > c->nele) return set_error(FLGPU_EHIP, "flgpu_colouring_build: no progress");
+            FLGPU_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int), c->stream));
+            hipLaunchKernelGGL(k_colour_pick, grid, block, 0, c->stream, c->nele, c->loc, c->conn.p, iptr.p, inc.p, cin,
+                               pick.p, cnt.p + 1);
+            hipLaunchKernelGGL(k_colour_resolve, grid, block, 0, c->stream, c->nele, c->loc, c->conn.p, iptr.p, inc.p,
+                               cin, pick.p, cout, cnt.p);
+            FLGPU_HIP(hipGetLastError());
+            int h[2];
+            FLGPU_HIP(hipMemcpyAsync(h, cnt.p, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
+            FLGPU_HIP(hipStreamSynchronize(c->stream));
+            if (h[1]) return set_error(FLGPU_EUNSUPPORTED, "flgpu_colouring_build: more than %d colours", kMaxColours);
+            left = h[0];
+            std::swap(cin, cout);
+        }
+        std::vector<int32_t> colour(c->nele);
+        FLGPU_HIP(hipMemcpyAsync(colour.data(), cin, sizeof(int32_t) * c->nele, hipMemcpyDeviceToHost, c->stream));
         FLGPU_HIP(hipStreamSynchronize(c->stream));
-        if (h[1]) return set_error(FLGPU_EUNSUPPORTED, "flgpu_colouring_build: more than %d colours", kMaxColours);
-        left = h[0];
-        std::swap(cin, cout);
-    }
-    std::vector<int32_t> colour(c->nele);
-    FLGPU_HIP(hipMemcpyAsync(colour.data(), cin, sizeof(int32_t) * c->nele, hipMemcpyDeviceToHost, c->stream));
-    FLGPU_HIP(hipStreamSynchronize(c->stream));
-    // colour sets, ascending element ids inside each (counting sort)
-    int ncol = 0;
-    for (int e = 0; e < c->nele; e++) ncol = std::max(ncol, colour[e] + 1);
-    std::vector<int32_t> ptr(ncol + 1, 0), elems(c->nele);
-    for (int e = 0; e < c->nele; e++) ptr[colour[e] + 1]++;
-    for (int k = 0; k < ncol; k++) ptr[k + 1] += ptr[k];
-    {
-        std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
-        for (int e = 0; e < c->nele; e++) elems[fill[colour[e]]++] = e;
-    }
-    // valid by construction (no two finals of one colour are neighbours):
-    // bound without the host check of flgpu_colouring_bind
-    FLGPU_TRY(colouring_adopt(c, ptr, elems));
-    if (ncol_out) *ncol_out = ncol;
-    return FLGPU_OK;
+        // colour sets, ascending element ids inside each (counting sort)
+        int ncol = 0;
+        for (int e = 0; e < c->nele; e++) ncol = std::max(ncol, colour[e] + 1);
+        std::vector<int32_t> ptr(ncol + 1, 0), elems(c->nele);
+        for (int e = 0; e < c->nele; e++) ptr[colour[e] + 1]++;
+        for (int k = 0; k < ncol; k++) ptr[k + 1] += ptr[k];
+        {
+            std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+            for (int e = 0; e < c->nele; e++) elems[fill[colour[e]]++] = e;
+        }
+        // valid by construction (no two finals of one colour are neighbours):
+        // bound without the host check of flgpu_colouring_bind
+        FLGPU_TRY(colouring_adopt(c, ptr, elems));
+        if (ncol_out) *ncol_out = ncol;
+        return FLGPU_OK;
+    });
 }
 
 extern "C" int flgpu_colouring_get(flgpu_ctx* c, int32_t* col_ptr, int32_t* elems) {
-    if (!c) return set_error(FLGPU_EINVAL, "flgpu_colouring_get: null ctx");
-    if (c->ncol <= 0) return set_error(FLGPU_EINVAL, "flgpu_colouring_get: no colouring bound");
-    if (col_ptr)
-        for (int k = 0; k <= c->ncol; k++) col_ptr[k] = c->h_col_ptr[k] + 1;
-    if (elems)
-        for (int e = 0; e < c->nele; e++) elems[e] = c->h_col_elems[e] + 1;
-    return FLGPU_OK;
+    return guarded("flgpu_colouring_get", [&]() -> int {
+        if (!c) return set_error(FLGPU_EINVAL, "flgpu_colouring_get: null ctx");
+        if (c->ncol <= 0) return set_error(FLGPU_EINVAL, "flgpu_colouring_get: no colouring bound");
+        if (col_ptr)
+            for (int k = 0; k <= c->ncol; k++) col_ptr[k] = c->h_col_ptr[k] + 1;
+        if (elems)
+            for (int e = 0; e < c->nele; e++) elems[e] = c->h_col_elems[e] + 1;
+        return FLGPU_OK;
+    });
 }

@@ -446,64 +446,68 @@ extern "C" int flgpu_surface_bind(flgpu_ctx* c, int32_t nfaces, int32_t floc, in
                                   const int32_t* face_ndglno, const int32_t* face_ndglno_x,
                                   const int32_t* face_ele, const double* fn, const double* fw,
                                   const double* fx_dn) {
-    if (!c || !face_ndglno || !face_ndglno_x || !face_ele || !fn || !fw || !fx_dn)
-        return set_error(FLGPU_EINVAL, "flgpu_surface_bind: null argument");
-    if (c->nnodes <= 0 || c->nele <= 0)
-        return set_error(FLGPU_EINVAL, "flgpu_surface_bind: bind the coordinate and field meshes first");
-    const int dim = c->dim;
-    const int flin = dim, fquad = dim * (dim + 1) / 2;
-    if (nfaces < 0 || !(floc == flin || floc == fquad) || floc > kMaxFaceLoc || fngi < 1 || fngi > kMaxFaceGi)
-        return set_error(FLGPU_EUNSUPPORTED, "flgpu_surface_bind: face loc %d / ngi %d (dim %d)", floc, fngi, dim);
-    if ((floc == flin) != c->linear)
-        return set_error(FLGPU_EINVAL, "flgpu_surface_bind: face degree differs from the field mesh's");
-    FLGPU_HIP(hipSetDevice(c->device));
-    std::vector<int32_t> fnd((size_t)nfaces * floc), fx((size_t)nfaces * dim), fe(nfaces);
-    for (size_t k = 0; k < fnd.size(); k++) {
-        const int v = face_ndglno[k];
-        if (v < 1 || v > c->nnodes)
-            return set_error(FLGPU_EINVAL, "flgpu_surface_bind: face_ndglno[%zu]=%d out of 1..%d", k, v, c->nnodes);
-        fnd[k] = v - 1;
-    }
-    for (size_t k = 0; k < fx.size(); k++) {
-        const int v = face_ndglno_x[k];
-        if (v < 1 || v > c->nx)
-            return set_error(FLGPU_EINVAL, "flgpu_surface_bind: face_ndglno_x[%zu]=%d out of 1..%d", k, v, c->nx);
-        fx[k] = v - 1;
-    }
-    for (int f = 0; f < nfaces; f++) {
-        const int e = face_ele[f];
-        if (e < 1 || e > c->nele)
-            return set_error(FLGPU_EINVAL, "flgpu_surface_bind: face_ele(%d)=%d out of 1..%d", f + 1, e, c->nele);
-        fe[f] = e - 1;
-    }
-    FaceTables t;
-    std::memset(&t, 0, sizeof t);
-    t.floc = floc;
-    t.fngi = fngi;
-    t.floc_x = dim;
-    for (int k = 0; k < floc * fngi; k++) t.fn[k] = fn[k];
-    for (int g = 0; g < fngi; g++) t.fw[g] = fw[g];
-    for (int k = 0; k < dim * fngi * (dim - 1); k++) t.fxdn[k] = fx_dn[k];
-    SurfaceState& S = c->surf;
-    FLGPU_TRY(S.fnodes.upload(fnd.data(), fnd.size(), c->stream));
-    FLGPU_TRY(S.fnodes_x.upload(fx.data(), fx.size(), c->stream));
-    FLGPU_TRY(S.fele.upload(fe.data(), fe.size(), c->stream));
-    FLGPU_TRY(S.d_tab.upload(&t, 1, c->stream));
-    FLGPU_HIP(hipStreamSynchronize(c->stream));
-    S.h_fnodes.swap(fnd);
-    S.tab = t;
-    S.nfaces = nfaces;
-    S.floc = floc;
-    S.fngi = fngi;
-    S.lists_gen = -1;
-    S.bound = true;
-    return FLGPU_OK;
+    return guarded("flgpu_surface_bind", [&]() -> int {
+        if (!c || !face_ndglno || !face_ndglno_x || !face_ele || !fn || !fw || !fx_dn)
+            return set_error(FLGPU_EINVAL, "flgpu_surface_bind: null argument");
+        if (c->nnodes <= 0 || c->nele <= 0)
+            return set_error(FLGPU_EINVAL, "flgpu_surface_bind: bind the coordinate and field meshes first");
+        const int dim = c->dim;
+        const int flin = dim, fquad = dim * (dim + 1) / 2;
+        if (nfaces < 0 || !(floc == flin || floc == fquad) || floc > kMaxFaceLoc || fngi < 1 || fngi > kMaxFaceGi)
+            return set_error(FLGPU_EUNSUPPORTED, "flgpu_surface_bind: face loc %d / ngi %d (dim %d)", floc, fngi, dim);
+        if ((floc == flin) != c->linear)
+            return set_error(FLGPU_EINVAL, "flgpu_surface_bind: face degree differs from the field mesh's");
+        FLGPU_HIP(hipSetDevice(c->device));
+        std::vector<int32_t> fnd((size_t)nfaces * floc), fx((size_t)nfaces * dim), fe(nfaces);
+        for (size_t k = 0; k < fnd.size(); k++) {
+            const int v = face_ndglno[k];
+            if (v < 1 || v > c->nnodes)
+                return set_error(FLGPU_EINVAL, "flgpu_surface_bind: face_ndglno[%zu]=%d out of 1..%d", k, v, c->nnodes);
+            fnd[k] = v - 1;
+        }
+        for (size_t k = 0; k < fx.size(); k++) {
+            const int v = face_ndglno_x[k];
+            if (v < 1 || v > c->nx)
+                return set_error(FLGPU_EINVAL, "flgpu_surface_bind: face_ndglno_x[%zu]=%d out of 1..%d", k, v, c->nx);
+            fx[k] = v - 1;
+        }
+        for (int f = 0; f < nfaces; f++) {
+            const int e = face_ele[f];
+            if (e < 1 || e > c->nele)
+                return set_error(FLGPU_EINVAL, "flgpu_surface_bind: face_ele(%d)=%d out of 1..%d", f + 1, e, c->nele);
+            fe[f] = e - 1;
+        }
+        FaceTables t;
+        std::memset(&t, 0, sizeof t);
+        t.floc = floc;
+        t.fngi = fngi;
+        t.floc_x = dim;
+        for (int k = 0; k < floc * fngi; k++) t.fn[k] = fn[k];
+        for (int g = 0; g < fngi; g++) t.fw[g] = fw[g];
+        for (int k = 0; k < dim * fngi * (dim - 1); k++) t.fxdn[k] = fx_dn[k];
+        SurfaceState& S = c->surf;
+        FLGPU_TRY(S.fnodes.upload(fnd.data(), fnd.size(), c->stream));
+        FLGPU_TRY(S.fnodes_x.upload(fx.data(), fx.size(), c->stream));
+        FLGPU_TRY(S.fele.upload(fe.data(), fe.size(), c->stream));
+        FLGPU_TRY(S.d_tab.upload(&t, 1, c->stream));
+        FLGPU_HIP(hipStreamSynchronize(c->stream));
+        S.h_fnodes.swap(fnd);
+        S.tab = t;
+        S.nfaces = nfaces;
+        S.floc = floc;
+        S.fngi = fngi;
+        S.lists_gen = -1;
+        S.bound = true;
+        return FLGPU_OK;
+    });
 }
 
 extern "C" int flgpu_advdif_surface(flgpu_ctx* c, const flgpu_advdif_params* p, const int32_t* bc_type,
                                     const double* bc_value, const double* bc_value2) {
-    if (!c || !p) return set_error(FLGPU_EINVAL, "flgpu_advdif_surface: null argument");
-    FLGPU_HIP(hipSetDevice(c->device));
-    krylov_invalidate(c);
-    return surface_assemble(c, p, bc_type, bc_value, bc_value2);
+    return guarded("flgpu_advdif_surface", [&]() -> int {
+        if (!c || !p) return set_error(FLGPU_EINVAL, "flgpu_advdif_surface: null argument");
+        FLGPU_HIP(hipSetDevice(c->device));
+        krylov_invalidate(c);
+        return surface_assemble(c, p, bc_type, bc_value, bc_value2);
+    });
 }

@@ -73,6 +73,9 @@ def test_reader_errors(tmp_path):
         "truncated": good[:40],
         "unsorted": good[:16 + 12] + struct.pack(">7i", 1, 0, 0, 1, 2, 1, 2) + good[16 + 12 + 28:],
         "nonsquare": struct.pack(">4i", 1211216, 3, 4, 7) + good[16:],
+        # a row length near 2^31 would wrap a 32-bit running offset
+        "huge_row": good[:16] + struct.pack(">3i", 2, 2**31 - 2, 3) + good[28:],
+        "short_rows": good[:16] + struct.pack(">3i", 2, 3, 1) + good[28:],
     }
     for name, data in cases.items():
         p = str(tmp_path / name)
