@@ -186,14 +186,17 @@ def next_rows_report(cfg, pr, log_each) -> dict:
     pr.set_fields(velocity=np.array([1.0, 0.5, 0.25]))
     ms = C.c_double(0)
     for stab in ("supg", "streamline_upwind"):
-        so = cg.options_from_config(cfg)
-        so.exclude_advection_terms = False
-        so.stabilisation = stab
-        ps = so.params(cfg.dt)
-        L.check(lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(ps)), "assemble " + stab)
-        L.check(lib.flgpu_time_assemble(pr.ctx, C.byref(ps), 5, C.byref(ms)), "time " + stab)
-        rep[f"assembly_{stab}"] = {"ms": round(ms.value, 4), "Melem/s": round(cfg.mesh.nele / ms.value / 1e3, 2)}
-        log_each(f"assembly with {stab}: {ms.value:.3f} ms ({cfg.mesh.nele / ms.value / 1e3:.1f} Melem/s)")
+        for scheme in ("blocked", "colour"):  # node-staged slabs, then the colour scheme
+            so = cg.options_from_config(cfg, scheme=scheme)
+            so.exclude_advection_terms = False
+            so.stabilisation = stab
+            ps = so.params(cfg.dt)
+            L.check(lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(ps)), "assemble " + stab)
+            L.check(lib.flgpu_time_assemble(pr.ctx, C.byref(ps), 5, C.byref(ms)), "time " + stab)
+            key = f"assembly_{stab}" + ("" if scheme == "blocked" else "_colour_scheme")
+            rep[key] = {"ms": round(ms.value, 4), "Melem/s": round(cfg.mesh.nele / ms.value / 1e3, 2)}
+            log_each(f"assembly with {stab} ({scheme} scheme): {ms.value:.3f} ms "
+                     f"({cfg.mesh.nele / ms.value / 1e3:.1f} Melem/s)")
     # set-up: host routines, then the GPU ones (bound afterwards)
     t0 = time.perf_counter()
     cg.make_sparsity(cfg.mesh)

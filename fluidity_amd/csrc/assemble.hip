@@ -648,15 +648,16 @@ k_asm_blocked(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int n
 }
 
 // Nodal field values from the slab's LDS copy (a constant field is staged
-// as its one value at every node); diffusivity must be isotropic
-// (component 0 staged).
+// as its one value at every node).  The diffusivity is staged as its first
+// component (isotropic, the unstabilised kernel) or as the whole tensor
+// (the stabilised kernel, whose nu_bar reads every component).
 template <int DIM, typename R>
 struct StagedNodal {
     const int* lid;
     const R *sU, *sK, *sS, *sB;
     int ncap;
     __device__ __forceinline__ R U(int k, int d) const { return (R)sU[d * ncap + lid[k]]; }
-    __device__ __forceinline__ R K(int k, int) const { return (R)sK[lid[k]]; }
+    __device__ __forceinline__ R K(int k, int comp) const { return (R)sK[comp * ncap + lid[k]]; }
     __device__ __forceinline__ R S(int k) const { return (R)sS[lid[k]]; }
     __device__ __forceinline__ R B(int k) const { return (R)sB[lid[k]]; }
 };
@@ -685,6 +686,7 @@ struct StagedPlan {
     const uint8_t* srb;
     int nslab, rows_cap, nnz_cap, slot_cap, node_cap;
     int stage_u, stage_k, stage_s, stage_b;
+    int kcomp;  // diffusivity components staged per node: 1 or DIM*DIM
 };
 
 // A slab's ranges (its 32-byte record) and its entry offsets' whole words.
@@ -741,18 +743,21 @@ __device__ __forceinline__ void load_slab_words(const StagedPlan& sp, const Slab
     w.vn = tid < m.NN ? sp.nodes[m.n0 + tid] : 0;
 }
 
-// One staged node's coordinates and fields (thread q holds node q).
-template <int DIM>
+// One staged node's coordinates and fields (thread q holds node q); KC
+// diffusivity components (sp.kcomp of them are staged).
+template <int DIM, int KC = 1>
 struct NodeVals {
-    double x[DIM], t, k, s, b, u[DIM];
+    double x[DIM], t, k[KC], s, b, u[DIM];
 };
 
-template <int DIM>
+template <int DIM, int KC>
 __device__ __forceinline__ void load_node_vals(const StagedPlan& sp, const double* __restrict__ X4,
-                                               const AsmFields& fd, int vn, bool have, NodeVals<DIM>& v) {
+                                               const AsmFields& fd, int vn, bool have, NodeVals<DIM, KC>& v) {
 #pragma unroll
     for (int a = 0; a < DIM; a++) v.x[a] = v.u[a] = 0.0;
-    v.t = v.k = v.s = v.b = 0.0;
+#pragma unroll
+    for (int a = 0; a < KC; a++) v.k[a] = 0.0;
+    v.t = v.s = v.b = 0.0;
     if (have) {
 #pragma unroll
         for (int a = 0; a < DIM; a++) v.x[a] = X4[(int64_t)vn * 4 + a];
@@ -760,7 +765,9 @@ __device__ __forceinline__ void load_node_vals(const StagedPlan& sp, const doubl
         if (sp.stage_u)
 #pragma unroll
             for (int a = 0; a < DIM; a++) v.u[a] = fd.U[fd.u_c ? a : (int64_t)vn * DIM + a];
-        if (sp.stage_k) v.k = fd.K[fd.k_c ? 0 : (int64_t)vn * DIM * DIM];
+        if (sp.stage_k)
+#pragma unroll
+            for (int a = 0; a < KC; a++) v.k[a] = fd.K[(fd.k_c ? 0 : (int64_t)vn * DIM * DIM) + a];
         if (sp.stage_s) v.s = fd.S[fd.s_c ? 0 : vn];
         if (sp.stage_b) v.b = fd.B[fd.b_c ? 0 : vn];
     }
@@ -777,7 +784,8 @@ struct SlabLds {
 
 template <int DIM, typename R>
 __host__ __device__ __forceinline__ size_t slab_lds_bytes(const StagedPlan& sp) {
-    const size_t nf = (size_t)DIM + 1 + (sp.stage_u ? DIM : 0) + sp.stage_k + sp.stage_s + sp.stage_b;
+    const size_t nf =
+        (size_t)DIM + 1 + (sp.stage_u ? DIM : 0) + (sp.stage_k ? sp.kcomp : 0) + sp.stage_s + sp.stage_b;
     const size_t b = sizeof(R) * nf * sp.node_cap +
                      sizeof(int32_t) * ((size_t)(sp.rows_cap + 1) + sp.rows_cap + ((sp.nnz_cap + 3) >> 1));
     return (b + 15) & ~(size_t)15;
@@ -791,7 +799,7 @@ __device__ __forceinline__ SlabLds<R> slab_lds(char* base, const StagedPlan& sp)
     L.sT = L.sX + (size_t)DIM * nc;
     L.sU = L.sT + nc;
     L.sK = L.sU + (sp.stage_u ? (size_t)DIM * nc : 0);
-    L.sS = L.sK + (sp.stage_k ? nc : 0);
+    L.sS = L.sK + (sp.stage_k ? (size_t)sp.kcomp * nc : 0);
     L.sB = L.sS + (sp.stage_s ? nc : 0);
     L.soff = reinterpret_cast<int32_t*>(L.sB + (sp.stage_b ? nc : 0));
     L.sdg = L.soff + sp.rows_cap + 1;
@@ -809,8 +817,8 @@ __device__ __forceinline__ void store_slab_words(const SlabWords& w, const SlabM
     if (tid + NT < m.nwe) L.scp[tid + NT] = w.vcp1;
 }
 
-template <int DIM, typename R>
-__device__ __forceinline__ void store_node_vals(const StagedPlan& sp, const NodeVals<DIM>& v, int NN,
+template <int DIM, int KC, typename R>
+__device__ __forceinline__ void store_node_vals(const StagedPlan& sp, const NodeVals<DIM, KC>& v, int NN,
                                                 const SlabLds<R>& L, int q) {
     if (q >= NN) return;
     const int nc = sp.node_cap;
@@ -820,16 +828,56 @@ __device__ __forceinline__ void store_node_vals(const StagedPlan& sp, const Node
     if (sp.stage_u)
 #pragma unroll
         for (int a = 0; a < DIM; a++) L.sU[a * nc + q] = (R)v.u[a];
-    if (sp.stage_k) L.sK[q] = (R)v.k;
+    if (sp.stage_k)
+#pragma unroll
+        for (int a = 0; a < KC; a++) L.sK[a * nc + q] = (R)v.k[a];
     if (sp.stage_s) L.sS[q] = (R)v.s;
     if (sp.stage_b) L.sB[q] = (R)v.b;
 }
 
-// Phase A: one element row per slot, from LDS; its values to their places.
+// The element row of the unstabilised kernel: geometry by differences of
+// the vertex coordinates, the compiled term set FEAT.
 template <int DIM, int FEAT, typename R>
-__device__ __forceinline__ void staged_phase_a(const LTabLin<R>& tb, const AsmFlags& fl, const StagedPlan& sp,
-                                               const SlabMeta& m, const SlabLds<R>& L, const SlabWords& w,
-                                               R* sc, R* srhs, int tid) {
+struct StagedRowLinear {
+    const LTabLin<R>& tb;
+    const AsmFlags& fl;
+    const SlabLds<R>& L;
+    int nc;
+    __device__ __forceinline__ void operator()(const Elem<DIM, DIM + 1, R>& e, const int (&lid)[DIM + 1], int i,
+                                               R (&vals)[DIM + 1], R& r) const {
+        GeomLin<DIM, DIM + 1, R> gm;
+        geom_linear_p1<DIM, DIM + 1, R>(e, gm);
+        row_linear_g<DIM, DIM + 1, R, FEAT, StagedNodal<DIM, R>, LTabLin<R>>(
+            e, gm, tb, fl, StagedNodal<DIM, R>{lid, L.sU, L.sK, L.sS, L.sB, nc}, i, vals, r);
+    }
+};
+
+// The element row with streamline-upwind / SUPG stabilisation: the
+// Jacobian from the coordinate tables, velocity, nu_bar and the fields at
+// every quadrature point from the staged nodal values (row_stab_q, as the
+// colour scheme's stabilised kernel computes them, so the same bits).
+template <int DIM, typename R>
+struct StagedRowStab {
+    const LTab<R>& tb;
+    const AsmFlags& fl;
+    const SlabLds<R>& L;
+    int nc;
+    __device__ __forceinline__ void operator()(const Elem<DIM, DIM + 1, R>& e, const int (&lid)[DIM + 1], int i,
+                                               R (&vals)[DIM + 1], R& r) const {
+        constexpr int LOC = DIM + 1;
+        R JT[DIM][DIM], iJ[DIM][DIM], adet;
+        geometry_jt<DIM, R>(e.X, tb, JT, iJ, adet);
+        const StagedNodal<DIM, R> na{lid, L.sU, L.sK, L.sS, L.sB, nc};
+        row_stab_q<DIM, LOC, R>(tb, fl, JT, iJ, adet, NodalQuad<DIM, LOC, R, StagedNodal<DIM, R>>{na, tb, iJ}, e.T,
+                                i, vals, r);
+    }
+};
+
+// Phase A: one element row per slot, from LDS (ROW computes it); its
+// values to their places.
+template <int DIM, typename R, class ROW>
+__device__ __forceinline__ void staged_phase_a(const ROW& row, const StagedPlan& sp, const SlabMeta& m,
+                                               const SlabLds<R>& L, const SlabWords& w, R* sc, R* srhs, int tid) {
     constexpr int LOC = DIM + 1;
     constexpr int NT = kStagedThreads;
     const int nc = sp.node_cap;
@@ -857,11 +905,8 @@ __device__ __forceinline__ void staged_phase_a(const LTabLin<R>& tb, const AsmFl
             for (int a = 0; a < DIM; a++) e.X[a][k] = L.sX[a * nc + lid[k]];
             e.T[k] = L.sT[lid[k]];
         }
-        GeomLin<DIM, LOC, R> gm;
-        geom_linear_p1<DIM, LOC, R>(e, gm);
         R vals[LOC], r;
-        row_linear_g<DIM, LOC, R, FEAT, StagedNodal<DIM, R>, LTabLin<R>>(
-            e, gm, tb, fl, StagedNodal<DIM, R>{lid, L.sU, L.sK, L.sS, L.sB, nc}, i, vals, r);
+        row(e, lid, i, vals, r);
         const int ks = L.soff[l];
         R* cr = sc + LOC * ks;
         const int dg = L.sdg[l] + (sl - ks);
@@ -924,12 +969,45 @@ k_asm_staged(StagedPlan sp, const double* __restrict__ X4, AsmFields fd, AsmFlag
     SlabWords w;
     load_slab_words(sp, m, tid, w);
     NodeVals<DIM> nv;
-    load_node_vals<DIM>(sp, X4, fd, w.vn, tid < m.NN, nv);
+    load_node_vals<DIM, 1>(sp, X4, fd, w.vn, tid < m.NN, nv);
     if (tid < TW) reinterpret_cast<uint32_t*>(&tb)[tid] = tword;
     store_slab_words(w, m, L, tid);
-    store_node_vals<DIM, R>(sp, nv, m.NN, L, tid);
+    store_node_vals<DIM, 1, R>(sp, nv, m.NN, L, tid);
     __syncthreads();
-    staged_phase_a<DIM, FEAT, R>(tb, fl, sp, m, L, w, sc, srhs, tid);
+    staged_phase_a<DIM, R>(StagedRowLinear<DIM, FEAT, R>{tb, fl, L, sp.node_cap}, sp, m, L, w, sc, srhs, tid);
+    __syncthreads();
+    staged_phase_b<LOC, R>(m, L, sc, srhs, A, rhs, tid);
+}
+
+// The node-staged slab kernel with stabilisation: the same slabs, plan and
+// summation order, the full element tables (the quadrature points' shape
+// values and weights) in LDS beside the slab, the whole diffusivity tensor
+// staged per node (nu_bar reads every component).
+template <int DIM, typename R>
+__global__ void __launch_bounds__(kStagedThreads, 2)
+k_asm_staged_stab(StagedPlan sp, const uint32_t* __restrict__ ltab, const double* __restrict__ X4, AsmFields fd,
+                  AsmFlags fl, double* __restrict__ A, double* __restrict__ rhs) {
+    constexpr int LOC = DIM + 1;
+    __shared__ LTab<R> tb;
+    extern __shared__ double dyn[];
+    R* sc = reinterpret_cast<R*>(dyn);
+    R* srhs = sc + (size_t)sp.slot_cap * LOC;
+    const SlabLds<R> L = slab_lds<DIM, R>(reinterpret_cast<char*>(srhs + sp.slot_cap), sp);
+
+    const int per = (sp.nslab + 7) >> 3;
+    const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (b >= sp.nslab) return;
+    const int tid = threadIdx.x;
+    stage_table_image<R, kStagedThreads>(ltab, tb);
+    const SlabMeta m = slab_meta(sp.meta + 8 * (size_t)b);
+    SlabWords w;
+    load_slab_words(sp, m, tid, w);
+    NodeVals<DIM, DIM * DIM> nv;
+    load_node_vals<DIM, DIM * DIM>(sp, X4, fd, w.vn, tid < m.NN, nv);
+    store_slab_words(w, m, L, tid);
+    store_node_vals<DIM, DIM * DIM, R>(sp, nv, m.NN, L, tid);
+    __syncthreads();
+    staged_phase_a<DIM, R>(StagedRowStab<DIM, R>{tb, fl, L, sp.node_cap}, sp, m, L, w, sc, srhs, tid);
     __syncthreads();
     staged_phase_b<LOC, R>(m, L, sc, srhs, A, rhs, tid);
 }
@@ -1224,6 +1302,7 @@ static int launch_staged_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags&
     sp.stage_k = (FEAT & F_DIFF) && fl.diff;
     sp.stage_s = (FEAT & F_SRC) && fl.src;
     sp.stage_b = (FEAT & F_ABS) && fl.absorp;
+    sp.kcomp = 1;
     const size_t head = sizeof(R) * (size_t)P.slot_cap * (LOC + 1), bb = slab_lds_bytes<DIM, R>(sp);
     const int per = (P.nblocks + 7) / 8;  // slabs per XCD
     const size_t lds = ((head + 15) & ~(size_t)15) + bb;
@@ -1232,6 +1311,47 @@ static int launch_staged_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags&
     FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_staged<DIM, FEAT, R>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL((k_asm_staged<DIM, FEAT, R>), dim3(8 * per), dim3(kStagedThreads), lds, c->stream, sp,
+                       c->X4.p, fd, fl, c->A.p, c->rhs.p);
+    FLGPU_HIP(hipGetLastError());
+    return FLGPU_OK;
+}
+
+// Stabilised assembly on the node-staged slabs (the plan's node lists and
+// standard P1 tables); any diffusivity, the whole tensor staged.
+template <int DIM, typename R>
+static int launch_staged_stab(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl) {
+    const BlockPlan& P = c->plan;
+    constexpr int LOC = DIM + 1;
+    const uint32_t* img = nullptr;
+    FLGPU_TRY(table_image<R>(c, &img));
+    StagedPlan sp;
+    sp.tabw = nullptr;
+    sp.meta = P.meta.p;
+    sp.iptr = P.iptr.p;
+    sp.nodes = P.nodes.p;
+    sp.rdg = P.rdg.p;
+    sp.ecp = P.ecp.p;
+    sp.slotw = P.slotw.p;
+    sp.slotd = P.slotd.p;
+    sp.srb = P.srb.p;
+    sp.nslab = P.nblocks;
+    sp.rows_cap = P.rows_cap;
+    sp.nnz_cap = P.nnz_cap;
+    sp.slot_cap = P.slot_cap;
+    sp.node_cap = P.node_cap;
+    sp.stage_u = 1;  // nu_bar needs the velocity whether or not advection is on
+    sp.stage_k = fl.diff;
+    sp.stage_s = fl.src;
+    sp.stage_b = fl.absorp;
+    sp.kcomp = DIM * DIM;
+    const size_t head = sizeof(R) * (size_t)P.slot_cap * (LOC + 1), bb = slab_lds_bytes<DIM, R>(sp);
+    const int per = (P.nblocks + 7) / 8;
+    const size_t lds = ((head + 15) & ~(size_t)15) + bb;
+    if (lds + sizeof(LTab<R>) + 64 > 160 * 1024)
+        return set_error(FLGPU_EUNSUPPORTED, "stabilised blocked assembly: a slab needs %zu bytes of LDS", lds);
+    FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_staged_stab<DIM, R>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_asm_staged_stab<DIM, R>), dim3(8 * per), dim3(kStagedThreads), lds, c->stream, sp, img,
                        c->X4.p, fd, fl, c->A.p, c->rhs.p);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
@@ -1305,8 +1425,6 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
         if (!(fl.nu_scale >= 0.0))
             return set_error(FLGPU_EINVAL, "flgpu assemble: nu_bar scale %g < 0", fl.nu_scale);
         if (!c->fU.set) return set_error(FLGPU_EINVAL, "flgpu assemble: stabilisation needs the velocity field");
-        if (p->scheme == FLGPU_ASM_BLOCKED)
-            return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: stabilised assembly runs on the colour scheme");
     }
     AsmFields fd;
     fd.T = c->fT.v.p;
@@ -1319,23 +1437,27 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     const bool fp32 = p->precision == FLGPU_FP32;
     int rc = FLGPU_OK;
     int scheme = p->scheme;
-    // the blocked scheme: linear T; in FP32 with the node-staged kernel
-    // only.  Chosen automatically, it gives way to the colour scheme when
-    // the mesh does not fit its plan (a node in more elements than a slab
-    // holds, a row longer than 255 entries).
-    bool blocked_ok = c->linear && !fl.stab;
-    if (blocked_ok && (fp32 || scheme == FLGPU_ASM_AUTO)) {
+    // the blocked scheme: linear T; in FP32, and with stabilisation, with
+    // the node-staged kernels only.  Chosen automatically, it gives way to
+    // the colour scheme when the mesh does not fit its plan (a node in more
+    // elements than a slab holds, a row longer than 255 entries).  The
+    // colour scheme asked for by name needs no plan.
+    bool blocked_ok = c->linear && scheme != FLGPU_ASM_COLOUR;
+    if (blocked_ok && (fp32 || fl.stab || scheme == FLGPU_ASM_AUTO)) {
         const int rc = build_block_plan(c);
         if (rc == FLGPU_EUNSUPPORTED && scheme == FLGPU_ASM_AUTO)
             blocked_ok = false;
         else if (rc != FLGPU_OK)
             return rc;
+        else if (fl.stab)
+            blocked_ok = c->plan.staged;
         else if (fp32)
             blocked_ok = stageable(c, fl);
     }
     if (scheme == FLGPU_ASM_BLOCKED && !blocked_ok)
-        return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: the blocked scheme covers linear T (in FP32: with "
-                                             "standard P1 tables and isotropic diffusivity)");
+        return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: the blocked scheme covers linear T (in FP32 and with "
+                                             "stabilisation: with standard P1 tables and the field on the "
+                                             "coordinate mesh; in FP32 unstabilised: isotropic diffusivity)");
     if (scheme == FLGPU_ASM_AUTO) scheme = blocked_ok ? FLGPU_ASM_BLOCKED : FLGPU_ASM_COLOUR;
     if (scheme != FLGPU_ASM_COLOUR && scheme != FLGPU_ASM_BLOCKED)
         return set_error(FLGPU_EINVAL, "flgpu assemble: scheme %d", p->scheme);
@@ -1348,6 +1470,10 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     }
     if (scheme == FLGPU_ASM_BLOCKED) {
         FLGPU_TRY(build_block_plan(c));
+        if (fl.stab) {
+            if (dim == 3) return fp32 ? launch_staged_stab<3, float>(c, fd, fl) : launch_staged_stab<3, double>(c, fd, fl);
+            return fp32 ? launch_staged_stab<2, float>(c, fd, fl) : launch_staged_stab<2, double>(c, fd, fl);
+        }
         return dim == 3 ? launch_blocked<3>(c, fd, fl, d_err, fp32) : launch_blocked<2>(c, fd, fl, d_err, fp32);
     }
     FLGPU_HIP(hipMemsetAsync(c->A.p, 0, sizeof(double) * c->nnz, c->stream));
