@@ -183,6 +183,7 @@ static int table_image(flgpu_ctx* c, const uint32_t** out) {
 // per-mesh table.  The values are those of row_stab, bit for bit.
 template <int DIM, int LOC, typename R>
 struct StabLdsQuad : NodalQuad<DIM, LOC, R> {
+    static constexpr bool nu_reads_k = false;
     const R* su;   // u at the points, [g][d]
     const R* snu;  // nu_bar at the points
     FLGPU_HD_INLINE void u(int g, R (&uq)[DIM]) const {
@@ -649,8 +650,8 @@ k_asm_blocked(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int n
 
 // Nodal field values from the slab's LDS copy (a constant field is staged
 // as its one value at every node).  The diffusivity is staged as its first
-// component (isotropic, the unstabilised kernel) or as the whole tensor
-// (the stabilised kernel, whose nu_bar reads every component).
+// component (isotropic) or as the whole tensor (the stabilised kernel with
+// anisotropic diffusivity).
 template <int DIM, typename R>
 struct StagedNodal {
     const int* lid;
@@ -767,7 +768,8 @@ __device__ __forceinline__ void load_node_vals(const StagedPlan& sp, const doubl
             for (int a = 0; a < DIM; a++) v.u[a] = fd.U[fd.u_c ? a : (int64_t)vn * DIM + a];
         if (sp.stage_k)
 #pragma unroll
-            for (int a = 0; a < KC; a++) v.k[a] = fd.K[(fd.k_c ? 0 : (int64_t)vn * DIM * DIM) + a];
+            for (int a = 0; a < KC; a++)
+                if (a < sp.kcomp) v.k[a] = fd.K[(fd.k_c ? 0 : (int64_t)vn * DIM * DIM) + a];
         if (sp.stage_s) v.s = fd.S[fd.s_c ? 0 : vn];
         if (sp.stage_b) v.b = fd.B[fd.b_c ? 0 : vn];
     }
@@ -830,7 +832,8 @@ __device__ __forceinline__ void store_node_vals(const StagedPlan& sp, const Node
         for (int a = 0; a < DIM; a++) L.sU[a * nc + q] = (R)v.u[a];
     if (sp.stage_k)
 #pragma unroll
-        for (int a = 0; a < KC; a++) L.sK[a * nc + q] = (R)v.k[a];
+        for (int a = 0; a < KC; a++)
+            if (a < sp.kcomp) L.sK[a * nc + q] = (R)v.k[a];
     if (sp.stage_s) L.sS[q] = (R)v.s;
     if (sp.stage_b) L.sB[q] = (R)v.b;
 }
@@ -843,7 +846,7 @@ struct StagedRowLinear {
     const AsmFlags& fl;
     const SlabLds<R>& L;
     int nc;
-    __device__ __forceinline__ void operator()(const Elem<DIM, DIM + 1, R>& e, const int (&lid)[DIM + 1], int i,
+    __device__ __forceinline__ void operator()(const Elem<DIM, DIM + 1, R>& e, const int (&lid)[DIM + 1], int i, int,
                                                R (&vals)[DIM + 1], R& r) const {
         GeomLin<DIM, DIM + 1, R> gm;
         geom_linear_p1<DIM, DIM + 1, R>(e, gm);
@@ -852,24 +855,40 @@ struct StagedRowLinear {
     }
 };
 
+// The fields at the quadrature points from the staged nodal values, and
+// nu_bar from the per-element array k_nu_points wrote.
+template <int DIM, int LOC, typename R>
+struct StagedNuQuad : NodalQuad<DIM, LOC, R, StagedNodal<DIM, R>> {
+    static constexpr bool nu_reads_k = false;
+    const R* __restrict__ nu_e;  // nu_bar at the element's points
+    __device__ __forceinline__ R nu(int g, const R (&)[DIM], const R (&)[DIM][DIM], const R (&)[DIM][DIM],
+                                    const AsmFlags&) const {
+        return nu_e[g];
+    }
+};
+
 // The element row with streamline-upwind / SUPG stabilisation: the
-// Jacobian from the coordinate tables, velocity, nu_bar and the fields at
-// every quadrature point from the staged nodal values (row_stab_q, as the
-// colour scheme's stabilised kernel computes them, so the same bits).
+// Jacobian from the coordinate tables, velocity and the fields at every
+// quadrature point from the staged nodal values, nu_bar from the
+// per-element array (row_stab_q with the values the colour scheme's
+// stabilised kernel computes, so the same bits).
 template <int DIM, typename R>
 struct StagedRowStab {
     const LTab<R>& tb;
     const AsmFlags& fl;
     const SlabLds<R>& L;
     int nc;
+    const int32_t* __restrict__ inc;  // element of every slot
+    const R* __restrict__ nub;        // nu_bar per element and point
     __device__ __forceinline__ void operator()(const Elem<DIM, DIM + 1, R>& e, const int (&lid)[DIM + 1], int i,
-                                               R (&vals)[DIM + 1], R& r) const {
+                                               int slot, R (&vals)[DIM + 1], R& r) const {
         constexpr int LOC = DIM + 1;
+        const int ele = inc[slot];
         R JT[DIM][DIM], iJ[DIM][DIM], adet;
         geometry_jt<DIM, R>(e.X, tb, JT, iJ, adet);
         const StagedNodal<DIM, R> na{lid, L.sU, L.sK, L.sS, L.sB, nc};
-        row_stab_q<DIM, LOC, R>(tb, fl, JT, iJ, adet, NodalQuad<DIM, LOC, R, StagedNodal<DIM, R>>{na, tb, iJ}, e.T,
-                                i, vals, r);
+        const StagedNuQuad<DIM, LOC, R> qv{{na, tb, iJ}, nub + (int64_t)ele * tb.ngi};
+        row_stab_q<DIM, LOC, R, StagedNuQuad<DIM, LOC, R>, R[LOC], true>(tb, fl, JT, iJ, adet, qv, e.T, i, vals, r);
     }
 };
 
@@ -906,7 +925,7 @@ __device__ __forceinline__ void staged_phase_a(const ROW& row, const StagedPlan&
             e.T[k] = L.sT[lid[k]];
         }
         R vals[LOC], r;
-        row(e, lid, i, vals, r);
+        row(e, lid, i, m.s0 + sl, vals, r);
         const int ks = L.soff[l];
         R* cr = sc + LOC * ks;
         const int dg = L.sdg[l] + (sl - ks);
@@ -979,14 +998,49 @@ k_asm_staged(StagedPlan sp, const double* __restrict__ X4, AsmFields fd, AsmFlag
     staged_phase_b<LOC, R>(m, L, sc, srhs, A, rhs, tid);
 }
 
+// nu_bar at every quadrature point of every element, once per element (the
+// staged rows of an element's vertices read it instead of each computing
+// it): the values the colour scheme's stabilised kernel computes, from the
+// same gathered coordinates, velocity and diffusivity.
+template <int DIM, typename R>
+__global__ void __launch_bounds__(256)
+k_nu_points(const uint32_t* __restrict__ ltab, int nele, const int32_t* __restrict__ conn,
+            const int32_t* __restrict__ conn_x, const double* __restrict__ X4, AsmFields fd, AsmFlags fl,
+            R* __restrict__ nub) {
+    constexpr int LOC = DIM + 1;
+    __shared__ LTab<R> tb;
+    stage_table_image<R, 256>(ltab, tb);
+    __syncthreads();
+    const int ele = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (ele >= nele) return;
+    Elem<DIM, LOC, R> e;
+    gather_elem<DIM, LOC, R>(ele, conn, conn_x, X4, fd.T, e);
+    R JT[DIM][DIM], iJ[DIM][DIM], adet;
+    geometry_jt<DIM, R>(e.X, tb, JT, iJ, adet);
+    const GlobalNodal<DIM, R> na{fd, e.nodes};
+    const NodalQuad<DIM, LOC, R> nq{na, tb, iJ};
+    for (int g = 0; g < tb.ngi; g++) {
+        R uq[DIM], kt[DIM][DIM];
+        nq.u(g, uq);
+#pragma unroll
+        for (int a = 0; a < DIM; a++)
+#pragma unroll
+            for (int b = 0; b < DIM; b++) kt[a][b] = (R)0;
+        if (fl.diff) nq.k_tensor(g, kt);
+        nub[(int64_t)ele * tb.ngi + g] = nq.nu(g, uq, JT, kt, fl);
+    }
+}
+
 // The node-staged slab kernel with stabilisation: the same slabs, plan and
 // summation order, the full element tables (the quadrature points' shape
-// values and weights) in LDS beside the slab, the whole diffusivity tensor
-// staged per node (nu_bar reads every component).
+// values and weights) in LDS beside the slab, nu_bar from k_nu_points, the
+// diffusivity staged per node as its first component (isotropic) or the
+// whole tensor.
 template <int DIM, typename R>
 __global__ void __launch_bounds__(kStagedThreads, 2)
-k_asm_staged_stab(StagedPlan sp, const uint32_t* __restrict__ ltab, const double* __restrict__ X4, AsmFields fd,
-                  AsmFlags fl, double* __restrict__ A, double* __restrict__ rhs) {
+k_asm_staged_stab(StagedPlan sp, const uint32_t* __restrict__ ltab, const int32_t* __restrict__ inc,
+                  const R* __restrict__ nub, const double* __restrict__ X4, AsmFields fd, AsmFlags fl,
+                  double* __restrict__ A, double* __restrict__ rhs) {
     constexpr int LOC = DIM + 1;
     __shared__ LTab<R> tb;
     extern __shared__ double dyn[];
@@ -1007,7 +1061,7 @@ k_asm_staged_stab(StagedPlan sp, const uint32_t* __restrict__ ltab, const double
     store_slab_words(w, m, L, tid);
     store_node_vals<DIM, DIM * DIM, R>(sp, nv, m.NN, L, tid);
     __syncthreads();
-    staged_phase_a<DIM, R>(StagedRowStab<DIM, R>{tb, fl, L, sp.node_cap}, sp, m, L, w, sc, srhs, tid);
+    staged_phase_a<DIM, R>(StagedRowStab<DIM, R>{tb, fl, L, sp.node_cap, inc, nub}, sp, m, L, w, sc, srhs, tid);
     __syncthreads();
     staged_phase_b<LOC, R>(m, L, sc, srhs, A, rhs, tid);
 }
@@ -1317,7 +1371,8 @@ static int launch_staged_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags&
 }
 
 // Stabilised assembly on the node-staged slabs (the plan's node lists and
-// standard P1 tables); any diffusivity, the whole tensor staged.
+// standard P1 tables); any diffusivity.  nu_bar once per element and
+// quadrature point (k_nu_points), then the slabs.
 template <int DIM, typename R>
 static int launch_staged_stab(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl) {
     const BlockPlan& P = c->plan;
@@ -1343,16 +1398,22 @@ static int launch_staged_stab(flgpu_ctx* c, const AsmFields& fd, const AsmFlags&
     sp.stage_k = fl.diff;
     sp.stage_s = fl.src;
     sp.stage_b = fl.absorp;
-    sp.kcomp = DIM * DIM;
+    // an isotropic row reads the first component only (nu_bar comes from
+    // k_nu_points); an anisotropic one the whole tensor
+    sp.kcomp = fl.iso ? 1 : DIM * DIM;
     const size_t head = sizeof(R) * (size_t)P.slot_cap * (LOC + 1), bb = slab_lds_bytes<DIM, R>(sp);
     const int per = (P.nblocks + 7) / 8;
     const size_t lds = ((head + 15) & ~(size_t)15) + bb;
     if (lds + sizeof(LTab<R>) + 64 > 160 * 1024)
         return set_error(FLGPU_EUNSUPPORTED, "stabilised blocked assembly: a slab needs %zu bytes of LDS", lds);
+    FLGPU_TRY(c->plan.nub.reserve(((size_t)c->nele * c->tab.ngi * sizeof(R) + 7) / 8));
+    R* nub = reinterpret_cast<R*>(c->plan.nub.p);
+    hipLaunchKernelGGL((k_nu_points<DIM, R>), dim3(grid_for(c->nele, 256)), dim3(256), 0, c->stream, img, c->nele,
+                       c->conn.p, c->conn_x.p, c->X4.p, fd, fl, nub);
     FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_staged_stab<DIM, R>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL((k_asm_staged_stab<DIM, R>), dim3(8 * per), dim3(kStagedThreads), lds, c->stream, sp, img,
-                       c->X4.p, fd, fl, c->A.p, c->rhs.p);
+                       P.inc.p, nub, c->X4.p, fd, fl, c->A.p, c->rhs.p);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
@@ -1372,6 +1433,16 @@ static int launch_staged(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, 
 }
 
 static bool stageable(const flgpu_ctx* c, const AsmFlags& fl) { return c->plan.staged && (!fl.diff || fl.iso); }
+
+// The stabilised node-staged kernel forms the gradients once per row: the
+// tables' derivatives must be the same at every quadrature point.
+static bool gradients_constant(const ElemTables& t) {
+    for (int d = 0; d < t.dim; d++)
+        for (int g = 1; g < t.ngi; g++)
+            for (int n = 0; n < t.loc; n++)
+                if (t.dn[n + t.loc * (g + t.ngi * d)] != t.dn[n + t.loc * (0 + t.ngi * d)]) return false;
+    return true;
+}
 
 template <int DIM>
 static int launch_blocked(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err, bool fp32) {
@@ -1450,7 +1521,7 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
         else if (rc != FLGPU_OK)
             return rc;
         else if (fl.stab)
-            blocked_ok = c->plan.staged;
+            blocked_ok = c->plan.staged && gradients_constant(c->tab);
         else if (fp32)
             blocked_ok = stageable(c, fl);
     }

@@ -411,6 +411,9 @@ FLGPU_HD_INLINE R nu_bar_scaled(const R (&u)[DIM], const R (&JT)[DIM][DIM], cons
 
 template <int DIM, int LOC, typename R, class NA = GlobalNodal<DIM, R>>
 struct NodalQuad {
+    // nu() reads the diffusivity tensor (a source with nu_bar precomputed
+    // says false, and an isotropic row then skips the tensor)
+    static constexpr bool nu_reads_k = true;
     const NA& na;
     const LTab<R>& tb;
     const R (&iJ)[DIM][DIM];
@@ -784,12 +787,29 @@ FLGPU_HD_INLINE R nu_bar_scaled(const R (&u)[DIM], const R (&JT)[DIM][DIM], cons
 // terms, the diffusion and the by-parts u.grad N_i term keeping N_i; and
 // (:1110-1121, Upwind_Stabilisation.F90:52-84) for streamline upwind the
 // term nu_bar (u.grad N_i)(u.grad N_j) added to the advection matrix.
-template <int DIM, int LOC, typename R, class QV, class TV>
+// CG: the tables' shape derivatives are the same at every quadrature point
+// (linear T, checked on the host), so the physical gradients and the
+// velocity divergence are formed once, not per point: the same values.
+template <int DIM, int LOC, typename R, class QV, class TV, bool CG = false>
 FLGPU_HD_INLINE void row_stab_q(const LTab<R>& tb, const AsmFlags& fl, const R (&JT)[DIM][DIM],
                                 const R (&iJ)[DIM][DIM], R adet, const QV& qv, const TV& T, int i, R (&vals)[LOC],
                                 R& rhs_i) {
     const int ngi = tb.ngi;
     const bool supg = fl.stab == 2, su = fl.stab == 1 && fl.adv;
+    R Gc[CG ? LOC : 1][DIM];
+    R div_c = (R)0;
+    if constexpr (CG) {
+#pragma unroll
+        for (int j = 0; j < LOC; j++)
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                R s = (R)0;
+#pragma unroll
+                for (int k = 0; k < DIM; k++) s = FMA(tb.dn[j + LOC * (0 + ngi * k)], iJ[d][k], s);
+                Gc[j][d] = s;
+            }
+        if (fl.adv && fl.need_div) div_c = qv.div(0);
+    }
     R M[LOC], Sm[LOC];
 #pragma unroll
     for (int j = 0; j < LOC; j++) {
@@ -802,16 +822,27 @@ FLGPU_HD_INLINE void row_stab_q(const LTab<R>& tb, const AsmFlags& fl, const R (
         const R dw = adet * tb.w[g];
         const R ni = tb.n[i + LOC * g];
         R Gi[DIM];
+        if constexpr (CG) {
+            // select, not index: i is a run-time value
 #pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            R s = (R)0;
+            for (int d = 0; d < DIM; d++) {
+                R s = Gc[0][d];
 #pragma unroll
-            for (int k = 0; k < DIM; k++) s = FMA(tb.dn[i + LOC * (g + ngi * k)], iJ[d][k], s);
-            Gi[d] = s;
+                for (int j = 1; j < LOC; j++) s = i == j ? Gc[j][d] : s;
+                Gi[d] = s;
+            }
+        } else {
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                R s = (R)0;
+#pragma unroll
+                for (int k = 0; k < DIM; k++) s = FMA(tb.dn[i + LOC * (g + ngi * k)], iJ[d][k], s);
+                Gi[d] = s;
+            }
         }
         R uq[DIM];
         qv.u(g, uq);
-        const R div = (fl.adv && fl.need_div) ? qv.div(g) : (R)0;
+        const R div = (fl.adv && fl.need_div) ? (CG ? div_c : qv.div(g)) : (R)0;
         R kt[DIM][DIM];
 #pragma unroll
         for (int a = 0; a < DIM; a++)
@@ -819,7 +850,7 @@ FLGPU_HD_INLINE void row_stab_q(const LTab<R>& tb, const AsmFlags& fl, const R (
             for (int b = 0; b < DIM; b++) kt[a][b] = (R)0;
         R kq = (R)0;
         if (fl.diff) {
-            qv.k_tensor(g, kt);
+            if (!fl.iso || QV::nu_reads_k) qv.k_tensor(g, kt);
             if (fl.iso) kq = qv.k_iso(g);
         }
         const R nu = qv.nu(g, uq, JT, kt, fl);
@@ -850,10 +881,14 @@ FLGPU_HD_INLINE void row_stab_q(const LTab<R>& tb, const AsmFlags& fl, const R (
             R Gj[DIM];
 #pragma unroll
             for (int d = 0; d < DIM; d++) {
-                R s = (R)0;
+                if constexpr (CG) {
+                    Gj[d] = Gc[j][d];
+                } else {
+                    R s = (R)0;
 #pragma unroll
-                for (int k = 0; k < DIM; k++) s = FMA(tb.dn[j + LOC * (g + ngi * k)], iJ[d][k], s);
-                Gj[d] = s;
+                    for (int k = 0; k < DIM; k++) s = FMA(tb.dn[j + LOC * (g + ngi * k)], iJ[d][k], s);
+                    Gj[d] = s;
+                }
             }
             if (fl.adv) {
                 R ugj = (R)0;

@@ -145,6 +145,9 @@ struct BlockPlan {
     bool epos_built = false;
     DevBuf<uint32_t> ltab64, ltab32;
     std::vector<uint32_t> h_ltab64, h_ltab32;  // what the images hold
+    // stabilised node-staged assembly: nu_bar per element and quadrature
+    // point (FP64 or FP32 values, as 8-byte words)
+    DevBuf<double> nub;
 };
 
 // Surface elements of T for the face loop (surface.hip).
