@@ -650,15 +650,21 @@ k_asm_blocked(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int n
 
 // Nodal field values from the slab's LDS copy (a constant field is staged
 // as its one value at every node).  The diffusivity is staged as its first
-// component (isotropic) or as the whole tensor (the stabilised kernel with
-// anisotropic diffusivity).
-template <int DIM, typename R>
+// component (isotropic: every component reads it, so a row compiled for
+// both cases reads one value) or, for the stabilised kernel with
+// anisotropic diffusivity, as the whole tensor (TENSOR: component c at
+// c * kstride; kstride 0 when only the first is staged).
+template <int DIM, typename R, bool TENSOR = false>
 struct StagedNodal {
     const int* lid;
     const R *sU, *sK, *sS, *sB;
     int ncap;
+    int kstride = 0;
     __device__ __forceinline__ R U(int k, int d) const { return (R)sU[d * ncap + lid[k]]; }
-    __device__ __forceinline__ R K(int k, int comp) const { return (R)sK[comp * ncap + lid[k]]; }
+    __device__ __forceinline__ R K(int k, int comp) const {
+        if constexpr (TENSOR) return (R)sK[comp * kstride + lid[k]];
+        return (R)sK[lid[k]];
+    }
     __device__ __forceinline__ R S(int k) const { return (R)sS[lid[k]]; }
     __device__ __forceinline__ R B(int k) const { return (R)sB[lid[k]]; }
 };
@@ -769,7 +775,7 @@ __device__ __forceinline__ void load_node_vals(const StagedPlan& sp, const doubl
         if (sp.stage_k)
 #pragma unroll
             for (int a = 0; a < KC; a++)
-                if (a < sp.kcomp) v.k[a] = fd.K[(fd.k_c ? 0 : (int64_t)vn * DIM * DIM) + a];
+                if (KC == 1 || a < sp.kcomp) v.k[a] = fd.K[(fd.k_c ? 0 : (int64_t)vn * DIM * DIM) + a];
         if (sp.stage_s) v.s = fd.S[fd.s_c ? 0 : vn];
         if (sp.stage_b) v.b = fd.B[fd.b_c ? 0 : vn];
     }
@@ -833,7 +839,7 @@ __device__ __forceinline__ void store_node_vals(const StagedPlan& sp, const Node
     if (sp.stage_k)
 #pragma unroll
         for (int a = 0; a < KC; a++)
-            if (a < sp.kcomp) L.sK[a * nc + q] = (R)v.k[a];
+            if (KC == 1 || a < sp.kcomp) L.sK[a * nc + q] = (R)v.k[a];
     if (sp.stage_s) L.sS[q] = (R)v.s;
     if (sp.stage_b) L.sB[q] = (R)v.b;
 }
@@ -858,7 +864,7 @@ struct StagedRowLinear {
 // The fields at the quadrature points from the staged nodal values, and
 // nu_bar from the per-element array k_nu_points wrote.
 template <int DIM, int LOC, typename R>
-struct StagedNuQuad : NodalQuad<DIM, LOC, R, StagedNodal<DIM, R>> {
+struct StagedNuQuad : NodalQuad<DIM, LOC, R, StagedNodal<DIM, R, true>> {
     static constexpr bool nu_reads_k = false;
     const R* __restrict__ nu_e;  // nu_bar at the element's points
     __device__ __forceinline__ R nu(int g, const R (&)[DIM], const R (&)[DIM][DIM], const R (&)[DIM][DIM],
@@ -878,6 +884,7 @@ struct StagedRowStab {
     const AsmFlags& fl;
     const SlabLds<R>& L;
     int nc;
+    int kstride;                      // diffusivity component stride in LDS (0: first only)
     const int32_t* __restrict__ inc;  // element of every slot
     const R* __restrict__ nub;        // nu_bar per element and point
     __device__ __forceinline__ void operator()(const Elem<DIM, DIM + 1, R>& e, const int (&lid)[DIM + 1], int i,
@@ -886,7 +893,7 @@ struct StagedRowStab {
         const int ele = inc[slot];
         R JT[DIM][DIM], iJ[DIM][DIM], adet;
         geometry_jt<DIM, R>(e.X, tb, JT, iJ, adet);
-        const StagedNodal<DIM, R> na{lid, L.sU, L.sK, L.sS, L.sB, nc};
+        const StagedNodal<DIM, R, true> na{lid, L.sU, L.sK, L.sS, L.sB, nc, kstride};
         const StagedNuQuad<DIM, LOC, R> qv{{na, tb, iJ}, nub + (int64_t)ele * tb.ngi};
         row_stab_q<DIM, LOC, R, StagedNuQuad<DIM, LOC, R>, R[LOC], true>(tb, fl, JT, iJ, adet, qv, e.T, i, vals, r);
     }
@@ -1061,7 +1068,9 @@ k_asm_staged_stab(StagedPlan sp, const uint32_t* __restrict__ ltab, const int32_
     store_slab_words(w, m, L, tid);
     store_node_vals<DIM, DIM * DIM, R>(sp, nv, m.NN, L, tid);
     __syncthreads();
-    staged_phase_a<DIM, R>(StagedRowStab<DIM, R>{tb, fl, L, sp.node_cap, inc, nub}, sp, m, L, w, sc, srhs, tid);
+    const int kstride = sp.kcomp > 1 ? sp.node_cap : 0;
+    staged_phase_a<DIM, R>(StagedRowStab<DIM, R>{tb, fl, L, sp.node_cap, kstride, inc, nub}, sp, m, L, w, sc, srhs,
+                           tid);
     __syncthreads();
     staged_phase_b<LOC, R>(m, L, sc, srhs, A, rhs, tid);
 }
