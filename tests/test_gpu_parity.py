@@ -147,6 +147,30 @@ def test_auto_scheme_falls_back_to_colour(require_gpu):
     A, rhs, _ = pr.matrix()
     assert rel(A, ref["A"]) < FP64_RTOL
     assert rel(rhs, ref["rhs"]) < FP64_RTOL
+    # FP32 element arithmetic: the colour scheme, named or chosen, needs no plan
+    for scheme in ("colour", "auto"):
+        print(f"\n[case] fan fp32 {scheme}", flush=True)
+        cg.assemble_advection_diffusion_cg(pr, cg.options_from_config(cfg, precision="fp32", scheme=scheme), cfg.dt)
+        cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
+        A32, rhs32, _ = pr.matrix()
+        assert rel(A32, ref["A"]) < FP32_RTOL
+        assert rel(rhs32, ref["rhs"]) < FP32_RTOL
+    # SUPG: the automatic scheme gives way to the colour kernel as well
+    print("\n[case] fan supg", flush=True)
+    var = {"stabilisation": "supg"}
+    ref_s = O.reference_system_opts(cfg, var)
+    for scheme in ("blocked", "auto"):
+        opts = cg.options_from_config(cfg, scheme=scheme)
+        opts.stabilisation = "supg"
+        if scheme == "blocked":
+            with pytest.raises(FlgpuError):
+                cg.assemble_advection_diffusion_cg(pr, opts, cfg.dt)
+            continue
+        cg.assemble_advection_diffusion_cg(pr, opts, cfg.dt)
+        cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
+        A_s, rhs_s, _ = pr.matrix()
+        assert rel(A_s, ref_s["A"]) < FP64_RTOL
+        assert rel(rhs_s, ref_s["rhs"]) < FP64_RTOL
     pr.close()
 
 
