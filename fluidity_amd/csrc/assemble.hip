@@ -878,7 +878,7 @@ struct StagedNuQuad : NodalQuad<DIM, LOC, R, StagedNodal<DIM, R, true>> {
 // quadrature point from the staged nodal values, nu_bar from the
 // per-element array (row_stab_q with the values the colour scheme's
 // stabilised kernel computes, so the same bits).
-template <int DIM, typename R>
+template <int DIM, typename R, int FEAT>
 struct StagedRowStab {
     const LTab<R>& tb;
     const AsmFlags& fl;
@@ -895,7 +895,8 @@ struct StagedRowStab {
         geometry_jt<DIM, R>(e.X, tb, JT, iJ, adet);
         const StagedNodal<DIM, R, true> na{lid, L.sU, L.sK, L.sS, L.sB, nc, kstride};
         const StagedNuQuad<DIM, LOC, R> qv{{na, tb, iJ}, nub + (int64_t)ele * tb.ngi};
-        row_stab_q<DIM, LOC, R, StagedNuQuad<DIM, LOC, R>, R[LOC], true>(tb, fl, JT, iJ, adet, qv, e.T, i, vals, r);
+        row_stab_q<DIM, LOC, R, StagedNuQuad<DIM, LOC, R>, R[LOC], true, FEAT>(tb, fl, JT, iJ, adet, qv, e.T, i, vals,
+                                                                               r);
     }
 };
 
@@ -1043,8 +1044,8 @@ k_nu_points(const uint32_t* __restrict__ ltab, int nele, const int32_t* __restri
 // values and weights) in LDS beside the slab, nu_bar from k_nu_points, the
 // diffusivity staged per node as its first component (isotropic) or the
 // whole tensor.
-template <int DIM, typename R>
-__global__ void __launch_bounds__(kStagedThreads, 2)
+template <int DIM, typename R, int FEAT>
+__global__ void __launch_bounds__(kStagedThreads, (FEAT & (F_ABS | F_ANISO)) ? 2 : 3)
 k_asm_staged_stab(StagedPlan sp, const uint32_t* __restrict__ ltab, const int32_t* __restrict__ inc,
                   const R* __restrict__ nub, const double* __restrict__ X4, AsmFields fd, AsmFlags fl,
                   double* __restrict__ A, double* __restrict__ rhs) {
@@ -1069,8 +1070,8 @@ k_asm_staged_stab(StagedPlan sp, const uint32_t* __restrict__ ltab, const int32_
     store_node_vals<DIM, DIM * DIM, R>(sp, nv, m.NN, L, tid);
     __syncthreads();
     const int kstride = sp.kcomp > 1 ? sp.node_cap : 0;
-    staged_phase_a<DIM, R>(StagedRowStab<DIM, R>{tb, fl, L, sp.node_cap, kstride, inc, nub}, sp, m, L, w, sc, srhs,
-                           tid);
+    staged_phase_a<DIM, R>(StagedRowStab<DIM, R, FEAT>{tb, fl, L, sp.node_cap, kstride, inc, nub}, sp, m, L, w, sc,
+                           srhs, tid);
     __syncthreads();
     staged_phase_b<LOC, R>(m, L, sc, srhs, A, rhs, tid);
 }
@@ -1419,12 +1420,20 @@ static int launch_staged_stab(flgpu_ctx* c, const AsmFields& fd, const AsmFlags&
     R* nub = reinterpret_cast<R*>(c->plan.nub.p);
     hipLaunchKernelGGL((k_nu_points<DIM, R>), dim3(grid_for(c->nele, 256)), dim3(256), 0, c->stream, img, c->nele,
                        c->conn.p, c->conn_x.p, c->X4.p, fd, fl, nub);
-    FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_staged_stab<DIM, R>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_asm_staged_stab<DIM, R>), dim3(8 * per), dim3(kStagedThreads), lds, c->stream, sp, img,
-                       P.inc.p, nub, c->X4.p, fd, fl, c->A.p, c->rhs.p);
-    FLGPU_HIP(hipGetLastError());
-    return FLGPU_OK;
+    // the common term set (no absorption, isotropic or no diffusivity)
+    // compiled without the others, which frees registers for a third
+    // workgroup per CU; the full set otherwise
+    auto run = [&](auto kern) -> int {
+        FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+        hipLaunchKernelGGL(kern, dim3(8 * per), dim3(kStagedThreads), lds, c->stream, sp, img, P.inc.p,
+                           (const R*)nub, (const double*)c->X4.p, fd, fl, c->A.p, c->rhs.p);
+        FLGPU_HIP(hipGetLastError());
+        return FLGPU_OK;
+    };
+    if (!fl.absorp && (!fl.diff || fl.iso))
+        return run(&k_asm_staged_stab<DIM, R, F_MASS | F_ADV | F_DIFF | F_SRC>);
+    return run(&k_asm_staged_stab<DIM, R, F_ALL | F_ANISO>);
 }
 
 // The smallest compiled feature set that holds the terms switched on; the

@@ -54,7 +54,7 @@ struct AsmFields {
 // its bit is set here and its flag is set at run time; the arithmetic of a
 // term does not depend on the set, so every instantiation gives the same
 // bits for the same flags.
-enum : int { F_MASS = 1, F_ADV = 2, F_ABS = 4, F_DIFF = 8, F_SRC = 16, F_ALL = 31 };
+enum : int { F_MASS = 1, F_ADV = 2, F_ABS = 4, F_DIFF = 8, F_SRC = 16, F_ALL = 31, F_ANISO = 32 };
 
 // Tables in LDS, converted to the arithmetic type R.
 template <typename R>
@@ -790,10 +790,19 @@ FLGPU_HD_INLINE R nu_bar_scaled(const R (&u)[DIM], const R (&JT)[DIM][DIM], cons
 // CG: the tables' shape derivatives are the same at every quadrature point
 // (linear T, checked on the host), so the physical gradients and the
 // velocity divergence are formed once, not per point: the same values.
-template <int DIM, int LOC, typename R, class QV, class TV, bool CG = false>
-FLGPU_HD_INLINE void row_stab_q(const LTab<R>& tb, const AsmFlags& fl, const R (&JT)[DIM][DIM],
+// FEAT: the terms compiled in (F_ANISO: anisotropic diffusivity); the caller
+// picks a set that holds the flags switched on.
+template <int DIM, int LOC, typename R, class QV, class TV, bool CG = false, int FEAT = F_ALL | F_ANISO>
+FLGPU_HD_INLINE void row_stab_q(const LTab<R>& tb, const AsmFlags& fl_in, const R (&JT)[DIM][DIM],
                                 const R (&iJ)[DIM][DIM], R adet, const QV& qv, const TV& T, int i, R (&vals)[LOC],
                                 R& rhs_i) {
+    AsmFlags fl = fl_in;
+    fl.have_mass = (FEAT & F_MASS) ? fl.have_mass : 0;
+    fl.adv = (FEAT & F_ADV) ? fl.adv : 0;
+    fl.absorp = (FEAT & F_ABS) ? fl.absorp : 0;
+    fl.diff = (FEAT & F_DIFF) ? fl.diff : 0;
+    fl.src = (FEAT & F_SRC) ? fl.src : 0;
+    fl.iso = (FEAT & F_ANISO) ? fl.iso : 1;
     const int ngi = tb.ngi;
     const bool supg = fl.stab == 2, su = fl.stab == 1 && fl.adv;
     R Gc[CG ? LOC : 1][DIM];
