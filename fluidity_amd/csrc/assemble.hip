@@ -1010,12 +1010,13 @@ k_asm_staged(StagedPlan sp, const double* __restrict__ X4, AsmFields fd, AsmFlag
 // staged rows of an element's vertices read it instead of each computing
 // it): the values the colour scheme's stabilised kernel computes, from the
 // same gathered coordinates, velocity and diffusivity.
-template <int DIM, typename R>
+template <int DIM, typename R, int SCHEME>
 __global__ void __launch_bounds__(256)
 k_nu_points(const uint32_t* __restrict__ ltab, int nele, const int32_t* __restrict__ conn,
             const int32_t* __restrict__ conn_x, const double* __restrict__ X4, AsmFields fd, AsmFlags fl,
             R* __restrict__ nub) {
     constexpr int LOC = DIM + 1;
+    fl.nu_scheme = SCHEME;  // one scheme compiled in: the others' branches fold away
     __shared__ LTab<R> tb;
     stage_table_image<R, 256>(ltab, tb);
     __syncthreads();
@@ -1418,8 +1419,18 @@ static int launch_staged_stab(flgpu_ctx* c, const AsmFields& fd, const AsmFlags&
         return set_error(FLGPU_EUNSUPPORTED, "stabilised blocked assembly: a slab needs %zu bytes of LDS", lds);
     FLGPU_TRY(c->plan.nub.reserve(((size_t)c->nele * c->tab.ngi * sizeof(R) + 7) / 8));
     R* nub = reinterpret_cast<R*>(c->plan.nub.p);
-    hipLaunchKernelGGL((k_nu_points<DIM, R>), dim3(grid_for(c->nele, 256)), dim3(256), 0, c->stream, img, c->nele,
-                       c->conn.p, c->conn_x.p, c->X4.p, fd, fl, nub);
+    auto nu_pass = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid_for(c->nele, 256)), dim3(256), 0, c->stream, img, c->nele,
+                           (const int32_t*)c->conn.p, (const int32_t*)c->conn_x.p, (const double*)c->X4.p, fd, fl,
+                           nub);
+    };
+    switch (fl.nu_scheme) {
+        case FLGPU_NU_BAR_OPTIMAL: nu_pass(&k_nu_points<DIM, R, FLGPU_NU_BAR_OPTIMAL>); break;
+        case FLGPU_NU_BAR_DOUBLY_ASYMPTOTIC: nu_pass(&k_nu_points<DIM, R, FLGPU_NU_BAR_DOUBLY_ASYMPTOTIC>); break;
+        case FLGPU_NU_BAR_CRITICAL_RULE: nu_pass(&k_nu_points<DIM, R, FLGPU_NU_BAR_CRITICAL_RULE>); break;
+        default: nu_pass(&k_nu_points<DIM, R, FLGPU_NU_BAR_UNITY>); break;
+    }
+    FLGPU_HIP(hipGetLastError());
     // the common term set (no absorption, isotropic or no diffusivity)
     // compiled without the others, which frees registers for a third
     // workgroup per CU; the full set otherwise
