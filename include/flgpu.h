@@ -167,8 +167,9 @@ typedef struct {
     /* spatial_discretisation/continuous_galerkin/stabilisation
      * (assemble/Advection_Diffusion_CG.F90:440-465): FLGPU_STAB_*, the
      * nu_bar scheme (FLGPU_NU_BAR_*, assemble/Upwind_Stabilisation.F90:
-     * 18-19) and nu_scale.  Stabilised assembly runs on the colour scheme;
-     * it needs the velocity field. */
+     * 18-19) and nu_scale.  Stabilised assembly needs the velocity field;
+     * FLGPU_ASM_BLOCKED takes it for linear T on the coordinate mesh (the
+     * same bits as FLGPU_ASM_COLOUR), the colour scheme for any degree. */
     int32_t stabilisation, nu_bar_scheme;
     double nu_bar_scale;
 } flgpu_advdif_params;
