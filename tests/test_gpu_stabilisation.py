@@ -80,8 +80,10 @@ def test_stabilised_assembly_fp32(require_gpu):
     assert rel(rhs, ref["rhs"]) < FP32_RTOL
 
 
-def assembled(cfg, variant, scheme, precision="fp64"):
+def assembled(cfg, variant, scheme, precision="fp64", const_u=None):
     pr = cg.problem_from_config(cfg)
+    if const_u is not None:  # a constant field: one value, read as such by every kernel
+        pr.set_fields(velocity=np.asarray(const_u, float))
     opts = cg.options_from_config(cfg, precision=precision, scheme=scheme)
     for k, v in variant.items():
         setattr(opts, k, v)
@@ -103,6 +105,8 @@ def assembled(cfg, variant, scheme, precision="fp64"):
     ("c2adv", 6, {"stabilisation": "supg", "have_absorption": True}, "fp64"),
     ("cube1", None, {"stabilisation": "supg"}, "fp64"),
     ("c2adv", 6, {"stabilisation": "supg"}, "fp32"),
+    ("c2adv", 6, {"stabilisation": "supg", "const_u": [1.0, 0.5, 0.25]}, "fp64"),
+    ("c1", 12, {"stabilisation": "streamline_upwind", "const_u": [0.4, -0.7]}, "fp64"),
     ("c1", 12, {"stabilisation": "streamline_upwind"}, "fp32"),
 ])
 def test_staged_stabilised_equals_colour_bitwise(require_gpu, name, scale, variant, precision):
@@ -120,9 +124,10 @@ def test_staged_stabilised_equals_colour_bitwise(require_gpu, name, scale, varia
         K[:, 0, 1] = K[:, 1, 0] = 2e-3 * pos[:, 2]
         cfg.kappa = K
         var["isotropic_diffusivity"] = False
+    const_u = var.pop("const_u", None)
     print(f"\n[case] {name} staged vs colour {variant} {precision}", flush=True)
-    A_c, b_c = assembled(cfg, var, "colour", precision)
-    A_s, b_s = assembled(cfg, var, "blocked", precision)
+    A_c, b_c = assembled(cfg, var, "colour", precision, const_u)
+    A_s, b_s = assembled(cfg, var, "blocked", precision, const_u)
     assert np.array_equal(A_s, A_c)
     assert np.array_equal(b_s, b_c)
     assert np.count_nonzero(A_s) > 0.5 * A_s.size
