@@ -6,10 +6,14 @@ toward z = 0.5, log-normal diffusivity (contrast 1e3), f = sin sin sin,
 Dirichlet 0, synthetic inputs from a splitmix64 seed.
 
 A step = one assembly of the whole mesh (matrix + rhs) with every input
-resident in HBM.  K timed steps give `value` (Melem/s).  After the steps the
-assembled system is solved to a true relative residual of 1e-8 twice:
-plain FP64 Jacobi-PCG and the mixed-precision (FP32 inner, FP64 refinement)
-solve; both times are reported.
+resident in HBM.  K timed steps give `value` (Melem/s).  After the steps:
+the solves to a true relative residual of 1e-8 (FP64 and mixed Jacobi-PCG,
+learned FCG in FP64 and mixed), the SpMV with warm and with flushed caches,
+the full-size parity leg against the oracle (matrix, rhs, pattern, blocked
+== colour bit for bit, solutions against the oracle's FP64 solution), the
+rows beyond the element loop, the 2-D and 3-D P1 configurations, the P2
+mixed-precision configuration, the held-out learned-preconditioner system,
+and the oracle timed on the host cores (median of 5 runs).
 
 Prints ONE JSON line.
 """
@@ -135,6 +139,8 @@ def c4_heldout_report(weights, log_each) -> dict:
     for label, so in (
         ("fp64_jacobi_pcg", cg.SolverOptions("cg", "jacobi", relative_error=1e-8, norm="unpreconditioned",
                                              max_iterations=50000, check_every=32)),
+        ("mixed_jacobi_pcg", cg.SolverOptions("cg", "jacobi", relative_error=1e-8, norm="unpreconditioned",
+                                              max_iterations=50000, precision="mixed", check_every=32)),
         ("learned_fp64_fcg", cg.SolverOptions("fcg", "learned", relative_error=1e-8, norm="unpreconditioned",
                                               max_iterations=50000, check_every=32)),
         ("learned_mixed_fcg", cg.SolverOptions("fcg", "learned", relative_error=1e-8, norm="unpreconditioned",
@@ -149,6 +155,7 @@ def c4_heldout_report(weights, log_each) -> dict:
     j, lm = out["fp64_jacobi_pcg"], out["learned_mixed_fcg"]
     out["iteration_ratio_jacobi_over_learned"] = round(j["its"] / max(out["learned_fp64_fcg"]["its"], 1), 3)
     out["speedup_learned_mixed_vs_fp64_jacobi"] = round(j["seconds"] / lm["seconds"], 3)
+    out["speedup_learned_mixed_vs_mixed_jacobi"] = round(out["mixed_jacobi_pcg"]["seconds"] / lm["seconds"], 3)
     log_each(f"c4 held-out: jacobi {j['its']} its {j['seconds']:.4f}s, learned {out['learned_fp64_fcg']['its']} its, "
              f"learned+mixed {lm['seconds']:.4f}s ({out['speedup_learned_mixed_vs_fp64_jacobi']}x)")
     return out
@@ -157,9 +164,11 @@ def c4_heldout_report(weights, log_each) -> dict:
 def next_rows_report(cfg, pr, log_each) -> dict:
     """Timings of the rows beyond the element assembly and the solve, on the
     same mesh: the face loop (Neumann on x = 0, Robin on x = 1), the
-    SUPG-stabilised assembly (a constant velocity field is set for it), and
-    the sparsity and colouring built on the GPU against the host routines.
-    Leaves the context with the GPU-built sparsity and colouring bound."""
+    SUPG-stabilised assembly (a constant velocity field is set for it; the
+    node-staged and colour schemes compared bit for bit), and the sparsity
+    and colouring built on the GPU against the host routines, with the
+    host-side block-plan build.  Leaves the context with the GPU-built
+    sparsity and colouring bound."""
     import ctypes as C
     from fluidity_amd import cg, lib as L, meshgen as M
     lib = L.load()
@@ -186,17 +195,36 @@ def next_rows_report(cfg, pr, log_each) -> dict:
     pr.set_fields(velocity=np.array([1.0, 0.5, 0.25]))
     ms = C.c_double(0)
     for stab in ("supg", "streamline_upwind"):
+        mats = {}
         for scheme in ("blocked", "colour"):  # node-staged slabs, then the colour scheme
             so = cg.options_from_config(cfg, scheme=scheme)
             so.exclude_advection_terms = False
             so.stabilisation = stab
             ps = so.params(cfg.dt)
             L.check(lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(ps)), "assemble " + stab)
+            mats[scheme] = pr.matrix()[:2]
             L.check(lib.flgpu_time_assemble(pr.ctx, C.byref(ps), 5, C.byref(ms)), "time " + stab)
             key = f"assembly_{stab}" + ("" if scheme == "blocked" else "_colour_scheme")
             rep[key] = {"ms": round(ms.value, 4), "Melem/s": round(cfg.mesh.nele / ms.value / 1e3, 2)}
             log_each(f"assembly with {stab} ({scheme} scheme): {ms.value:.3f} ms "
                      f"({cfg.mesh.nele / ms.value / 1e3:.1f} Melem/s)")
+        same = bool(np.array_equal(mats["blocked"][0], mats["colour"][0]) and
+                    np.array_equal(mats["blocked"][1], mats["colour"][1]))
+        rep[f"assembly_{stab}"]["blocked_equals_colour_bitwise"] = same
+        log_each(f"{stab}: node-staged == colour scheme bit for bit: {same}")
+    # the host-side block plan (slabs, slots, places), rebuilt after every
+    # adapt: a fresh bind of the same mesh, then two assemblies; the first
+    # builds the plan
+    pr2 = cg.problem_from_config(cfg)
+    pa = cg.options_from_config(cfg).params(cfg.dt)
+    t0 = time.perf_counter()
+    L.check(lib.flgpu_advdif_assemble_resident(pr2.ctx, C.byref(pa)), "assemble")
+    t1 = time.perf_counter()
+    L.check(lib.flgpu_advdif_assemble_resident(pr2.ctx, C.byref(pa)), "assemble")
+    t2 = time.perf_counter()
+    pr2.close()
+    rep["block_plan_build_s"] = round((t1 - t0) - (t2 - t1), 4)
+    log_each(f"block plan build (host, once per mesh): {rep['block_plan_build_s']:.3f}s")
     # set-up: host routines, then the GPU ones (bound afterwards)
     t0 = time.perf_counter()
     cg.make_sparsity(cfg.mesh)
@@ -219,27 +247,129 @@ def next_rows_report(cfg, pr, log_each) -> dict:
     return rep
 
 
-def cpu_baseline(cfg, budget_s: float = 20.0) -> dict:
-    """The oracle (C restatement, OpenMP over the colour loop like the
-    reference) timed on this host: the assembly of the whole mesh and a
-    bounded sample of Jacobi-PCG iterations."""
+def rel(a, b) -> float:
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def oracle_mod():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    return O
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t0 = time.time()
-    ref = O.reference_system(cfg, nthreads=threads)
-    asm_s = ref["asm_seconds"]
-    its = 50
-    x, k, reason, relres, secs = O.petsc_solve("cg", "jacobi", 1e-30, ref["sparsity"], ref["A"], ref["inactive"],
-                                               ref["rhs"], max_its=its, nthreads=threads)
-    per_it = secs / max(k, 1)
-    log(f"cpu baseline: assembly {asm_s:.3f}s, PCG {per_it * 1e3:.2f} ms/it ({threads} threads), "
-        f"wall {time.time() - t0:.1f}s")
-    return {"value": cfg.mesh.nele / asm_s / 1e6, "unit": "Melem/s", "cores": threads, "kind": "port",
-            "sample": f"oracle assembly of the full mesh (colour loop, {threads} OpenMP threads) + "
-                      f"{k} Jacobi-PCG iterations ({per_it * 1e3:.2f} ms/iteration)",
-            "pcg_ms_per_iteration": per_it * 1e3}
+
+def threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def spmv_report(lib, L, pr, nnodes, nnz, log_each) -> dict:
+    """The solver-matrix SpMV, FP64 and FP32 values: warm (50 products back
+    to back) and cold (each product after a 1 GiB flush of the caches)."""
+    import ctypes as C
+    ms = C.c_double(0)
+    out = {}
+    for prec, vb in (("fp64", 8), ("mixed", 4)):
+        code = L.FLGPU_FP64 if prec == "fp64" else L.FLGPU_MIXED
+        L.check(lib.flgpu_solver_prepare(pr.ctx, code), "prep")
+        L.check(lib.flgpu_time_spmv(pr.ctx, code, 50, C.byref(ms)), "time_spmv")
+        warm = ms.value
+        L.check(lib.flgpu_time_spmv_cold(pr.ctx, code, 10, 1024, C.byref(ms)), "time_spmv_cold")
+        cold = ms.value
+        b = spmv_bytes(nnodes, nnz, vb)
+        gw, gc = b / (warm * 1e-3) / 1e9, b / (cold * 1e-3) / 1e9
+        out[f"spmv_{prec}"] = {"ms": round(warm, 4), "GB/s": round(gw, 1), "frac": round(gw / PEAK_HBM_GBS, 4),
+                               "cold_ms": round(cold, 4), "cold_GB/s": round(gc, 1),
+                               "cold_frac": round(gc / PEAK_HBM_GBS, 4), "algorithmic_bytes": b}
+        log_each(f"spmv {prec}: warm {warm:.4f} ms ({gw:.0f} GB/s), cold {cold:.4f} ms ({gc:.0f} GB/s)")
+    return out
+
+
+def p1_config_report(name, log_each) -> dict:
+    """A P1 configuration at its stated size (C1: 64^2 x 2 triangles; C2:
+    64^3 x 6 tets): assembly Melem/s against its HBM roofline, the matrix
+    and rhs against the oracle, SpMV warm and cold, and the configuration's
+    solve (C1: GMRES(30) and Jacobi-PCG to 1e-10; C2: Jacobi-PCG to 1e-8)
+    against the HBM floor of its iterations."""
+    import ctypes as C
+    from fluidity_amd import cg, lib as L, meshgen as M
+    O = oracle_mod()
+    lib = L.load()
+    cfg = M.config(name, seed=1)
+    pr = cg.problem_from_config(cfg)
+    nnz = pr.sparsity.nnz
+    p = cg.options_from_config(cfg).params(cfg.dt)
+    ms = C.c_double(0)
+    for _ in range(3):
+        L.check(lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(p)), "assemble")
+    L.check(lib.flgpu_time_assemble(pr.ctx, C.byref(p), 20, C.byref(ms)), "time_assemble")
+    B = asm_bytes(cfg, nnz)
+    gbs = B / (ms.value * 1e-3) / 1e9
+    out = {"elements": cfg.mesh.nele, "nodes": cfg.mesh.nnodes, "nnz": nnz,
+           "assembly": {"ms": round(ms.value, 4), "Melem/s": round(cfg.mesh.nele / ms.value / 1e3, 2),
+                        "GB/s": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4), "algorithmic_bytes": B}}
+    ref = O.reference_system(cfg, nthreads=threads())
+    cg.assemble_advection_diffusion_cg(pr, cg.options_from_config(cfg), cfg.dt)
+    cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
+    A, rhs, inact = pr.matrix()
+    f, c, _ = ref["sparsity"]
+    out["parity"] = {"pattern_bit_exact": bool(np.array_equal(pr.sparsity.findrm, f) and
+                                               np.array_equal(pr.sparsity.colm, c)),
+                     "A_rel_err_vs_oracle": rel(A, ref["A"]), "rhs_rel_err_vs_oracle": rel(rhs, ref["rhs"]),
+                     "inactive_equal": bool(np.array_equal(inact, ref["inactive"]))}
+    # C1 is non-symmetric (advection): GMRES(30); C2: Jacobi-PCG
+    solves = ([("fp64_gmres", cg.SolverOptions("gmres", "jacobi", relative_error=1e-10, restart=30))] if name == "c1"
+              else [("fp64_jacobi_pcg", cg.SolverOptions("cg", "jacobi", relative_error=cfg.rtol,
+                                                         norm="unpreconditioned", max_iterations=50000))])
+    b_it = spmv_bytes(cfg.mesh.nnodes, nnz, 8) + 128 * cfg.mesh.nnodes
+    for label, so in solves:
+        cg.petsc_solve(pr, so)
+        t0 = time.perf_counter()
+        x, info = cg.petsc_solve(pr, so)
+        tot = time.perf_counter() - t0
+        d = {"seconds": round(tot, 5), "krylov_seconds": round(info.seconds, 5), "its": info.its,
+             "reason": info.reason, "true_relres": info.true_relres}
+        if label == "fp64_jacobi_pcg":  # Jacobi-PCG iterations against their HBM floor
+            floor = info.its * b_it / (PEAK_HBM_GBS * 1e9)
+            d["hbm_floor_seconds"] = round(floor, 6)
+            d["roofline_frac"] = round(floor / max(info.seconds, 1e-12), 4)
+        xo, oits, _, _, _ = O.petsc_solve("gmres" if name == "c1" else "cg", "jacobi", 1e-12, ref["sparsity"], ref["A"],
+                                          ref["inactive"], ref["rhs"], max_its=50000, nthreads=threads())
+        d["x_rel_err_vs_oracle_1e-12"] = rel(x, xo)
+        out[label] = d
+    out.update(spmv_report(lib, L, pr, cfg.mesh.nnodes, nnz, lambda m: None))
+    pr.close()
+    a = out["assembly"]
+    sv = out[solves[0][0]]
+    log_each(f"{name}: assembly {a['ms']:.4f} ms ({a['Melem/s']:.0f} Melem/s, {a['frac']:.3f} of HBM), "
+             f"A err {out['parity']['A_rel_err_vs_oracle']:.1e}, {solves[0][0]} {sv['its']} its "
+             f"{sv['seconds']:.4f}s, x err {sv['x_rel_err_vs_oracle_1e-12']:.1e}")
+    return out
+
+
+def cpu_baseline(cfg, ref, log_each) -> dict:
+    """The oracle (C restatement, OpenMP over the colour loop like the
+    reference) on this host's cores: the assembly of the whole mesh, one
+    warm-up then the median of 5 runs; and the time to solution of FP64
+    Jacobi-PCG to a 1e-8 relative residual (the reference's KSPCG default
+    preconditioned norm), the median of 5 runs."""
+    O = oracle_mod()
+    nt = threads()
+    O.assemble(ref["params"], ref["asm"], nt)
+    asm = [O.assemble(ref["params"], ref["asm"], nt)[2] for _ in range(5)]
+    sol = []
+    its = 0
+    for _ in range(5):
+        t0 = time.perf_counter()
+        _, its, reason, relres, _ = O.petsc_solve("cg", "jacobi", 1e-8, ref["sparsity"], ref["A"], ref["inactive"],
+                                                  ref["rhs"], max_its=50000, nthreads=nt)
+        sol.append(time.perf_counter() - t0)
+    a, s = float(np.median(asm)), float(np.median(sol))
+    log_each(f"cpu baseline ({nt} threads): assembly median {a:.3f}s, PCG to 1e-8 median {s:.3f}s ({its} its)")
+    return {"value": round(cfg.mesh.nele / a / 1e6, 3), "unit": "Melem/s", "cores": nt, "kind": "port",
+            "sample": f"oracle assembly of the whole mesh (colour loop, {nt} OpenMP threads), median of 5 runs after "
+                      f"1 warm-up; oracle Jacobi-PCG to a 1e-8 relative residual, median of 5 runs",
+            "assembly_seconds_median": round(a, 4), "assembly_seconds_runs": [round(t, 4) for t in asm],
+            "pcg_time_to_solution_s_median": round(s, 4), "pcg_its": its,
+            "pcg_ms_per_iteration": round(s / max(its, 1) * 1e3, 3)}
 
 
 def main() -> int:
@@ -249,11 +379,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="c5")
     ap.add_argument("--scale", type=int, default=None)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle: no parity leg, no CPU baseline")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--scheme", default="auto")
-    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the C3 and C4 configurations")
     ap.add_argument("--no-next", action="store_true")
+    ap.add_argument("--no-p1", action="store_true", help="skip the C1 and C2 configurations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (sharding the path "
@@ -282,20 +413,29 @@ def main() -> int:
     B = asm_bytes(cfg, nnz)
     achieved = B / t_asm / 1e9
     log(f"assembly: {ms.value:.3f} ms/step, {melems:.1f} Melem/s, {achieved:.1f} GB/s algorithmic")
+    L.check(lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(p)), "assemble")
+    A_blocked, rhs_blocked, _ = pr.matrix()
     schemes = {}
     if args.scheme == "auto":
         p_col = cg.options_from_config(cfg, scheme="colour").params(cfg.dt)
         L.check(lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(p_col)), "assemble colour")
+        A_col, rhs_col, _ = pr.matrix()
         L.check(lib.flgpu_time_assemble(pr.ctx, C.byref(p_col), max(2, args.steps // 4), C.byref(ms)), "time")
-        schemes["colour"] = {"ms": round(ms.value, 4), "Melem/s": round(cfg.mesh.nele / ms.value / 1e3, 2)}
-        log(f"assembly colour scheme: {ms.value:.3f} ms/step")
+        schemes["colour"] = {"ms": round(ms.value, 4), "Melem/s": round(cfg.mesh.nele / ms.value / 1e3, 2),
+                             "equals_blocked_bitwise": bool(np.array_equal(A_col, A_blocked) and
+                                                            np.array_equal(rhs_col, rhs_blocked))}
+        del A_col, rhs_col
+        log(f"assembly colour scheme: {ms.value:.3f} ms/step, bitwise equal to the blocked scheme: "
+            f"{schemes['colour']['equals_blocked_bitwise']}")
         # FP32 element arithmetic (mixed precision), values widened into the FP64 matrix
         p32 = cg.options_from_config(cfg, precision="fp32").params(cfg.dt)
         for _ in range(3):
             L.check(lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(p32)), "assemble fp32")
+        A32, rhs32, _ = pr.matrix()
         L.check(lib.flgpu_time_assemble(pr.ctx, C.byref(p32), args.steps, C.byref(ms)), "time")
         schemes["blocked_fp32_elements"] = {"ms": round(ms.value, 4),
-                                            "Melem/s": round(cfg.mesh.nele / ms.value / 1e3, 2)}
+                                            "Melem/s": round(cfg.mesh.nele / ms.value / 1e3, 2),
+                                            "A32": A32, "rhs32": rhs32}
         log(f"assembly with FP32 element arithmetic: {ms.value:.3f} ms/step")
         L.check(lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(p)), "assemble")
 
@@ -321,15 +461,17 @@ def main() -> int:
                      "traffic": measured_traffic(("void flgpu::k_asm_staged", "void flgpu::k_asm_blocked")),
                      "kernel": "k_asm_staged / k_asm_blocked (one launch per step)", "algorithmic_bytes": B},
     }
+    A32 = schemes.get("blocked_fp32_elements", {}).pop("A32", None)
+    rhs32 = schemes.get("blocked_fp32_elements", {}).pop("rhs32", None)
     if schemes:
         out["assembly_other_schemes"] = schemes
 
+    xs = {}
     if not args.no_solve:
         cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
         solves = {}
         from fluidity_amd import learned
         learned.set_params(pr, learned.load())
-        xs = {}
         for label, so in (
             ("fp64_jacobi_pcg", cg.SolverOptions("cg", "jacobi", relative_error=1e-8, norm="unpreconditioned",
                                                  max_iterations=50000, check_every=32)),
@@ -365,27 +507,64 @@ def main() -> int:
             f"{j['roofline_frac']:.2f} of roofline")
         solves["speedup_learned_mixed_vs_fp64_jacobi"] = round(
             solves["fp64_jacobi_pcg"]["seconds"] / solves["learned_mixed_fcg"]["seconds"], 3)
-        log(f"speedup learned+mixed vs FP64 Jacobi-PCG: {solves['speedup_learned_mixed_vs_fp64_jacobi']}x")
-        for prec, vb in (("fp64", 8), ("mixed", 4)):
-            L.check(lib.flgpu_solver_prepare(pr.ctx, L.FLGPU_FP64 if prec == "fp64" else L.FLGPU_MIXED), "prep")
-            L.check(lib.flgpu_time_spmv(pr.ctx, L.FLGPU_FP64 if prec == "fp64" else L.FLGPU_MIXED, 50,
-                                        C.byref(ms)), "time_spmv")
-            gbs = spmv_bytes(cfg.mesh.nnodes, nnz, vb) / (ms.value * 1e-3) / 1e9
-            solves[f"spmv_{prec}"] = {"ms": round(ms.value, 4), "GB/s": round(gbs, 1),
-                                      "frac": round(gbs / PEAK_HBM_GBS, 4)}
-            log(f"spmv {prec}: {ms.value:.4f} ms, {gbs:.1f} GB/s")
+        solves["speedup_learned_mixed_vs_mixed_jacobi"] = round(
+            solves["mixed_jacobi_pcg"]["seconds"] / solves["learned_mixed_fcg"]["seconds"], 3)
+        log(f"speedup learned+mixed vs FP64 Jacobi-PCG: {solves['speedup_learned_mixed_vs_fp64_jacobi']}x, "
+            f"vs mixed Jacobi-PCG: {solves['speedup_learned_mixed_vs_mixed_jacobi']}x")
+        solves.update(spmv_report(lib, L, pr, cfg.mesh.nnodes, nnz, log))
         out["solve"] = solves
+
+    ref_sys = None
+    if not args.no_cpu:
+        # full-size parity against the oracle (after the timed region)
+        O = oracle_mod()
+        t0 = time.time()
+        ref_sys = O.reference_system(cfg, nthreads=threads())
+        f, c, _ = ref_sys["sparsity"]
+        cg.assemble_advection_diffusion_cg(pr, opts, cfg.dt)
+        cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
+        A, rhs, inact = pr.matrix()
+        par = {"pattern_bit_exact": bool(np.array_equal(pr.sparsity.findrm, f) and
+                                         np.array_equal(pr.sparsity.colm, c)),
+               "inactive_equal": bool(np.array_equal(inact, ref_sys["inactive"])),
+               "A_rel_err_vs_oracle": rel(A, ref_sys["A"]), "rhs_rel_err_vs_oracle": rel(rhs, ref_sys["rhs"]),
+               "A_rtol": 1e-12}
+        # the matrix of the timed launches (Dirichlet changes only the rhs
+        # and the mask) against the oracle's
+        par["timed_assembly_A_rel_err_vs_oracle"] = rel(A_blocked, ref_sys["A"])
+        if "colour" in schemes:
+            par["blocked_equals_colour_bitwise"] = schemes["colour"]["equals_blocked_bitwise"]
+        if A32 is not None:
+            par["fp32_elements_A_rel_err_vs_oracle"] = rel(A32, ref_sys["A"])
+            par["fp32_elements_rhs_rel_err_vs_fp64"] = rel(rhs32, rhs_blocked)
+            par["fp32_rtol"] = 1e-5
+        del A, rhs, A_blocked, rhs_blocked, A32, rhs32
+        if xs:
+            xo, oits, oreason, orel, _ = O.petsc_solve("cg", "jacobi", 1e-12, ref_sys["sparsity"], ref_sys["A"],
+                                                       ref_sys["inactive"], ref_sys["rhs"], max_its=50000,
+                                                       nthreads=threads())
+            par["oracle_fp64_pcg"] = {"rtol": 1e-12, "its": oits, "reason": oreason, "true_relres": orel}
+            par["x_bound"] = 1e-8
+            for label, x in xs.items():
+                par[f"x_rel_err_{label}_vs_oracle"] = rel(x, xo)
+        out["parity_full"] = par
+        log(f"parity (full size, {time.time() - t0:.1f}s): pattern {par['pattern_bit_exact']}, A err "
+            f"{par['A_rel_err_vs_oracle']:.2e}, rhs err {par['rhs_rel_err_vs_oracle']:.2e}"
+            + (f", learned+mixed x err {par['x_rel_err_learned_mixed_fcg_vs_oracle']:.2e}" if xs else ""))
 
     if not args.no_next:
         out["next_rows"] = next_rows_report(cfg, pr, log)
+    pr.close()
+    if not args.no_p1:
+        out["c1"] = p1_config_report("c1", log)
+        out["c2"] = p1_config_report("c2", log)
     if not args.no_c3:
         out["c3_mixed_precision"] = c3_mixed_report(log)
         if not args.no_solve:
             from fluidity_amd import learned
             out["c4_heldout"] = c4_heldout_report(learned.load(), log)
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg)
-    pr.close()
+    if ref_sys is not None:
+        out["cpu_baseline"] = cpu_baseline(cfg, ref_sys, log)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -145,6 +145,7 @@ class CGProblem:
                  colouring: tuple | None = None, setup: str = "host"):
         self.lib = L.load()
         self.ctx = C.c_void_p()
+        self.sig_int = False  # set when a solve failed and the run should dump and finish
         L.check(self.lib.flgpu_init(C.byref(self.ctx), device), "flgpu_init")
         self.bind(mesh, element, quad_degree, sparsity, colouring, setup)
 
@@ -379,14 +380,24 @@ def convergence_check(problem: CGProblem, info, name: str = "", x0: np.ndarray |
 
 
 def solve_field_equation_cg(problem: CGProblem, opts: CGOptions, solver: SolverOptions, dt: float,
-                            bc_nodes: np.ndarray | None = None, bc_values=0.0, bcs: SurfaceBCs | None = None):
+                            bc_nodes: np.ndarray | None = None, bc_values=0.0, bcs: SurfaceBCs | None = None,
+                            name: str = "Tracer", matrixdump: str = "matrixdump",
+                            ignore_all_solver_failures: bool = False):
     """One solve_field_equation_cg step: assemble (elements, then faces
-    with bcs), strong Dirichlet, solve for the rate of change and update T
-    (T += dt*dT).  Returns (T_new, iterations_taken, info)."""
+    with bcs), strong Dirichlet, solve for the rate of change, check the
+    solve as petsc_solve does (ConvergenceCheck, Solvers.F90:1369-1479: a
+    failure is reported, the first one writes the matrix dump and sets
+    problem.sig_int, the "dump and finish" signal of Signal_Vars) and update
+    T (T += dt*dT, which the reference applies whatever the outcome,
+    Advection_Diffusion_CG.F90:1389-1407).  Returns (T_new,
+    iterations_taken, info)."""
     assemble_advection_diffusion_cg(problem, opts, dt, bcs)
     if bc_nodes is not None and bc_nodes.size:
         apply_dirichlet_conditions(problem, bc_nodes, bc_values, dt)
     dT, info = petsc_solve(problem, solver)
+    if convergence_check(problem, info, name, filename=matrixdump,
+                         ignore_all_solver_failures=ignore_all_solver_failures):
+        problem.sig_int = True
     T_new = problem.T + dt * dT
     return T_new, info.its, info
 

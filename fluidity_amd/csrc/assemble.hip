@@ -79,11 +79,21 @@ __device__ __forceinline__ void gather_elem(int ele, const int32_t* __restrict__
                                             const double* __restrict__ T, Elem<DIM, LOC, R>& e) {
 #pragma unroll
     for (int k = 0; k < LOC; k++) e.nodes[k] = conn[(int64_t)ele * LOC + k];
+    // vertices relative to the last one, differenced in FP64 (exact
+    // geometry for either precision: an FP32 difference of two rounded
+    // coordinates loses the digits a graded mesh's small elements need)
+    const int xl = conn_x[(int64_t)ele * (DIM + 1) + DIM];
+    double xo[DIM];
 #pragma unroll
-    for (int nn = 0; nn < DIM + 1; nn++) {
+    for (int a = 0; a < DIM; a++) {
+        xo[a] = X4[(int64_t)xl * 4 + a];
+        e.X[a][DIM] = (R)0;
+    }
+#pragma unroll
+    for (int nn = 0; nn < DIM; nn++) {
         const int xn = conn_x[(int64_t)ele * (DIM + 1) + nn];
 #pragma unroll
-        for (int a = 0; a < DIM; a++) e.X[a][nn] = (R)X4[(int64_t)xn * 4 + a];
+        for (int a = 0; a < DIM; a++) e.X[a][nn] = (R)(X4[(int64_t)xn * 4 + a] - xo[a]);
     }
 #pragma unroll
     for (int k = 0; k < LOC; k++) e.T[k] = (R)T[e.nodes[k]];
@@ -418,10 +428,12 @@ k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict_
                 for (int cpt = 0; cpt < kst; cpt++) n_k[i * kst + cpt] = fld<R>(fd.K, fd.k_c, nd, cpt, DIM * DIM);
             if (fl.src) n_s[i] = fld<R>(fd.S, fd.s_c, nd, 0, 1);
             if (fl.absorp) n_b[i] = fld<R>(fd.B, fd.b_c, nd, 0, 1);
-            if (i < DIM + 1) {
+            if (i < DIM + 1) {  // relative to the last vertex, differenced in FP64 (as gather_elem)
                 const int xn = conn_x[(int64_t)ele * (DIM + 1) + i];
+                const int xl = conn_x[(int64_t)ele * (DIM + 1) + DIM];
 #pragma unroll
-                for (int a = 0; a < DIM; a++) s_x[a * (DIM + 1) + i] = (R)X4[(int64_t)xn * 4 + a];
+                for (int a = 0; a < DIM; a++)
+                    s_x[a * (DIM + 1) + i] = (R)(X4[(int64_t)xn * 4 + a] - X4[(int64_t)xl * 4 + a]);
             }
         }
     }
@@ -785,16 +797,16 @@ __device__ __forceinline__ void load_node_vals(const StagedPlan& sp, const doubl
 // R), row slot offsets, row diagonal offsets, entry offsets.
 template <typename R>
 struct SlabLds {
-    R *sX, *sT, *sU, *sK, *sS, *sB;
+    double* sX;  // coordinates in FP64 for either precision (differenced before rounding)
+    R *sT, *sU, *sK, *sS, *sB;
     int32_t *soff, *sdg;
     uint32_t* scp;
 };
 
 template <int DIM, typename R>
 __host__ __device__ __forceinline__ size_t slab_lds_bytes(const StagedPlan& sp) {
-    const size_t nf =
-        (size_t)DIM + 1 + (sp.stage_u ? DIM : 0) + (sp.stage_k ? sp.kcomp : 0) + sp.stage_s + sp.stage_b;
-    const size_t b = sizeof(R) * nf * sp.node_cap +
+    const size_t nf = (size_t)1 + (sp.stage_u ? DIM : 0) + (sp.stage_k ? sp.kcomp : 0) + sp.stage_s + sp.stage_b;
+    const size_t b = sizeof(double) * DIM * sp.node_cap + sizeof(R) * nf * sp.node_cap +
                      sizeof(int32_t) * ((size_t)(sp.rows_cap + 1) + sp.rows_cap + ((sp.nnz_cap + 3) >> 1));
     return (b + 15) & ~(size_t)15;
 }
@@ -803,8 +815,8 @@ template <int DIM, typename R>
 __device__ __forceinline__ SlabLds<R> slab_lds(char* base, const StagedPlan& sp) {
     SlabLds<R> L;
     const int nc = sp.node_cap;
-    L.sX = reinterpret_cast<R*>(base);
-    L.sT = L.sX + (size_t)DIM * nc;
+    L.sX = reinterpret_cast<double*>(base);
+    L.sT = reinterpret_cast<R*>(L.sX + (size_t)DIM * nc);
     L.sU = L.sT + nc;
     L.sK = L.sU + (sp.stage_u ? (size_t)DIM * nc : 0);
     L.sS = L.sK + (sp.stage_k ? (size_t)sp.kcomp * nc : 0);
@@ -831,7 +843,7 @@ __device__ __forceinline__ void store_node_vals(const StagedPlan& sp, const Node
     if (q >= NN) return;
     const int nc = sp.node_cap;
 #pragma unroll
-    for (int a = 0; a < DIM; a++) L.sX[a * nc + q] = (R)v.x[a];
+    for (int a = 0; a < DIM; a++) L.sX[a * nc + q] = v.x[a];
     L.sT[q] = (R)v.t;
     if (sp.stage_u)
 #pragma unroll
@@ -928,9 +940,15 @@ __device__ __forceinline__ void staged_phase_a(const ROW& row, const StagedPlan&
 #pragma unroll
         for (int k = 0; k < LOC; k++) {
             lid[k] = (int)((vw >> (8 * k)) & 255u);
-#pragma unroll
-            for (int a = 0; a < DIM; a++) e.X[a][k] = L.sX[a * nc + lid[k]];
             e.T[k] = L.sT[lid[k]];
+        }
+        // vertices relative to the last one, differenced in FP64 (gather_elem)
+#pragma unroll
+        for (int a = 0; a < DIM; a++) {
+            const double xo = L.sX[a * nc + lid[DIM]];
+#pragma unroll
+            for (int k = 0; k < DIM; k++) e.X[a][k] = (R)(L.sX[a * nc + lid[k]] - xo);
+            e.X[a][DIM] = (R)0;
         }
         R vals[LOC], r;
         row(e, lid, i, m.s0 + sl, vals, r);
@@ -1492,6 +1510,11 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
         return set_error(FLGPU_EINVAL, "flgpu assemble: bind mesh, field mesh, sparsity and colouring first");
     if (c->rows != c->nnodes)
         return set_error(FLGPU_EINVAL, "flgpu assemble: sparsity rows %d != field nodes %d", c->rows, c->nnodes);
+    if (c->h_conn.size() != (size_t)c->nele * c->loc || c->h_conn_x.size() != (size_t)c->nele * c->loc_x ||
+        c->h_col_elems.size() != (size_t)c->nele || c->h_col_ptr.size() != (size_t)c->ncol + 1)
+        return set_error(FLGPU_EINVAL, "flgpu assemble: the field mesh or colouring was bound for another mesh "
+                                       "(%zu connectivity entries, %zu coloured elements for %d elements)",
+                         c->h_conn.size(), c->h_col_elems.size(), c->nele);
     if (!c->fT.set) return set_error(FLGPU_EINVAL, "flgpu assemble: field T not set");
     if (p->have_advection && !c->fU.set) return set_error(FLGPU_EINVAL, "flgpu assemble: velocity not set");
     if (p->have_diffusivity && !c->fK.set) return set_error(FLGPU_EINVAL, "flgpu assemble: diffusivity not set");
@@ -1615,6 +1638,7 @@ static int assemble_check(flgpu_ctx* c, int* d_err) {
 }
 
 int assemble(flgpu_ctx* c, const flgpu_advdif_params* p) {
+    c->assembled = false;  // set again only when this assembly succeeds
     DevBuf<int> d_err;
     FLGPU_TRY(d_err.alloc(1));
     FLGPU_HIP(hipMemsetAsync(d_err.p, 0, sizeof(int), c->stream));
@@ -1625,6 +1649,7 @@ int assemble(flgpu_ctx* c, const flgpu_advdif_params* p) {
 int time_assemble(flgpu_ctx* c, const flgpu_advdif_params* p, int reps, double* ms) {
     if (reps < 1) return set_error(FLGPU_EINVAL, "flgpu_time_assemble: reps < 1");
     krylov_invalidate(c);
+    c->assembled = false;
     DevBuf<int> d_err;
     FLGPU_TRY(d_err.alloc(1));
     FLGPU_HIP(hipMemsetAsync(d_err.p, 0, sizeof(int), c->stream));

@@ -49,6 +49,34 @@ __global__ void k_rhs_add(int n, const double* __restrict__ f, int f_const, doub
     rhs[i] = rhs[i] + (f_const ? f[0] : f[i]);
 }
 
+// A new coordinate mesh topology: the field mesh, its fields, the sparsity,
+// the colouring, the surface and the assembled system named the previous
+// mesh's elements and nodes; all are unbound (the calls that need them
+// then fail with a message until they are bound again).
+static void unbind_from_mesh(flgpu_ctx* c) {
+    c->loc = 0;
+    c->ngi = 0;
+    c->nnodes = 0;
+    c->linear = false;
+    c->h_conn.clear();
+    c->conn.release();
+    for (DevField* f : {&c->fT, &c->fU, &c->fK, &c->fS, &c->fA}) f->set = 0;
+    c->rows = 0;
+    c->cols = 0;
+    c->nnz = 0;
+    c->sparsity_gen++;
+    c->h_rowptr.clear();
+    c->h_col.clear();
+    c->ncol = 0;
+    c->h_col_ptr.clear();
+    c->h_col_elems.clear();
+    c->col_elems.release();
+    c->surf.bound = false;
+    c->have_inactive = false;
+    c->assembled = false;
+    krylov_invalidate(c);
+}
+
 }  // namespace flgpu
 
 using namespace flgpu;
@@ -114,6 +142,19 @@ int flgpu_mesh_bind(flgpu_ctx* c, int64_t key_id, int64_t key_event, int32_t dim
         std::vector<double> x4((size_t)nx * 4, 0.0);
         for (int64_t i = 0; i < nx; i++)
             for (int d = 0; d < dim; d++) x4[i * 4 + d] = X[i * dim + d];
+        // Moved coordinates on the same connectivity (a new event count, or
+        // a new key for the same topology) keep the field mesh, sparsity,
+        // colouring and surface; any other mesh unbinds everything derived
+        // from the previous one, so a later call cannot index the new
+        // arrays with the old element or node numbering.
+        const bool same_topology = dim == c->dim && loc_x == c->loc_x && nele == c->nele && nx == c->nx &&
+                                   cx == c->h_conn_x;
+        c->assembled = false;
+        c->key_id = -1;  // the device mirror is being replaced
+        c->plan.built = false;
+        c->plan.epos_built = false;
+        c->plan.fail_rc = 0;
+        if (!same_topology) unbind_from_mesh(c);
         FLGPU_TRY(c->conn_x.upload(cx.data(), cx.size(), c->stream));
         FLGPU_TRY(c->X4.upload(x4.data(), x4.size(), c->stream));
         FLGPU_HIP(hipStreamSynchronize(c->stream));
@@ -182,6 +223,22 @@ int flgpu_field_mesh_bind(flgpu_ctx* c, int32_t loc, int32_t ngi, int32_t nnodes
                             m += w[g] * n[i + loc * g] * n[j + loc * g] * n[k + loc * g];
                         t.that[i + 4 * (j + 4 * k)] = m;
                     }
+        // A field mesh of other nodes leaves the fields and the sparsity of
+        // the previous one unbound; other connectivity, the colouring too.
+        if (nnodes != c->nnodes) {
+            for (DevField* f : {&c->fT, &c->fU, &c->fK, &c->fS, &c->fA}) f->set = 0;
+            c->rows = c->cols = 0;
+            c->nnz = 0;
+            c->sparsity_gen++;
+        }
+        if (cn != c->h_conn) {
+            c->ncol = 0;
+            c->h_col_ptr.clear();
+            c->h_col_elems.clear();
+        }
+        c->assembled = false;
+        c->have_inactive = false;
+        krylov_invalidate(c);
         FLGPU_TRY(c->conn.upload(cn.data(), cn.size(), c->stream));
         FLGPU_TRY(c->d_tab.upload(&t, 1, c->stream));
         FLGPU_HIP(hipStreamSynchronize(c->stream));
@@ -597,6 +654,16 @@ int flgpu_time_spmv(flgpu_ctx* c, int32_t precision, int32_t reps, double* ms) {
         if (!c->assembled) return set_error(FLGPU_EINVAL, "flgpu_time_spmv: no system on the device");
         FLGPU_HIP(hipSetDevice(c->device));
         return time_spmv(c, precision, reps, ms);
+    });
+}
+
+int flgpu_time_spmv_cold(flgpu_ctx* c, int32_t precision, int32_t reps, int32_t flush_mib, double* ms) {
+    return guarded("flgpu_time_spmv_cold", [&]() -> int {
+        if (!c || !ms || reps < 1 || flush_mib < 1 || flush_mib > 8192)
+            return set_error(FLGPU_EINVAL, "flgpu_time_spmv_cold: bad argument");
+        if (!c->assembled) return set_error(FLGPU_EINVAL, "flgpu_time_spmv_cold: no system on the device");
+        FLGPU_HIP(hipSetDevice(c->device));
+        return time_spmv_cold(c, precision, reps, flush_mib, ms);
     });
 }
 

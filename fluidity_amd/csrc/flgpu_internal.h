@@ -243,6 +243,7 @@ void krylov_free(flgpu_ctx* c);
 void krylov_invalidate(flgpu_ctx* c);  // the assembled system changed
 int solver_prepare(flgpu_ctx* c, int precision);
 int time_spmv(flgpu_ctx* c, int precision, int reps, double* ms);
+int time_spmv_cold(flgpu_ctx* c, int precision, int reps, int flush_mib, double* ms);
 int csr_mult(flgpu_ctx* c, const double* x, double* y);
 int solution_get(flgpu_ctx* c, double* x);
 // learned.hip

@@ -193,4 +193,10 @@ program flgpu_driver
   write(*, '(i0,1x,i0,1x,i0,1x,i0,1x,i0,1x,es24.16,1x,es24.16,1x,es24.16)') &
        nn, nnz, ncol, its, reason, relres, sum(T), maxval(T)
   call flgpu_check(flgpu_finalize(ctx), "finalize")
+  ! ConvergenceCheck (Solvers.F90:1369-1479): a failed solve is reported
+  ! and the run ends ("dump and finish") with a nonzero status
+  if (reason <= 0) then
+     write(0, '(a,i0,a,i0)') 'WARNING: Failed to converge. PETSc reason ', reason, ', iterations ', its
+     stop 3
+  end if
 end program flgpu_driver

@@ -398,6 +398,12 @@ int flgpu_time_assemble(flgpu_ctx *ctx, const flgpu_advdif_params *p,
  * or flgpu_solver_prepare; *ms = mean device milliseconds per product. */
 int flgpu_time_spmv(flgpu_ctx *ctx, int32_t precision, int32_t reps,
                     double *ms);
+/* As flgpu_time_spmv with cold caches: before each product a kernel reads
+ * and writes flush_mib MiB (1..8192) of scratch memory, evicting the L2
+ * caches and the Infinity Cache; each product is timed alone; *ms = their
+ * mean. */
+int flgpu_time_spmv_cold(flgpu_ctx *ctx, int32_t precision, int32_t reps,
+                         int32_t flush_mib, double *ms);
 /* Build the reduced system of flgpu_solve without iterating (for timing). */
 int flgpu_solver_prepare(flgpu_ctx *ctx, int32_t precision);
 /* Number of CSR rows / nonzeros / colours bound (any pointer may be NULL). */

@@ -127,3 +127,79 @@ def test_source_added_directly_to_rhs(require_gpu, name, scale):
     pr.close()
     assert rel(A, ref["A"]) < 1e-12
     assert rel(rhs, ref["rhs"]) < 1e-12
+
+
+def test_coordinate_mesh_rebind_unbinds_derived_state(require_gpu):
+    """Binding a coordinate mesh of another topology unbinds the field mesh,
+    fields, sparsity and colouring of the previous one: the next assembly
+    is an error (not a read of the old numbering).  Moving the same mesh
+    (same connectivity, new event count) keeps them, and the assembly
+    follows the new coordinates."""
+    cfg = M.config("c2", seed=44, scale=5)
+    other = M.config("c2", seed=44, scale=4)
+    pr = cg.problem_from_config(cfg)
+    opts = cg.options_from_config(cfg)
+    cg.assemble_advection_diffusion_cg(pr, opts, cfg.dt)
+    A0 = pr.matrix()[0]
+    m = cfg.mesh
+    X2 = np.ascontiguousarray(m.X * 1.5, dtype=np.float64)
+    print("\n[case] moved coordinates, same connectivity", flush=True)
+    L.check(pr.lib.flgpu_mesh_bind(pr.ctx, 777, 1, m.dim, m.dim + 1, m.nele, m.nx, L.ptr(m.ndglno_x), L.ptr(X2)),
+            "flgpu_mesh_bind")
+    cg.assemble_advection_diffusion_cg(pr, opts, cfg.dt)
+    A1 = pr.matrix()[0]
+    # unit diffusivity stiffness scales with the length: 1.5 x the values
+    assert rel(A1, 1.5 * A0) < 1e-12
+    print("[case] coordinate mesh of another topology", flush=True)
+    o = other.mesh
+    L.check(pr.lib.flgpu_mesh_bind(pr.ctx, 778, 0, o.dim, o.dim + 1, o.nele, o.nx, L.ptr(o.ndglno_x),
+                                   L.ptr(np.ascontiguousarray(o.X, dtype=np.float64))), "flgpu_mesh_bind")
+    with pytest.raises(L.FlgpuError):
+        cg.assemble_advection_diffusion_cg(pr, opts, cfg.dt)
+    with pytest.raises(L.FlgpuError):
+        cg.petsc_solve(pr, cg.SolverOptions())
+    pr.close()
+
+
+@pytest.mark.parametrize("norm", ["preconditioned", "unpreconditioned"])
+def test_mixed_reason_agrees_with_true_residual_at_max_its(require_gpu, norm):
+    """Mixed precision: a solve cut at (or just before) the iteration where
+    the FP32 recurrence converges reports convergence only if the FP64 true
+    residual meets the tolerance, KSP_DIVERGED_ITS otherwise."""
+    cfg = M.config("c2", seed=45, scale=8)
+    pr = cg.problem_from_config(cfg)
+    cg.assemble_advection_diffusion_cg(pr, cg.options_from_config(cfg), cfg.dt)
+    cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
+    rtol = 1e-9
+    so = cg.SolverOptions("cg", "jacobi", relative_error=rtol, precision="mixed", norm=norm, check_every=1)
+    _, info = cg.petsc_solve(pr, so)
+    assert info.reason > 0 and info.true_relres <= rtol * (1 + 1e-9)
+    for cut in range(max(1, info.its - 3), info.its + 1):
+        so.max_iterations = cut
+        _, i2 = cg.petsc_solve(pr, so)
+        print(f"[case] max_its {cut}: its {i2.its} reason {i2.reason} relres {i2.true_relres:.3e}", flush=True)
+        if i2.reason > 0:
+            assert i2.true_relres <= rtol * (1 + 1e-9)
+        else:
+            assert i2.reason == -3
+    pr.close()
+
+
+def test_failed_step_sets_sig_int_and_dumps(require_gpu, tmp_path):
+    """solve_field_equation_cg runs ConvergenceCheck: a solve stopped at
+    max_its reports KSP_DIVERGED_ITS, writes the matrix dump once and sets
+    the dump-and-finish flag; T is still updated, as in the reference."""
+    from fluidity_amd import readnsolve
+    cfg = M.config("c2", seed=46, scale=6)
+    pr = cg.problem_from_config(cfg)
+    so = cg.SolverOptions("cg", "jacobi", relative_error=1e-12, max_iterations=1, check_every=1)
+    path = str(tmp_path / "matrixdump")
+    cg._matrixdumped = False
+    T1, its, info = cg.solve_field_equation_cg(pr, cg.options_from_config(cfg), so, cfg.dt, cfg.mesh.boundary, 0.0,
+                                               matrixdump=path)
+    assert info.reason == -3 and its == 1
+    assert pr.sig_int
+    assert np.all(np.isfinite(T1))
+    findrm, colm, val, b, x0 = readnsolve.matrixdump_read(path)
+    assert findrm.size == cfg.mesh.nnodes + 1 and np.all(x0 == 0.0)
+    pr.close()
