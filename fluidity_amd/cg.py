@@ -65,11 +65,18 @@ class CGOptions:
             nu_bar_scale=self.nu_scale)
 
 
+# preconditioner names of the solver options (setup_pc_from_options,
+# femtools/Solvers.F90:1844-1854): "eisenstat" is SSOR applied with
+# Eisenstat's trick, the same preconditioner as "sor" (one symmetric sweep).
+PRECONDITIONERS = {"none": L.FLGPU_PC_NONE, "jacobi": L.FLGPU_PC_JACOBI, "sor": L.FLGPU_PC_SOR,
+                   "eisenstat": L.FLGPU_PC_SOR, "learned": L.FLGPU_PC_LEARNED}
+
+
 @dataclass
 class SolverOptions:
     """solver options (schemas/solvers.rnc:107-612)."""
     iterative_method: str = "cg"         # cg | gmres | fcg
-    preconditioner: str = "jacobi"       # jacobi | none | learned
+    preconditioner: str = "jacobi"       # jacobi | none | sor | eisenstat | learned
     relative_error: float = 1e-8
     absolute_error: float = 0.0
     max_iterations: int = 10000
@@ -84,7 +91,7 @@ class SolverOptions:
     def params(self) -> L.SolverParams:
         return L.SolverParams(
             ksp={"cg": L.FLGPU_KSP_CG, "gmres": L.FLGPU_KSP_GMRES, "fcg": L.FLGPU_KSP_FCG}[self.iterative_method],
-            pc={"none": L.FLGPU_PC_NONE, "jacobi": L.FLGPU_PC_JACOBI, "learned": L.FLGPU_PC_LEARNED}[self.preconditioner],
+            pc=PRECONDITIONERS[self.preconditioner],
             rtol=self.relative_error, atol=self.absolute_error, max_its=self.max_iterations,
             restart=self.restart, precision={"fp64": L.FLGPU_FP64, "mixed": L.FLGPU_MIXED}[self.precision],
             start_from_zero=int(self.start_from_zero), check_every=self.check_every,

@@ -623,6 +623,15 @@ int flgpu_petsc_solve(flgpu_ctx* c, const flgpu_solver_params* sp, const double*
     });
 }
 
+int flgpu_pc_apply(flgpu_ctx* c, const flgpu_solver_params* sp, const double* r, double* z) {
+    return guarded("flgpu_pc_apply", [&]() -> int {
+        if (!c || !sp || !r || !z) return set_error(FLGPU_EINVAL, "flgpu_pc_apply: null argument");
+        if (!c->assembled) return set_error(FLGPU_EINVAL, "flgpu_pc_apply: no system on the device");
+        FLGPU_HIP(hipSetDevice(c->device));
+        return pc_apply_host(c, sp, r, z);
+    });
+}
+
 int flgpu_solution_get(flgpu_ctx* c, double* x) {
     return guarded("flgpu_solution_get", [&]() -> int {
         if (!c || !x) return set_error(FLGPU_EINVAL, "flgpu_solution_get: null argument");

@@ -49,22 +49,31 @@ int guarded(const char* name, F&& body) noexcept {
     }
 }
 
-// Owning device buffer.
+// Device buffer: owning (alloc / upload), or a view of memory owned
+// elsewhere (borrow), which release() forgets without freeing.
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool owned = true;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        owned = true;
+    }
+    void borrow(T* ptr, size_t count) {
+        release();
+        p = ptr;
+        n = count;
+        owned = false;
     }
     int alloc(size_t count) {
-        if (count == n && p) return FLGPU_OK;
+        if (count == n && p && owned) return FLGPU_OK;
         release();
         if (count == 0) return FLGPU_OK;
         hipError_t e = hipMalloc(&p, count * sizeof(T));
@@ -78,7 +87,7 @@ struct DevBuf {
     }
     // grow only: keeps the buffer when it is already large enough
     int reserve(size_t count) {
-        if (p && count <= n) return FLGPU_OK;
+        if (p && owned && count <= n) return FLGPU_OK;
         return alloc(count);
     }
     int upload(const T* host, size_t count, hipStream_t s) {
@@ -246,6 +255,7 @@ int time_spmv(flgpu_ctx* c, int precision, int reps, double* ms);
 int time_spmv_cold(flgpu_ctx* c, int precision, int reps, int flush_mib, double* ms);
 int csr_mult(flgpu_ctx* c, const double* x, double* y);
 int solution_get(flgpu_ctx* c, double* x);
+int pc_apply_host(flgpu_ctx* c, const flgpu_solver_params* sp, const double* r, double* z);
 // learned.hip
 void learned_free(flgpu_ctx* c);
 // shared helpers

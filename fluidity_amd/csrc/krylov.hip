@@ -43,10 +43,15 @@ struct KrylovWork {
     bool use16 = false;
     int nrb = 0;
     int64_t rb_key = -1;     // sparsity generation rb was built for
-    Scal* h_scal = nullptr;  // pinned
+    Scal* h_scal = nullptr;  // host copy of the current slot (pinned when h_scal_owned)
+    bool h_scal_owned = false;
     int nblk = 1;
     Scal* S() { return scal.p + cur; }
     double* part(int region) { return partials.p + (size_t)region * kPartStride; }
+    // multicolour SSOR: rows grouped by colour (ascending inside a colour)
+    DevBuf<int32_t> mc_rows;
+    std::vector<int32_t> mc_ptr;  // colour c owns mc_rows[mc_ptr[c] .. mc_ptr[c+1])
+    int64_t mc_key = -1;          // sparsity generation of the colouring
     // GMRES
     DevBuf<double> V, H, gv, cs, sn, hcol;
     DevBuf<float> V32;
@@ -54,7 +59,7 @@ struct KrylovWork {
     bool have_x = false;
     bool prepared = false, prepared_mixed = false;
     ~KrylovWork() {
-        if (h_scal) (void)hipHostFree(h_scal);
+        if (h_scal && h_scal_owned) (void)hipHostFree(h_scal);
     }
 };
 
@@ -332,6 +337,12 @@ k_copy(int n, const V* __restrict__ a, V* __restrict__ b) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = a[i];
 }
 
+template <typename V>
+__global__ void __launch_bounds__(kVecBlock)
+k_fill(int n, V v, V* __restrict__ a) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = v;
+}
+
 // r_V = (V) r64 ; z = D^-1 r ; partials r.z, r.r, z.z
 template <typename V>
 __global__ void __launch_bounds__(kVecBlock)
@@ -411,6 +422,68 @@ k_finish_x(int n, double* __restrict__ x, const double* __restrict__ rhs,
            const int32_t* __restrict__ inactive, int have_inactive) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && have_inactive && inactive[i]) x[i] = rhs[i];
+}
+
+// ---------------------------------------------------------------------
+// Multicolour SSOR (the "sor" preconditioner)
+// ---------------------------------------------------------------------
+//
+// PETSc's PCSOR, which femtools/Solvers.F90:1844-1854 selects for "sor",
+// applies one local symmetric Gauss-Seidel sweep (omega = 1) in the natural
+// row order -- a sequential recurrence.  Here the rows are taken colour by
+// colour, ascending inside a colour, where the colours are a first-fit
+// colouring of the matrix graph in row order (rows of one colour share no
+// entry, so a colour's rows update independently): the same sweep on the
+// symmetrically permuted matrix, again symmetric positive definite for an
+// SPD system, one launch per colour.
+//   forward:  z = 0; for c = 0 .. C-1:  z_i = (r_i - sum_{j != i} a_ij z_j) / a_ii
+//   backward:        for c = C-2 .. 0:  the same with the current z
+// (the last colour's backward step would recompute its forward values).
+// Each row is summed by one thread in entry order with explicit fused
+// multiply-adds, the order of the oracle's restatement.
+
+template <typename V>
+__global__ void __launch_bounds__(kVecBlock)
+k_fill_done(int n, V* __restrict__ a, const int* __restrict__ done_flag) {
+    if (done_flag && *done_flag) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = (V)0;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256)
+k_sor_colour(int nr, const int32_t* __restrict__ rows, const int32_t* __restrict__ rowptr,
+             const int32_t* __restrict__ col, const V* __restrict__ val, const V* __restrict__ r, V* __restrict__ z,
+             const int* __restrict__ done_flag) {
+    if (done_flag && *done_flag) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nr) return;
+    const int i = rows[t];
+    V acc = r[i], aii = (V)0;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+        const int j = col[k];
+        if (j == i)
+            aii = val[k];
+        else
+            acc = fma(-val[k], z[j], acc);
+    }
+    z[i] = acc / aii;
+}
+
+// Per-block sums r.z, z.z, z.w (w may be null: z.w = 0).
+template <typename V>
+__global__ void __launch_bounds__(kVecBlock)
+k_pc_dots(int n, const V* __restrict__ r, const V* __restrict__ z, const V* __restrict__ w,
+          double* __restrict__ partials, const int* __restrict__ done_flag) {
+    if (done_flag && *done_flag) return;
+    double s_rz = 0.0, s_zz = 0.0, s_zw = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double zi = (double)z[i];
+        s_rz += (double)r[i] * zi;
+        s_zz += zi * zi;
+        if (w) s_zw += zi * (double)w[i];
+    }
+    double v[3] = {s_rz, s_zz, s_zw};
+    block_partials<3>(v, partials);
 }
 
 // ---------------------------------------------------------------------
@@ -504,7 +577,10 @@ static int ensure_work(flgpu_ctx* c, bool mixed) {
         k->rb_key = c->sparsity_gen;
     }
     FLGPU_TRY(k->scal.alloc(2));
-    if (!k->h_scal) FLGPU_HIP(hipHostMalloc((void**)&k->h_scal, sizeof(Scal), hipHostMallocDefault));
+    if (!k->h_scal) {
+        FLGPU_HIP(hipHostMalloc((void**)&k->h_scal, sizeof(Scal), hipHostMallocDefault));
+        k->h_scal_owned = true;
+    }
     k->nblk = (int)std::min<int64_t>(kMaxVecGrid, (n + kVecBlock - 1) / kVecBlock);
     k->nblk = std::max(k->nblk, 1);
     return FLGPU_OK;
@@ -518,6 +594,8 @@ static int read_scal(flgpu_ctx* c, KrylovWork* k) {
 
 static int write_scal(flgpu_ctx* c, KrylovWork* k) {
     FLGPU_HIP(hipMemcpyAsync(k->S(), k->h_scal, sizeof(Scal), hipMemcpyHostToDevice, c->stream));
+    // a pageable host copy is read by the copy after the call returns
+    if (!k->h_scal_owned) FLGPU_HIP(hipStreamSynchronize(c->stream));
     return FLGPU_OK;
 }
 
@@ -553,6 +631,76 @@ static int norm_b(flgpu_ctx* c, KrylovWork* k) {
     return finalize(c, k, FIN_BB, PART_MISC, k->nblk, false, false);
 }
 
+// ------------------------- multicolour SSOR ------------------------------
+
+// First-fit colouring of the rows of a CSR pattern in row order: row i
+// takes the smallest colour that none of its columns j < i holds.  Rows are
+// then listed colour by colour, ascending inside a colour.
+static void colour_rows(int n, const int32_t* rowptr, const int32_t* col, std::vector<int32_t>& ptr,
+                 std::vector<int32_t>& rows) {
+    std::vector<int32_t> colour(n, -1), mark;  // mark[q] == i: colour q is taken next to row i
+    int ncol = 0;
+    for (int i = 0; i < n; i++) {
+        for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+            const int j = col[k];
+            if (j < 0 || j >= i) continue;
+            const int q = colour[j];
+            if (q >= (int)mark.size()) mark.resize(q + 1, -1);
+            mark[q] = i;
+        }
+        int q = 0;
+        while (q < (int)mark.size() && mark[q] == i) q++;
+        colour[i] = q;
+        ncol = std::max(ncol, q + 1);
+    }
+    ptr.assign(ncol + 1, 0);
+    for (int i = 0; i < n; i++) ptr[colour[i] + 1]++;
+    for (int c = 0; c < ncol; c++) ptr[c + 1] += ptr[c];
+    rows.resize(n);
+    std::vector<int32_t> pos(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < n; i++) rows[pos[colour[i]]++] = i;
+}
+
+static int ssor_prepare(flgpu_ctx* c, KrylovWork* k) {
+    if (k->mc_key == c->sparsity_gen) return FLGPU_OK;
+    if (!c->all_diag) return set_error(FLGPU_EUNSUPPORTED, "sor preconditioner: a row has no diagonal entry");
+    std::vector<int32_t> rows;
+    colour_rows(c->rows, c->h_rowptr.data(), c->h_col.data(), k->mc_ptr, rows);
+    FLGPU_TRY(k->mc_rows.upload(rows.data(), rows.size(), c->stream));
+    FLGPU_HIP(hipStreamSynchronize(c->stream));  // rows is freed on return
+    k->mc_key = c->sparsity_gen;
+    return FLGPU_OK;
+}
+
+// z = M^-1 r (the forward and backward sweeps); with done_flag set the
+// kernels return at once.
+template <typename V>
+static int ssor_sweeps(flgpu_ctx* c, KrylovWork* k, const V* val, const V* r, V* z, const int* done_flag) {
+    const int ncol = (int)k->mc_ptr.size() - 1;
+    hipLaunchKernelGGL((k_fill_done<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, z, done_flag);
+    auto sweep = [&](int q) {
+        const int nr = k->mc_ptr[q + 1] - k->mc_ptr[q];
+        if (nr > 0)
+            hipLaunchKernelGGL((k_sor_colour<V>), dim3(grid_for(nr, 256)), dim3(256), 0, c->stream, nr,
+                               k->mc_rows.p + k->mc_ptr[q], c->rowptr.p, c->col.p, val, r, z, done_flag);
+    };
+    for (int q = 0; q < ncol; q++) sweep(q);
+    for (int q = ncol - 2; q >= 0; q--) sweep(q);
+    FLGPU_HIP(hipGetLastError());
+    return FLGPU_OK;
+}
+
+template <typename V>
+static int ssor_apply(flgpu_ctx* c, KrylovWork* k, const V* val, const V* r, V* z, const V* w, double* partials,
+                      int* nparts, const int* done_flag) {
+    FLGPU_TRY(ssor_sweeps<V>(c, k, val, r, z, done_flag));
+    hipLaunchKernelGGL((k_pc_dots<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, r, z, w, partials,
+                       done_flag);
+    FLGPU_HIP(hipGetLastError());
+    *nparts = k->nblk;
+    return FLGPU_OK;
+}
+
 // ------------------------- (F)PCG ----------------------------------------
 
 template <typename V>
@@ -572,57 +720,72 @@ CgVecs<float> cg_vecs<float>(KrylovWork* k) {
     return {k->r32.p, k->z32.p, k->p32.p, k->w32.p, k->dinv32.p, k->y32.p, k->rt32.p, k->val32.p};
 }
 
-// Apply the learned preconditioner z = G^T (G r); the per-block sums
-// r.z, z.z, z.w (w = A p of this iteration) go to the PART_PC region.
+// The preconditioner of a (F)PCG iteration: the diagonal (Jacobi, or the
+// identity as a unit diagonal), applied inside the vector kernels; or the
+// learned factor or the multicolour SSOR sweeps, applied by kernels of their
+// own after the residual update.
+enum PcMode { PCM_DIAG = 0, PCM_LEARNED = 1, PCM_SSOR = 2 };
+
+template <typename V>
+static int ssor_apply(flgpu_ctx* c, KrylovWork* k, const V* val, const V* r, V* z, const V* w, double* partials,
+                      int* nparts, const int* done_flag);
+
+// z = M^-1 r for the learned factor (z = G^T (G r)) or SSOR; the per-block
+// sums r.z, z.z, z.w (w = A p of this iteration) go to the PART_PC region.
 // Returns the number of partial sums in *nparts.
 template <typename V>
-static int learned_apply(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, const int* done_flag, int* nparts) {
+static int pc_apply(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, PcMode mode, const int* done_flag,
+                    int* nparts) {
+    if (mode == PCM_SSOR) return ssor_apply<V>(c, k, v.val, v.r, v.z, v.w, k->part(PART_PC), nparts, done_flag);
     return learned_precond_apply<V>(c, v.r, v.rt, v.z, v.y, v.w, k->part(PART_PC), nparts, done_flag);
 }
 
 // r = b - A x (FP64), r_V = r, z = M^-1 r with the sums r.z, r.r, z.z
 // (and the initial norm when norm0) in the current scalar slot.
 template <typename V>
-static int restart_residual(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, bool jacobi, bool norm0,
+static int restart_residual(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, PcMode mode, bool norm0,
                             bool recompute) {
     if (recompute) FLGPU_TRY(true_residual(c, k));
-    const float* dsc = jacobi ? nullptr : learned_scale(c);
+    const bool diag = mode == PCM_DIAG;
+    const float* dsc = mode == PCM_LEARNED ? learned_scale(c) : nullptr;
     hipLaunchKernelGGL((k_restart_vecs<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->r.p,
-                       v.r, v.z, v.dinv, jacobi ? 1 : 0, dsc, v.rt, k->part(PART_VEC));
+                       v.r, v.z, v.dinv, diag ? 1 : 0, dsc, v.rt, k->part(PART_VEC));
     FLGPU_HIP(hipGetLastError());
-    FLGPU_TRY(finalize(c, k, FIN_RESTART, PART_VEC, k->nblk, jacobi, norm0 && jacobi));
-    if (!jacobi) {
+    FLGPU_TRY(finalize(c, k, FIN_RESTART, PART_VEC, k->nblk, diag, norm0 && diag));
+    if (!diag) {
         int np = 0;
-        FLGPU_TRY(learned_apply<V>(c, k, v, nullptr, &np));
+        FLGPU_TRY(pc_apply<V>(c, k, v, mode, nullptr, &np));
         FLGPU_TRY(finalize(c, k, FIN_LEARNED, PART_PC, np, false, norm0));
     }
     return FLGPU_OK;
 }
 
 template <typename V>
-static int cg_restart(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, bool jacobi, bool set_norm0) {
-    FLGPU_TRY(restart_residual<V>(c, k, v, jacobi, set_norm0, true));
+static int cg_restart(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, PcMode mode, bool set_norm0) {
+    FLGPU_TRY(restart_residual<V>(c, k, v, mode, set_norm0, true));
     hipLaunchKernelGGL((k_copy<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, (const V*)v.z, v.p);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
 
-// One (F)PCG iteration: w = A p; update x, r, z; [z = G^T G r]; test,
-// beta, p.  Four (Jacobi) or six (learned) kernels, no host round trip.
+// One (F)PCG iteration: w = A p; update x, r, z; [z = M^-1 r]; test,
+// beta, p.  Four kernels (diagonal), six (learned) or 2C + 3 (SSOR over C
+// colours), no host round trip.
 template <typename V>
-static int cg_iteration(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, bool jacobi) {
+static int cg_iteration(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, PcMode mode) {
+    const bool diag = mode == PCM_DIAG;
     FLGPU_TRY((spmv<V, V, V>(c, k, v.val, v.p, v.w, nullptr, 1, &k->S()->done)));
     Scal* in = k->S();
     k->cur ^= 1;
     hipLaunchKernelGGL((k_cg_update<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->x.p, v.p, v.r,
-                       v.w, v.z, v.dinv, jacobi ? 1 : 0, jacobi ? nullptr : learned_scale(c), v.rt, k->part(PART_SPMV), spmv_grid(c), k->part(PART_VEC), in,
-                       k->S());
+                       v.w, v.z, v.dinv, diag ? 1 : 0, mode == PCM_LEARNED ? learned_scale(c) : nullptr, v.rt,
+                       k->part(PART_SPMV), spmv_grid(c), k->part(PART_VEC), in, k->S());
     int npc = 0;
-    if (!jacobi) FLGPU_TRY(learned_apply<V>(c, k, v, &k->S()->done, &npc));
+    if (!diag) FLGPU_TRY(pc_apply<V>(c, k, v, mode, &k->S()->done, &npc));
     in = k->S();
     k->cur ^= 1;
     hipLaunchKernelGGL((k_cg_p<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, v.z, v.p,
-                       k->part(PART_VEC), k->nblk, jacobi ? nullptr : k->part(PART_PC), npc, in, k->S());
+                       k->part(PART_VEC), k->nblk, diag ? nullptr : k->part(PART_PC), npc, in, k->S());
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
@@ -632,7 +795,7 @@ static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info*
     KrylovWork* k = c->kw;
     const bool mixed = std::is_same<V, float>::value;
     CgVecs<V> v = cg_vecs<V>(k);
-    bool jacobi = !learned && sp->pc == FLGPU_PC_JACOBI;
+    PcMode mode = learned ? PCM_LEARNED : sp->pc == FLGPU_PC_SOR ? PCM_SSOR : PCM_DIAG;
     bool use_learned = learned;
     const int check_every = sp->check_every > 0 ? sp->check_every : 16;
     const double stall_factor = sp->stall_factor > 0 ? sp->stall_factor : 0.9;
@@ -641,15 +804,13 @@ static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info*
     // identity preconditioner = Jacobi with dinv = 1: handled by the
     // "jacobi" kernels when pc == NONE by filling dinv with ones
     if (!learned && sp->pc == FLGPU_PC_NONE) {
-        std::vector<double> ones(c->rows, 1.0);
-        std::vector<float> ones32(c->rows, 1.0f);
-        FLGPU_HIP(hipMemcpyAsync(k->dinv.p, ones.data(), sizeof(double) * c->rows, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL((k_fill<double>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, 1.0, k->dinv.p);
         if (mixed)
-            FLGPU_HIP(hipMemcpyAsync(k->dinv32.p, ones32.data(), sizeof(float) * c->rows, hipMemcpyHostToDevice,
-                                     c->stream));
-        FLGPU_HIP(hipStreamSynchronize(c->stream));
-        jacobi = true;
+            hipLaunchKernelGGL((k_fill<float>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, 1.0f,
+                               k->dinv32.p);
+        FLGPU_HIP(hipGetLastError());
     }
+    if (mode == PCM_SSOR) FLGPU_TRY(ssor_prepare(c, k));
     Scal& h = *k->h_scal;
     // keep the pivot computed by build_reduced
     FLGPU_TRY(read_scal(c, k));
@@ -667,7 +828,7 @@ static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info*
     h.flexible = (use_learned || sp->ksp == FLGPU_KSP_FCG) ? 1 : 0;
     FLGPU_TRY(write_scal(c, k));
     FLGPU_TRY(norm_b(c, k));
-    FLGPU_TRY(cg_restart<V>(c, k, v, !use_learned, true));
+    FLGPU_TRY(cg_restart<V>(c, k, v, mode, true));
     FLGPU_TRY(read_scal(c, k));
     info->refinements = 0;
     info->fallback = 0;
@@ -685,7 +846,7 @@ static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info*
     double window_norm = h.last_norm;
     auto t0 = std::chrono::steady_clock::now();
     while (!h.done) {
-        for (int q = 0; q < check_every; q++) FLGPU_TRY(cg_iteration<V>(c, k, v, jacobi));
+        for (int q = 0; q < check_every; q++) FLGPU_TRY(cg_iteration<V>(c, k, v, mode));
         FLGPU_TRY(read_scal(c, k));
         const double rnow = std::sqrt(h.rr);
         if (mixed && (h.done ? h.reason > 0 : rnow < delta * rmax)) {
@@ -712,7 +873,7 @@ static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info*
                 FLGPU_TRY(write_scal(c, k));
             }
             if (!h.done) {
-                FLGPU_TRY(restart_residual<V>(c, k, v, jacobi, false, false));
+                FLGPU_TRY(restart_residual<V>(c, k, v, mode, false, false));
                 FLGPU_TRY(read_scal(c, k));
                 rmax = std::sqrt(h.rr);
             }
@@ -723,11 +884,11 @@ static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info*
                 if (h.last_norm > stall_factor * window_norm) {
                     // stalled: fall back to Jacobi-PCG from the current x
                     use_learned = false;
-                    jacobi = true;
+                    mode = PCM_DIAG;
                     info->fallback = 1;
                     h.flexible = sp->ksp == FLGPU_KSP_FCG ? 1 : 0;
                     FLGPU_TRY(write_scal(c, k));
-                    FLGPU_TRY(cg_restart<V>(c, k, v, true, false));
+                    FLGPU_TRY(cg_restart<V>(c, k, v, PCM_DIAG, false));
                     FLGPU_TRY(read_scal(c, k));
                     rmax = std::sqrt(h.rr);
                 }
@@ -769,7 +930,7 @@ k_gm_matvec(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict
         if (row < n)
             for (int k = rowptr[row] + lane; k < rowptr[row + 1]; k += L) acc += val[k] * x[col[k]];
         acc = lane_sum<V>(acc, L);
-        if (row < n && lane == 0) y[row] = dinv[row] * acc;
+        if (row < n && lane == 0) y[row] = dinv ? dinv[row] * acc : acc;
     }
 }
 
@@ -859,7 +1020,7 @@ template <typename V>
 __global__ void __launch_bounds__(kVecBlock)
 k_gm_precres(int n, const double* __restrict__ r64, const double* __restrict__ dinv, V* __restrict__ out) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        out[i] = (V)(dinv[i] * r64[i]);
+        out[i] = dinv ? (V)(dinv[i] * r64[i]) : (V)r64[i];
 }
 
 template <typename V>
@@ -879,12 +1040,15 @@ static int run_gmres(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_in
         Vb = (V*)k->V.p;
     }
     if (sp->pc == FLGPU_PC_NONE) {
-        std::vector<double> ones(n, 1.0);
-        std::vector<float> ones32(n, 1.0f);
-        FLGPU_HIP(hipMemcpyAsync(k->dinv.p, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL((k_fill<double>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, n, 1.0, k->dinv.p);
         if (mixed)
-            FLGPU_HIP(hipMemcpyAsync(k->dinv32.p, ones32.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+            hipLaunchKernelGGL((k_fill<float>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, n, 1.0f, k->dinv32.p);
+        FLGPU_HIP(hipGetLastError());
     }
+    // left SSOR: the product and the residual go through tmp, then the sweeps
+    const bool ssor = sp->pc == FLGPU_PC_SOR;
+    if (ssor) FLGPU_TRY(ssor_prepare(c, k));
+    V* tmp = mixed ? (V*)k->rt32.p : (V*)k->rt.p;
     FLGPU_TRY(k->hcol.alloc(m + 2));
     FLGPU_TRY(k->partials.reserve((size_t)kMaxVecGrid * (m + 2)));
     const int nb = k->nblk;
@@ -896,7 +1060,7 @@ static int run_gmres(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_in
 #define FL_GMV(LL)                                                                                         \
     case LL:                                                                                               \
         hipLaunchKernelGGL((k_gm_matvec<LL, V>), dim3(mgrid), dim3(kRowBlock), 0, c->stream, n, c->rowptr.p, \
-                           c->col.p, val, dinv, xin, yout);                                                \
+                           c->col.p, val, ssor ? nullptr : dinv, xin, ssor ? tmp : yout);                  \
         break;
         switch (L) {
             FL_GMV(4)
@@ -906,6 +1070,7 @@ static int run_gmres(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_in
             FL_GMV(64)
         }
 #undef FL_GMV
+        if (ssor) FLGPU_TRY(ssor_sweeps<V>(c, k, val, tmp, yout, nullptr));
         return FLGPU_OK;
     };
     std::vector<double> H((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), y(m), hc(m + 2);
@@ -923,7 +1088,9 @@ static int run_gmres(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_in
     while (!reason) {
         // residual in FP64, preconditioned, in the working precision
         FLGPU_TRY(true_residual(c, k));
-        hipLaunchKernelGGL((k_gm_precres<V>), dim3(nb), dim3(kVecBlock), 0, c->stream, n, k->r.p, k->dinv.p, Vb);
+        hipLaunchKernelGGL((k_gm_precres<V>), dim3(nb), dim3(kVecBlock), 0, c->stream, n, k->r.p,
+                           ssor ? nullptr : k->dinv.p, ssor ? tmp : Vb);
+        if (ssor) FLGPU_TRY(ssor_sweeps<V>(c, k, val, tmp, Vb, nullptr));
         hipLaunchKernelGGL((k_gm_norm<V>), dim3(nb), dim3(kVecBlock), 0, c->stream, n, Vb, k->partials.p);
         hipLaunchKernelGGL(k_gm_sqrt_reduce, dim3(1), dim3(kVecBlock), 0, c->stream, nb, k->partials.p, k->hcol.p);
         FLGPU_HIP(hipMemcpyAsync(&hn, k->hcol.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1023,7 +1190,7 @@ int solve(flgpu_ctx* c, const flgpu_solver_params* sp, double* x_host, flgpu_sol
         return set_error(FLGPU_EINVAL, "flgpu_solve: precision %d", sp->precision);
     if (sp->ksp != FLGPU_KSP_CG && sp->ksp != FLGPU_KSP_GMRES && sp->ksp != FLGPU_KSP_FCG)
         return set_error(FLGPU_EINVAL, "flgpu_solve: ksp %d", sp->ksp);
-    if (sp->pc != FLGPU_PC_NONE && sp->pc != FLGPU_PC_JACOBI && sp->pc != FLGPU_PC_LEARNED)
+    if (sp->pc != FLGPU_PC_NONE && sp->pc != FLGPU_PC_JACOBI && sp->pc != FLGPU_PC_LEARNED && sp->pc != FLGPU_PC_SOR)
         return set_error(FLGPU_EINVAL, "flgpu_solve: pc %d", sp->pc);
     if (sp->pc == FLGPU_PC_LEARNED && sp->ksp == FLGPU_KSP_GMRES)
         return set_error(FLGPU_EUNSUPPORTED, "flgpu_solve: the learned preconditioner runs with cg/fcg");
@@ -1077,6 +1244,29 @@ int csr_mult(flgpu_ctx* c, const double* x, double* y) {
     FLGPU_TRY(dy.alloc(c->rows));
     FLGPU_TRY((spmv<double, double, double>(c, k, c->A.p, dx.p, dy.p, nullptr, 0, nullptr)));
     FLGPU_HIP(hipMemcpyAsync(y, dy.p, sizeof(double) * c->rows, hipMemcpyDeviceToHost, c->stream));
+    FLGPU_HIP(hipStreamSynchronize(c->stream));
+    return FLGPU_OK;
+}
+
+int pc_apply_host(flgpu_ctx* c, const flgpu_solver_params* sp, const double* r, double* z) {
+    if (sp->pc != FLGPU_PC_NONE && sp->pc != FLGPU_PC_JACOBI && sp->pc != FLGPU_PC_SOR)
+        return set_error(FLGPU_EUNSUPPORTED, "flgpu_pc_apply: pc %d (none, jacobi or sor)", sp->pc);
+    FLGPU_TRY(ensure_work(c, false));
+    KrylovWork* k = c->kw;
+    FLGPU_TRY(build_reduced(c, k, false, true));
+    k->prepared = true;
+    k->prepared_mixed = false;
+    const int n = c->rows;
+    FLGPU_HIP(hipMemcpyAsync(k->r.p, r, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    if (sp->pc == FLGPU_PC_SOR) {
+        FLGPU_TRY(ssor_prepare(c, k));
+        FLGPU_TRY(ssor_sweeps<double>(c, k, k->val64.p, k->r.p, k->z.p, nullptr));
+    } else {
+        hipLaunchKernelGGL((k_gm_precres<double>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, n, k->r.p,
+                           sp->pc == FLGPU_PC_JACOBI ? k->dinv.p : nullptr, k->z.p);
+        FLGPU_HIP(hipGetLastError());
+    }
+    FLGPU_HIP(hipMemcpyAsync(z, k->z.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     FLGPU_HIP(hipStreamSynchronize(c->stream));
     return FLGPU_OK;
 }

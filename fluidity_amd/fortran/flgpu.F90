@@ -22,7 +22,8 @@ module flgpu
   integer(c_int32_t), parameter, public :: FLGPU_FP64 = 0, FLGPU_MIXED = 1
   integer(c_int32_t), parameter, public :: FLGPU_ASM_AUTO = 0, FLGPU_ASM_COLOUR = 1, FLGPU_ASM_BLOCKED = 2
   integer(c_int32_t), parameter, public :: FLGPU_KSP_CG = 0, FLGPU_KSP_GMRES = 1, FLGPU_KSP_FCG = 2
-  integer(c_int32_t), parameter, public :: FLGPU_PC_NONE = 0, FLGPU_PC_JACOBI = 1, FLGPU_PC_LEARNED = 2
+  integer(c_int32_t), parameter, public :: FLGPU_PC_NONE = 0, FLGPU_PC_JACOBI = 1, FLGPU_PC_LEARNED = 2, &
+                                            FLGPU_PC_SOR = 3
   integer(c_int32_t), parameter, public :: FLGPU_NORM_PRECONDITIONED = 0, FLGPU_NORM_UNPRECONDITIONED = 1
   integer(c_int32_t), parameter, public :: FLGPU_FIELD_T = 0, FLGPU_FIELD_VELOCITY = 1, &
        FLGPU_FIELD_DIFFUSIVITY = 2, FLGPU_FIELD_SOURCE = 3, FLGPU_FIELD_ABSORPTION = 4
@@ -64,7 +65,7 @@ module flgpu
        flgpu_sparsity_build, flgpu_sparsity_get, flgpu_colouring_build, flgpu_colouring_get, &
        flgpu_field_set, flgpu_advdif_assemble, flgpu_rhs_add_field, flgpu_surface_bind, flgpu_advdif_surface, &
        flgpu_dirichlet_apply, flgpu_matrix_get, &
-       flgpu_petsc_solve, flgpu_check
+       flgpu_petsc_solve, flgpu_check, flgpu_pc_from_name
 
   interface
      integer(c_int) function flgpu_init(ctx, device) bind(c, name="flgpu_init")
@@ -255,5 +256,26 @@ contains
     write(0, '(a,a,i0,a,*(a))') trim(what), " failed (", rc, "): ", chars(1:n)
     error stop 1
   end subroutine flgpu_check
+
+  !! The pc field of flgpu_solver_params for a preconditioner name of the
+  !! solver options, as setup_pc_from_options maps it onto PETSc
+  !! (femtools/Solvers.F90:1844-1854); -1 for a name the library lacks.
+  !! "eisenstat" is SSOR applied with Eisenstat's trick: the same
+  !! preconditioner as "sor".
+  integer(c_int32_t) function flgpu_pc_from_name(name) result(pc)
+    character(len=*), intent(in) :: name
+    select case (trim(name))
+    case ("none")
+       pc = FLGPU_PC_NONE
+    case ("jacobi")
+       pc = FLGPU_PC_JACOBI
+    case ("sor", "eisenstat")
+       pc = FLGPU_PC_SOR
+    case ("learned")
+       pc = FLGPU_PC_LEARNED
+    case default
+       pc = -1
+    end select
+  end function flgpu_pc_from_name
 
 end module flgpu

@@ -274,7 +274,11 @@ int flgpu_matrix_set(flgpu_ctx *ctx, const double *A_val, const double *rhs,
 /* ---- solve ------------------------------------------------------------ */
 
 enum { FLGPU_KSP_CG = 0, FLGPU_KSP_GMRES = 1, FLGPU_KSP_FCG = 2 };
-enum { FLGPU_PC_NONE = 0, FLGPU_PC_JACOBI = 1, FLGPU_PC_LEARNED = 2 };
+/* FLGPU_PC_SOR: "sor" (and "eisenstat", SSOR by Eisenstat's trick) of
+ * femtools/Solvers.F90:1844-1854 -- one symmetric Gauss-Seidel sweep
+ * (omega = 1) taken colour by colour over a first-fit colouring of the
+ * matrix graph, instead of PETSc's sequential natural-order sweep. */
+enum { FLGPU_PC_NONE = 0, FLGPU_PC_JACOBI = 1, FLGPU_PC_LEARNED = 2, FLGPU_PC_SOR = 3 };
 
 typedef struct {
     int32_t ksp, pc;
@@ -328,6 +332,12 @@ int flgpu_petsc_solve(flgpu_ctx *ctx, const flgpu_solver_params *sp,
                       const double *A_val, const int32_t *inactive,
                       const double *rhs, double *x, int32_t *its,
                       int32_t *reason, double *true_relres);
+
+/* PCApply of the preconditioner flgpu_solve builds for the reduced system
+ * (setup_pc_from_options, femtools/Solvers.F90:1844-1854): z = M^-1 r for
+ * pc none, jacobi or sor, FP64, host arrays of rows doubles. */
+int flgpu_pc_apply(flgpu_ctx *ctx, const flgpu_solver_params *sp,
+                   const double *r, double *z);
 
 /* Copy the last device-resident solution (rows) to the host. */
 int flgpu_solution_get(flgpu_ctx *ctx, double *x);

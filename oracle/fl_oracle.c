@@ -1158,6 +1158,96 @@ enum {
     KSP_DIVERGED_INDEFINITE_MAT = -10
 };
 
+/* ---- preconditioners of the reduced system -------------------------------
+ * Jacobi (PCJACOBI) and "sor" (femtools/Solvers.F90:1844-1854 selects
+ * PCSOR: one local symmetric Gauss-Seidel sweep, omega = 1).  PETSc sweeps
+ * in the natural row order; this library (and so this checker) sweeps
+ * colour by colour over a first-fit colouring of the matrix graph in row
+ * order, ascending rows inside a colour: SSOR of the symmetrically permuted
+ * matrix.  Each row is summed in entry order with fused multiply-adds. */
+
+/* First-fit colouring of the rows of a 1-based CSR pattern in row order;
+ * rows listed colour by colour (0-based), ptr[ncol+1].  Returns ncol. */
+int orc_colour_rows(int n, const int32_t *findrm, const int32_t *colm, int32_t *ptr_out,
+                    int32_t *rows_out)
+{
+    int32_t *colour = malloc(sizeof(int32_t) * (size_t)n);
+    int32_t *mark = malloc(sizeof(int32_t) * (size_t)(n + 1));
+    int ncol = 0;
+    for (int q = 0; q <= n; q++) mark[q] = -1;
+    for (int i = 0; i < n; i++) {
+        for (int64_t k = findrm[i] - 1; k < findrm[i + 1] - 1; k++) {
+            int j = colm[k] - 1;
+            if (j >= 0 && j < i) mark[colour[j]] = i;
+        }
+        int q = 0;
+        while (mark[q] == i) q++;
+        colour[i] = q;
+        if (q + 1 > ncol) ncol = q + 1;
+    }
+    if (ptr_out) {
+        for (int q = 0; q <= ncol; q++) ptr_out[q] = 0;
+        for (int i = 0; i < n; i++) ptr_out[colour[i] + 1]++;
+        for (int q = 0; q < ncol; q++) ptr_out[q + 1] += ptr_out[q];
+        int32_t *pos = malloc(sizeof(int32_t) * (size_t)(ncol + 1));
+        for (int q = 0; q < ncol; q++) pos[q] = ptr_out[q];
+        for (int i = 0; i < n; i++) rows_out[pos[colour[i]]++] = i;
+        free(pos);
+    }
+    free(colour);
+    free(mark);
+    return ncol;
+}
+
+typedef struct {
+    int kind;              /* 0 none, 1 jacobi, 3 sor */
+    const double *dinv;
+    int ncol;
+    const int32_t *ptr, *rows;
+} pcond;
+
+static void sor_row(const rsys *s, int i, const double *r, double *z)
+{
+    double acc = r[i], aii = 0.0;
+    for (int64_t k = s->findrm[i] - 1; k < s->findrm[i + 1] - 1; k++) {
+        int j = s->colm[k] - 1;
+        if (j == i) aii = s->val[k];
+        else acc = fma(-s->val[k], z[j], acc);
+    }
+    z[i] = acc / aii;
+}
+
+static void ssor(const rsys *s, const pcond *pc, const double *r, double *z)
+{
+    for (int i = 0; i < s->n; i++) z[i] = 0.0;
+    for (int q = 0; q < pc->ncol; q++)
+        for (int t = pc->ptr[q]; t < pc->ptr[q + 1]; t++) sor_row(s, pc->rows[t], r, z);
+    for (int q = pc->ncol - 2; q >= 0; q--)
+        for (int t = pc->ptr[q]; t < pc->ptr[q + 1]; t++) sor_row(s, pc->rows[t], r, z);
+}
+
+static void pc_apply(const rsys *s, const pcond *pc, const double *r, double *z)
+{
+    if (pc->kind == 3) {
+        ssor(s, pc, r, z);
+        return;
+    }
+    for (int i = 0; i < s->n; i++) z[i] = pc->kind == 1 ? pc->dinv[i] * r[i] : r[i];
+}
+
+/* z = M^-1 r of the multicolour SSOR on a 1-based CSR matrix (test hook). */
+void orc_ssor_apply(int n, const int32_t *findrm, const int32_t *colm, const double *val,
+                    const double *r, double *z)
+{
+    int32_t *ptr = malloc(sizeof(int32_t) * (size_t)(n + 1)), *rows = malloc(sizeof(int32_t) * (size_t)n);
+    pcond pc = {3, NULL, 0, ptr, rows};
+    pc.ncol = orc_colour_rows(n, findrm, colm, ptr, rows);
+    rsys s = {n, findrm, colm, (double *)val};
+    ssor(&s, &pc, r, z);
+    free(ptr);
+    free(rows);
+}
+
 static int conv_test(double rnorm, double rnorm0, double rtol, double atol)
 {
     if (isnan(rnorm) || isinf(rnorm)) return KSP_DIVERGED_NANORINF;
@@ -1167,7 +1257,7 @@ static int conv_test(double rnorm, double rnorm0, double rtol, double atol)
     return 0;
 }
 
-static int ksp_cg(const rsys *s, const double *dinv, const double *b, double *x,
+static int ksp_cg(const rsys *s, const pcond *pc, const double *b, double *x,
                   double rtol, double atol, int maxits, int *its)
 {
     int n = s->n;
@@ -1176,7 +1266,7 @@ static int ksp_cg(const rsys *s, const double *dinv, const double *b, double *x,
     int reason = 0;
     rmult(s, x, w);
     for (int i = 0; i < n; i++) r[i] = b[i] - w[i];
-    for (int i = 0; i < n; i++) z[i] = dinv ? dinv[i] * r[i] : r[i];
+    pc_apply(s, pc, r, z);
     double dp = sqrt(dot(n, z, z)); /* KSP_NORM_PRECONDITIONED */
     double rnorm0 = dp;
     double beta = dot(n, z, r), betaold = 1.0;
@@ -1201,8 +1291,8 @@ static int ksp_cg(const rsys *s, const double *dinv, const double *b, double *x,
         for (int i = 0; i < n; i++) {
             x[i] = x[i] + a * p[i];
             r[i] = r[i] - a * w[i];
-            z[i] = dinv ? dinv[i] * r[i] : r[i];
         }
+        pc_apply(s, pc, r, z);
         dp = sqrt(dot(n, z, z));
         *its = it + 1;
         betaold = beta;
@@ -1216,7 +1306,7 @@ static int ksp_cg(const rsys *s, const double *dinv, const double *b, double *x,
 }
 
 /* restarted GMRES, left preconditioning, modified Gram-Schmidt */
-static int ksp_gmres(const rsys *s, const double *dinv, const double *b,
+static int ksp_gmres(const rsys *s, const pcond *pc, const double *b,
                      double *x, double rtol, double atol, int maxits,
                      int restart, int *its)
 {
@@ -1231,10 +1321,8 @@ static int ksp_gmres(const rsys *s, const double *dinv, const double *b,
     *its = 0;
     while (!reason) {
         rmult(s, x, w);
-        for (int i = 0; i < n; i++) {
-            double r = b[i] - w[i];
-            V[i] = dinv ? dinv[i] * r : r;
-        }
+        for (int i = 0; i < n; i++) w[i] = b[i] - w[i];
+        pc_apply(s, pc, w, V);
         double beta = sqrt(dot(n, V, V));
         if (rnorm0 < 0) {
             rnorm0 = beta;
@@ -1249,7 +1337,7 @@ static int ksp_gmres(const rsys *s, const double *dinv, const double *b,
         for (k = 0; k < m && !reason; k++) {
             double *vk = V + (size_t)n * k, *vn = V + (size_t)n * (k + 1);
             rmult(s, vk, t);
-            for (int i = 0; i < n; i++) vn[i] = dinv ? dinv[i] * t[i] : t[i];
+            pc_apply(s, pc, t, vn);
             for (int j = 0; j <= k; j++) {
                 double *vj = V + (size_t)n * j;
                 double h = dot(n, vn, vj);
@@ -1326,19 +1414,30 @@ int orc_petsc_solve(const orc_solver *sv, int n, const int32_t *findrm,
             s.val[k] = v;
         }
     }
+    pcond pc = {sv->pc, NULL, 0, NULL, NULL};
+    int32_t *mc_ptr = NULL, *mc_rows = NULL;
     if (sv->pc == 1) {
         dinv = malloc(sizeof(double) * n);
         for (int i = 0; i < n; i++) {
             double d = centrm[i] > 0 ? s.val[centrm[i] - 1] : 0.0;
             dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
         }
+        pc.dinv = dinv;
+    } else if (sv->pc == 3) {
+        mc_ptr = malloc(sizeof(int32_t) * (size_t)(n + 1));
+        mc_rows = malloc(sizeof(int32_t) * (size_t)n);
+        pc.ncol = orc_colour_rows(n, findrm, colm, mc_ptr, mc_rows);
+        pc.ptr = mc_ptr;
+        pc.rows = mc_rows;
     }
     double t0 = wall_now();
     int its = 0, reason;
     if (sv->ksp == 1)
-        reason = ksp_gmres(&s, dinv, b, y, sv->rtol, sv->atol, sv->max_its, sv->restart, &its);
+        reason = ksp_gmres(&s, &pc, b, y, sv->rtol, sv->atol, sv->max_its, sv->restart, &its);
     else
-        reason = ksp_cg(&s, dinv, b, y, sv->rtol, sv->atol, sv->max_its, &its);
+        reason = ksp_cg(&s, &pc, b, y, sv->rtol, sv->atol, sv->max_its, &its);
+    free(mc_ptr);
+    free(mc_rows);
     if (seconds) *seconds = wall_now() - t0;
     /* true residual of the reduced system */
     rmult(&s, y, tmp);

@@ -140,7 +140,7 @@ void orc_csr_mult(int nrows, const int32_t *findrm, const int32_t *colm,
 
 typedef struct {
     int32_t ksp;      /* 0 = cg, 1 = gmres */
-    int32_t pc;       /* 0 = none, 1 = jacobi */
+    int32_t pc;       /* 0 = none, 1 = jacobi, 3 = sor (multicolour SSOR) */
     double rtol, atol;
     int32_t max_its, restart;
     int32_t nthreads;
@@ -155,6 +155,17 @@ int orc_petsc_solve(const orc_solver *s, int n, const int32_t *findrm,
                     const double *A, const int32_t *inactive,
                     const double *rhs, double *x, int32_t *its,
                     int32_t *reason, double *true_relres, double *seconds);
+
+/* First-fit colouring of the rows of a 1-based CSR pattern in row order;
+ * ptr_out[ncol+1] and rows_out[n] (0-based rows, colour by colour,
+ * ascending inside a colour) when ptr_out is not NULL.  Returns ncol. */
+int orc_colour_rows(int n, const int32_t *findrm, const int32_t *colm, int32_t *ptr_out,
+                    int32_t *rows_out);
+
+/* z = M^-1 r of the multicolour SSOR (forward sweep over the colours from
+ * z = 0, backward over all but the last) on a 1-based CSR matrix. */
+void orc_ssor_apply(int n, const int32_t *findrm, const int32_t *colm, const double *val,
+                    const double *r, double *z);
 
 /* Ghost pivot of Petsc_Tools.F90:959-984 over all rows' |diagonal|. */
 double orc_ghost_pivot(int n, const int32_t *centrm, const double *A);

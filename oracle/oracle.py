@@ -89,6 +89,9 @@ def load():
                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]
         lib.orc_ghost_pivot.argtypes = [C.c_int, P, P]
         lib.orc_ghost_pivot.restype = C.c_double
+        lib.orc_colour_rows.argtypes = [C.c_int, P, P, P, P]
+        lib.orc_ssor_apply.argtypes = [C.c_int, P, P, P, P, P]
+        lib.orc_ssor_apply.restype = None
         _lib = lib
     return _lib
 
@@ -295,8 +298,8 @@ def petsc_solve(ksp: str, pc: str, rtol: float, sparsity, A, inactive, rhs, x0=N
                 max_its=10000, restart=30, nthreads=0):
     findrm, colm, centrm = sparsity
     n = findrm.size - 1
-    s = OrcSolver({"cg": 0, "gmres": 1}[ksp], {"none": 0, "jacobi": 1}[pc], rtol, atol, max_its, restart,
-                  nthreads)
+    s = OrcSolver({"cg": 0, "gmres": 1}[ksp], {"none": 0, "jacobi": 1, "sor": 3}[pc], rtol, atol, max_its,
+                  restart, nthreads)
     x = np.zeros(n) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
     its, reason, relres, secs = C.c_int32(0), C.c_int32(0), C.c_double(0), C.c_double(0)
     inact = None if inactive is None else np.ascontiguousarray(inactive, np.int32)
@@ -304,6 +307,25 @@ def petsc_solve(ksp: str, pc: str, rtol: float, sparsity, A, inactive, rhs, x0=N
                            _p(inact), _p(np.ascontiguousarray(rhs)), _p(x), C.byref(its), C.byref(reason),
                            C.byref(relres), C.byref(secs))
     return x, its.value, reason.value, relres.value, secs.value
+
+
+def colour_rows(findrm, colm):
+    """First-fit row colouring of a 1-based CSR pattern: (ptr, rows)."""
+    n = findrm.size - 1
+    ptr = np.zeros(n + 1, np.int32)
+    rows = np.zeros(n, np.int32)
+    ncol = load().orc_colour_rows(n, _p(np.ascontiguousarray(findrm, np.int32)),
+                                  _p(np.ascontiguousarray(colm, np.int32)), _p(ptr), _p(rows))
+    return ptr[:ncol + 1].copy(), rows
+
+
+def ssor_apply(findrm, colm, val, r):
+    """z = M^-1 r of the multicolour SSOR on a 1-based CSR matrix."""
+    z = np.zeros(findrm.size - 1)
+    load().orc_ssor_apply(findrm.size - 1, _p(np.ascontiguousarray(findrm, np.int32)),
+                          _p(np.ascontiguousarray(colm, np.int32)), _p(np.ascontiguousarray(val, np.float64)),
+                          _p(np.ascontiguousarray(r, np.float64)), _p(z))
+    return z
 
 
 def ghost_pivot(centrm, A):
