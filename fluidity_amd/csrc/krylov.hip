@@ -873,7 +873,19 @@ static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info*
                 FLGPU_TRY(write_scal(c, k));
             }
             if (!h.done) {
+                // The FP32 recurrence drifts from the true residual by about
+                // eps32 times the residual of the last replacement.  After a
+                // drop of more than 1e3 since then (a fast preconditioner
+                // between two host checks) that drift is no longer small
+                // against r, and the old direction p is not conjugate to the
+                // replaced residual: the direction restarts from z.
+                const bool restart_p = rnow < 1e-3 * rmax;
                 FLGPU_TRY(restart_residual<V>(c, k, v, mode, false, false));
+                if (restart_p) {
+                    hipLaunchKernelGGL((k_copy<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows,
+                                       (const V*)v.z, v.p);
+                    FLGPU_HIP(hipGetLastError());
+                }
                 FLGPU_TRY(read_scal(c, k));
                 rmax = std::sqrt(h.rr);
             }
