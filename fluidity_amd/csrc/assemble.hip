@@ -151,6 +151,7 @@ static int ensure_entry_positions(flgpu_ctx* c) {
                        c->rowptr.p, c->col.p, P.epos.p);
     FLGPU_HIP(hipGetLastError());
     P.epos_built = true;
+    P.gather_built = false;  // built from epos
     return FLGPU_OK;
 }
 
@@ -394,7 +395,11 @@ __host__ __device__ constexpr int entry_stage_per_element(int kst) {
     return quad_stage_per_element<DIM, LOC>(kst) + DIM * (DIM + 1) + 2 * LOC * LOC;
 }
 
-template <int DIM, int LOC, typename R>
+// GATHER: the first pass of the two-pass scheme -- elems is the whole
+// colour order and element q's entries and row values are stored at
+// A[q*LOC^2 + entry] and rhs[q*LOC + row] (zeros included) for the
+// per-entry sums of k_gather_sum; epos is not read.
+template <int DIM, int LOC, typename R, bool GATHER = false>
 __global__ void __launch_bounds__(kEntryThreads)
 k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict__ elems, int count,
                    const int32_t* __restrict__ conn, const int32_t* __restrict__ conn_x,
@@ -491,6 +496,16 @@ k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict_
     }
     const R dtte = fl.dtt_on ? (R)fl.dt_theta : (R)0;
     const double v = (double)FMA(dtte, s_ij, m);
+    if (GATHER) {
+        A[(int64_t)q * L2 + rr] = v;
+        if (j == 0) {
+            R r = (R)0;
+            for (int jj = 0; jj < LOC; jj++) r = FMA(-s_s[i * LOC + jj], n_t[jj], r);
+            if (fl.src) r = r + src_i;
+            rhs[(int64_t)q * LOC + i] = (double)r;
+        }
+        return;
+    }
     const int row = conn[(int64_t)ele * LOC + i];
     if (v != 0.0) {  // csr_addto skips zeros (Sparse_Tools.F90:2640)
         const int pos = epos[(int64_t)ele * L2 + rr];
@@ -524,6 +539,185 @@ static int launch_colour_entry(flgpu_ctx* c, const AsmFields& fd, const AsmFlags
                            c->stream, img, c->col_elems.p + off, cnt, c->conn.p, c->conn_x.p, c->X4.p, fd, fl,
                            c->plan.epos.p, c->A.p, c->rhs.p, d_err);
     }
+    FLGPU_HIP(hipGetLastError());
+    return FLGPU_OK;
+}
+
+// ---------------------------------------------------------------------
+// Two-pass scheme (quadratic T).  Pass 1 (k_asm_colour_entry<..., true>)
+// forms every element's entries and row values in one launch over the
+// whole colour order and stores them; pass 2 (k_gather_sum) gives each CSR
+// entry and rhs row one thread that adds its element values in colour
+// order, ascending element id inside a colour -- the order in which the
+// colour scheme's launches add them, and adding a zero leaves a sum's bits
+// unchanged as csr_addto's skip of zeros does, so both schemes give the
+// same bits.  The per-entry lists are built once per mesh, sparsity and
+// colouring from the entry positions (counted, scanned, filled, and each
+// list sorted by colour-order place).
+// ---------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256)
+k_gather_count(int64_t n, int loc, const int32_t* __restrict__ elems, const int32_t* __restrict__ epos,
+               const int32_t* __restrict__ conn, int32_t* __restrict__ ecnt, int32_t* __restrict__ rcnt,
+               int32_t* __restrict__ nbad) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int l2 = loc * loc;
+    const int64_t q = t / l2;
+    const int rr = (int)(t - q * l2);
+    const int64_t ele = elems[q];
+    const int pos = epos[ele * l2 + rr];
+    if (pos >= 0)
+        atomicAdd(&ecnt[pos], 1);
+    else
+        atomicAdd(nbad, 1);
+    if (rr % loc == 0) atomicAdd(&rcnt[conn[ele * loc + rr / loc]], 1);
+}
+
+__global__ void __launch_bounds__(256)
+k_gather_fill(int64_t n, int loc, const int32_t* __restrict__ elems, const int32_t* __restrict__ epos,
+              const int32_t* __restrict__ conn, const int32_t* __restrict__ eptr, const int32_t* __restrict__ rptr,
+              int32_t* __restrict__ ecur, int32_t* __restrict__ rcur, int32_t* __restrict__ elist,
+              int32_t* __restrict__ rlist, int32_t* __restrict__ bad, int32_t* __restrict__ nbad) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int l2 = loc * loc;
+    const int64_t q = t / l2;
+    const int rr = (int)(t - q * l2);
+    const int64_t ele = elems[q];
+    const int pos = epos[ele * l2 + rr];
+    if (pos >= 0)
+        elist[eptr[pos] + atomicAdd(&ecur[pos], 1)] = (int32_t)t;
+    else
+        bad[atomicAdd(nbad, 1)] = (int32_t)t;
+    if (rr % loc == 0) {
+        const int row = conn[ele * loc + rr / loc];
+        rlist[rptr[row] + atomicAdd(&rcur[row], 1)] = (int32_t)(q * loc + rr / loc);
+    }
+}
+
+// Each list ascending (insertion sort; a list holds one value per element
+// containing the entry's nodes, a few dozen at most).
+__global__ void __launch_bounds__(256)
+k_gather_sort(int n, const int32_t* __restrict__ ptr, int32_t* __restrict__ list) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int k0 = ptr[e], k1 = ptr[e + 1];
+    for (int k = k0 + 1; k < k1; k++) {
+        const int v = list[k];
+        int m = k - 1;
+        while (m >= k0 && list[m] > v) {
+            list[m + 1] = list[m];
+            m--;
+        }
+        list[m + 1] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_ptr32(int64_t n, const int64_t* __restrict__ in, int32_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) out[i] = (int32_t)in[i];
+}
+
+// Pass 2: thread per CSR entry (then per rhs row) summing its values.
+__global__ void __launch_bounds__(256)
+k_gather_sum(int nnz, int rows, const int32_t* __restrict__ eptr, const int32_t* __restrict__ elist,
+             const double* __restrict__ gval, const int32_t* __restrict__ rptr, const int32_t* __restrict__ rlist,
+             const double* __restrict__ grv, double* __restrict__ A, double* __restrict__ rhs) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nnz) {
+        double s = 0.0;
+        for (int k = eptr[t]; k < eptr[t + 1]; k++) s = s + gval[elist[k]];
+        A[t] = s;
+    } else if (t - nnz < rows) {
+        const int r = t - nnz;
+        double s = 0.0;
+        for (int k = rptr[r]; k < rptr[r + 1]; k++) s = s + grv[rlist[k]];
+        rhs[r] = s;
+    }
+}
+
+// An element entry outside the pattern is an error when its value is not
+// zero (csr_addto skips zeros before it looks for the entry).
+__global__ void __launch_bounds__(256)
+k_gather_bad(int nbad, const int32_t* __restrict__ bad, const double* __restrict__ gval, int* __restrict__ err) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nbad && gval[bad[t]] != 0.0) atomicOr(err, 1);
+}
+
+static int build_gather_plan(flgpu_ctx* c) {
+    BlockPlan& P = c->plan;
+    FLGPU_TRY(ensure_entry_positions(c));
+    if (P.gather_built) return FLGPU_OK;
+    const int loc = c->loc;
+    const int64_t n = (int64_t)c->nele * loc * loc;
+    if (c->nnz >= INT32_MAX || n >= INT32_MAX)
+        return set_error(FLGPU_EUNSUPPORTED, "two-pass assembly: more than 2^31 element entries");
+    const hipStream_t s = c->stream;
+    DevBuf<int32_t> ecnt, rcnt, nbad;
+    FLGPU_TRY(ecnt.alloc((size_t)c->nnz));
+    FLGPU_TRY(rcnt.alloc((size_t)c->rows));
+    FLGPU_TRY(nbad.alloc(1));
+    FLGPU_HIP(hipMemsetAsync(ecnt.p, 0, sizeof(int32_t) * c->nnz, s));
+    FLGPU_HIP(hipMemsetAsync(rcnt.p, 0, sizeof(int32_t) * c->rows, s));
+    FLGPU_HIP(hipMemsetAsync(nbad.p, 0, sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_gather_count, dim3(grid_for(n, 256)), dim3(256), 0, s, n, loc, c->col_elems.p, P.epos.p,
+                       c->conn.p, ecnt.p, rcnt.p, nbad.p);
+    FLGPU_HIP(hipGetLastError());
+    DevBuf<int64_t> e64, r64;
+    FLGPU_TRY(exclusive_scan(s, ecnt.p, c->nnz, e64));
+    FLGPU_TRY(exclusive_scan(s, rcnt.p, c->rows, r64));
+    int32_t h_nbad = 0;
+    int64_t h_ne = 0, h_nr = 0;
+    FLGPU_HIP(hipMemcpyAsync(&h_nbad, nbad.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    FLGPU_HIP(hipMemcpyAsync(&h_ne, e64.p + c->nnz, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    FLGPU_HIP(hipMemcpyAsync(&h_nr, r64.p + c->rows, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    FLGPU_HIP(hipStreamSynchronize(s));
+    FLGPU_TRY(P.gptr.alloc((size_t)c->nnz + 1));
+    FLGPU_TRY(P.rptr.alloc((size_t)c->rows + 1));
+    hipLaunchKernelGGL(k_ptr32, dim3(grid_for(c->nnz + 1, 256)), dim3(256), 0, s, c->nnz, e64.p, P.gptr.p);
+    hipLaunchKernelGGL(k_ptr32, dim3(grid_for((int64_t)c->rows + 1, 256)), dim3(256), 0, s, (int64_t)c->rows,
+                       r64.p, P.rptr.p);
+    FLGPU_TRY(P.glist.alloc((size_t)std::max<int64_t>(h_ne, 1)));
+    FLGPU_TRY(P.rlist.alloc((size_t)std::max<int64_t>(h_nr, 1)));
+    FLGPU_TRY(P.gbad.alloc((size_t)std::max(h_nbad, 1)));
+    FLGPU_HIP(hipMemsetAsync(ecnt.p, 0, sizeof(int32_t) * c->nnz, s));
+    FLGPU_HIP(hipMemsetAsync(rcnt.p, 0, sizeof(int32_t) * c->rows, s));
+    FLGPU_HIP(hipMemsetAsync(nbad.p, 0, sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_gather_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, n, loc, c->col_elems.p, P.epos.p,
+                       c->conn.p, P.gptr.p, P.rptr.p, ecnt.p, rcnt.p, P.glist.p, P.rlist.p, P.gbad.p, nbad.p);
+    hipLaunchKernelGGL(k_gather_sort, dim3(grid_for(c->nnz, 256)), dim3(256), 0, s, (int)c->nnz, P.gptr.p,
+                       P.glist.p);
+    hipLaunchKernelGGL(k_gather_sort, dim3(grid_for(c->rows, 256)), dim3(256), 0, s, c->rows, P.rptr.p, P.rlist.p);
+    FLGPU_HIP(hipGetLastError());
+    FLGPU_TRY(P.gval.alloc((size_t)n));
+    FLGPU_TRY(P.grv.alloc((size_t)c->nele * loc));
+    FLGPU_HIP(hipStreamSynchronize(s));  // the counters are freed on return
+    P.ngbad = h_nbad;
+    P.gather_built = true;
+    return FLGPU_OK;
+}
+
+template <int DIM, int LOC, typename R>
+static int launch_gather(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
+    constexpr int EPB = kEntryThreads / (LOC * LOC);
+    FLGPU_TRY(build_gather_plan(c));
+    BlockPlan& P = c->plan;
+    const uint32_t* img = nullptr;
+    FLGPU_TRY(table_image<R>(c, &img));
+    const int kst = fl.iso ? 1 : DIM * DIM;
+    const size_t lds = sizeof(R) * (size_t)EPB * entry_stage_per_element<DIM, LOC>(kst);
+    FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_colour_entry<DIM, LOC, R, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_asm_colour_entry<DIM, LOC, R, true>), dim3((c->nele + EPB - 1) / EPB),
+                       dim3(kEntryThreads), lds, c->stream, img, c->col_elems.p, c->nele, c->conn.p, c->conn_x.p,
+                       c->X4.p, fd, fl, nullptr, P.gval.p, P.grv.p, d_err);
+    hipLaunchKernelGGL(k_gather_sum, dim3(grid_for(c->nnz + c->rows, 256)), dim3(256), 0, c->stream, (int)c->nnz,
+                       c->rows, P.gptr.p, P.glist.p, P.gval.p, P.rptr.p, P.rlist.p, P.grv.p, c->A.p, c->rhs.p);
+    if (P.ngbad > 0)
+        hipLaunchKernelGGL(k_gather_bad, dim3(grid_for(P.ngbad, 256)), dim3(256), 0, c->stream, P.ngbad, P.gbad.p,
+                           P.gval.p, d_err);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
@@ -1577,10 +1771,13 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
         else if (fp32)
             blocked_ok = stageable(c, fl);
     }
-    if (scheme == FLGPU_ASM_BLOCKED && !blocked_ok)
+    // quadratic T, unstabilised: the two-pass scheme is the blocked one
+    const bool two_pass = !c->linear && !fl.stab && scheme != FLGPU_ASM_COLOUR;
+    if (scheme == FLGPU_ASM_BLOCKED && !blocked_ok && !two_pass)
         return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: the blocked scheme covers linear T (in FP32 and with "
                                              "stabilisation: with standard P1 tables and the field on the "
-                                             "coordinate mesh; in FP32 unstabilised: isotropic diffusivity)");
+                                             "coordinate mesh; in FP32 unstabilised: isotropic diffusivity) and "
+                                             "unstabilised quadratic T");
     if (scheme == FLGPU_ASM_AUTO) scheme = blocked_ok ? FLGPU_ASM_BLOCKED : FLGPU_ASM_COLOUR;
     if (scheme != FLGPU_ASM_COLOUR && scheme != FLGPU_ASM_BLOCKED)
         return set_error(FLGPU_EINVAL, "flgpu assemble: scheme %d", p->scheme);
@@ -1590,6 +1787,13 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     if (c->have_inactive) {
         FLGPU_HIP(hipMemsetAsync(c->inactive.p, 0, sizeof(int32_t) * c->rows, c->stream));
         c->have_inactive = false;
+    }
+    if (two_pass) {
+        if (dim == 3 && loc == 10)
+            return fp32 ? launch_gather<3, 10, float>(c, fd, fl, d_err) : launch_gather<3, 10, double>(c, fd, fl, d_err);
+        if (dim == 2 && loc == 6)
+            return fp32 ? launch_gather<2, 6, float>(c, fd, fl, d_err) : launch_gather<2, 6, double>(c, fd, fl, d_err);
+        return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: dim %d loc %d", dim, loc);
     }
     if (scheme == FLGPU_ASM_BLOCKED) {
         FLGPU_TRY(build_block_plan(c));

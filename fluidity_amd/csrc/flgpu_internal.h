@@ -154,6 +154,15 @@ struct BlockPlan {
     bool epos_built = false;
     DevBuf<uint32_t> ltab64, ltab32;
     std::vector<uint32_t> h_ltab64, h_ltab32;  // what the images hold
+    // quadratic T in two passes (element values, then per-entry sums): for
+    // every CSR entry (gptr, glist) and rhs row (rptr, rlist) the element
+    // values it receives, as indices q*LOC^2 + entry / q*LOC + row into
+    // gval / grv, where q is the element's place in colour order, ascending;
+    // gbad lists the element entries the pattern lacks
+    bool gather_built = false;
+    DevBuf<int32_t> gptr, glist, rptr, rlist, gbad;
+    int32_t ngbad = 0;
+    DevBuf<double> gval, grv;
     // stabilised node-staged assembly: nu_bar per element and quadrature
     // point (FP64 or FP32 values, as 8-byte words)
     DevBuf<double> nub;
@@ -260,6 +269,8 @@ int pc_apply_host(flgpu_ctx* c, const flgpu_solver_params* sp, const double* r, 
 void learned_free(flgpu_ctx* c);
 // shared helpers
 unsigned grid_for(int64_t n, int block);
+// setup_gpu.hip: out[0..n) = exclusive prefix sums of in[0..n), out[n] = the total
+int exclusive_scan(hipStream_t s, const int32_t* in, int64_t n, DevBuf<int64_t>& out);
 // flgpu_context.hip: bind a colouring already known to be valid (0-based)
 int colouring_adopt(flgpu_ctx* c, std::vector<int32_t>& col_ptr, std::vector<int32_t>& elems);
 }  // namespace flgpu

@@ -58,6 +58,8 @@ k_scan_add(int64_t n, int64_t* __restrict__ out, const int64_t* __restrict__ tof
     if (i < n) out[i] += toff[i / kScanTile];
 }
 
+}  // namespace
+
 // out[0..n) = exclusive prefix sums of in[0..n), out[n] = total (int64).
 int exclusive_scan(hipStream_t s, const int32_t* in, int64_t n, DevBuf<int64_t>& out) {
     FLGPU_TRY(out.alloc((size_t)n + 1));
@@ -83,6 +85,8 @@ int exclusive_scan(hipStream_t s, const int32_t* in, int64_t n, DevBuf<int64_t>&
     FLGPU_HIP(hipStreamSynchronize(s));  // h and run go out of scope
     return FLGPU_OK;
 }
+
+namespace {
 
 // node -> element incidence
 __global__ void __launch_bounds__(256)

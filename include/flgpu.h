@@ -186,8 +186,11 @@ enum {
     FLGPU_ASM_AUTO = 0,     /* fastest available */
     FLGPU_ASM_COLOUR = 1,   /* one launch per colour, element per thread,
                                read-modify-write of the CSR in HBM */
-    FLGPU_ASM_BLOCKED = 2   /* row blocks in LDS, colours in order inside
-                               each block (same bits as FLGPU_ASM_COLOUR) */
+    FLGPU_ASM_BLOCKED = 2   /* linear T: row blocks in LDS, colours in
+                               order inside each block; quadratic T
+                               (unstabilised): element values in one pass,
+                               then one thread per CSR entry adds them in
+                               colour order (same bits as FLGPU_ASM_COLOUR) */
 };
 
 /* assemble_advection_diffusion_cg's element loop
