@@ -72,6 +72,15 @@ def loss_grad(problem, nprobe: int = 8, seed: int = 0):
     return float(loss.value), g
 
 
+def loss(problem, nprobe: int = 8, seed: int = 0) -> float:
+    """The probed loss alone (no gradient), e.g. on a fixed validation system
+    with fixed probes."""
+    val = C.c_float(0)
+    L.check(problem.lib.flgpu_learned_loss_grad(problem.ctx, nprobe, C.c_uint64(seed), C.byref(val), None),
+            "flgpu_learned_loss_grad")
+    return float(val.value)
+
+
 def train_step(problem, lr: float = 1e-3, nprobe: int = 8, seed: int = 0) -> float:
     loss = C.c_float(0)
     L.check(problem.lib.flgpu_learned_train_step(problem.ctx, nprobe, C.c_uint64(seed), C.c_float(lr),
@@ -115,10 +124,25 @@ def training_systems(count: int = 512, seed: int = 2024, n_min: int = 27, n_max:
             k += 1
 
 
+def validation_system(seed: int = 777, n: int = 40, contrast: float = 1e3):
+    """A held-out system for tracking training: an n^3 x 6 jittered cube
+    mesh and a log-normal diffusivity field from seeds the training set does
+    not use."""
+    from . import meshgen as M
+
+    mesh = M.cube_tets(n, jitter=0.15, seed=seed)
+    kap = M.grf_lognormal(mesh.node_pos, seed * 7919 + 1, 0.1, contrast)
+    return M.Config("validation", mesh, 2, np.zeros(mesh.nnodes), None, kap, M.sin_source(mesh.node_pos), None,
+                    1.0, 1.0, 0.0, False, False, "cg", 1e-8)
+
+
 def train(problem_factory, systems, steps_per_system: int = 4, lr: float = 1e-3, nprobe: int = 8,
-          seed: int = 0, log=None, init_seed: int | None = 1, params: np.ndarray | None = None):
+          seed: int = 0, log=None, init_seed: int | None = 1, params: np.ndarray | None = None,
+          validate=None, every: int = 64):
     """Adam on the probed loss over a stream of systems.  problem_factory()
-    returns a CGProblem whose context keeps the network across rebinds."""
+    returns a CGProblem whose context keeps the network across rebinds.
+    validate(step, params), if given, runs every `every` steps and at the
+    end."""
     from . import cg
 
     pr = None
@@ -143,6 +167,10 @@ def train(problem_factory, systems, steps_per_system: int = 4, lr: float = 1e-3,
         for _ in range(steps_per_system):
             losses.append(train_step(pr, lr=lr, nprobe=nprobe, seed=seed * 100003 + step))
             step += 1
-        if log and step % 64 < steps_per_system:
+        if log and step % every < steps_per_system:
             log(f"step {step}: loss {np.mean(losses[-64:]):.4f} ({time.time() - t0:.1f}s)")
+        if validate and step % every < steps_per_system:
+            validate(step, get_params(pr))
+    if validate and pr is not None:
+        validate(step, get_params(pr))
     return pr, np.array(losses)

@@ -40,16 +40,38 @@ def main(argv=None) -> int:
 
     systems = learned.training_systems(args.systems, args.seed, args.n_min, args.n_max, args.meshes)
     params = learned.load(args.init) if args.init else None
+    # a held-out system: the loss with fixed probes every 64 steps, and the
+    # iterations of learned FCG against Jacobi-PCG at the start and the end
+    vcfg = learned.validation_system()
+    vpr = cg.problem_from_config(vcfg)
+    cg.assemble_advection_diffusion_cg(vpr, cg.options_from_config(vcfg), vcfg.dt)
+    cg.apply_dirichlet_conditions(vpr, vcfg.mesh.boundary, 0.0, vcfg.dt)
+    _, jinfo = cg.petsc_solve(vpr, cg.SolverOptions("cg", "jacobi", relative_error=1e-8, norm="unpreconditioned"))
+    history = []
+
+    def validate(step, th):
+        learned.set_params(vpr, th)
+        vloss = learned.loss(vpr, nprobe=8, seed=4242)
+        _, info = cg.petsc_solve(vpr, cg.SolverOptions("fcg", "learned", relative_error=1e-8,
+                                                       norm="unpreconditioned"))
+        history.append({"step": int(step), "val_loss": vloss, "val_its": int(info.its),
+                        "fallback": int(info.fallback)})
+        log(f"validation step {step}: loss {vloss:.5f}, learned FCG {info.its} its "
+            f"(Jacobi-PCG {jinfo.its})")
+
     t0 = time.time()
     pr, losses = learned.train(lambda cfg: cg.CGProblem(cfg.mesh, quad_degree=cfg.quad_degree), systems,
-                               args.steps_per_system, args.lr, args.nprobe, args.seed, log, params=params)
+                               args.steps_per_system, args.lr, args.nprobe, args.seed, log, params=params,
+                               validate=validate)
     th = learned.get_params(pr)
     learned.save(args.out, th)
     summary = {"steps": int(losses.size), "loss_first64": float(np.mean(losses[:64])),
                "loss_last64": float(np.mean(losses[-64:])), "seconds": round(time.time() - t0, 1),
+               "validation": {"nodes": vcfg.mesh.nnodes, "jacobi_its": int(jinfo.its), "history": history},
                "out": args.out}
     log(json.dumps(summary))
     pr.close()
+    vpr.close()
     return 0
 
 

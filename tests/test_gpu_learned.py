@@ -115,3 +115,24 @@ def test_stalled_network_falls_back_to_jacobi(require_gpu):
     assert info.fallback == 1
     assert info.reason > 0 and info.true_relres <= 1e-8
     pr.close()
+
+
+@pytest.mark.parametrize("name,scale,seed", [("c4", 12, 9101), ("c5", 14, 9102)])
+def test_committed_weights_on_unseen_system(require_gpu, name, scale, seed):
+    """The shipped weights (trained on the GPU by fluidity_amd.train_learned)
+    on a system none of the training seeds made: learned FCG converges to
+    the true residual without falling back, in fewer iterations than
+    Jacobi-PCG, and agrees with it."""
+    cfg, pr, g = system(name, scale=scale, seed=seed)
+    learned.set_params(pr, learned.load())
+    print(f"\n[case] committed weights on {name} scale {scale} seed {seed}", flush=True)
+    so_j = cg.SolverOptions("cg", "jacobi", relative_error=1e-8, norm="unpreconditioned")
+    xj, ij = cg.petsc_solve(pr, so_j)
+    so_l = cg.SolverOptions("fcg", "learned", relative_error=1e-8, norm="unpreconditioned")
+    xl, il = cg.petsc_solve(pr, so_l)
+    print("jacobi its", ij.its, "learned its", il.its, "fallback", il.fallback)
+    assert il.reason > 0 and il.true_relres <= 1e-8
+    assert il.fallback == 0
+    assert il.its < ij.its
+    assert np.max(np.abs(xl - xj)) <= 1e-6 * np.max(np.abs(xj))
+    pr.close()
