@@ -102,6 +102,12 @@ int flgpu_init(flgpu_ctx** out, int32_t device) {
             delete c;
             return set_error(FLGPU_EHIP, "flgpu_init: %s", hipGetErrorString(e));
         }
+        const int rc = noalloc_setup(device);  // the device address of flgpu_solve_noalloc's array
+        if (rc != FLGPU_OK) {
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return rc;
+        }
         *out = c;
         return FLGPU_OK;
     });
@@ -605,6 +611,15 @@ int flgpu_solve(flgpu_ctx* c, const flgpu_solver_params* sp, double* x, flgpu_so
         if (!c->assembled) return set_error(FLGPU_EINVAL, "flgpu_solve: no system on the device");
         FLGPU_HIP(hipSetDevice(c->device));
         return solve(c, sp, x, info);
+    });
+}
+
+int flgpu_solve_noalloc(flgpu_ctx* c, const flgpu_solver_params* sp, double* x, flgpu_solve_info* info) {
+    return guarded("flgpu_solve_noalloc", [&]() -> int {
+        if (!c || !sp) return set_error(FLGPU_EINVAL, "flgpu_solve_noalloc: null argument");
+        if (!c->assembled) return set_error(FLGPU_EINVAL, "flgpu_solve_noalloc: no system on the device");
+        FLGPU_HIP(hipSetDevice(c->device));
+        return solve_noalloc(c, sp, x, info);
     });
 }
 

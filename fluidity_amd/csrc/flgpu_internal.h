@@ -257,6 +257,10 @@ int time_assemble(flgpu_ctx* c, const flgpu_advdif_params* p, int reps, double* 
 // krylov.hip
 int solve(flgpu_ctx* c, const flgpu_solver_params* sp, double* x_host,
           flgpu_solve_info* info);
+// the allocation-free solve: its work lives in a fixed device array set
+// up once per device by noalloc_setup (flgpu_init)
+int solve_noalloc(flgpu_ctx* c, const flgpu_solver_params* sp, double* x_host, flgpu_solve_info* info);
+int noalloc_setup(int device);
 void krylov_free(flgpu_ctx* c);
 void krylov_invalidate(flgpu_ctx* c);  // the assembled system changed
 int solver_prepare(flgpu_ctx* c, int precision);

@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "flgpu_internal.h"
@@ -1239,6 +1240,141 @@ void krylov_invalidate(flgpu_ctx* c) {
 void krylov_free(flgpu_ctx* c) {
     delete c->kw;
     c->kw = nullptr;
+}
+
+// ---------------------------------------------------------------------
+// The solve without allocation
+// ---------------------------------------------------------------------
+//
+// Every buffer of the FP64 (F)PCG -- reduced matrix values, the vectors,
+// per-block partial sums, the two solver-scalar slots and the row blocks of
+// the streamed product -- is carved out of one file-scope device array.
+// The host code never names the array itself (in host code its name is a
+// host-side stand-in): it takes the array's device address once per GPU
+// from hipGetSymbolAddress (noalloc_setup, called by flgpu_init with the
+// device current) and passes that address on.  Every context on a GPU
+// shares that GPU's array, so solves that overlap on one GPU race.
+
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kNoAllocDevices] = {};
+
+int noalloc_setup(int device) {
+    if (device < 0 || device >= kNoAllocDevices)
+        return set_error(FLGPU_EUNSUPPORTED, "flgpu_init: device %d beyond the %d the library tracks", device,
+                         kNoAllocDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return FLGPU_OK;
+    void* p = nullptr;
+    FLGPU_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_error(FLGPU_EHIP, "flgpu_init: no device address for the solver work array");
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+    return FLGPU_OK;
+}
+
+static unsigned char* noalloc_ptr(int device) {
+    if (device < 0 || device >= kNoAllocDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+// Row blocks of a fixed number of rows, rows_per * maxlen <= kStreamNnz
+// (or one row per block): computed on the device, no host table.
+__global__ void __launch_bounds__(256)
+k_fixed_row_blocks(int n, int rows_per, int nrb, const int32_t* __restrict__ rowptr, int32_t* __restrict__ rb,
+                   int32_t* __restrict__ rbk) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nrb) return;
+    const int r = (int)std::min<int64_t>((int64_t)b * rows_per, n);
+    rb[b] = r;
+    rbk[b] = rowptr[r];
+}
+
+// Carves the work of an n-row, nnz-entry FP64 (F)PCG out of the array;
+// false when it does not fit (*need gets the bytes it would take).
+static bool noalloc_carve(unsigned char* base, int n, int64_t nnz, int nrb, KrylovWork& k, size_t* need) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off = (off + bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    const size_t o_val = take(sizeof(double) * (size_t)nnz);
+    size_t o_vec[7];
+    for (size_t& o : o_vec) o = take(sizeof(double) * (size_t)n);
+    const size_t o_part = take(sizeof(double) * (size_t)kPartStride * PART_COMPONENTS);
+    const size_t o_scal = take(sizeof(Scal) * 2);
+    const size_t o_rb = take(sizeof(int32_t) * ((size_t)nrb + 1));
+    const size_t o_rbk = take(sizeof(int32_t) * ((size_t)nrb + 1));
+    *need = off;
+    if (off > kNoAllocBytes) return false;
+    k.val64.borrow(reinterpret_cast<double*>(base + o_val), (size_t)nnz);
+    DevBuf<double>* vecs[7] = {&k.b, &k.x, &k.r, &k.z, &k.p, &k.w, &k.dinv};
+    for (int q = 0; q < 7; q++) vecs[q]->borrow(reinterpret_cast<double*>(base + o_vec[q]), (size_t)n);
+    k.partials.borrow(reinterpret_cast<double*>(base + o_part), (size_t)kPartStride * PART_COMPONENTS);
+    k.scal.borrow(reinterpret_cast<Scal*>(base + o_scal), 2);
+    k.rb.borrow(reinterpret_cast<int32_t*>(base + o_rb), (size_t)nrb + 1);
+    k.rbk.borrow(reinterpret_cast<int32_t*>(base + o_rbk), (size_t)nrb + 1);
+    return true;
+}
+
+int solve_noalloc(flgpu_ctx* c, const flgpu_solver_params* sp, double* x_host, flgpu_solve_info* info_out) {
+    if (!x_host) return set_error(FLGPU_EINVAL, "flgpu_solve_noalloc: x is required (host, rows values)");
+    if (sp->precision != FLGPU_FP64)
+        return set_error(FLGPU_EUNSUPPORTED, "flgpu_solve_noalloc: FP64 only (precision %d)", sp->precision);
+    if (sp->ksp != FLGPU_KSP_CG && sp->ksp != FLGPU_KSP_FCG)
+        return set_error(FLGPU_EUNSUPPORTED, "flgpu_solve_noalloc: cg or fcg only (ksp %d)", sp->ksp);
+    if (sp->pc != FLGPU_PC_NONE && sp->pc != FLGPU_PC_JACOBI)
+        return set_error(FLGPU_EUNSUPPORTED, "flgpu_solve_noalloc: pc none or jacobi only (pc %d)", sp->pc);
+    unsigned char* base = noalloc_ptr(c->device);
+    if (!base) return set_error(FLGPU_EINVAL, "flgpu_solve_noalloc: device %d was not set up by flgpu_init", c->device);
+    const int n = c->rows;
+    int maxlen = 0;
+    for (int i = 0; i < n; i++) maxlen = std::max(maxlen, c->h_rowptr[i + 1] - c->h_rowptr[i]);
+    const int rows_per = std::max(1, std::min(kRowBlock, kStreamNnz / std::max(1, maxlen)));
+    const int nrb = (int)(((int64_t)n + rows_per - 1) / rows_per);
+    KrylovWork k;  // every buffer a view of the array; nothing allocated or freed
+    size_t need = 0;
+    if (!noalloc_carve(base, n, c->nnz, nrb, k, &need))
+        return set_error(FLGPU_EUNSUPPORTED,
+                         "flgpu_solve_noalloc: a system of %d rows and %lld entries needs %zu bytes of solver work, "
+                         "more than the %zu bytes of its fixed array",
+                         n, (long long)c->nnz, need, kNoAllocBytes);
+    Scal h_scal;  // pageable: read_scal / write_scal synchronise
+    k.n = n;
+    k.nnz = c->nnz;
+    k.use16 = false;
+    k.nrb = nrb;
+    k.h_scal = &h_scal;
+    k.h_scal_owned = false;
+    k.nblk = std::max(1, (int)std::min<int64_t>(kMaxVecGrid, ((int64_t)n + kVecBlock - 1) / kVecBlock));
+    flgpu_solve_info info{};
+    KrylovWork* saved = c->kw;
+    c->kw = &k;
+    const int rc = [&]() -> int {
+        hipLaunchKernelGGL(k_fixed_row_blocks, dim3(grid_for((int64_t)nrb + 1, 256)), dim3(256), 0, c->stream, n,
+                           rows_per, nrb, c->rowptr.p, k.rb.p, k.rbk.p);
+        FLGPU_HIP(hipGetLastError());
+        if (!sp->start_from_zero)
+            FLGPU_HIP(hipMemcpyAsync(k.x.p, x_host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        FLGPU_TRY(build_reduced(c, &k, false, sp->start_from_zero != 0));
+        FLGPU_TRY(run_cg<double>(c, sp, &info, false));
+        hipLaunchKernelGGL(k_finish_x, dim3(grid_for(n, kVecBlock)), dim3(kVecBlock), 0, c->stream, n, k.x.p,
+                           c->rhs.p, c->inactive.p, c->have_inactive ? 1 : 0);
+        FLGPU_HIP(hipGetLastError());
+        FLGPU_HIP(hipMemcpyAsync(x_host, k.x.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        FLGPU_HIP(hipStreamSynchronize(c->stream));
+        return FLGPU_OK;
+    }();
+    c->kw = saved;
+    if (rc != FLGPU_OK) {
+        (void)hipStreamSynchronize(c->stream);  // no kernel still reads the views
+        return rc;
+    }
+    if (info_out) *info_out = info;
+    return FLGPU_OK;
 }
 
 int solution_get(flgpu_ctx* c, double* x) {

@@ -329,6 +329,22 @@ typedef struct {
 int flgpu_solve(flgpu_ctx *ctx, const flgpu_solver_params *sp, double *x,
                 flgpu_solve_info *info);
 
+/* flgpu_solve without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * Krylov work (reduced matrix values, seven vectors, partial sums, solver
+ * scalars, row blocks: about 8*nnz + 56*rows bytes plus 180 KiB) lives in
+ * one fixed device array of 64 MiB per GPU, set up by flgpu_init and held
+ * until the process ends; a larger system is rejected with
+ * FLGPU_EUNSUPPORTED.  FP64 CG or FCG with pc none or jacobi; x (host,
+ * rows values) is required: in = initial guess (unless start_from_zero),
+ * out = solution (the solution is not kept for flgpu_solution_get).  All
+ * contexts on a GPU share that GPU's array, which keeps the last call's
+ * contents: calls that overlap on one GPU (several threads or contexts)
+ * race. */
+int flgpu_solve_noalloc(flgpu_ctx *ctx, const flgpu_solver_params *sp,
+                        double *x, flgpu_solve_info *info);
+
 /* Drop-in form of (ii): takes host arrays like petsc_solve(x, matrix, rhs).
  * A_val / rhs / inactive replace the device-resident system when non-NULL. */
 int flgpu_petsc_solve(flgpu_ctx *ctx, const flgpu_solver_params *sp,
