@@ -392,19 +392,24 @@ constexpr int kEntryThreads = 256;
 
 template <int DIM, int LOC>
 __host__ __device__ constexpr int entry_stage_per_element(int kst) {
-    return quad_stage_per_element<DIM, LOC>(kst) + DIM * (DIM + 1) + 2 * LOC * LOC;
+    return quad_stage_per_element<DIM, LOC>(kst) + DIM * (DIM + 1) + 2 * LOC * LOC + kElemMaxGi * LOC * DIM;
 }
 
-// GATHER: the first pass of the two-pass scheme -- elems is the whole
-// colour order and element q's entries and row values are stored at
-// A[q*LOC^2 + entry] and rhs[q*LOC + row] (zeros included) for the
+// GATHER: the first pass of the two-pass scheme -- elems is null and the
+// workgroups take the elements in id order (neighbours share nodes, so the
+// staging gathers hit in L2); element e's entries and row values are stored
+// at A[e*LOC^2 + entry] and rhs[e*LOC + row] (zeros included) for the
 // per-entry sums of k_gather_sum; epos is not read.
-template <int DIM, int LOC, typename R, bool GATHER = false>
+// FEAT: F_ALL for isotropic (or no) diffusivity, F_ALL | F_ANISO for a
+// tensor, so that the isotropic kernel holds no tensor in registers.
+template <int DIM, int LOC, typename R, bool GATHER = false, int FEAT = F_ALL | F_ANISO>
 __global__ void __launch_bounds__(kEntryThreads)
 k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict__ elems, int count,
                    const int32_t* __restrict__ conn, const int32_t* __restrict__ conn_x,
-                   const double* __restrict__ X4, AsmFields fd, AsmFlags fl, const int32_t* __restrict__ epos,
+                   const double* __restrict__ X4, AsmFields fd, AsmFlags fl_in, const int32_t* __restrict__ epos,
                    double* __restrict__ A, double* __restrict__ rhs, int* __restrict__ err) {
+    AsmFlags fl = fl_in;
+    fl.iso = (FEAT & F_ANISO) ? fl.iso : 1;
     constexpr int L2 = LOC * LOC;
     constexpr int NT = kEntryThreads;
     constexpr int EPB = NT / L2;  // elements per workgroup
@@ -420,9 +425,10 @@ k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict_
     R *s_uq = me, *s_dv = s_uq + NG * DIM, *s_kq = s_dv + NG, *s_sq = s_kq + NG * kst, *s_fq = s_sq + NG;
     R *n_u = s_fq + NG, *n_k = n_u + LOC * DIM, *n_s = n_k + LOC * kst, *n_b = n_s + LOC, *n_t = n_b + LOC;
     R *s_x = n_t + LOC, *s_m = s_x + DIM * (DIM + 1), *s_s = s_m + L2;
+    R* s_gr = s_s + L2;  // physical shape gradients, [(g * LOC + i) * DIM + d]
     int ele = 0;
     if (live) {
-        ele = elems[q];
+        ele = elems ? elems[q] : q;
         if (j == 0) {  // thread (i, 0) stages node i (and vertex i's coordinates)
             const int nd = conn[(int64_t)ele * LOC + i];
             n_t[i] = (R)fd.T[nd];
@@ -451,6 +457,13 @@ k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict_
 #pragma unroll
             for (int nn = 0; nn < DIM + 1; nn++) X[a][nn] = s_x[a * (DIM + 1) + nn];
         geometry<DIM, R>(X, tb, iJ, adet);
+        // every (node, point) gradient once per element instead of twice per
+        // entry and point (the same operations, so the same bits)
+        for (int t = rr; t < LOC * tb.ngi; t += L2) {
+            const int g = t / LOC, nn = t - g * LOC;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) s_gr[t * DIM + d] = phys_grad<DIM, LOC, R>(tb, iJ, nn, g, d);
+        }
         if (rr < tb.ngi) {  // thread rr forms the values at point rr
             const int g = rr;
             const LdsNodal<DIM, LOC, R> ln{n_u, n_k, n_s, n_b, kst};
@@ -481,8 +494,9 @@ k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict_
     __syncthreads();  // point values
     R m_ij = (R)0, s_ij = (R)0, src_i = (R)0;
     if (live) {
-        entry_general_q<DIM, LOC, R>(tb, fl, iJ, adet, LdsQuad<DIM, R>{s_uq, s_dv, s_kq, s_sq, s_fq, kst}, i, j,
-                                     m_ij, s_ij, src_i);
+        entry_general_q<DIM, LOC, R, LdsQuad<DIM, R>, FEAT>(tb, fl, iJ, adet,
+                                                            LdsQuad<DIM, R>{s_uq, s_dv, s_kq, s_sq, s_fq, kst}, i,
+                                                            j, m_ij, s_ij, src_i, s_gr);
         s_m[rr] = m_ij;
         s_s[rr] = s_ij;
     }
@@ -497,12 +511,12 @@ k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict_
     const R dtte = fl.dtt_on ? (R)fl.dt_theta : (R)0;
     const double v = (double)FMA(dtte, s_ij, m);
     if (GATHER) {
-        A[(int64_t)q * L2 + rr] = v;
+        A[(int64_t)ele * L2 + rr] = v;
         if (j == 0) {
             R r = (R)0;
             for (int jj = 0; jj < LOC; jj++) r = FMA(-s_s[i * LOC + jj], n_t[jj], r);
             if (fl.src) r = r + src_i;
-            rhs[(int64_t)q * LOC + i] = (double)r;
+            rhs[(int64_t)ele * LOC + i] = (double)r;
         }
         return;
     }
@@ -522,38 +536,45 @@ k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict_
     }
 }
 
-template <int DIM, int LOC, typename R>
-static int launch_colour_entry(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
+template <int DIM, int LOC, typename R, int FEAT>
+static int launch_colour_entry_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
     constexpr int EPB = kEntryThreads / (LOC * LOC);
     FLGPU_TRY(ensure_entry_positions(c));
     const uint32_t* img = nullptr;
     FLGPU_TRY(table_image<R>(c, &img));
     const int kst = fl.iso ? 1 : DIM * DIM;
     const size_t lds = sizeof(R) * (size_t)EPB * entry_stage_per_element<DIM, LOC>(kst);
-    FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_colour_entry<DIM, LOC, R>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    auto kern = k_asm_colour_entry<DIM, LOC, R, false, FEAT>;
+    FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds));
     for (int k = 0; k < c->ncol; k++) {
         const int off = c->h_col_ptr[k], cnt = c->h_col_ptr[k + 1] - off;
         if (cnt == 0) continue;
-        hipLaunchKernelGGL((k_asm_colour_entry<DIM, LOC, R>), dim3((cnt + EPB - 1) / EPB), dim3(kEntryThreads), lds,
-                           c->stream, img, c->col_elems.p + off, cnt, c->conn.p, c->conn_x.p, c->X4.p, fd, fl,
-                           c->plan.epos.p, c->A.p, c->rhs.p, d_err);
+        hipLaunchKernelGGL(kern, dim3((cnt + EPB - 1) / EPB), dim3(kEntryThreads), lds, c->stream, img,
+                           c->col_elems.p + off, cnt, c->conn.p, c->conn_x.p, c->X4.p, fd, fl, c->plan.epos.p,
+                           c->A.p, c->rhs.p, d_err);
     }
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
 
+template <int DIM, int LOC, typename R>
+static int launch_colour_entry(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
+    if (fl.diff && !fl.iso) return launch_colour_entry_feat<DIM, LOC, R, F_ALL | F_ANISO>(c, fd, fl, d_err);
+    return launch_colour_entry_feat<DIM, LOC, R, F_ALL>(c, fd, fl, d_err);
+}
+
 // ---------------------------------------------------------------------
 // Two-pass scheme (quadratic T).  Pass 1 (k_asm_colour_entry<..., true>)
-// forms every element's entries and row values in one launch over the
-// whole colour order and stores them; pass 2 (k_gather_sum) gives each CSR
+// forms every element's entries and row values in one launch over all
+// elements and stores them by element id; pass 2 (k_gather_sum) gives each CSR
 // entry and rhs row one thread that adds its element values in colour
 // order, ascending element id inside a colour -- the order in which the
 // colour scheme's launches add them, and adding a zero leaves a sum's bits
 // unchanged as csr_addto's skip of zeros does, so both schemes give the
 // same bits.  The per-entry lists are built once per mesh, sparsity and
 // colouring from the entry positions (counted, scanned, filled, and each
-// list sorted by colour-order place).
+// list sorted by colour-order place, then rewritten as element ids).
 // ---------------------------------------------------------------------
 
 __global__ void __launch_bounds__(256)
@@ -612,6 +633,16 @@ k_gather_sort(int n, const int32_t* __restrict__ ptr, int32_t* __restrict__ list
         }
         list[m + 1] = v;
     }
+}
+
+// A list sorted by colour-order place (q*per + k) rewritten as element
+// ids (elems[q]*per + k), the indexing of the first pass's output.
+__global__ void __launch_bounds__(256)
+k_gather_remap(int64_t n, int per, const int32_t* __restrict__ elems, int32_t* __restrict__ list) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int t = list[k], q = t / per;
+    list[k] = elems[q] * per + (t - q * per);
 }
 
 __global__ void __launch_bounds__(256)
@@ -690,6 +721,15 @@ static int build_gather_plan(flgpu_ctx* c) {
     hipLaunchKernelGGL(k_gather_sort, dim3(grid_for(c->nnz, 256)), dim3(256), 0, s, (int)c->nnz, P.gptr.p,
                        P.glist.p);
     hipLaunchKernelGGL(k_gather_sort, dim3(grid_for(c->rows, 256)), dim3(256), 0, s, c->rows, P.rptr.p, P.rlist.p);
+    if (h_ne > 0)
+        hipLaunchKernelGGL(k_gather_remap, dim3(grid_for(h_ne, 256)), dim3(256), 0, s, h_ne, loc * loc,
+                           c->col_elems.p, P.glist.p);
+    if (h_nr > 0)
+        hipLaunchKernelGGL(k_gather_remap, dim3(grid_for(h_nr, 256)), dim3(256), 0, s, h_nr, loc, c->col_elems.p,
+                           P.rlist.p);
+    if (h_nbad > 0)
+        hipLaunchKernelGGL(k_gather_remap, dim3(grid_for(h_nbad, 256)), dim3(256), 0, s, (int64_t)h_nbad, loc * loc,
+                           c->col_elems.p, P.gbad.p);
     FLGPU_HIP(hipGetLastError());
     FLGPU_TRY(P.gval.alloc((size_t)n));
     FLGPU_TRY(P.grv.alloc((size_t)c->nele * loc));
@@ -708,11 +748,12 @@ static int launch_gather(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, 
     FLGPU_TRY(table_image<R>(c, &img));
     const int kst = fl.iso ? 1 : DIM * DIM;
     const size_t lds = sizeof(R) * (size_t)EPB * entry_stage_per_element<DIM, LOC>(kst);
-    FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_asm_colour_entry<DIM, LOC, R, true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_asm_colour_entry<DIM, LOC, R, true>), dim3((c->nele + EPB - 1) / EPB),
-                       dim3(kEntryThreads), lds, c->stream, img, c->col_elems.p, c->nele, c->conn.p, c->conn_x.p,
-                       c->X4.p, fd, fl, nullptr, P.gval.p, P.grv.p, d_err);
+    auto kern = (fl.diff && !fl.iso) ? k_asm_colour_entry<DIM, LOC, R, true, F_ALL | F_ANISO>
+                                     : k_asm_colour_entry<DIM, LOC, R, true, F_ALL>;
+    FLGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds));
+    hipLaunchKernelGGL(kern, dim3((c->nele + EPB - 1) / EPB), dim3(kEntryThreads), lds, c->stream, img, nullptr,
+                       c->nele, c->conn.p, c->conn_x.p, c->X4.p, fd, fl, nullptr, P.gval.p, P.grv.p, d_err);
     hipLaunchKernelGGL(k_gather_sum, dim3(grid_for(c->nnz + c->rows, 256)), dim3(256), 0, c->stream, (int)c->nnz,
                        c->rows, P.gptr.p, P.glist.p, P.gval.p, P.rptr.p, P.rlist.p, P.grv.p, c->A.p, c->rhs.p);
     if (P.ngbad > 0)

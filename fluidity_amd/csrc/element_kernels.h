@@ -609,13 +609,29 @@ FLGPU_HD_INLINE void row_general_q(const LTab<R>& tb, const AsmFlags& fl, const 
     }
 }
 
+// Component d of the physical gradient of shape function i at point g:
+// sum_k dn(i,g,k) invJ(d,k), accumulated in k order (the dshape of
+// transform_to_physical, Transform_elements.F90:769-781).
+template <int DIM, int LOC, typename R>
+FLGPU_HD_INLINE R phys_grad(const LTab<R>& tb, const R (&iJ)[DIM][DIM], int i, int g, int d) {
+    R s = (R)0;
+#pragma unroll
+    for (int k = 0; k < DIM; k++) s = FMA(tb.dn[i + LOC * (g + tb.ngi * k)], iJ[d][k], s);
+    return s;
+}
+
 // Entry (i, j) of the quadratic-T element sums of row_general_q, alone:
 // the mass term, the summed advection/absorption/diffusion term, and row
 // i's source integral, each accumulated over the quadrature points in the
 // same order with the same operations as there (so the same bits).
-template <int DIM, int LOC, typename R, class QV>
-FLGPU_HD_INLINE void entry_general_q(const LTab<R>& tb, const AsmFlags& fl, const R (&iJ)[DIM][DIM], R adet,
-                                     const QV& qv, int i, int j, R& m_ij, R& s_ij, R& src_i) {
+// FEAT: the terms compiled in (without F_ANISO the diffusivity is taken as
+// isotropic whatever fl says; the caller picks a set that holds the flags).
+template <int DIM, int LOC, typename R, class QV, int FEAT = F_ALL | F_ANISO>
+FLGPU_HD_INLINE void entry_general_q(const LTab<R>& tb, const AsmFlags& fl_in, const R (&iJ)[DIM][DIM], R adet,
+                                     const QV& qv, int i, int j, R& m_ij, R& s_ij, R& src_i,
+                                     const R* gt = nullptr) {
+    AsmFlags fl = fl_in;
+    fl.iso = (FEAT & F_ANISO) ? fl.iso : 1;
     const int ngi = tb.ngi;
     R M = (R)0, Sm = (R)0, src = (R)0;
 #pragma unroll 1
@@ -623,12 +639,12 @@ FLGPU_HD_INLINE void entry_general_q(const LTab<R>& tb, const AsmFlags& fl, cons
         const R dw = adet * tb.w[g];
         const R ni = tb.n[i + LOC * g];
         R Gi[DIM];
+        if (gt) {
 #pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            R s = (R)0;
+            for (int d = 0; d < DIM; d++) Gi[d] = gt[(g * LOC + i) * DIM + d];
+        } else {
 #pragma unroll
-            for (int k = 0; k < DIM; k++) s = FMA(tb.dn[i + LOC * (g + ngi * k)], iJ[d][k], s);
-            Gi[d] = s;
+            for (int d = 0; d < DIM; d++) Gi[d] = phys_grad<DIM, LOC, R>(tb, iJ, i, g, d);
         }
         R uq[DIM] = {};
         R div = (R)0;
@@ -666,12 +682,12 @@ FLGPU_HD_INLINE void entry_general_q(const LTab<R>& tb, const AsmFlags& fl, cons
         if (fl.have_mass) M = FMA(nn, dw, M);
         if (fl.adv || fl.diff) {
             R Gj[DIM];
+            if (gt) {
 #pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                R s = (R)0;
+                for (int d = 0; d < DIM; d++) Gj[d] = gt[(g * LOC + j) * DIM + d];
+            } else {
 #pragma unroll
-                for (int k = 0; k < DIM; k++) s = FMA(tb.dn[j + LOC * (g + ngi * k)], iJ[d][k], s);
-                Gj[d] = s;
+                for (int d = 0; d < DIM; d++) Gj[d] = phys_grad<DIM, LOC, R>(tb, iJ, j, g, d);
             }
             if (fl.adv) {
                 R a;

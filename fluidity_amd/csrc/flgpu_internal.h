@@ -156,9 +156,9 @@ struct BlockPlan {
     std::vector<uint32_t> h_ltab64, h_ltab32;  // what the images hold
     // quadratic T in two passes (element values, then per-entry sums): for
     // every CSR entry (gptr, glist) and rhs row (rptr, rlist) the element
-    // values it receives, as indices q*LOC^2 + entry / q*LOC + row into
-    // gval / grv, where q is the element's place in colour order, ascending;
-    // gbad lists the element entries the pattern lacks
+    // values it receives, as indices e*LOC^2 + entry / e*LOC + row into
+    // gval / grv (e the element id), in colour order, ascending element id
+    // inside a colour; gbad lists the element entries the pattern lacks
     bool gather_built = false;
     DevBuf<int32_t> gptr, glist, rptr, rlist, gbad;
     int32_t ngbad = 0;
