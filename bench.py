@@ -345,6 +345,56 @@ def p1_config_report(name, log_each) -> dict:
     return out
 
 
+def pressure_report(log_each) -> dict:
+    """The pressure path on the 3-D P1 configuration's mesh (64^3 x 6 tets,
+    P1 velocity and pressure): ct_m + lumped mass and C^T M_l^-1 C, device
+    time per call (host copies excluded, the call's own synchronisation
+    included), ct_m's compulsory traffic against HBM, and both results
+    against the oracle bit for bit."""
+    from fluidity_amd import cg, meshgen as M
+    O = oracle_mod()
+    cfg = M.config("c2", seed=1)
+    mesh = cfg.mesh
+    pr = cg.CGProblem(mesh, quad_degree=2)
+    pm = cg.vertex_pressure_mesh(pr)
+    cg.bind_pressure_mesh(pr, pm)
+    lib = pr.lib
+    import ctypes as C
+    from fluidity_amd import lib as L
+    one = np.ones(1)
+    L.check(lib.flgpu_ct_assemble(pr.ctx, 0, 1, L.ptr(one), None, None), "ct_assemble")  # builds the lists
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.check(lib.flgpu_ct_assemble(pr.ctx, 0, 1, L.ptr(one), None, None), "ct_assemble")
+    ct_ms = (time.perf_counter() - t0) / reps * 1e3
+    cf, cc = cg.cmc_sparsity(pm, mesh.nnodes)
+    cg.assemble_masslumped_cmc(pr, cf, cc, None)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.check(lib.flgpu_cmc_assemble(pr.ctx, None, cf.size - 1, L.ptr(cf), L.ptr(cc), None), "cmc_assemble")
+    cmc_ms = (time.perf_counter() - t0) / reps * 1e3
+    nnz_ct = pm.ct_colm.size
+    # compulsory bytes: both connectivities, coordinates, ct_m and the lumped mass
+    b_ct = 4 * (mesh.loc + pm.loc) * mesh.nele + 8 * mesh.dim * mesh.nx + 8 * mesh.dim * nnz_ct + 8 * mesh.nnodes
+    ct_g, ml_g = cg.construct_momentum_ct(pr, pm)
+    cmc_g = cg.assemble_masslumped_cmc(pr, cf, cc, None)
+    asm, pd, cts, pn, p_nodes = O.pressure_setup(mesh, 2, colouring=(pr.colouring[1] - 1, pr.colouring[2]))
+    ct_o, ml_o = O.ct_assemble(asm, pd)
+    cmc_o = O.cmc(cts, ct_o, np.tile(1.0 / ml_o, (mesh.dim, 1)), (cf, cc))
+    pr.close()
+    out = {"elements": mesh.nele, "pressure_nodes": pm.nnodes, "ct_nnz": int(nnz_ct), "cmc_nnz": int(cc.size),
+           "ct_and_masslump_ms": round(ct_ms, 4), "ct_GB/s": round(b_ct / (ct_ms * 1e-3) / 1e9, 1),
+           "ct_frac": round(b_ct / (ct_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "ct_algorithmic_bytes": int(b_ct),
+           "cmc_ms": round(cmc_ms, 4),
+           "ct_bitwise_equal_oracle": bool(np.array_equal(ct_g, ct_o) and np.array_equal(ml_g, ml_o)),
+           "cmc_bitwise_equal_oracle": bool(np.array_equal(cmc_g, cmc_o))}
+    log_each(f"pressure path (c2 mesh): ct_m + lumped mass {ct_ms:.3f} ms, cmc {cmc_ms:.3f} ms "
+             f"({cc.size} entries), oracle bitwise: ct {out['ct_bitwise_equal_oracle']}, "
+             f"cmc {out['cmc_bitwise_equal_oracle']}")
+    return out
+
+
 def cpu_baseline(cfg, ref, log_each) -> dict:
     """The oracle (C restatement, OpenMP over the colour loop like the
     reference) on this host's cores: the assembly of the whole mesh, one
@@ -558,6 +608,8 @@ def main() -> int:
     if not args.no_p1:
         out["c1"] = p1_config_report("c1", log)
         out["c2"] = p1_config_report("c2", log)
+        if not args.no_cpu:
+            out["pressure_path"] = pressure_report(log)
     if not args.no_c3:
         out["c3_mixed_precision"] = c3_mixed_report(log)
         if not args.no_solve:

@@ -32,9 +32,10 @@ COMMON = [
 EXTRA = {
     "assemble.hip": ["-ffp-contract=off"],
     "surface.hip": ["-ffp-contract=off"],
+    "pressure.hip": ["-ffp-contract=off"],
 }
 SOURCES = ["flgpu_context.hip", "assemble.hip", "surface.hip", "krylov.hip", "learned.hip", "matrixdump.hip",
-           "setup_gpu.hip"]
+           "setup_gpu.hip", "pressure.hip"]
 
 
 def _compile(src: str) -> str:

@@ -409,6 +409,88 @@ def solve_field_equation_cg(problem: CGProblem, opts: CGOptions, solver: SolverO
     return T_new, info.its, info
 
 
+# ---------------------------------------------------------------------------
+# The pressure path of the momentum solve (assemble/Momentum_CG.F90,
+# assemble/Assemble_CMC.F90) on the velocity mesh bound to a CGProblem
+# ---------------------------------------------------------------------------
+
+@dataclass
+class PressureMesh:
+    """The pressure mesh of a momentum solve: connectivity (1-based,
+    p_loc per element), node count, element tables on the velocity's
+    quadrature, and ct_m's sparsity (rows: pressure nodes, columns: velocity
+    nodes, 1-based)."""
+    ndglno: np.ndarray
+    loc: int
+    nnodes: int
+    element: Element
+    ct_findrm: np.ndarray
+    ct_colm: np.ndarray
+
+
+def vertex_pressure_mesh(problem: CGProblem) -> PressureMesh:
+    """The P1 pressure mesh on the vertices of the problem's mesh (P1-P1,
+    or Taylor-Hood P2-P1 for a quadratic velocity), with ct_m's sparsity:
+    row i holds every velocity node of every element containing pressure
+    node i (make_sparsity(p_mesh, u_mesh), femtools/Sparsity_Patterns.F90:
+    47-85)."""
+    mesh = problem.mesh
+    pl = mesh.dim + 1
+    pe = np.asarray(mesh.ndglno_x, np.int64).reshape(-1, pl) - 1
+    ue = np.asarray(mesh.ndglno, np.int64).reshape(-1, mesh.loc) - 1
+    rows = np.repeat(pe, mesh.loc, axis=1).ravel()
+    cols = np.tile(ue, (1, pl)).ravel()
+    key = np.unique(rows * mesh.nnodes + cols)
+    r, c = key // mesh.nnodes, key % mesh.nnodes
+    findrm = np.concatenate([[0], np.cumsum(np.bincount(r, minlength=mesh.nx))]).astype(np.int32) + 1
+    return PressureMesh(np.asarray(mesh.ndglno_x, np.int32), pl, mesh.nx,
+                        make_element(mesh.dim, 1, problem.element.quad_degree), findrm, (c + 1).astype(np.int32))
+
+
+def bind_pressure_mesh(problem: CGProblem, pm: PressureMesh) -> None:
+    el = pm.element
+    L.check(problem.lib.flgpu_pressure_bind(problem.ctx, pm.loc, pm.nnodes, L.ptr(pm.ndglno),
+                                            L.ptr(el.fortran(el.n)), L.ptr(el.fortran(el.dn)), pm.nnodes,
+                                            L.ptr(pm.ct_findrm), L.ptr(pm.ct_colm)), "flgpu_pressure_bind")
+
+
+def construct_momentum_ct(problem: CGProblem, pm: PressureMesh, integrate_continuity_by_parts: bool = False,
+                          density=1.0):
+    """ct_m and the lumped velocity mass of construct_momentum_cg
+    (Momentum_CG.F90:1377-1403, :1492-1600) for a bound pressure mesh:
+    returns (ct_val (dim, nnz), masslump (velocity nodes))."""
+    d = problem.mesh.dim
+    rho = np.atleast_1d(np.ascontiguousarray(density, dtype=np.float64))
+    ct = np.zeros((d, pm.ct_colm.size))
+    ml = np.zeros(problem.mesh.nnodes)
+    L.check(problem.lib.flgpu_ct_assemble(problem.ctx, int(integrate_continuity_by_parts), int(rho.size == 1),
+                                          L.ptr(rho), L.ptr(ct), L.ptr(ml)), "flgpu_ct_assemble")
+    return ct, ml
+
+
+def cmc_sparsity(pm: PressureMesh, u_nodes: int):
+    """The pattern of ct_m ct_m^T (get_csr_sparsity_secondorder): row i holds
+    every pressure node whose ct_m row shares a velocity node with row i."""
+    import scipy.sparse as sp
+    c = sp.csr_matrix((np.ones(pm.ct_colm.size), pm.ct_colm - 1, pm.ct_findrm - 1), shape=(pm.nnodes, u_nodes))
+    m = (c @ c.T).tocsr()
+    m.sort_indices()
+    return (m.indptr + 1).astype(np.int32), (m.indices + 1).astype(np.int32)
+
+
+def assemble_masslumped_cmc(problem: CGProblem, cmc_findrm: np.ndarray, cmc_colm: np.ndarray,
+                            inverse_masslump: np.ndarray | None = None) -> np.ndarray:
+    """C_P^T M_l^-1 C (Assemble_CMC.F90:119-135) from the last
+    construct_momentum_ct, on the given pattern; inverse_masslump (dim,
+    velocity nodes) or None for 1/masslump in every component."""
+    im = None if inverse_masslump is None else np.ascontiguousarray(
+        np.asarray(inverse_masslump, np.float64).reshape(problem.mesh.dim, -1).T)
+    out = np.zeros(cmc_colm.size)
+    L.check(problem.lib.flgpu_cmc_assemble(problem.ctx, None if im is None else L.ptr(im), cmc_findrm.size - 1,
+                                           L.ptr(cmc_findrm), L.ptr(cmc_colm), L.ptr(out)), "flgpu_cmc_assemble")
+    return out
+
+
 def problem_from_config(cfg, device: int = 0, **kw) -> CGProblem:
     """Bind a meshgen.Config and upload its fields."""
     pr = CGProblem(cfg.mesh, quad_degree=cfg.quad_degree, device=device, **kw)

@@ -120,6 +120,7 @@ int flgpu_finalize(flgpu_ctx* c) {
         (void)hipStreamSynchronize(c->stream);
         krylov_free(c);
         learned_free(c);
+        pressure_free(c);
         (void)hipStreamDestroy(c->stream);
         delete c;
         return FLGPU_OK;
@@ -159,6 +160,7 @@ int flgpu_mesh_bind(flgpu_ctx* c, int64_t key_id, int64_t key_event, int32_t dim
         c->key_id = -1;  // the device mirror is being replaced
         c->plan.built = false;
         c->plan.epos_built = false;
+        c->plan_gen++;
         c->plan.fail_rc = 0;
         if (!same_topology) unbind_from_mesh(c);
         FLGPU_TRY(c->conn_x.upload(cx.data(), cx.size(), c->stream));
@@ -173,6 +175,7 @@ int flgpu_mesh_bind(flgpu_ctx* c, int64_t key_id, int64_t key_event, int32_t dim
         c->nx = nx;
         c->plan.built = false;
         c->plan.epos_built = false;
+        c->plan_gen++;
         c->plan.fail_rc = 0;
         return FLGPU_OK;
     });
@@ -256,6 +259,7 @@ int flgpu_field_mesh_bind(flgpu_ctx* c, int32_t loc, int32_t ngi, int32_t nnodes
         c->linear = (loc == lin);
         c->plan.built = false;
         c->plan.epos_built = false;
+        c->plan_gen++;
         c->plan.fail_rc = 0;
         c->surf.bound = false;  // the surface names nodes of the previous field mesh
         return FLGPU_OK;
@@ -303,6 +307,7 @@ int flgpu_sparsity_bind(flgpu_ctx* c, int32_t rows, int32_t cols, const int32_t*
         c->assembled = false;
         c->plan.built = false;
         c->plan.epos_built = false;
+        c->plan_gen++;
         c->plan.fail_rc = 0;
         c->sparsity_gen++;
         krylov_invalidate(c);
@@ -359,6 +364,7 @@ int colouring_adopt(flgpu_ctx* c, std::vector<int32_t>& cp, std::vector<int32_t>
     c->h_col_elems.swap(el);
     c->plan.built = false;
     c->plan.epos_built = false;
+    c->plan_gen++;
     c->plan.fail_rc = 0;
     return FLGPU_OK;
 }

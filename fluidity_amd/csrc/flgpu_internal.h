@@ -198,6 +198,7 @@ struct SurfaceState {
 
 struct KrylovWork;   // krylov.hip
 struct LearnedState; // learned.hip
+struct PressureState; // pressure.hip
 
 }  // namespace flgpu
 
@@ -247,6 +248,8 @@ struct flgpu_ctx {
     flgpu::SurfaceState surf;
     flgpu::KrylovWork* kw = nullptr;
     flgpu::LearnedState* ls = nullptr;
+    flgpu::PressureState* ps = nullptr;
+    int64_t plan_gen = 0;            // bumped whenever the meshes, sparsity or colouring change
 };
 
 namespace flgpu {
@@ -271,6 +274,8 @@ int solution_get(flgpu_ctx* c, double* x);
 int pc_apply_host(flgpu_ctx* c, const flgpu_solver_params* sp, const double* r, double* z);
 // learned.hip
 void learned_free(flgpu_ctx* c);
+// pressure.hip
+void pressure_free(flgpu_ctx* c);
 // shared helpers
 unsigned grid_for(int64_t n, int block);
 // setup_gpu.hip: out[0..n) = exclusive prefix sums of in[0..n), out[n] = the total

@@ -232,6 +232,46 @@ module flgpu
        integer(c_int32_t), intent(out) :: its, reason
        real(c_double), intent(out) :: true_relres
      end function flgpu_petsc_solve
+
+     !! the pressure path (Momentum_CG.F90 / Assemble_CMC.F90): the pressure
+     !! mesh with ct_m's sparsity, then ct_m and the lumped velocity mass,
+     !! then assemble_masslumped_cmc's C_P^T M_l^-1 C
+     integer(c_int) function flgpu_pressure_bind(ctx, p_loc, p_nodes, p_ndglno, p_n, p_dn, ct_rows, &
+          ct_findrm, ct_colm) bind(c, name="flgpu_pressure_bind")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: p_loc, p_nodes, ct_rows
+       integer(c_int32_t), intent(in) :: p_ndglno(*), ct_findrm(*), ct_colm(*)
+       real(c_double), intent(in) :: p_n(*), p_dn(*)
+     end function flgpu_pressure_bind
+
+     integer(c_int) function flgpu_ct_assemble(ctx, integrate_by_parts, density_constant, density, ct_val, &
+          masslump) bind(c, name="flgpu_ct_assemble")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: integrate_by_parts, density_constant
+       real(c_double), intent(in) :: density(*)
+       type(c_ptr), value :: ct_val, masslump
+     end function flgpu_ct_assemble
+
+     integer(c_int) function flgpu_cmc_assemble(ctx, inverse_masslump, cmc_rows, cmc_findrm, cmc_colm, cmc_val) &
+          bind(c, name="flgpu_cmc_assemble")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx, inverse_masslump
+       integer(c_int32_t), value :: cmc_rows
+       integer(c_int32_t), intent(in) :: cmc_findrm(*), cmc_colm(*)
+       type(c_ptr), value :: cmc_val
+     end function flgpu_cmc_assemble
+
+     !! flgpu_solve for callers that cannot allocate device memory (its work
+     !! in a fixed 64 MiB device array per GPU)
+     integer(c_int) function flgpu_solve_noalloc(ctx, sp, x, info) bind(c, name="flgpu_solve_noalloc")
+       import :: c_int, c_ptr, c_double, flgpu_solver_params
+       type(c_ptr), value :: ctx
+       type(flgpu_solver_params), intent(in) :: sp
+       real(c_double), intent(inout) :: x(*)
+       type(c_ptr), value :: info
+     end function flgpu_solve_noalloc
   end interface
 
 contains

@@ -384,6 +384,46 @@ int flgpu_matrixdump_write(const char *filename, int32_t n,
                            const int32_t *findrm, const int32_t *colm,
                            const double *val, const double *b,
                            const double *x0);
+/* ---- the pressure path ---------------------------------------------------
+ * construct_momentum_cg's divergence block ct_m and lumped velocity mass
+ * (assemble/Momentum_CG.F90:1377-1403, :1468-1469, :1492-1600) and
+ * assemble_masslumped_cmc (assemble/Assemble_CMC.F90:119-135).  The
+ * velocity mesh is the bound field mesh (flgpu_field_mesh_bind: u's
+ * connectivity and element tables) with its colouring. */
+
+/* The pressure mesh: ndglno of p (p_loc per element, 1-based),
+ * ele_shape(p)%n(p_loc,ngi) and %dn(p_loc,ngi,dim) on the velocity's
+ * quadrature, and ct_m's sparsity (get_csr_sparsity_firstorder(state,
+ * p%mesh, u%mesh): ct_rows = pressure nodes, columns = velocity nodes,
+ * 1-based, sorted rows). */
+int flgpu_pressure_bind(flgpu_ctx *ctx, int32_t p_loc, int32_t p_nodes,
+                        const int32_t *p_ndglno, const double *p_n,
+                        const double *p_dn, int32_t ct_rows,
+                        const int32_t *ct_findrm, const int32_t *ct_colm);
+
+/* ct_m = block (1,d) of grad_p_u_mat, shape_dshape(p_shape, du_t, detwei)
+ * or, with integrate_by_parts (integrate_continuity_by_parts),
+ * -dshape_shape(dp_t, u_shape, detwei), added colour by colour as
+ * addto(ct_m, p_ele, u_ele, ...) adds it; and the lumped mass
+ * sum(shape_shape(u_shape, u_shape, density*detwei), 2) added per velocity
+ * node (the non-inverted inverse_masslump, one component).  density: one
+ * value (density_constant) or one per velocity node.  ct_val (dim blocks of
+ * nnz(ct), block d at d*nnz) and masslump (velocity nodes) are copied out
+ * when not NULL; both stay on the device for flgpu_cmc_assemble. */
+int flgpu_ct_assemble(flgpu_ctx *ctx, int32_t integrate_by_parts,
+                      int32_t density_constant, const double *density,
+                      double *ct_val, double *masslump);
+
+/* cmc = ct_m diag(inverse_masslump) ct_m^T on the given pattern
+ * (mult_div_vector_div_T, femtools/Sparse_Matrices_Fields.F90:590-671):
+ * inverse_masslump is dim x velocity nodes (vector_field%val, Dirichlet
+ * rows already zeroed by the caller), or NULL for 1/masslump in every
+ * component; cmc_findrm(cmc_rows+1) / cmc_colm 1-based, sorted; cmc_val
+ * (nnz) copied out when not NULL. */
+int flgpu_cmc_assemble(flgpu_ctx *ctx, const double *inverse_masslump,
+                       int32_t cmc_rows, const int32_t *cmc_findrm,
+                       const int32_t *cmc_colm, double *cmc_val);
+
 /* ---- learned preconditioner --------------------------------------------
  * A message-passing network on the matrix graph (3 layers, hidden 32,
  * BF16 MFMA with FP32 accumulation) predicts a lower-triangular factor G on

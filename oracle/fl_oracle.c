@@ -482,25 +482,16 @@ double orc_nu_bar_scaled(int dim, const double *u, const double *Jflat, const do
 #define DN_(i, g, d) (m->dn[(i) + loc * ((g) + ngi * (d))])
 #define XDN_(i, g, d) (m->x_dn[(i) + lx * ((g) + ngi * (d))])
 
-static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
-                      double *mat, double *rhs)
+/* transform_to_physical's cached geometry of a linear element
+ * (Transform_elements.F90:227-306 construct_cache, :766-802 retrieval):
+ * J^T from the coordinate element's dn, its inverse by cofactors over the
+ * determinant. */
+static void elem_geometry(const orc_mesh *m, int ele, double JT[MAXDIM][MAXDIM], double iJ[MAXDIM][MAXDIM],
+                          double *detJ_out)
 {
-    const int dim = m->dim, loc = m->loc, ngi = m->ngi, lx = m->loc_x;
-    const int32_t *en = m->ndglno + (int64_t)ele * loc;
+    const int dim = m->dim, ngi = m->ngi, lx = m->loc_x;
     const int32_t *exn = m->ndglno_x + (int64_t)ele * lx;
-    const double eps = DBL_EPSILON; /* epsilon(0.0) with real*8 */
-    const double dt_theta = p->dt * p->theta;
-    double Xv[MAXDIM][MAXLOC], JT[MAXDIM][MAXDIM], iJ[MAXDIM][MAXDIM];
-    double detJ;
-    double dsh[MAXLOC][MAXGI][MAXDIM];
-    double detwei[MAXGI];
-    double Te[MAXLOC];
-
-    for (int i = 0; i < loc * loc; i++) mat[i] = 0.0;
-    for (int i = 0; i < loc; i++) rhs[i] = 0.0;
-
-    /* --- transform_to_physical, cache path (Transform_elements.F90:
-     * 227-306 construct_cache, :766-802 retrieval) --- */
+    double Xv[MAXDIM][MAXLOC];
     for (int n = 0; n < lx; n++)
         for (int a = 0; a < dim; a++)
             Xv[a][n] = m->X[a + dim * (int64_t)(exn[n] - 1)];
@@ -523,10 +514,34 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
     } else {
         iJ[0][0] = 1.0;
     }
-    detJ = 0.0;
+    double detJ = 0.0;
     for (int a = 0; a < dim; a++) detJ = detJ + JT[a][0] * iJ[a][0];
     for (int i = 0; i < dim; i++)
         for (int k = 0; k < dim; k++) iJ[i][k] = iJ[i][k] / detJ;
+    *detJ_out = detJ;
+}
+
+static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
+                      double *mat, double *rhs)
+{
+    const int dim = m->dim, loc = m->loc, ngi = m->ngi, lx = m->loc_x;
+    const int32_t *en = m->ndglno + (int64_t)ele * loc;
+    const int32_t *exn = m->ndglno_x + (int64_t)ele * lx;
+    const double eps = DBL_EPSILON; /* epsilon(0.0) with real*8 */
+    const double dt_theta = p->dt * p->theta;
+    double Xv[MAXDIM][MAXLOC], JT[MAXDIM][MAXDIM], iJ[MAXDIM][MAXDIM];
+    double detJ;
+    double dsh[MAXLOC][MAXGI][MAXDIM];
+    double detwei[MAXGI];
+    double Te[MAXLOC];
+
+    for (int i = 0; i < loc * loc; i++) mat[i] = 0.0;
+    for (int i = 0; i < loc; i++) rhs[i] = 0.0;
+
+    /* --- transform_to_physical, cache path --- */
+    (void)Xv;
+    (void)exn;
+    elem_geometry(m, ele, JT, iJ, &detJ);
 
     /* dshape = dn . invJ^T; once for linear T (m_nonlinear false),
      * per quadrature point otherwise (Transform_elements.F90:769-781) */
@@ -874,6 +889,151 @@ int orc_assemble(const orc_params *p, const orc_mesh *m, double *A,
     }
     if (seconds) *seconds = wall_now() - t0;
     return err ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* The pressure path (Momentum_CG.F90, Assemble_CMC.F90)               */
+/* ------------------------------------------------------------------ */
+
+#define PN_(i, g) (pp->p_n[(i) + pl * (g)])
+#define PDN_(i, g, d) (pp->p_dn[(i) + pl * ((g) + ngi * (d))])
+
+/* construct_momentum_element_cg's divergence block (:1377-1403) and
+ * add_mass_element_cg's lumped mass (:1492-1600) for one element. */
+int orc_ct_element(const orc_mesh *m, const orc_pressure *pp, int ele, double *mat, double *mass_lump)
+{
+    const int dim = m->dim, loc = m->loc, ngi = m->ngi, pl = pp->p_loc;
+    const int32_t *en = m->ndglno + (int64_t)ele * loc;
+    double JT[MAXDIM][MAXDIM], iJ[MAXDIM][MAXDIM], detJ;
+    double du[MAXLOC][MAXGI][MAXDIM], dp[MAXLOC][MAXGI][MAXDIM], detwei[MAXGI];
+    elem_geometry(m, ele, JT, iJ, &detJ);
+    /* transform_to_physical(X, ele, u_shape, dshape=du_t, detwei=detwei)
+     * and, by parts, (X, ele, p_shape, dshape=dp_t): dshape = dn . invJ^T,
+     * once for a linear element, per point otherwise */
+    const int u_nonlinear = (loc != dim + 1), p_nonlinear = (pl != dim + 1);
+    for (int g = 0; g < ngi; g++) {
+        const int gu = u_nonlinear ? g : 0, gp = p_nonlinear ? g : 0;
+        for (int d = 0; d < dim; d++) {
+            for (int n = 0; n < loc; n++) {
+                double s = 0.0;
+                for (int k = 0; k < dim; k++) s = s + DN_(n, gu, k) * iJ[d][k];
+                du[n][g][d] = s;
+            }
+            for (int n = 0; n < pl; n++) {
+                double s = 0.0;
+                for (int k = 0; k < dim; k++) s = s + PDN_(n, gp, k) * iJ[d][k];
+                dp[n][g][d] = s;
+            }
+        }
+        detwei[g] = fabs(detJ) * m->w[g];
+    }
+    /* grad_p_u_mat(dim, p_loc, u_loc): shape_dshape(p_shape, du_t, detwei)
+     * (FETools.F90:332-362: sum_g detwei_g (n_p(i,g) du_t(j,g,d))), or by
+     * parts -dshape_shape(dp_t, u_shape, detwei) (:364-389) */
+    for (int i = 0; i < pl; i++)
+        for (int j = 0; j < loc; j++)
+            for (int d = 0; d < dim; d++) {
+                double s = 0.0;
+                if (pp->integrate_by_parts) {
+                    for (int g = 0; g < ngi; g++) s = s + detwei[g] * (dp[i][g][d] * N_(j, g));
+                    s = -s;
+                } else {
+                    for (int g = 0; g < ngi; g++) s = s + detwei[g] * (PN_(i, g) * du[j][g][d]);
+                }
+                mat[d + dim * (i + pl * j)] = s;
+            }
+    /* mass_mat = shape_shape(u_shape, u_shape, density_gi*detwei)
+     * (FETools.F90:206-226: dot_product(n(i,:)*n(j,:), coefficient));
+     * mass_lump = sum(mass_mat, 2) */
+    double cdw[MAXGI];
+    for (int g = 0; g < ngi; g++) {
+        double rho = 0.0;
+        for (int i = 0; i < loc; i++) {
+            const double v = pp->density_const ? pp->density[0] : pp->density[en[i] - 1];
+            rho = rho + v * N_(i, g);
+        }
+        cdw[g] = rho * detwei[g];
+    }
+    for (int i = 0; i < loc; i++) {
+        double lump = 0.0;
+        for (int j = 0; j < loc; j++) {
+            double mij = 0.0;
+            for (int g = 0; g < ngi; g++) mij = mij + (N_(i, g) * N_(j, g)) * cdw[g];
+            lump = lump + mij;
+        }
+        mass_lump[i] = lump;
+    }
+    return 0;
+}
+
+int orc_ct_assemble(const orc_mesh *m, const orc_pressure *pp, double *ct_val, double *masslump)
+{
+    const int dim = m->dim, loc = m->loc, pl = pp->p_loc;
+    const int64_t nnz = pp->ct_findrm[pp->p_nodes] - 1;
+    int err = 0;
+    for (int64_t k = 0; k < (int64_t)dim * nnz; k++) ct_val[k] = 0.0;
+    for (int i = 0; i < m->nnodes; i++) masslump[i] = 0.0;
+    double mat[MAXDIM * MAXLOC * MAXLOC], ml[MAXLOC];
+    int64_t pos[MAXLOC][MAXLOC];
+    /* colour loop (Momentum_CG.F90:722-741): colours in order, elements
+     * ascending (sequential here: elements of one colour share no node) */
+    for (int c = 0; c < m->ncolours; c++)
+        for (int k = m->col_ptr[c]; k < m->col_ptr[c + 1]; k++) {
+            const int ele = m->col_elems[k] - 1;
+            const int32_t *en = m->ndglno + (int64_t)ele * loc;
+            const int32_t *pn = pp->p_ndglno + (int64_t)ele * pl;
+            orc_ct_element(m, pp, ele, mat, ml);
+            /* addto(ct_m, p_ele, u_ele, spread(grad_p_u_mat, 1, 1)) ->
+             * block_csr_blocks_addto (Sparse_Tools.F90:2764-2812): a
+             * position is looked up unless all blocks' values are zero, and
+             * a zero value is skipped */
+            for (int i = 0; i < pl; i++)
+                for (int j = 0; j < loc; j++) {
+                    int any = 0;
+                    for (int d = 0; d < dim; d++) any |= mat[d + dim * (i + pl * j)] != 0.0;
+                    pos[i][j] = any ? csr_pos(pp->ct_findrm, pp->ct_colm, pn[i], en[j]) : -2;
+                    if (any && pos[i][j] < 0) err = 1;
+                }
+            for (int d = 0; d < dim; d++)
+                for (int i = 0; i < pl; i++)
+                    for (int j = 0; j < loc; j++) {
+                        const double v = mat[d + dim * (i + pl * j)];
+                        if (v == 0.0 || pos[i][j] < 0) continue;
+                        ct_val[d * nnz + pos[i][j]] = ct_val[d * nnz + pos[i][j]] + v;
+                    }
+            /* addto(masslump, dim, u_ele, mass_lump) (:1560-1564) */
+            for (int i = 0; i < loc; i++) masslump[en[i] - 1] = masslump[en[i] - 1] + ml[i];
+        }
+    return err ? -1 : 0;
+}
+
+void orc_cmc(int rows, const int32_t *ctp_findrm, const int32_t *ctp_colm, const double *ctp_val, int64_t ctp_nnz,
+             const int32_t *ct_findrm, const int32_t *ct_colm, const double *ct_val, int64_t ct_nnz, int dim,
+             const double *inv_ml, const int32_t *cmc_findrm, const int32_t *cmc_colm, double *cmc_val)
+{
+    /* mult_div_vector_div_T (Sparse_Matrices_Fields.F90:590-671):
+     * M_ij = sum_k A_ik D_k B_jk, rows of A and B walked together */
+    for (int i = 0; i < rows; i++)
+        for (int e = cmc_findrm[i] - 1; e < cmc_findrm[i + 1] - 1; e++) {
+            const int j = cmc_colm[e] - 1;
+            int64_t k1 = ctp_findrm[i] - 1, k2 = ct_findrm[j] - 1;
+            const int64_t e1 = ctp_findrm[i + 1] - 1, e2 = ct_findrm[j + 1] - 1;
+            double entry0 = 0.0;
+            while (k1 < e1 && k2 < e2) {
+                if (ctp_colm[k1] < ct_colm[k2]) {
+                    k1++;
+                } else if (ctp_colm[k1] == ct_colm[k2]) {
+                    const int64_t node = ctp_colm[k1] - 1;
+                    for (int d = 0; d < dim; d++)
+                        entry0 = entry0 + ctp_val[d * ctp_nnz + k1] * ct_val[d * ct_nnz + k2] * inv_ml[d + dim * node];
+                    k1++;
+                    k2++;
+                } else {
+                    k2++;
+                }
+            }
+            cmc_val[e] = entry0;
+        }
 }
 
 /* ------------------------------------------------------------------ */

@@ -167,6 +167,44 @@ int orc_colour_rows(int n, const int32_t *findrm, const int32_t *colm, int32_t *
 void orc_ssor_apply(int n, const int32_t *findrm, const int32_t *colm, const double *val,
                     const double *r, double *z);
 
+/* ---- the pressure path ------------------------------------------------
+ * The divergence block of construct_momentum_cg (assemble/Momentum_CG.F90:
+ * 1377-1403, addto :1468-1469) and the lumped velocity mass it assembles
+ * alongside (add_mass_element_cg, :1492-1600, the sum of shape_shape(u, u,
+ * density*detwei) over its columns), then assemble_masslumped_cmc
+ * (assemble/Assemble_CMC.F90:119-135 -> mult_div_vector_div_T,
+ * femtools/Sparse_Matrices_Fields.F90:590-671).  The velocity mesh is the
+ * field mesh of orc_mesh (its colouring, quadrature and shape tables); the
+ * pressure mesh has its own connectivity and tables on the same quadrature. */
+typedef struct {
+    int32_t p_loc, p_nodes;
+    const int32_t *p_ndglno;      /* p_loc*nele, 1-based */
+    const double *p_n, *p_dn;     /* p_n(p_loc,ngi), p_dn(p_loc,ngi,dim) */
+    /* ct_m's sparsity: rows = pressure nodes, columns = velocity nodes */
+    const int32_t *ct_findrm, *ct_colm;
+    const double *density;        /* on the velocity mesh (one value if const) */
+    int32_t density_const;
+    int32_t integrate_by_parts;   /* integrate_continuity_by_parts */
+} orc_pressure;
+
+/* ct_val: dim blocks of nnz(ct) values, block (1,d) at d*nnz; masslump:
+ * one value per velocity node (every component holds the same).  Colour
+ * loop, block addto skipping zeros.  Returns 0, or -1 for a nonzero value
+ * outside the pattern. */
+int orc_ct_assemble(const orc_mesh *m, const orc_pressure *pp, double *ct_val, double *masslump);
+
+/* One element's grad_p_u_mat(dim, p_loc, u_loc) (column-major) and
+ * mass_lump(u_loc). */
+int orc_ct_element(const orc_mesh *m, const orc_pressure *pp, int ele, double *mat, double *mass_lump);
+
+/* cmc = ctp diag(inv_ml) ct^T on the given pattern: entry (i, j) walks rows
+ * i of ctp and j of ct together (ascending columns), adding
+ * ctp_d(i,k) ct_d(j,k) inv_ml(d,k) for each common k, d inner.  inv_ml is
+ * dim x velocity nodes (vector_field%val).  All CSR arrays 1-based. */
+void orc_cmc(int rows, const int32_t *ctp_findrm, const int32_t *ctp_colm, const double *ctp_val, int64_t ctp_nnz,
+             const int32_t *ct_findrm, const int32_t *ct_colm, const double *ct_val, int64_t ct_nnz, int dim,
+             const double *inv_ml, const int32_t *cmc_findrm, const int32_t *cmc_colm, double *cmc_val);
+
 /* Ghost pivot of Petsc_Tools.F90:959-984 over all rows' |diagonal|. */
 double orc_ghost_pivot(int n, const int32_t *centrm, const double *A);
 
