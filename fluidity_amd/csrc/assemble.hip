@@ -379,20 +379,34 @@ __host__ __device__ constexpr int quad_stage_per_element(int kst) {
     return kElemMaxGi * (DIM + 3 + kst) + LOC * (DIM + kst + 3);
 }
 
-// Quadratic T, one thread per element-matrix entry (LOC^2 threads per
-// element): the element's nodal values and coordinates are staged in LDS
-// by its threads (i, 0), the fields' point values by threads 0..ngi-1;
-// thread (i, j) forms entry (i, j) with entry_general_q, and row i's
-// lumped mass and rhs are summed over the row's entries in column order by
-// thread (i, 0) -- the arithmetic and order of row_general.  The CSR
-// positions of the entries come from a table built once per mesh and
-// sparsity, the element tables from a flat image (one memory round trip).
-// workgroup of the per-entry kernel (512 and 768 measured slower on C3)
+// Quadratic T, one thread per JB consecutive entries of an element row
+// (LOC^2 / JB threads per element): the element's nodal values and
+// coordinates are staged in LDS by the threads of each row's first part,
+// the physical shape gradients at every point once per element, the
+// fields' point values by the first ngi threads; the thread of entries
+// (i, j0..j0+JB-1) forms them with entry_general_q, which reads row i's
+// point values once for all of them, and row i's lumped mass and rhs are
+// summed over the row's entries in column order, the row's two threads
+// handing the partial sum on -- the arithmetic and order of row_general.  The CSR positions of the
+// entries come from a table built once per mesh and sparsity, the element
+// tables from a flat image (one memory round trip).
 constexpr int kEntryThreads = 256;
+
+// entries per thread: half a P2 row (tets: 5, triangles: 3)
+template <int LOC>
+__host__ __device__ constexpr int entry_jb() {
+    return LOC == 10 ? 5 : LOC == 6 ? 3 : 1;
+}
 
 template <int DIM, int LOC>
 __host__ __device__ constexpr int entry_stage_per_element(int kst) {
-    return quad_stage_per_element<DIM, LOC>(kst) + DIM * (DIM + 1) + 2 * LOC * LOC + kElemMaxGi * LOC * DIM;
+    return quad_stage_per_element<DIM, LOC>(kst) + DIM * (DIM + 1) + kElemMaxGi * LOC * DIM;
+}
+
+// elements per workgroup
+template <int LOC>
+__host__ __device__ constexpr int entry_epb() {
+    return kEntryThreads / (LOC * (LOC / entry_jb<LOC>()));
 }
 
 // GATHER: the first pass of the two-pass scheme -- elems is null and the
@@ -412,24 +426,28 @@ k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict_
     fl.iso = (FEAT & F_ANISO) ? fl.iso : 1;
     constexpr int L2 = LOC * LOC;
     constexpr int NT = kEntryThreads;
-    constexpr int EPB = NT / L2;  // elements per workgroup
+    constexpr int JB = entry_jb<LOC>();
+    constexpr int RPL = LOC / JB;     // parts of a row
+    constexpr int NTE = LOC * RPL;    // threads per element
+    constexpr int EPB = NT / NTE;     // elements per workgroup
     constexpr int NG = kElemMaxGi;
+    static_assert(LOC % JB == 0, "whole row parts");
     __shared__ LTab<R> tb;
     extern __shared__ double dynq[];
     const int kst = fl.iso ? 1 : DIM * DIM;
-    const int t = threadIdx.x, el = t / L2, rr = t - el * L2, i = rr / LOC, j = rr - i * LOC;
+    const int t = threadIdx.x, el = t / NTE, te = t - el * NTE, i = te / RPL, jb = te - i * RPL, j0 = jb * JB;
     stage_table_image<R, NT>(ltab, tb);
     const int q = (int)blockIdx.x * EPB + el;
     const bool live = el < EPB && q < count;
     R* me = reinterpret_cast<R*>(dynq) + (size_t)(live ? el : 0) * entry_stage_per_element<DIM, LOC>(kst);
     R *s_uq = me, *s_dv = s_uq + NG * DIM, *s_kq = s_dv + NG, *s_sq = s_kq + NG * kst, *s_fq = s_sq + NG;
     R *n_u = s_fq + NG, *n_k = n_u + LOC * DIM, *n_s = n_k + LOC * kst, *n_b = n_s + LOC, *n_t = n_b + LOC;
-    R *s_x = n_t + LOC, *s_m = s_x + DIM * (DIM + 1), *s_s = s_m + L2;
-    R* s_gr = s_s + L2;  // physical shape gradients, [(g * LOC + i) * DIM + d]
+    R* s_x = n_t + LOC;
+    R* s_gr = s_x + DIM * (DIM + 1);  // physical shape gradients, [(g * LOC + i) * DIM + d]
     int ele = 0;
     if (live) {
         ele = elems ? elems[q] : q;
-        if (j == 0) {  // thread (i, 0) stages node i (and vertex i's coordinates)
+        if (jb == 0) {  // the row's first thread stages node i (and vertex i's coordinates)
             const int nd = conn[(int64_t)ele * LOC + i];
             n_t[i] = (R)fd.T[nd];
             if (fl.adv)
@@ -459,13 +477,12 @@ k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict_
         geometry<DIM, R>(X, tb, iJ, adet);
         // every (node, point) gradient once per element instead of twice per
         // entry and point (the same operations, so the same bits)
-        for (int t = rr; t < LOC * tb.ngi; t += L2) {
-            const int g = t / LOC, nn = t - g * LOC;
+        for (int u = te; u < LOC * tb.ngi; u += NTE) {
+            const int g = u / LOC, nn = u - g * LOC;
 #pragma unroll
-            for (int d = 0; d < DIM; d++) s_gr[t * DIM + d] = phys_grad<DIM, LOC, R>(tb, iJ, nn, g, d);
+            for (int d = 0; d < DIM; d++) s_gr[u * DIM + d] = phys_grad<DIM, LOC, R>(tb, iJ, nn, g, d);
         }
-        if (rr < tb.ngi) {  // thread rr forms the values at point rr
-            const int g = rr;
+        for (int g = te; g < tb.ngi; g += NTE) {  // the values at point g
             const LdsNodal<DIM, LOC, R> ln{n_u, n_k, n_s, n_b, kst};
             const NodalQuad<DIM, LOC, R, LdsNodal<DIM, LOC, R>> nq{ln, tb, iJ};
             if (fl.adv) {
@@ -492,53 +509,74 @@ k_asm_colour_entry(const uint32_t* __restrict__ ltab, const int32_t* __restrict_
         }
     }
     __syncthreads();  // point values
-    R m_ij = (R)0, s_ij = (R)0, src_i = (R)0;
-    if (live) {
-        entry_general_q<DIM, LOC, R, LdsQuad<DIM, R>, FEAT>(tb, fl, iJ, adet,
-                                                            LdsQuad<DIM, R>{s_uq, s_dv, s_kq, s_sq, s_fq, kst}, i,
-                                                            j, m_ij, s_ij, src_i, s_gr);
-        s_m[rr] = m_ij;
-        s_s[rr] = s_ij;
-    }
-    __syncthreads();  // the element's entries
-    if (!live) return;
-    R m = (fl.have_mass && !fl.lump) ? m_ij : (R)0;
-    if (fl.have_mass && fl.lump && j == i) {
-        R lump = (R)0;
-        for (int jj = 0; jj < LOC; jj++) lump += s_m[i * LOC + jj];
-        m = lump;
-    }
-    const R dtte = fl.dtt_on ? (R)fl.dt_theta : (R)0;
-    const double v = (double)FMA(dtte, s_ij, m);
-    if (GATHER) {
-        A[(int64_t)ele * L2 + rr] = v;
-        if (j == 0) {
-            R r = (R)0;
-            for (int jj = 0; jj < LOC; jj++) r = FMA(-s_s[i * LOC + jj], n_t[jj], r);
-            if (fl.src) r = r + src_i;
-            rhs[(int64_t)ele * LOC + i] = (double)r;
+    R m_ij[JB], s_ij[JB], src_i = (R)0;
+#pragma unroll
+    for (int b = 0; b < JB; b++) m_ij[b] = s_ij[b] = (R)0;
+    if (live)
+        entry_general_q<DIM, LOC, R, LdsQuad<DIM, R>, FEAT, JB>(
+            tb, fl, iJ, adet, LdsQuad<DIM, R>{s_uq, s_dv, s_kq, s_sq, s_fq, kst}, i, j0, m_ij, s_ij, src_i, s_gr);
+    // Row i's lumped mass and rhs: one chain over the row's entries in
+    // column order, the first part's partial handed to the second part
+    // (the adjacent lane: a row's two threads never straddle a wavefront).
+    static_assert(RPL == 2, "a row in two parts");
+    const int lane = threadIdx.x & 63;
+    R lump = (R)0, r = (R)0;
+    if (jb == 0) {
+#pragma unroll
+        for (int b = 0; b < JB; b++) {
+            lump += m_ij[b];
+            r = FMA(-s_ij[b], n_t[j0 + b], r);
         }
+    }
+    const R lump_in = __shfl(lump, (lane + 63) & 63), r_in = __shfl(r, (lane + 63) & 63);
+    if (jb == 1) {
+        lump = lump_in;
+        r = r_in;
+#pragma unroll
+        for (int b = 0; b < JB; b++) {
+            lump += m_ij[b];
+            r = FMA(-s_ij[b], n_t[j0 + b], r);
+        }
+        if (fl.src) r = r + src_i;
+    }
+    const R lump_back = __shfl(lump, (lane + 1) & 63);
+    if (jb == 0) lump = lump_back;  // the row's whole lumped mass in both parts
+    if (!live) return;
+    const R dtte = fl.dtt_on ? (R)fl.dt_theta : (R)0;
+    if (GATHER) {
+#pragma unroll
+        for (int b = 0; b < JB; b++) {
+            const int j = j0 + b;
+            R m = (fl.have_mass && !fl.lump) ? m_ij[b] : (R)0;
+            if (fl.have_mass && fl.lump && j == i) m = lump;
+            A[(int64_t)ele * L2 + i * LOC + j] = (double)FMA(dtte, s_ij[b], m);
+        }
+        if (jb == 1) rhs[(int64_t)ele * LOC + i] = (double)r;
         return;
     }
-    const int row = conn[(int64_t)ele * LOC + i];
-    if (v != 0.0) {  // csr_addto skips zeros (Sparse_Tools.F90:2640)
-        const int pos = epos[(int64_t)ele * L2 + rr];
-        if (pos < 0)
-            atomicOr(err, 1);
-        else
-            A[pos] = A[pos] + v;
+#pragma unroll
+    for (int b = 0; b < JB; b++) {
+        const int j = j0 + b;
+        R m = (fl.have_mass && !fl.lump) ? m_ij[b] : (R)0;
+        if (fl.have_mass && fl.lump && j == i) m = lump;
+        const double v = (double)FMA(dtte, s_ij[b], m);
+        if (v != 0.0) {  // csr_addto skips zeros (Sparse_Tools.F90:2640)
+            const int pos = epos[(int64_t)ele * L2 + i * LOC + j];
+            if (pos < 0)
+                atomicOr(err, 1);
+            else
+                A[pos] = A[pos] + v;
+        }
     }
-    if (j == 0) {
-        R r = (R)0;
-        for (int jj = 0; jj < LOC; jj++) r = FMA(-s_s[i * LOC + jj], n_t[jj], r);
-        if (fl.src) r = r + src_i;
+    if (jb == 1) {
+        const int row = conn[(int64_t)ele * LOC + i];
         rhs[row] = rhs[row] + (double)r;
     }
 }
 
 template <int DIM, int LOC, typename R, int FEAT>
 static int launch_colour_entry_feat(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
-    constexpr int EPB = kEntryThreads / (LOC * LOC);
+    constexpr int EPB = entry_epb<LOC>();
     FLGPU_TRY(ensure_entry_positions(c));
     const uint32_t* img = nullptr;
     FLGPU_TRY(table_image<R>(c, &img));
@@ -741,7 +779,7 @@ static int build_gather_plan(flgpu_ctx* c) {
 
 template <int DIM, int LOC, typename R>
 static int launch_gather(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
-    constexpr int EPB = kEntryThreads / (LOC * LOC);
+    constexpr int EPB = entry_epb<LOC>();
     FLGPU_TRY(build_gather_plan(c));
     BlockPlan& P = c->plan;
     const uint32_t* img = nullptr;

@@ -620,20 +620,24 @@ FLGPU_HD_INLINE R phys_grad(const LTab<R>& tb, const R (&iJ)[DIM][DIM], int i, i
     return s;
 }
 
-// Entry (i, j) of the quadratic-T element sums of row_general_q, alone:
-// the mass term, the summed advection/absorption/diffusion term, and row
-// i's source integral, each accumulated over the quadrature points in the
-// same order with the same operations as there (so the same bits).
+// Entries (i, j0) .. (i, j0+JB-1) of the quadratic-T element sums of
+// row_general_q: the mass term, the summed advection/absorption/diffusion
+// term, and row i's source integral, each entry accumulated over the
+// quadrature points in the same order with the same operations as there
+// (so the same bits); the row's point values are formed once for its JB
+// entries.  gt: the physical gradients [(g*LOC + i)*DIM + d] when staged.
 // FEAT: the terms compiled in (without F_ANISO the diffusivity is taken as
 // isotropic whatever fl says; the caller picks a set that holds the flags).
-template <int DIM, int LOC, typename R, class QV, int FEAT = F_ALL | F_ANISO>
+template <int DIM, int LOC, typename R, class QV, int FEAT = F_ALL | F_ANISO, int JB = 1>
 FLGPU_HD_INLINE void entry_general_q(const LTab<R>& tb, const AsmFlags& fl_in, const R (&iJ)[DIM][DIM], R adet,
-                                     const QV& qv, int i, int j, R& m_ij, R& s_ij, R& src_i,
+                                     const QV& qv, int i, int j0, R (&m_ij)[JB], R (&s_ij)[JB], R& src_i,
                                      const R* gt = nullptr) {
     AsmFlags fl = fl_in;
     fl.iso = (FEAT & F_ANISO) ? fl.iso : 1;
     const int ngi = tb.ngi;
-    R M = (R)0, Sm = (R)0, src = (R)0;
+    R M[JB], Sm[JB], src = (R)0;
+#pragma unroll
+    for (int b = 0; b < JB; b++) M[b] = Sm[b] = (R)0;
 #pragma unroll 1
     for (int g = 0; g < ngi; g++) {
         const R dw = adet * tb.w[g];
@@ -677,49 +681,57 @@ FLGPU_HD_INLINE void entry_general_q(const LTab<R>& tb, const AsmFlags& fl_in, c
                 vK[b] = s;
             }
         }
-        const R nj = tb.n[j + LOC * g];
-        const R nn = ni * nj;
-        if (fl.have_mass) M = FMA(nn, dw, M);
-        if (fl.adv || fl.diff) {
-            R Gj[DIM];
-            if (gt) {
+        const R dwk = dw * kq;
 #pragma unroll
-                for (int d = 0; d < DIM; d++) Gj[d] = gt[(g * LOC + j) * DIM + d];
-            } else {
+        for (int b = 0; b < JB; b++) {
+            const int j = j0 + b;
+            const R nj = tb.n[j + LOC * g];
+            const R nn = ni * nj;
+            if (fl.have_mass) M[b] = FMA(nn, dw, M[b]);
+            if (fl.adv || fl.diff) {
+                R Gj[DIM];
+                if (gt) {
 #pragma unroll
-                for (int d = 0; d < DIM; d++) Gj[d] = phys_grad<DIM, LOC, R>(tb, iJ, j, g, d);
-            }
-            if (fl.adv) {
-                R a;
-                if (!fl.ibp) {
+                    for (int d = 0; d < DIM; d++) Gj[d] = gt[(g * LOC + j) * DIM + d];
+                } else {
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) Gj[d] = phys_grad<DIM, LOC, R>(tb, iJ, j, g, d);
+                }
+                if (fl.adv) {
+                    R a;
+                    if (!fl.ibp) {
+                        R t = (R)0;
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) t = FMA(Gj[d], uq[d], t);
+                        a = ni * t;
+                        if (fl.need_div) a = FMA((R)fl.beta * nn, div, a);
+                    } else {
+                        a = -(ug_i * nj);
+                        if (fl.need_div) a = FMA(-((R)1 - (R)fl.beta) * nn, div, a);
+                    }
+                    Sm[b] = FMA(a, dw, Sm[b]);
+                }
+                if (fl.diff) {
                     R t = (R)0;
+                    if (fl.iso) {
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) t = FMA(Gj[d], uq[d], t);
-                    a = ni * t;
-                    if (fl.need_div) a = FMA((R)fl.beta * nn, div, a);
-                } else {
-                    a = -(ug_i * nj);
-                    if (fl.need_div) a = FMA(-((R)1 - (R)fl.beta) * nn, div, a);
-                }
-                Sm = FMA(a, dw, Sm);
-            }
-            if (fl.diff) {
-                R t = (R)0;
-                if (fl.iso) {
+                        for (int d = 0; d < DIM; d++) t = FMA(Gi[d], Gj[d], t);
+                        Sm[b] = FMA(t, dwk, Sm[b]);
+                    } else {
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) t = FMA(Gi[d], Gj[d], t);
-                    Sm = FMA(t, dw * kq, Sm);
-                } else {
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) t = FMA(vK[d], Gj[d], t);
-                    Sm = FMA(t, dw, Sm);
+                        for (int d = 0; d < DIM; d++) t = FMA(vK[d], Gj[d], t);
+                        Sm[b] = FMA(t, dw, Sm[b]);
+                    }
                 }
             }
+            if (fl.absorp) Sm[b] = FMA(nn, dw * sq, Sm[b]);
         }
-        if (fl.absorp) Sm = FMA(nn, dw * sq, Sm);
     }
-    m_ij = M;
-    s_ij = Sm;
+#pragma unroll
+    for (int b = 0; b < JB; b++) {
+        m_ij[b] = M[b];
+        s_ij[b] = Sm[b];
+    }
     src_i = src;
 }
 
