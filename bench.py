@@ -145,6 +145,10 @@ def c4_heldout_report(weights, log_each) -> dict:
                                               max_iterations=50000, check_every=32)),
         ("learned_mixed_fcg", cg.SolverOptions("fcg", "learned", relative_error=1e-8, norm="unpreconditioned",
                                                max_iterations=50000, precision="mixed", check_every=32)),
+        ("fsai_fp64_pcg", cg.SolverOptions("cg", "fsai", relative_error=1e-8, norm="unpreconditioned",
+                                           max_iterations=50000, check_every=32)),
+        ("fsai_mixed_pcg", cg.SolverOptions("cg", "fsai", relative_error=1e-8, norm="unpreconditioned",
+                                            max_iterations=50000, precision="mixed", check_every=32)),
     ):
         cg.petsc_solve(pr, so)
         t0 = time.perf_counter()
@@ -156,8 +160,11 @@ def c4_heldout_report(weights, log_each) -> dict:
     out["iteration_ratio_jacobi_over_learned"] = round(j["its"] / max(out["learned_fp64_fcg"]["its"], 1), 3)
     out["speedup_learned_mixed_vs_fp64_jacobi"] = round(j["seconds"] / lm["seconds"], 3)
     out["speedup_learned_mixed_vs_mixed_jacobi"] = round(out["mixed_jacobi_pcg"]["seconds"] / lm["seconds"], 3)
+    out["iteration_ratio_learned_over_fsai"] = round(out["learned_fp64_fcg"]["its"] /
+                                                     max(out["fsai_fp64_pcg"]["its"], 1), 3)
     log_each(f"c4 held-out: jacobi {j['its']} its {j['seconds']:.4f}s, learned {out['learned_fp64_fcg']['its']} its, "
-             f"learned+mixed {lm['seconds']:.4f}s ({out['speedup_learned_mixed_vs_fp64_jacobi']}x)")
+             f"learned+mixed {lm['seconds']:.4f}s ({out['speedup_learned_mixed_vs_fp64_jacobi']}x), "
+             f"fsai {out['fsai_fp64_pcg']['its']} its, fsai+mixed {out['fsai_mixed_pcg']['seconds']:.4f}s")
     return out
 
 
@@ -531,6 +538,10 @@ def main() -> int:
                                                   max_iterations=50000, check_every=32)),
             ("learned_mixed_fcg", cg.SolverOptions("fcg", "learned", relative_error=1e-8, norm="unpreconditioned",
                                                    max_iterations=50000, precision="mixed", check_every=32)),
+            ("fsai_fp64_pcg", cg.SolverOptions("cg", "fsai", relative_error=1e-8, norm="unpreconditioned",
+                                               max_iterations=50000, check_every=32)),
+            ("fsai_mixed_pcg", cg.SolverOptions("cg", "fsai", relative_error=1e-8, norm="unpreconditioned",
+                                                max_iterations=50000, precision="mixed", check_every=32)),
         ):
             cg.petsc_solve(pr, so)  # warm-up (allocations, first-touch)
             t0s = time.perf_counter()
@@ -559,8 +570,16 @@ def main() -> int:
             solves["fp64_jacobi_pcg"]["seconds"] / solves["learned_mixed_fcg"]["seconds"], 3)
         solves["speedup_learned_mixed_vs_mixed_jacobi"] = round(
             solves["mixed_jacobi_pcg"]["seconds"] / solves["learned_mixed_fcg"]["seconds"], 3)
+        solves["speedup_fsai_mixed_vs_fp64_jacobi"] = round(
+            solves["fp64_jacobi_pcg"]["seconds"] / solves["fsai_mixed_pcg"]["seconds"], 3)
+        # the exact FSAI on the same pattern is the best factor of the family
+        # the network predicts: its iteration count is the network's ceiling
+        solves["iteration_ratio_learned_over_fsai"] = round(
+            solves["learned_fp64_fcg"]["its"] / max(solves["fsai_fp64_pcg"]["its"], 1), 3)
         log(f"speedup learned+mixed vs FP64 Jacobi-PCG: {solves['speedup_learned_mixed_vs_fp64_jacobi']}x, "
-            f"vs mixed Jacobi-PCG: {solves['speedup_learned_mixed_vs_mixed_jacobi']}x")
+            f"vs mixed Jacobi-PCG: {solves['speedup_learned_mixed_vs_mixed_jacobi']}x; "
+            f"fsai+mixed {solves['speedup_fsai_mixed_vs_fp64_jacobi']}x; learned/fsai its "
+            f"{solves['iteration_ratio_learned_over_fsai']}")
         solves.update(spmv_report(lib, L, pr, cfg.mesh.nnodes, nnz, log))
         out["solve"] = solves
 

@@ -69,14 +69,15 @@ class CGOptions:
 # femtools/Solvers.F90:1844-1854): "eisenstat" is SSOR applied with
 # Eisenstat's trick, the same preconditioner as "sor" (one symmetric sweep).
 PRECONDITIONERS = {"none": L.FLGPU_PC_NONE, "jacobi": L.FLGPU_PC_JACOBI, "sor": L.FLGPU_PC_SOR,
-                   "eisenstat": L.FLGPU_PC_SOR, "learned": L.FLGPU_PC_LEARNED}
+                   "eisenstat": L.FLGPU_PC_SOR, "learned": L.FLGPU_PC_LEARNED,
+                   "fsai": L.FLGPU_PC_FSAI}
 
 
 @dataclass
 class SolverOptions:
     """solver options (schemas/solvers.rnc:107-612)."""
     iterative_method: str = "cg"         # cg | gmres | fcg
-    preconditioner: str = "jacobi"       # jacobi | none | sor | eisenstat | learned
+    preconditioner: str = "jacobi"       # jacobi | none | sor | eisenstat | learned | fsai
     relative_error: float = 1e-8
     absolute_error: float = 0.0
     max_iterations: int = 10000

@@ -1203,10 +1203,12 @@ int solve(flgpu_ctx* c, const flgpu_solver_params* sp, double* x_host, flgpu_sol
         return set_error(FLGPU_EINVAL, "flgpu_solve: precision %d", sp->precision);
     if (sp->ksp != FLGPU_KSP_CG && sp->ksp != FLGPU_KSP_GMRES && sp->ksp != FLGPU_KSP_FCG)
         return set_error(FLGPU_EINVAL, "flgpu_solve: ksp %d", sp->ksp);
-    if (sp->pc != FLGPU_PC_NONE && sp->pc != FLGPU_PC_JACOBI && sp->pc != FLGPU_PC_LEARNED && sp->pc != FLGPU_PC_SOR)
+    if (sp->pc != FLGPU_PC_NONE && sp->pc != FLGPU_PC_JACOBI && sp->pc != FLGPU_PC_LEARNED && sp->pc != FLGPU_PC_SOR &&
+        sp->pc != FLGPU_PC_FSAI)
         return set_error(FLGPU_EINVAL, "flgpu_solve: pc %d", sp->pc);
-    if (sp->pc == FLGPU_PC_LEARNED && sp->ksp == FLGPU_KSP_GMRES)
-        return set_error(FLGPU_EUNSUPPORTED, "flgpu_solve: the learned preconditioner runs with cg/fcg");
+    const bool factor = sp->pc == FLGPU_PC_LEARNED || sp->pc == FLGPU_PC_FSAI;
+    if (factor && sp->ksp == FLGPU_KSP_GMRES)
+        return set_error(FLGPU_EUNSUPPORTED, "flgpu_solve: the learned and fsai preconditioners run with cg/fcg");
     FLGPU_TRY(ensure_work(c, mixed));
     KrylovWork* k = c->kw;
     const bool zero_x = sp->start_from_zero || !x_host;
@@ -1215,12 +1217,11 @@ int solve(flgpu_ctx* c, const flgpu_solver_params* sp, double* x_host, flgpu_sol
     FLGPU_TRY(build_reduced(c, k, mixed, zero_x));
     k->prepared = true;
     k->prepared_mixed = mixed;
-    if (sp->pc == FLGPU_PC_LEARNED) FLGPU_TRY(learned_prepare(c, mixed));
+    if (factor) FLGPU_TRY(learned_prepare(c, mixed, sp->pc == FLGPU_PC_FSAI));
     if (sp->ksp == FLGPU_KSP_GMRES) {
         FLGPU_TRY(mixed ? run_gmres<float>(c, sp, &info) : run_gmres<double>(c, sp, &info));
     } else {
-        const bool learned = sp->pc == FLGPU_PC_LEARNED;
-        FLGPU_TRY(mixed ? run_cg<float>(c, sp, &info, learned) : run_cg<double>(c, sp, &info, learned));
+        FLGPU_TRY(mixed ? run_cg<float>(c, sp, &info, factor) : run_cg<double>(c, sp, &info, factor));
     }
     hipLaunchKernelGGL(k_finish_x, dim3(grid_for(c->rows, kVecBlock)), dim3(kVecBlock), 0, c->stream, c->rows,
                        k->x.p, c->rhs.p, c->inactive.p, c->have_inactive ? 1 : 0);
@@ -1396,9 +1397,17 @@ int csr_mult(flgpu_ctx* c, const double* x, double* y) {
     return FLGPU_OK;
 }
 
+// rt = D r (the residual in the factor's scaled space)
+__global__ void __launch_bounds__(kVecBlock)
+k_scale_res(int n, const double* __restrict__ r, const float* __restrict__ dsq, double* __restrict__ rt) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        rt[i] = r[i] * (double)dsq[i];
+}
+
 int pc_apply_host(flgpu_ctx* c, const flgpu_solver_params* sp, const double* r, double* z) {
-    if (sp->pc != FLGPU_PC_NONE && sp->pc != FLGPU_PC_JACOBI && sp->pc != FLGPU_PC_SOR)
-        return set_error(FLGPU_EUNSUPPORTED, "flgpu_pc_apply: pc %d (none, jacobi or sor)", sp->pc);
+    if (sp->pc != FLGPU_PC_NONE && sp->pc != FLGPU_PC_JACOBI && sp->pc != FLGPU_PC_SOR &&
+        sp->pc != FLGPU_PC_LEARNED && sp->pc != FLGPU_PC_FSAI)
+        return set_error(FLGPU_EUNSUPPORTED, "flgpu_pc_apply: pc %d", sp->pc);
     FLGPU_TRY(ensure_work(c, false));
     KrylovWork* k = c->kw;
     FLGPU_TRY(build_reduced(c, k, false, true));
@@ -1409,6 +1418,14 @@ int pc_apply_host(flgpu_ctx* c, const flgpu_solver_params* sp, const double* r, 
     if (sp->pc == FLGPU_PC_SOR) {
         FLGPU_TRY(ssor_prepare(c, k));
         FLGPU_TRY(ssor_sweeps<double>(c, k, k->val64.p, k->r.p, k->z.p, nullptr));
+    } else if (sp->pc == FLGPU_PC_LEARNED || sp->pc == FLGPU_PC_FSAI) {
+        FLGPU_TRY(learned_prepare(c, false, sp->pc == FLGPU_PC_FSAI));
+        hipLaunchKernelGGL(k_scale_res, dim3(k->nblk), dim3(kVecBlock), 0, c->stream, n, k->r.p, learned_scale(c),
+                           k->rt.p);
+        FLGPU_HIP(hipGetLastError());
+        int np = 0;
+        FLGPU_TRY(learned_precond_apply<double>(c, k->r.p, k->rt.p, k->z.p, k->y.p, k->r.p, k->part(PART_PC), &np,
+                                                nullptr));
     } else {
         hipLaunchKernelGGL((k_gm_precres<double>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, n, k->r.p,
                            sp->pc == FLGPU_PC_JACOBI ? k->dinv.p : nullptr, k->z.p);

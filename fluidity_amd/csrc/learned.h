@@ -10,7 +10,7 @@ namespace flgpu {
 
 // Predict G for the device-resident reduced system (values in the solver's
 // working precision).  Called once per solve.
-int learned_prepare(flgpu_ctx* c, bool mixed);
+int learned_prepare(flgpu_ctx* c, bool mixed, bool fsai = false);
 
 // The diagonal scaling D = diag(A)^(-1/2) of the preconditioner's scaled
 // space (valid after learned_prepare).

@@ -76,6 +76,7 @@ struct LearnedState {
     int n = 0;
     int64_t nnz = 0;
     DevBuf<int32_t> tmap;
+    DevBuf<int32_t> fsai_rows;  // FSAI: the rows longer than the first kernel takes, then their count
     DevBuf<float> dsq, sval, feat;
     // activations (training keeps every layer)
     DevBuf<float> h, PQ, M, U, ab;  // ab: alpha,beta,psi per node (3N)
@@ -410,6 +411,103 @@ k_build_g(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__
         }
         gv[k] = (GV)g;
     }
+}
+
+// ---------------------------------------------------------------------------
+// The exact factorised sparse approximate inverse (FSAI) on the same
+// pattern: the ceiling of the network's factor family, and a preconditioner
+// of its own.  Row i of Gs, on the lower pattern P_i of row i of
+// S = D A D (columns j <= i, the diagonal last), minimises the Kaporin
+// condition of Gs S Gs^T: with S[P_i, P_i] = L L^T (Cholesky),
+// Gs[i, P_i] = L^-T e_last, so that (Gs S Gs^T)_ii = 1 and
+// (Gs S)_ij = 0 for the other j in P_i (Kolotilina & Yeremin 1993).
+// ---------------------------------------------------------------------------
+constexpr int kFsaiMax = 48;  // largest lower pattern solved; longer rows keep the Jacobi row
+
+template <typename GV>
+__device__ inline void fsai_jacobi_row(int k0, int kd, GV* gv) {
+    for (int q = k0; q < kd; q++) gv[q] = (GV)0;
+    gv[kd] = (GV)1;  // S_ii = 1
+}
+
+// One thread per row, NT threads per block.  The first launch (LONG false)
+// takes every row whose lower pattern has k <= KMAX entries (all rows of a
+// P1 mesh) and lists the longer ones in long_rows / *nlong; the second
+// (LONG true) takes the listed rows (the long rows of a P2 mesh) with a
+// larger KMAX.  The local matrix S[P_i, P_i] sits in the block's LDS,
+// entry-major so that a block's threads touch consecutive words; S is
+// formed in FP64 from the reduced FP64 values and D.  Writes Gs at the
+// row's lower positions of A's pattern, the diagonal included.
+template <typename GV, int KMAX, int NT, bool LONG>
+__global__ void __launch_bounds__(NT)
+k_fsai_lower(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+             const int32_t* __restrict__ diag, const double* __restrict__ val, const float* __restrict__ dsq,
+             GV* __restrict__ gv, int32_t* __restrict__ long_rows, int32_t* __restrict__ nlong) {
+    __shared__ double sl[KMAX * (KMAX + 1) / 2][NT];
+    __shared__ double sg[KMAX][NT];
+    const int idx = blockIdx.x * NT + threadIdx.x;
+    if (idx >= n) return;
+    const int i = LONG ? long_rows[idx] : idx;
+    const int t = threadIdx.x;
+    const int k0 = rowptr[i], kd = diag[i];
+    const int k = kd - k0 + 1;  // lower pattern size, the diagonal last (columns sorted)
+    if (k > KMAX) {
+        if (LONG) fsai_jacobi_row(k0, kd, gv);
+        else long_rows[atomicAdd(nlong, 1)] = i;
+        return;
+    }
+#define B(a, b) sl[(a) * ((a) + 1) / 2 + (b)][t]
+    // B(a, b) = S[P_a, P_b] for b <= a: row P_a of A merged with P
+    for (int a = 0; a < k; a++) {
+        const int pa = col[k0 + a];
+        const double da = (double)dsq[pa];
+        int q = rowptr[pa];
+        const int qe = rowptr[pa + 1];
+        for (int b = 0; b <= a; b++) {
+            const int pb = col[k0 + b];
+            while (q < qe && col[q] < pb) q++;
+            B(a, b) = (q < qe && col[q] == pb) ? val[q] * da * (double)dsq[pb] : 0.0;
+        }
+    }
+    // Cholesky B = L L^T in place
+    for (int a = 0; a < k; a++) {
+        double d = B(a, a);
+        for (int c = 0; c < a; c++) d -= B(a, c) * B(a, c);
+        if (!(d > 0.0)) {  // not positive definite
+            fsai_jacobi_row(k0, kd, gv);
+            return;
+        }
+        const double l = sqrt(d);
+        B(a, a) = l;
+        for (int r = a + 1; r < k; r++) {
+            double v = B(r, a);
+            for (int c = 0; c < a; c++) v -= B(r, c) * B(a, c);
+            B(r, a) = v / l;
+        }
+    }
+    // L^T g = e_last, backwards
+    for (int a = k - 1; a >= 0; a--) {
+        double v = a == k - 1 ? 1.0 : 0.0;
+        for (int r = a + 1; r < k; r++) v -= B(r, a) * sg[r][t];
+        sg[a][t] = v / B(a, a);
+    }
+#undef B
+    for (int a = 0; a < k; a++) gv[k0 + a] = (GV)sg[a][t];
+}
+
+// The upper positions of A's pattern take the transposed lower values.
+template <typename GV>
+__global__ void __launch_bounds__(256)
+k_fsai_upper(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+             const int32_t* __restrict__ tmap, GV* __restrict__ gv) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t >> 5, lane = t & 31;
+    if (i >= n) return;
+    for (int k = rowptr[i] + lane; k < rowptr[i + 1]; k += 32)
+        if (col[k] > i) {
+            const int kt = tmap[k];
+            gv[k] = kt >= 0 ? gv[kt] : (GV)0;
+        }
 }
 
 // gathers the factor values of one half from A's pattern
@@ -852,19 +950,41 @@ int reduced_prepare(flgpu_ctx* c);
 template <typename GV, typename HV>
 static int split_halves(flgpu_ctx* c, LearnedState* s, const GV* gv, DevBuf<HV>& gl, DevBuf<HV>& gu);
 
-int learned_prepare(flgpu_ctx* c, bool mixed) {
+template <typename GV>
+static int build_fsai(flgpu_ctx* c, LearnedState* s, GV* out) {
+    const int n = c->rows;
+    const double* val = reduced_values(c);
+    FLGPU_TRY(s->fsai_rows.alloc((size_t)n + 1));
+    int32_t* cnt = s->fsai_rows.p + n;
+    FLGPU_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t), c->stream));
+    hipLaunchKernelGGL((k_fsai_lower<GV, 16, 64, false>), dim3(grid_for(n, 64)), dim3(64), 0, c->stream, n,
+                       c->rowptr.p, c->col.p, c->diag.p, val, s->dsq.p, out, s->fsai_rows.p, cnt);
+    FLGPU_HIP(hipGetLastError());
+    int32_t nlong = 0;
+    FLGPU_HIP(hipMemcpyAsync(&nlong, cnt, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    FLGPU_HIP(hipStreamSynchronize(c->stream));
+    if (nlong > 0)
+        hipLaunchKernelGGL((k_fsai_lower<GV, kFsaiMax, 8, true>), dim3(grid_for(nlong, 8)), dim3(8), 0, c->stream,
+                           nlong, c->rowptr.p, c->col.p, c->diag.p, val, s->dsq.p, out, s->fsai_rows.p, cnt);
+    hipLaunchKernelGGL((k_fsai_upper<GV>), dim3(grid_for((int64_t)n * 32, 256)), dim3(256), 0, c->stream, n,
+                       c->rowptr.p, c->col.p, s->tmap.p, out);
+    FLGPU_HIP(hipGetLastError());
+    return FLGPU_OK;
+}
+
+int learned_prepare(flgpu_ctx* c, bool mixed, bool fsai) {
     if (!c->all_diag) return set_error(FLGPU_EUNSUPPORTED, "learned preconditioner: a row has no diagonal entry");
     LearnedState* s = state(c);
-    FLGPU_TRY(ensure_params(c, s));
+    if (!fsai) FLGPU_TRY(ensure_params(c, s));
     FLGPU_TRY(graph_prepare(c, s, reduced_values(c)));
-    FLGPU_TRY(forward(c, s, false));
+    if (!fsai) FLGPU_TRY(forward(c, s, false));
     if (mixed) {
         FLGPU_TRY(s->gv.alloc(c->nnz));
-        FLGPU_TRY(build_g<float>(c, s, s->gv.p, 1));
+        FLGPU_TRY(fsai ? build_fsai<float>(c, s, s->gv.p) : build_g<float>(c, s, s->gv.p, 1));
         FLGPU_TRY((split_halves<float, _Float16>(c, s, s->gv.p, s->gl16, s->gu16)));
     } else {
         FLGPU_TRY(s->gv64.alloc(c->nnz));
-        FLGPU_TRY(build_g<double>(c, s, s->gv64.p, 1));
+        FLGPU_TRY(fsai ? build_fsai<double>(c, s, s->gv64.p) : build_g<double>(c, s, s->gv64.p, 1));
         FLGPU_TRY((split_halves<double, double>(c, s, s->gv64.p, s->gl64, s->gu64)));
     }
     FLGPU_HIP(hipStreamSynchronize(c->stream));

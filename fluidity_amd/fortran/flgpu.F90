@@ -23,7 +23,7 @@ module flgpu
   integer(c_int32_t), parameter, public :: FLGPU_ASM_AUTO = 0, FLGPU_ASM_COLOUR = 1, FLGPU_ASM_BLOCKED = 2
   integer(c_int32_t), parameter, public :: FLGPU_KSP_CG = 0, FLGPU_KSP_GMRES = 1, FLGPU_KSP_FCG = 2
   integer(c_int32_t), parameter, public :: FLGPU_PC_NONE = 0, FLGPU_PC_JACOBI = 1, FLGPU_PC_LEARNED = 2, &
-                                            FLGPU_PC_SOR = 3
+                                            FLGPU_PC_SOR = 3, FLGPU_PC_FSAI = 4
   integer(c_int32_t), parameter, public :: FLGPU_NORM_PRECONDITIONED = 0, FLGPU_NORM_UNPRECONDITIONED = 1
   integer(c_int32_t), parameter, public :: FLGPU_FIELD_T = 0, FLGPU_FIELD_VELOCITY = 1, &
        FLGPU_FIELD_DIFFUSIVITY = 2, FLGPU_FIELD_SOURCE = 3, FLGPU_FIELD_ABSORPTION = 4
@@ -313,6 +313,8 @@ contains
        pc = FLGPU_PC_SOR
     case ("learned")
        pc = FLGPU_PC_LEARNED
+    case ("fsai")
+       pc = FLGPU_PC_FSAI
     case default
        pc = -1
     end select

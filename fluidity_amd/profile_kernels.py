@@ -1,13 +1,15 @@
 """Standalone launches of the hot kernels, for profiling under rocprofv3.
 
     python -m fluidity_amd.profile_kernels [--config c5] [--reps 3] [--copy-mib 1024]
+                                           [--solves fsai:mixed,learned:mixed,...]
 
 runs, on the device-resident system of the configuration: `reps` blocked
 assemblies, `reps` colour-scheme assemblies, the solver-matrix SpMV (FP64 and
-FP32 values) `reps` times each, and finally one plain device-to-device copy
-of `copy-mib` MiB through the HIP runtime (its blit kernel reads and writes
-exactly that many bytes), which calibrates the FETCH_SIZE / WRITE_SIZE
-counters.
+FP32 values) `reps` times each, then each listed solve (preconditioner:
+precision, CG with the learned preconditioner's FCG, to a 1e-8 true relative
+residual) once, and finally one plain device-to-device copy of `copy-mib`
+MiB through the HIP runtime (its blit kernel reads and writes exactly that
+many bytes), which calibrates the FETCH_SIZE / WRITE_SIZE counters.
 """
 from __future__ import annotations
 
@@ -21,6 +23,7 @@ def main() -> int:
     ap.add_argument("--config", default="c5")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--solves", default="")
     args = ap.parse_args()
     from . import cg, lib as L, meshgen as M
 
@@ -38,6 +41,15 @@ def main() -> int:
         L.check(lib.flgpu_solver_prepare(pr.ctx, prec), "solver_prepare")
         L.check(lib.flgpu_time_spmv(pr.ctx, prec, args.reps, C.byref(ms)), "time_spmv")
     print(f"spmv {args.reps}x per precision", flush=True)
+    if args.solves:
+        from . import learned
+        learned.set_params(pr, learned.load())
+    for item in filter(None, args.solves.split(",")):
+        pc, prec = item.split(":")
+        so = cg.SolverOptions("fcg" if pc == "learned" else "cg", pc, relative_error=1e-8, norm="unpreconditioned",
+                              max_iterations=50000, precision=prec, check_every=32)
+        _, info = cg.petsc_solve(pr, so)
+        print(f"solve {pc} {prec}: {info.its} its, reason {info.reason}", flush=True)
     pr.close()
     if args.copy_mib > 0:
         _device_copy(args.copy_mib << 20)

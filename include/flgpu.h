@@ -281,7 +281,16 @@ enum { FLGPU_KSP_CG = 0, FLGPU_KSP_GMRES = 1, FLGPU_KSP_FCG = 2 };
  * femtools/Solvers.F90:1844-1854 -- one symmetric Gauss-Seidel sweep
  * (omega = 1) taken colour by colour over a first-fit colouring of the
  * matrix graph, instead of PETSc's sequential natural-order sweep. */
-enum { FLGPU_PC_NONE = 0, FLGPU_PC_JACOBI = 1, FLGPU_PC_LEARNED = 2, FLGPU_PC_SOR = 3 };
+/* FLGPU_PC_FSAI: the factorised sparse approximate inverse M^-1 = G^T G
+ * with G lower triangular on A's lower pattern, each row the exact
+ * minimiser of the Kaporin condition number (Kolotilina & Yeremin): the
+ * best factor of the family the learned preconditioner predicts, built
+ * per solve from one small Cholesky solve per row; no counterpart in
+ * Fluidity's option tree (a preconditioner PETSc's PCHPDDM/PCSPAI family
+ * would supply).  A is taken as symmetric, as CG takes it: each row's
+ * local matrix is read from A's lower triangle.  Rows with more than 48
+ * lower entries keep the Jacobi row. */
+enum { FLGPU_PC_NONE = 0, FLGPU_PC_JACOBI = 1, FLGPU_PC_LEARNED = 2, FLGPU_PC_SOR = 3, FLGPU_PC_FSAI = 4 };
 
 typedef struct {
     int32_t ksp, pc;
@@ -354,7 +363,9 @@ int flgpu_petsc_solve(flgpu_ctx *ctx, const flgpu_solver_params *sp,
 
 /* PCApply of the preconditioner flgpu_solve builds for the reduced system
  * (setup_pc_from_options, femtools/Solvers.F90:1844-1854): z = M^-1 r for
- * pc none, jacobi or sor, FP64, host arrays of rows doubles. */
+ * pc none, jacobi, sor, learned or fsai (the factor built and applied in
+ * FP64, as flgpu_solve does with precision FLGPU_FP64), host arrays of rows
+ * doubles. */
 int flgpu_pc_apply(flgpu_ctx *ctx, const flgpu_solver_params *sp,
                    const double *r, double *z);
 

@@ -421,6 +421,54 @@ def reduced_system(sparsity, A, inactive):
     return rowptr, col, val
 
 
+def fsai_factor(rowptr, col, val, kmax: int = 48):
+    """The factorised sparse approximate inverse on the lower pattern of a
+    0-based CSR SPD matrix (Kolotilina & Yeremin 1993; no counterpart in
+    Fluidity, whose preconditioners come from PETSc): row i of G on
+    P_i = {j <= i in row i} is g = y / sqrt(y_i) with A[P_i, P_i] y = e_i,
+    so that G A G^T has a unit diagonal and (G A)_ij = 0 for the other j in
+    P_i.  A is taken as symmetric, as CG takes it: the local matrix is read
+    from A's lower triangle and mirrored.  A row with more than kmax lower
+    entries, or whose local matrix is not positive definite, keeps the
+    Jacobi row g = e_i / sqrt(a_ii).  Returns G as a dict
+    row -> (columns, values)."""
+    n = rowptr.size - 1
+    rows = {}
+    for i in range(n):
+        c = col[rowptr[i]:rowptr[i + 1]]
+        P = np.sort(c[c <= i])
+        dii = val[rowptr[i]:rowptr[i + 1]][c == i][0]
+        if P.size > kmax:
+            rows[i] = (np.array([i]), np.array([1.0 / np.sqrt(dii)]))
+            continue
+        Ai = np.zeros((P.size, P.size))
+        for a, pa in enumerate(P):
+            ca = col[rowptr[pa]:rowptr[pa + 1]]
+            va = val[rowptr[pa]:rowptr[pa + 1]]
+            m = {int(j): v for j, v in zip(ca, va)}
+            Ai[a, :a + 1] = [m.get(int(pb), 0.0) for pb in P[:a + 1]]
+        Ai = np.tril(Ai) + np.tril(Ai, -1).T
+        try:
+            np.linalg.cholesky(Ai)
+        except np.linalg.LinAlgError:
+            rows[i] = (np.array([i]), np.array([1.0 / np.sqrt(dii)]))
+            continue
+        e = np.zeros(P.size)
+        e[-1] = 1.0
+        y = np.linalg.solve(Ai, e)
+        rows[i] = (P, y / np.sqrt(y[-1]))
+    return rows
+
+
+def fsai_apply(G, r):
+    """z = G^T (G r) for the rows of fsai_factor."""
+    y = np.array([np.dot(v, r[c]) for i, (c, v) in sorted(G.items())])
+    z = np.zeros_like(r)
+    for i, (c, v) in G.items():
+        z[c] += v * y[i]
+    return z
+
+
 # ---------------------------------------------------------------------------
 # The pressure path: ct_m, the lumped velocity mass, and C^T M_l^-1 C
 # (assemble/Momentum_CG.F90, assemble/Assemble_CMC.F90)
