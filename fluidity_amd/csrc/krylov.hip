@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "flgpu_internal.h"
@@ -44,6 +45,14 @@ struct KrylovWork {
     bool use16 = false;
     int nrb = 0;
     int64_t rb_key = -1;     // sparsity generation rb was built for
+    // sliced ELL copy of the solver matrix, used by spmv when its padding is small
+    bool use_sell = false, sell16 = false;
+    int nsl = 0;
+    int64_t nslot = 0;
+    DevBuf<int32_t> sptr, ssrc, scol32;
+    DevBuf<int16_t> scol16;
+    DevBuf<double> sval64;
+    DevBuf<float> sval32;
     Scal* h_scal = nullptr;  // host copy of the current slot (pinned when h_scal_owned)
     bool h_scal_owned = false;
     int nblk = 1;
@@ -500,7 +509,31 @@ static int lanes_for(const flgpu_ctx* c) {
     return 64;
 }
 
-static int spmv_grid(const flgpu_ctx* c) { return stream_grid(c->kw->nrb); }
+static int spmv_grid(const flgpu_ctx* c) {
+    return c->kw->use_sell ? sell_grid(c->kw->nsl) : stream_grid(c->kw->nrb);
+}
+
+// The sliced ELL values matching a CSR value array of the solver matrix
+// (null for any other array, which takes the CSR product).
+static const double* sell_values(const KrylovWork* k, const double* val) {
+    return k->use_sell && val == k->val64.p ? k->sval64.p : nullptr;
+}
+static const float* sell_values(const KrylovWork* k, const float* val) {
+    return k->use_sell && val == k->val32.p ? k->sval32.p : nullptr;
+}
+
+template <typename V, typename CT, typename XV, typename YV>
+static void sell_launch(flgpu_ctx* c, KrylovWork* k, const V* sval, const CT* col, const XV* x, YV* y,
+                        const double* bsub, int want_dot, const int* done_flag) {
+    const SellOp<V, CT> m{k->nsl, c->rows, k->sptr.p, c->rowptr.p, col, sval, nullptr};
+    const int grid = sell_grid(k->nsl);
+    if (want_dot)
+        hipLaunchKernelGGL((k_sell<V, V, CT, XV, YV, DOT_XY>), dim3(grid), dim3(kRowBlock), 0, c->stream, m, x, y,
+                           bsub, (const V*)nullptr, (const V*)nullptr, k->part(PART_SPMV), done_flag);
+    else
+        hipLaunchKernelGGL((k_sell<V, V, CT, XV, YV, DOT_NONE>), dim3(grid), dim3(kRowBlock), 0, c->stream, m, x,
+                           y, bsub, (const V*)nullptr, (const V*)nullptr, k->part(PART_SPMV), done_flag);
+}
 
 template <typename AV, typename CT>
 static CsrOp<AV, CT> matrix_op(flgpu_ctx* c, KrylovWork* k, const AV* val, const CT* col) {
@@ -526,11 +559,46 @@ static void spmv_launch(flgpu_ctx* c, KrylovWork* k, const V* val, const CT* col
 template <typename V, typename XV, typename YV>
 static int spmv(flgpu_ctx* c, KrylovWork* k, const V* val, const XV* x, YV* y, const double* bsub,
                 int want_dot, const int* done_flag) {
-    if (k->use16)
+    if (const V* sv = sell_values(k, val)) {
+        if (k->sell16)
+            sell_launch<V, int16_t, XV, YV>(c, k, sv, k->scol16.p, x, y, bsub, want_dot, done_flag);
+        else
+            sell_launch<V, int32_t, XV, YV>(c, k, sv, k->scol32.p, x, y, bsub, want_dot, done_flag);
+    } else if (k->use16)
         spmv_launch<V, int16_t, XV, YV>(c, k, val, k->col16.p, x, y, bsub, want_dot, done_flag);
     else
         spmv_launch<V, int32_t, XV, YV>(c, k, val, c->col.p, x, y, bsub, want_dot, done_flag);
     FLGPU_HIP(hipGetLastError());
+    return FLGPU_OK;
+}
+
+// The sliced ELL plan of the solver matrix, kept when its padding is at
+// most kSellMaxFill times the entries (FLGPU_SPMV=csr keeps the CSR
+// stream product throughout).
+constexpr double kSellMaxFill = 1.2;
+
+static int sell_setup(flgpu_ctx* c, KrylovWork* k) {
+    const char* e = getenv("FLGPU_SPMV");
+    k->use_sell = false;
+    k->sptr.release();
+    k->ssrc.release();
+    k->scol16.release();
+    k->scol32.release();
+    k->sval64.release();
+    k->sval32.release();
+    if (e && std::string(e) == "csr") return FLGPU_OK;
+    SellHost h;
+    sell_plan(c->h_rowptr.data(), c->h_col.data(), c->rows, nullptr, h);
+    if (h.fill(c->nnz) > kSellMaxFill) return FLGPU_OK;
+    FLGPU_TRY(k->sptr.upload(h.sptr.data(), h.sptr.size(), c->stream));
+    FLGPU_TRY(k->ssrc.upload(h.src.data(), h.src.size(), c->stream));
+    if (h.use16) FLGPU_TRY(k->scol16.upload(h.c16.data(), h.c16.size(), c->stream));
+    else FLGPU_TRY(k->scol32.upload(h.c32.data(), h.c32.size(), c->stream));
+    FLGPU_HIP(hipStreamSynchronize(c->stream));  // h is freed on return
+    k->nsl = h.nsl;
+    k->nslot = h.nslot;
+    k->sell16 = h.use16;
+    k->use_sell = true;
     return FLGPU_OK;
 }
 
@@ -575,7 +643,12 @@ static int ensure_work(flgpu_ctx* c, bool mixed) {
         else k->col16.release();
         FLGPU_HIP(hipStreamSynchronize(c->stream));  // rb and c16 are freed on return
         k->nrb = (int)rb.size() - 1;
+        FLGPU_TRY(sell_setup(c, k));
         k->rb_key = c->sparsity_gen;
+    }
+    if (k->use_sell) {
+        FLGPU_TRY(k->sval64.alloc((size_t)k->nslot));
+        if (mixed) FLGPU_TRY(k->sval32.alloc((size_t)k->nslot));
     }
     FLGPU_TRY(k->scal.alloc(2));
     if (!k->h_scal) {
@@ -617,6 +690,14 @@ static int build_reduced(flgpu_ctx* c, KrylovWork* k, bool mixed, bool zero_x) {
                        c->rowptr.p, c->col.p, c->A.p, c->rhs.p, c->inactive.p, c->have_inactive ? 1 : 0, k->S(),
                        k->val64.p, mixed ? k->val32.p : nullptr, k->b.p, k->dinv.p, mixed ? k->dinv32.p : nullptr,
                        k->x.p, zero_x ? 1 : 0);
+    if (k->use_sell) {
+        const unsigned g = grid_for(k->nslot, 256);
+        hipLaunchKernelGGL((k_sell_gather<double, double>), dim3(g), dim3(256), 0, c->stream, k->nslot, k->ssrc.p,
+                           k->val64.p, k->sval64.p);
+        if (mixed)
+            hipLaunchKernelGGL((k_sell_gather<float, float>), dim3(g), dim3(256), 0, c->stream, k->nslot,
+                               k->ssrc.p, k->val32.p, k->sval32.p);
+    }
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <vector>
@@ -283,6 +284,170 @@ k_csr_stream(CsrOp<AV, CT> m, const XV* __restrict__ x, YV* __restrict__ y, cons
     }
 }
 
+// ---------------------------------------------------------------------
+// Sliced ELL products
+// ---------------------------------------------------------------------
+// A slice is 64 consecutive rows, one wave: entry j of the slice's rows
+// sits at slots sptr[s] + 64 j + lane, so every load of values and column
+// offsets is one contiguous line per wave, with no row bounds, no LDS and
+// no barrier.  A slice is as wide as its longest row; the slots past a
+// row's end are padding (value 0, column the row itself) and are skipped.
+// Meshes numbered in space-filling order give slices of near-equal rows,
+// so the padding is small; sell_plan reports it and the caller keeps the
+// CSR stream product where it is not.
+constexpr int kSlice = 64;
+
+template <typename AV, typename CT>
+struct SellOp {
+    int nsl, n;
+    const int32_t* sptr;    // nsl+1 first slots (multiples of 64)
+    const int32_t* rowptr;  // row lengths from the CSR row pointers
+    const CT* col;          // column, or int16 offset from the slice's first row
+    const AV* val;
+    const float* rs;        // optional row scale: y_i = rs_i (A x)_i
+};
+
+// y = A x (or y = bsub - A x) over slices; each lane adds its row's
+// products in entry order, each product rounded before the addition (no
+// contraction), the same sums as k_csr_stream.  Dot products as there.
+// Slices go to waves in groups of kRowBlock/64 through RowBlocks, so each
+// XCD takes a contiguous share.
+template <typename V, typename AV, typename CT, typename XV, typename YV, int DOTS>
+__global__ void __launch_bounds__(kRowBlock)
+k_sell(SellOp<AV, CT> m, const XV* __restrict__ x, YV* __restrict__ y, const double* __restrict__ bsub,
+       const V* __restrict__ dr, const V* __restrict__ dw, double* __restrict__ partials,
+       const int* __restrict__ done_flag) {
+#pragma clang fp contract(off)
+    if (done_flag && *done_flag) return;
+    constexpr int WPB = kRowBlock / kSlice;
+    constexpr int U = 8;
+    const int wave = threadIdx.x / kSlice, lane = threadIdx.x % kSlice;
+    const CT* __restrict__ col = m.col;
+    const AV* __restrict__ val = m.val;
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+    const RowBlocks it((m.nsl + WPB - 1) / WPB);
+    for (int g = it.b; g < it.hi; g += it.step) {
+        const int sl = g * WPB + wave;
+        if (sl >= m.nsl) continue;
+        const int r0 = sl * kSlice, row = r0 + lane;
+        const int base = m.sptr[sl], w = (m.sptr[sl + 1] - base) / kSlice;
+        const int len = row < m.n ? m.rowptr[row + 1] - m.rowptr[row] : 0;
+        V acc = (V)0;
+        for (int j0 = 0; j0 < w; j0 += U) {
+            // Every load is unconditional (a step past the slice's width
+            // re-reads its last slot, padding holds an in-range column), so
+            // the U loads of a step are all in flight together; only the
+            // additions are predicated on the row's length.
+            int cj[U];
+            AV av[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int slot = base + min(j0 + u, w - 1) * kSlice + lane;
+                cj[u] = csr_col(col, slot, r0);
+                av[u] = val[slot];
+            }
+            XV xv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) xv[u] = x[cj[u]];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const V pv = (V)av[u] * (V)xv[u];
+                if (j0 + u < len) acc += pv;
+            }
+        }
+        if (row < m.n) {
+            if (m.rs) acc = acc * (V)m.rs[row];
+            double yv = (double)acc;
+            if (bsub) yv = bsub[row] - yv;
+            const YV yo = (YV)yv;
+            y[row] = yo;
+            if (DOTS == DOT_XY) {
+                d0 += (double)x[row] * yv;
+                d1 += yv * yv;
+            } else if (DOTS == DOT_PC) {
+                const double z = (double)yo;
+                d0 += (double)dr[row] * z;
+                d1 += z * z;
+                d2 += z * (double)dw[row];
+            }
+        }
+    }
+    if (DOTS == DOT_XY) {
+        double v[2] = {d0, d1};
+        block_partials<2>(v, partials);
+    } else if (DOTS == DOT_PC) {
+        double v[3] = {d0, d1, d2};
+        block_partials<3>(v, partials);
+    }
+}
+
+// Host-side plan of a sliced ELL operand: slot offsets, column indices
+// (int16 offsets from the slice's first row when they all fit) and, per
+// slot, the entry of the source value array it takes (vmap[k] for CSR
+// entry k, or k itself; -1 for padding).
+struct SellHost {
+    int nsl = 0;
+    int64_t nslot = 0;
+    bool use16 = false;
+    std::vector<int32_t> sptr, src, c32;
+    std::vector<int16_t> c16;
+    double fill(int64_t nnz) const { return nnz > 0 ? (double)nslot / (double)nnz : 1.0; }
+};
+
+inline void sell_plan(const int32_t* rowptr, const int32_t* col, int n, const int32_t* vmap, SellHost& h) {
+    h.nsl = (n + kSlice - 1) / kSlice;
+    h.sptr.assign((size_t)h.nsl + 1, 0);
+    for (int sl = 0; sl < h.nsl; sl++) {
+        int w = 0;
+        for (int r = sl * kSlice; r < std::min(n, (sl + 1) * kSlice); r++) w = std::max(w, rowptr[r + 1] - rowptr[r]);
+        h.sptr[sl + 1] = h.sptr[sl] + w * kSlice;
+    }
+    h.nslot = h.sptr[h.nsl];
+    h.src.assign((size_t)h.nslot, -1);
+    h.c32.assign((size_t)h.nslot, 0);
+    bool fits = true;
+    for (int sl = 0; sl < h.nsl; sl++) {
+        const int r0 = sl * kSlice, w = (h.sptr[sl + 1] - h.sptr[sl]) / kSlice;
+        for (int lane = 0; lane < kSlice; lane++) {
+            const int r = r0 + lane;
+            const int len = r < n ? rowptr[r + 1] - rowptr[r] : 0;
+            for (int j = 0; j < w; j++) {
+                const size_t slot = (size_t)h.sptr[sl] + (size_t)j * kSlice + lane;
+                if (j < len) {
+                    const int k = rowptr[r] + j;
+                    h.src[slot] = vmap ? vmap[k] : k;
+                    h.c32[slot] = col[k];
+                } else {
+                    h.c32[slot] = r < n ? r : r0;
+                }
+                const int d = h.c32[slot] - r0;
+                if (d < -32768 || d > 32767) fits = false;
+            }
+        }
+    }
+    h.use16 = fits;
+    if (fits) {
+        h.c16.resize((size_t)h.nslot);
+        for (int sl = 0; sl < h.nsl; sl++)
+            for (int64_t q = h.sptr[sl]; q < h.sptr[sl + 1]; q++) h.c16[q] = (int16_t)(h.c32[q] - sl * kSlice);
+        std::vector<int32_t>().swap(h.c32);
+    }
+}
+
+// out[t] = src[t] >= 0 ? in[src[t]] : 0 (a value array in slot order)
+template <typename SV, typename DV>
+__global__ void __launch_bounds__(256)
+k_sell_gather(int64_t m, const int32_t* __restrict__ src, const SV* __restrict__ in, DV* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) {
+        const int32_t k = src[t];
+        out[t] = k >= 0 ? (DV)in[k] : (DV)0;
+    }
+}
+
+// Fixed grid for k_sell: the slice groups through stream_grid.
+inline int sell_grid(int nsl);
+
 // int16 column offsets from each row block's first row; false (and out
 // untouched) when an offset does not fit.
 inline bool compress_columns(const int32_t* rowptr, const int32_t* col, const std::vector<int32_t>& rb,
@@ -312,5 +477,7 @@ inline int stream_grid(int nrb) {
     const int g = nrb < cap ? nrb : cap;
     return g & ~7;
 }
+
+inline int sell_grid(int nsl) { return stream_grid((nsl + kRowBlock / kSlice - 1) / (kRowBlock / kSlice)); }
 
 }  // namespace flgpu

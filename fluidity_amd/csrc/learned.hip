@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "flgpu_internal.h"
@@ -91,6 +92,13 @@ struct LearnedState {
     bool l16 = false, u16 = false;
     int nrbl = 0, nrbu = 0;
     int64_t nnzl = 0, nnzu = 0;
+    // the halves as sliced ELL operands (when both pad little): the value
+    // arrays gl/gu then hold slots, gathered from the factor through lss/uss
+    bool sell = false, ls16 = false, us16 = false;
+    int nsll = 0, nslu = 0;
+    int64_t nslotl = 0, nslotu = 0;
+    DevBuf<int32_t> lsp, usp, lss, uss, lsc32, usc32;
+    DevBuf<int16_t> lsc16, usc16;
     // factor of the scaled matrix, Gs: FP16 for the mixed solve, FP64 for
     // the FP64 solve; the scaling D is applied by the products
     DevBuf<_Float16> gl16, gu16;
@@ -993,6 +1001,9 @@ int learned_prepare(flgpu_ctx* c, bool mixed, bool fsai) {
 
 // Compact lower/upper CSR of A's pattern (positions into A for the values)
 // and their stream row blocks; rebuilt when the sparsity changes.
+// The factor halves take the sliced ELL product when each pads at most this much.
+constexpr double kSellMaxFillHalf = 1.2;
+
 static int lu_prepare(flgpu_ctx* c, LearnedState* s) {
     if (s->lu_key == c->sparsity_gen) return FLGPU_OK;
     const int n = c->rows;
@@ -1037,6 +1048,31 @@ static int lu_prepare(flgpu_ctx* c, LearnedState* s) {
     s->u16 = compress_columns(up.data(), uc.data(), rbu, u16);
     if (s->l16) FLGPU_TRY(s->lcol16.upload(l16.data(), l16.size(), c->stream));
     if (s->u16) FLGPU_TRY(s->ucol16.upload(u16.data(), u16.size(), c->stream));
+    // sliced ELL plans of both halves, the slots mapped to the factor's entries
+    SellHost hl, hu;
+    const char* e = getenv("FLGPU_SPMV");
+    s->sell = false;
+    if (!(e && std::string(e) == "csr")) {
+        sell_plan(lp.data(), lc.data(), n, ls.data(), hl);
+        sell_plan(up.data(), uc.data(), n, us.data(), hu);
+        s->sell = hl.fill((int64_t)lc.size()) <= kSellMaxFillHalf && hu.fill((int64_t)uc.size()) <= kSellMaxFillHalf;
+    }
+    if (s->sell) {
+        FLGPU_TRY(s->lsp.upload(hl.sptr.data(), hl.sptr.size(), c->stream));
+        FLGPU_TRY(s->usp.upload(hu.sptr.data(), hu.sptr.size(), c->stream));
+        FLGPU_TRY(s->lss.upload(hl.src.data(), hl.src.size(), c->stream));
+        FLGPU_TRY(s->uss.upload(hu.src.data(), hu.src.size(), c->stream));
+        if (hl.use16) FLGPU_TRY(s->lsc16.upload(hl.c16.data(), hl.c16.size(), c->stream));
+        else FLGPU_TRY(s->lsc32.upload(hl.c32.data(), hl.c32.size(), c->stream));
+        if (hu.use16) FLGPU_TRY(s->usc16.upload(hu.c16.data(), hu.c16.size(), c->stream));
+        else FLGPU_TRY(s->usc32.upload(hu.c32.data(), hu.c32.size(), c->stream));
+        s->ls16 = hl.use16;
+        s->us16 = hu.use16;
+        s->nsll = hl.nsl;
+        s->nslu = hu.nsl;
+        s->nslotl = hl.nslot;
+        s->nslotu = hu.nslot;
+    }
     FLGPU_HIP(hipStreamSynchronize(c->stream));
     s->nnzl = (int64_t)lc.size();
     s->nnzu = (int64_t)uc.size();
@@ -1049,6 +1085,16 @@ static int lu_prepare(flgpu_ctx* c, LearnedState* s) {
 template <typename GV, typename HV>
 static int split_halves(flgpu_ctx* c, LearnedState* s, const GV* gv, DevBuf<HV>& gl, DevBuf<HV>& gu) {
     FLGPU_TRY(lu_prepare(c, s));
+    if (s->sell) {
+        FLGPU_TRY(gl.alloc(s->nslotl));
+        FLGPU_TRY(gu.alloc(s->nslotu));
+        hipLaunchKernelGGL((k_sell_gather<GV, HV>), dim3(grid_for(s->nslotl, 256)), dim3(256), 0, c->stream,
+                           s->nslotl, s->lss.p, gv, gl.p);
+        hipLaunchKernelGGL((k_sell_gather<GV, HV>), dim3(grid_for(s->nslotu, 256)), dim3(256), 0, c->stream,
+                           s->nslotu, s->uss.p, gv, gu.p);
+        FLGPU_HIP(hipGetLastError());
+        return FLGPU_OK;
+    }
     FLGPU_TRY(gl.alloc(s->nnzl));
     FLGPU_TRY(gu.alloc(s->nnzu));
     hipLaunchKernelGGL((k_gather_half<GV, HV>), dim3(grid_for(s->nnzl, 256)), dim3(256), 0, c->stream, s->nnzl,
@@ -1080,9 +1126,40 @@ static void apply_launch(flgpu_ctx* c, LearnedState* s, const HV* gl, const HV* 
                        (const V*)y, z, (const double*)nullptr, r, w, partials, done);
 }
 
+// The same two products over the sliced ELL halves.
+template <typename V, typename HV, typename LC, typename UC>
+static void apply_launch_sell(flgpu_ctx* c, LearnedState* s, const HV* gl, const HV* gu, const LC* lcol,
+                              const UC* ucol, const V* r, const V* rt, V* z, V* y, const V* w, double* partials,
+                              const int* done) {
+    const int n = c->rows;
+    const SellOp<HV, LC> lo{s->nsll, n, s->lsp.p, s->lptr.p, lcol, gl, nullptr};
+    const SellOp<HV, UC> up{s->nslu, n, s->usp.p, s->uptr.p, ucol, gu, s->dsq.p};
+    hipLaunchKernelGGL((k_sell<V, HV, LC, V, V, DOT_NONE>), dim3(sell_grid(s->nsll)), dim3(kRowBlock), 0, c->stream,
+                       lo, rt, y, (const double*)nullptr, (const V*)nullptr, (const V*)nullptr, partials, done);
+    hipLaunchKernelGGL((k_sell<V, HV, UC, V, V, DOT_PC>), dim3(sell_grid(s->nslu)), dim3(kRowBlock), 0, c->stream,
+                       up, (const V*)y, z, (const double*)nullptr, r, w, partials, done);
+}
+
 template <typename V, typename HV>
 static int apply_impl(flgpu_ctx* c, LearnedState* s, const HV* gl, const HV* gu, const V* r, const V* rt, V* z,
                       V* y, const V* w, double* partials, int* nparts, const int* done) {
+    if (s->sell) {
+        *nparts = sell_grid(s->nslu);
+        if (s->ls16 && s->us16)
+            apply_launch_sell<V, HV, int16_t, int16_t>(c, s, gl, gu, s->lsc16.p, s->usc16.p, r, rt, z, y, w,
+                                                       partials, done);
+        else if (s->ls16)
+            apply_launch_sell<V, HV, int16_t, int32_t>(c, s, gl, gu, s->lsc16.p, s->usc32.p, r, rt, z, y, w,
+                                                       partials, done);
+        else if (s->us16)
+            apply_launch_sell<V, HV, int32_t, int16_t>(c, s, gl, gu, s->lsc32.p, s->usc16.p, r, rt, z, y, w,
+                                                       partials, done);
+        else
+            apply_launch_sell<V, HV, int32_t, int32_t>(c, s, gl, gu, s->lsc32.p, s->usc32.p, r, rt, z, y, w,
+                                                       partials, done);
+        FLGPU_HIP(hipGetLastError());
+        return FLGPU_OK;
+    }
     *nparts = stream_grid(s->nrbu);
     if (s->l16 && s->u16)
         apply_launch<V, HV, int16_t, int16_t>(c, s, gl, gu, s->lcol16.p, s->ucol16.p, r, rt, z, y, w, partials,

@@ -3,6 +3,7 @@
 // usual).  Built by tests/sanitize/Makefile, run by
 // tests/test_host_sanitizers.py on the CPU:
 //   - make_sparsity and colour_mesh on a jittered-free Kuhn cube mesh;
+//   - the sliced ELL plan of the solver's SpMV against its CSR source;
 //   - the matrix-dump writer and reader, then the reader on every
 //     truncation of a dump and on dumps with corrupted headers, counts,
 //     row lengths and column indices (each must fail cleanly or succeed);
@@ -20,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "../../fluidity_amd/csrc/krylov_common.h"
 #include "../../include/flgpu.h"
 #include "../../oracle/fl_oracle.h"
 
@@ -307,6 +309,68 @@ static void check_gpu(const Mesh& m, const std::vector<int32_t>& findrm, const s
     std::printf("gpu: blocked and colour assembly, solve, no-alloc solve, pressure path (%d its)\n", info.its);
 }
 
+// The sliced ELL plan (flgpu::sell_plan) against its CSR source, walked
+// with the GPU kernel's index arithmetic: every entry in exactly one slot,
+// at slot sptr[s] + 64 j + lane of its row, its column recovered from the
+// stored index, padding marked -1 with an in-range column; the product in
+// slot order equals the CSR product bit for bit.
+static void check_sell(const std::vector<int32_t>& findrm, const std::vector<int32_t>& colm) {
+    const int n = (int)findrm.size() - 1;
+    std::vector<int32_t> rp(n + 1), cl(colm.size()), vmap(colm.size());
+    for (int i = 0; i <= n; i++) rp[i] = findrm[i] - 1;
+    for (size_t k = 0; k < colm.size(); k++) {
+        cl[k] = colm[k] - 1;
+        vmap[k] = 3 * (int32_t)k + 1;
+    }
+    std::mt19937_64 g(5);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    std::vector<double> val(cl.size()), x(n), ycsr(n), ysell(n);
+    for (double& v : val) v = U(g);
+    for (double& v : x) v = U(g);
+    for (int i = 0; i < n; i++) {
+        double acc = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; k++) acc += val[k] * x[cl[k]];
+        ycsr[i] = acc;
+    }
+    for (int use_map = 0; use_map < 2; use_map++) {
+        flgpu::SellHost h;
+        flgpu::sell_plan(rp.data(), cl.data(), n, use_map ? vmap.data() : nullptr, h);
+        CHECK(h.nsl == (n + flgpu::kSlice - 1) / flgpu::kSlice, "sell slices");
+        CHECK(h.nslot == h.sptr[h.nsl] && (int64_t)h.src.size() == h.nslot, "sell slots");
+        std::vector<int> seen(cl.size(), 0);
+        for (int sl = 0; sl < h.nsl; sl++) {
+            const int r0 = sl * flgpu::kSlice, base = h.sptr[sl];
+            CHECK((h.sptr[sl + 1] - base) % flgpu::kSlice == 0, "sell slice width");
+            const int w = (h.sptr[sl + 1] - base) / flgpu::kSlice;
+            for (int lane = 0; lane < flgpu::kSlice; lane++) {
+                const int row = r0 + lane;
+                const int len = row < n ? rp[row + 1] - rp[row] : 0;
+                CHECK(len <= w, "sell row longer than its slice");
+                double acc = 0.0;
+                for (int j = 0; j < w; j++) {
+                    const int64_t slot = base + (int64_t)j * flgpu::kSlice + lane;
+                    const int cj = h.use16 ? r0 + h.c16[slot] : h.c32[slot];
+                    CHECK(cj >= 0 && cj < n, "sell column out of range");
+                    if (j < len) {
+                        const int k = rp[row] + j;
+                        CHECK(h.src[slot] == (use_map ? vmap[k] : k), "sell source entry");
+                        CHECK(cj == cl[k], "sell column");
+                        seen[k]++;
+                        acc += val[k] * x[cj];
+                    } else {
+                        CHECK(h.src[slot] == -1, "sell padding");
+                    }
+                }
+                if (row < n) ysell[row] = acc;
+            }
+        }
+        for (size_t k = 0; k < seen.size(); k++) CHECK(seen[k] == 1, "sell entry %zu in %d slots\n", k, seen[k]);
+        CHECK(std::memcmp(ycsr.data(), ysell.data(), sizeof(double) * n) == 0, "sell product differs from CSR");
+        std::printf("sell: %d slices, %lld slots for %zu entries (fill %.3f), int16 columns %d\n", h.nsl,
+                    (long long)h.nslot, cl.size(), h.fill((int64_t)cl.size()), h.use16 ? 1 : 0);
+    }
+}
+
 int main(int argc, char** argv) {
     const bool gpu = argc > 1 && std::strcmp(argv[1], "--gpu") == 0;
     const Mesh m = kuhn_cube(gpu ? 12 : 6);
@@ -329,6 +393,7 @@ int main(int argc, char** argv) {
               FLGPU_OK, "make_sparsity again");
     std::printf("host: %d nodes, %d elements, %lld entries, %d colours\n", m.nx, m.nele, (long long)nnz, ncol);
     check_matrixdump(findrm, colm);
+    check_sell(findrm, colm);
     check_oracle(m, findrm, colm, centrm);
     if (gpu) check_gpu(m, findrm, colm, centrm);
     std::printf("%s\n", failures ? "FAILED" : "host_check ok");
