@@ -252,10 +252,27 @@ __device__ __forceinline__ bool carry_if_done(const Scal* __restrict__ in, Scal*
     return true;
 }
 
+// R consecutive entries as one vector load / store (R * sizeof(T) bytes;
+// the caller checks the alignment with rows_aligned)
+template <typename T, int R>
+using Rows = T __attribute__((ext_vector_type(R)));
+
+template <int R, typename T>
+__device__ __forceinline__ Rows<T, R> ld_rows(const T* __restrict__ a, int i) {
+    return *reinterpret_cast<const Rows<T, R>*>(a + i);
+}
+template <int R, typename T>
+__device__ __forceinline__ void st_rows(T* __restrict__ a, int i, const Rows<T, R>& v) {
+    *reinterpret_cast<Rows<T, R>*>(a + i) = v;
+}
+
 // pAp from the SpMV partials, alpha = rz / pAp; x += alpha p;
 // r -= alpha w; [Jacobi: z = D^-1 r]; [learned: rt = D_s r, the residual in
-// the preconditioner's scaled space]; partials r.z, r.r, z.z, z.w
-template <typename V>
+// the preconditioner's scaled space]; partials r.z, r.r, z.z, z.w.  Each
+// thread takes R consecutive rows per step with vector loads (the rows past
+// the last whole group go one at a time), so several loads of every vector
+// are in flight per thread.
+template <typename V, int R>
 __global__ void __launch_bounds__(kVecBlock)
 k_cg_update(int n, double* __restrict__ x, const V* __restrict__ p, V* __restrict__ r,
             const V* __restrict__ w, V* __restrict__ z, const V* __restrict__ dinv, int jacobi,
@@ -286,28 +303,59 @@ k_cg_update(int n, double* __restrict__ x, const V* __restrict__ p, V* __restric
     const double alpha = s_alpha;
     const V a = (V)alpha;
     double s_rz = 0.0, s_rr = 0.0, s_zz = 0.0, s_zw = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const V pi = p[i], wi = w[i];
-        x[i] = fma(alpha, (double)pi, x[i]);
-        const V ri = r[i] - a * wi;
-        r[i] = ri;
-        if (dsc) rt[i] = ri * (V)dsc[i];
+    auto row = [&](double& xi, V pi, V wi, V& ri, V di, float si, V& zi, V& ti) {
+        xi = fma(alpha, (double)pi, xi);
+        ri = ri - a * wi;
+        if (dsc) ti = ri * (V)si;
         if (jacobi) {
-            const V zi = dinv[i] * ri;
-            z[i] = zi;
+            zi = di * ri;
             s_rz += (double)ri * (double)zi;
             s_zz += (double)zi * (double)zi;
             s_zw += (double)zi * (double)wi;
         }
         s_rr += (double)ri * (double)ri;
+    };
+    const int ng = n / R, tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+    for (int g = tid; g < ng; g += nt) {
+        const int i = g * R;
+        const Rows<V, R> pv = ld_rows<R>(p, i), wv = ld_rows<R>(w, i);
+        Rows<V, R> rv = ld_rows<R>(r, i), dv = {}, zv = {}, tv = {};
+        Rows<double, R> xv = ld_rows<R>((const double*)x, i);
+        Rows<float, R> sv = {};
+        if (jacobi) dv = ld_rows<R>(dinv, i);
+        if (dsc) sv = ld_rows<R>(dsc, i);
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+            double xi = xv[u];
+            V ri = rv[u], zi = (V)0, ti = (V)0;
+            row(xi, pv[u], wv[u], ri, dv[u], sv[u], zi, ti);
+            xv[u] = xi;
+            rv[u] = ri;
+            zv[u] = zi;
+            tv[u] = ti;
+        }
+        st_rows<R>(x, i, xv);
+        st_rows<R>(r, i, rv);
+        if (jacobi) st_rows<R>(z, i, zv);
+        if (dsc) st_rows<R>(rt, i, tv);
+    }
+    for (int i = ng * R + tid; i < n; i += nt) {
+        double xi = x[i];
+        V ri = r[i], zi = (V)0, ti = (V)0;
+        row(xi, p[i], w[i], ri, jacobi ? dinv[i] : (V)0, dsc ? dsc[i] : 0.f, zi, ti);
+        x[i] = xi;
+        r[i] = ri;
+        if (jacobi) z[i] = zi;
+        if (dsc) rt[i] = ti;
     }
     double v[4] = {s_rz, s_rr, s_zz, s_zw};
     block_partials<4>(v, pvec);
 }
 
 // r.z, r.r, z.z, z.w from the update partials (or r.z, z.z, z.w from the
-// learned apply), convergence test and beta in every block; p = z + beta p.
-template <typename V>
+// learned apply), convergence test and beta in every block; p = z + beta p
+// (R rows per thread and step, as k_cg_update).
+template <typename V, int R>
 __global__ void __launch_bounds__(kVecBlock)
 k_cg_p(int n, const V* __restrict__ z, V* __restrict__ p, const double* __restrict__ pvec, int nvec,
        const double* __restrict__ ppc, int npc, const Scal* __restrict__ in, Scal* __restrict__ out) {
@@ -337,8 +385,16 @@ k_cg_p(int n, const V* __restrict__ z, V* __restrict__ p, const double* __restri
     __syncthreads();
     if (s_stop) return;
     const V beta = (V)s_beta;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        p[i] = z[i] + beta * p[i];
+    const int ng = n / R, tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+    for (int g = tid; g < ng; g += nt) {
+        const int i = g * R;
+        const Rows<V, R> zv = ld_rows<R>(z, i);
+        Rows<V, R> pv = ld_rows<R>(p, i);
+#pragma unroll
+        for (int k = 0; k < R; k++) pv[k] = fma(beta, pv[k], zv[k]);
+        st_rows<R>(p, i, pv);
+    }
+    for (int i = ng * R + tid; i < n; i += nt) p[i] = fma(beta, p[i], z[i]);
 }
 
 template <typename V>
@@ -850,6 +906,13 @@ static int cg_restart(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, PcMode mo
     return FLGPU_OK;
 }
 
+// True when every given array (null ones aside) starts on a multiple of R
+// of its entries: the vector kernels then load R rows at once.
+template <int R, typename... P>
+static bool rows_aligned(const P*... ptrs) {
+    return ((ptrs == nullptr || reinterpret_cast<uintptr_t>(ptrs) % (R * sizeof(P)) == 0) && ...);
+}
+
 // One (F)PCG iteration: w = A p; update x, r, z; [z = M^-1 r]; test,
 // beta, p.  Four kernels (diagonal), six (learned) or 2C + 3 (SSOR over C
 // colours), no host round trip.
@@ -859,15 +922,44 @@ static int cg_iteration(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, PcMode 
     FLGPU_TRY((spmv<V, V, V>(c, k, v.val, v.p, v.w, nullptr, 1, &k->S()->done)));
     Scal* in = k->S();
     k->cur ^= 1;
-    hipLaunchKernelGGL((k_cg_update<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->x.p, v.p, v.r,
-                       v.w, v.z, v.dinv, diag ? 1 : 0, mode == PCM_LEARNED ? learned_scale(c) : nullptr, v.rt,
-                       k->part(PART_SPMV), spmv_grid(c), k->part(PART_VEC), in, k->S());
+        const int grid = sell_grid(k->nsl);
+        const V* sval = sell_values(k, v.val);
+        const double* ppc = diag ? nullptr : k->part(PART_PC);
+        if (k->sell16) {
+            const SellOp<V, int16_t> m{k->nsl, c->rows, k->sptr.p, c->rowptr.p, k->scol16.p, sval, nullptr};
+            hipLaunchKernelGGL((k_sell_cg<V, int16_t>), dim3(grid), dim3(kRowBlock), 0, c->stream, m, v.z, v.p,
+                               v.palt, v.w, k->part(PART_SPMV), k->part(PART_VEC), k->nblk, ppc, k->npc, in, k->S());
+        } else {
+            const SellOp<V, int32_t> m{k->nsl, c->rows, k->sptr.p, c->rowptr.p, k->scol32.p, sval, nullptr};
+            hipLaunchKernelGGL((k_sell_cg<V, int32_t>), dim3(grid), dim3(kRowBlock), 0, c->stream, m, v.z, v.p,
+                               v.palt, v.w, k->part(PART_SPMV), k->part(PART_VEC), k->nblk, ppc, k->npc, in, k->S());
+        }
+        std::swap(v.p, v.palt);
+        in = k->S();
+    } else {
+        FLGPU_TRY((spmv<V, V, V>(c, k, v.val, v.p, v.w, nullptr, 1, &k->S()->done)));
+    }
+    k->cur ^= 1;
+    const float* dsc = mode == PCM_LEARNED ? learned_scale(c) : nullptr;
+    constexpr int R = 16 / sizeof(V);
+    if (rows_aligned<R>(k->x.p, v.p, v.r, v.w, v.z, v.dinv, dsc, v.rt))
+        hipLaunchKernelGGL((k_cg_update<V, R>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->x.p, v.p,
+                           v.r, v.w, v.z, v.dinv, diag ? 1 : 0, dsc, v.rt, k->part(PART_SPMV), spmv_grid(c),
+                           k->part(PART_VEC), in, k->S());
+    else
+        hipLaunchKernelGGL((k_cg_update<V, 1>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->x.p, v.p,
+                           v.r, v.w, v.z, v.dinv, diag ? 1 : 0, dsc, v.rt, k->part(PART_SPMV), spmv_grid(c),
+                           k->part(PART_VEC), in, k->S());
     int npc = 0;
     if (!diag) FLGPU_TRY(pc_apply<V>(c, k, v, mode, &k->S()->done, &npc));
     in = k->S();
     k->cur ^= 1;
-    hipLaunchKernelGGL((k_cg_p<V>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, v.z, v.p,
-                       k->part(PART_VEC), k->nblk, diag ? nullptr : k->part(PART_PC), npc, in, k->S());
+    if (rows_aligned<R>(v.z, v.p))
+        hipLaunchKernelGGL((k_cg_p<V, R>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, v.z, v.p,
+                           k->part(PART_VEC), k->nblk, diag ? nullptr : k->part(PART_PC), npc, in, k->S());
+    else
+        hipLaunchKernelGGL((k_cg_p<V, 1>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, v.z, v.p,
+                           k->part(PART_VEC), k->nblk, diag ? nullptr : k->part(PART_PC), npc, in, k->S());
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }

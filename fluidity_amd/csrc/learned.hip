@@ -356,17 +356,30 @@ k_edge_agg(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict_
     const float c1 = lp[Lay::C1 + c], c2 = lp[Lay::C2 + c], bm = lp[Lay::BM + c];
     const int kb = rowptr[i], ke = rowptr[i + 1];
     float m = 0.f;
+    constexpr int U = 8;
     for (int k0 = kb; k0 < ke; k0 += 32) {
         const int len = min(32, ke - k0);
         const int myj = c < len ? col[k0 + c] : i;
         const float mys = c < len ? sval[k0 + c] : 0.f;
-#pragma unroll 4
-        for (int t = 0; t < len; t++) {
-            const int j = __shfl(myj, t, 32);
-            const float sv = __shfl(mys, t, 32);
-            if (j == i) continue;
-            const float q = PQ[(size_t)j * 2 * H + H + c];
-            m += fmaxf(pi + q + c1 * sv + c2 * fabsf(sv) + bm, 0.f);
+        // U gathers of Q_j in flight per step: the loads are unconditional
+        // (a step past the row's end re-reads its last edge) and only the
+        // additions are predicated, in entry order
+        for (int t0 = 0; t0 < len; t0 += U) {
+            int jj[U];
+            float sv[U], q[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = min(t0 + u, len - 1);
+                jj[u] = __shfl(myj, t, 32);
+                sv[u] = __shfl(mys, t, 32);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) q[u] = PQ[(size_t)jj[u] * 2 * H + H + c];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const float e = fmaxf(pi + q[u] + c1 * sv[u] + c2 * fabsf(sv[u]) + bm, 0.f);
+                if (t0 + u < len && jj[u] != i) m += e;
+            }
         }
     }
     M[(size_t)i * H + c] = m;
