@@ -922,24 +922,6 @@ static int cg_iteration(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, PcMode 
     FLGPU_TRY((spmv<V, V, V>(c, k, v.val, v.p, v.w, nullptr, 1, &k->S()->done)));
     Scal* in = k->S();
     k->cur ^= 1;
-        const int grid = sell_grid(k->nsl);
-        const V* sval = sell_values(k, v.val);
-        const double* ppc = diag ? nullptr : k->part(PART_PC);
-        if (k->sell16) {
-            const SellOp<V, int16_t> m{k->nsl, c->rows, k->sptr.p, c->rowptr.p, k->scol16.p, sval, nullptr};
-            hipLaunchKernelGGL((k_sell_cg<V, int16_t>), dim3(grid), dim3(kRowBlock), 0, c->stream, m, v.z, v.p,
-                               v.palt, v.w, k->part(PART_SPMV), k->part(PART_VEC), k->nblk, ppc, k->npc, in, k->S());
-        } else {
-            const SellOp<V, int32_t> m{k->nsl, c->rows, k->sptr.p, c->rowptr.p, k->scol32.p, sval, nullptr};
-            hipLaunchKernelGGL((k_sell_cg<V, int32_t>), dim3(grid), dim3(kRowBlock), 0, c->stream, m, v.z, v.p,
-                               v.palt, v.w, k->part(PART_SPMV), k->part(PART_VEC), k->nblk, ppc, k->npc, in, k->S());
-        }
-        std::swap(v.p, v.palt);
-        in = k->S();
-    } else {
-        FLGPU_TRY((spmv<V, V, V>(c, k, v.val, v.p, v.w, nullptr, 1, &k->S()->done)));
-    }
-    k->cur ^= 1;
     const float* dsc = mode == PCM_LEARNED ? learned_scale(c) : nullptr;
     constexpr int R = 16 / sizeof(V);
     if (rows_aligned<R>(k->x.p, v.p, v.r, v.w, v.z, v.dinv, dsc, v.rt))
