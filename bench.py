@@ -470,6 +470,10 @@ def main() -> int:
     B = asm_bytes(cfg, nnz)
     achieved = B / t_asm / 1e9
     log(f"assembly: {ms.value:.3f} ms/step, {melems:.1f} Melem/s, {achieved:.1f} GB/s algorithmic")
+    plan = cg.assembly_plan_info(pr)
+    plan["node_stagings_per_node"] = round(plan["node_visits"] / cfg.mesh.nnodes, 3)
+    log(f"assembly plan: {plan['slabs']} slabs, {plan['node_stagings_per_node']} stagings per node, "
+        f"{plan['lds_bytes_fp64']} B LDS ({plan['workgroups_per_cu']} per CU), built in {plan['build_seconds']} s")
     L.check(lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(p)), "assemble")
     A_blocked, rhs_blocked, _ = pr.matrix()
     schemes = {}
@@ -520,6 +524,7 @@ def main() -> int:
     }
     A32 = schemes.get("blocked_fp32_elements", {}).pop("A32", None)
     rhs32 = schemes.get("blocked_fp32_elements", {}).pop("rhs32", None)
+    out["assembly_plan"] = plan
     if schemes:
         out["assembly_other_schemes"] = schemes
 

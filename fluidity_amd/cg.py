@@ -429,6 +429,24 @@ class PressureMesh:
     ct_colm: np.ndarray
 
 
+PLAN_INFO_KEYS = ("staged", "slabs", "max_rows", "max_entries", "max_slots", "max_nodes", "node_visits",
+                  "element_rows", "lds_bytes_fp64", "workgroups_per_cu")
+
+
+def assembly_plan_info(problem) -> dict:
+    """The blocked scheme's plan for the bound mesh (flgpu_assembly_plan_info):
+    slabs, their largest extents, staged nodes in all (node_visits /
+    nodes = how often a node is staged), the LDS of a slab and the host
+    seconds of the plan build."""
+    info = np.zeros(len(PLAN_INFO_KEYS), np.int64)
+    secs = C.c_double(0)
+    L.check(problem.lib.flgpu_assembly_plan_info(problem.ctx, L.ptr(info), len(PLAN_INFO_KEYS), C.byref(secs)),
+            "flgpu_assembly_plan_info")
+    out = {k: int(v) for k, v in zip(PLAN_INFO_KEYS, info)}
+    out["build_seconds"] = round(secs.value, 4)
+    return out
+
+
 def vertex_pressure_mesh(problem: CGProblem) -> PressureMesh:
     """The P1 pressure mesh on the vertices of the problem's mesh (P1-P1,
     or Taylor-Hood P2-P1 for a quadratic velocity), with ct_m's sparsity:

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -1382,7 +1383,9 @@ int build_block_plan(flgpu_ctx* c) {
     BlockPlan& P = c->plan;
     if (P.built) return FLGPU_OK;
     if (P.fail_rc) return set_error(P.fail_rc, "%s", P.fail_msg.c_str());
+    const auto t0 = std::chrono::steady_clock::now();
     const int rc = build_block_plan_impl(c);
+    P.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (rc != FLGPU_OK) {
         P.fail_rc = rc;
         P.fail_msg = flgpu_last_error();
@@ -1604,6 +1607,7 @@ static int build_block_plan_impl(flgpu_ctx* c) {
         FLGPU_TRY(P.ecp.upload(ecp.data(), ecp.size(), c->stream));
         FLGPU_TRY(P.rdg.upload(rdg.data(), rdg.size(), c->stream));
         P.node_cap = max_nodes;
+        P.node_visits = (int64_t)nodes.size();
     }
     FLGPU_TRY(P.brow.upload(brow.data(), brow.size(), c->stream));
     FLGPU_TRY(P.iptr.upload(iptr.data(), iptr.size(), c->stream));
@@ -1927,6 +1931,30 @@ int assemble(flgpu_ctx* c, const flgpu_advdif_params* p) {
     FLGPU_HIP(hipMemsetAsync(d_err.p, 0, sizeof(int), c->stream));
     FLGPU_TRY(assemble_launch(c, p, d_err.p));
     return assemble_check(c, d_err.p);
+}
+
+int assembly_plan_info(flgpu_ctx* c, int64_t* info, int n, double* build_seconds) {
+    if (c->nele <= 0 || c->nnz <= 0 || c->ncol <= 0)
+        return set_error(FLGPU_EINVAL, "flgpu_assembly_plan_info: bind mesh, sparsity and colouring first");
+    FLGPU_TRY(build_block_plan(c));
+    const BlockPlan& P = c->plan;
+    StagedPlan sp{};
+    sp.rows_cap = P.rows_cap;
+    sp.nnz_cap = P.nnz_cap;
+    sp.node_cap = P.node_cap;
+    sp.stage_k = sp.stage_s = 1;
+    sp.kcomp = 1;
+    size_t lds = 0;
+    if (P.staged) {
+        const size_t head = (sizeof(double) * (size_t)P.slot_cap * (c->loc + 1) + 15) & ~(size_t)15;
+        lds = head + (c->dim == 3 ? slab_lds_bytes<3, double>(sp) : slab_lds_bytes<2, double>(sp)) +
+              sizeof(LTabLin<double>);
+    }
+    const int64_t v[10] = {P.staged ? 1 : 0, P.nblocks, P.rows_cap, P.nnz_cap, P.slot_cap, P.node_cap, P.node_visits,
+                           (int64_t)c->nele * c->loc, (int64_t)lds, lds ? (int64_t)(160 * 1024 / lds) : 0};
+    for (int q = 0; q < n && q < 10; q++) info[q] = v[q];
+    if (build_seconds) *build_seconds = P.build_seconds;
+    return FLGPU_OK;
 }
 
 int time_assemble(flgpu_ctx* c, const flgpu_advdif_params* p, int reps, double* ms) {

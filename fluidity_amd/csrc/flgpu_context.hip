@@ -678,6 +678,14 @@ int flgpu_time_assemble(flgpu_ctx* c, const flgpu_advdif_params* p, int32_t reps
     });
 }
 
+int flgpu_assembly_plan_info(flgpu_ctx* c, int64_t* info, int32_t n, double* build_seconds) {
+    return guarded("flgpu_assembly_plan_info", [&]() -> int {
+        if (!c || (!info && n > 0)) return set_error(FLGPU_EINVAL, "flgpu_assembly_plan_info: null argument");
+        FLGPU_HIP(hipSetDevice(c->device));
+        return assembly_plan_info(c, info, n, build_seconds);
+    });
+}
+
 int flgpu_time_spmv(flgpu_ctx* c, int32_t precision, int32_t reps, double* ms) {
     return guarded("flgpu_time_spmv", [&]() -> int {
         if (!c || !ms || reps < 1) return set_error(FLGPU_EINVAL, "flgpu_time_spmv: bad argument");

@@ -147,6 +147,8 @@ struct BlockPlan {
     DevBuf<uint16_t> ecp;        // per CSR entry: slab-relative offset of its run of values
     DevBuf<uint16_t> rdg;        // per row: row-relative offset of its diagonal run
     int32_t node_cap = 0;
+    int64_t node_visits = 0;     // staged nodes summed over the slabs
+    double build_seconds = 0.0;  // host time of the last plan build
     // per element, its LOC^2 entries' CSR positions (-1: not in the
     // pattern), for the quadratic-T colour kernel; the element tables as
     // the kernel keeps them in LDS (FP64 and FP32 images)
@@ -257,6 +259,7 @@ namespace flgpu {
 int assemble(flgpu_ctx* c, const flgpu_advdif_params* p);
 int build_block_plan(flgpu_ctx* c);
 int time_assemble(flgpu_ctx* c, const flgpu_advdif_params* p, int reps, double* ms);
+int assembly_plan_info(flgpu_ctx* c, int64_t* info, int n, double* build_seconds);
 // krylov.hip
 int solve(flgpu_ctx* c, const flgpu_solver_params* sp, double* x_host,
           flgpu_solve_info* info);

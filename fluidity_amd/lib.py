@@ -37,7 +37,7 @@ EXPORTS = (
     "flgpu_learned_param_count",
     "flgpu_learned_set_params", "flgpu_learned_get_params", "flgpu_learned_init_params",
     "flgpu_learned_train_step", "flgpu_learned_loss_grad", "flgpu_learned_factor",
-    "flgpu_time_assemble", "flgpu_time_spmv", "flgpu_time_spmv_cold", "flgpu_solver_prepare", "flgpu_sizes",
+    "flgpu_time_assemble", "flgpu_assembly_plan_info", "flgpu_time_spmv", "flgpu_time_spmv_cold", "flgpu_solver_prepare", "flgpu_sizes",
 )
 
 
@@ -134,6 +134,7 @@ def load() -> C.CDLL:
         "flgpu_learned_loss_grad": [P, I32, C.c_uint64, C.POINTER(C.c_float), P],
         "flgpu_learned_factor": [P, P, C.POINTER(C.c_int64)],
         "flgpu_time_assemble": [P, C.POINTER(AdvdifParams), I32, C.POINTER(C.c_double)],
+        "flgpu_assembly_plan_info": [P, P, I32, C.POINTER(C.c_double)],
         "flgpu_time_spmv": [P, I32, I32, C.POINTER(C.c_double)],
         "flgpu_time_spmv_cold": [P, I32, I32, I32, C.POINTER(C.c_double)],
         "flgpu_solver_prepare": [P, I32],

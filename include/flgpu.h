@@ -474,6 +474,15 @@ int flgpu_learned_factor(flgpu_ctx *ctx, float *G_val, int64_t *nnz);
  * call (the element loop only: zeroing included, host checks excluded). */
 int flgpu_time_assemble(flgpu_ctx *ctx, const flgpu_advdif_params *p,
                         int32_t reps, double *ms);
+/* The blocked scheme's plan for the bound mesh (built here if it is not
+ * yet): info[0..n-1] of staged (1: node-staged slabs), slabs, the largest
+ * slab's rows, entries, slots and staged nodes, staged nodes summed over
+ * the slabs, element rows (slots) in all, the LDS bytes of a slab for the
+ * FP64 kernel with diffusivity and source staged, and the workgroups per
+ * CU that LDS allows; *build_seconds (may be NULL) = host seconds of the
+ * last plan build. */
+int flgpu_assembly_plan_info(flgpu_ctx *ctx, int64_t *info, int32_t n,
+                             double *build_seconds);
 /* reps solver-matrix SpMVs (FLGPU_FP64 or FLGPU_MIXED values) after a solve
  * or flgpu_solver_prepare; *ms = mean device milliseconds per product. */
 int flgpu_time_spmv(flgpu_ctx *ctx, int32_t precision, int32_t reps,
