@@ -413,7 +413,7 @@ template <typename GV>
 __global__ void __launch_bounds__(256)
 k_build_g(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
           const int32_t* __restrict__ tmap, const float* __restrict__ sval, const float* __restrict__ dsq,
-          const float* __restrict__ ab, const float* __restrict__ th, GV* __restrict__ gv, int scaled) {
+          const float* __restrict__ ab, const float* __restrict__ th, GV* __restrict__ gv, int scaled, int upper) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = t >> 5, lane = t & 31;  // half a wave per row: coalesced entries
     if (i >= n) return;
@@ -426,6 +426,7 @@ k_build_g(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__
         } else if (j == i) {
             g = expf(ab[3 * (size_t)i + 2]) * di;
         } else {
+            if (!upper) continue;
             const int kt = tmap[k];
             const float s = kt >= 0 ? sval[kt] : 0.f;
             g = di * gs_lower(s, ab[3 * (size_t)j], ab[3 * (size_t)i + 1], th);
@@ -443,7 +444,20 @@ k_build_g(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__
 // Gs[i, P_i] = L^-T e_last, so that (Gs S Gs^T)_ii = 1 and
 // (Gs S)_ij = 0 for the other j in P_i (Kolotilina & Yeremin 1993).
 // ---------------------------------------------------------------------------
-constexpr int kFsaiMax = 48;  // largest lower pattern solved; longer rows keep the Jacobi row
+constexpr int kFsaiMax = 48;   // largest lower pattern solved; longer rows keep the Jacobi row
+constexpr int kFsaiShort = 8;  // rows up to this long are solved in registers (P1 tetrahedra: all)
+
+// Rows whose lower pattern has at most K entries, the local matrix in
+// registers: every index below is a compile-time constant after
+// unrolling.  The pattern is padded to K with the identity (P[a] = i for
+// a >= k): the padded Cholesky factor is L extended by an identity block
+// and the padded solve gives g = 0 there, so no step needs the row's own
+// length.  Longer rows are listed for k_fsai_lower.
+template <typename GV, int K>
+__global__ void __launch_bounds__(256)
+k_fsai_short(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+             const int32_t* __restrict__ diag, const double* __restrict__ val, const float* __restrict__ dsq,
+             GV* __restrict__ gv, int32_t* __restrict__ long_rows, int32_t* __restrict__ nlong);
 
 template <typename GV>
 __device__ inline void fsai_jacobi_row(int k0, int kd, GV* gv) {
@@ -451,30 +465,27 @@ __device__ inline void fsai_jacobi_row(int k0, int kd, GV* gv) {
     gv[kd] = (GV)1;  // S_ii = 1
 }
 
-// One thread per row, NT threads per block.  The first launch (LONG false)
-// takes every row whose lower pattern has k <= KMAX entries (all rows of a
-// P1 mesh) and lists the longer ones in long_rows / *nlong; the second
-// (LONG true) takes the listed rows (the long rows of a P2 mesh) with a
-// larger KMAX.  The local matrix S[P_i, P_i] sits in the block's LDS,
-// entry-major so that a block's threads touch consecutive words; S is
-// formed in FP64 from the reduced FP64 values and D.  Writes Gs at the
-// row's lower positions of A's pattern, the diagonal included.
-template <typename GV, int KMAX, int NT, bool LONG>
+// The listed long rows (long_rows[0 .. n-1], at most KMAX lower entries;
+// longer ones keep the Jacobi row), one thread per row, NT threads per
+// block: the local matrix S[P_i, P_i] in the block's LDS, entry-major so
+// that a block's threads touch consecutive words, formed in FP64 from the
+// reduced FP64 values and D.  Writes Gs at the row's lower positions of
+// A's pattern, the diagonal included.
+template <typename GV, int KMAX, int NT>
 __global__ void __launch_bounds__(NT)
 k_fsai_lower(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
              const int32_t* __restrict__ diag, const double* __restrict__ val, const float* __restrict__ dsq,
-             GV* __restrict__ gv, int32_t* __restrict__ long_rows, int32_t* __restrict__ nlong) {
+             GV* __restrict__ gv, const int32_t* __restrict__ long_rows) {
     __shared__ double sl[KMAX * (KMAX + 1) / 2][NT];
     __shared__ double sg[KMAX][NT];
     const int idx = blockIdx.x * NT + threadIdx.x;
     if (idx >= n) return;
-    const int i = LONG ? long_rows[idx] : idx;
+    const int i = long_rows[idx];
     const int t = threadIdx.x;
     const int k0 = rowptr[i], kd = diag[i];
     const int k = kd - k0 + 1;  // lower pattern size, the diagonal last (columns sorted)
     if (k > KMAX) {
-        if (LONG) fsai_jacobi_row(k0, kd, gv);
-        else long_rows[atomicAdd(nlong, 1)] = i;
+        fsai_jacobi_row(k0, kd, gv);
         return;
     }
 #define B(a, b) sl[(a) * ((a) + 1) / 2 + (b)][t]
@@ -516,19 +527,94 @@ k_fsai_lower(int n, const int32_t* __restrict__ rowptr, const int32_t* __restric
     for (int a = 0; a < k; a++) gv[k0 + a] = (GV)sg[a][t];
 }
 
-// The upper positions of A's pattern take the transposed lower values.
-template <typename GV>
+template <typename GV, int K>
 __global__ void __launch_bounds__(256)
-k_fsai_upper(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-             const int32_t* __restrict__ tmap, GV* __restrict__ gv) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = t >> 5, lane = t & 31;
+k_fsai_short(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+             const int32_t* __restrict__ diag, const double* __restrict__ val, const float* __restrict__ dsq,
+             GV* __restrict__ gv, int32_t* __restrict__ long_rows, int32_t* __restrict__ nlong) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    for (int k = rowptr[i] + lane; k < rowptr[i + 1]; k += 32)
-        if (col[k] > i) {
-            const int kt = tmap[k];
-            gv[k] = kt >= 0 ? gv[kt] : (GV)0;
+    const int k0 = rowptr[i], kd = diag[i];
+    const int k = kd - k0 + 1;
+    if (k > K) {
+        long_rows[atomicAdd(nlong, 1)] = i;
+        return;
+    }
+    int P[K];
+    double dp[K];
+#pragma unroll
+    for (int a = 0; a < K; a++) {
+        P[a] = a < k ? col[k0 + a] : i;
+        dp[a] = (double)dsq[P[a]];
+    }
+    // B(a, b) = S[P_a, P_b] (b <= a): entry P_b of row P_a lies in that
+    // row's lower part; a lower part of at most K entries is loaded whole
+    // (its loads all in flight) and matched against P by comparisons, a
+    // longer one searched entry by entry
+    double B[K * (K + 1) / 2];
+#pragma unroll
+    for (int a = 0; a < K; a++) {
+        const int q0 = rowptr[P[a]], ql = diag[P[a]] + 1;
+        if (a < k && ql - q0 <= K) {
+            int lc[K];
+            double lv[K];
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                lc[q] = q0 + q < ql ? col[q0 + q] : -1;
+                lv[q] = q0 + q < ql ? val[q0 + q] : 0.0;
+            }
+#pragma unroll
+            for (int b = 0; b <= a; b++) {
+                double v = 0.0;
+#pragma unroll
+                for (int q = 0; q < K; q++) v = lc[q] == P[b] ? lv[q] : v;
+                B[a * (a + 1) / 2 + b] = v * dp[a] * dp[b];
+            }
+        } else {
+            int q = q0;
+#pragma unroll
+            for (int b = 0; b <= a; b++) {
+                double v = (a == b) ? 1.0 : 0.0;  // the identity padding
+                if (a < k) {
+                    while (q < ql && col[q] < P[b]) q++;
+                    v = (q < ql && col[q] == P[b]) ? val[q] * dp[a] * dp[b] : 0.0;
+                }
+                B[a * (a + 1) / 2 + b] = v;
+            }
         }
+    }
+    bool pd = true;
+#pragma unroll
+    for (int a = 0; a < K; a++) {
+        double d = B[a * (a + 1) / 2 + a];
+#pragma unroll
+        for (int c = 0; c < a; c++) d -= B[a * (a + 1) / 2 + c] * B[a * (a + 1) / 2 + c];
+        pd = pd && d > 0.0;
+        const double l = sqrt(d > 0.0 ? d : 1.0);
+        B[a * (a + 1) / 2 + a] = l;
+#pragma unroll
+        for (int r = a + 1; r < K; r++) {
+            double v = B[r * (r + 1) / 2 + a];
+#pragma unroll
+            for (int c = 0; c < a; c++) v -= B[r * (r + 1) / 2 + c] * B[a * (a + 1) / 2 + c];
+            B[r * (r + 1) / 2 + a] = v / l;
+        }
+    }
+    if (!pd) {  // not positive definite: the Jacobi row
+        fsai_jacobi_row(k0, kd, gv);
+        return;
+    }
+    double g[K];
+#pragma unroll
+    for (int a = K - 1; a >= 0; a--) {
+        double v = a == k - 1 ? 1.0 : 0.0;
+#pragma unroll
+        for (int r = a + 1; r < K; r++) v -= B[r * (r + 1) / 2 + a] * g[r];
+        g[a] = v / B[a * (a + 1) / 2 + a];
+    }
+#pragma unroll
+    for (int a = 0; a < K; a++)
+        if (a < k) gv[k0 + a] = (GV)g[a];
 }
 
 // gathers the factor values of one half from A's pattern
@@ -536,7 +622,10 @@ template <typename GV, typename HV>
 __global__ void k_gather_half(int64_t m, const int32_t* __restrict__ src, const GV* __restrict__ gv,
                               HV* __restrict__ out) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < m) out[t] = (HV)gv[src[t]];
+    if (t < m) {
+        const int32_t k = src[t];
+        out[t] = k >= 0 ? (HV)gv[k] : (HV)0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -891,7 +980,7 @@ static int ensure_params(flgpu_ctx* c, LearnedState* s) {
 
 // The graph of the reduced system: d, s, features, tmap.  val64 is the
 // reduced FP64 matrix built by the Krylov setup.
-static int graph_prepare(flgpu_ctx* c, LearnedState* s, const double* val64) {
+static int graph_prepare(flgpu_ctx* c, LearnedState* s, const double* val64, bool features = true) {
     const int n = c->rows;
     const unsigned g = grid_for(n, 256);
     if (s->graph_key != c->sparsity_gen || s->n != n || s->nnz != c->nnz) {
@@ -907,8 +996,9 @@ static int graph_prepare(flgpu_ctx* c, LearnedState* s, const double* val64) {
     FLGPU_TRY(s->feat.alloc((size_t)n * NFEAT));
     hipLaunchKernelGGL((k_graph_prep<double>), dim3(g), dim3(256), 0, c->stream, n, c->rowptr.p, c->col.p, c->diag.p,
                        val64, s->dsq.p, s->sval.p, s->feat.p, 0);
-    hipLaunchKernelGGL((k_graph_prep_rows<double>), dim3(grid_for((int64_t)n * 32, 256)), dim3(256), 0, c->stream,
-                       n, c->rowptr.p, c->col.p, val64, s->dsq.p, s->sval.p, s->feat.p);
+    if (features)  // the scaled values and node features (the network's inputs)
+        hipLaunchKernelGGL((k_graph_prep_rows<double>), dim3(grid_for((int64_t)n * 32, 256)), dim3(256), 0,
+                           c->stream, n, c->rowptr.p, c->col.p, val64, s->dsq.p, s->sval.p, s->feat.p);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
@@ -955,11 +1045,13 @@ static int forward(flgpu_ctx* c, LearnedState* s, bool keep_all) {
     return FLGPU_OK;
 }
 
+// upper = 0: the lower pattern and the diagonal only (what the solver's
+// halves read: the upper half gathers the transposed lower values)
 template <typename GV>
-static int build_g(flgpu_ctx* c, LearnedState* s, GV* out, int scaled = 0) {
+static int build_g(flgpu_ctx* c, LearnedState* s, GV* out, int scaled = 0, int upper = 1) {
     const int n = c->rows;
     hipLaunchKernelGGL((k_build_g<GV>), dim3(grid_for((int64_t)n * 32, 256)), dim3(256), 0, c->stream, n, c->rowptr.p, c->col.p,
-                       s->tmap.p, s->sval.p, s->dsq.p, s->ab.p, s->theta.p, out, scaled);
+                       s->tmap.p, s->sval.p, s->dsq.p, s->ab.p, s->theta.p, out, scaled, upper);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
@@ -978,17 +1070,15 @@ static int build_fsai(flgpu_ctx* c, LearnedState* s, GV* out) {
     FLGPU_TRY(s->fsai_rows.alloc((size_t)n + 1));
     int32_t* cnt = s->fsai_rows.p + n;
     FLGPU_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t), c->stream));
-    hipLaunchKernelGGL((k_fsai_lower<GV, 16, 64, false>), dim3(grid_for(n, 64)), dim3(64), 0, c->stream, n,
+    hipLaunchKernelGGL((k_fsai_short<GV, kFsaiShort>), dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n,
                        c->rowptr.p, c->col.p, c->diag.p, val, s->dsq.p, out, s->fsai_rows.p, cnt);
     FLGPU_HIP(hipGetLastError());
     int32_t nlong = 0;
     FLGPU_HIP(hipMemcpyAsync(&nlong, cnt, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     FLGPU_HIP(hipStreamSynchronize(c->stream));
     if (nlong > 0)
-        hipLaunchKernelGGL((k_fsai_lower<GV, kFsaiMax, 8, true>), dim3(grid_for(nlong, 8)), dim3(8), 0, c->stream,
-                           nlong, c->rowptr.p, c->col.p, c->diag.p, val, s->dsq.p, out, s->fsai_rows.p, cnt);
-    hipLaunchKernelGGL((k_fsai_upper<GV>), dim3(grid_for((int64_t)n * 32, 256)), dim3(256), 0, c->stream, n,
-                       c->rowptr.p, c->col.p, s->tmap.p, out);
+        hipLaunchKernelGGL((k_fsai_lower<GV, kFsaiMax, 8>), dim3(grid_for(nlong, 8)), dim3(8), 0, c->stream, nlong,
+                           c->rowptr.p, c->col.p, c->diag.p, val, s->dsq.p, out, (const int32_t*)s->fsai_rows.p);
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
 }
@@ -997,15 +1087,15 @@ int learned_prepare(flgpu_ctx* c, bool mixed, bool fsai) {
     if (!c->all_diag) return set_error(FLGPU_EUNSUPPORTED, "learned preconditioner: a row has no diagonal entry");
     LearnedState* s = state(c);
     if (!fsai) FLGPU_TRY(ensure_params(c, s));
-    FLGPU_TRY(graph_prepare(c, s, reduced_values(c)));
+    FLGPU_TRY(graph_prepare(c, s, reduced_values(c), !fsai));
     if (!fsai) FLGPU_TRY(forward(c, s, false));
     if (mixed) {
         FLGPU_TRY(s->gv.alloc(c->nnz));
-        FLGPU_TRY(fsai ? build_fsai<float>(c, s, s->gv.p) : build_g<float>(c, s, s->gv.p, 1));
+        FLGPU_TRY(fsai ? build_fsai<float>(c, s, s->gv.p) : build_g<float>(c, s, s->gv.p, 1, 0));
         FLGPU_TRY((split_halves<float, _Float16>(c, s, s->gv.p, s->gl16, s->gu16)));
     } else {
         FLGPU_TRY(s->gv64.alloc(c->nnz));
-        FLGPU_TRY(fsai ? build_fsai<double>(c, s, s->gv64.p) : build_g<double>(c, s, s->gv64.p, 1));
+        FLGPU_TRY(fsai ? build_fsai<double>(c, s, s->gv64.p) : build_g<double>(c, s, s->gv64.p, 1, 0));
         FLGPU_TRY((split_halves<double, double>(c, s, s->gv64.p, s->gl64, s->gu64)));
     }
     FLGPU_HIP(hipStreamSynchronize(c->stream));
@@ -1036,8 +1126,18 @@ static int lu_prepare(flgpu_ctx* c, LearnedState* s) {
                 ls.push_back(k);
             }
             if (cl[k] >= i) {
+                // the upper half takes each value from its transposed
+                // lower position (j, i) of row j, so the factor is only
+                // ever built on the lower pattern (-1: not in the pattern)
+                int src = k;
+                if (cl[k] > i) {
+                    const int32_t* b0 = cl + rp[cl[k]];
+                    const int32_t* b1 = cl + rp[cl[k] + 1];
+                    const int32_t* hit = std::lower_bound(b0, b1, i);
+                    src = (hit != b1 && *hit == i) ? (int)(hit - cl) : -1;
+                }
                 uc.push_back(cl[k]);
-                us.push_back(k);
+                us.push_back(src);
             }
         }
     }

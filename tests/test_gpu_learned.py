@@ -57,6 +57,34 @@ def test_forward_factor_matches_oracle(require_gpu):
     pr.close()
 
 
+def test_pc_apply_matches_exported_factor(require_gpu):
+    """flgpu_pc_apply with the learned preconditioner (the solver's FP64
+    halves, the upper one gathered from the transposed lower values) equals
+    G^T (G r) formed with the exported factor in A's pattern (FP32
+    values, hence 1e-5)."""
+    cfg, pr, g = system(scale=8)
+    learned.set_params(pr, perturbed_params())
+    gv = learned.factor(pr).astype(np.float64)
+    f, c = pr.sparsity.findrm - 1, pr.sparsity.colm - 1
+    n = f.size - 1
+    row = np.repeat(np.arange(n), np.diff(f))
+    low = c <= row
+    r = np.random.default_rng(5).standard_normal(n)
+    y = np.zeros(n)
+    np.add.at(y, row[low], gv[low] * r[c[low]])
+    z_ref = np.zeros(n)
+    np.add.at(z_ref, c[low], gv[low] * y[row[low]])
+    z = np.zeros(n)
+    import ctypes as C
+    from fluidity_amd import lib as L
+    L.check(pr.lib.flgpu_pc_apply(pr.ctx, C.byref(cg.SolverOptions("cg", "learned").params()), L.ptr(r), L.ptr(z)),
+            "flgpu_pc_apply")
+    err = np.max(np.abs(z - z_ref)) / np.max(np.abs(z_ref))
+    print("learned pc_apply rel err", err)
+    assert err < 1e-5
+    pr.close()
+
+
 def test_gradient_matches_oracle(require_gpu):
     cfg, pr, g = system(scale=6)
     th = perturbed_params()
