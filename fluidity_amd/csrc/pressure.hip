@@ -55,7 +55,12 @@ struct PressureState {
     DevBuf<double> ct, ml;                   // ct_m (dim blocks of ct_nnz) and the lumped mass
     bool assembled = false;
     DevBuf<int32_t> cmc_rowptr, cmc_col;     // the last cmc pattern
+    int32_t cmc_rows = 0;
+    int64_t cmc_nnz = -1;                    // -1: no cmc assembled yet
     DevBuf<double> cmc, inv;
+    DevBuf<double> rho;                      // the last density (one value when constant)
+    DevBuf<int> err;
+    int last_ibp = 0, last_rho_const = 1;
 };
 
 namespace {
@@ -197,12 +202,14 @@ __device__ __forceinline__ void elem_geometry(int ele, const int32_t* __restrict
         for (int k = 0; k < DIM; k++) iJ[i][k] = iJ[i][k] / detJ;
 }
 
-// First pass: thread per element entry (p node i, u node j): grad_p_u_mat
-// (d, i, j) = sum_g detwei_g (n_p(i,g) du_t(j,g,d)) (shape_dshape,
-// femtools/FETools.F90:332-362), or by parts -sum_g detwei_g (dp_t(i,g,d)
-// n_u(j,g)) (dshape_shape, :364-389); threads with i = 0 also the lumped
-// mass of u node j, sum_k sum_g (n(j,g) n(k,g)) (rho_g detwei_g)
-// (add_mass_element_cg, Momentum_CG.F90:1519-1532).
+// First pass: thread per (element, u node j), for every p node i the
+// element entry grad_p_u_mat(d, i, j) = sum_g detwei_g (n_p(i,g)
+// du_t(j,g,d)) (shape_dshape, femtools/FETools.F90:332-362), or by parts
+// -sum_g detwei_g (dp_t(i,g,d) n_u(j,g)) (dshape_shape, :364-389), then
+// the lumped mass of u node j, sum_k sum_g (n(j,g) n(k,g)) (rho_g
+// detwei_g) (add_mass_element_cg, Momentum_CG.F90:1519-1532): the
+// element's geometry once for its p_loc entries, every thread the same
+// work.
 template <int DIM>
 __global__ void __launch_bounds__(256)
 k_ct_elem(int nele, int ploc, int uloc, int ngi, const int32_t* __restrict__ conn, const int32_t* __restrict__ conn_x,
@@ -210,42 +217,45 @@ k_ct_elem(int nele, int ploc, int uloc, int ngi, const int32_t* __restrict__ con
           const double* __restrict__ rho, int rho_const, int ibp, double* __restrict__ cval,
           double* __restrict__ mval) {
     const int per = ploc * uloc;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nele * per) return;
-    const int ele = (int)(t / per), r = (int)(t - (int64_t)ele * per), i = r / uloc, j = r - i * uloc;
+    const int64_t tj = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tj >= (int64_t)nele * uloc) return;
+    const int ele = (int)(tj / uloc), j = (int)(tj - (int64_t)ele * uloc);
     double iJ[DIM][DIM], detJ;
     elem_geometry<DIM>(ele, conn_x, X4, ut, iJ, detJ);
     const bool u_nonlinear = uloc != DIM + 1, p_nonlinear = ploc != DIM + 1;
-    double s[DIM];
+    for (int i = 0; i < ploc; i++) {
+        const int64_t t = (int64_t)ele * per + (int64_t)i * uloc + j;
+        double s[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; d++) s[d] = 0.0;
-    for (int g = 0; g < ngi; g++) {
-        const double detwei = fabs(detJ) * ut->w[g];
-        if (ibp) {
-            const int gp = p_nonlinear ? g : 0;
-            const double nj = ut->n[j + uloc * g];
+        for (int d = 0; d < DIM; d++) s[d] = 0.0;
+        for (int g = 0; g < ngi; g++) {
+            const double detwei = fabs(detJ) * ut->w[g];
+            if (ibp) {
+                const int gp = p_nonlinear ? g : 0;
+                const double nj = ut->n[j + uloc * g];
 #pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                double dp = 0.0;
+                for (int d = 0; d < DIM; d++) {
+                    double dp = 0.0;
 #pragma unroll
-                for (int k = 0; k < DIM; k++) dp = dp + pt->dn[i + ploc * (gp + ngi * k)] * iJ[d][k];
-                s[d] = s[d] + detwei * (dp * nj);
-            }
-        } else {
-            const int gu = u_nonlinear ? g : 0;
-            const double ni = pt->n[i + ploc * g];
+                    for (int k = 0; k < DIM; k++) dp = dp + pt->dn[i + ploc * (gp + ngi * k)] * iJ[d][k];
+                    s[d] = s[d] + detwei * (dp * nj);
+                }
+            } else {
+                const int gu = u_nonlinear ? g : 0;
+                const double ni = pt->n[i + ploc * g];
 #pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                double du = 0.0;
+                for (int d = 0; d < DIM; d++) {
+                    double du = 0.0;
 #pragma unroll
-                for (int k = 0; k < DIM; k++) du = du + ut->dn[j + uloc * (gu + ngi * k)] * iJ[d][k];
-                s[d] = s[d] + detwei * (ni * du);
+                    for (int k = 0; k < DIM; k++) du = du + ut->dn[j + uloc * (gu + ngi * k)] * iJ[d][k];
+                    s[d] = s[d] + detwei * (ni * du);
+                }
             }
         }
-    }
 #pragma unroll
-    for (int d = 0; d < DIM; d++) cval[t * DIM + d] = ibp ? -s[d] : s[d];
-    if (i == 0) {
+        for (int d = 0; d < DIM; d++) cval[t * DIM + d] = ibp ? -s[d] : s[d];
+    }
+    {
         double lump = 0.0;
         for (int k = 0; k < uloc; k++) {
             double mjk = 0.0;
@@ -314,42 +324,86 @@ __global__ void __launch_bounds__(256) k_invert_ml(int n, const double* __restri
     for (int d = 0; d < DIM; d++) inv[(int64_t)i * DIM + d] = v;
 }
 
-// C_P^T M_l^-1 C: thread per cmc entry (its row by bisection of the row
-// pointers); rows i of ct and j of ct walked together in ascending column
-// order, for each common velocity node k and each component d (inner) the
-// product ct_d(i,k) ct_d(j,k) inv(d,k) added (mult_div_vector_div_T,
-// femtools/Sparse_Matrices_Fields.F90:627-669).
+// C_P^T M_l^-1 C a wave per cmc row i: row i of ct (columns, the dim
+// values and the inverse lumped mass of each column) staged in the wave's
+// LDS slice once, then each lane takes an entry (i, j) and walks row j of
+// ct in ascending column order, its column loads issued eight at a time, finding
+// each column in the staged row by a
+// pointer that only moves forward: for each common velocity node k and
+// each component d (inner) ct_d(i,k) ct_d(j,k) inv(d,k) is added, the
+// order and the products of k_cmc, so the same bits.  A row i longer than
+// the slice (kCmcRow entries) takes k_cmc's walk from global memory.
+constexpr int kCmcRow = 96;
+
 template <int DIM>
 __global__ void __launch_bounds__(256)
-k_cmc(int rows, int64_t cmc_nnz, const int32_t* __restrict__ cmc_rowptr, const int32_t* __restrict__ cmc_col,
-      const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const double* __restrict__ ct,
-      int64_t ct_nnz, const double* __restrict__ inv, double* __restrict__ out) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= cmc_nnz) return;
-    int lo = 0, hi = rows - 1;  // the row holding entry e
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (cmc_rowptr[mid] <= e) lo = mid;
-        else hi = mid - 1;
-    }
-    const int i = lo, j = cmc_col[e];
-    int k1 = rowptr[i], k2 = rowptr[j];
-    const int e1 = rowptr[i + 1], e2 = rowptr[j + 1];
-    double entry0 = 0.0;
-    while (k1 < e1 && k2 < e2) {
-        const int c1 = col[k1], c2 = col[k2];
-        if (c1 < c2) {
-            k1++;
-        } else if (c1 == c2) {
+k_cmc_rows(int rows, const int32_t* __restrict__ cmc_rowptr, const int32_t* __restrict__ cmc_col,
+           const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const double* __restrict__ ct,
+           int64_t ct_nnz, const double* __restrict__ inv, double* __restrict__ out) {
+    __shared__ int32_t s_col[4][kCmcRow];
+    __shared__ double s_val[4][DIM][kCmcRow], s_inv[4][DIM][kCmcRow];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + wave;
+    if (i >= rows) return;
+    const int a0 = rowptr[i], a1 = rowptr[i + 1], la = a1 - a0;
+    const bool staged = la <= kCmcRow;
+    if (staged)
+        for (int q = lane; q < la; q += 64) {
+            const int k = col[a0 + q];
+            s_col[wave][q] = k;
 #pragma unroll
-            for (int d = 0; d < DIM; d++) entry0 = entry0 + ct[d * ct_nnz + k1] * ct[d * ct_nnz + k2] * inv[(int64_t)c1 * DIM + d];
-            k1++;
-            k2++;
-        } else {
-            k2++;
+            for (int d = 0; d < DIM; d++) {
+                s_val[wave][d][q] = ct[d * ct_nnz + a0 + q];
+                s_inv[wave][d][q] = inv[(int64_t)k * DIM + d];
+            }
         }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    constexpr int U = 8;
+    for (int e = cmc_rowptr[i] + lane; e < cmc_rowptr[i + 1]; e += 64) {
+        const int j = cmc_col[e];
+        const int b0 = rowptr[j], b1 = rowptr[j + 1];
+        double entry0 = 0.0;
+        if (staged) {
+            int p = 0;
+            for (int q0 = b0; q0 < b1; q0 += U) {
+                int cq[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) cq[u] = col[min(q0 + u, b1 - 1)];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (q0 + u >= b1) break;
+                    const int k = cq[u];
+                    while (p < la && s_col[wave][p] < k) p++;
+                    if (p < la && s_col[wave][p] == k) {
+#pragma unroll
+                        for (int d = 0; d < DIM; d++)
+                            entry0 = entry0 + s_val[wave][d][p] * ct[d * ct_nnz + q0 + u] * s_inv[wave][d][p];
+                        p++;
+                    }
+                }
+            }
+        } else {
+            int k1 = a0, k2 = b0;
+            while (k1 < a1 && k2 < b1) {
+                const int c1 = col[k1], c2 = col[k2];
+                if (c1 < c2) {
+                    k1++;
+                } else if (c1 == c2) {
+#pragma unroll
+                    for (int d = 0; d < DIM; d++)
+                        entry0 = entry0 + ct[d * ct_nnz + k1] * ct[d * ct_nnz + k2] * inv[(int64_t)c1 * DIM + d];
+                    k1++;
+                    k2++;
+                } else {
+                    k2++;
+                }
+            }
+        }
+        out[e] = entry0;
     }
-    out[e] = entry0;
 }
 
 // One set of lists (KIND 0 or 1), sorted in colour order and remapped to
@@ -413,37 +467,65 @@ int build_plan(flgpu_ctx* c, PressureState& S) {
     return FLGPU_OK;
 }
 
+// The ct_m and lumped-mass kernels on the device-resident inputs (density
+// in S.rho, the error flag in S.err).
+static int ct_launch(flgpu_ctx* c, PressureState& S, int ibp, int density_const) {
+    const hipStream_t s = c->stream;
+    auto launch = [&](auto dimtag) {
+        constexpr int D = decltype(dimtag)::value;
+        hipLaunchKernelGGL(k_ct_elem<D>, dim3(grid_for((int64_t)c->nele * c->loc, 256)), dim3(256), 0, s, c->nele,
+                           S.p_loc, c->loc, c->ngi, c->conn.p, c->conn_x.p, c->X4.p, c->d_tab.p, S.d_tab.p, S.rho.p,
+                           density_const, ibp, S.cval.p, S.mval.p);
+        hipLaunchKernelGGL(k_ct_sum<D>, dim3(grid_for(S.ct_nnz + c->nnodes, 256)), dim3(256), 0, s, S.ct_nnz,
+                           c->nnodes, S.cptr.p, S.clist.p, S.cval.p, S.mptr.p, S.mlist.p, S.mval.p, S.ct.p, S.ml.p);
+        if (S.ncbad > 0)
+            hipLaunchKernelGGL(k_ct_bad<D>, dim3(grid_for(S.ncbad, 256)), dim3(256), 0, s, S.ncbad, S.cbad.p,
+                               S.cval.p, S.err.p);
+    };
+    if (c->dim == 3)
+        launch(std::integral_constant<int, 3>());
+    else
+        launch(std::integral_constant<int, 2>());
+    FLGPU_HIP(hipGetLastError());
+    return FLGPU_OK;
+}
+
+// C^T M_l^-1 C on the bound cmc pattern (own_inverse: 1 / the lumped mass).
+static int cmc_launch(flgpu_ctx* c, PressureState& S, bool own_inverse) {
+    const hipStream_t s = c->stream;
+    auto launch = [&](auto dimtag) {
+        constexpr int D = decltype(dimtag)::value;
+        if (own_inverse)
+            hipLaunchKernelGGL(k_invert_ml<D>, dim3(grid_for(c->nnodes, 256)), dim3(256), 0, s, c->nnodes, S.ml.p,
+                               S.inv.p);
+        if (S.cmc_nnz > 0)
+            hipLaunchKernelGGL(k_cmc_rows<D>, dim3((S.cmc_rows + 3) / 4), dim3(256), 0, s, S.cmc_rows,
+                               S.cmc_rowptr.p, S.cmc_col.p, S.ct_rowptr.p, S.ct_col.p, S.ct.p, S.ct_nnz, S.inv.p,
+                               S.cmc.p);
+    };
+    if (c->dim == 3)
+        launch(std::integral_constant<int, 3>());
+    else
+        launch(std::integral_constant<int, 2>());
+    FLGPU_HIP(hipGetLastError());
+    return FLGPU_OK;
+}
+
 int ct_assemble(flgpu_ctx* c, PressureState& S, int ibp, int density_const, const double* density, double* ct_out,
                 double* ml_out) {
     const int dim = c->dim;
     FLGPU_TRY(build_plan(c, S));
     S.assembled = false;
     const hipStream_t s = c->stream;
-    DevBuf<double> rho;
-    FLGPU_TRY(rho.upload(density, density_const ? 1 : (size_t)c->nnodes, s));
-    DevBuf<int> err;
-    FLGPU_TRY(err.alloc(1));
-    FLGPU_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
-    const int64_t n = (int64_t)c->nele * S.p_loc * c->loc;
+    FLGPU_TRY(S.rho.upload(density, density_const ? 1 : (size_t)c->nnodes, s));
+    FLGPU_TRY(S.err.alloc(1));
+    FLGPU_HIP(hipMemsetAsync(S.err.p, 0, sizeof(int), s));
     if (!c->d_tab.p) FLGPU_TRY(c->d_tab.upload(&c->tab, 1, s));
-    auto launch = [&](auto dimtag) {
-        constexpr int D = decltype(dimtag)::value;
-        hipLaunchKernelGGL(k_ct_elem<D>, dim3(grid_for(n, 256)), dim3(256), 0, s, c->nele, S.p_loc, c->loc, c->ngi,
-                           c->conn.p, c->conn_x.p, c->X4.p, c->d_tab.p, S.d_tab.p, rho.p, density_const, ibp,
-                           S.cval.p, S.mval.p);
-        hipLaunchKernelGGL(k_ct_sum<D>, dim3(grid_for(S.ct_nnz + c->nnodes, 256)), dim3(256), 0, s, S.ct_nnz,
-                           c->nnodes, S.cptr.p, S.clist.p, S.cval.p, S.mptr.p, S.mlist.p, S.mval.p, S.ct.p, S.ml.p);
-        if (S.ncbad > 0)
-            hipLaunchKernelGGL(k_ct_bad<D>, dim3(grid_for(S.ncbad, 256)), dim3(256), 0, s, S.ncbad, S.cbad.p,
-                               S.cval.p, err.p);
-    };
-    if (dim == 3)
-        launch(std::integral_constant<int, 3>());
-    else
-        launch(std::integral_constant<int, 2>());
-    FLGPU_HIP(hipGetLastError());
+    FLGPU_TRY(ct_launch(c, S, ibp, density_const));
+    S.last_ibp = ibp;
+    S.last_rho_const = density_const;
     int h_err = 0;
-    FLGPU_HIP(hipMemcpyAsync(&h_err, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    FLGPU_HIP(hipMemcpyAsync(&h_err, S.err.p, sizeof(int), hipMemcpyDeviceToHost, s));
     if (ct_out)
         FLGPU_HIP(hipMemcpyAsync(ct_out, S.ct.p, sizeof(double) * S.ct_nnz * dim, hipMemcpyDeviceToHost, s));
     if (ml_out) FLGPU_HIP(hipMemcpyAsync(ml_out, S.ml.p, sizeof(double) * c->nnodes, hipMemcpyDeviceToHost, s));
@@ -481,26 +563,51 @@ int cmc_assemble(flgpu_ctx* c, PressureState& S, const double* inverse_masslump,
     if (inverse_masslump)
         FLGPU_HIP(hipMemcpyAsync(S.inv.p, inverse_masslump, sizeof(double) * c->nnodes * dim, hipMemcpyHostToDevice,
                                  s));
-    auto launch = [&](auto dimtag) {
-        constexpr int D = decltype(dimtag)::value;
-        if (!inverse_masslump)
-            hipLaunchKernelGGL(k_invert_ml<D>, dim3(grid_for(c->nnodes, 256)), dim3(256), 0, s, c->nnodes, S.ml.p,
-                               S.inv.p);
-        if (nnz > 0)
-            hipLaunchKernelGGL(k_cmc<D>, dim3(grid_for(nnz, 256)), dim3(256), 0, s, rows, nnz, S.cmc_rowptr.p,
-                               S.cmc_col.p, S.ct_rowptr.p, S.ct_col.p, S.ct.p, S.ct_nnz, S.inv.p, S.cmc.p);
-    };
-    if (dim == 3)
-        launch(std::integral_constant<int, 3>());
-    else
-        launch(std::integral_constant<int, 2>());
-    FLGPU_HIP(hipGetLastError());
+    S.cmc_rows = rows;
+    S.cmc_nnz = nnz;
+    FLGPU_TRY(cmc_launch(c, S, !inverse_masslump));
     if (out && nnz > 0) FLGPU_HIP(hipMemcpyAsync(out, S.cmc.p, sizeof(double) * nnz, hipMemcpyDeviceToHost, s));
     FLGPU_HIP(hipStreamSynchronize(s));  // rp, cl and the host arrays
     return FLGPU_OK;
 }
 
 }  // namespace
+
+// Device milliseconds per call of the ct_m + lumped-mass kernels and of
+// the C^T M_l^-1 C kernel (with the device's own inverse lumped mass), on
+// the inputs of the last flgpu_ct_assemble / flgpu_cmc_assemble (host
+// copies and checks excluded).
+int time_pressure(flgpu_ctx* c, int reps, double* ct_ms, double* cmc_ms) {
+    PressureState* S = c->ps;
+    if (!S || !S->assembled || S->cmc_nnz < 0)
+        return set_error(FLGPU_EINVAL, "flgpu_time_pressure: assemble ct_m and C^T M^-1 C once first");
+    hipEvent_t e0, e1, e2;
+    FLGPU_HIP(hipEventCreate(&e0));
+    FLGPU_HIP(hipEventCreate(&e1));
+    FLGPU_HIP(hipEventCreate(&e2));
+    int rc = FLGPU_OK;
+    if (hipEventRecord(e0, c->stream) != hipSuccess) rc = set_error(FLGPU_EHIP, "flgpu_time_pressure: event");
+    for (int r = 0; r < reps && rc == FLGPU_OK; r++) rc = ct_launch(c, *S, S->last_ibp, S->last_rho_const);
+    if (rc == FLGPU_OK && hipEventRecord(e1, c->stream) != hipSuccess)
+        rc = set_error(FLGPU_EHIP, "flgpu_time_pressure: event");
+    for (int r = 0; r < reps && rc == FLGPU_OK; r++) rc = cmc_launch(c, *S, true);
+    if (rc == FLGPU_OK && hipEventRecord(e2, c->stream) != hipSuccess)
+        rc = set_error(FLGPU_EHIP, "flgpu_time_pressure: event");
+    if (rc == FLGPU_OK && hipEventSynchronize(e2) != hipSuccess)
+        rc = set_error(FLGPU_EHIP, "flgpu_time_pressure: synchronize");
+    float a = 0.f, b = 0.f;
+    if (rc == FLGPU_OK && (hipEventElapsedTime(&a, e0, e1) != hipSuccess ||
+                           hipEventElapsedTime(&b, e1, e2) != hipSuccess))
+        rc = set_error(FLGPU_EHIP, "flgpu_time_pressure: elapsed time");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (rc == FLGPU_OK) {
+        *ct_ms = a / reps;
+        *cmc_ms = b / reps;
+    }
+    return rc;
+}
 
 void pressure_free(flgpu_ctx* c) {
     delete c->ps;
@@ -574,6 +681,14 @@ extern "C" int flgpu_pressure_bind(flgpu_ctx* c, int32_t p_loc, int32_t p_nodes,
         S.assembled = false;
         S.bound = true;
         return FLGPU_OK;
+    });
+}
+
+extern "C" int flgpu_time_pressure(flgpu_ctx* c, int32_t reps, double* ct_ms, double* cmc_ms) {
+    return flgpu::guarded("flgpu_time_pressure", [&]() -> int {
+        if (!c || !ct_ms || !cmc_ms || reps < 1) return flgpu::set_error(FLGPU_EINVAL, "flgpu_time_pressure: bad argument");
+        FLGPU_HIP(hipSetDevice(c->device));
+        return flgpu::time_pressure(c, reps, ct_ms, cmc_ms);
     });
 }
 

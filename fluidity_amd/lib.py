@@ -32,7 +32,7 @@ EXPORTS = (
     "flgpu_colouring_build", "flgpu_colouring_get", "flgpu_field_set",
     "flgpu_advdif_assemble_resident", "flgpu_advdif_assemble", "flgpu_rhs_add_field", "flgpu_surface_bind",
     "flgpu_advdif_surface", "flgpu_dirichlet_apply",
-    "flgpu_matrix_get", "flgpu_matrix_set", "flgpu_solve", "flgpu_solve_noalloc", "flgpu_pressure_bind", "flgpu_ct_assemble", "flgpu_cmc_assemble", "flgpu_petsc_solve", "flgpu_pc_apply",
+    "flgpu_matrix_get", "flgpu_matrix_set", "flgpu_solve", "flgpu_solve_noalloc", "flgpu_pressure_bind", "flgpu_ct_assemble", "flgpu_cmc_assemble", "flgpu_time_pressure", "flgpu_petsc_solve", "flgpu_pc_apply",
     "flgpu_solution_get", "flgpu_csr_mult", "flgpu_matrixdump_read", "flgpu_matrixdump_write",
     "flgpu_learned_param_count",
     "flgpu_learned_set_params", "flgpu_learned_get_params", "flgpu_learned_init_params",
@@ -114,6 +114,7 @@ def load() -> C.CDLL:
         "flgpu_pressure_bind": [P, I32, I32, P, P, P, I32, P, P],
         "flgpu_ct_assemble": [P, I32, I32, P, P, P],
         "flgpu_cmc_assemble": [P, P, I32, P, P, P],
+        "flgpu_time_pressure": [P, I32, C.POINTER(C.c_double), C.POINTER(C.c_double)],
         "flgpu_pc_apply": [P, C.POINTER(SolverParams), P, P],
         "flgpu_petsc_solve": [P, C.POINTER(SolverParams), P, P, P, P,
                               C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_double)],

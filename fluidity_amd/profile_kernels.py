@@ -7,7 +7,8 @@ runs, on the device-resident system of the configuration: `reps` blocked
 assemblies, `reps` colour-scheme assemblies, the solver-matrix SpMV (FP64 and
 FP32 values) `reps` times each, then each listed solve (preconditioner:
 precision, CG with the learned preconditioner's FCG, to a 1e-8 true relative
-residual) once, and finally one plain device-to-device copy of `copy-mib`
+residual) once, `pressure` ct_m and C^T M_l^-1 C assemblies on the 3-D P1
+configuration's mesh, and finally one plain device-to-device copy of `copy-mib`
 MiB through the HIP runtime (its blit kernel reads and writes exactly that
 many bytes), which calibrates the FETCH_SIZE / WRITE_SIZE counters.
 """
@@ -24,6 +25,8 @@ def main() -> int:
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--solves", default="")
+    ap.add_argument("--pressure", type=int, default=0, help="then this many ct_m and C^T M^-1 C assemblies "
+                                                             "on the 3-D P1 configuration's mesh")
     args = ap.parse_args()
     from . import cg, lib as L, meshgen as M
 
@@ -51,6 +54,19 @@ def main() -> int:
         _, info = cg.petsc_solve(pr, so)
         print(f"solve {pc} {prec}: {info.its} its, reason {info.reason}", flush=True)
     pr.close()
+    if args.pressure > 0:
+        import numpy as np
+        cfg2 = M.config("c2", seed=1)
+        pq = cg.CGProblem(cfg2.mesh, quad_degree=2)
+        pm = cg.vertex_pressure_mesh(pq)
+        cg.bind_pressure_mesh(pq, pm)
+        cf, cc = cg.cmc_sparsity(pm, cfg2.mesh.nnodes)
+        one = np.ones(1)
+        for _ in range(args.pressure):
+            L.check(lib.flgpu_ct_assemble(pq.ctx, 0, 1, L.ptr(one), None, None), "ct_assemble")
+            L.check(lib.flgpu_cmc_assemble(pq.ctx, None, cf.size - 1, L.ptr(cf), L.ptr(cc), None), "cmc_assemble")
+        print(f"pressure path {args.pressure}x", flush=True)
+        pq.close()
     if args.copy_mib > 0:
         _device_copy(args.copy_mib << 20)
         print(f"copied {args.copy_mib} MiB", flush=True)

@@ -434,6 +434,14 @@ int flgpu_ct_assemble(flgpu_ctx *ctx, int32_t integrate_by_parts,
 int flgpu_cmc_assemble(flgpu_ctx *ctx, const double *inverse_masslump,
                        int32_t cmc_rows, const int32_t *cmc_findrm,
                        const int32_t *cmc_colm, double *cmc_val);
+/* Device milliseconds per call of the ct_m + lumped-mass kernels and of
+ * the C^T M_l^-1 C kernel (the inverse lumped mass from the device's own
+ * lumped mass), reps calls each on the inputs of the last
+ * flgpu_ct_assemble / flgpu_cmc_assemble, host copies and checks excluded
+ * (no counterpart in Fluidity; measurement).  The device copies of ct_m
+ * and C^T M_l^-1 C are recomputed in place (cmc with 1/masslump). */
+int flgpu_time_pressure(flgpu_ctx *ctx, int32_t reps, double *ct_ms,
+                        double *cmc_ms);
 
 /* ---- learned preconditioner --------------------------------------------
  * A message-passing network on the matrix graph (3 layers, hidden 32,

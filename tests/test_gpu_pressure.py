@@ -88,6 +88,28 @@ def test_full_size_p1_configuration(require_gpu):
     pr.close()
 
 
+def test_time_pressure_leaves_results(require_gpu):
+    """flgpu_time_pressure re-runs the kernels on the device-resident inputs:
+    device times are positive and ct_m / C^T M_l^-1 C (own inverse) read
+    back afterwards equal the first results bit for bit."""
+    import ctypes as C
+    from fluidity_amd import lib as L
+    mesh = M.cube_tets(6, jitter=0.15, seed=8)
+    pr, pm = gpu_problem(mesh)
+    ct1, ml1 = cg.construct_momentum_ct(pr, pm)
+    cf, cc = cg.cmc_sparsity(pm, mesh.nnodes)
+    cmc1 = cg.assemble_masslumped_cmc(pr, cf, cc, None)
+    a, b = C.c_double(0), C.c_double(0)
+    with pytest.raises(FlgpuError):
+        L.check(pr.lib.flgpu_time_pressure(pr.ctx, 0, C.byref(a), C.byref(b)), "time_pressure")
+    L.check(pr.lib.flgpu_time_pressure(pr.ctx, 3, C.byref(a), C.byref(b)), "time_pressure")
+    assert a.value > 0 and b.value > 0
+    cmc2 = cg.assemble_masslumped_cmc(pr, cf, cc, None)
+    ct2, ml2 = cg.construct_momentum_ct(pr, pm)
+    assert np.array_equal(cmc1, cmc2) and np.array_equal(ct1, ct2) and np.array_equal(ml1, ml2)
+    pr.close()
+
+
 def test_ct_out_of_pattern_is_an_error(require_gpu):
     mesh = M.cube_tets(3, jitter=0.1, seed=7)
     pr = cg.CGProblem(mesh, quad_degree=2)
