@@ -696,15 +696,30 @@ k_gather_sum(int nnz, int rows, const int32_t* __restrict__ eptr, const int32_t*
              const double* __restrict__ gval, const int32_t* __restrict__ rptr, const int32_t* __restrict__ rlist,
              const double* __restrict__ grv, double* __restrict__ A, double* __restrict__ rhs) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nnz) {
+    // U list items a step: their index loads, then their value gathers, all
+    // in flight before the additions (in list order; a step past the list's
+    // end re-reads its last item and adds nothing)
+    constexpr int U = 4;
+    auto run = [&](int k0, int k1, const int32_t* __restrict__ list, const double* __restrict__ val) {
         double s = 0.0;
-        for (int k = eptr[t]; k < eptr[t + 1]; k++) s = s + gval[elist[k]];
-        A[t] = s;
+        for (int kb = k0; kb < k1; kb += U) {
+            int ix[U];
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) ix[u] = list[min(kb + u, k1 - 1)];
+#pragma unroll
+            for (int u = 0; u < U; u++) v[u] = val[ix[u]];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (kb + u < k1) s = s + v[u];
+        }
+        return s;
+    };
+    if (t < nnz) {
+        A[t] = run(eptr[t], eptr[t + 1], elist, gval);
     } else if (t - nnz < rows) {
         const int r = t - nnz;
-        double s = 0.0;
-        for (int k = rptr[r]; k < rptr[r + 1]; k++) s = s + grv[rlist[k]];
-        rhs[r] = s;
+        rhs[r] = run(rptr[r], rptr[r + 1], rlist, grv);
     }
 }
 
