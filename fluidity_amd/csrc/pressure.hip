@@ -281,21 +281,44 @@ k_ct_sum(int64_t nnz, int nodes, const int32_t* __restrict__ cptr, const int32_t
          const double* __restrict__ cval, const int32_t* __restrict__ mptr, const int32_t* __restrict__ mlist,
          const double* __restrict__ mval, double* __restrict__ ct, double* __restrict__ ml) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // U list items a step: their index loads, then their value gathers, all
+    // in flight before the additions in list order (a step past the end
+    // re-reads the last item and adds nothing)
+    constexpr int U = 4;
     if (t < nnz) {
         double s[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; d++) s[d] = 0.0;
-        for (int k = cptr[t]; k < cptr[t + 1]; k++) {
-            const int64_t it = clist[k];
+        const int k0 = cptr[t], k1 = cptr[t + 1];
+        for (int kb = k0; kb < k1; kb += U) {
+            int64_t it[U];
+            double v[U][DIM];
 #pragma unroll
-            for (int d = 0; d < DIM; d++) s[d] = s[d] + cval[it * DIM + d];
+            for (int u = 0; u < U; u++) it[u] = clist[min(kb + u, k1 - 1)];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) v[u][d] = cval[it[u] * DIM + d];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (kb + u < k1)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) s[d] = s[d] + v[u][d];
         }
 #pragma unroll
         for (int d = 0; d < DIM; d++) ct[d * nnz + t] = s[d];
     } else if (t - nnz < nodes) {
         const int n = (int)(t - nnz);
         double s = 0.0;
-        for (int k = mptr[n]; k < mptr[n + 1]; k++) s = s + mval[mlist[k]];
+        const int k0 = mptr[n], k1 = mptr[n + 1];
+        for (int kb = k0; kb < k1; kb += U) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) v[u] = mval[mlist[min(kb + u, k1 - 1)]];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (kb + u < k1) s = s + v[u];
+        }
         ml[n] = s;
     }
 }
