@@ -356,12 +356,13 @@ k_edge_agg(int n, const int32_t* __restrict__ rowptr, const int32_t* __restrict_
     const float c1 = lp[Lay::C1 + c], c2 = lp[Lay::C2 + c], bm = lp[Lay::BM + c];
     const int kb = rowptr[i], ke = rowptr[i + 1];
     float m = 0.f;
-    constexpr int U = 8;
+    constexpr int U = 16;
     for (int k0 = kb; k0 < ke; k0 += 32) {
         const int len = min(32, ke - k0);
         const int myj = c < len ? col[k0 + c] : i;
         const float mys = c < len ? sval[k0 + c] : 0.f;
-        // U gathers of Q_j in flight per step: the loads are unconditional
+        // U gathers of Q_j in flight per step (a P1 tet row in one step):
+        // the loads are unconditional
         // (a step past the row's end re-reads its last edge) and only the
         // additions are predicated, in entry order
         for (int t0 = 0; t0 < len; t0 += U) {
