@@ -552,6 +552,13 @@ def main() -> int:
                                                max_iterations=50000, check_every=32)),
             ("fsai_mixed_pcg", cg.SolverOptions("cg", "fsai", relative_error=1e-8, norm="unpreconditioned",
                                                 max_iterations=50000, precision="mixed", check_every=32)),
+            ("mixed_jacobi_pipecg", cg.SolverOptions("pipecg", "jacobi", relative_error=1e-8, norm="unpreconditioned",
+                                                     max_iterations=50000, precision="mixed", check_every=32)),
+            ("learned_mixed_pipecg", cg.SolverOptions("pipecg", "learned", relative_error=1e-8,
+                                                      norm="unpreconditioned", max_iterations=50000,
+                                                      precision="mixed", check_every=32)),
+            ("fsai_mixed_pipecg", cg.SolverOptions("pipecg", "fsai", relative_error=1e-8, norm="unpreconditioned",
+                                                   max_iterations=50000, precision="mixed", check_every=32)),
         ):
             cg.petsc_solve(pr, so)  # warm-up (allocations, first-touch)
             t0s = time.perf_counter()
@@ -582,6 +589,9 @@ def main() -> int:
             solves["mixed_jacobi_pcg"]["seconds"] / solves["learned_mixed_fcg"]["seconds"], 3)
         solves["speedup_fsai_mixed_vs_fp64_jacobi"] = round(
             solves["fp64_jacobi_pcg"]["seconds"] / solves["fsai_mixed_pcg"]["seconds"], 3)
+        for lab in ("learned_mixed_pipecg", "fsai_mixed_pipecg", "mixed_jacobi_pipecg"):
+            solves[f"speedup_{lab}_vs_fp64_jacobi"] = round(
+                solves["fp64_jacobi_pcg"]["seconds"] / solves[lab]["seconds"], 3)
         # the exact FSAI on the same pattern is the best factor of the family
         # the network predicts: its iteration count is the network's ceiling
         solves["iteration_ratio_learned_over_fsai"] = round(
@@ -589,7 +599,10 @@ def main() -> int:
         log(f"speedup learned+mixed vs FP64 Jacobi-PCG: {solves['speedup_learned_mixed_vs_fp64_jacobi']}x, "
             f"vs mixed Jacobi-PCG: {solves['speedup_learned_mixed_vs_mixed_jacobi']}x; "
             f"fsai+mixed {solves['speedup_fsai_mixed_vs_fp64_jacobi']}x; learned/fsai its "
-            f"{solves['iteration_ratio_learned_over_fsai']}")
+            f"{solves['iteration_ratio_learned_over_fsai']}; single-reduction CG: learned+mixed "
+            f"{solves['speedup_learned_mixed_pipecg_vs_fp64_jacobi']}x, fsai+mixed "
+            f"{solves['speedup_fsai_mixed_pipecg_vs_fp64_jacobi']}x, mixed Jacobi "
+            f"{solves['speedup_mixed_jacobi_pipecg_vs_fp64_jacobi']}x")
         solves.update(spmv_report(lib, L, pr, cfg.mesh.nnodes, nnz, log))
         out["solve"] = solves
 

@@ -76,7 +76,7 @@ PRECONDITIONERS = {"none": L.FLGPU_PC_NONE, "jacobi": L.FLGPU_PC_JACOBI, "sor": 
 @dataclass
 class SolverOptions:
     """solver options (schemas/solvers.rnc:107-612)."""
-    iterative_method: str = "cg"         # cg | gmres | fcg
+    iterative_method: str = "cg"         # cg | gmres | fcg | pipecg
     preconditioner: str = "jacobi"       # jacobi | none | sor | eisenstat | learned | fsai
     relative_error: float = 1e-8
     absolute_error: float = 0.0
@@ -91,7 +91,8 @@ class SolverOptions:
 
     def params(self) -> L.SolverParams:
         return L.SolverParams(
-            ksp={"cg": L.FLGPU_KSP_CG, "gmres": L.FLGPU_KSP_GMRES, "fcg": L.FLGPU_KSP_FCG}[self.iterative_method],
+            ksp={"cg": L.FLGPU_KSP_CG, "gmres": L.FLGPU_KSP_GMRES, "fcg": L.FLGPU_KSP_FCG,
+                 "pipecg": L.FLGPU_KSP_PIPECG}[self.iterative_method],
             pc=PRECONDITIONERS[self.preconditioner],
             rtol=self.relative_error, atol=self.absolute_error, max_its=self.max_iterations,
             restart=self.restart, precision={"fp64": L.FLGPU_FP64, "mixed": L.FLGPU_MIXED}[self.precision],

@@ -37,6 +37,9 @@ struct KrylovWork {
     int64_t nnz = 0;
     DevBuf<double> val64, b, x, r, z, p, w, dinv, y, rt;
     DevBuf<float> val32, r32, z32, p32, w32, dinv32, y32, rt32;
+    DevBuf<double> s;        // single-reduction CG: s = A p by recurrence
+    DevBuf<float> s32;
+    int npc = 0;             // partial sums of the last preconditioner apply
     DevBuf<double> partials;
     DevBuf<Scal> scal;       // two slots, see carry_if_done
     int cur = 0;             // slot holding the current state
@@ -212,7 +215,7 @@ k_build(int nrb, const int32_t* __restrict__ rb, const int32_t* __restrict__ row
 
 // Convergence test of KSPConvergedDefault on the norm chosen by norm_type,
 // then beta (Fletcher-Reeves, or Polak-Ribiere when flexible).
-__device__ void cg_test_and_beta(Scal* sc) {
+__device__ void cg_test(Scal* sc) {
     sc->its += 1;
     const double nrm = sc->norm_type == FLGPU_NORM_UNPRECONDITIONED ? sqrt(sc->rr) : sqrt(sc->zz);
     sc->last_norm = nrm;
@@ -232,6 +235,10 @@ __device__ void cg_test_and_beta(Scal* sc) {
         sc->reason = KSP_DIVERGED_ITS;
         sc->done = 1;
     }
+}
+
+__device__ void cg_test_and_beta(Scal* sc) {
+    cg_test(sc);
     if (sc->flexible)
         sc->beta = -sc->alpha * sc->zw / sc->rz_old;  // z.(r_new - r_old) / (z_old.r_old)
     else
@@ -395,6 +402,127 @@ k_cg_p(int n, const V* __restrict__ z, V* __restrict__ p, const double* __restri
         st_rows<R>(p, i, pv);
     }
     for (int i = ng * R + tid; i < n; i += nt) p[i] = fma(beta, p[i], z[i]);
+}
+
+// Single-reduction CG (Chronopoulos & Gear 1989), the recurrence PETSc's
+// KSPPIPECG builds on: with u = M r, w = A u, gamma = r.u and delta = u.w
+// from one reduction phase,
+//   beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma / alpha_old),
+//   p = u + beta p, s = w + beta s (s = A p), x += alpha p, r -= alpha s,
+// then u = M r (Jacobi here; a factor by its own kernels) and w = A u (the
+// next product).  One vector kernel an iteration instead of k_cg_update
+// and k_cg_p.  The previous iteration's sums: delta (the product's u.w,
+// pspmv), gamma, u.u and r.r (pg: this kernel's own region from the
+// previous iteration, or the factor apply's r.z, z.z with pc_layout set,
+// then r.r from prr); first: the iteration after a restart (gamma, r.r, u.u
+// already in the scalars, p = u, s = w).  Partials r.u, r.r, u.u, 0 to
+// pvec (a region other than pg and prr).
+template <typename V, int R>
+__global__ void __launch_bounds__(kVecBlock)
+k_cg1_vec(int n, double* __restrict__ x, V* __restrict__ p, V* __restrict__ s, V* __restrict__ r,
+          V* __restrict__ u, const V* __restrict__ w, const V* __restrict__ dinv, int jacobi,
+          const float* __restrict__ dsc, V* __restrict__ rt, const double* __restrict__ pspmv, int nspmv,
+          const double* __restrict__ pg, int ng, int pc_layout, const double* __restrict__ prr, int nrr, int first,
+          double* __restrict__ pvec, const Scal* __restrict__ in, Scal* __restrict__ out) {
+    __shared__ double s_ab[2];
+    __shared__ int s_stop;
+    if (carry_if_done(in, out)) return;
+    double dl[1], g[3] = {0.0, 0.0, 0.0}, q[3] = {0.0, 0.0, 0.0};
+    sum_partials<1>(pspmv, nspmv, dl);
+    if (!first) {
+        sum_partials<3>(pg, ng, g);
+        if (pc_layout) sum_partials<3>(prr, nrr, q);
+    }
+    if (threadIdx.x == 0) {
+        Scal sc = *in;
+        if (!first) {
+            sc.rz_old = sc.rz;
+            sc.rz = g[0];
+            sc.rr = pc_layout ? q[1] : g[1];
+            sc.zz = pc_layout ? g[1] : g[2];
+            cg_test(&sc);
+        }
+        const double delta = dl[0];
+        sc.pAp = delta;
+        if (!sc.done) {
+            const double beta = first ? 0.0 : sc.rz / sc.rz_old;
+            const double denom = first ? delta : delta - beta * sc.rz / sc.alpha;
+            if (sc.rz < 0.0) {
+                sc.reason = KSP_DIVERGED_INDEFINITE_PC;
+                sc.done = 1;
+            } else if (!(denom > 0.0)) {
+                sc.reason = (denom != denom) ? KSP_DIVERGED_NANORINF : KSP_DIVERGED_INDEFINITE_MAT;
+                sc.done = 1;
+            } else {
+                sc.beta = beta;
+                sc.alpha = sc.rz / denom;
+            }
+        }
+        s_ab[0] = sc.alpha;
+        s_ab[1] = sc.beta;
+        s_stop = sc.done;
+        if (blockIdx.x == 0) *out = sc;
+    }
+    __syncthreads();
+    if (s_stop) return;
+    const double alpha = s_ab[0];
+    const V a = (V)alpha, b = (V)s_ab[1];
+    double s_g = 0.0, s_rr = 0.0, s_uu = 0.0;
+    auto row = [&](double& xi, V& pi, V& si, V ui, V wi, V& ri, V di, float dsi, V& uo, V& ti) {
+        pi = first ? ui : fma(b, pi, ui);
+        si = first ? wi : fma(b, si, wi);
+        xi = fma(alpha, (double)pi, xi);
+        ri = ri - a * si;
+        if (dsc) ti = ri * (V)dsi;
+        if (jacobi) {
+            uo = di * ri;
+            s_g += (double)ri * (double)uo;
+            s_uu += (double)uo * (double)uo;
+        }
+        s_rr += (double)ri * (double)ri;
+    };
+    const int ng_ = n / R, tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+    for (int gi = tid; gi < ng_; gi += nt) {
+        const int i = gi * R;
+        Rows<V, R> pv = ld_rows<R>((const V*)p, i), sv = ld_rows<R>((const V*)s, i), rv = ld_rows<R>((const V*)r, i);
+        const Rows<V, R> uv = ld_rows<R>((const V*)u, i), wv = ld_rows<R>(w, i);
+        Rows<double, R> xv = ld_rows<R>((const double*)x, i);
+        Rows<V, R> dv = {}, uo = {}, tv = {};
+        Rows<float, R> fv = {};
+        if (jacobi) dv = ld_rows<R>(dinv, i);
+        if (dsc) fv = ld_rows<R>(dsc, i);
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            double xi = xv[k];
+            V pi = pv[k], si = sv[k], ri = rv[k], uk = (V)0, ti = (V)0;
+            row(xi, pi, si, uv[k], wv[k], ri, dv[k], fv[k], uk, ti);
+            xv[k] = xi;
+            pv[k] = pi;
+            sv[k] = si;
+            rv[k] = ri;
+            uo[k] = uk;
+            tv[k] = ti;
+        }
+        st_rows<R>(x, i, xv);
+        st_rows<R>(p, i, pv);
+        st_rows<R>(s, i, sv);
+        st_rows<R>(r, i, rv);
+        if (jacobi) st_rows<R>(u, i, uo);
+        if (dsc) st_rows<R>(rt, i, tv);
+    }
+    for (int i = ng_ * R + tid; i < n; i += nt) {
+        double xi = x[i];
+        V pi = p[i], si = s[i], ri = r[i], uk = (V)0, ti = (V)0;
+        row(xi, pi, si, u[i], w[i], ri, jacobi ? dinv[i] : (V)0, dsc ? dsc[i] : 0.f, uk, ti);
+        x[i] = xi;
+        p[i] = pi;
+        s[i] = si;
+        r[i] = ri;
+        if (jacobi) u[i] = uk;
+        if (dsc) rt[i] = ti;
+    }
+    double v[4] = {s_g, s_rr, s_uu, 0.0};
+    block_partials<4>(v, pvec);
 }
 
 template <typename V>
@@ -845,6 +973,7 @@ template <typename V>
 struct CgVecs {
     V *r, *z, *p, *w, *dinv, *y, *rt;
     const V* val;
+    V* s = nullptr;  // single-reduction CG only
 };
 
 template <typename V>
@@ -893,6 +1022,7 @@ static int restart_residual(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, PcM
     if (!diag) {
         int np = 0;
         FLGPU_TRY(pc_apply<V>(c, k, v, mode, nullptr, &np));
+        k->npc = np;
         FLGPU_TRY(finalize(c, k, FIN_LEARNED, PART_PC, np, false, norm0));
     }
     return FLGPU_OK;
@@ -1058,6 +1188,191 @@ static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info*
                     FLGPU_TRY(write_scal(c, k));
                     FLGPU_TRY(cg_restart<V>(c, k, v, PCM_DIAG, false));
                     FLGPU_TRY(read_scal(c, k));
+                    rmax = std::sqrt(h.rr);
+                }
+                window_its = 0;
+                window_norm = h.last_norm;
+            }
+        }
+    }
+    FLGPU_TRY(true_residual(c, k));
+    FLGPU_TRY(read_scal(c, k));
+    auto t1 = std::chrono::steady_clock::now();
+    info->seconds = std::chrono::duration<double>(t1 - t0).count();
+    info->its = h.its;
+    info->reason = h.reason;
+    info->true_relres = bnorm > 0 ? std::sqrt(h.true_rr) / bnorm : std::sqrt(h.true_rr);
+    info->precond_relres = h.rnorm0 > 0 ? h.last_norm / h.rnorm0 : h.last_norm;
+    return FLGPU_OK;
+}
+
+// ------------------------- single-reduction CG ---------------------------
+
+// The restart of the single-reduction recurrence: r = b - A x (FP64), r_V,
+// u = M r with gamma, r.r, u.u in the scalars (restart_residual), then
+// w = A u with u.w in the PART_SPMV partials; the next iteration takes
+// p = u, s = w.
+template <typename V>
+static int cg1_restart(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, PcMode mode, bool norm0, bool recompute) {
+    FLGPU_TRY(restart_residual<V>(c, k, v, mode, norm0, recompute));
+    return spmv<V, V, V>(c, k, v.val, v.z, v.w, nullptr, 1, &k->S()->done);
+}
+
+// One iteration: the vector update (k_cg1_vec), [the factor or SSOR
+// apply], the product w = A u.  Two kernels with the diagonal
+// preconditioner, four with a factor.  reg: the partial-sum region the
+// previous update wrote (read here), the other one is written.
+template <typename V>
+static int cg1_iteration(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, PcMode mode, bool first, int& reg) {
+    const bool diag = mode == PCM_DIAG;
+    const int reg_out = reg == PART_VEC ? PART_VEC2 : PART_VEC;
+    Scal* in = k->S();
+    k->cur ^= 1;
+    const float* dsc = mode == PCM_LEARNED ? learned_scale(c) : nullptr;
+    const double* pg = diag ? k->part(reg) : k->part(PART_PC);
+    const int ng = diag ? k->nblk : k->npc;
+    constexpr int R = 16 / sizeof(V);
+    auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->x.p, v.p, v.s, v.r, v.z,
+                           (const V*)v.w, (const V*)v.dinv, diag ? 1 : 0, dsc, v.rt, (const double*)k->part(PART_SPMV),
+                           spmv_grid(c), pg, ng, diag ? 0 : 1, (const double*)k->part(reg), k->nblk, first ? 1 : 0,
+                           k->part(reg_out), (const Scal*)in, k->S());
+    };
+    if (rows_aligned<R>(k->x.p, v.p, v.s, v.r, v.z, v.w, v.dinv, dsc, v.rt))
+        launch(&k_cg1_vec<V, R>);
+    else
+        launch(&k_cg1_vec<V, 1>);
+    FLGPU_HIP(hipGetLastError());
+    reg = reg_out;
+    if (!diag) {
+        int npc = 0;
+        FLGPU_TRY(pc_apply<V>(c, k, v, mode, &k->S()->done, &npc));
+        k->npc = npc;
+    }
+    return spmv<V, V, V>(c, k, v.val, v.z, v.w, nullptr, 1, &k->S()->done);
+}
+
+template <typename V>
+static int run_cg1(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info* info, bool factor) {
+    KrylovWork* k = c->kw;
+    const bool mixed = std::is_same<V, float>::value;
+    CgVecs<V> v = cg_vecs<V>(k);
+    if (mixed) {
+        FLGPU_TRY(k->s32.alloc(c->rows));
+        v.s = reinterpret_cast<V*>(k->s32.p);
+    } else {
+        FLGPU_TRY(k->s.alloc(c->rows));
+        v.s = reinterpret_cast<V*>(k->s.p);
+    }
+    PcMode mode = factor ? PCM_LEARNED : sp->pc == FLGPU_PC_SOR ? PCM_SSOR : PCM_DIAG;
+    bool use_factor = factor;
+    const int check_every = sp->check_every > 0 ? sp->check_every : 16;
+    const double stall_factor = sp->stall_factor > 0 ? sp->stall_factor : 0.9;
+    const int stall_window = sp->stall_window > 0 ? sp->stall_window : 50;
+    if (!factor && sp->pc == FLGPU_PC_NONE) {
+        hipLaunchKernelGGL((k_fill<double>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, 1.0, k->dinv.p);
+        if (mixed)
+            hipLaunchKernelGGL((k_fill<float>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, 1.0f,
+                               k->dinv32.p);
+        FLGPU_HIP(hipGetLastError());
+    }
+    if (mode == PCM_SSOR) FLGPU_TRY(ssor_prepare(c, k));
+    Scal& h = *k->h_scal;
+    FLGPU_TRY(read_scal(c, k));
+    {
+        const double pivot = h.pivot, dmin = h.dmin, dmax = h.dmax;
+        std::memset(&h, 0, sizeof h);
+        h.pivot = pivot;
+        h.dmin = dmin;
+        h.dmax = dmax;
+    }
+    h.max_its = sp->max_its > 0 ? sp->max_its : 10000;
+    h.rtol = sp->rtol;
+    h.atol = sp->atol;
+    h.norm_type = sp->norm_type;
+    FLGPU_TRY(write_scal(c, k));
+    FLGPU_TRY(norm_b(c, k));
+    FLGPU_TRY(cg1_restart<V>(c, k, v, mode, true, true));
+    FLGPU_TRY(read_scal(c, k));
+    info->refinements = 0;
+    info->fallback = 0;
+    const double bnorm = std::sqrt(h.bb);
+    if (h.last_norm == 0.0 || h.last_norm <= sp->atol) {
+        h.reason = KSP_CONVERGED_ATOL;
+        h.done = 1;
+        FLGPU_TRY(write_scal(c, k));
+    }
+    double rmax = std::sqrt(h.rr);
+    const double delta = 0.1;
+    int window_its = 0;
+    double window_norm = h.last_norm;
+    bool first = true;
+    int reg = PART_VEC;  // restart_residual's partials
+    auto t0 = std::chrono::steady_clock::now();
+    while (!h.done) {
+        for (int q = 0; q < check_every; q++) {
+            FLGPU_TRY(cg1_iteration<V>(c, k, v, mode, first, reg));
+            first = false;
+        }
+        FLGPU_TRY(read_scal(c, k));
+        const double rnow = std::sqrt(h.rr);
+        if (mixed && (h.done ? h.reason > 0 : rnow < delta * rmax)) {
+            // reliable update, as in the (F)PCG loop: the FP64 true residual
+            // replaces the FP32 one and decides convergence
+            const int reason = h.reason, done = h.done;
+            FLGPU_TRY(true_residual(c, k));
+            FLGPU_TRY(read_scal(c, k));
+            const double tr = std::sqrt(h.true_rr);
+            info->refinements++;
+            const double target = std::max(sp->rtol * (sp->norm_type == FLGPU_NORM_UNPRECONDITIONED ? h.rnorm0 : bnorm),
+                                           sp->atol);
+            if (done && reason > 0 && tr > target) {
+                if (h.its < h.max_its) {
+                    h.done = 0;
+                    h.reason = 0;
+                } else {
+                    h.reason = KSP_DIVERGED_ITS;
+                }
+                FLGPU_TRY(write_scal(c, k));
+            }
+            if (!h.done) {
+                // As in the (F)PCG loop, the direction survives a drop of
+                // up to 1e3 since the last replacement: s = A p is then
+                // recomputed (its FP32 recurrence drifts like r's) and the
+                // next update forms beta from the replaced gamma and the
+                // gamma that formed p.  After a larger drop the recurrence
+                // restarts from the replaced residual (p = u, s = w); the
+                // update that follows tests no residual, so the count takes
+                // the replaced one here.
+                const bool restart_p = rnow < 1e-3 * rmax;
+                const double rz_p = h.rz;
+                if (!restart_p) FLGPU_TRY((spmv<V, V, V>(c, k, v.val, v.p, v.s, nullptr, 0, &k->S()->done)));
+                FLGPU_TRY(cg1_restart<V>(c, k, v, mode, false, false));
+                FLGPU_TRY(read_scal(c, k));
+                if (restart_p)
+                    h.its += 1;
+                else
+                    h.rz = rz_p;
+                FLGPU_TRY(write_scal(c, k));
+                first = restart_p;
+                reg = PART_VEC;
+                rmax = std::sqrt(h.rr);
+            }
+        }
+        if (use_factor && !h.done) {
+            window_its += check_every;
+            if (window_its >= stall_window) {
+                if (h.last_norm > stall_factor * window_norm) {
+                    use_factor = false;
+                    mode = PCM_DIAG;
+                    info->fallback = 1;
+                    FLGPU_TRY(write_scal(c, k));
+                    FLGPU_TRY(cg1_restart<V>(c, k, v, PCM_DIAG, false, true));
+                    FLGPU_TRY(read_scal(c, k));
+                    h.its += 1;
+                    FLGPU_TRY(write_scal(c, k));
+                    first = true;
+                    reg = PART_VEC;
                     rmax = std::sqrt(h.rr);
                 }
                 window_its = 0;
@@ -1356,14 +1671,15 @@ int solve(flgpu_ctx* c, const flgpu_solver_params* sp, double* x_host, flgpu_sol
     const bool mixed = sp->precision == FLGPU_MIXED;
     if (sp->precision != FLGPU_FP64 && sp->precision != FLGPU_MIXED)
         return set_error(FLGPU_EINVAL, "flgpu_solve: precision %d", sp->precision);
-    if (sp->ksp != FLGPU_KSP_CG && sp->ksp != FLGPU_KSP_GMRES && sp->ksp != FLGPU_KSP_FCG)
+    if (sp->ksp != FLGPU_KSP_CG && sp->ksp != FLGPU_KSP_GMRES && sp->ksp != FLGPU_KSP_FCG &&
+        sp->ksp != FLGPU_KSP_PIPECG)
         return set_error(FLGPU_EINVAL, "flgpu_solve: ksp %d", sp->ksp);
     if (sp->pc != FLGPU_PC_NONE && sp->pc != FLGPU_PC_JACOBI && sp->pc != FLGPU_PC_LEARNED && sp->pc != FLGPU_PC_SOR &&
         sp->pc != FLGPU_PC_FSAI)
         return set_error(FLGPU_EINVAL, "flgpu_solve: pc %d", sp->pc);
     const bool factor = sp->pc == FLGPU_PC_LEARNED || sp->pc == FLGPU_PC_FSAI;
     if (factor && sp->ksp == FLGPU_KSP_GMRES)
-        return set_error(FLGPU_EUNSUPPORTED, "flgpu_solve: the learned and fsai preconditioners run with cg/fcg");
+        return set_error(FLGPU_EUNSUPPORTED, "flgpu_solve: the learned and fsai preconditioners run with cg/fcg/pipecg");
     FLGPU_TRY(ensure_work(c, mixed));
     KrylovWork* k = c->kw;
     const bool zero_x = sp->start_from_zero || !x_host;
@@ -1375,6 +1691,8 @@ int solve(flgpu_ctx* c, const flgpu_solver_params* sp, double* x_host, flgpu_sol
     if (factor) FLGPU_TRY(learned_prepare(c, mixed, sp->pc == FLGPU_PC_FSAI));
     if (sp->ksp == FLGPU_KSP_GMRES) {
         FLGPU_TRY(mixed ? run_gmres<float>(c, sp, &info) : run_gmres<double>(c, sp, &info));
+    } else if (sp->ksp == FLGPU_KSP_PIPECG) {
+        FLGPU_TRY(mixed ? run_cg1<float>(c, sp, &info, factor) : run_cg1<double>(c, sp, &info, factor));
     } else {
         FLGPU_TRY(mixed ? run_cg<float>(c, sp, &info, factor) : run_cg<double>(c, sp, &info, factor));
     }

@@ -45,7 +45,8 @@ enum {
     PART_VEC = 2,   // 4 components: r.z, r.r, z.z, z.w of the CG update
     PART_PC = 6,    // 3 components: r.z, z.z, z.w of the learned apply
     PART_MISC = 9,  // 2 components: b.b, diagonal min/max
-    PART_COMPONENTS = 11
+    PART_VEC2 = 11, // 4 components: the single-reduction CG's update, alternating with PART_VEC
+    PART_COMPONENTS = 15
 };
 
 // Sum over a block in a fixed order; result valid in thread 0.

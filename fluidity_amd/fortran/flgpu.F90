@@ -21,7 +21,8 @@ module flgpu
   integer(c_int32_t), parameter, public :: FLGPU_OK = 0
   integer(c_int32_t), parameter, public :: FLGPU_FP64 = 0, FLGPU_MIXED = 1
   integer(c_int32_t), parameter, public :: FLGPU_ASM_AUTO = 0, FLGPU_ASM_COLOUR = 1, FLGPU_ASM_BLOCKED = 2
-  integer(c_int32_t), parameter, public :: FLGPU_KSP_CG = 0, FLGPU_KSP_GMRES = 1, FLGPU_KSP_FCG = 2
+  integer(c_int32_t), parameter, public :: FLGPU_KSP_CG = 0, FLGPU_KSP_GMRES = 1, FLGPU_KSP_FCG = 2, &
+                                            FLGPU_KSP_PIPECG = 3
   integer(c_int32_t), parameter, public :: FLGPU_PC_NONE = 0, FLGPU_PC_JACOBI = 1, FLGPU_PC_LEARNED = 2, &
                                             FLGPU_PC_SOR = 3, FLGPU_PC_FSAI = 4
   integer(c_int32_t), parameter, public :: FLGPU_NORM_PRECONDITIONED = 0, FLGPU_NORM_UNPRECONDITIONED = 1

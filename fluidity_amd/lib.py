@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libflgpu.so")
 FLGPU_OK = 0
 FLGPU_FP64, FLGPU_FP32, FLGPU_MIXED = 0, 1, 1
 FLGPU_ASM_AUTO, FLGPU_ASM_COLOUR, FLGPU_ASM_BLOCKED = 0, 1, 2
-FLGPU_KSP_CG, FLGPU_KSP_GMRES, FLGPU_KSP_FCG = 0, 1, 2
+FLGPU_KSP_CG, FLGPU_KSP_GMRES, FLGPU_KSP_FCG, FLGPU_KSP_PIPECG = 0, 1, 2, 3
 FLGPU_PC_NONE, FLGPU_PC_JACOBI, FLGPU_PC_LEARNED, FLGPU_PC_SOR, FLGPU_PC_FSAI = 0, 1, 2, 3, 4
 FLGPU_NORM_PRECONDITIONED, FLGPU_NORM_UNPRECONDITIONED = 0, 1
 FIELD_T, FIELD_VELOCITY, FIELD_DIFFUSIVITY, FIELD_SOURCE, FIELD_ABSORPTION = range(5)

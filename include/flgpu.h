@@ -276,7 +276,15 @@ int flgpu_matrix_set(flgpu_ctx *ctx, const double *A_val, const double *rhs,
 
 /* ---- solve ------------------------------------------------------------ */
 
-enum { FLGPU_KSP_CG = 0, FLGPU_KSP_GMRES = 1, FLGPU_KSP_FCG = 2 };
+/* FLGPU_KSP_PIPECG: "pipecg", the single-reduction CG recurrence of
+ * Chronopoulos & Gear (1989) that PETSc's KSPPIPECG is built on (u = M r,
+ * w = A u, gamma = r.u and delta = u.w in one reduction phase, s = A p by
+ * recurrence): one vector kernel an iteration instead of two; PETSc's
+ * overlap of the reduction with the next product is not needed on one
+ * in-order stream.  The same preconditioners as cg, and the same FP64
+ * reliable updates in mixed precision (each recomputes s = A p and keeps
+ * the direction, which restarts after a residual drop above 1e3). */
+enum { FLGPU_KSP_CG = 0, FLGPU_KSP_GMRES = 1, FLGPU_KSP_FCG = 2, FLGPU_KSP_PIPECG = 3 };
 /* FLGPU_PC_SOR: "sor" (and "eisenstat", SSOR by Eisenstat's trick) of
  * femtools/Solvers.F90:1844-1854 -- one symmetric Gauss-Seidel sweep
  * (omega = 1) taken colour by colour over a first-fit colouring of the
