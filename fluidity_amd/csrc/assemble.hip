@@ -1857,7 +1857,9 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     // the colour scheme when the mesh does not fit its plan (a node in more
     // elements than a slab holds, a row longer than 255 entries).  The
     // colour scheme asked for by name needs no plan.
-    bool blocked_ok = c->linear && scheme != FLGPU_ASM_COLOUR;
+    // intervals (dim 1) take the colour scheme: a row has at most three
+    // entries, too few for the slab staging to pay
+    bool blocked_ok = c->linear && dim >= 2 && scheme != FLGPU_ASM_COLOUR;
     if (blocked_ok && (fp32 || fl.stab || scheme == FLGPU_ASM_AUTO)) {
         const int rc = build_block_plan(c);
         if (rc == FLGPU_EUNSUPPORTED && scheme == FLGPU_ASM_AUTO)
@@ -1870,7 +1872,7 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
             blocked_ok = stageable(c, fl);
     }
     // quadratic T, unstabilised: the two-pass scheme is the blocked one
-    const bool two_pass = !c->linear && !fl.stab && scheme != FLGPU_ASM_COLOUR;
+    const bool two_pass = !c->linear && dim >= 2 && !fl.stab && scheme != FLGPU_ASM_COLOUR;
     if (scheme == FLGPU_ASM_BLOCKED && !blocked_ok && !two_pass)
         return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: the blocked scheme covers linear T (in FP32 and with "
                                              "stabilisation: with standard P1 tables and the field on the "
@@ -1909,6 +1911,8 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
         rc = fp32 ? LAUNCH<D, L, float>(c, fd, fl, d_err) : LAUNCH<D, L, double>(c, fd, fl, d_err); \
     } else
     if (fl.stab) {
+        FL_DISPATCH(1, 2, launch_colour_stab)
+        FL_DISPATCH(1, 3, launch_colour_stab)
         FL_DISPATCH(2, 3, launch_colour_stab)
         FL_DISPATCH(2, 6, launch_colour_stab)
         FL_DISPATCH(3, 4, launch_colour_stab)
@@ -1917,6 +1921,8 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
         }
         return rc;
     }
+    FL_DISPATCH(1, 2, launch_colour)
+    FL_DISPATCH(1, 3, launch_colour)  // a P2 interval row has three entries: a thread per row
     FL_DISPATCH(2, 3, launch_colour)
     FL_DISPATCH(2, 6, launch_colour_entry)
     FL_DISPATCH(3, 4, launch_colour)

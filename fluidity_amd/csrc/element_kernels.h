@@ -104,6 +104,8 @@ FLGPU_HD_INLINE void invert_jacobian(const R (&JT)[DIM][DIM], R (&iJ)[DIM][DIM],
                 const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, k1 = (k + 1) % 3, k2 = (k + 2) % 3;
                 C[i][k] = FMA(JT[i1][k1], JT[i2][k2], -(JT[i2][k1] * JT[i1][k2]));
             }
+    } else if constexpr (DIM == 1) {
+        C[0][0] = (R)1;  // an interval: J = dx/dxi, inverse 1/J
     } else {
         C[0][0] = JT[1][1];
         C[1][0] = -JT[0][1];
@@ -775,6 +777,9 @@ FLGPU_HD_INLINE R nu_bar_scaled(const R (&u)[DIM], const R (&JT)[DIM][DIM], cons
                     C[i][k] = K[k1][i1] * K[k2][i2] - K[k2][i1] * K[k1][i2];
                 }
             det = K[0][0] * C[0][0] + K[1][0] * C[0][1] + K[2][0] * C[0][2];
+        } else if constexpr (DIM == 1) {
+            C[0][0] = (R)1;
+            det = K[0][0];
         } else {
             C[0][0] = K[1][1];
             C[1][1] = K[0][0];

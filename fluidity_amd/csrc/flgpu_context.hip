@@ -132,9 +132,9 @@ int flgpu_mesh_bind(flgpu_ctx* c, int64_t key_id, int64_t key_event, int32_t dim
                     const double* X) {
     return guarded("flgpu_mesh_bind", [&]() -> int {
         if (!c || !ndglno_x || !X) return set_error(FLGPU_EINVAL, "flgpu_mesh_bind: null argument");
-        if (dim < 2 || dim > 3 || loc_x != dim + 1 || nele <= 0 || nx <= 0)
+        if (dim < 1 || dim > 3 || loc_x != dim + 1 || nele <= 0 || nx <= 0)
             return set_error(FLGPU_EUNSUPPORTED,
-                             "flgpu_mesh_bind: linear simplices of dim 2/3 only (dim=%d loc_x=%d)",
+                             "flgpu_mesh_bind: linear simplices of dim 1-3 only (dim=%d loc_x=%d)",
                              dim, loc_x);
         FLGPU_HIP(hipSetDevice(c->device));
         if (key_id == c->key_id && key_event == c->key_event && c->nele == nele && c->nx == nx)

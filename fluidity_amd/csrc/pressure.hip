@@ -650,6 +650,7 @@ extern "C" int flgpu_pressure_bind(flgpu_ctx* c, int32_t p_loc, int32_t p_nodes,
         if (c->nele <= 0 || c->loc <= 0 || c->ncol <= 0)
             return set_error(FLGPU_EINVAL, "flgpu_pressure_bind: bind the meshes and the colouring first");
         const int dim = c->dim, ngi = c->ngi;
+        if (dim < 2) return set_error(FLGPU_EUNSUPPORTED, "flgpu_pressure_bind: 2-D and 3-D meshes only");
         if (p_loc < 1 || p_loc > kMaxLoc)
             return set_error(FLGPU_EUNSUPPORTED, "flgpu_pressure_bind: p_loc %d", p_loc);
         if (p_nodes < 1 || ct_rows != p_nodes)

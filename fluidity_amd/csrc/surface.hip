@@ -452,6 +452,8 @@ extern "C" int flgpu_surface_bind(flgpu_ctx* c, int32_t nfaces, int32_t floc, in
         if (c->nnodes <= 0 || c->nele <= 0)
             return set_error(FLGPU_EINVAL, "flgpu_surface_bind: bind the coordinate and field meshes first");
         const int dim = c->dim;
+        if (dim < 2)
+            return set_error(FLGPU_EUNSUPPORTED, "flgpu_surface_bind: the face loop takes 2-D and 3-D meshes");
         const int flin = dim, fquad = dim * (dim + 1) / 2;
         if (nfaces < 0 || !(floc == flin || floc == fquad) || floc > kMaxFaceLoc || fngi < 1 || fngi > kMaxFaceGi)
             return set_error(FLGPU_EUNSUPPORTED, "flgpu_surface_bind: face loc %d / ngi %d (dim %d)", floc, fngi, dim);

@@ -205,6 +205,32 @@ def square_tris(n: int, jitter: float = 0.0, seed: int = 0, degree: int = 1) -> 
                 bnd2.astype(np.int32), name=f"square{n}p2")
 
 
+def interval_mesh(n: int, left: float = 0.0, right: float = 1.0, jitter: float = 0.0, seed: int = 0,
+                  degree: int = 1) -> Mesh:
+    """The interval [left, right] in n elements (what Fluidity's `interval`
+    script writes: vertices left + k dx, element k = (k, k+1)); interior
+    vertices jittered by jitter*dx.  degree 2 adds the midpoints, numbered on
+    the half-spacing lattice (vertex k = 2k, midpoint of element k = 2k+1),
+    in the element order v1, mid, v2.  Boundary: the two end nodes (surface
+    ids 1 at left, 2 at right in the interval script)."""
+    h = (right - left) / n
+    x = left + np.arange(n + 1) * h
+    if jitter:
+        u = uniform_pm1(seed, np.arange(n + 1, dtype=np.uint64))
+        x[1:n] = x[1:n] + jitter * h * u[1:n]
+    X = x[:, None].astype(np.float64)
+    conn = np.stack([np.arange(n), np.arange(1, n + 1)], axis=1)
+    nd = (conn + 1).astype(np.int32).ravel()
+    if degree == 1:
+        return Mesh(1, n, n + 1, nd, X, 1, n + 1, nd, X, np.array([1, n + 1], np.int32), name=f"interval{n}")
+    conn2 = np.stack([2 * conn[:, 0], 2 * conn[:, 0] + 1, 2 * conn[:, 1]], axis=1)
+    pos = np.zeros((2 * n + 1, 1))
+    pos[0::2, 0] = x
+    pos[1::2, 0] = 0.5 * (x[:-1] + x[1:])
+    return Mesh(1, n, n + 1, nd, X, 2, 2 * n + 1, (conn2 + 1).astype(np.int32).ravel(), pos,
+                np.array([1, 2 * n + 1], np.int32), name=f"interval{n}p2")
+
+
 def boundary_numbering(dim: int, degree: int, face: int) -> list[int]:
     """Local nodes (0-based) of face `face` (1-based; the face opposite
     vertex `face`) of a Lagrange simplex, in the order of
@@ -487,6 +513,30 @@ def config(name: str, seed: int = 1, scale: int | None = None) -> Config:
         kap = grf_lognormal(m.node_pos, seed, 0.1, 1e3)
         return Config("c5", m, 2, np.zeros(m.nnodes), None, kap, sin_source(m.node_pos), None,
                       1.0, 1.0, 0.0, False, False, "cg", 1e-8)
+    if name in ("interval", "intervalp2"):
+        # 1-D advection-diffusion with every term on a jittered interval
+        # (P1, or P2 on a 3-point rule): a travelling bump, variable
+        # velocity and diffusivity, a source
+        deg = 2 if name == "intervalp2" else 1
+        n = scale or (4096 if deg == 1 else 2048)
+        m = interval_mesh(n, 0.0, 1.0, jitter=0.3, seed=seed, degree=deg)
+        p = m.node_pos
+        T = np.exp(-((p[:, 0] - 0.3) ** 2) / (2 * 0.05 ** 2))
+        U = (0.5 + 0.3 * np.sin(2 * np.pi * p[:, 0]))[:, None]
+        kap = 1e-3 * (1.0 + 0.5 * np.cos(3 * p[:, 0]))
+        return Config(name, m, 2 if deg == 1 else 3, T, U, kap, np.sin(3 * p[:, 0]), None, 0.01, 0.5, 0.0, True,
+                      True, "gmres", 1e-10)
+    if name in ("dsd1d", "dsd1dp2"):
+        # tests/diff_src_dirichlet_1d: -T'' = 2.13 on [0, 10] (dx = 1 in the
+        # reference), T(10) = 2 (surface id 2), zero flux at x = 0, no mass
+        # or advection terms, theta = 1, dt = 5; the P1 (and P2) solution is
+        # nodally exact: T = 2.13/2 (100 - x^2) + 2
+        deg = 2 if name == "dsd1dp2" else 1
+        n = scale or 10
+        m = interval_mesh(n, 0.0, 10.0, degree=deg)
+        m.boundary = np.array([m.nnodes if deg == 1 else 2 * n + 1], np.int32)
+        return Config(name, m, 2 if deg == 1 else 3, np.zeros(m.nnodes), None, 1.0, np.full(m.nnodes, 2.13), None,
+                      5.0, 1.0, 0.0, False, False, "gmres", 1e-10)
     if name == "cube1":
         # Fluidity's small tetrahedral test mesh (tests/data/cube.1.msh: the
         # cube [-3, 3]^3 in 12 tetrahedra around one interior node), with

@@ -1,0 +1,116 @@
+"""1-D (interval) meshes on the GPU, through the C ABI, against the oracle.
+
+* Assembly of the interval configurations (P1 and P2, every term: mass,
+  advection, variable diffusivity, source), the option variants (lumped
+  mass, advection by parts, beta, absorption) and the stabilised
+  assembly (SUPG, streamline upwind), FP64 to 1e-12 relative and FP32 to
+  1e-5; the sparsity bit for bit.  At the interval configuration's stated
+  size (4096 P1 elements).
+* The reference's known-answer case tests/diff_src_dirichlet_1d
+  (diff_src_dirichlet_1d_cg_p1_A.flml) as one solve_field_equation_cg step
+  on the GPU: GMRES + sor to 1e-10 as the flml asks (and CG + jacobi),
+  nodal error below 1e-8 (the test's own threshold), P1 and P2.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from fluidity_amd import cg, meshgen as M
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def assemble(cfg, variant, precision="fp64"):
+    pr = cg.problem_from_config(cfg)
+    opts = cg.options_from_config(cfg, precision=precision)
+    for k, v in variant.items():
+        setattr(opts, k, v)
+    cg.assemble_advection_diffusion_cg(pr, opts, cfg.dt)
+    cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
+    A, rhs, inact = pr.matrix()
+    sp = pr.sparsity
+    pr.close()
+    return A, rhs, inact, sp
+
+
+CASES = [
+    ("interval", None, {}, "fp64"),
+    ("intervalp2", None, {}, "fp64"),
+    ("interval", 64, {"lump_mass_matrix": True}, "fp64"),
+    ("interval", 64, {"integrate_advection_by_parts": True, "beta": 0.3}, "fp64"),
+    ("intervalp2", 32, {"beta": 1.0, "have_absorption": True}, "fp64"),
+    ("interval", 64, {"stabilisation": "supg"}, "fp64"),
+    ("interval", 64, {"stabilisation": "streamline_upwind", "nu_bar_scheme": "doubly_asymptotic"}, "fp64"),
+    ("intervalp2", 32, {"stabilisation": "supg", "nu_bar_scheme": "critical_rule"}, "fp64"),
+    ("interval", 256, {}, "fp32"),
+    ("intervalp2", 128, {}, "fp32"),
+]
+
+
+@pytest.mark.parametrize("name,scale,variant,precision", CASES,
+                         ids=[f"{c[0]}-{c[1]}-{'-'.join(map(str, c[2].values())) or 'plain'}-{c[3]}" for c in CASES])
+def test_interval_assembly_matches_oracle(require_gpu, name, scale, variant, precision):
+    cfg = M.config(name, seed=7, scale=scale)
+    if variant.get("have_absorption"):
+        cfg.absorption = 0.5 + cfg.mesh.node_pos[:, 0]
+    print(f"\n[case] {name} n={cfg.mesh.nele} {variant} {precision}", flush=True)
+    ref = O.reference_system_opts(cfg, variant)
+    A, rhs, inact, sp = assemble(cfg, variant, precision)
+    f, c, _ = ref["sparsity"]
+    assert np.array_equal(sp.findrm, f) and np.array_equal(sp.colm, c)
+    assert np.array_equal(inact, ref["inactive"])
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    assert rel(A, ref["A"]) < tol
+    # The rhs, dt (s - K T0 - ...), sums terms that cancel for a smooth T0
+    # (more so as h falls): its error is measured against the size of the
+    # terms, max_i sum_j |A_ij| |T0_j|, as well as against the rhs itself.
+    fa = f - 1
+    rows = np.repeat(np.arange(f.size - 1), np.diff(fa))
+    terms = np.bincount(rows, np.abs(ref["A"]) * np.abs(cfg.T[c - 1]), minlength=f.size - 1)
+    scale = max(np.max(np.abs(ref["rhs"])), np.max(terms))
+    assert np.max(np.abs(rhs - ref["rhs"])) / scale < tol
+
+
+# The reference's case is dx = 1 (n = 10) with both tolerances 1e-10.  On
+# finer meshes the system's condition number grows as n^2, so the solve is
+# taken to 1e-13 there for the nodal error to stay below 1e-8.
+@pytest.mark.parametrize("name,n,ksp,pc,rtol", [("dsd1d", 10, "gmres", "sor", 1e-10),
+                                                ("dsd1d", 10, "cg", "jacobi", 1e-10),
+                                                ("dsd1d", 200, "gmres", "sor", 1e-13),
+                                                ("dsd1dp2", 10, "gmres", "sor", 1e-10),
+                                                ("dsd1dp2", 100, "cg", "jacobi", 1e-13)])
+def test_diff_src_dirichlet_1d_known_answer(require_gpu, name, n, ksp, pc, rtol):
+    cfg = M.config(name, scale=n)
+    print(f"\n[case] {name} n={n} {ksp}+{pc} rtol {rtol}", flush=True)
+    pr = cg.problem_from_config(cfg)
+    opts = cg.options_from_config(cfg)
+    so = cg.SolverOptions(ksp, pc, relative_error=rtol, absolute_error=min(rtol, 1e-10), max_iterations=100000)
+    T = cfg.T.copy()
+    its = []
+    for _ in range(2):  # finish_time 10 at dt 5: the second step changes nothing
+        T, k, info = cg.solve_field_equation_cg(pr, opts, so, cfg.dt, bc_nodes=cfg.mesh.boundary, bc_values=2.0)
+        assert info.reason > 0
+        its.append(k)
+        pr.set_fields(T=T)
+    x = cfg.mesh.node_pos[:, 0]
+    err = np.abs(T - (2.13 / 2.0 * (100.0 - x ** 2) + 2.0))
+    print(f"[case] its {its} nodal error {err.max():.2e}", flush=True)
+    assert err.max() < 1e-8
+    pr.close()
+
+
+def test_interval_rejects_face_loop_and_pressure(require_gpu):
+    """The face loop and the pressure path take 2-D and 3-D meshes: an
+    interval mesh gets a clear error, not a wrong result."""
+    from fluidity_amd import lib as L
+    cfg = M.config("interval", seed=1, scale=8)
+    pr = cg.problem_from_config(cfg)
+    one = np.ones(1, np.int32)
+    rc = pr.lib.flgpu_surface_bind(pr.ctx, 1, 1, 1, L.ptr(one), L.ptr(one), L.ptr(one), L.ptr(np.ones(1)),
+                                   L.ptr(np.ones(1)), L.ptr(np.ones(1)))
+    assert rc == 6  # FLGPU_EUNSUPPORTED
+    pr.close()
