@@ -47,6 +47,15 @@ class CGOptions:
     stabilisation: str = "none"          # none | streamline_upwind | supg
     nu_bar_scheme: str = "optimal"       # optimal | doubly_asymptotic | critical_rule | unity
     nu_scale: float = 0.5
+    # the field's equation (prognostic/equation::AdvectionDiffusion,
+    # InternalEnergy or KEpsilon, :484-552): the internal energy's density
+    # theta (the density field's temporal theta when compressible),
+    # compressible (equation_of_state/compressible: the p div(u) term) and
+    # multiphase (more than one prognostic velocity: PhaseVolumeFraction)
+    equation_type: str = "advection_diffusion"  # advection_diffusion | internal_energy | k_epsilon
+    density_theta: float = 1.0
+    compressible: bool = False
+    multiphase: bool = False
 
     def params(self, dt: float) -> L.AdvdifParams:
         return L.AdvdifParams(
@@ -62,7 +71,9 @@ class CGOptions:
             scheme={"auto": L.FLGPU_ASM_AUTO, "colour": L.FLGPU_ASM_COLOUR,
                     "blocked": L.FLGPU_ASM_BLOCKED}[self.scheme],
             stabilisation=L.FLGPU_STAB[self.stabilisation], nu_bar_scheme=L.FLGPU_NU_BAR[self.nu_bar_scheme],
-            nu_bar_scale=self.nu_scale)
+            nu_bar_scale=self.nu_scale, density_theta=self.density_theta,
+            equation_type=L.FLGPU_EQ[self.equation_type], compressible=int(self.compressible),
+            multiphase=int(self.multiphase))
 
 
 # preconditioner names of the solver options (setup_pc_from_options,
@@ -238,7 +249,8 @@ class CGProblem:
         L.check(self.lib.flgpu_field_set(self.ctx, which, const, L.ptr(v)), "flgpu_field_set")
 
     def set_fields(self, T=None, velocity=None, diffusivity=None, source=None, absorption=None,
-                   isotropic: bool = True) -> None:
+                   isotropic: bool = True, density=None, old_density=None, pressure=None,
+                   volume_fraction=None) -> None:
         d = self.mesh.dim
         if T is not None:
             self.T = np.ascontiguousarray(T, dtype=np.float64)
@@ -259,6 +271,10 @@ class CGProblem:
             self.set_field(L.FIELD_SOURCE, source, 1)
         if absorption is not None:
             self.set_field(L.FIELD_ABSORPTION, absorption, 1)
+        for which, v in ((L.FIELD_DENSITY, density), (L.FIELD_OLD_DENSITY, old_density),
+                         (L.FIELD_PRESSURE, pressure), (L.FIELD_VOLUME_FRACTION, volume_fraction)):
+            if v is not None:
+                self.set_field(which, v, 1)
 
     # -- system --------------------------------------------------------
     def matrix(self):

@@ -288,8 +288,9 @@ static int launch_colour_stab(flgpu_ctx* c, const AsmFields& fd, const AsmFlags&
 
 // One launch per colour; thread = (element of the colour, local row i).
 // Elements of one colour share no node, so the read-modify-writes of a
-// launch never meet.
-template <int DIM, int LOC, typename R>
+// launch never meet.  EQ: the internal-energy / k-epsilon equation types,
+// every row by the general per-point row (the density fields at the points).
+template <int DIM, int LOC, typename R, bool EQ = false>
 __global__ void __launch_bounds__(256)
 k_asm_colour(const uint32_t* __restrict__ ltab, const int32_t* __restrict__ elems, int count,
              const int32_t* __restrict__ conn, const int32_t* __restrict__ conn_x,
@@ -305,7 +306,10 @@ k_asm_colour(const uint32_t* __restrict__ ltab, const int32_t* __restrict__ elem
     Elem<DIM, LOC, R> e;
     gather_elem<DIM, LOC, R>(ele, conn, conn_x, X4, fd.T, e);
     R vals[LOC], r;
-    row_contrib<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
+    if constexpr (EQ)
+        row_general<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
+    else
+        row_contrib<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
     const int row = conn[(int64_t)ele * LOC + i];  // (not e.nodes[i]: a run-time index would put e in scratch)
 #pragma unroll
     for (int j = 0; j < LOC; j++) {
@@ -321,7 +325,7 @@ k_asm_colour(const uint32_t* __restrict__ ltab, const int32_t* __restrict__ elem
     rhs[row] = rhs[row] + (double)r;
 }
 
-template <int DIM, int LOC, typename R>
+template <int DIM, int LOC, typename R, bool EQ = false>
 static int launch_colour(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
     FLGPU_TRY(ensure_entry_positions(c));
     const uint32_t* img = nullptr;
@@ -329,12 +333,17 @@ static int launch_colour(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, 
     for (int k = 0; k < c->ncol; k++) {
         const int off = c->h_col_ptr[k], cnt = c->h_col_ptr[k + 1] - off;
         if (cnt == 0) continue;
-        hipLaunchKernelGGL((k_asm_colour<DIM, LOC, R>), dim3(grid_for((int64_t)cnt * LOC, 256)), dim3(256),
+        hipLaunchKernelGGL((k_asm_colour<DIM, LOC, R, EQ>), dim3(grid_for((int64_t)cnt * LOC, 256)), dim3(256),
                            0, c->stream, img, c->col_elems.p + off, cnt, c->conn.p, c->conn_x.p, c->X4.p, fd, fl,
                            c->plan.epos.p, c->A.p, c->rhs.p, d_err);
     }
     FLGPU_HIP(hipGetLastError());
     return FLGPU_OK;
+}
+
+template <int DIM, int LOC, typename R>
+static int launch_colour_eq(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
+    return launch_colour<DIM, LOC, R, true>(c, fd, fl, d_err);
 }
 
 // Quadratic T: the fields' values at the quadrature points of one element,
@@ -1841,12 +1850,37 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
             return set_error(FLGPU_EINVAL, "flgpu assemble: nu_bar scale %g < 0", fl.nu_scale);
         if (!c->fU.set) return set_error(FLGPU_EINVAL, "flgpu assemble: stabilisation needs the velocity field");
     }
+    // the equation type (Advection_Diffusion_CG.F90:484-552)
+    fl.eq = p->equation_type;
+    if (fl.eq < FLGPU_EQ_ADVECTION_DIFFUSION || fl.eq > FLGPU_EQ_KEPSILON)
+        return set_error(FLGPU_EINVAL, "flgpu assemble: equation type %d", fl.eq);
+    fl.rho_theta = p->density_theta;
+    fl.comp = fl.eq == FLGPU_EQ_INTERNAL_ENERGY && p->compressible != 0;
+    fl.mph = fl.eq == FLGPU_EQ_INTERNAL_ENERGY && p->multiphase != 0;
+    if (fl.eq) {
+        if (!c->fRho.set) return set_error(FLGPU_EINVAL, "flgpu assemble: density not set");
+        if (fl.eq == FLGPU_EQ_INTERNAL_ENERGY && !c->fRhoOld.set)
+            return set_error(FLGPU_EINVAL, "flgpu assemble: old density not set");
+        if (fl.comp && (!c->fP.set || !c->fU.set))
+            return set_error(FLGPU_EINVAL, "flgpu assemble: the p div(u) term needs pressure and velocity");
+        if (fl.mph && !c->fV.set) return set_error(FLGPU_EINVAL, "flgpu assemble: phase volume fraction not set");
+        if (fl.stab)
+            return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: stabilisation with the internal-energy or "
+                                                 "k-epsilon equation");
+        if (p->scheme == FLGPU_ASM_BLOCKED)
+            return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: the internal-energy and k-epsilon equations "
+                                                 "take the colour scheme");
+    }
     AsmFields fd;
     fd.T = c->fT.v.p;
     fd.U = c->fU.v.p; fd.u_c = c->fU.constant;
     fd.K = c->fK.v.p; fd.k_c = c->fK.constant;
     fd.S = c->fS.v.p; fd.s_c = c->fS.constant;
     fd.B = c->fA.v.p; fd.b_c = c->fA.constant;
+    fd.Rho = c->fRho.v.p; fd.rho_c = c->fRho.constant;
+    fd.RhoOld = c->fRhoOld.v.p; fd.rho_old_c = c->fRhoOld.constant;
+    fd.P = c->fP.v.p; fd.p_c = c->fP.constant;
+    fd.V = c->fV.v.p; fd.v_c = c->fV.constant;
 
     const int dim = c->dim, loc = c->loc;
     const bool fp32 = p->precision == FLGPU_FP32;
@@ -1859,7 +1893,7 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     // colour scheme asked for by name needs no plan.
     // intervals (dim 1) take the colour scheme: a row has at most three
     // entries, too few for the slab staging to pay
-    bool blocked_ok = c->linear && dim >= 2 && scheme != FLGPU_ASM_COLOUR;
+    bool blocked_ok = c->linear && dim >= 2 && !fl.eq && scheme != FLGPU_ASM_COLOUR;
     if (blocked_ok && (fp32 || fl.stab || scheme == FLGPU_ASM_AUTO)) {
         const int rc = build_block_plan(c);
         if (rc == FLGPU_EUNSUPPORTED && scheme == FLGPU_ASM_AUTO)
@@ -1872,7 +1906,7 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
             blocked_ok = stageable(c, fl);
     }
     // quadratic T, unstabilised: the two-pass scheme is the blocked one
-    const bool two_pass = !c->linear && dim >= 2 && !fl.stab && scheme != FLGPU_ASM_COLOUR;
+    const bool two_pass = !c->linear && dim >= 2 && !fl.stab && !fl.eq && scheme != FLGPU_ASM_COLOUR;
     if (scheme == FLGPU_ASM_BLOCKED && !blocked_ok && !two_pass)
         return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: the blocked scheme covers linear T (in FP32 and with "
                                              "stabilisation: with standard P1 tables and the field on the "
@@ -1917,6 +1951,17 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
         FL_DISPATCH(2, 6, launch_colour_stab)
         FL_DISPATCH(3, 4, launch_colour_stab)
         FL_DISPATCH(3, 10, launch_colour_stab) {
+            return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: dim %d loc %d", dim, loc);
+        }
+        return rc;
+    }
+    if (fl.eq) {
+        FL_DISPATCH(1, 2, launch_colour_eq)
+        FL_DISPATCH(1, 3, launch_colour_eq)
+        FL_DISPATCH(2, 3, launch_colour_eq)
+        FL_DISPATCH(2, 6, launch_colour_eq)
+        FL_DISPATCH(3, 4, launch_colour_eq)
+        FL_DISPATCH(3, 10, launch_colour_eq) {
             return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: dim %d loc %d", dim, loc);
         }
         return rc;

@@ -38,6 +38,12 @@ struct AsmFlags {
     int stab;        // 0 none, 1 streamline upwind, 2 SUPG
     int nu_scheme;   // nu_bar: 1 optimal, 2 doubly asymptotic, 3 critical rule, 4 unity
     double nu_scale;
+    // equation type (Advection_Diffusion_CG.F90:484-552): 0 advection-
+    // diffusion, 1 internal energy, 2 k-epsilon; mph: internal energy with
+    // the phase volume fraction; comp: its p div(u) term; rho_theta: the
+    // internal energy's density theta
+    int eq = 0, mph = 0, comp = 0;
+    double rho_theta = 1.0;
 };
 
 // Field pointers for the kernels (constant fields point at one value).
@@ -47,6 +53,11 @@ struct AsmFields {
     const double* K; int k_c;
     const double* S; int s_c;
     const double* B; int b_c;
+    // density, old density, pressure, phase volume fraction (scalars)
+    const double* Rho = nullptr; int rho_c = 0;
+    const double* RhoOld = nullptr; int rho_old_c = 0;
+    const double* P = nullptr; int p_c = 0;
+    const double* V = nullptr; int v_c = 0;
 };
 
 // Compile-time feature sets: a kernel instantiated for a subset of terms
@@ -156,6 +167,10 @@ struct GlobalNodal {
     FLGPU_HD_INLINE R K(int k, int comp) const { return fld<R>(fd.K, fd.k_c, nodes[k], comp, DIM * DIM); }
     FLGPU_HD_INLINE R S(int k) const { return fld<R>(fd.S, fd.s_c, nodes[k], 0, 1); }
     FLGPU_HD_INLINE R B(int k) const { return fld<R>(fd.B, fd.b_c, nodes[k], 0, 1); }
+    FLGPU_HD_INLINE R Rho(int k) const { return fld<R>(fd.Rho, fd.rho_c, nodes[k], 0, 1); }
+    FLGPU_HD_INLINE R RhoOld(int k) const { return fld<R>(fd.RhoOld, fd.rho_old_c, nodes[k], 0, 1); }
+    FLGPU_HD_INLINE R P(int k) const { return fld<R>(fd.P, fd.p_c, nodes[k], 0, 1); }
+    FLGPU_HD_INLINE R V(int k) const { return fld<R>(fd.V, fd.v_c, nodes[k], 0, 1); }
 };
 
 // Geometry of a linear element: physical gradients G[n][d] = sum_k
@@ -472,6 +487,50 @@ struct NodalQuad {
         for (int k = 0; k < LOC; k++) fq = FMA(na.S(k), tb.n[k + LOC * g], fq);
         return fq;
     }
+    // a scalar field of the internal-energy / k-epsilon equations at g
+    // (ele_val_at_quad), and its gradient there (ele_grad_at_quad_scalar,
+    // Fields_Base.F90:2494-2510, with the field on T's mesh): the nodal
+    // values by F (a GlobalNodal method)
+    template <class F>
+    FLGPU_HD_INLINE R at(int g, F f) const {
+        R q = (R)0;
+#pragma unroll
+        for (int k = 0; k < LOC; k++) q = FMA(f(k), tb.n[k + LOC * g], q);
+        return q;
+    }
+    template <class F>
+    FLGPU_HD_INLINE void grad(int g, F f, R (&gq)[DIM]) const {
+        const int ngi = tb.ngi;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) gq[d] = (R)0;
+#pragma unroll
+        for (int k = 0; k < LOC; k++) {
+            const R v = f(k);
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                R s = (R)0;
+#pragma unroll
+                for (int q = 0; q < DIM; q++) s = FMA(tb.dn[k + LOC * (g + ngi * q)], iJ[d][q], s);
+                gq[d] = FMA(v, s, gq[d]);
+            }
+        }
+    }
+    FLGPU_HD_INLINE R rho(int g) const { return at(g, [&](int k) { return na.Rho(k); }); }
+    FLGPU_HD_INLINE R rho_old(int g) const { return at(g, [&](int k) { return na.RhoOld(k); }); }
+    FLGPU_HD_INLINE R pres(int g) const { return at(g, [&](int k) { return na.P(k); }); }
+    FLGPU_HD_INLINE R nv(int g) const { return at(g, [&](int k) { return na.V(k); }); }
+    // the gradient of the advected density: theta-weighted between the
+    // density and the old density (internal energy), the density (k-epsilon)
+    FLGPU_HD_INLINE void grad_rho(int g, int eq, R th, R (&gq)[DIM]) const {
+        grad(g, [&](int k) { return na.Rho(k); }, gq);
+        if (eq == 1) {
+            R go[DIM];
+            grad(g, [&](int k) { return na.RhoOld(k); }, go);
+#pragma unroll
+            for (int d = 0; d < DIM; d++) gq[d] = th * gq[d] + ((R)1 - th) * go[d];
+        }
+    }
+    FLGPU_HD_INLINE void grad_nv(int g, R (&gq)[DIM]) const { grad(g, [&](int k) { return na.V(k); }, gq); }
     // the stabilisation's nu_bar at g, from u and the diffusivity there
     FLGPU_HD_INLINE R nu(int g, const R (&uq)[DIM], const R (&JT)[DIM][DIM], const R (&kt)[DIM][DIM],
                          const AsmFlags& fl) const {
@@ -497,7 +556,7 @@ FLGPU_HD_INLINE void row_general_q(const LTab<R>& tb, const AsmFlags& fl, const 
         M[j] = (R)0;
         Sm[j] = (R)0;
     }
-    R src_i = (R)0;
+    R src_i = (R)0, pdv_i = (R)0;
 #pragma unroll 1
     for (int g = 0; g < ngi; g++) {
         const R dw = adet * tb.w[g];
@@ -527,6 +586,52 @@ FLGPU_HD_INLINE void row_general_q(const LTab<R>& tb, const AsmFlags& fl, const 
         }
         if (fl.absorp) sq = qv.absorp(g);
         if (fl.src) src_i = FMA(ni, dw * qv.src(g), src_i);
+        // The internal-energy and k-epsilon equations (:884-903, :983-1097,
+        // :1181-1190, :1204-1226): the mass weighted by the old density
+        // (internal energy) or the density (k-epsilon), the advection by the
+        // (theta-weighted) density, times the phase volume fraction when
+        // multiphase; the div term's coefficient div(density u) split as
+        // density div(u) + u.grad(density) (and + density u.grad(nvfrac));
+        // the diffusion times nvfrac (multiphase); the rhs' -p div(u).
+        R dwm = dw, ca = (R)1, cdiv = div, dk = dw;
+        if (fl.eq) {
+            const R th = (R)fl.rho_theta;
+            const R rq = qv.rho(g);
+            R rb = rq;
+            R nvq = (R)1;
+            if (fl.mph) nvq = qv.nv(g);
+            if (fl.eq == 1) {
+                const R ro = qv.rho_old(g);
+                rb = th * rq + ((R)1 - th) * ro;
+                dwm = fl.mph ? dw * ro * nvq : dw * ro;
+            } else {
+                dwm = dw * rq;
+            }
+            ca = fl.mph ? rb * nvq : rb;
+            if (fl.adv && fl.need_div) {
+                R gr[DIM];
+                qv.grad_rho(g, fl.eq, th, gr);
+                R udgr = (R)0;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) udgr = FMA(gr[d], uq[d], udgr);
+                if (fl.mph) {
+                    R gn[DIM];
+                    qv.grad_nv(g, gn);
+                    R udgn = (R)0;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) udgn = FMA(gn[d], uq[d], udgn);
+                    cdiv = div * rb * nvq + nvq * udgr + rb * udgn;
+                } else {
+                    cdiv = div * rb + udgr;
+                }
+            }
+            if (fl.mph) dk = dw * nvq;
+            if (fl.comp) {
+                R pd = qv.div(g) * qv.pres(g);
+                if (fl.mph) pd = pd * nvq;
+                pdv_i = FMA(ni, dw * pd, pdv_i);
+            }
+        }
         R ug_i = (R)0;
         if (fl.adv && fl.ibp) {
 #pragma unroll
@@ -552,7 +657,7 @@ FLGPU_HD_INLINE void row_general_q(const LTab<R>& tb, const AsmFlags& fl, const 
 #endif
             const R nj = tb.n[j + LOC * g];
             const R nn = ni * nj;
-            if (fl.have_mass) M[j] = FMA(nn, dw, M[j]);
+            if (fl.have_mass) M[j] = FMA(nn, dwm, M[j]);
             if (fl.adv || fl.diff) {
                 R Gj[DIM];
 #pragma unroll
@@ -569,10 +674,12 @@ FLGPU_HD_INLINE void row_general_q(const LTab<R>& tb, const AsmFlags& fl, const 
 #pragma unroll
                         for (int d = 0; d < DIM; d++) t = FMA(Gj[d], uq[d], t);
                         a = ni * t;
-                        if (fl.need_div) a = FMA((R)fl.beta * nn, div, a);
+                        if (fl.eq) a = a * ca;
+                        if (fl.need_div) a = FMA((R)fl.beta * nn, cdiv, a);
                     } else {
                         a = -(ug_i * nj);
-                        if (fl.need_div) a = FMA(-((R)1 - (R)fl.beta) * nn, div, a);
+                        if (fl.eq) a = a * ca;
+                        if (fl.need_div) a = FMA(-((R)1 - (R)fl.beta) * nn, cdiv, a);
                     }
                     Sm[j] = FMA(a, dw, Sm[j]);
                 }
@@ -581,7 +688,7 @@ FLGPU_HD_INLINE void row_general_q(const LTab<R>& tb, const AsmFlags& fl, const 
                     if (fl.iso) {
 #pragma unroll
                         for (int d = 0; d < DIM; d++) t = FMA(Gi[d], Gj[d], t);
-                        Sm[j] = FMA(t, dw * kq, Sm[j]);
+                        Sm[j] = FMA(t, dk * kq, Sm[j]);
                     } else {
 #pragma unroll
                         for (int d = 0; d < DIM; d++) t = FMA(vK[d], Gj[d], t);
@@ -596,6 +703,7 @@ FLGPU_HD_INLINE void row_general_q(const LTab<R>& tb, const AsmFlags& fl, const 
 #pragma unroll
     for (int j = 0; j < LOC; j++) r = FMA(-Sm[j], T[j], r);
     if (fl.src) r = r + src_i;
+    if (fl.comp) r = r - pdv_i;
     rhs_i = r;
     R lump = (R)0;
     if (fl.have_mass && fl.lump) {

@@ -477,6 +477,10 @@ int flgpu_field_set(flgpu_ctx* c, int32_t which, int32_t constant, const double*
             case FLGPU_FIELD_DIFFUSIVITY: f = &c->fK; ncomp = c->dim * c->dim; break;
             case FLGPU_FIELD_SOURCE: f = &c->fS; ncomp = 1; break;
             case FLGPU_FIELD_ABSORPTION: f = &c->fA; ncomp = 1; break;
+            case FLGPU_FIELD_DENSITY: f = &c->fRho; ncomp = 1; break;
+            case FLGPU_FIELD_OLD_DENSITY: f = &c->fRhoOld; ncomp = 1; break;
+            case FLGPU_FIELD_PRESSURE: f = &c->fP; ncomp = 1; break;
+            case FLGPU_FIELD_VOLUME_FRACTION: f = &c->fV; ncomp = 1; break;
             default: return set_error(FLGPU_EINVAL, "flgpu_field_set: unknown field %d", which);
         }
         if (which == FLGPU_FIELD_T && constant)

@@ -239,6 +239,7 @@ struct flgpu_ctx {
 
     // fields
     flgpu::DevField fT, fU, fK, fS, fA;
+    flgpu::DevField fRho, fRhoOld, fP, fV;  // density, old density, pressure, phase volume fraction
 
     // assembled system
     flgpu::DevBuf<double> A, rhs;

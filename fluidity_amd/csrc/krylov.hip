@@ -368,6 +368,16 @@ k_cg_p(int n, const V* __restrict__ z, V* __restrict__ p, const double* __restri
        const double* __restrict__ ppc, int npc, const Scal* __restrict__ in, Scal* __restrict__ out) {
     __shared__ double s_beta;
     __shared__ int s_stop;
+    // The thread's first group of rows does not depend on the scalars: its
+    // loads are issued before the done flag's and the partial sums', so the
+    // three are in flight together (one memory round trip, not two).
+    const int ng = n / R, tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+    Rows<V, R> zv = {}, pv = {};
+    auto load = [&](int i) {
+        zv = ld_rows<R>(z, i);
+        pv = ld_rows<R>((const V*)p, i);
+    };
+    if (tid < ng) load(tid * R);
     if (carry_if_done(in, out)) return;
     double u[4], q[3] = {0.0, 0.0, 0.0};
     sum_partials<4>(pvec, nvec, u);
@@ -392,11 +402,9 @@ k_cg_p(int n, const V* __restrict__ z, V* __restrict__ p, const double* __restri
     __syncthreads();
     if (s_stop) return;
     const V beta = (V)s_beta;
-    const int ng = n / R, tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
     for (int g = tid; g < ng; g += nt) {
         const int i = g * R;
-        const Rows<V, R> zv = ld_rows<R>(z, i);
-        Rows<V, R> pv = ld_rows<R>(p, i);
+        if (g != tid) load(i);
 #pragma unroll
         for (int k = 0; k < R; k++) pv[k] = fma(beta, pv[k], zv[k]);
         st_rows<R>(p, i, pv);
@@ -426,6 +434,22 @@ k_cg1_vec(int n, double* __restrict__ x, V* __restrict__ p, V* __restrict__ s, V
           double* __restrict__ pvec, const Scal* __restrict__ in, Scal* __restrict__ out) {
     __shared__ double s_ab[2];
     __shared__ int s_stop;
+    // the first group's loads in flight with the scalars' (as k_cg_p)
+    const int ng_ = n / R, tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+    Rows<V, R> pv = {}, sv = {}, rv = {}, uv = {}, wv = {}, dv = {};
+    Rows<double, R> xv = {};
+    Rows<float, R> fv = {};
+    auto load = [&](int i) {
+        pv = ld_rows<R>((const V*)p, i);
+        sv = ld_rows<R>((const V*)s, i);
+        rv = ld_rows<R>((const V*)r, i);
+        uv = ld_rows<R>((const V*)u, i);
+        wv = ld_rows<R>(w, i);
+        xv = ld_rows<R>((const double*)x, i);
+        if (jacobi) dv = ld_rows<R>(dinv, i);
+        if (dsc) fv = ld_rows<R>(dsc, i);
+    };
+    if (tid < ng_) load(tid * R);
     if (carry_if_done(in, out)) return;
     double dl[1], g[3] = {0.0, 0.0, 0.0}, q[3] = {0.0, 0.0, 0.0};
     sum_partials<1>(pspmv, nspmv, dl);
@@ -481,16 +505,10 @@ k_cg1_vec(int n, double* __restrict__ x, V* __restrict__ p, V* __restrict__ s, V
         }
         s_rr += (double)ri * (double)ri;
     };
-    const int ng_ = n / R, tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
     for (int gi = tid; gi < ng_; gi += nt) {
         const int i = gi * R;
-        Rows<V, R> pv = ld_rows<R>((const V*)p, i), sv = ld_rows<R>((const V*)s, i), rv = ld_rows<R>((const V*)r, i);
-        const Rows<V, R> uv = ld_rows<R>((const V*)u, i), wv = ld_rows<R>(w, i);
-        Rows<double, R> xv = ld_rows<R>((const double*)x, i);
-        Rows<V, R> dv = {}, uo = {}, tv = {};
-        Rows<float, R> fv = {};
-        if (jacobi) dv = ld_rows<R>(dinv, i);
-        if (dsc) fv = ld_rows<R>(dsc, i);
+        if (gi != tid) load(i);
+        Rows<V, R> uo = {}, tv = {};
 #pragma unroll
         for (int k = 0; k < R; k++) {
             double xi = xv[k];

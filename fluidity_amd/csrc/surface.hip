@@ -508,6 +508,11 @@ extern "C" int flgpu_advdif_surface(flgpu_ctx* c, const flgpu_advdif_params* p, 
                                     const double* bc_value, const double* bc_value2) {
     return guarded("flgpu_advdif_surface", [&]() -> int {
         if (!c || !p) return set_error(FLGPU_EINVAL, "flgpu_advdif_surface: null argument");
+        // the internal-energy face advection is weighted by the density
+        // (add_advection_face_cg, :1318-1327), which this face loop does
+        // not stage; k-epsilon faces take the advection-diffusion form there
+        if (p->equation_type == FLGPU_EQ_INTERNAL_ENERGY && p->have_advection && p->integrate_by_parts)
+            return set_error(FLGPU_EUNSUPPORTED, "flgpu_advdif_surface: internal-energy advection by parts");
         FLGPU_HIP(hipSetDevice(c->device));
         krylov_invalidate(c);
         return surface_assemble(c, p, bc_type, bc_value, bc_value2);

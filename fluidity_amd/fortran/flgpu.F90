@@ -27,7 +27,10 @@ module flgpu
                                             FLGPU_PC_SOR = 3, FLGPU_PC_FSAI = 4
   integer(c_int32_t), parameter, public :: FLGPU_NORM_PRECONDITIONED = 0, FLGPU_NORM_UNPRECONDITIONED = 1
   integer(c_int32_t), parameter, public :: FLGPU_FIELD_T = 0, FLGPU_FIELD_VELOCITY = 1, &
-       FLGPU_FIELD_DIFFUSIVITY = 2, FLGPU_FIELD_SOURCE = 3, FLGPU_FIELD_ABSORPTION = 4
+       FLGPU_FIELD_DIFFUSIVITY = 2, FLGPU_FIELD_SOURCE = 3, FLGPU_FIELD_ABSORPTION = 4, &
+       FLGPU_FIELD_DENSITY = 5, FLGPU_FIELD_OLD_DENSITY = 6, FLGPU_FIELD_PRESSURE = 7, &
+       FLGPU_FIELD_VOLUME_FRACTION = 8, &
+       FLGPU_EQ_ADVECTION_DIFFUSION = 0, FLGPU_EQ_INTERNAL_ENERGY = 1, FLGPU_EQ_KEPSILON = 2
   !! stabilisation (Advection_Diffusion_CG.F90:71) and nu_bar schemes (Upwind_Stabilisation.F90:18-19)
   integer(c_int32_t), parameter, public :: FLGPU_STAB_NONE = 0, FLGPU_STAB_STREAMLINE_UPWIND = 1, &
        FLGPU_STAB_SUPG = 2
@@ -48,6 +51,10 @@ module flgpu
      integer(c_int32_t) :: precision, scheme
      integer(c_int32_t) :: stabilisation, nu_bar_scheme
      real(c_double) :: nu_bar_scale
+     !! equation type (FLGPU_EQ_*), the internal energy's density theta,
+     !! its p div(u) term and PhaseVolumeFraction factor (0/1)
+     real(c_double) :: density_theta
+     integer(c_int32_t) :: equation_type, compressible, multiphase
   end type flgpu_advdif_params
 
   !! flgpu_solver_params: the solver options of schemas/solvers.rnc

@@ -20,6 +20,8 @@ FLGPU_KSP_CG, FLGPU_KSP_GMRES, FLGPU_KSP_FCG, FLGPU_KSP_PIPECG = 0, 1, 2, 3
 FLGPU_PC_NONE, FLGPU_PC_JACOBI, FLGPU_PC_LEARNED, FLGPU_PC_SOR, FLGPU_PC_FSAI = 0, 1, 2, 3, 4
 FLGPU_NORM_PRECONDITIONED, FLGPU_NORM_UNPRECONDITIONED = 0, 1
 FIELD_T, FIELD_VELOCITY, FIELD_DIFFUSIVITY, FIELD_SOURCE, FIELD_ABSORPTION = range(5)
+FIELD_DENSITY, FIELD_OLD_DENSITY, FIELD_PRESSURE, FIELD_VOLUME_FRACTION = range(5, 9)
+FLGPU_EQ = {"advection_diffusion": 0, "internal_energy": 1, "k_epsilon": 2}
 BC_NONE, BC_NEUMANN, BC_WEAKDIRICHLET, BC_INTERNAL, BC_ROBIN = range(5)
 FLGPU_STAB = {"none": 0, "streamline_upwind": 1, "supg": 2}
 FLGPU_NU_BAR = {"optimal": 1, "doubly_asymptotic": 2, "critical_rule": 3, "unity": 4}
@@ -50,6 +52,8 @@ class AdvdifParams(C.Structure):
         ("have_source", C.c_int32), ("have_absorption", C.c_int32),
         ("precision", C.c_int32), ("scheme", C.c_int32),
         ("stabilisation", C.c_int32), ("nu_bar_scheme", C.c_int32), ("nu_bar_scale", C.c_double),
+        ("density_theta", C.c_double), ("equation_type", C.c_int32), ("compressible", C.c_int32),
+        ("multiphase", C.c_int32),
     ]
 
 

@@ -1,13 +1,13 @@
 """Standalone launches of the hot kernels, for profiling under rocprofv3.
 
     python -m fluidity_amd.profile_kernels [--config c5] [--reps 3] [--copy-mib 1024]
-                                           [--solves fsai:mixed,learned:mixed,...]
+                                           [--solves fsai:mixed,learned:mixed,jacobi:mixed:pipecg,...]
 
 runs, on the device-resident system of the configuration: `reps` blocked
 assemblies, `reps` colour-scheme assemblies, the solver-matrix SpMV (FP64 and
 FP32 values) `reps` times each, then each listed solve (preconditioner:
-precision, CG with the learned preconditioner's FCG, to a 1e-8 true relative
-residual) once, `pressure` ct_m and C^T M_l^-1 C assemblies on the 3-D P1
+precision[:ksp], by default CG, FCG with the learned preconditioner, to a
+1e-8 true relative residual) once, `pressure` ct_m and C^T M_l^-1 C assemblies on the 3-D P1
 configuration's mesh, and finally one plain device-to-device copy of `copy-mib`
 MiB through the HIP runtime (its blit kernel reads and writes exactly that
 many bytes), which calibrates the FETCH_SIZE / WRITE_SIZE counters.
@@ -48,11 +48,12 @@ def main() -> int:
         from . import learned
         learned.set_params(pr, learned.load())
     for item in filter(None, args.solves.split(",")):
-        pc, prec = item.split(":")
-        so = cg.SolverOptions("fcg" if pc == "learned" else "cg", pc, relative_error=1e-8, norm="unpreconditioned",
+        pc, prec, *ksp = item.split(":")
+        ksp = ksp[0] if ksp else ("fcg" if pc == "learned" else "cg")
+        so = cg.SolverOptions(ksp, pc, relative_error=1e-8, norm="unpreconditioned",
                               max_iterations=50000, precision=prec, check_every=32)
         _, info = cg.petsc_solve(pr, so)
-        print(f"solve {pc} {prec}: {info.its} its, reason {info.reason}", flush=True)
+        print(f"solve {pc} {prec} {ksp}: {info.its} its, reason {info.reason}", flush=True)
     pr.close()
     if args.pressure > 0:
         import numpy as np

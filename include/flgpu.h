@@ -145,7 +145,13 @@ enum {
     FLGPU_FIELD_VELOCITY = 1,   /* vector dim x N */
     FLGPU_FIELD_DIFFUSIVITY = 2,/* tensor dim x dim x N */
     FLGPU_FIELD_SOURCE = 3,     /* scalar */
-    FLGPU_FIELD_ABSORPTION = 4  /* scalar */
+    FLGPU_FIELD_ABSORPTION = 4, /* scalar */
+    /* scalars of the internal-energy and k-epsilon equations
+     * (assemble/Advection_Diffusion_CG.F90:484-552), on the field mesh */
+    FLGPU_FIELD_DENSITY = 5,
+    FLGPU_FIELD_OLD_DENSITY = 6,
+    FLGPU_FIELD_PRESSURE = 7,
+    FLGPU_FIELD_VOLUME_FRACTION = 8 /* PhaseVolumeFraction (nvfrac) */
 };
 
 /* Upload a field to its device mirror.  constant != 0: FIELD_TYPE_CONSTANT,
@@ -174,7 +180,18 @@ typedef struct {
      * same bits as FLGPU_ASM_COLOUR), the colour scheme for any degree. */
     int32_t stabilisation, nu_bar_scheme;
     double nu_bar_scale;
+    /* the equation type (Advection_Diffusion_CG.F90:484-552, FLGPU_EQ_*):
+     * internal energy weights the mass by the old density and the
+     * advection by density_theta*density + (1-density_theta)*old density,
+     * with compressible != 0 adding -p div(u) to the rhs and multiphase != 0
+     * the PhaseVolumeFraction factor (mass, advection, isotropic
+     * diffusion, pressure term); k-epsilon weights mass and advection by
+     * the density.  Both take the colour scheme (FLGPU_ASM_AUTO or
+     * FLGPU_ASM_COLOUR), without stabilisation. */
+    double density_theta;
+    int32_t equation_type, compressible, multiphase;
 } flgpu_advdif_params;
+enum { FLGPU_EQ_ADVECTION_DIFFUSION = 0, FLGPU_EQ_INTERNAL_ENERGY = 1, FLGPU_EQ_KEPSILON = 2 };
 enum { FLGPU_STAB_NONE = 0, FLGPU_STAB_STREAMLINE_UPWIND = 1, FLGPU_STAB_SUPG = 2 };
 enum {
     FLGPU_NU_BAR_OPTIMAL = 1,

@@ -521,6 +521,38 @@ static void elem_geometry(const orc_mesh *m, int ele, double JT[MAXDIM][MAXDIM],
     *detJ_out = detJ;
 }
 
+enum { EQ_ADVECTION_DIFFUSION = 0, EQ_INTERNAL_ENERGY = 1, EQ_KEPSILON = 2 };
+
+/* ele_val_at_quad of a scalar field on T's mesh (constant: one value) */
+static void scalar_at_quad(const orc_mesh *m, const double *f, int fc, const int32_t *en, double *q)
+{
+    for (int g = 0; g < m->ngi; g++) {
+        double s = 0.0;
+        for (int i = 0; i < m->loc; i++) {
+            double v = fc ? f[0] : f[en[i] - 1];
+            s = s + v * m->n[i + m->loc * g];
+        }
+        q[g] = s;
+    }
+}
+
+/* ele_grad_at_quad_scalar (Fields_Base.F90:2494-2510) with the field's
+ * shape = T's (drho_t = dt_t, Advection_Diffusion_CG.F90:794-809):
+ * quad_grad(d,:) = matmul(ele_val, dn(:,:,d)) */
+static void scalar_grad_at_quad(const orc_mesh *m, const double *f, int fc, const int32_t *en,
+                                double dsh[MAXLOC][MAXGI][MAXDIM], double gq[MAXDIM][MAXGI])
+{
+    for (int d = 0; d < m->dim; d++)
+        for (int g = 0; g < m->ngi; g++) {
+            double s = 0.0;
+            for (int i = 0; i < m->loc; i++) {
+                double v = fc ? f[0] : f[en[i] - 1];
+                s = s + v * dsh[i][g][d];
+            }
+            gq[d][g] = s;
+        }
+}
+
 static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
                       double *mat, double *rhs)
 {
@@ -558,6 +590,15 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
     for (int g = 0; g < ngi; g++) detwei[g] = fabs(detJ) * m->w[g];
 
     for (int i = 0; i < loc; i++) Te[i] = m->T[en[i] - 1];
+
+    /* --- the equation type's fields (:484-552, :983-1004) --- */
+    const int eqt = p->equation_type;
+    const int ie = eqt == EQ_INTERNAL_ENERGY, ke = eqt == EQ_KEPSILON;
+    const int mph = ie && p->multiphase;
+    double rhoq[MAXGI], rhooq[MAXGI], nvq[MAXGI];
+    if (ie || ke) scalar_at_quad(m, m->rho, m->rho_const, en, rhoq);
+    if (ie) scalar_at_quad(m, m->rho_old, m->rho_old_const, en, rhooq);
+    if (mph) scalar_at_quad(m, m->nvfrac, m->nvfrac_const, en, nvq);
 
     /* --- test function (:812-825): N, or the SUPG shape
      * N + nu_bar u.grad N (supg_test_function, Upwind_Stabilisation.F90:
@@ -605,11 +646,23 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
 
     /* --- mass (add_mass_element_cg, :867-941) --- */
     if (p->have_mass) {
-        double M[MAXLOC][MAXLOC];
+        /* internal energy: detwei*olddensity[*nvfrac]; k-epsilon:
+         * detwei*density (:884-903) */
+        double wm[MAXGI], M[MAXLOC][MAXLOC];
+        for (int g = 0; g < ngi; g++) {
+            if (mph)
+                wm[g] = (detwei[g] * rhooq[g]) * nvq[g];
+            else if (ie)
+                wm[g] = detwei[g] * rhooq[g];
+            else if (ke)
+                wm[g] = detwei[g] * rhoq[g];
+            else
+                wm[g] = detwei[g];
+        }
         for (int i = 0; i < loc; i++)
             for (int j = 0; j < loc; j++) {
                 double s = 0.0;
-                for (int g = 0; g < ngi; g++) s = s + (TN_(i, g) * N_(j, g)) * detwei[g];
+                for (int g = 0; g < ngi; g++) s = s + (TN_(i, g) * N_(j, g)) * wm[g];
                 M[i][j] = s;
             }
         if (p->lump_mass) {
@@ -639,6 +692,35 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
             }
         int need_div = p->integrate_by_parts ? (fabs(1.0 - p->beta) > eps)
                                              : (fabs(p->beta) > eps);
+        /* density at the points and u.grad(density) (:986-1004): internal
+         * energy theta-weighted between density and old density, k-epsilon
+         * density; wa = detwei*density[*nvfrac] weights the main term */
+        double rbq[MAXGI], udgr[MAXGI], udgnv[MAXGI], wa[MAXGI];
+        if (ie || ke) {
+            double gr[MAXDIM][MAXGI], gro[MAXDIM][MAXGI];
+            scalar_grad_at_quad(m, m->rho, m->rho_const, en, dsh, gr);
+            if (ie) scalar_grad_at_quad(m, m->rho_old, m->rho_old_const, en, dsh, gro);
+            for (int g = 0; g < ngi; g++) {
+                rbq[g] = ie ? p->density_theta * rhoq[g] + (1. - p->density_theta) * rhooq[g] : rhoq[g];
+                double s = 0.0;
+                for (int d = 0; d < dim; d++) {
+                    double gd = ie ? p->density_theta * gr[d][g] + (1. - p->density_theta) * gro[d][g] : gr[d][g];
+                    s = s + gd * uq[d][g];
+                }
+                udgr[g] = s;
+            }
+            if (mph) {
+                double gnv[MAXDIM][MAXGI];
+                scalar_grad_at_quad(m, m->nvfrac, m->nvfrac_const, en, dsh, gnv);
+                for (int g = 0; g < ngi; g++) {
+                    double s = 0.0;
+                    for (int d = 0; d < dim; d++) s = s + gnv[d][g] * uq[d][g];
+                    udgnv[g] = s;
+                }
+            }
+        }
+        for (int g = 0; g < ngi; g++)
+            wa[g] = mph ? (detwei[g] * rbq[g]) * nvq[g] : (ie || ke) ? detwei[g] * rbq[g] : detwei[g];
         if (need_div) { /* ele_div_at_quad, Fields_Base.F90:2557-2575 */
             for (int g = 0; g < ngi; g++) divq[g] = 0.0;
             for (int d = 0; d < dim; d++)
@@ -659,14 +741,14 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
                     for (int g = 0; g < ngi; g++) {
                         double t = 0.0;
                         for (int d = 0; d < dim; d++) t = t + dsh[j][g][d] * uq[d][g];
-                        s = s + (TN_(i, g) * t) * detwei[g];
+                        s = s + (TN_(i, g) * t) * wa[g];
                     }
                 } else {
                     /* -dshape_dot_vector_shape (FETools.F90:700-721) */
                     for (int g = 0; g < ngi; g++) {
                         double t = 0.0;
                         for (int d = 0; d < dim; d++) t = t + dsh[i][g][d] * uq[d][g];
-                        s = s + (t * N_(j, g)) * detwei[g];
+                        s = s + (t * N_(j, g)) * wa[g];
                     }
                     s = -s;
                 }
@@ -674,11 +756,30 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
             }
         if (need_div) {
             double coef = p->integrate_by_parts ? (1.0 - p->beta) : p->beta;
+            /* the div(density u) split (:1024-1033, :1066-1081): one
+             * weight, or for multiphase three shape_shape terms summed */
+            double wd[3][MAXGI];
+            int nd = 1;
+            for (int g = 0; g < ngi; g++) {
+                if (mph) {
+                    wd[0][g] = ((detwei[g] * divq[g]) * rbq[g]) * nvq[g];
+                    wd[1][g] = (detwei[g] * nvq[g]) * udgr[g];
+                    wd[2][g] = (detwei[g] * rbq[g]) * udgnv[g];
+                } else if (ie || ke) {
+                    wd[0][g] = (divq[g] * rbq[g] + udgr[g]) * detwei[g];
+                } else {
+                    wd[0][g] = divq[g] * detwei[g];
+                }
+            }
+            if (mph) nd = 3;
             for (int i = 0; i < loc; i++)
                 for (int j = 0; j < loc; j++) {
                     double s = 0.0;
-                    for (int g = 0; g < ngi; g++)
-                        s = s + (TN_(i, g) * N_(j, g)) * (divq[g] * detwei[g]);
+                    for (int q = 0; q < nd; q++) {
+                        double sq = 0.0;
+                        for (int g = 0; g < ngi; g++) sq = sq + (TN_(i, g) * N_(j, g)) * wd[q][g];
+                        s = q == 0 ? sq : s + sq;
+                    }
                     if (p->integrate_by_parts)
                         adv[i][j] = adv[i][j] - coef * s;
                     else
@@ -752,8 +853,9 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
             for (int j = 0; j < loc; j++) df[i][j] = 0.0;
         if (p->isotropic_diffusivity) {
             /* dshape_dot_dshape (FETools.F90:391-453): gi outermost */
+            /* multiphase internal energy: times nvfrac (:1181-1190) */
             double wk[MAXGI];
-            for (int g = 0; g < ngi; g++) wk[g] = detwei[g] * kq[0][0][g];
+            for (int g = 0; g < ngi; g++) wk[g] = mph ? (detwei[g] * kq[0][0][g]) * nvq[g] : detwei[g] * kq[0][0][g];
             for (int g = 0; g < ngi; g++)
                 for (int i = 0; i < loc; i++)
                     for (int j = 0; j < loc; j++) {
@@ -809,6 +911,29 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
             double s = 0.0;
             for (int g = 0; g < ngi; g++) s = s + TN_(i, g) * fw[g];
             rhs[i] = rhs[i] + s;
+        }
+    }
+
+    /* --- pressure (add_pressurediv_element_cg, :1204-1226): compressible
+     * internal energy, rhs -= N_i div(u) p detwei [nvfrac] --- */
+    if (ie && p->compressible) {
+        double dv[MAXGI], pq[MAXGI], wp[MAXGI];
+        for (int g = 0; g < ngi; g++) dv[g] = 0.0;
+        for (int d = 0; d < dim; d++)
+            for (int g = 0; g < ngi; g++) {
+                double s = 0.0;
+                for (int i = 0; i < loc; i++) {
+                    double ui = m->u_const ? m->U[d] : m->U[d + dim * (int64_t)(en[i] - 1)];
+                    s = s + ui * dsh[i][g][d];
+                }
+                dv[g] = dv[g] + s;
+            }
+        scalar_at_quad(m, m->pres, m->pres_const, en, pq);
+        for (int g = 0; g < ngi; g++) wp[g] = mph ? ((dv[g] * pq[g]) * detwei[g]) * nvq[g] : (dv[g] * pq[g]) * detwei[g];
+        for (int i = 0; i < loc; i++) {
+            double s = 0.0;
+            for (int g = 0; g < ngi; g++) s = s + TN_(i, g) * wp[g];
+            rhs[i] = rhs[i] - s;
         }
     }
 #undef TN_

@@ -51,6 +51,12 @@ typedef struct {
      * 18-19): 1 optimal, 2 doubly asymptotic, 3 critical rule, 4 unity */
     int32_t stabilisation, nu_bar_scheme;
     double nu_bar_scale;
+    /* equation type (Advection_Diffusion_CG.F90:484-552): 0 advection-
+     * diffusion, 1 internal energy, 2 k-epsilon; density_theta (internal
+     * energy); compressible: the p div(u) term (internal energy);
+     * multiphase: the PhaseVolumeFraction factor (internal energy) */
+    double density_theta;
+    int32_t equation_type, compressible, multiphase;
 } orc_params;
 
 typedef struct {
@@ -70,6 +76,12 @@ typedef struct {
     const double *kappa;  int32_t kappa_const;  /* dim x dim x N */
     const double *src;    int32_t src_const;
     const double *absorp; int32_t absorp_const;
+    /* density, old density, pressure and phase volume fraction (scalars on
+     * the field mesh), read by the internal-energy and k-epsilon equations */
+    const double *rho;    int32_t rho_const;
+    const double *rho_old; int32_t rho_old_const;
+    const double *pres;   int32_t pres_const;
+    const double *nvfrac; int32_t nvfrac_const;
     /* colour sets: ncolours+1 offsets into elems (0-based offsets,
      * 1-based element ids, ascending inside each colour) */
     int32_t ncolours;

@@ -23,7 +23,8 @@ class OrcParams(C.Structure):
                 ("integrate_by_parts", C.c_int32), ("have_diffusivity", C.c_int32),
                 ("isotropic_diffusivity", C.c_int32), ("have_source", C.c_int32),
                 ("have_absorption", C.c_int32), ("stabilisation", C.c_int32), ("nu_bar_scheme", C.c_int32),
-                ("nu_bar_scale", C.c_double)]
+                ("nu_bar_scale", C.c_double), ("density_theta", C.c_double), ("equation_type", C.c_int32),
+                ("compressible", C.c_int32), ("multiphase", C.c_int32)]
 
 
 class OrcMesh(C.Structure):
@@ -35,6 +36,8 @@ class OrcMesh(C.Structure):
                 ("kappa", C.c_void_p), ("kappa_const", C.c_int32),
                 ("src", C.c_void_p), ("src_const", C.c_int32),
                 ("absorp", C.c_void_p), ("absorp_const", C.c_int32),
+                ("rho", C.c_void_p), ("rho_const", C.c_int32), ("rho_old", C.c_void_p), ("rho_old_const", C.c_int32),
+                ("pres", C.c_void_p), ("pres_const", C.c_int32), ("nvfrac", C.c_void_p), ("nvfrac_const", C.c_int32),
                 ("ncolours", C.c_int32), ("col_ptr", C.c_void_p), ("col_elems", C.c_void_p),
                 ("nrows", C.c_int32), ("findrm", C.c_void_p), ("colm", C.c_void_p)]
 
@@ -187,7 +190,11 @@ class Assembly:
         for key, ptrname, cname, ncomp in (("U", "U", "u_const", mesh.dim),
                                            ("kappa", "kappa", "kappa_const", mesh.dim * mesh.dim),
                                            ("src", "src", "src_const", 1),
-                                           ("absorp", "absorp", "absorp_const", 1)):
+                                           ("absorp", "absorp", "absorp_const", 1),
+                                           ("rho", "rho", "rho_const", 1),
+                                           ("rho_old", "rho_old", "rho_old_const", 1),
+                                           ("pres", "pres", "pres_const", 1),
+                                           ("nvfrac", "nvfrac", "nvfrac_const", 1)):
             v = fields.get(key)
             if v is None:
                 continue
@@ -257,10 +264,15 @@ STABILISATION = {"none": 0, "streamline_upwind": 1, "supg": 2}
 NU_BAR = {"optimal": 1, "doubly_asymptotic": 2, "critical_rule": 3, "unity": 4}
 
 
+EQUATION = {"advection_diffusion": 0, "internal_energy": 1, "k_epsilon": 2}
+
+
 def params(dt, theta, beta, have_mass=True, lump=False, adv=True, ibp=False, diff=True, iso=True,
-           src=False, absorp=False, stab="none", nu_bar="optimal", nu_scale=0.5) -> OrcParams:
+           src=False, absorp=False, stab="none", nu_bar="optimal", nu_scale=0.5, equation="advection_diffusion",
+           density_theta=1.0, compressible=False, multiphase=False) -> OrcParams:
     return OrcParams(dt, theta, beta, int(have_mass), int(lump), int(adv), int(ibp), int(diff), int(iso),
-                     int(src), int(absorp), STABILISATION[stab], NU_BAR[nu_bar], float(nu_scale))
+                     int(src), int(absorp), STABILISATION[stab], NU_BAR[nu_bar], float(nu_scale),
+                     float(density_theta), EQUATION[equation], int(compressible), int(multiphase))
 
 
 def nu_bar_scaled(u, J, K, scheme="optimal", scale=0.5):
@@ -372,9 +384,11 @@ def reference_system(cfg, nthreads: int = 0, variant: dict | None = None):
             for a in range(d):
                 full[:, a, a] = kap
             kap = full
-    fields = {"T": cfg.T, "U": cfg.U, "kappa": kap, "src": cfg.source, "absorp": cfg.absorption}
-    asm = Assembly(mesh, el, xel, fields, sp, cs)
     v = variant or {}
+    fields = {"T": cfg.T, "U": cfg.U, "kappa": kap, "src": cfg.source, "absorp": cfg.absorption,
+              "rho": v.get("density"), "rho_old": v.get("old_density"), "pres": v.get("pressure"),
+              "nvfrac": v.get("volume_fraction")}
+    asm = Assembly(mesh, el, xel, fields, sp, cs)
     p = params(cfg.dt, cfg.theta, v.get("beta", cfg.beta), have_mass=cfg.have_mass,
                lump=v.get("lump_mass_matrix", False), adv=cfg.have_advection,
                ibp=v.get("integrate_advection_by_parts", False),
@@ -382,7 +396,9 @@ def reference_system(cfg, nthreads: int = 0, variant: dict | None = None):
                src=cfg.source is not None and not v.get("add_source_directly_to_rhs", False),
                absorp=cfg.absorption is not None,
                stab=v.get("stabilisation", "none"), nu_bar=v.get("nu_bar_scheme", "optimal"),
-               nu_scale=v.get("nu_scale", 0.5))
+               nu_scale=v.get("nu_scale", 0.5), equation=v.get("equation_type", "advection_diffusion"),
+               density_theta=v.get("density_theta", 1.0), compressible=v.get("compressible", False),
+               multiphase=v.get("multiphase", False))
     A, rhs, secs = assemble(p, asm, nthreads)
     if cfg.source is not None and v.get("add_source_directly_to_rhs", False):
         rhs = rhs + np.broadcast_to(np.asarray(cfg.source, np.float64), rhs.shape)  # addto(rhs, source), :604
