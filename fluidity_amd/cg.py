@@ -56,6 +56,9 @@ class CGOptions:
     density_theta: float = 1.0
     compressible: bool = False
     multiphase: bool = False
+    # /mesh_adaptivity/mesh_movement (with the mass terms): positions and
+    # grid velocity from CGProblem.set_mesh_motion
+    move_mesh: bool = False
 
     def params(self, dt: float) -> L.AdvdifParams:
         return L.AdvdifParams(
@@ -73,7 +76,7 @@ class CGOptions:
             stabilisation=L.FLGPU_STAB[self.stabilisation], nu_bar_scheme=L.FLGPU_NU_BAR[self.nu_bar_scheme],
             nu_bar_scale=self.nu_scale, density_theta=self.density_theta,
             equation_type=L.FLGPU_EQ[self.equation_type], compressible=int(self.compressible),
-            multiphase=int(self.multiphase))
+            multiphase=int(self.multiphase), move_mesh=int(self.move_mesh))
 
 
 # preconditioner names of the solver options (setup_pc_from_options,
@@ -238,6 +241,13 @@ class CGProblem:
         self.surface = surface
 
     # -- fields --------------------------------------------------------
+    def set_mesh_motion(self, old_positions, new_positions, grid_velocity) -> None:
+        """OldCoordinate, IteratedCoordinate and GridVelocity, (nx, dim)
+        arrays on the coordinate mesh, for assemblies with move_mesh."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in
+                (old_positions, new_positions, grid_velocity)]
+        L.check(self.lib.flgpu_mesh_motion_set(self.ctx, *[L.ptr(a) for a in arrs]), "flgpu_mesh_motion_set")
+
     def set_field(self, which: int, values, ncomp: int) -> None:
         """Nodal array (nnodes,) / (nnodes, ncomp) or a constant."""
         if np.isscalar(values) or np.asarray(values).size == ncomp:

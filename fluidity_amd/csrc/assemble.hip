@@ -306,10 +306,42 @@ k_asm_colour(const uint32_t* __restrict__ ltab, const int32_t* __restrict__ elem
     Elem<DIM, LOC, R> e;
     gather_elem<DIM, LOC, R>(ele, conn, conn_x, X4, fd.T, e);
     R vals[LOC], r;
-    if constexpr (EQ)
-        row_general<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
-    else
+    if constexpr (EQ) {
+        if constexpr (LOC == DIM + 1) {
+            if (fl.move) {
+                // |detJ| of the old and new positions (vertices relative to
+                // the last one, as gather_elem) and the grid velocity at the
+                // vertices
+                MoveData<DIM, R> mv;
+                const int32_t* cx = conn_x + (int64_t)ele * (DIM + 1);
+                for (int w = 0; w < 2; w++) {
+                    const double* Xw = w ? fd.Xn4 : fd.Xo4;
+                    R Xr[DIM][DIM + 1], iJ[DIM][DIM], ad;
+#pragma unroll
+                    for (int nn = 0; nn <= DIM; nn++)
+#pragma unroll
+                        for (int d = 0; d < DIM; d++)
+                            Xr[d][nn] = (R)(Xw[(int64_t)cx[nn] * 4 + d] - Xw[(int64_t)cx[DIM] * 4 + d]);
+                    geometry<DIM, R>(Xr, tb, iJ, ad);
+                    if (w)
+                        mv.adet_new = ad;
+                    else
+                        mv.adet_old = ad;
+                }
+#pragma unroll
+                for (int nn = 0; nn <= DIM; nn++)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) mv.ug[nn][d] = (R)fd.UG4[(int64_t)cx[nn] * 4 + d];
+                row_general<DIM, LOC, R>(e, tb, fl, fd, i, vals, r, &mv);
+            } else {
+                row_general<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
+            }
+        } else {
+            row_general<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
+        }
+    } else {
         row_contrib<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
+    }
     const int row = conn[(int64_t)ele * LOC + i];  // (not e.nodes[i]: a run-time index would put e in scratch)
 #pragma unroll
     for (int j = 0; j < LOC; j++) {
@@ -1857,6 +1889,22 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     fl.rho_theta = p->density_theta;
     fl.comp = fl.eq == FLGPU_EQ_INTERNAL_ENERGY && p->compressible != 0;
     fl.mph = fl.eq == FLGPU_EQ_INTERNAL_ENERGY && p->multiphase != 0;
+    // mesh movement (:421-437: move_mesh needs the mass terms)
+    fl.move = p->move_mesh != 0 && fl.have_mass;
+    fl.dt = p->dt;
+    if (fl.move) {
+        if (c->motion_nx != c->nx || c->motion_dim != c->dim)
+            return set_error(FLGPU_EINVAL, "flgpu assemble: mesh movement needs flgpu_mesh_motion_set for this mesh");
+        if (!c->linear)
+            return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: mesh movement takes linear T");
+        // the reference stops here too (:447, :462, :500, :531)
+        if (fl.stab || fl.eq)
+            return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: mesh movement with stabilisation or the "
+                                                 "internal-energy / k-epsilon equations");
+        if (p->scheme == FLGPU_ASM_BLOCKED)
+            return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: mesh movement takes the colour scheme");
+        if (!(p->dt > 0.0)) return set_error(FLGPU_EINVAL, "flgpu assemble: mesh movement needs dt > 0");
+    }
     if (fl.eq) {
         if (!c->fRho.set) return set_error(FLGPU_EINVAL, "flgpu assemble: density not set");
         if (fl.eq == FLGPU_EQ_INTERNAL_ENERGY && !c->fRhoOld.set)
@@ -1881,6 +1929,9 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     fd.RhoOld = c->fRhoOld.v.p; fd.rho_old_c = c->fRhoOld.constant;
     fd.P = c->fP.v.p; fd.p_c = c->fP.constant;
     fd.V = c->fV.v.p; fd.v_c = c->fV.constant;
+    fd.Xo4 = c->Xo4.p;
+    fd.Xn4 = c->Xn4.p;
+    fd.UG4 = c->UG4.p;
 
     const int dim = c->dim, loc = c->loc;
     const bool fp32 = p->precision == FLGPU_FP32;
@@ -1893,7 +1944,7 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     // colour scheme asked for by name needs no plan.
     // intervals (dim 1) take the colour scheme: a row has at most three
     // entries, too few for the slab staging to pay
-    bool blocked_ok = c->linear && dim >= 2 && !fl.eq && scheme != FLGPU_ASM_COLOUR;
+    bool blocked_ok = c->linear && dim >= 2 && !fl.eq && !fl.move && scheme != FLGPU_ASM_COLOUR;
     if (blocked_ok && (fp32 || fl.stab || scheme == FLGPU_ASM_AUTO)) {
         const int rc = build_block_plan(c);
         if (rc == FLGPU_EUNSUPPORTED && scheme == FLGPU_ASM_AUTO)
@@ -1955,7 +2006,7 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
         }
         return rc;
     }
-    if (fl.eq) {
+    if (fl.eq || fl.move) {
         FL_DISPATCH(1, 2, launch_colour_eq)
         FL_DISPATCH(1, 3, launch_colour_eq)
         FL_DISPATCH(2, 3, launch_colour_eq)

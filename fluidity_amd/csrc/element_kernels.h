@@ -44,6 +44,20 @@ struct AsmFlags {
     // internal energy's density theta
     int eq = 0, mph = 0, comp = 0;
     double rho_theta = 1.0;
+    // mesh movement (:421-437, with mass): the mass at the new positions,
+    // the (N^{n+1} - N^n) T / dt rhs term, the velocity relative to the
+    // grid and the grid velocity's divergence term; dt is local_dt
+    int move = 0;
+    double dt = 0.0;
+};
+
+// An element's mesh-movement data: |detJ| of its old and new positions and
+// the grid velocity at its vertices (linear T: the grid velocity's element
+// is T's).
+template <int DIM, typename R>
+struct MoveData {
+    R adet_old, adet_new;
+    R ug[DIM + 1][DIM];
 };
 
 // Field pointers for the kernels (constant fields point at one value).
@@ -58,6 +72,11 @@ struct AsmFields {
     const double* RhoOld = nullptr; int rho_old_c = 0;
     const double* P = nullptr; int p_c = 0;
     const double* V = nullptr; int v_c = 0;
+    // mesh movement: old and new positions and the grid velocity, 4 doubles
+    // per coordinate node (as the positions)
+    const double* Xo4 = nullptr;
+    const double* Xn4 = nullptr;
+    const double* UG4 = nullptr;
 };
 
 // Compile-time feature sets: a kernel instantiated for a subset of terms
@@ -547,14 +566,16 @@ struct NodalQuad {
 // column instead of four, which is what keeps a P2 row in registers.
 template <int DIM, int LOC, typename R, class QV, class TV>
 FLGPU_HD_INLINE void row_general_q(const LTab<R>& tb, const AsmFlags& fl, const R (&iJ)[DIM][DIM], R adet,
-                                   const QV& qv, const TV& T, int i, R (&vals)[LOC], R& rhs_i) {
+                                   const QV& qv, const TV& T, int i, R (&vals)[LOC], R& rhs_i,
+                                   const MoveData<DIM, R>* mv = nullptr) {
     const int ngi = tb.ngi;
     const R dtt = (R)fl.dt_theta;
-    R M[LOC], Sm[LOC];
+    R M[LOC], Sm[LOC], Mv[LOC];
 #pragma unroll
     for (int j = 0; j < LOC; j++) {
         M[j] = (R)0;
         Sm[j] = (R)0;
+        Mv[j] = (R)0;
     }
     R src_i = (R)0, pdv_i = (R)0;
 #pragma unroll 1
@@ -571,10 +592,31 @@ FLGPU_HD_INLINE void row_general_q(const LTab<R>& tb, const AsmFlags& fl, const 
             Gi[d] = s;
         }
         R uq[DIM] = {};
-        R div = (R)0;
+        R div = (R)0, div_ug = (R)0;
         if (fl.adv) {
             qv.u(g, uq);
             if (fl.need_div) div = qv.div(g);
+        }
+        if (mv && fl.adv) {
+            // the velocity relative to the grid (:983-985) and, not by
+            // parts, div(u_grid) (:1099-1102), from the vertex values
+#pragma unroll
+            for (int k = 0; k < LOC; k++) {
+                const R nk = tb.n[k + LOC * g];
+#pragma unroll
+                for (int d = 0; d < DIM; d++) uq[d] = uq[d] - mv->ug[k][d] * nk;
+            }
+            if (!fl.ibp) {
+#pragma unroll
+                for (int k = 0; k < LOC; k++)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) {
+                        R s = (R)0;
+#pragma unroll
+                        for (int q = 0; q < DIM; q++) s = FMA(tb.dn[k + LOC * (g + ngi * q)], iJ[d][q], s);
+                        div_ug = FMA(mv->ug[k][d], s, div_ug);
+                    }
+            }
         }
         R kq = (R)0, sq = (R)0;
         R kt[DIM][DIM];
@@ -593,7 +635,11 @@ FLGPU_HD_INLINE void row_general_q(const LTab<R>& tb, const AsmFlags& fl, const 
         // multiphase; the div term's coefficient div(density u) split as
         // density div(u) + u.grad(density) (and + density u.grad(nvfrac));
         // the diffusion times nvfrac (multiphase); the rhs' -p div(u).
-        R dwm = dw, ca = (R)1, cdiv = div, dk = dw;
+        R dwm = dw, ca = (R)1, cdiv = div, dk = dw, dwv = (R)0;
+        if (mv) {
+            dwm = mv->adet_new * tb.w[g];
+            dwv = dwm - mv->adet_old * tb.w[g];
+        }
         if (fl.eq) {
             const R th = (R)fl.rho_theta;
             const R rq = qv.rho(g);
@@ -658,6 +704,7 @@ FLGPU_HD_INLINE void row_general_q(const LTab<R>& tb, const AsmFlags& fl, const 
             const R nj = tb.n[j + LOC * g];
             const R nn = ni * nj;
             if (fl.have_mass) M[j] = FMA(nn, dwm, M[j]);
+            if (mv) Mv[j] = FMA(nn, dwv, Mv[j]);
             if (fl.adv || fl.diff) {
                 R Gj[DIM];
 #pragma unroll
@@ -676,6 +723,7 @@ FLGPU_HD_INLINE void row_general_q(const LTab<R>& tb, const AsmFlags& fl, const 
                         a = ni * t;
                         if (fl.eq) a = a * ca;
                         if (fl.need_div) a = FMA((R)fl.beta * nn, cdiv, a);
+                        if (mv) a = FMA(-nn, div_ug, a);
                     } else {
                         a = -(ug_i * nj);
                         if (fl.eq) a = a * ca;
@@ -704,6 +752,18 @@ FLGPU_HD_INLINE void row_general_q(const LTab<R>& tb, const AsmFlags& fl, const 
     for (int j = 0; j < LOC; j++) r = FMA(-Sm[j], T[j], r);
     if (fl.src) r = r + src_i;
     if (fl.comp) r = r - pdv_i;
+    if (mv) {  // - (N^{n+1} - N^n) T / dt (:922-938)
+        R s = (R)0;
+        if (fl.lump) {
+#pragma unroll
+            for (int j = 0; j < LOC; j++) s += Mv[j];
+            s = s * T[i];
+        } else {
+#pragma unroll
+            for (int j = 0; j < LOC; j++) s = FMA(Mv[j], T[j], s);
+        }
+        r = r - s / (R)fl.dt;
+    }
     rhs_i = r;
     R lump = (R)0;
     if (fl.have_mass && fl.lump) {
@@ -1117,11 +1177,12 @@ FLGPU_HD_INLINE void row_stab(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, con
 
 template <int DIM, int LOC, typename R>
 FLGPU_HD void row_general(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, const AsmFlags& fl,
-                            const AsmFields& fd, int i, R (&vals)[LOC], R& rhs_i) {
+                            const AsmFields& fd, int i, R (&vals)[LOC], R& rhs_i,
+                            const MoveData<DIM, R>* mv = nullptr) {
     R iJ[DIM][DIM], adet;
     geometry<DIM, R>(e.X, tb, iJ, adet);
     const GlobalNodal<DIM, R> na{fd, e.nodes};
-    row_general_q<DIM, LOC, R>(tb, fl, iJ, adet, NodalQuad<DIM, LOC, R>{na, tb, iJ}, e.T, i, vals, rhs_i);
+    row_general_q<DIM, LOC, R>(tb, fl, iJ, adet, NodalQuad<DIM, LOC, R>{na, tb, iJ}, e.T, i, vals, rhs_i, mv);
 }
 
 template <int DIM, int LOC, typename R>

@@ -60,7 +60,8 @@ static void unbind_from_mesh(flgpu_ctx* c) {
     c->linear = false;
     c->h_conn.clear();
     c->conn.release();
-    for (DevField* f : {&c->fT, &c->fU, &c->fK, &c->fS, &c->fA}) f->set = 0;
+    for (DevField* f : {&c->fT, &c->fU, &c->fK, &c->fS, &c->fA, &c->fRho, &c->fRhoOld, &c->fP, &c->fV}) f->set = 0;
+    c->motion_nx = 0;
     c->rows = 0;
     c->cols = 0;
     c->nnz = 0;
@@ -235,7 +236,7 @@ int flgpu_field_mesh_bind(flgpu_ctx* c, int32_t loc, int32_t ngi, int32_t nnodes
         // A field mesh of other nodes leaves the fields and the sparsity of
         // the previous one unbound; other connectivity, the colouring too.
         if (nnodes != c->nnodes) {
-            for (DevField* f : {&c->fT, &c->fU, &c->fK, &c->fS, &c->fA}) f->set = 0;
+            for (DevField* f : {&c->fT, &c->fU, &c->fK, &c->fS, &c->fA, &c->fRho, &c->fRhoOld, &c->fP, &c->fV}) f->set = 0;
             c->rows = c->cols = 0;
             c->nnz = 0;
             c->sparsity_gen++;
@@ -461,6 +462,28 @@ int flgpu_colour_mesh(int32_t nnodes, int32_t nele, int32_t loc, const int32_t* 
             }
         }
         *ncol_out = ncol;
+        return FLGPU_OK;
+    });
+}
+
+int flgpu_mesh_motion_set(flgpu_ctx* c, const double* X_old, const double* X_new, const double* grid_velocity) {
+    return guarded("flgpu_mesh_motion_set", [&]() -> int {
+        if (!c || !X_old || !X_new || !grid_velocity)
+            return set_error(FLGPU_EINVAL, "flgpu_mesh_motion_set: null argument");
+        if (c->nx <= 0) return set_error(FLGPU_EINVAL, "flgpu_mesh_motion_set: bind the coordinate mesh first");
+        FLGPU_HIP(hipSetDevice(c->device));
+        const int dim = c->dim;
+        std::vector<double> pad((size_t)c->nx * 4, 0.0);
+        const double* src[3] = {X_old, X_new, grid_velocity};
+        DevBuf<double>* dst[3] = {&c->Xo4, &c->Xn4, &c->UG4};
+        for (int w = 0; w < 3; w++) {
+            for (int64_t i = 0; i < c->nx; i++)
+                for (int d = 0; d < dim; d++) pad[i * 4 + d] = src[w][i * dim + d];
+            FLGPU_TRY(dst[w]->upload(pad.data(), pad.size(), c->stream));
+            FLGPU_HIP(hipStreamSynchronize(c->stream));  // pad is reused
+        }
+        c->motion_nx = c->nx;
+        c->motion_dim = dim;
         return FLGPU_OK;
     });
 }

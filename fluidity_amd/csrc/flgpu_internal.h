@@ -214,6 +214,10 @@ struct flgpu_ctx {
     int32_t dim = 0, nele = 0, loc_x = 0, nx = 0;
     flgpu::DevBuf<int32_t> conn_x;   // loc_x*nele, 0-based, element-major
     flgpu::DevBuf<double> X4;        // 4 doubles per node (x,y,z,0)
+    // mesh movement: old and new positions and the grid velocity, laid out
+    // as X4, for a mesh of motion_nx nodes in motion_dim dimensions
+    flgpu::DevBuf<double> Xo4, Xn4, UG4;
+    int motion_nx = 0, motion_dim = 0;
     std::vector<int32_t> h_conn_x;
 
     // field mesh

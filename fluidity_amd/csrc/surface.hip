@@ -513,6 +513,10 @@ extern "C" int flgpu_advdif_surface(flgpu_ctx* c, const flgpu_advdif_params* p, 
         // not stage; k-epsilon faces take the advection-diffusion form there
         if (p->equation_type == FLGPU_EQ_INTERNAL_ENERGY && p->have_advection && p->integrate_by_parts)
             return set_error(FLGPU_EUNSUPPORTED, "flgpu_advdif_surface: internal-energy advection by parts");
+        // with mesh movement the face advection takes the velocity relative
+        // to the grid (:1315-1317), which this face loop does not stage
+        if (p->move_mesh && p->have_mass && p->have_advection && p->integrate_by_parts)
+            return set_error(FLGPU_EUNSUPPORTED, "flgpu_advdif_surface: advection by parts on a moving mesh");
         FLGPU_HIP(hipSetDevice(c->device));
         krylov_invalidate(c);
         return surface_assemble(c, p, bc_type, bc_value, bc_value2);

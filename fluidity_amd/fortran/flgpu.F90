@@ -55,6 +55,8 @@ module flgpu
      !! its p div(u) term and PhaseVolumeFraction factor (0/1)
      real(c_double) :: density_theta
      integer(c_int32_t) :: equation_type, compressible, multiphase
+     !! mesh movement (0/1; positions and grid velocity by flgpu_mesh_motion_set)
+     integer(c_int32_t) :: move_mesh
   end type flgpu_advdif_params
 
   !! flgpu_solver_params: the solver options of schemas/solvers.rnc
@@ -71,7 +73,7 @@ module flgpu
   public :: flgpu_init, flgpu_finalize, flgpu_mesh_bind, flgpu_field_mesh_bind, &
        flgpu_sparsity_bind, flgpu_colouring_bind, flgpu_make_sparsity, flgpu_colour_mesh, &
        flgpu_sparsity_build, flgpu_sparsity_get, flgpu_colouring_build, flgpu_colouring_get, &
-       flgpu_field_set, flgpu_advdif_assemble, flgpu_rhs_add_field, flgpu_surface_bind, flgpu_advdif_surface, &
+       flgpu_field_set, flgpu_mesh_motion_set, flgpu_advdif_assemble, flgpu_rhs_add_field, flgpu_surface_bind, flgpu_advdif_surface, &
        flgpu_dirichlet_apply, flgpu_matrix_get, &
        flgpu_petsc_solve, flgpu_check, flgpu_pc_from_name
 
@@ -176,6 +178,14 @@ module flgpu
        integer(c_int32_t), value :: which, constant
        real(c_double), intent(in) :: values(*)
      end function flgpu_field_set
+
+     !! OldCoordinate, IteratedCoordinate, GridVelocity (dim x nx each)
+     integer(c_int) function flgpu_mesh_motion_set(ctx, X_old, X_new, grid_velocity) &
+          bind(c, name="flgpu_mesh_motion_set")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: X_old(*), X_new(*), grid_velocity(*)
+     end function flgpu_mesh_motion_set
 
      integer(c_int) function flgpu_advdif_assemble(ctx, p, T, U, kappa, src, absorp, A_val, rhs) &
           bind(c, name="flgpu_advdif_assemble")

@@ -130,7 +130,8 @@ program flgpu_driver
        have_advection=1, integrate_by_parts=0, have_diffusivity=1, isotropic_diffusivity=1, &
        have_source=0, have_absorption=0, precision=FLGPU_FP64, scheme=FLGPU_ASM_AUTO, &
        stabilisation=FLGPU_STAB_NONE, nu_bar_scheme=FLGPU_NU_BAR_OPTIMAL, nu_bar_scale=0.5d0, &
-       density_theta=1.0d0, equation_type=FLGPU_EQ_ADVECTION_DIFFUSION, compressible=0, multiphase=0)
+       density_theta=1.0d0, equation_type=FLGPU_EQ_ADVECTION_DIFFUSION, compressible=0, multiphase=0, &
+       move_mesh=0)
   call flgpu_check(flgpu_advdif_assemble(ctx, p, c_loc(T), c_loc(U), c_loc(kap), c_null_ptr, c_null_ptr, &
        c_null_ptr, c_null_ptr), "advdif_assemble")
 

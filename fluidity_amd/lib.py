@@ -31,7 +31,7 @@ EXPORTS = (
     "flgpu_init", "flgpu_finalize", "flgpu_last_error", "flgpu_mesh_bind",
     "flgpu_field_mesh_bind", "flgpu_sparsity_bind", "flgpu_colouring_bind",
     "flgpu_make_sparsity", "flgpu_colour_mesh", "flgpu_sparsity_build", "flgpu_sparsity_get",
-    "flgpu_colouring_build", "flgpu_colouring_get", "flgpu_field_set",
+    "flgpu_colouring_build", "flgpu_colouring_get", "flgpu_field_set", "flgpu_mesh_motion_set",
     "flgpu_advdif_assemble_resident", "flgpu_advdif_assemble", "flgpu_rhs_add_field", "flgpu_surface_bind",
     "flgpu_advdif_surface", "flgpu_dirichlet_apply",
     "flgpu_matrix_get", "flgpu_matrix_set", "flgpu_solve", "flgpu_solve_noalloc", "flgpu_pressure_bind", "flgpu_ct_assemble", "flgpu_cmc_assemble", "flgpu_time_pressure", "flgpu_petsc_solve", "flgpu_pc_apply",
@@ -53,7 +53,7 @@ class AdvdifParams(C.Structure):
         ("precision", C.c_int32), ("scheme", C.c_int32),
         ("stabilisation", C.c_int32), ("nu_bar_scheme", C.c_int32), ("nu_bar_scale", C.c_double),
         ("density_theta", C.c_double), ("equation_type", C.c_int32), ("compressible", C.c_int32),
-        ("multiphase", C.c_int32),
+        ("multiphase", C.c_int32), ("move_mesh", C.c_int32),
     ]
 
 
@@ -105,6 +105,7 @@ def load() -> C.CDLL:
         "flgpu_make_sparsity": [I32, I32, I32, P, P, P, P, C.POINTER(C.c_int64)],
         "flgpu_colour_mesh": [I32, I32, I32, P, P, C.POINTER(C.c_int32)],
         "flgpu_field_set": [P, I32, I32, P],
+        "flgpu_mesh_motion_set": [P, P, P, P],
         "flgpu_advdif_assemble_resident": [P, C.POINTER(AdvdifParams)],
         "flgpu_advdif_assemble": [P, C.POINTER(AdvdifParams), P, P, P, P, P, P, P],
         "flgpu_rhs_add_field": [P, I32],

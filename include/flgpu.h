@@ -154,6 +154,14 @@ enum {
     FLGPU_FIELD_VOLUME_FRACTION = 8 /* PhaseVolumeFraction (nvfrac) */
 };
 
+/* Mesh movement inputs (Advection_Diffusion_CG.F90:421-437):
+ * OldCoordinate and IteratedCoordinate positions and GridVelocity, each
+ * dim x nx on the coordinate mesh (vector_field%val).  Used by assemblies
+ * with flgpu_advdif_params.move_mesh; kept until the coordinate mesh
+ * changes topology. */
+int flgpu_mesh_motion_set(flgpu_ctx *ctx, const double *X_old, const double *X_new,
+                          const double *grid_velocity);
+
 /* Upload a field to its device mirror.  constant != 0: FIELD_TYPE_CONSTANT,
  * values holds one value (ncomp numbers); it is interpolated like a nodal
  * field, as ele_val_at_quad does (femtools/Fields_Base.F90:2256-2310). */
@@ -190,6 +198,13 @@ typedef struct {
      * FLGPU_ASM_COLOUR), without stabilisation. */
     double density_theta;
     int32_t equation_type, compressible, multiphase;
+    /* mesh movement (/mesh_adaptivity/mesh_movement with mass terms,
+     * :421-437): the mass at the new positions, -(N^{n+1} - N^n) T / dt on
+     * the rhs, the velocity relative to the grid and, not by parts,
+     * -N_A div(u_grid) N_B (flgpu_mesh_motion_set gives the positions and
+     * grid velocity).  Linear T on the colour scheme, advection-diffusion
+     * without stabilisation (the reference stops on the other cases). */
+    int32_t move_mesh;
 } flgpu_advdif_params;
 enum { FLGPU_EQ_ADVECTION_DIFFUSION = 0, FLGPU_EQ_INTERNAL_ENERGY = 1, FLGPU_EQ_KEPSILON = 2 };
 enum { FLGPU_STAB_NONE = 0, FLGPU_STAB_STREAMLINE_UPWIND = 1, FLGPU_STAB_SUPG = 2 };

@@ -486,15 +486,15 @@ double orc_nu_bar_scaled(int dim, const double *u, const double *Jflat, const do
  * (Transform_elements.F90:227-306 construct_cache, :766-802 retrieval):
  * J^T from the coordinate element's dn, its inverse by cofactors over the
  * determinant. */
-static void elem_geometry(const orc_mesh *m, int ele, double JT[MAXDIM][MAXDIM], double iJ[MAXDIM][MAXDIM],
-                          double *detJ_out)
+static void elem_geometry_of(const orc_mesh *m, const double *X, int ele, double JT[MAXDIM][MAXDIM],
+                             double iJ[MAXDIM][MAXDIM], double *detJ_out)
 {
     const int dim = m->dim, ngi = m->ngi, lx = m->loc_x;
     const int32_t *exn = m->ndglno_x + (int64_t)ele * lx;
     double Xv[MAXDIM][MAXLOC];
     for (int n = 0; n < lx; n++)
         for (int a = 0; a < dim; a++)
-            Xv[a][n] = m->X[a + dim * (int64_t)(exn[n] - 1)];
+            Xv[a][n] = X[a + dim * (int64_t)(exn[n] - 1)];
     for (int a = 0; a < dim; a++)
         for (int b = 0; b < dim; b++) {
             double s = 0.0;
@@ -519,6 +519,12 @@ static void elem_geometry(const orc_mesh *m, int ele, double JT[MAXDIM][MAXDIM],
     for (int i = 0; i < dim; i++)
         for (int k = 0; k < dim; k++) iJ[i][k] = iJ[i][k] / detJ;
     *detJ_out = detJ;
+}
+
+static void elem_geometry(const orc_mesh *m, int ele, double JT[MAXDIM][MAXDIM], double iJ[MAXDIM][MAXDIM],
+                          double *detJ_out)
+{
+    elem_geometry_of(m, m->X, ele, JT, iJ, detJ_out);
 }
 
 enum { EQ_ADVECTION_DIFFUSION = 0, EQ_INTERNAL_ENERGY = 1, EQ_KEPSILON = 2 };
@@ -572,7 +578,6 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
 
     /* --- transform_to_physical, cache path --- */
     (void)Xv;
-    (void)exn;
     elem_geometry(m, ele, JT, iJ, &detJ);
 
     /* dshape = dn . invJ^T; once for linear T (m_nonlinear false),
@@ -590,6 +595,38 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
     for (int g = 0; g < ngi; g++) detwei[g] = fabs(detJ) * m->w[g];
 
     for (int i = 0; i < loc; i++) Te[i] = m->T[en[i] - 1];
+
+    /* --- mesh movement (:421-437, :788-791): detwei of the old and new
+     * positions, the grid velocity (on the coordinate mesh) and its
+     * divergence at the points; linear T here, so the grid velocity's shape
+     * and derivatives are T's.  The reference refuses it with stabilisation
+     * and with the internal-energy and k-epsilon equations (:447, :462,
+     * :500, :531). --- */
+    const int mv = p->move_mesh && p->have_mass;
+    double detwei_old[MAXGI], detwei_new[MAXGI], ugq[MAXDIM][MAXGI], divug[MAXGI];
+    if (mv) {
+        if (m_nonlinear || p->stabilisation != STAB_NONE || p->equation_type != 0) return -3;
+        double JTo[MAXDIM][MAXDIM], iJo[MAXDIM][MAXDIM], dto, dtn;
+        elem_geometry_of(m, m->X_old, ele, JTo, iJo, &dto);
+        elem_geometry_of(m, m->X_new, ele, JTo, iJo, &dtn);
+        for (int g = 0; g < ngi; g++) {
+            detwei_old[g] = fabs(dto) * m->w[g];
+            detwei_new[g] = fabs(dtn) * m->w[g];
+            divug[g] = 0.0;
+        }
+        for (int d = 0; d < dim; d++)
+            for (int g = 0; g < ngi; g++) {
+                double s = 0.0;
+                for (int i = 0; i < loc; i++) s = s + m->ug[d + dim * (int64_t)(exn[i] - 1)] * N_(i, g);
+                ugq[d][g] = s;
+            }
+        for (int d = 0; d < dim; d++)
+            for (int g = 0; g < ngi; g++) {
+                double s = 0.0;
+                for (int i = 0; i < loc; i++) s = s + m->ug[d + dim * (int64_t)(exn[i] - 1)] * dsh[i][g][d];
+                divug[g] = divug[g] + s;
+            }
+    }
 
     /* --- the equation type's fields (:484-552, :983-1004) --- */
     const int eqt = p->equation_type;
@@ -657,7 +694,7 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
             else if (ke)
                 wm[g] = detwei[g] * rhoq[g];
             else
-                wm[g] = detwei[g];
+                wm[g] = mv ? detwei_new[g] : detwei[g];
         }
         for (int i = 0; i < loc; i++)
             for (int j = 0; j < loc; j++) {
@@ -676,6 +713,24 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
                 for (int j = 0; j < loc; j++)
                     mat[i + loc * j] = mat[i + loc * j] + M[i][j];
         }
+        if (mv) { /* (N^{n+1} - N^n) T^n / dt on the rhs (:922-938) */
+            for (int i = 0; i < loc; i++)
+                for (int j = 0; j < loc; j++) {
+                    double s = 0.0;
+                    for (int g = 0; g < ngi; g++) s = s + (TN_(i, g) * N_(j, g)) * (detwei_new[g] - detwei_old[g]);
+                    M[i][j] = s;
+                }
+            for (int i = 0; i < loc; i++) {
+                double s = 0.0;
+                if (p->lump_mass) {
+                    for (int j = 0; j < loc; j++) s = s + M[i][j];
+                    s = s * Te[i];
+                } else {
+                    for (int j = 0; j < loc; j++) s = s + M[i][j] * Te[j];
+                }
+                rhs[i] = rhs[i] - s / p->dt;
+            }
+        }
     }
 
     /* --- advection (add_advection_element_cg, :943-1127) --- */
@@ -690,6 +745,9 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
                 }
                 uq[d][g] = s;
             }
+        if (mv) /* the velocity relative to the grid (:983-985) */
+            for (int d = 0; d < dim; d++)
+                for (int g = 0; g < ngi; g++) uq[d][g] = uq[d][g] - ugq[d][g];
         int need_div = p->integrate_by_parts ? (fabs(1.0 - p->beta) > eps)
                                              : (fabs(p->beta) > eps);
         /* density at the points and u.grad(density) (:986-1004): internal
@@ -786,6 +844,13 @@ static int element_cg(const orc_params *p, const orc_mesh *m, int ele,
                         adv[i][j] = adv[i][j] + coef * s;
                 }
         }
+        if (mv && !p->integrate_by_parts) /* - N_A div(u_grid) N_B (:1099-1102) */
+            for (int i = 0; i < loc; i++)
+                for (int j = 0; j < loc; j++) {
+                    double s = 0.0;
+                    for (int g = 0; g < ngi; g++) s = s + (TN_(i, g) * N_(j, g)) * (divug[g] * detwei[g]);
+                    adv[i][j] = adv[i][j] - s;
+                }
         if (p->stabilisation == STAB_SU) {
             /* element_upwind_stabilisation (Upwind_Stabilisation.F90:52-84) */
             for (int i = 0; i < loc; i++)
@@ -975,6 +1040,10 @@ int orc_assemble(const orc_params *p, const orc_mesh *m, double *A,
     const int loc = m->loc;
     int64_t nnz = m->findrm[m->nrows] - 1;
     int err = 0;
+    /* mesh movement: linear T, no stabilisation, advection-diffusion only */
+    if (p->move_mesh && p->have_mass &&
+        (loc != m->dim + 1 || p->stabilisation != STAB_NONE || p->equation_type != 0))
+        return -3;
     /* zero(matrix); zero(rhs) (:560-561) */
     for (int64_t k = 0; k < nnz; k++) A[k] = 0.0;
     for (int i = 0; i < m->nrows; i++) rhs[i] = 0.0;

@@ -57,6 +57,9 @@ typedef struct {
      * multiphase: the PhaseVolumeFraction factor (internal energy) */
     double density_theta;
     int32_t equation_type, compressible, multiphase;
+    /* mesh movement (:421-437, move_mesh with mass): the old and new
+     * positions and the grid velocity of orc_mesh */
+    int32_t move_mesh;
 } orc_params;
 
 typedef struct {
@@ -82,6 +85,9 @@ typedef struct {
     const double *rho_old; int32_t rho_old_const;
     const double *pres;   int32_t pres_const;
     const double *nvfrac; int32_t nvfrac_const;
+    /* mesh movement: OldCoordinate and IteratedCoordinate (dim x nx, on the
+     * coordinate mesh) and GridVelocity (dim x nx, on the coordinate mesh) */
+    const double *X_old, *X_new, *ug;
     /* colour sets: ncolours+1 offsets into elems (0-based offsets,
      * 1-based element ids, ascending inside each colour) */
     int32_t ncolours;

@@ -24,7 +24,7 @@ class OrcParams(C.Structure):
                 ("isotropic_diffusivity", C.c_int32), ("have_source", C.c_int32),
                 ("have_absorption", C.c_int32), ("stabilisation", C.c_int32), ("nu_bar_scheme", C.c_int32),
                 ("nu_bar_scale", C.c_double), ("density_theta", C.c_double), ("equation_type", C.c_int32),
-                ("compressible", C.c_int32), ("multiphase", C.c_int32)]
+                ("compressible", C.c_int32), ("multiphase", C.c_int32), ("move_mesh", C.c_int32)]
 
 
 class OrcMesh(C.Structure):
@@ -38,6 +38,7 @@ class OrcMesh(C.Structure):
                 ("absorp", C.c_void_p), ("absorp_const", C.c_int32),
                 ("rho", C.c_void_p), ("rho_const", C.c_int32), ("rho_old", C.c_void_p), ("rho_old_const", C.c_int32),
                 ("pres", C.c_void_p), ("pres_const", C.c_int32), ("nvfrac", C.c_void_p), ("nvfrac_const", C.c_int32),
+                ("X_old", C.c_void_p), ("X_new", C.c_void_p), ("ug", C.c_void_p),
                 ("ncolours", C.c_int32), ("col_ptr", C.c_void_p), ("col_elems", C.c_void_p),
                 ("nrows", C.c_int32), ("findrm", C.c_void_p), ("colm", C.c_void_p)]
 
@@ -203,6 +204,12 @@ class Assembly:
             self.keep.append(arr)
             setattr(m, ptrname, _p(arr))
             setattr(m, cname, const)
+        for key in ("X_old", "X_new", "ug"):  # mesh movement: dim x nx arrays on the coordinate mesh
+            v = fields.get(key)
+            if v is not None:
+                arr = np.ascontiguousarray(v, dtype=np.float64).ravel()
+                self.keep.append(arr)
+                setattr(m, key, _p(arr))
         findrm, colm, _ = sparsity
         col_ptr0, elems = colouring
         m.ncolours = col_ptr0.size - 1
@@ -269,10 +276,10 @@ EQUATION = {"advection_diffusion": 0, "internal_energy": 1, "k_epsilon": 2}
 
 def params(dt, theta, beta, have_mass=True, lump=False, adv=True, ibp=False, diff=True, iso=True,
            src=False, absorp=False, stab="none", nu_bar="optimal", nu_scale=0.5, equation="advection_diffusion",
-           density_theta=1.0, compressible=False, multiphase=False) -> OrcParams:
+           density_theta=1.0, compressible=False, multiphase=False, move_mesh=False) -> OrcParams:
     return OrcParams(dt, theta, beta, int(have_mass), int(lump), int(adv), int(ibp), int(diff), int(iso),
                      int(src), int(absorp), STABILISATION[stab], NU_BAR[nu_bar], float(nu_scale),
-                     float(density_theta), EQUATION[equation], int(compressible), int(multiphase))
+                     float(density_theta), EQUATION[equation], int(compressible), int(multiphase), int(move_mesh))
 
 
 def nu_bar_scaled(u, J, K, scheme="optimal", scale=0.5):
@@ -289,6 +296,8 @@ def assemble(p: OrcParams, asm: Assembly, nthreads: int = 0):
     rhs = np.zeros(asm.nrows)
     secs = C.c_double(0)
     rc = load().orc_assemble(C.byref(p), C.byref(asm.m), _p(A), _p(rhs), nthreads, C.byref(secs))
+    if rc == -3:
+        raise ValueError("oracle: mesh movement takes linear T, no stabilisation, advection-diffusion only")
     if rc != 0:
         raise RuntimeError("oracle: entry outside the sparsity pattern")
     return A, rhs, secs.value
@@ -387,7 +396,8 @@ def reference_system(cfg, nthreads: int = 0, variant: dict | None = None):
     v = variant or {}
     fields = {"T": cfg.T, "U": cfg.U, "kappa": kap, "src": cfg.source, "absorp": cfg.absorption,
               "rho": v.get("density"), "rho_old": v.get("old_density"), "pres": v.get("pressure"),
-              "nvfrac": v.get("volume_fraction")}
+              "nvfrac": v.get("volume_fraction"), "X_old": v.get("old_positions"),
+              "X_new": v.get("new_positions"), "ug": v.get("grid_velocity")}
     asm = Assembly(mesh, el, xel, fields, sp, cs)
     p = params(cfg.dt, cfg.theta, v.get("beta", cfg.beta), have_mass=cfg.have_mass,
                lump=v.get("lump_mass_matrix", False), adv=cfg.have_advection,
@@ -398,7 +408,7 @@ def reference_system(cfg, nthreads: int = 0, variant: dict | None = None):
                stab=v.get("stabilisation", "none"), nu_bar=v.get("nu_bar_scheme", "optimal"),
                nu_scale=v.get("nu_scale", 0.5), equation=v.get("equation_type", "advection_diffusion"),
                density_theta=v.get("density_theta", 1.0), compressible=v.get("compressible", False),
-               multiphase=v.get("multiphase", False))
+               multiphase=v.get("multiphase", False), move_mesh=v.get("move_mesh", False))
     A, rhs, secs = assemble(p, asm, nthreads)
     if cfg.source is not None and v.get("add_source_directly_to_rhs", False):
         rhs = rhs + np.broadcast_to(np.asarray(cfg.source, np.float64), rhs.shape)  # addto(rhs, source), :604
