@@ -35,6 +35,19 @@ struct FaceFlags {
     int u_c;             // velocity constant
     int mat_terms;       // |dt_theta| > epsilon(0.0)
     double dt_theta, theta;
+    // the internal energy's density-weighted face advection (:1318-1327:
+    // rho_theta density + (1 - rho_theta) old density, times the volume
+    // fraction when multiphase) and, with mesh movement, the velocity
+    // relative to the grid (:1315-1317)
+    int ie, mph, move;
+    double rho_theta;
+};
+
+// The fields those read: scalars on T's mesh (constant: one value) and the
+// grid velocity by coordinate node, 4 doubles a node.
+struct FaceFields {
+    const double *rho, *rho_old, *nv, *ug4;
+    int rho_c, rho_old_c, nv_c;
 };
 
 // One face, FP64.  Arithmetic order follows assemble_advection_diffusion_
@@ -47,8 +60,8 @@ k_face_local(const FaceTables* __restrict__ tb, int nfaces, const int32_t* __res
              const int32_t* __restrict__ fnodes_x, const int32_t* __restrict__ fele,
              const int32_t* __restrict__ conn_x, const double* __restrict__ X4, const double* __restrict__ T,
              const double* __restrict__ U, const int32_t* __restrict__ bct, const double* __restrict__ bv,
-             const double* __restrict__ bv2, FaceFlags ff, double* __restrict__ fm, double* __restrict__ fr,
-             int* __restrict__ err) {
+             const double* __restrict__ bv2, FaceFlags ff, FaceFields fx, double* __restrict__ fm,
+             double* __restrict__ fr, int* __restrict__ err) {
     constexpr int FLX = DIM;   // vertices of a linear face
     constexpr int LX = DIM + 1;
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -158,9 +171,27 @@ k_face_local(const FaceTables* __restrict__ tb, int nfaces, const int32_t* __res
                     double uq = 0.0;
 #pragma unroll
                     for (int k = 0; k < FL; k++) uq = uq + Uf[d][k] * n_g[k];
+                    if (ff.move) {  // linear T: the face's vertices are its nodes
+                        double ugq = 0.0;
+#pragma unroll
+                        for (int k = 0; k < FLX && k < FL; k++)
+                            ugq = ugq + fx.ug4[(int64_t)fnodes_x[(int64_t)f * FLX + k] * 4 + d] * n_g[k];
+                        uq = uq - ugq;
+                    }
                     un = un + uq * nrm[d];
                 }
-                const double cq = (detJ * tb->fw[g]) * un;
+                double cq = (detJ * tb->fw[g]) * un;
+                if (ff.ie) {
+                    double r = 0.0, ro = 0.0, nv = 0.0;
+#pragma unroll
+                    for (int k = 0; k < FL; k++) {
+                        r = r + (fx.rho_c ? fx.rho[0] : fx.rho[nd[k]]) * n_g[k];
+                        ro = ro + (fx.rho_old_c ? fx.rho_old[0] : fx.rho_old[nd[k]]) * n_g[k];
+                        if (ff.mph) nv = nv + (fx.nv_c ? fx.nv[0] : fx.nv[nd[k]]) * n_g[k];
+                    }
+                    cq = cq * (ff.rho_theta * r + (1.0 - ff.rho_theta) * ro);
+                    if (ff.mph) cq = cq * nv;
+                }
 #pragma unroll
                 for (int i = 0; i < FL; i++)
 #pragma unroll
@@ -363,9 +394,11 @@ int build_lists(flgpu_ctx* c) {
 template <int DIM, int FL>
 void launch_face_local(flgpu_ctx* c, const FaceFlags& ff, int* d_err) {
     SurfaceState& S = c->surf;
+    const FaceFields fx{c->fRho.v.p, c->fRhoOld.v.p, c->fV.v.p, c->UG4.p, c->fRho.constant, c->fRhoOld.constant,
+                        c->fV.constant};
     hipLaunchKernelGGL((k_face_local<DIM, FL>), dim3(grid_for(S.nfaces, 128)), dim3(128), 0, c->stream,
                        S.d_tab.p, S.nfaces, S.fnodes.p, S.fnodes_x.p, S.fele.p, c->conn_x.p, c->X4.p, c->fT.v.p,
-                       c->fU.v.p, S.bct.p, S.bcv.p, S.bcv2.p, ff, S.fm.p, S.fr.p, d_err);
+                       c->fU.v.p, S.bct.p, S.bcv.p, S.bcv2.p, ff, fx, S.fm.p, S.fr.p, d_err);
 }
 
 }  // namespace
@@ -410,8 +443,17 @@ int surface_assemble(flgpu_ctx* c, const flgpu_advdif_params* p, const int32_t* 
     FLGPU_TRY(d_err.alloc(1));
     FLGPU_HIP(hipMemsetAsync(d_err.p, 0, sizeof(int), c->stream));
     const double dt_theta = p->dt * p->theta;
+    const bool ie = adv_face && p->equation_type == FLGPU_EQ_INTERNAL_ENERGY;
+    const bool mph = ie && p->multiphase != 0;
+    const bool move = adv_face && p->move_mesh != 0 && p->have_mass != 0;
+    if (ie && (!c->fRho.set || !c->fRhoOld.set || (mph && !c->fV.set)))
+        return set_error(FLGPU_EINVAL, "flgpu_advdif_surface: internal-energy faces need density, old density%s",
+                         mph ? " and volume fraction" : "");
+    if (move && (c->motion_nx != c->nx || c->motion_dim != c->dim))
+        return set_error(FLGPU_EINVAL, "flgpu_advdif_surface: mesh movement needs flgpu_mesh_motion_set");
+    if (move && !c->linear) return set_error(FLGPU_EUNSUPPORTED, "flgpu_advdif_surface: mesh movement takes linear T");
     const FaceFlags ff{adv_face ? 1 : 0, diff ? 1 : 0, c->fU.constant, std::fabs(dt_theta) > 2.220446049250313e-16,
-                       dt_theta, p->theta};
+                       dt_theta, p->theta, ie ? 1 : 0, mph ? 1 : 0, move ? 1 : 0, p->density_theta};
     const int key = c->dim * 10 + S.floc;
     switch (key) {
         case 22: launch_face_local<2, 2>(c, ff, d_err.p); break;
@@ -511,12 +553,6 @@ extern "C" int flgpu_advdif_surface(flgpu_ctx* c, const flgpu_advdif_params* p, 
         // the internal-energy face advection is weighted by the density
         // (add_advection_face_cg, :1318-1327), which this face loop does
         // not stage; k-epsilon faces take the advection-diffusion form there
-        if (p->equation_type == FLGPU_EQ_INTERNAL_ENERGY && p->have_advection && p->integrate_by_parts)
-            return set_error(FLGPU_EUNSUPPORTED, "flgpu_advdif_surface: internal-energy advection by parts");
-        // with mesh movement the face advection takes the velocity relative
-        // to the grid (:1315-1317), which this face loop does not stage
-        if (p->move_mesh && p->have_mass && p->have_advection && p->integrate_by_parts)
-            return set_error(FLGPU_EUNSUPPORTED, "flgpu_advdif_surface: advection by parts on a moving mesh");
         FLGPU_HIP(hipSetDevice(c->device));
         krylov_invalidate(c);
         return surface_assemble(c, p, bc_type, bc_value, bc_value2);

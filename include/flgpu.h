@@ -287,7 +287,10 @@ int flgpu_surface_bind(flgpu_ctx *ctx, int32_t nfaces, int32_t floc,
  * nodes, ele_val(t_bc, face) and ele_val(t_bc_2, face) (the Robin
  * coefficient); either may be NULL when no face needs it.  Weak Dirichlet
  * with diffusivity is FLGPU_EUNSUPPORTED, as it is an FLExit in Fluidity
- * (:1374-1377).  FP64 arithmetic whatever p->precision says. */
+ * (:1374-1377).  The internal-energy equation weights the face advection by
+ * its theta-weighted density (and volume fraction), and mesh movement takes
+ * the velocity relative to the grid there (:1313-1327).  FP64 arithmetic
+ * whatever p->precision says. */
 int flgpu_advdif_surface(flgpu_ctx *ctx, const flgpu_advdif_params *p,
                          const int32_t *bc_type, const double *bc_value,
                          const double *bc_value2);

@@ -1319,6 +1319,12 @@ static int face_cg(const orc_params *p, const orc_mesh *m, const orc_surface *s,
     /* Step 2a: add_advection_face_cg (:1292-1347) */
     if (adv_face) {
         double cq[MAXGI], adv[MAXLOC][MAXLOC];
+        /* internal energy: times the theta-weighted density [and nvfrac]
+         * (:1318-1327); mesh movement: the velocity relative to the grid
+         * (:1315-1317; linear T, the grid velocity's face is T's) */
+        const int ie = p->equation_type == EQ_INTERNAL_ENERGY;
+        const int mv = p->move_mesh && p->have_mass;
+        if (mv && fl != flx) return -3;
         for (int g = 0; g < fg; g++) {
             double un = 0.0;
             for (int d = 0; d < dim; d++) {
@@ -1327,9 +1333,24 @@ static int face_cg(const orc_params *p, const orc_mesh *m, const orc_surface *s,
                     double uk = m->u_const ? m->U[d] : m->U[d + dim * (int64_t)(fnod[k] - 1)];
                     uq = uq + uk * FN_(k, g);
                 }
+                if (mv) {
+                    double ugq = 0.0;
+                    for (int k = 0; k < flx; k++) ugq = ugq + m->ug[d + dim * (int64_t)(fxn[k] - 1)] * FN_(k, g);
+                    uq = uq - ugq;
+                }
                 un = un + uq * nrm[d];
             }
             cq[g] = detwei[g] * un;
+            if (ie) {
+                double r = 0.0, ro = 0.0, nv = 0.0;
+                for (int k = 0; k < fl; k++) {
+                    r = r + (m->rho_const ? m->rho[0] : m->rho[fnod[k] - 1]) * FN_(k, g);
+                    ro = ro + (m->rho_old_const ? m->rho_old[0] : m->rho_old[fnod[k] - 1]) * FN_(k, g);
+                    if (p->multiphase) nv = nv + (m->nvfrac_const ? m->nvfrac[0] : m->nvfrac[fnod[k] - 1]) * FN_(k, g);
+                }
+                cq[g] = cq[g] * (p->density_theta * r + (1.0 - p->density_theta) * ro);
+                if (p->multiphase) cq[g] = cq[g] * nv;
+            }
         }
         for (int i = 0; i < fl; i++)
             for (int j = 0; j < fl; j++) {

@@ -255,6 +255,8 @@ def surface_assemble(p: OrcParams, asm: Assembly, sd: SurfaceData, A, rhs):
         raise RuntimeError("oracle: face entry outside the sparsity pattern")
     if rc == -2:
         raise RuntimeError("oracle: weak Dirichlet with diffusivity is not supported")
+    if rc == -3:
+        raise ValueError("oracle: mesh movement takes linear T")
 
 
 def face(p: OrcParams, asm: Assembly, sd: SurfaceData, f: int):

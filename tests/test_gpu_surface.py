@@ -125,3 +125,52 @@ def test_surface_lists_follow_a_sparsity_rebind(require_gpu):
     for o in out[1:]:
         assert np.array_equal(o[0], out[0][0]) and np.array_equal(o[1], out[0][1])
     pr.close()
+
+
+@pytest.mark.parametrize("name,scale,kind,opts", [
+    ("c1", 10, "neumann_robin", dict(equation_type="internal_energy", density_theta=0.6)),
+    ("c2", 4, "neumann_robin", dict(equation_type="internal_energy", density_theta=0.3, multiphase=True,
+                                    compressible=True)),
+    ("c1", 10, "weak", dict(equation_type="internal_energy", multiphase=True, have_diffusivity=False)),
+    ("c1", 10, "neumann_robin", dict(move_mesh=True)),
+    ("c2", 4, "neumann_robin", dict(move_mesh=True, beta=0.4)),
+])
+def test_surface_equation_types_and_moving_mesh(require_gpu, name, scale, kind, opts):
+    """The face advection by parts with the internal energy's density (and
+    volume fraction) and with the velocity relative to the grid."""
+    cfg = M.config(name, seed=9, scale=scale)
+    p = cfg.mesh.node_pos
+    if kind == "weak":
+        cfg.U = np.tile(np.array([1.0, 0.25]), (cfg.mesh.nnodes, 1))
+        cfg.kappa = None
+    if cfg.U is None:
+        cfg.U = M.abc_flow(p)
+        cfg.have_advection = cfg.have_mass = True
+        cfg.dt, cfg.theta = 0.005, 0.5
+    fields = dict(density=1.0 + 0.5 * p[:, 0], old_density=0.7 + 0.3 * p[:, 1], volume_fraction=0.4 + 0.2 * p[:, 0],
+                  pressure=1.5 + p[:, 1])
+    X = cfg.mesh.X
+    motion = dict(old_positions=X - 0.01 * np.sin(np.pi * X), new_positions=X + 0.01 * np.sin(2 * np.pi * X))
+    motion["grid_velocity"] = (motion["new_positions"] - motion["old_positions"]) / cfg.dt
+    opts = dict(opts, integrate_advection_by_parts=True)
+    variant = dict(opts, bcs=kind)
+    variant.update(fields if opts.get("equation_type") else motion)
+    print(f"\n[case] {name} n={cfg.mesh.nele} bcs={kind} {opts}", flush=True)
+    ref = O.reference_system(cfg, variant=variant)
+    pr = cg.problem_from_config(cfg)
+    if opts.get("equation_type"):
+        pr.set_fields(**fields)
+    else:
+        pr.set_mesh_motion(motion["old_positions"], motion["new_positions"], motion["grid_velocity"])
+    surf = M.surface_of(cfg.mesh)
+    pr.bind_surface(surf)
+    bt, b1, b2, strong = M.unit_box_bcs(cfg.mesh, surf, kind)
+    o = cg.options_from_config(cfg)
+    for k, v in opts.items():
+        setattr(o, k, v)
+    cg.assemble_advection_diffusion_cg(pr, o, cfg.dt, cg.SurfaceBCs(bt, b1, b2))
+    cg.apply_dirichlet_conditions(pr, strong, 0.0, cfg.dt)
+    A, rhs, _ = pr.matrix()
+    pr.close()
+    assert rel(A, ref["A"]) < FP64_RTOL
+    assert rel(rhs, ref["rhs"]) < FP64_RTOL

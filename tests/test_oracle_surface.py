@@ -202,3 +202,61 @@ def test_neumann_robin_linear_solution_is_exact(name, scale):
     b = -g0
     a = (g1 - b) / h - b
     np.testing.assert_allclose(x, a + b * mesh.node_pos[:, 0], rtol=0, atol=1e-10)
+
+
+def _ibp_identity(cfg, fields_extra, pkw, qd):
+    """Non-IBP (beta = 1) matrix and rhs against IBP (beta = 1) plus the face
+    loop over every boundary face, with extra fields and params."""
+    cfg.quad_degree = qd
+    out = {}
+    for ibp in (False, True):
+        mesh, surf, asm, sd, p, _, _ = _setup(cfg, "neumann_robin", ibp=ibp, diff=False, beta=1.0, adv=True,
+                                              mass=True, dt=0.1, theta=0.5)
+        fields = {"T": cfg.T, "U": cfg.U, "kappa": None, "src": None, "absorp": None}
+        fields.update(fields_extra)
+        el = E.make_element(mesh.dim, mesh.degree, qd)
+        sp = O.make_sparsity(mesh)
+        colour, ncol = O.colour_elements(mesh)
+        asm = O.Assembly(mesh, el, el, fields, sp, O.colour_sets(colour, ncol))
+        p = O.params(0.1, 0.5, 1.0, have_mass=True, adv=True, ibp=ibp, diff=False, **pkw)
+        sd.s.bc_type = O._p(np.zeros(surf.nfaces, np.int32))
+        A, rhs, _ = O.assemble(p, asm)
+        if ibp:
+            O.surface_assemble(p, asm, sd, A, rhs)
+        out[ibp] = (A, rhs)
+    sa = np.abs(out[False][0]).max()
+    return (np.abs(out[True][0] - out[False][0]).max() / sa,
+            np.abs(out[True][1] - out[False][1]).max() / np.abs(out[False][1]).max())
+
+
+@pytest.mark.parametrize("name,scale,multiphase", [("c1", 4, False), ("c2", 2, False), ("c1", 4, True),
+                                                   ("c2", 2, True)])
+def test_integration_by_parts_identity_internal_energy(name, scale, multiphase):
+    """The identity with the internal energy's theta-weighted density (and
+    the volume fraction): it holds only if the face term carries the same
+    density as the volume terms (degree-4 rules, exact here)."""
+    cfg = M.config(name, seed=6, scale=scale)
+    p = cfg.mesh.node_pos
+    cfg.U = np.tile(np.array([0.4, -0.3, 0.2])[:cfg.mesh.dim], (cfg.mesh.nnodes, 1))
+    cfg.T = np.sin(3 * p[:, 0]) + p[:, 1]
+    extra = {"rho": 1.0 + 0.5 * p[:, 0], "rho_old": 0.7 + 0.3 * p[:, 1],
+             "nvfrac": (0.4 + 0.2 * p[:, 0]) if multiphase else None}
+    ea, er = _ibp_identity(cfg, extra, dict(equation="internal_energy", density_theta=0.6, multiphase=multiphase),
+                           4)
+    assert ea < 1e-12 and er < 1e-12
+
+
+@pytest.mark.parametrize("name,scale", [("c1", 4), ("c2", 2)])
+def test_integration_by_parts_identity_moving_mesh(name, scale):
+    """With mesh movement the identity holds for the velocity relative to the
+    grid: the grid divergence term of the non-IBP form and the face term's
+    relative velocity (linear velocity and grid velocity, exact rules)."""
+    cfg = M.config(name, seed=6, scale=scale)
+    X = cfg.mesh.X
+    d = cfg.mesh.dim
+    G = np.array([[0.3, -0.7, 0.2], [1.1, 0.4, -0.5], [0.2, 0.9, 0.6]])[:d, :d]
+    cfg.U = np.tile(np.array([0.4, -0.3, 0.2])[:d], (cfg.mesh.nnodes, 1))
+    cfg.T = np.sin(3 * X[:, 0]) + X[:, 1]
+    extra = {"X_old": X * 0.98, "X_new": X * 1.03, "ug": X @ G.T}
+    ea, er = _ibp_identity(cfg, extra, dict(move_mesh=True), 3)
+    assert ea < 1e-12 and er < 1e-12
