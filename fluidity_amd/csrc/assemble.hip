@@ -998,6 +998,7 @@ k_asm_blocked(const ElemTables* __restrict__ tab, int nslab, int rows_cap, int n
 // c * kstride; kstride 0 when only the first is staged).
 template <int DIM, typename R, bool TENSOR = false>
 struct StagedNodal {
+    static constexpr bool eq_fields = false;  // the slabs stage no density
     const int* lid;
     const R *sU, *sK, *sS, *sB;
     int ncap;
@@ -1912,9 +1913,6 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
         if (fl.comp && (!c->fP.set || !c->fU.set))
             return set_error(FLGPU_EINVAL, "flgpu assemble: the p div(u) term needs pressure and velocity");
         if (fl.mph && !c->fV.set) return set_error(FLGPU_EINVAL, "flgpu assemble: phase volume fraction not set");
-        if (fl.stab)
-            return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: stabilisation with the internal-energy or "
-                                                 "k-epsilon equation");
         if (p->scheme == FLGPU_ASM_BLOCKED)
             return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: the internal-energy and k-epsilon equations "
                                                  "take the colour scheme");

@@ -180,6 +180,7 @@ FLGPU_HD_INLINE R fld(const double* __restrict__ f, int c, int node, int comp, i
 // data elsewhere passes its own accessor with the same four methods.
 template <int DIM, typename R>
 struct GlobalNodal {
+    static constexpr bool eq_fields = true;  // has the equation types' fields
     const AsmFields& fd;
     const int* nodes;
     FLGPU_HD_INLINE R U(int k, int d) const { return fld<R>(fd.U, fd.u_c, nodes[k], d, DIM); }
@@ -450,6 +451,7 @@ struct NodalQuad {
     // nu() reads the diffusivity tensor (a source with nu_bar precomputed
     // says false, and an isotropic row then skips the tensor)
     static constexpr bool nu_reads_k = true;
+    static constexpr bool eq_fields = NA::eq_fields;
     const NA& na;
     const LTab<R>& tb;
     const R (&iJ)[DIM][DIM];
@@ -1026,7 +1028,7 @@ FLGPU_HD_INLINE void row_stab_q(const LTab<R>& tb, const AsmFlags& fl_in, const 
         M[j] = (R)0;
         Sm[j] = (R)0;
     }
-    R src_i = (R)0;
+    R src_i = (R)0, pdv_i = (R)0;
 #pragma unroll 1
     for (int g = 0; g < ngi; g++) {
         const R dw = adet * tb.w[g];
@@ -1069,6 +1071,50 @@ FLGPU_HD_INLINE void row_stab_q(const LTab<R>& tb, const AsmFlags& fl_in, const 
         for (int d = 0; d < DIM; d++) ug_i = FMA(Gi[d], uq[d], ug_i);
         const R tn_i = supg ? FMA(nu, ug_i, ni) : ni;
         const R sq = fl.absorp ? qv.absorp(g) : (R)0;
+        // the internal-energy / k-epsilon weights, as in row_general_q, with
+        // the SUPG test function where the reference takes it (mass,
+        // advection, the p div(u) term); the streamline-upwind term is not
+        // density-weighted (:1105-1116)
+        R dwm = dw, ca = (R)1, cdiv = div, dk = dw;
+        if constexpr (QV::eq_fields) {
+            if (fl.eq) {
+                const R th = (R)fl.rho_theta;
+                const R rq = qv.rho(g);
+                R rb = rq, nvq = (R)1;
+                if (fl.mph) nvq = qv.nv(g);
+                if (fl.eq == 1) {
+                    const R ro = qv.rho_old(g);
+                    rb = th * rq + ((R)1 - th) * ro;
+                    dwm = fl.mph ? dw * ro * nvq : dw * ro;
+                } else {
+                    dwm = dw * rq;
+                }
+                ca = fl.mph ? rb * nvq : rb;
+                if (fl.adv && fl.need_div) {
+                    R gr[DIM];
+                    qv.grad_rho(g, fl.eq, th, gr);
+                    R udgr = (R)0;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) udgr = FMA(gr[d], uq[d], udgr);
+                    if (fl.mph) {
+                        R gn[DIM];
+                        qv.grad_nv(g, gn);
+                        R udgn = (R)0;
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) udgn = FMA(gn[d], uq[d], udgn);
+                        cdiv = div * rb * nvq + nvq * udgr + rb * udgn;
+                    } else {
+                        cdiv = div * rb + udgr;
+                    }
+                }
+                if (fl.mph) dk = dw * nvq;
+                if (fl.comp) {
+                    R pd = qv.div(g) * qv.pres(g);
+                    if (fl.mph) pd = pd * nvq;
+                    pdv_i = FMA(tn_i, dw * pd, pdv_i);
+                }
+            }
+        }
         if (fl.src) src_i = FMA(tn_i, dw * qv.src(g), src_i);
         R vK[DIM];
         if (fl.diff && !fl.iso) {
@@ -1087,7 +1133,7 @@ FLGPU_HD_INLINE void row_stab_q(const LTab<R>& tb, const AsmFlags& fl_in, const 
 #endif
             const R nj = tb.n[j + LOC * g];
             const R tnn = tn_i * nj;
-            if (fl.have_mass) M[j] = FMA(tnn, dw, M[j]);
+            if (fl.have_mass) M[j] = FMA(tnn, dwm, M[j]);
             R Gj[DIM];
 #pragma unroll
             for (int d = 0; d < DIM; d++) {
@@ -1107,10 +1153,12 @@ FLGPU_HD_INLINE void row_stab_q(const LTab<R>& tb, const AsmFlags& fl_in, const 
                 R a;
                 if (!fl.ibp) {
                     a = tn_i * ugj;
-                    if (fl.need_div) a = FMA((R)fl.beta * tnn, div, a);
+                    if (fl.eq) a = a * ca;
+                    if (fl.need_div) a = FMA((R)fl.beta * tnn, cdiv, a);
                 } else {
                     a = -(ug_i * nj);
-                    if (fl.need_div) a = FMA(-((R)1 - (R)fl.beta) * tnn, div, a);
+                    if (fl.eq) a = a * ca;
+                    if (fl.need_div) a = FMA(-((R)1 - (R)fl.beta) * tnn, cdiv, a);
                 }
                 if (su) a = FMA(ug_i * nu, ugj, a);
                 Sm[j] = FMA(a, dw, Sm[j]);
@@ -1120,7 +1168,7 @@ FLGPU_HD_INLINE void row_stab_q(const LTab<R>& tb, const AsmFlags& fl_in, const 
                 if (fl.iso) {
 #pragma unroll
                     for (int d = 0; d < DIM; d++) t = FMA(Gi[d], Gj[d], t);
-                    Sm[j] = FMA(t, dw * kq, Sm[j]);
+                    Sm[j] = FMA(t, dk * kq, Sm[j]);
                 } else {
 #pragma unroll
                     for (int d = 0; d < DIM; d++) t = FMA(vK[d], Gj[d], t);
@@ -1134,6 +1182,7 @@ FLGPU_HD_INLINE void row_stab_q(const LTab<R>& tb, const AsmFlags& fl_in, const 
 #pragma unroll
     for (int j = 0; j < LOC; j++) r = FMA(-Sm[j], T[j], r);
     if (fl.src) r = r + src_i;
+    if (fl.comp) r = r - pdv_i;
     rhs_i = r;
     R lump = (R)0;
     if (fl.have_mass && fl.lump) {

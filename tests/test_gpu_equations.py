@@ -3,9 +3,10 @@
 the GPU against the oracle, through the C ABI: density-weighted mass and
 advection with the div(density u) split, the phase volume fraction
 (multiphase), the compressible p div(u) term; P1 and P2, 1-D to 3-D, FP64
-to 1e-12 and FP32 to 1e-5 relative (as the other assembly tests).  The
-equation types take the colour scheme: asked for by name, the blocked scheme
-and stabilisation are refused with FLGPU_EUNSUPPORTED.
+to 1e-12 and FP32 to 1e-5 relative (as the other assembly tests), also
+with SUPG and streamline-upwind stabilisation.  The equation types take the
+colour scheme: asked for by name, the blocked scheme is refused with
+FLGPU_EUNSUPPORTED.
 """
 import ctypes as C
 
@@ -56,11 +57,17 @@ CASES = [
     ("c3", 2, dict(equation_type="k_epsilon", beta=0.5), "fp64"),
     ("interval", 64, dict(equation_type="internal_energy", compressible=True, multiphase=True, beta=0.2), "fp64"),
     ("c2", 5, dict(equation_type="internal_energy", compressible=True, multiphase=True, beta=0.4), "fp32"),
+    ("c2", 5, dict(equation_type="internal_energy", density_theta=0.6, beta=0.4, stabilisation="supg",
+                   compressible=True), "fp64"),
+    ("c1", 10, dict(equation_type="internal_energy", multiphase=True, stabilisation="supg",
+                    integrate_advection_by_parts=True, beta=0.3), "fp64"),
+    ("c1p2", 4, dict(equation_type="k_epsilon", stabilisation="streamline_upwind", beta=1.0), "fp64"),
 ]
 
 
 @pytest.mark.parametrize("name,scale,opts,precision", CASES,
-                         ids=[f"{c[0]}-{c[2]['equation_type']}-{i}-{c[3]}" for i, c in enumerate(CASES)])
+                         ids=[f"{c[0]}-{c[2]['equation_type']}-{c[2].get('stabilisation', 'none')}-{i}-{c[3]}"
+                              for i, c in enumerate(CASES)])
 def test_equation_types_match_oracle(require_gpu, name, scale, opts, precision):
     cfg = make_cfg(name, scale)
     fields = eq_fields(cfg)
@@ -86,11 +93,11 @@ def test_equation_types_match_oracle(require_gpu, name, scale, opts, precision):
     assert rel(base["A"], ref["A"]) > 1e-3 or rel(base["rhs"], ref["rhs"]) > 1e-3
 
 
-def test_equation_types_refuse_blocked_scheme_and_stabilisation(require_gpu):
+def test_equation_types_refuse_blocked_scheme(require_gpu):
     cfg = make_cfg("c2", 3)
     pr = cg.problem_from_config(cfg)
     pr.set_fields(**eq_fields(cfg))
-    for kw in (dict(scheme="blocked"), dict(stabilisation="supg")):
+    for kw in (dict(scheme="blocked"), dict(scheme="blocked", stabilisation="supg")):
         o = cg.options_from_config(cfg)
         o.equation_type = "internal_energy"
         for k, v in kw.items():
