@@ -195,7 +195,7 @@ typedef struct {
      * the PhaseVolumeFraction factor (mass, advection, isotropic
      * diffusion, pressure term); k-epsilon weights mass and advection by
      * the density.  Both take the colour scheme (FLGPU_ASM_AUTO or
-     * FLGPU_ASM_COLOUR), without stabilisation. */
+     * FLGPU_ASM_COLOUR), with or without stabilisation. */
     double density_theta;
     int32_t equation_type, compressible, multiphase;
     /* mesh movement (/mesh_adaptivity/mesh_movement with mass terms,
