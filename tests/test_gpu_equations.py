@@ -115,3 +115,24 @@ def test_equation_type_needs_its_fields(require_gpu):
     p = o.params(cfg.dt)
     assert pr.lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(p)) == 1  # FLGPU_EINVAL: no density
     pr.close()
+
+
+def test_equation_types_with_constant_fields(require_gpu):
+    """Constant density, old density, pressure and volume fraction fields
+    (FIELD_TYPE_CONSTANT: one value, interpolated like a nodal field)."""
+    cfg = make_cfg("c1", 8)
+    consts = dict(density=1.3, old_density=0.9, pressure=2.5, volume_fraction=0.6)
+    opts = dict(equation_type="internal_energy", density_theta=0.4, compressible=True, multiphase=True, beta=0.5)
+    print(f"\n[case] c1 constant fields {opts}", flush=True)
+    ref = O.reference_system_opts(cfg, dict(opts, **{k: np.array([v]) for k, v in consts.items()}))
+    pr = cg.problem_from_config(cfg)
+    pr.set_fields(**consts)
+    o = cg.options_from_config(cfg)
+    for k, v in opts.items():
+        setattr(o, k, v)
+    cg.assemble_advection_diffusion_cg(pr, o, cfg.dt)
+    cg.apply_dirichlet_conditions(pr, cfg.mesh.boundary, 0.0, cfg.dt)
+    A, rhs, _ = pr.matrix()
+    pr.close()
+    assert rel(A, ref["A"]) < 1e-12
+    assert rel(rhs, ref["rhs"]) < 1e-12
