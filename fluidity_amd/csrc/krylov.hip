@@ -11,9 +11,11 @@
 //     (femtools/Petsc_Tools.F90:724-731).
 // Methods: KSPCG-style preconditioned CG, flexible CG (Polak-Ribiere beta)
 // for the learned preconditioner, restarted GMRES with left Jacobi.
-// Precision: FP64, or FP32 inner iterations with FP64 solution updates,
-// FP64 dot-product accumulation and FP64 true-residual replacement
-// ("reliable updates", iterative refinement without restarting CG).
+// Precision: FP64, or FP32 inner iterations with FP64 dot-product
+// accumulation and FP64 true-residual replacement ("reliable updates",
+// iterative refinement without restarting CG); the FP32 iterations add
+// their steps alpha p to an FP32 increment that is added to the FP64
+// solution at every replacement (and at the end), so x itself is FP64.
 // Every reduction has a fixed order: results repeat bit for bit.
 #include <hip/hip_runtime.h>
 
@@ -39,6 +41,8 @@ struct KrylovWork {
     DevBuf<float> val32, r32, z32, p32, w32, dinv32, y32, rt32;
     DevBuf<double> s;        // single-reduction CG: s = A p by recurrence
     DevBuf<float> s32;
+    DevBuf<float> dx32;      // mixed CG: the FP32 increment of x since the last replacement
+    bool dx_on = false;      // dx32 holds steps not yet added to x
     int npc = 0;             // partial sums of the last preconditioner apply
     DevBuf<double> partials;
     DevBuf<Scal> scal;       // two slots, see carry_if_done
@@ -281,7 +285,7 @@ __device__ __forceinline__ void st_rows(T* __restrict__ a, int i, const Rows<T, 
 // are in flight per thread.
 template <typename V, int R>
 __global__ void __launch_bounds__(kVecBlock)
-k_cg_update(int n, double* __restrict__ x, const V* __restrict__ p, V* __restrict__ r,
+k_cg_update(int n, V* __restrict__ x, const V* __restrict__ p, V* __restrict__ r,
             const V* __restrict__ w, V* __restrict__ z, const V* __restrict__ dinv, int jacobi,
             const float* __restrict__ dsc, V* __restrict__ rt,
             const double* __restrict__ pspmv, int nspmv, double* __restrict__ pvec,
@@ -310,8 +314,8 @@ k_cg_update(int n, double* __restrict__ x, const V* __restrict__ p, V* __restric
     const double alpha = s_alpha;
     const V a = (V)alpha;
     double s_rz = 0.0, s_rr = 0.0, s_zz = 0.0, s_zw = 0.0;
-    auto row = [&](double& xi, V pi, V wi, V& ri, V di, float si, V& zi, V& ti) {
-        xi = fma(alpha, (double)pi, xi);
+    auto row = [&](V& xi, V pi, V wi, V& ri, V di, float si, V& zi, V& ti) {
+        xi = fma((V)alpha, pi, xi);
         ri = ri - a * wi;
         if (dsc) ti = ri * (V)si;
         if (jacobi) {
@@ -327,13 +331,13 @@ k_cg_update(int n, double* __restrict__ x, const V* __restrict__ p, V* __restric
         const int i = g * R;
         const Rows<V, R> pv = ld_rows<R>(p, i), wv = ld_rows<R>(w, i);
         Rows<V, R> rv = ld_rows<R>(r, i), dv = {}, zv = {}, tv = {};
-        Rows<double, R> xv = ld_rows<R>((const double*)x, i);
+        Rows<V, R> xv = ld_rows<R>((const V*)x, i);
         Rows<float, R> sv = {};
         if (jacobi) dv = ld_rows<R>(dinv, i);
         if (dsc) sv = ld_rows<R>(dsc, i);
 #pragma unroll
         for (int u = 0; u < R; u++) {
-            double xi = xv[u];
+            V xi = xv[u];
             V ri = rv[u], zi = (V)0, ti = (V)0;
             row(xi, pv[u], wv[u], ri, dv[u], sv[u], zi, ti);
             xv[u] = xi;
@@ -347,7 +351,7 @@ k_cg_update(int n, double* __restrict__ x, const V* __restrict__ p, V* __restric
         if (dsc) st_rows<R>(rt, i, tv);
     }
     for (int i = ng * R + tid; i < n; i += nt) {
-        double xi = x[i];
+        V xi = x[i];
         V ri = r[i], zi = (V)0, ti = (V)0;
         row(xi, p[i], w[i], ri, jacobi ? dinv[i] : (V)0, dsc ? dsc[i] : 0.f, zi, ti);
         x[i] = xi;
@@ -904,8 +908,21 @@ static int build_reduced(flgpu_ctx* c, KrylovWork* k, bool mixed, bool zero_x) {
     return FLGPU_OK;
 }
 
+// x += dx; dx = 0 (the mixed solves' FP32 increment into the FP64 solution)
+__global__ void __launch_bounds__(kVecBlock)
+k_fold_x(int n, double* __restrict__ x, float* __restrict__ dx) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x[i] = x[i] + (double)dx[i];
+        dx[i] = 0.f;
+    }
+}
+
 // True residual r64 = b - A x (FP64) and its squared norm in scal.true_rr.
 static int true_residual(flgpu_ctx* c, KrylovWork* k) {
+    if (k->dx_on) {
+        hipLaunchKernelGGL(k_fold_x, dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->x.p, k->dx32.p);
+        FLGPU_HIP(hipGetLastError());
+    }
     FLGPU_TRY((spmv<double, double, double>(c, k, k->val64.p, k->x.p, k->r.p, k->b.p, 1, nullptr)));
     return finalize(c, k, FIN_TRUE_RR, PART_SPMV, spmv_grid(c), false, false);
 }
@@ -992,6 +1009,7 @@ struct CgVecs {
     V *r, *z, *p, *w, *dinv, *y, *rt;
     const V* val;
     V* s = nullptr;  // single-reduction CG only
+    V* xw = nullptr; // what the vector kernels add alpha p to: x, or the FP32 increment
 };
 
 template <typename V>
@@ -1046,6 +1064,27 @@ static int restart_residual(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, PcM
     return FLGPU_OK;
 }
 
+// The x the (F)PCG vector kernels update: FP64 x itself, or (mixed) the
+// FP32 increment dx, zeroed here and added to x by every true residual
+// (true_residual folds it first), so x is whole at each replacement and
+// at the end of the solve.  (The single-reduction CG keeps its updates in
+// FP64: with the FP32 increment its counts rose by 5-9% on the 3-D
+// systems, against about 1 for (F)PCG.)
+template <typename V>
+static int x_work(flgpu_ctx* c, KrylovWork* k, CgVecs<V>& v) {
+    if constexpr (std::is_same<V, float>::value) {
+        FLGPU_TRY(k->dx32.alloc(c->rows));
+        hipLaunchKernelGGL((k_fill<float>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, 0.0f, k->dx32.p);
+        FLGPU_HIP(hipGetLastError());
+        v.xw = k->dx32.p;
+        k->dx_on = true;
+    } else {
+        v.xw = k->x.p;
+        k->dx_on = false;
+    }
+    return FLGPU_OK;
+}
+
 template <typename V>
 static int cg_restart(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, PcMode mode, bool set_norm0) {
     FLGPU_TRY(restart_residual<V>(c, k, v, mode, set_norm0, true));
@@ -1072,12 +1111,12 @@ static int cg_iteration(flgpu_ctx* c, KrylovWork* k, const CgVecs<V>& v, PcMode 
     k->cur ^= 1;
     const float* dsc = mode == PCM_LEARNED ? learned_scale(c) : nullptr;
     constexpr int R = 16 / sizeof(V);
-    if (rows_aligned<R>(k->x.p, v.p, v.r, v.w, v.z, v.dinv, dsc, v.rt))
-        hipLaunchKernelGGL((k_cg_update<V, R>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->x.p, v.p,
+    if (rows_aligned<R>(v.xw, v.p, v.r, v.w, v.z, v.dinv, dsc, v.rt))
+        hipLaunchKernelGGL((k_cg_update<V, R>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, v.xw, v.p,
                            v.r, v.w, v.z, v.dinv, diag ? 1 : 0, dsc, v.rt, k->part(PART_SPMV), spmv_grid(c),
                            k->part(PART_VEC), in, k->S());
     else
-        hipLaunchKernelGGL((k_cg_update<V, 1>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, k->x.p, v.p,
+        hipLaunchKernelGGL((k_cg_update<V, 1>), dim3(k->nblk), dim3(kVecBlock), 0, c->stream, c->rows, v.xw, v.p,
                            v.r, v.w, v.z, v.dinv, diag ? 1 : 0, dsc, v.rt, k->part(PART_SPMV), spmv_grid(c),
                            k->part(PART_VEC), in, k->S());
     int npc = 0;
@@ -1099,6 +1138,7 @@ static int run_cg(flgpu_ctx* c, const flgpu_solver_params* sp, flgpu_solve_info*
     KrylovWork* k = c->kw;
     const bool mixed = std::is_same<V, float>::value;
     CgVecs<V> v = cg_vecs<V>(k);
+    FLGPU_TRY(x_work<V>(c, k, v));
     PcMode mode = learned ? PCM_LEARNED : sp->pc == FLGPU_PC_SOR ? PCM_SSOR : PCM_DIAG;
     bool use_learned = learned;
     const int check_every = sp->check_every > 0 ? sp->check_every : 16;
@@ -1707,6 +1747,7 @@ int solve(flgpu_ctx* c, const flgpu_solver_params* sp, double* x_host, flgpu_sol
     k->prepared = true;
     k->prepared_mixed = mixed;
     if (factor) FLGPU_TRY(learned_prepare(c, mixed, sp->pc == FLGPU_PC_FSAI));
+    k->dx_on = false;  // set by the mixed CG loops for their FP32 increment
     if (sp->ksp == FLGPU_KSP_GMRES) {
         FLGPU_TRY(mixed ? run_gmres<float>(c, sp, &info) : run_gmres<double>(c, sp, &info));
     } else if (sp->ksp == FLGPU_KSP_PIPECG) {
@@ -1714,6 +1755,7 @@ int solve(flgpu_ctx* c, const flgpu_solver_params* sp, double* x_host, flgpu_sol
     } else {
         FLGPU_TRY(mixed ? run_cg<float>(c, sp, &info, factor) : run_cg<double>(c, sp, &info, factor));
     }
+    k->dx_on = false;  // the loops end with a true residual, which adds the increment to x
     hipLaunchKernelGGL(k_finish_x, dim3(grid_for(c->rows, kVecBlock)), dim3(kVecBlock), 0, c->stream, c->rows,
                        k->x.p, c->rhs.p, c->inactive.p, c->have_inactive ? 1 : 0);
     FLGPU_HIP(hipGetLastError());
