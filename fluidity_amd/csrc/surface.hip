@@ -88,7 +88,7 @@ k_face_local(const FaceTables* __restrict__ tb, int nfaces, const int32_t* __res
         for (int a = 0; a < DIM; a++) nrm[a] = 0.0;
         if (ff.adv_face || neu_rob) {
             // transform_facet_to_physical: one Jacobian per linear face
-            double Xf[DIM][FLX], J[DIM][DIM - 1];
+            double Xf[DIM][FLX], J[DIM][DIM > 1 ? DIM - 1 : 1];
 #pragma unroll
             for (int k = 0; k < FLX; k++) {
                 const int v = fnodes_x[(int64_t)f * FLX + k];
@@ -104,7 +104,9 @@ k_face_local(const FaceTables* __restrict__ tb, int nfaces, const int32_t* __res
                     for (int k = 0; k < FLX; k++) t = t + Xf[a][k] * tb->fxdn[k + FLX * (0 + fg * d)];
                     J[a][d] = t;
                 }
-            if constexpr (DIM == 2) {
+            if constexpr (DIM == 1) {
+                detJ = 1.0;  // a point facet: detwei = the weight (1)
+            } else if constexpr (DIM == 2) {
                 detJ = sqrt(J[0][0] * J[0][0] + J[1][0] * J[1][0]);
             } else {
 #pragma unroll
@@ -129,7 +131,9 @@ k_face_local(const FaceTables* __restrict__ tb, int nfaces, const int32_t* __res
                     o[a] = sf / FLX - se / LX;
                 }
                 // facet_normal; cross_product (femtools/Vector_Tools.F90:76-85)
-                if constexpr (DIM == 2) {
+                if constexpr (DIM == 1) {
+                    nrm[0] = 1.0;  // its sign from the outward vector below
+                } else if constexpr (DIM == 2) {
                     nrm[0] = -J[1][0];
                     nrm[1] = J[0][0];
                 } else {
@@ -456,6 +460,7 @@ int surface_assemble(flgpu_ctx* c, const flgpu_advdif_params* p, const int32_t* 
                        dt_theta, p->theta, ie ? 1 : 0, mph ? 1 : 0, move ? 1 : 0, p->density_theta};
     const int key = c->dim * 10 + S.floc;
     switch (key) {
+        case 11: launch_face_local<1, 1>(c, ff, d_err.p); break;
         case 22: launch_face_local<2, 2>(c, ff, d_err.p); break;
         case 23: launch_face_local<2, 3>(c, ff, d_err.p); break;
         case 33: launch_face_local<3, 3>(c, ff, d_err.p); break;
@@ -489,17 +494,15 @@ extern "C" int flgpu_surface_bind(flgpu_ctx* c, int32_t nfaces, int32_t floc, in
                                   const int32_t* face_ele, const double* fn, const double* fw,
                                   const double* fx_dn) {
     return guarded("flgpu_surface_bind", [&]() -> int {
-        if (!c || !face_ndglno || !face_ndglno_x || !face_ele || !fn || !fw || !fx_dn)
+        if (!c || !face_ndglno || !face_ndglno_x || !face_ele || !fn || !fw || (!fx_dn && c->dim > 1))
             return set_error(FLGPU_EINVAL, "flgpu_surface_bind: null argument");
         if (c->nnodes <= 0 || c->nele <= 0)
             return set_error(FLGPU_EINVAL, "flgpu_surface_bind: bind the coordinate and field meshes first");
         const int dim = c->dim;
-        if (dim < 2)
-            return set_error(FLGPU_EUNSUPPORTED, "flgpu_surface_bind: the face loop takes 2-D and 3-D meshes");
-        const int flin = dim, fquad = dim * (dim + 1) / 2;
+        const int flin = dim, fquad = dim * (dim + 1) / 2;  // an interval's face is one node at any degree
         if (nfaces < 0 || !(floc == flin || floc == fquad) || floc > kMaxFaceLoc || fngi < 1 || fngi > kMaxFaceGi)
             return set_error(FLGPU_EUNSUPPORTED, "flgpu_surface_bind: face loc %d / ngi %d (dim %d)", floc, fngi, dim);
-        if ((floc == flin) != c->linear)
+        if (dim > 1 && (floc == flin) != c->linear)
             return set_error(FLGPU_EINVAL, "flgpu_surface_bind: face degree differs from the field mesh's");
         FLGPU_HIP(hipSetDevice(c->device));
         std::vector<int32_t> fnd((size_t)nfaces * floc), fx((size_t)nfaces * dim), fe(nfaces);

@@ -195,5 +195,8 @@ def make_element(dim: int, degree: int, quad_degree: int) -> Element:
 def make_face_element(dim: int, degree: int, quad_degree: int) -> Element:
     """face_shape of a dim-dimensional Lagrange simplex mesh: the element of
     dimension dim-1 and the same degree on the surface rule of the same
-    degree as the volume rule."""
+    degree as the volume rule; for an interval the face is a point: one
+    node, one point of weight 1 (the detwei of a point facet)."""
+    if dim == 1:
+        return Element(0, degree, quad_degree, np.ones((1, 1)), np.ones(1), np.ones((1, 1)), np.zeros((1, 1, 0)))
     return make_element(dim - 1, degree, quad_degree)

@@ -345,7 +345,7 @@ def unit_box_bcs(mesh: Mesh, surf: Surface, kind: str = "neumann_robin"):
     x1 = np.abs(cen[:, 0] - xmax) < tol * max(1.0, abs(xmax))
     t = np.zeros(surf.nfaces, np.int32)
     pos = mesh.node_pos[surf.face_nodes.reshape(surf.nfaces, surf.floc) - 1]  # (nf, floc, dim)
-    y = pos[:, :, 1]
+    y = pos[:, :, 1] if mesh.dim > 1 else np.zeros(pos.shape[:2])
     v1 = np.zeros_like(y)
     v2 = np.zeros_like(y)
     if kind == "neumann_robin":

@@ -68,8 +68,8 @@ const char *flgpu_last_error(void);
  * by X%refcount%id and eventcount(EVENT_MESH_MOVEMENT), :227-306
  * construct_cache).  A call with the same (key_id, key_event) is a no-op.
  * Linear simplices only (loc_x == dim+1), dim 1, 2 or 3 (intervals
- * assemble on the colour scheme; the face loop and the pressure path take
- * dim 2 and 3).
+ * assemble on the colour scheme and have point faces; the pressure path
+ * takes dim 2 and 3).
  * ndglno_x: loc_x*nele; X: dim x nx. */
 int flgpu_mesh_bind(flgpu_ctx *ctx, int64_t key_id, int64_t key_event,
                     int32_t dim, int32_t loc_x, int32_t nele, int32_t nx,

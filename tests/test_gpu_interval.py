@@ -103,14 +103,41 @@ def test_diff_src_dirichlet_1d_known_answer(require_gpu, name, n, ksp, pc, rtol)
     pr.close()
 
 
-def test_interval_rejects_face_loop_and_pressure(require_gpu):
-    """The face loop and the pressure path take 2-D and 3-D meshes: an
-    interval mesh gets a clear error, not a wrong result."""
+def test_interval_rejects_pressure_path(require_gpu):
+    """The pressure path takes 2-D and 3-D meshes: an interval mesh gets a
+    clear error, not a wrong result."""
     from fluidity_amd import lib as L
     cfg = M.config("interval", seed=1, scale=8)
     pr = cg.problem_from_config(cfg)
-    one = np.ones(1, np.int32)
-    rc = pr.lib.flgpu_surface_bind(pr.ctx, 1, 1, 1, L.ptr(one), L.ptr(one), L.ptr(one), L.ptr(np.ones(1)),
-                                   L.ptr(np.ones(1)), L.ptr(np.ones(1)))
-    assert rc == 6  # FLGPU_EUNSUPPORTED
+    pm = cg.vertex_pressure_mesh(pr)
+    with pytest.raises(Exception, match="2-D and 3-D"):
+        cg.bind_pressure_mesh(pr, pm)
     pr.close()
+
+
+@pytest.mark.parametrize("name,kind,opts", [
+    ("interval", "neumann_robin", {}),
+    ("intervalp2", "neumann_robin", {"integrate_advection_by_parts": True, "beta": 0.3}),
+    ("interval", "weak", {"integrate_advection_by_parts": True, "have_diffusivity": False}),
+])
+def test_interval_face_loop_matches_oracle(require_gpu, name, kind, opts):
+    """Point faces (detwei 1, N = 1, outward normal +-1): Neumann, Robin,
+    advection by parts, weak Dirichlet, against the oracle's face loop."""
+    cfg = M.config(name, seed=5, scale=32)
+    if kind == "weak":
+        cfg.kappa = None
+    print(f"\n[case] {name} faces {kind} {opts}", flush=True)
+    ref = O.reference_system(cfg, variant=dict(opts, bcs=kind))
+    pr = cg.problem_from_config(cfg)
+    surf = M.surface_of(cfg.mesh)
+    pr.bind_surface(surf)
+    bt, b1, b2, strong = M.unit_box_bcs(cfg.mesh, surf, kind)
+    o = cg.options_from_config(cfg)
+    for k, v in opts.items():
+        setattr(o, k, v)
+    cg.assemble_advection_diffusion_cg(pr, o, cfg.dt, cg.SurfaceBCs(bt, b1, b2))
+    cg.apply_dirichlet_conditions(pr, strong, 0.0, cfg.dt)
+    A, rhs, _ = pr.matrix()
+    pr.close()
+    assert rel(A, ref["A"]) < 1e-12
+    assert rel(rhs, ref["rhs"]) < 1e-12

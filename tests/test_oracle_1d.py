@@ -113,3 +113,66 @@ def test_p1_interval_element_closed_forms():
             else:
                 want = u / 2.0 * np.array([[-1.0, 1.0], [-1.0, 1.0]])
             assert np.allclose(mat, want, rtol=1e-13, atol=1e-15), (what, e, mat, want)
+
+
+def _interval_surface_setup(m, T, U, kappa, ibp, beta=1.0, qd=2):
+    from fluidity_amd.meshgen import surface_of, unit_box_bcs
+    el = E.make_element(1, m.degree, qd)
+    xel = E.make_element(1, 1, qd)
+    K = None if kappa is None else np.full((m.nnodes, 1, 1), kappa)
+    fields = {"T": T, "U": U, "kappa": K, "src": None, "absorp": None}
+    sp = O.make_sparsity(m)
+    col, ncol = O.colour_elements(m)
+    asm = O.Assembly(m, el, xel, fields, sp, O.colour_sets(col, ncol))
+    surf = surface_of(m)
+    bt, b1, b2, strong = unit_box_bcs(m, surf, "neumann_robin")
+    sd = O.SurfaceData(m, surf, E.make_face_element(1, m.degree, qd), E.make_face_element(1, 1, qd), bt, b1, b2)
+    p = O.params(0.1, 0.5, beta, have_mass=True, adv=U is not None, ibp=ibp, diff=kappa is not None)
+    return asm, sd, surf, p, (bt, b1, b2)
+
+
+@pytest.mark.parametrize("degree", [1, 2])
+def test_interval_point_faces_neumann_robin(degree):
+    """A point facet has detwei 1 and N = 1: Neumann adds g to the end
+    node's rhs; Robin adds dt theta h to its diagonal and g - h T to its
+    rhs (Advection_Diffusion_CG.F90:1349-1379)."""
+    m = M.interval_mesh(7, 0.0, 1.0, jitter=0.2, seed=1, degree=degree)
+    T = np.cos(2 * m.node_pos[:, 0])
+    asm, sd, surf, p, (bt, b1, b2) = _interval_surface_setup(m, T, None, 0.7, ibp=False)
+    A0, r0, _ = O.assemble(p, asm)
+    A1, r1 = A0.copy(), r0.copy()
+    O.surface_assemble(p, asm, sd, A1, r1)
+    f, c, _ = O.make_sparsity(m)
+    n0, n1 = 0, m.nnodes - 1 if degree == 1 else 2 * 7
+    left = np.nonzero(bt == M.BC_NEUMANN)[0]  # x = 0
+    assert left.size == 1 and surf.face_nodes[left[0]] == n0 + 1
+    dr = r1 - r0
+    g0 = b1[left[0]]
+    assert abs(dr[n0] - g0) < 1e-15
+    right = [i for i in range(surf.nfaces) if i != left[0]][0]
+    g1, h1 = b1[right], b2[right]
+    assert abs(dr[n1] - (g1 - h1 * T[n1])) < 1e-14
+    diag = [k for k in range(f[n1] - 1, f[n1 + 1] - 1) if c[k] == n1 + 1][0]
+    assert abs((A1 - A0)[diag] - 0.1 * 0.5 * h1) < 1e-15
+    assert np.count_nonzero(A1 - A0) == 1 and np.count_nonzero(dr) == 2
+
+
+@pytest.mark.parametrize("degree", [1, 2])
+def test_interval_integration_by_parts_identity(degree):
+    """Non-by-parts (beta = 1) = by parts + the point faces' u.n T term
+    (rules exact for the integrands: degree 2 for P1, 4 for P2)."""
+    m = M.interval_mesh(9, 0.0, 1.0, jitter=0.3, seed=2, degree=degree)
+    x = m.node_pos[:, 0]
+    T = np.sin(3 * x) + x
+    U = (0.5 + 0.3 * x)[:, None]
+    out = {}
+    for ibp in (False, True):
+        asm, sd, surf, p, _ = _interval_surface_setup(m, T, U, None, ibp=ibp, qd=2 * degree)
+        sd.s.bc_type = O._p(np.zeros(surf.nfaces, np.int32))
+        A, r, _ = O.assemble(p, asm)
+        if ibp:
+            O.surface_assemble(p, asm, sd, A, r)
+        out[ibp] = (A, r)
+    sa = np.abs(out[False][0]).max()
+    assert np.abs(out[True][0] - out[False][0]).max() < 1e-13 * sa
+    assert np.abs(out[True][1] - out[False][1]).max() < 1e-13 * np.abs(out[False][1]).max()
