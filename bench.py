@@ -219,6 +219,27 @@ def next_rows_report(cfg, pr, log_each) -> dict:
                     np.array_equal(mats["blocked"][1], mats["colour"][1]))
         rep[f"assembly_{stab}"]["blocked_equals_colour_bitwise"] = same
         log_each(f"{stab}: node-staged == colour scheme bit for bit: {same}")
+    # the element routine's other branches on the same mesh (colour scheme,
+    # every row by the general per-point row): the compressible multiphase
+    # internal-energy equation, and mesh movement
+    x = cfg.mesh.node_pos
+    pr.set_fields(density=1.0 + 0.5 * x[:, 0], old_density=0.8 + 0.4 * x[:, 1], pressure=2.0 + x[:, 2],
+                  volume_fraction=0.3 + 0.4 * x[:, 0] * x[:, 1])
+    X = cfg.mesh.X
+    pr.set_mesh_motion(X, X + 1e-3 * np.sin(np.pi * X), 1e-3 * np.sin(np.pi * X) / cfg.dt)
+    for label, kw in (("assembly_internal_energy", dict(equation_type="internal_energy", compressible=True,
+                                                        multiphase=True, density_theta=0.5, beta=0.5)),
+                      ("assembly_moving_mesh", dict(move_mesh=True, beta=0.5))):
+        so = cg.options_from_config(cfg)
+        so.exclude_advection_terms = False
+        so.exclude_mass_terms = False
+        for k, v in kw.items():
+            setattr(so, k, v)
+        ps = so.params(cfg.dt)
+        L.check(lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(ps)), "assemble " + label)
+        L.check(lib.flgpu_time_assemble(pr.ctx, C.byref(ps), 5, C.byref(ms)), "time " + label)
+        rep[label] = {"ms": round(ms.value, 4), "Melem/s": round(cfg.mesh.nele / ms.value / 1e3, 2)}
+        log_each(f"{label}: {ms.value:.3f} ms ({cfg.mesh.nele / ms.value / 1e3:.1f} Melem/s)")
     # the host-side block plan (slabs, slots, places), rebuilt after every
     # adapt: a fresh bind of the same mesh, then two assemblies; the first
     # builds the plan
