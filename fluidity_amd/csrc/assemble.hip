@@ -286,10 +286,72 @@ static int launch_colour_stab(flgpu_ctx* c, const AsmFields& fd, const AsmFlags&
     return FLGPU_OK;
 }
 
+// Row i of an element by the general per-point row, with the equation
+// types' fields and, for linear T, the mesh movement: |detJ| of the old and
+// new positions (vertices relative to the last one, as gather_elem) and
+// the grid velocity at the vertices.
+template <int DIM, int LOC, typename R>
+__device__ __forceinline__ void eq_row(int ele, const int32_t* __restrict__ conn_x, const Elem<DIM, LOC, R>& e,
+                                       const LTab<R>& tb, const AsmFlags& fl, const AsmFields& fd, int i,
+                                       R (&vals)[LOC], R& r) {
+    if constexpr (LOC == DIM + 1) {
+        if (fl.move) {
+            MoveData<DIM, R> mv;
+            const int32_t* cx = conn_x + (int64_t)ele * (DIM + 1);
+            for (int w = 0; w < 2; w++) {
+                const double* Xw = w ? fd.Xn4 : fd.Xo4;
+                R Xr[DIM][DIM + 1], iJ[DIM][DIM], ad;
+#pragma unroll
+                for (int nn = 0; nn <= DIM; nn++)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++)
+                        Xr[d][nn] = (R)(Xw[(int64_t)cx[nn] * 4 + d] - Xw[(int64_t)cx[DIM] * 4 + d]);
+                geometry<DIM, R>(Xr, tb, iJ, ad);
+                if (w)
+                    mv.adet_new = ad;
+                else
+                    mv.adet_old = ad;
+            }
+#pragma unroll
+            for (int nn = 0; nn <= DIM; nn++)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) mv.ug[nn][d] = (R)fd.UG4[(int64_t)cx[nn] * 4 + d];
+            row_general<DIM, LOC, R>(e, tb, fl, fd, i, vals, r, &mv);
+            return;
+        }
+    }
+    row_general<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
+}
+
+// The first pass of the two-pass scheme for the equation types and mesh
+// movement: a thread per (element, row) in element order, one launch, each
+// row's values stored by element id at gval[e*LOC^2 + i*LOC + j] and its rhs
+// at grv[e*LOC + i] (zeros included) for k_gather_sum's colour-order sums.
+template <int DIM, int LOC, typename R>
+__global__ void __launch_bounds__(256)
+k_asm_rows_eq(const uint32_t* __restrict__ ltab, int nele, const int32_t* __restrict__ conn,
+              const int32_t* __restrict__ conn_x, const double* __restrict__ X4, AsmFields fd, AsmFlags fl,
+              double* __restrict__ gval, double* __restrict__ grv) {
+    __shared__ LTab<R> tb;
+    stage_table_image<R, 256>(ltab, tb);
+    __syncthreads();
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nele * LOC) return;
+    const int ele = (int)(t / LOC), i = (int)(t % LOC);
+    Elem<DIM, LOC, R> e;
+    gather_elem<DIM, LOC, R>(ele, conn, conn_x, X4, fd.T, e);
+    R vals[LOC], r;
+    eq_row<DIM, LOC, R>(ele, conn_x, e, tb, fl, fd, i, vals, r);
+    double* gv = gval + (int64_t)ele * LOC * LOC + i * LOC;
+#pragma unroll
+    for (int j = 0; j < LOC; j++) gv[j] = (double)vals[j];
+    grv[(int64_t)ele * LOC + i] = (double)r;
+}
+
 // One launch per colour; thread = (element of the colour, local row i).
 // Elements of one colour share no node, so the read-modify-writes of a
-// launch never meet.  EQ: the internal-energy / k-epsilon equation types,
-// every row by the general per-point row (the density fields at the points).
+// launch never meet.  EQ: the internal-energy / k-epsilon equation types
+// and mesh movement, every row by the general per-point row (eq_row).
 template <int DIM, int LOC, typename R, bool EQ = false>
 __global__ void __launch_bounds__(256)
 k_asm_colour(const uint32_t* __restrict__ ltab, const int32_t* __restrict__ elems, int count,
@@ -306,42 +368,10 @@ k_asm_colour(const uint32_t* __restrict__ ltab, const int32_t* __restrict__ elem
     Elem<DIM, LOC, R> e;
     gather_elem<DIM, LOC, R>(ele, conn, conn_x, X4, fd.T, e);
     R vals[LOC], r;
-    if constexpr (EQ) {
-        if constexpr (LOC == DIM + 1) {
-            if (fl.move) {
-                // |detJ| of the old and new positions (vertices relative to
-                // the last one, as gather_elem) and the grid velocity at the
-                // vertices
-                MoveData<DIM, R> mv;
-                const int32_t* cx = conn_x + (int64_t)ele * (DIM + 1);
-                for (int w = 0; w < 2; w++) {
-                    const double* Xw = w ? fd.Xn4 : fd.Xo4;
-                    R Xr[DIM][DIM + 1], iJ[DIM][DIM], ad;
-#pragma unroll
-                    for (int nn = 0; nn <= DIM; nn++)
-#pragma unroll
-                        for (int d = 0; d < DIM; d++)
-                            Xr[d][nn] = (R)(Xw[(int64_t)cx[nn] * 4 + d] - Xw[(int64_t)cx[DIM] * 4 + d]);
-                    geometry<DIM, R>(Xr, tb, iJ, ad);
-                    if (w)
-                        mv.adet_new = ad;
-                    else
-                        mv.adet_old = ad;
-                }
-#pragma unroll
-                for (int nn = 0; nn <= DIM; nn++)
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) mv.ug[nn][d] = (R)fd.UG4[(int64_t)cx[nn] * 4 + d];
-                row_general<DIM, LOC, R>(e, tb, fl, fd, i, vals, r, &mv);
-            } else {
-                row_general<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
-            }
-        } else {
-            row_general<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
-        }
-    } else {
+    if constexpr (EQ)
+        eq_row<DIM, LOC, R>(ele, conn_x, e, tb, fl, fd, i, vals, r);
+    else
         row_contrib<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
-    }
     const int row = conn[(int64_t)ele * LOC + i];  // (not e.nodes[i]: a run-time index would put e in scratch)
 #pragma unroll
     for (int j = 0; j < LOC; j++) {
@@ -849,6 +879,26 @@ static int launch_gather(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, 
                                   (int)lds));
     hipLaunchKernelGGL(kern, dim3((c->nele + EPB - 1) / EPB), dim3(kEntryThreads), lds, c->stream, img, nullptr,
                        c->nele, c->conn.p, c->conn_x.p, c->X4.p, fd, fl, nullptr, P.gval.p, P.grv.p, d_err);
+    hipLaunchKernelGGL(k_gather_sum, dim3(grid_for(c->nnz + c->rows, 256)), dim3(256), 0, c->stream, (int)c->nnz,
+                       c->rows, P.gptr.p, P.glist.p, P.gval.p, P.rptr.p, P.rlist.p, P.grv.p, c->A.p, c->rhs.p);
+    if (P.ngbad > 0)
+        hipLaunchKernelGGL(k_gather_bad, dim3(grid_for(P.ngbad, 256)), dim3(256), 0, c->stream, P.ngbad, P.gbad.p,
+                           P.gval.p, d_err);
+    FLGPU_HIP(hipGetLastError());
+    return FLGPU_OK;
+}
+
+// The equation types and mesh movement in two passes (the default for them):
+// every row by eq_row in one launch, then the per-entry colour-order sums of
+// the quadratic scheme (the same bits as their colour scheme).
+template <int DIM, int LOC, typename R>
+static int launch_gather_eq(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& fl, int* d_err) {
+    FLGPU_TRY(build_gather_plan(c));
+    BlockPlan& P = c->plan;
+    const uint32_t* img = nullptr;
+    FLGPU_TRY(table_image<R>(c, &img));
+    hipLaunchKernelGGL((k_asm_rows_eq<DIM, LOC, R>), dim3(grid_for((int64_t)c->nele * LOC, 256)), dim3(256), 0,
+                       c->stream, img, c->nele, c->conn.p, c->conn_x.p, c->X4.p, fd, fl, P.gval.p, P.grv.p);
     hipLaunchKernelGGL(k_gather_sum, dim3(grid_for(c->nnz + c->rows, 256)), dim3(256), 0, c->stream, (int)c->nnz,
                        c->rows, P.gptr.p, P.glist.p, P.gval.p, P.rptr.p, P.rlist.p, P.grv.p, c->A.p, c->rhs.p);
     if (P.ngbad > 0)
@@ -1902,8 +1952,6 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
         if (fl.stab || fl.eq)
             return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: mesh movement with stabilisation or the "
                                                  "internal-energy / k-epsilon equations");
-        if (p->scheme == FLGPU_ASM_BLOCKED)
-            return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: mesh movement takes the colour scheme");
         if (!(p->dt > 0.0)) return set_error(FLGPU_EINVAL, "flgpu assemble: mesh movement needs dt > 0");
     }
     if (fl.eq) {
@@ -1913,9 +1961,6 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
         if (fl.comp && (!c->fP.set || !c->fU.set))
             return set_error(FLGPU_EINVAL, "flgpu assemble: the p div(u) term needs pressure and velocity");
         if (fl.mph && !c->fV.set) return set_error(FLGPU_EINVAL, "flgpu assemble: phase volume fraction not set");
-        if (p->scheme == FLGPU_ASM_BLOCKED)
-            return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: the internal-energy and k-epsilon equations "
-                                                 "take the colour scheme");
     }
     AsmFields fd;
     fd.T = c->fT.v.p;
@@ -1956,7 +2001,10 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
     }
     // quadratic T, unstabilised: the two-pass scheme is the blocked one
     const bool two_pass = !c->linear && dim >= 2 && !fl.stab && !fl.eq && scheme != FLGPU_ASM_COLOUR;
-    if (scheme == FLGPU_ASM_BLOCKED && !blocked_ok && !two_pass)
+    // the equation types and mesh movement, unstabilised: two passes unless
+    // the colour scheme is asked for by name (the same bits)
+    const bool two_pass_eq = (fl.eq || fl.move) && !fl.stab && scheme != FLGPU_ASM_COLOUR;
+    if (scheme == FLGPU_ASM_BLOCKED && !blocked_ok && !two_pass && !two_pass_eq)
         return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: the blocked scheme covers linear T (in FP32 and with "
                                              "stabilisation: with standard P1 tables and the field on the "
                                              "coordinate mesh; in FP32 unstabilised: isotropic diffusivity) and "
@@ -1976,6 +2024,19 @@ static int assemble_launch(flgpu_ctx* c, const flgpu_advdif_params* p, int* d_er
             return fp32 ? launch_gather<3, 10, float>(c, fd, fl, d_err) : launch_gather<3, 10, double>(c, fd, fl, d_err);
         if (dim == 2 && loc == 6)
             return fp32 ? launch_gather<2, 6, float>(c, fd, fl, d_err) : launch_gather<2, 6, double>(c, fd, fl, d_err);
+        return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: dim %d loc %d", dim, loc);
+    }
+    if (two_pass_eq) {
+#define FL_GATHER_EQ(D, L) \
+    if (dim == D && loc == L) \
+        return fp32 ? launch_gather_eq<D, L, float>(c, fd, fl, d_err) : launch_gather_eq<D, L, double>(c, fd, fl, d_err);
+        FL_GATHER_EQ(1, 2)
+        FL_GATHER_EQ(1, 3)
+        FL_GATHER_EQ(2, 3)
+        FL_GATHER_EQ(2, 6)
+        FL_GATHER_EQ(3, 4)
+        FL_GATHER_EQ(3, 10)
+#undef FL_GATHER_EQ
         return set_error(FLGPU_EUNSUPPORTED, "flgpu assemble: dim %d loc %d", dim, loc);
     }
     if (scheme == FLGPU_ASM_BLOCKED) {

@@ -194,15 +194,17 @@ typedef struct {
      * with compressible != 0 adding -p div(u) to the rhs and multiphase != 0
      * the PhaseVolumeFraction factor (mass, advection, isotropic
      * diffusion, pressure term); k-epsilon weights mass and advection by
-     * the density.  Both take the colour scheme (FLGPU_ASM_AUTO or
-     * FLGPU_ASM_COLOUR), with or without stabilisation. */
+     * the density.  Unstabilised they assemble in two passes (FLGPU_ASM_AUTO
+     * or FLGPU_ASM_BLOCKED: rows in one launch, then per-entry colour-order
+     * sums) or on FLGPU_ASM_COLOUR, the same bits; stabilised on the colour
+     * scheme. */
     double density_theta;
     int32_t equation_type, compressible, multiphase;
     /* mesh movement (/mesh_adaptivity/mesh_movement with mass terms,
      * :421-437): the mass at the new positions, -(N^{n+1} - N^n) T / dt on
      * the rhs, the velocity relative to the grid and, not by parts,
      * -N_A div(u_grid) N_B (flgpu_mesh_motion_set gives the positions and
-     * grid velocity).  Linear T on the colour scheme, advection-diffusion
+     * grid velocity).  Linear T (schemes as for the equation types), advection-diffusion
      * without stabilisation (the reference stops on the other cases). */
     int32_t move_mesh;
 } flgpu_advdif_params;

@@ -5,8 +5,9 @@ advection with the div(density u) split, the phase volume fraction
 (multiphase), the compressible p div(u) term; P1 and P2, 1-D to 3-D, FP64
 to 1e-12 and FP32 to 1e-5 relative (as the other assembly tests), also
 with SUPG and streamline-upwind stabilisation.  The equation types take the
-colour scheme: asked for by name, the blocked scheme is refused with
-FLGPU_EUNSUPPORTED.
+two-pass scheme by default (or the colour scheme by name, the same bits);
+stabilised, the colour scheme (the blocked scheme is refused with
+FLGPU_EUNSUPPORTED).
 """
 import ctypes as C
 
@@ -93,11 +94,34 @@ def test_equation_types_match_oracle(require_gpu, name, scale, opts, precision):
     assert rel(base["A"], ref["A"]) > 1e-3 or rel(base["rhs"], ref["rhs"]) > 1e-3
 
 
-def test_equation_types_refuse_blocked_scheme(require_gpu):
+@pytest.mark.parametrize("name,scale,opts", [
+    ("c2", 4, dict(equation_type="internal_energy", compressible=True, multiphase=True, beta=0.4)),
+    ("c1p2", 4, dict(equation_type="k_epsilon", integrate_advection_by_parts=True, beta=0.2)),
+    ("interval", 32, dict(equation_type="internal_energy", lump_mass_matrix=True, beta=1.0)),
+])
+def test_two_pass_equals_colour_bitwise(require_gpu, name, scale, opts):
+    """The default two-pass scheme (rows in one launch, colour-order sums
+    per entry) gives the colour scheme's bits."""
+    cfg = make_cfg(name, scale)
+    pr = cg.problem_from_config(cfg)
+    pr.set_fields(**eq_fields(cfg))
+    mats = {}
+    for scheme in ("auto", "colour", "blocked"):
+        o = cg.options_from_config(cfg, scheme=scheme)
+        for k, v in opts.items():
+            setattr(o, k, v)
+        cg.assemble_advection_diffusion_cg(pr, o, cfg.dt)
+        mats[scheme] = pr.matrix()[:2]
+    pr.close()
+    for scheme in ("auto", "blocked"):
+        assert np.array_equal(mats[scheme][0], mats["colour"][0]) and np.array_equal(mats[scheme][1], mats["colour"][1])
+
+
+def test_equation_types_refuse_blocked_stabilised(require_gpu):
     cfg = make_cfg("c2", 3)
     pr = cg.problem_from_config(cfg)
     pr.set_fields(**eq_fields(cfg))
-    for kw in (dict(scheme="blocked"), dict(scheme="blocked", stabilisation="supg")):
+    for kw in (dict(scheme="blocked", stabilisation="supg"),):
         o = cg.options_from_config(cfg)
         o.equation_type = "internal_energy"
         for k, v in kw.items():

@@ -4,7 +4,8 @@ oracle, through the C ABI (flgpu_mesh_motion_set + move_mesh): linear T in
 1-D to 3-D, by parts or not, consistent or lumped mass, FP64 to 1e-12 and
 FP32 to 1e-5 relative.  What the reference stops on (stabilisation, the
 internal-energy / k-epsilon equations) and what this build leaves out
-(quadratic T, the blocked scheme) are refused with FLGPU_EUNSUPPORTED.
+(quadratic T) are refused with FLGPU_EUNSUPPORTED; the default two-pass
+scheme gives the colour scheme's bits.
 """
 import ctypes as C
 
@@ -84,6 +85,22 @@ def test_moving_mesh_matches_oracle(require_gpu, name, scale, opts, precision):
     assert rel(still["A"], ref["A"]) > 1e-4  # the motion changed the system
 
 
+def test_moving_mesh_two_pass_equals_colour_bitwise(require_gpu):
+    cfg = make_cfg("c2", 4)
+    X_old, X_new, ug = motion(cfg.mesh, cfg.dt)
+    pr = cg.problem_from_config(cfg)
+    pr.set_mesh_motion(X_old, X_new, ug)
+    mats = {}
+    for scheme in ("auto", "colour"):
+        o = cg.options_from_config(cfg, scheme=scheme)
+        o.move_mesh = True
+        o.beta = 0.4
+        cg.assemble_advection_diffusion_cg(pr, o, cfg.dt)
+        mats[scheme] = pr.matrix()[:2]
+    pr.close()
+    assert np.array_equal(mats["auto"][0], mats["colour"][0]) and np.array_equal(mats["auto"][1], mats["colour"][1])
+
+
 def test_moving_mesh_refusals(require_gpu):
     cfg = make_cfg("c2", 3)
     X_old, X_new, ug = motion(cfg.mesh, cfg.dt)
@@ -94,7 +111,7 @@ def test_moving_mesh_refusals(require_gpu):
     assert pr.lib.flgpu_advdif_assemble_resident(pr.ctx, C.byref(p)) == 1  # FLGPU_EINVAL: no motion set
     pr.set_mesh_motion(X_old, X_new, ug)
     pr.set_fields(density=1.0, old_density=1.0)
-    for kw in (dict(scheme="blocked"), dict(stabilisation="supg"), dict(equation_type="internal_energy")):
+    for kw in (dict(stabilisation="supg"), dict(equation_type="internal_energy")):
         o = cg.options_from_config(cfg)
         o.move_mesh = True
         for k, v in kw.items():
