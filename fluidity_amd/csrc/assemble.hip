@@ -290,10 +290,10 @@ static int launch_colour_stab(flgpu_ctx* c, const AsmFields& fd, const AsmFlags&
 // types' fields and, for linear T, the mesh movement: |detJ| of the old and
 // new positions (vertices relative to the last one, as gather_elem) and
 // the grid velocity at the vertices.
-template <int DIM, int LOC, typename R>
+template <int DIM, int LOC, typename R, class NA>
 __device__ __forceinline__ void eq_row(int ele, const int32_t* __restrict__ conn_x, const Elem<DIM, LOC, R>& e,
-                                       const LTab<R>& tb, const AsmFlags& fl, const AsmFields& fd, int i,
-                                       R (&vals)[LOC], R& r) {
+                                       const LTab<R>& tb, const AsmFlags& fl, const AsmFields& fd, const NA& na,
+                                       int i, R (&vals)[LOC], R& r) {
     if constexpr (LOC == DIM + 1) {
         if (fl.move) {
             MoveData<DIM, R> mv;
@@ -316,32 +316,47 @@ __device__ __forceinline__ void eq_row(int ele, const int32_t* __restrict__ conn
             for (int nn = 0; nn <= DIM; nn++)
 #pragma unroll
                 for (int d = 0; d < DIM; d++) mv.ug[nn][d] = (R)fd.UG4[(int64_t)cx[nn] * 4 + d];
-            row_general<DIM, LOC, R>(e, tb, fl, fd, i, vals, r, &mv);
+            row_general_na<DIM, LOC, R>(e, tb, fl, na, i, vals, r, &mv);
             return;
         }
     }
-    row_general<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
+    row_general_na<DIM, LOC, R>(e, tb, fl, na, i, vals, r);
 }
 
 // The first pass of the two-pass scheme for the equation types and mesh
-// movement: a thread per (element, row) in element order, one launch, each
-// row's values stored by element id at gval[e*LOC^2 + i*LOC + j] and its rhs
-// at grv[e*LOC + i] (zeros included) for k_gather_sum's colour-order sums.
+// movement: a thread per (element, row) in element order, one launch (the
+// element's LOC row threads adjacent: each stages its node's field values
+// in shared memory for all of them), each row's values stored by element id
+// at gval[e*LOC^2 + i*LOC + j] and its rhs at grv[e*LOC + i] (zeros
+// included) for k_gather_sum's colour-order sums.
+constexpr int kRowsEqThreads = 256;
+
+template <int LOC>
+__host__ __device__ constexpr int rows_eq_epb() {
+    return kRowsEqThreads / LOC;
+}
+
 template <int DIM, int LOC, typename R>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kRowsEqThreads)
 k_asm_rows_eq(const uint32_t* __restrict__ ltab, int nele, const int32_t* __restrict__ conn,
               const int32_t* __restrict__ conn_x, const double* __restrict__ X4, AsmFields fd, AsmFlags fl,
               double* __restrict__ gval, double* __restrict__ grv) {
+    constexpr int EPB = rows_eq_epb<LOC>();
+    constexpr int NF = shared_nodal_fields<DIM>();
     __shared__ LTab<R> tb;
-    stage_table_image<R, 256>(ltab, tb);
+    __shared__ R s_nodal[EPB * LOC * NF];
+    stage_table_image<R, kRowsEqThreads>(ltab, tb);
+    const int el = threadIdx.x / LOC, i = threadIdx.x - el * LOC;
+    const int64_t ele64 = (int64_t)blockIdx.x * EPB + el;
+    const bool live = el < EPB && ele64 < nele;
+    const int ele = live ? (int)ele64 : 0;
+    if (live) stage_shared_nodal<DIM, R>(fd, fl, conn[(int64_t)ele * LOC + i], s_nodal + (el * LOC + i) * NF);
     __syncthreads();
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nele * LOC) return;
-    const int ele = (int)(t / LOC), i = (int)(t % LOC);
+    if (!live) return;
     Elem<DIM, LOC, R> e;
     gather_elem<DIM, LOC, R>(ele, conn, conn_x, X4, fd.T, e);
     R vals[LOC], r;
-    eq_row<DIM, LOC, R>(ele, conn_x, e, tb, fl, fd, i, vals, r);
+    eq_row<DIM, LOC, R>(ele, conn_x, e, tb, fl, fd, SharedNodal<DIM, R>{s_nodal + el * LOC * NF}, i, vals, r);
     double* gv = gval + (int64_t)ele * LOC * LOC + i * LOC;
 #pragma unroll
     for (int j = 0; j < LOC; j++) gv[j] = (double)vals[j];
@@ -369,7 +384,7 @@ k_asm_colour(const uint32_t* __restrict__ ltab, const int32_t* __restrict__ elem
     gather_elem<DIM, LOC, R>(ele, conn, conn_x, X4, fd.T, e);
     R vals[LOC], r;
     if constexpr (EQ)
-        eq_row<DIM, LOC, R>(ele, conn_x, e, tb, fl, fd, i, vals, r);
+        eq_row<DIM, LOC, R>(ele, conn_x, e, tb, fl, fd, GlobalNodal<DIM, R>{fd, e.nodes}, i, vals, r);
     else
         row_contrib<DIM, LOC, R>(e, tb, fl, fd, i, vals, r);
     const int row = conn[(int64_t)ele * LOC + i];  // (not e.nodes[i]: a run-time index would put e in scratch)
@@ -897,7 +912,8 @@ static int launch_gather_eq(flgpu_ctx* c, const AsmFields& fd, const AsmFlags& f
     BlockPlan& P = c->plan;
     const uint32_t* img = nullptr;
     FLGPU_TRY(table_image<R>(c, &img));
-    hipLaunchKernelGGL((k_asm_rows_eq<DIM, LOC, R>), dim3(grid_for((int64_t)c->nele * LOC, 256)), dim3(256), 0,
+    constexpr int EPB = rows_eq_epb<LOC>();
+    hipLaunchKernelGGL((k_asm_rows_eq<DIM, LOC, R>), dim3((c->nele + EPB - 1) / EPB), dim3(kRowsEqThreads), 0,
                        c->stream, img, c->nele, c->conn.p, c->conn_x.p, c->X4.p, fd, fl, P.gval.p, P.grv.p);
     hipLaunchKernelGGL(k_gather_sum, dim3(grid_for(c->nnz + c->rows, 256)), dim3(256), 0, c->stream, (int)c->nnz,
                        c->rows, P.gptr.p, P.glist.p, P.gval.p, P.rptr.p, P.rlist.p, P.grv.p, c->A.p, c->rhs.p);

@@ -1224,14 +1224,63 @@ FLGPU_HD_INLINE void row_stab(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, con
     row_stab_q<DIM, LOC, R>(tb, fl, JT, iJ, adet, NodalQuad<DIM, LOC, R>{na, tb, iJ}, e.T, i, vals, rhs_i);
 }
 
+template <int DIM, int LOC, typename R, class NA>
+FLGPU_HD_INLINE void row_general_na(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, const AsmFlags& fl, const NA& na,
+                                    int i, R (&vals)[LOC], R& rhs_i, const MoveData<DIM, R>* mv = nullptr) {
+    R iJ[DIM][DIM], adet;
+    geometry<DIM, R>(e.X, tb, iJ, adet);
+    row_general_q<DIM, LOC, R>(tb, fl, iJ, adet, NodalQuad<DIM, LOC, R, NA>{na, tb, iJ}, e.T, i, vals, rhs_i, mv);
+}
+
 template <int DIM, int LOC, typename R>
 FLGPU_HD void row_general(const Elem<DIM, LOC, R>& e, const LTab<R>& tb, const AsmFlags& fl,
                             const AsmFields& fd, int i, R (&vals)[LOC], R& rhs_i,
                             const MoveData<DIM, R>* mv = nullptr) {
-    R iJ[DIM][DIM], adet;
-    geometry<DIM, R>(e.X, tb, iJ, adet);
     const GlobalNodal<DIM, R> na{fd, e.nodes};
-    row_general_q<DIM, LOC, R>(tb, fl, iJ, adet, NodalQuad<DIM, LOC, R>{na, tb, iJ}, e.T, i, vals, rhs_i, mv);
+    row_general_na<DIM, LOC, R>(e, tb, fl, na, i, vals, rhs_i, mv);
+}
+
+// An element's nodal field values staged in shared memory by its row
+// threads (node k's values at base[k * kSharedNodalFields + ...]: velocity,
+// the diffusivity tensor, source, absorption, density, old density,
+// pressure, volume fraction), read with GlobalNodal's methods.
+template <int DIM>
+constexpr int shared_nodal_fields() {
+    return DIM + DIM * DIM + 6;
+}
+
+template <int DIM, typename R>
+struct SharedNodal {
+    static constexpr bool eq_fields = true;
+    static constexpr int NF = shared_nodal_fields<DIM>();
+    const R* base;
+    FLGPU_HD_INLINE R U(int k, int d) const { return base[k * NF + d]; }
+    FLGPU_HD_INLINE R K(int k, int comp) const { return base[k * NF + DIM + comp]; }
+    FLGPU_HD_INLINE R S(int k) const { return base[k * NF + DIM + DIM * DIM]; }
+    FLGPU_HD_INLINE R B(int k) const { return base[k * NF + DIM + DIM * DIM + 1]; }
+    FLGPU_HD_INLINE R Rho(int k) const { return base[k * NF + DIM + DIM * DIM + 2]; }
+    FLGPU_HD_INLINE R RhoOld(int k) const { return base[k * NF + DIM + DIM * DIM + 3]; }
+    FLGPU_HD_INLINE R P(int k) const { return base[k * NF + DIM + DIM * DIM + 4]; }
+    FLGPU_HD_INLINE R V(int k) const { return base[k * NF + DIM + DIM * DIM + 5]; }
+};
+
+// Node `node`'s values into dst (the SharedNodal layout); only the fields
+// the flags use are read, the others are zero.
+template <int DIM, typename R>
+FLGPU_HD_INLINE void stage_shared_nodal(const AsmFields& fd, const AsmFlags& fl, int node, R* dst) {
+    const GlobalNodal<DIM, R> g{fd, &node};
+    const bool u = fl.adv || fl.comp;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) dst[d] = u ? g.U(0, d) : (R)0;
+#pragma unroll
+    for (int c = 0; c < DIM * DIM; c++) dst[DIM + c] = (fl.diff && (!fl.iso || c == 0)) ? g.K(0, c) : (R)0;
+    R* t = dst + DIM + DIM * DIM;
+    t[0] = fl.src ? g.S(0) : (R)0;
+    t[1] = fl.absorp ? g.B(0) : (R)0;
+    t[2] = fl.eq ? g.Rho(0) : (R)0;
+    t[3] = fl.eq == 1 ? g.RhoOld(0) : (R)0;
+    t[4] = fl.comp ? g.P(0) : (R)0;
+    t[5] = fl.mph ? g.V(0) : (R)0;
 }
 
 template <int DIM, int LOC, typename R>
